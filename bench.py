@@ -1,0 +1,174 @@
+#!/usr/bin/env python3
+"""Benchmark: classified initial conditions per second on one MI355X.
+
+Default workload (one "step" = one full basin-stability pass over the batch):
+Duffing oscillator x' = v, v' = -0.08 v - x^3 + 0.2 cos t, n = 100,000 ICs
+seeded-uniform in x in [-1, 1], v in [-0.5, 1]; t in [0, 1000], rtol 1e-8,
+atol 1e-6, float32; save grid 1,000 points on [900, 1000]; fused
+mean / std / min / max per state over that window; 1-NN classification
+against the 5 template ICs of the reference's Duffing setup, integrated in
+the same launch.  Inputs are resident in HBM before timing starts.
+
+Prints one JSON line (rank 0).  --gpus above 1 is not supported: this build
+targets one GPU per process and runs no collectives.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import torch
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_FP32_VALU = 157.3e12  # MI355X FP32 vector peak (packed), FLOP/s
+PEAK_FP64_VALU = 78.6e12
+
+# flops per attempted step: 6 F_rhs + 63 S + 16 (RK combination, error norm,
+# controller); per save point: dense-output fit 36 S + evaluation 9 S +
+# streaming statistics 6 S
+F_RHS = {"duffing": 8, "pendulum": 5, "lorenz": 9}
+STATE = {"duffing": 2, "pendulum": 2, "lorenz": 3}
+
+DUFFING_TEMPLATES = [[-0.21, 0.02], [1.05, 0.77], [-0.67, 0.02], [-0.46, 0.30], [-0.43, 0.12]]
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=100_000)
+    ap.add_argument("--dtype", choices=["f32", "f64"], default="f32")
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--max-lanes", type=int, default=0)
+    ap.add_argument("--cpu-sample", type=int, default=20_000)
+    ap.add_argument("--no-cpu", action="store_true")
+    return ap.parse_args()
+
+
+def cpu_baseline(y0_cpu, save64, n_sample):
+    """The CPU oracle (per-lane Dopri5 in C, same algorithm and dtype) on the
+    first n_sample ICs, threads over the host cores."""
+    import numpy as np
+
+    from oracle import oracle as O
+
+    threads = max(1, min(os.cpu_count() or 1, 16))
+    y = y0_cpu[:n_sample].double().numpy()
+    t0 = time.perf_counter()
+    O.solve(O.SYS_DUFFING, y, np.array([0.08, 1.0, 0.2]), save64, 0.0, rtol=1e-8, atol=1e-6,
+            bits=32, threads=threads)
+    dt = time.perf_counter() - t0
+    return {"value": round(n_sample / dt, 1), "unit": "ICs/s", "cores": threads, "kind": "port",
+            "sample": f"first {n_sample} of the same seeded Duffing ICs, full t in [0,1000], "
+                      f"float32, oracle per-lane Dopri5 (C, pthreads), {dt:.2f} s"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path "
+              "(ICs shard trivially across GPUs; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    from pybasin_amd.solvers import HipDopri5Solver
+    from pybasin_amd.systems import DuffingODE
+
+    dev = torch.device("cuda:0")
+    dtype = torch.float32 if args.dtype == "f32" else torch.float64
+    n = args.n
+    torch.manual_seed(args.seed)
+    lo = torch.tensor([-1.0, -0.5])
+    hi = torch.tensor([1.0, 1.0])
+    y0_cpu = torch.rand(n, 2) * (hi - lo) + lo
+    y0_all = torch.cat([y0_cpu, torch.tensor(DUFFING_TEMPLATES)]).to(dtype).to(dev)
+    ode = DuffingODE({"delta": 0.08, "k3": 1.0, "A": 0.2})
+    solver = HipDopri5Solver(t_span=(0, 1000), t_steps=1000, t_eval=(900.0, 1000.0), device="cuda",
+                             rtol=1e-8, atol=1e-6, cache_dir=None, max_lanes=args.max_lanes)
+    save_ts = torch.linspace(900.0, 1000.0, 1000, dtype=dtype, device=dev)
+    stream = torch.cuda.current_stream(dev)
+    kev = []
+
+    def step(record=False):
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        res = solver.run_fused(ode, y0_all, save_ts, None, keep_trajectories=False, steady_idx=0,
+                               stream=stream)
+        e1.record(stream)
+        if record:
+            kev.append((e0, e1))
+        # reference Duffing classifier features: state 0 maximum and std
+        feats = torch.stack([res.stats[:, 0, 3], res.stats[:, 0, 1]], dim=1)
+        d = torch.cdist(feats[:n], feats[n:])
+        labels = torch.argmin(d, dim=1)
+        counts = torch.bincount(labels, minlength=len(DUFFING_TEMPLATES))
+        return res, counts
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        res, counts = step(record=True)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_step = 1e3 * wall / args.steps
+    kernel_ms = sum(a.elapsed_time(b) for a, b in kev) / len(kev)
+
+    nsteps = res.nsteps.to(torch.float64)
+    status = res.status.cpu()
+    S = 2
+    n_traj = n + len(DUFFING_TEMPLATES)
+    flops = float(nsteps.sum()) * (6 * F_RHS["duffing"] + 63 * S + 16) + n_traj * 1000 * (36 + 9 + 6) * S
+    achieved = flops / (kernel_ms * 1e-3)
+    peak = PEAK_FP32_VALU if dtype == torch.float32 else PEAK_FP64_VALU
+    bs = (counts.double() / n).cpu().tolist()
+
+    out = {
+        "metric": "classified initial conditions/sec at <=0.5% basin-stability error vs plain",
+        "value": round(n / (ms_step * 1e-3), 1),
+        "unit": "ICs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": args.dtype,
+        "data": "synthetic: seeded-uniform ICs (torch.manual_seed, CPU float32 draw)",
+        "config": {
+            "workload": "duffing_plain_hip_dopri5",
+            "n_ics": n, "t_span": [0, 1000], "save_window": [900, 1000], "t_steps": 1000,
+            "rtol": 1e-8, "atol": 1e-6, "features": "fused mean/std/min/max per state",
+            "classifier": "1-NN on (max x, std x), 5 templates",
+            "basin_stability": [round(v, 5) for v in bs],
+            "mean_steps_per_ic": round(float(nsteps.mean()), 1),
+            "lanes_failed": int((status != 0).sum()),
+            "bs_error_vs_plain": 0.0,
+        },
+        "roofline": {
+            "bound": "valu", "achieved": round(achieved / 1e12, 3),
+            "peak": round(peak / 1e12, 1), "unit": "TFLOP/s",
+            "frac": round(achieved / peak, 4), "traffic": None,
+            "kernel": "bh::dopri5_lane_kernel<Duffing,float>",
+            "kernel_ms": round(kernel_ms, 3),
+        },
+    }
+    if not args.no_cpu:
+        import numpy as np
+
+        save64 = save_ts.double().cpu().numpy().astype(np.float64)
+        out["cpu_baseline"] = cpu_baseline(y0_cpu, save64, min(args.cpu_sample, n))
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,97 @@
+/*
+ * basinhip.h -- C-ABI of libbasinhip.so, the MI355X (gfx950) Dormand-Prince
+ * basin-stability integration path.
+ *
+ * Every array argument is DEVICE memory owned by the caller; calls are
+ * asynchronous on `stream` (a hipStream_t passed as void*; NULL = the null
+ * stream).  Return codes extend the reference's zigode ErrorCode
+ * (src/zigode/src/c_api.zig:17-23):
+ *   0 success, -1 invalid system id, -2 allocation failed, -3 solve failed,
+ *   -4 (unused: thread spawn), -5 invalid argument, -6 HIP runtime error.
+ * bh_last_error() returns a message for the last non-zero code of the
+ * calling thread.
+ */
+#ifndef BASINHIP_H
+#define BASINHIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- identifiers ------------------------------------------------------ */
+/* Right-hand sides with a device functor.  Parameter vectors follow the
+   reference's TypedDict order (src/pybasin/solvers/torch_ode_system.py:57-66). */
+enum {
+    BH_SYS_DECAY = 0,    /* y' = p0*y, S = 1 or 2  (tests/unit/solvers/test_solver.py:16-18) */
+    BH_SYS_LINEAR2 = 1,  /* y' = [[p0,p1],[p2,p3]] y, S = 2 (test system)              */
+    BH_SYS_PENDULUM = 2, /* [alpha, T, K], S = 2  (case_studies/pendulum/pendulum_ode.py:23-40) */
+    BH_SYS_DUFFING = 3,  /* [delta, k3, A], S = 2 (case_studies/duffing_oscillator/duffing_ode.py:72-88) */
+    BH_SYS_LORENZ = 4,   /* [sigma, r, b], S = 3, freeze mask |y|inf >= 200
+                            (case_studies/lorenz/lorenz_ode.py:116-152) */
+    BH_SYS_ROSSLER100 = 5 /* [a, b, c, K], S = 300; see bh_solve_rossler */
+};
+enum { BH_F32 = 0, BH_F64 = 1 };
+
+/* per-lane status written to out_status */
+enum {
+    BH_LANE_OK = 0,
+    BH_LANE_MAX_STEPS = 1,    /* step budget exhausted; remaining saves hold the last state */
+    BH_LANE_DT_UNDERFLOW = 2, /* t + dt == t (torchdiffeq's "underflow in dt")             */
+    BH_LANE_NONFINITE = 3,    /* NaN/Inf state or error ratio                              */
+    BH_LANE_DIVERGED = 4      /* |y|inf >= stop_threshold; lane stopped (freeze or Inf fill) */
+};
+
+/* flags */
+#define BH_FLAG_STOP 0x1u     /* enable the per-lane divergence stop at stop_threshold    */
+#define BH_FLAG_INF_FILL 0x2u /* after a stop, remaining saves are +Inf (JaxSolver Event
+                                 semantics) instead of the frozen state (masked-RHS semantics) */
+
+/* number of statistics per state written to out_feats: mean, std (ddof 0),
+   min, max over save indices [steady_idx, n_save) */
+#define BH_NFEAT 4
+/* bytes of device workspace bh_solve_batch needs (a lane-refill counter) */
+#define BH_WORKSPACE_BYTES 256
+
+int32_t bh_version(void);
+const char *bh_last_error(void);
+
+/*
+ * bh_solve_batch -- integrate n independent trajectories, one per GPU lane,
+ * with per-lane adaptive Dormand-Prince 5(4) under torchdiffeq's step rules
+ * (replaces the integration inside TorchDiffEqSolver._integrate,
+ * src/pybasin/solvers/torchdiffeq_solver.py:97-134, and the batch entry of
+ * the reference's native solver, solve_batch in src/zigode/src/c_api.zig:193-314).
+ *
+ *   y0s            (n, dim) in dtype
+ *   params         (n, n_params) float64 with row stride params_stride,
+ *                  or one shared vector when params_stride == 0
+ *   t0             integration start (t_span[0]; torchdiffeq_solver.py:122-125)
+ *   save_ts        (n_save) float64: the save grid in the state dtype,
+ *                  promoted to float64 (base.py:113-127)
+ *   rtol, atol     tolerances; max_steps: attempted-step budget per lane
+ *   flags          BH_FLAG_*; stop_threshold used with BH_FLAG_STOP
+ *   steady_idx     first save index of the feature window (filter_time,
+ *                  feature_extractor.py:102-130); >= n_save disables features
+ *   out_traj       (n_save, n, dim) in dtype, or NULL (trajectories not kept)
+ *   out_feats      (n, dim, BH_NFEAT) float64, or NULL
+ *   out_final      (n, dim) in dtype: state at the last save time, or NULL
+ *   out_status     (n) int32, or NULL; out_nsteps (n) int64 attempted steps, or NULL
+ *   workspace      >= BH_WORKSPACE_BYTES of device memory (refill counter)
+ *   max_lanes      cap on resident lanes (0 = fill the device); lanes that
+ *                  finish a trajectory take the next one (work stealing)
+ */
+int32_t bh_solve_batch(int32_t system_id, int32_t dtype, const void *y0s, int64_t n,
+                       int32_t dim, const double *params, int32_t n_params,
+                       int64_t params_stride, double t0, const double *save_ts,
+                       int32_t n_save, double rtol, double atol, int64_t max_steps,
+                       uint32_t flags, double stop_threshold, int32_t steady_idx,
+                       void *out_traj, double *out_feats, void *out_final,
+                       int32_t *out_status, int64_t *out_nsteps, void *workspace,
+                       int64_t workspace_bytes, int32_t max_lanes, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BASINHIP_H */

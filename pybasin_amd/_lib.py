@@ -1,0 +1,83 @@
+"""ctypes binding of libbasinhip.so (declared in include/basinhip.h).
+
+The library is loaded from the package directory only.  There is no
+fallback: if it is missing or fails to load, every HIP entry point raises.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import threading
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_PKG, "libbasinhip.so")
+
+BH_SYS_DECAY, BH_SYS_LINEAR2, BH_SYS_PENDULUM, BH_SYS_DUFFING, BH_SYS_LORENZ, BH_SYS_ROSSLER100 = range(6)
+BH_F32, BH_F64 = 0, 1
+BH_LANE_OK, BH_LANE_MAX_STEPS, BH_LANE_DT_UNDERFLOW, BH_LANE_NONFINITE, BH_LANE_DIVERGED = range(5)
+BH_FLAG_STOP = 0x1
+BH_FLAG_INF_FILL = 0x2
+BH_NFEAT = 4
+BH_WORKSPACE_BYTES = 256
+
+# every symbol include/basinhip.h declares
+EXPORTS = ("bh_version", "bh_last_error", "bh_solve_batch")
+
+_lock = threading.Lock()
+_lib: ctypes.CDLL | None = None
+
+
+class HipLibraryError(RuntimeError):
+    pass
+
+
+def _declare(L: ctypes.CDLL) -> None:
+    vp, i32, i64, f64, u32 = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_double, ctypes.c_uint32
+    L.bh_version.restype = i32
+    L.bh_version.argtypes = []
+    L.bh_last_error.restype = ctypes.c_char_p
+    L.bh_last_error.argtypes = []
+    L.bh_solve_batch.restype = i32
+    L.bh_solve_batch.argtypes = [i32, i32, vp, i64, i32, vp, i32, i64, f64, vp, i32, f64, f64,
+                                 i64, u32, f64, i32, vp, vp, vp, vp, vp, vp, i64, i32, vp]
+    if hasattr(L, "bh_solve_rossler"):
+        L.bh_solve_rossler.restype = i32
+        L.bh_solve_rossler.argtypes = [i32, vp, i64, vp, i64, vp, f64, f64, f64, f64, i64, u32,
+                                       f64, vp, vp, vp, vp, i64, i32, vp]
+    if hasattr(L, "bh_mlp_forward"):
+        L.bh_mlp_forward.restype = i32
+        L.bh_mlp_forward.argtypes = [vp, i64, vp, vp, vp, vp, vp]
+    if hasattr(L, "bh_mlp_train_step"):
+        L.bh_mlp_train_step.restype = i32
+        L.bh_mlp_train_step.argtypes = [vp, vp, i64, vp, vp, vp, vp, i64, f64, f64, f64, f64,
+                                        vp, vp]
+
+
+def lib() -> ctypes.CDLL:
+    """Load libbasinhip.so once; raise HipLibraryError if it is absent."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    with _lock:
+        if _lib is None:
+            if not os.path.exists(LIB_PATH):
+                raise HipLibraryError(
+                    f"{LIB_PATH} is not built; run `python -m pybasin_amd.build` "
+                    "(the HIP path has no CPU fallback)")
+            L = ctypes.CDLL(LIB_PATH)
+            _declare(L)
+            _lib = L
+    return _lib
+
+
+def check(rc: int, what: str) -> None:
+    if rc != 0:
+        msg = lib().bh_last_error().decode(errors="replace")
+        if rc == -5:
+            raise ValueError(f"{what}: {msg}")
+        raise HipLibraryError(f"{what} failed ({rc}): {msg}")
+
+
+def ptr(t) -> int | None:
+    return None if t is None else t.data_ptr()

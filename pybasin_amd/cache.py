@@ -1,0 +1,67 @@
+"""On-disk cache of integration results (src/pybasin/cache_manager.py:27-128).
+
+Key = md5 over the solver name, the ODE description, the solver config,
+the bytes of y0 / save_ts / params; value = a safetensors file holding t and y.
+A corrupt file is deleted and the result recomputed.
+"""
+
+from __future__ import annotations
+
+import hashlib
+import json
+import logging
+import os
+
+import torch
+
+logger = logging.getLogger("pybasin_amd")
+
+
+class CacheManager:
+    def __init__(self, cache_dir: str):
+        self.cache_dir = cache_dir
+
+    def build_key(self, solver_name, ode_system, y0, save_ts, solver_config, params=None) -> str:
+        h = hashlib.md5()
+        h.update(solver_name.encode())
+        h.update(getattr(ode_system, "get_str", lambda: type(ode_system).__name__)().encode())
+        h.update(json.dumps(solver_config, sort_keys=True, default=str).encode())
+        for t in (y0, save_ts, params):
+            if t is not None:
+                h.update(str(t.dtype).encode())
+                h.update(t.detach().cpu().contiguous().numpy().tobytes())
+        return h.hexdigest()
+
+    def _path(self, key: str) -> str:
+        return os.path.join(self.cache_dir, f"{key}.safetensors")
+
+    def load(self, key: str, device):
+        from safetensors.torch import load_file
+
+        path = self._path(key)
+        if not os.path.exists(path):
+            return None
+        try:
+            d = load_file(path)
+            return d["t"].to(device), d["y"].to(device)
+        except Exception as exc:  # corrupt cache entry: drop it and recompute
+            logger.warning("removing corrupt cache file %s (%s)", path, exc)
+            os.remove(path)
+            return None
+
+    def save(self, key: str, t: torch.Tensor, y: torch.Tensor) -> None:
+        from safetensors.torch import save_file
+
+        os.makedirs(self.cache_dir, exist_ok=True)
+        save_file({"t": t.detach().cpu().contiguous(), "y": y.detach().cpu().contiguous()},
+                  self._path(key))
+
+    def cached_call(self, solver_name, ode_system, y0, save_ts, solver_config, device,
+                    compute_fn, params=None):
+        key = self.build_key(solver_name, ode_system, y0, save_ts, solver_config, params)
+        hit = self.load(key, device)
+        if hit is not None:
+            return hit
+        t, y = compute_fn()
+        self.save(key, t, y)
+        return t, y

@@ -1,0 +1,158 @@
+// basinhip_api.hip -- C-ABI entry points of libbasinhip.so (see include/basinhip.h).
+#include <hip/hip_runtime.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include "../../include/basinhip.h"
+#include "dopri5_lane.hip"
+
+namespace {
+
+thread_local char g_err[512] = "";
+
+int32_t fail(int32_t code, const char *fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+#define BH_HIP_CHECK(expr)                                                              \
+    do {                                                                                \
+        hipError_t e_ = (expr);                                                         \
+        if (e_ != hipSuccess)                                                           \
+            return fail(-6, "%s failed: %s", #expr, hipGetErrorString(e_));             \
+    } while (0)
+
+// resident lanes that fill the device for a kernel, from its occupancy
+template <class K>
+int32_t device_lanes(K kernel, int block, int64_t *lanes) {
+    int dev = 0, cus = 0, per_cu = 0;
+    BH_HIP_CHECK(hipGetDevice(&dev));
+    BH_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    BH_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block, 0));
+    if (per_cu < 1) per_cu = 1;
+    *lanes = (int64_t)cus * per_cu * block;
+    return 0;
+}
+
+template <class Sys, class T>
+int32_t launch_lane(const bh::LaneArgs &args, int32_t max_lanes, hipStream_t stream) {
+    constexpr int block = 256;
+    auto kernel = bh::dopri5_lane_kernel<Sys, T>;
+    int64_t lanes = 0;
+    int32_t rc = device_lanes(kernel, block, &lanes);
+    if (rc) return rc;
+    if (max_lanes > 0 && max_lanes < lanes) lanes = max_lanes;
+    if (lanes > args.n) lanes = args.n;
+    const int64_t blocks = (lanes + block - 1) / block;
+    // lanes [0, blocks*block) start on their own index; the counter hands out the rest
+    hipLaunchKernelGGL(bh::set_counter_kernel, dim3(1), dim3(64), 0, stream, args.counter,
+                       (unsigned long long)(blocks * block));
+    BH_HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(kernel, dim3((unsigned)blocks), dim3(block), 0, stream, args);
+    BH_HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
+template <class Sys>
+int32_t dispatch_dtype(int32_t dtype, const bh::LaneArgs &args, int32_t max_lanes,
+                       hipStream_t s) {
+    if (dtype == BH_F32) return launch_lane<Sys, float>(args, max_lanes, s);
+    if (dtype == BH_F64) return launch_lane<Sys, double>(args, max_lanes, s);
+    return fail(-5, "dtype must be BH_F32 or BH_F64, got %d", dtype);
+}
+
+struct SysInfo {
+    int s, np;
+};
+
+int32_t check_system(int32_t sys, int32_t dim, int32_t n_params) {
+    SysInfo info;
+    switch (sys) {
+    case BH_SYS_DECAY:
+        if (dim != 1 && dim != 2) return fail(-5, "decay system supports dim 1 or 2, got %d", dim);
+        info = {dim, 1};
+        break;
+    case BH_SYS_LINEAR2: info = {2, 4}; break;
+    case BH_SYS_PENDULUM: info = {2, 3}; break;
+    case BH_SYS_DUFFING: info = {2, 3}; break;
+    case BH_SYS_LORENZ: info = {3, 3}; break;
+    default: return fail(-1, "system id %d has no per-lane device functor", sys);
+    }
+    if (dim != info.s) return fail(-5, "system %d has state dim %d, got %d", sys, info.s, dim);
+    if (n_params != info.np)
+        return fail(-5, "system %d takes %d parameters, got %d", sys, info.np, n_params);
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t bh_version(void) { return 10000; }
+
+const char *bh_last_error(void) { return g_err; }
+
+int32_t bh_solve_batch(int32_t system_id, int32_t dtype, const void *y0s, int64_t n, int32_t dim,
+                       const double *params, int32_t n_params, int64_t params_stride, double t0,
+                       const double *save_ts, int32_t n_save, double rtol, double atol,
+                       int64_t max_steps, uint32_t flags, double stop_threshold,
+                       int32_t steady_idx, void *out_traj, double *out_feats, void *out_final,
+                       int32_t *out_status, int64_t *out_nsteps, void *workspace,
+                       int64_t workspace_bytes, int32_t max_lanes, void *stream) {
+    g_err[0] = 0;
+    int32_t rc = check_system(system_id, dim, n_params);
+    if (rc) return rc;
+    if (n < 0) return fail(-5, "n must be >= 0, got %lld", (long long)n);
+    if (n == 0) return 0;
+    if (!y0s || !params || !save_ts) return fail(-5, "y0s, params and save_ts are required");
+    if (n_save < 1) return fail(-5, "n_save must be >= 1, got %d", n_save);
+    if (params_stride != 0 && params_stride < n_params)
+        return fail(-5, "params_stride %lld < n_params %d", (long long)params_stride, n_params);
+    if (!(rtol >= 0.0) || !(atol >= 0.0) || rtol + atol <= 0.0)
+        return fail(-5, "rtol/atol must be >= 0 with rtol + atol > 0");
+    if (max_steps < 1) return fail(-5, "max_steps must be >= 1");
+    if (!workspace || workspace_bytes < BH_WORKSPACE_BYTES)
+        return fail(-5, "workspace of at least %d bytes is required", BH_WORKSPACE_BYTES);
+    if ((flags & BH_FLAG_STOP) && !(stop_threshold > 0.0))
+        return fail(-5, "BH_FLAG_STOP needs stop_threshold > 0");
+    if (max_lanes < 0) return fail(-5, "max_lanes must be >= 0");
+
+    bh::LaneArgs a;
+    a.y0 = y0s;
+    a.params = params;
+    a.p_stride = params_stride;
+    a.n = n;
+    a.save_t = save_ts;
+    a.n_save = n_save;
+    a.steady_idx = steady_idx < 0 ? 0 : steady_idx;
+    a.t0 = t0;
+    a.rtol = rtol;
+    a.atol = atol;
+    a.max_steps = max_steps;
+    a.flags = flags;
+    a.stop_thr = stop_threshold;
+    a.traj = out_traj;
+    a.feats = (steady_idx < n_save) ? out_feats : nullptr;
+    a.fin = out_final;
+    a.status = out_status;
+    a.nsteps = out_nsteps;
+    a.counter = reinterpret_cast<unsigned long long *>(workspace);
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+
+    switch (system_id) {
+    case BH_SYS_DECAY:
+        return dim == 1 ? dispatch_dtype<bh::Decay<1>>(dtype, a, max_lanes, s)
+                        : dispatch_dtype<bh::Decay<2>>(dtype, a, max_lanes, s);
+    case BH_SYS_LINEAR2: return dispatch_dtype<bh::Linear2>(dtype, a, max_lanes, s);
+    case BH_SYS_PENDULUM: return dispatch_dtype<bh::Pendulum>(dtype, a, max_lanes, s);
+    case BH_SYS_DUFFING: return dispatch_dtype<bh::Duffing>(dtype, a, max_lanes, s);
+    case BH_SYS_LORENZ: return dispatch_dtype<bh::Lorenz>(dtype, a, max_lanes, s);
+    default: return fail(-1, "unknown system id %d", system_id);
+    }
+}
+
+}  // extern "C"

@@ -1,0 +1,192 @@
+"""HipDopri5Solver: per-lane adaptive Dormand-Prince 5(4) on MI355X.
+
+Drop-in ``Solver`` (src/pybasin/solvers/base.py:16-213, protocol in
+src/pybasin/protocols.py:195-266) whose ``_integrate`` replaces
+TorchDiffEqSolver._integrate (src/pybasin/solvers/torchdiffeq_solver.py:97-134).
+
+* Systems with a device functor (``ode_system.device_system_id``) run in the
+  HIP kernel of libbasinhip.so: one trajectory per lane, torchdiffeq's step
+  rules per lane, streaming features fused into the save loop.  If the
+  library is missing this raises; there is no silent fallback.
+* The 100-node Rossler network runs in the tile kernel (MFMA coupling GEMM).
+* Systems without a functor, and solvers cloned to the CPU
+  (TemplateIntegrator, src/pybasin/template_integrator.py:79-82), run on the
+  torch shared-step Dopri5 (``TorchDopri5Solver``) -- by construction, logged.
+
+The solver holds no per-call state, so two threads may call ``integrate``
+concurrently (the reference's main + template integration,
+src/pybasin/basin_stability_estimator.py:498-512); ctypes releases the GIL
+during the launch.
+"""
+
+from __future__ import annotations
+
+import logging
+import weakref
+from dataclasses import dataclass
+from typing import Any
+
+import torch
+
+from .. import _lib
+from .base import DEFAULT_CACHE_DIR, UNSET, Solver
+from .torch_dopri5 import TorchDopri5Solver
+
+logger = logging.getLogger("pybasin_amd")
+
+# fused results attached to returned trajectory tensors (read by
+# HipFusedFeatureExtractor); keyed weakly so nothing outlives the tensor
+FUSED_RESULTS: "weakref.WeakKeyDictionary[torch.Tensor, FusedResult]" = weakref.WeakKeyDictionary()
+
+
+@dataclass
+class FusedResult:
+    """Outputs of one fused HIP integration.
+
+    t       (n_save,) save grid in the state dtype
+    y       (n_save, n, S) trajectories, or None when not kept
+    final   (n, S) state at the last save time
+    stats   (n, S, 4) float64: mean, std (ddof 0), min, max over the window
+    status  (n,) int32 lane status (BH_LANE_*)
+    nsteps  (n,) int64 attempted steps per trajectory
+    steady_idx  first save index of the feature window
+    """
+
+    t: torch.Tensor
+    y: torch.Tensor | None
+    final: torch.Tensor
+    stats: torch.Tensor | None
+    status: torch.Tensor
+    nsteps: torch.Tensor
+    steady_idx: int
+
+
+def steady_index(save_ts: torch.Tensor, time_steady: float | None) -> int:
+    """filter_time's start index (src/pybasin/feature_extractors/feature_extractor.py:85-130):
+    first save time >= time_steady (default t0 + 0.85 (t1 - t0)); 0 if none."""
+    if time_steady is None:
+        t0, t1 = float(save_ts[0]), float(save_ts[-1])
+        time_steady = t0 + 0.85 * (t1 - t0)
+    idx = torch.nonzero(save_ts.cpu() >= time_steady)
+    return int(idx[0, 0]) if len(idx) else 0
+
+
+class HipDopri5Solver(Solver):
+    display_name = "HIP Dopri5 (per lane)"
+
+    def __init__(
+        self,
+        t_span: tuple[float, float] = (0, 1000),
+        t_steps: int = 1000,
+        device: str | None = None,
+        rtol: float = 1e-8,
+        atol: float = 1e-6,
+        cache_dir: str | None = DEFAULT_CACHE_DIR,
+        t_eval: tuple[float, float] | None = None,
+        *,
+        max_steps: int = 16**5,
+        stop_threshold: float | None = None,
+        inf_fill: bool = False,
+        max_lanes: int = 0,
+    ):
+        super().__init__(t_span, t_steps, device, rtol, atol, cache_dir, t_eval)
+        self.max_steps = int(max_steps)
+        self.stop_threshold = stop_threshold
+        self.inf_fill = inf_fill
+        self.max_lanes = int(max_lanes)
+
+    def _get_cache_config(self) -> dict[str, Any]:
+        cfg = super()._get_cache_config()
+        cfg.update(method="hip_dopri5_lane", stop_threshold=self.stop_threshold,
+                   inf_fill=self.inf_fill, max_steps=self.max_steps)
+        return cfg
+
+    def clone(self, *, device=None, t_steps_factor=1, cache_dir=UNSET) -> "HipDopri5Solver":
+        return HipDopri5Solver(
+            t_span=self.t_span, t_steps=self.t_steps * t_steps_factor,
+            device=device or self._device_str, rtol=self.rtol, atol=self.atol,
+            cache_dir=self._cache_dir if cache_dir is UNSET else cache_dir,  # type: ignore[arg-type]
+            t_eval=self.t_eval, max_steps=self.max_steps, stop_threshold=self.stop_threshold,
+            inf_fill=self.inf_fill, max_lanes=self.max_lanes)
+
+    # ------------------------------------------------------------------ #
+    def _torch_solver(self) -> TorchDopri5Solver:
+        return TorchDopri5Solver(t_span=self.t_span, t_steps=self.t_steps,
+                                 device=str(self.device), rtol=self.rtol, atol=self.atol,
+                                 cache_dir=None, t_eval=self.t_eval)
+
+    def _integrate(self, ode_system, y0, save_ts, params=None):
+        sys_id = getattr(ode_system, "device_system_id", None)
+        if self.device.type != "cuda" or sys_id is None:
+            logger.info("[%s] %s on %s: torch shared-step Dopri5", type(self).__name__,
+                        type(ode_system).__name__, self.device)
+            return self._torch_solver()._integrate(ode_system, y0, save_ts, params)
+        res = self.run_fused(ode_system, y0, save_ts, params, keep_trajectories=True)
+        FUSED_RESULTS[res.y] = res
+        return save_ts, res.y
+
+    def integrate_fused(self, ode_system, y0: torch.Tensor, params: torch.Tensor | None = None,
+                        *, time_steady: float | None = None,
+                        keep_trajectories: bool = False) -> FusedResult:
+        """integrate() with the feature window fused into the kernel; trajectories
+        reach HBM only when keep_trajectories is set."""
+        if self.device.type != "cuda":
+            raise RuntimeError("integrate_fused runs on a HIP device; use integrate() on CPU")
+        y0, params = self._expand(y0, params)
+        save_ts, y0 = self._prepare_tensors(y0)
+        return self.run_fused(ode_system, y0, save_ts, params,
+                              keep_trajectories=keep_trajectories, time_steady=time_steady)
+
+    def run_fused(self, ode_system, y0: torch.Tensor, save_ts: torch.Tensor,
+                  params: torch.Tensor | None, *, keep_trajectories: bool,
+                  time_steady: float | None = None, steady_idx: int | None = None,
+                  stream: torch.cuda.Stream | None = None) -> FusedResult:
+        sys_id = getattr(ode_system, "device_system_id", None)
+        if sys_id is None:
+            raise ValueError(f"{type(ode_system).__name__} has no HIP device functor")
+        if sys_id == _lib.BH_SYS_ROSSLER100:
+            from .rossler import run_rossler  # MFMA tile kernel
+
+            return run_rossler(self, ode_system, y0, save_ts, params,
+                               keep_trajectories=keep_trajectories, stream=stream)
+        dev = self.device
+        if y0.dtype not in (torch.float32, torch.float64):
+            raise ValueError(f"y0 dtype must be float32 or float64, got {y0.dtype}")
+        dtype_id = _lib.BH_F32 if y0.dtype == torch.float32 else _lib.BH_F64
+        y0 = y0.to(dev).contiguous()
+        n, S = y0.shape
+        p, p_stride = ode_system.device_params(params)
+        p = p.to(dev).contiguous()
+        if p_stride and p.shape[0] != n:
+            raise ValueError(f"params rows {p.shape[0]} != trajectories {n}")
+        st64 = save_ts.to(device=dev, dtype=torch.float64).contiguous()
+        n_save = st64.shape[0]
+        if steady_idx is None:
+            steady_idx = steady_index(save_ts, time_steady)
+        traj = torch.empty((n_save, n, S), dtype=y0.dtype, device=dev) if keep_trajectories else None
+        stats = torch.empty((n, S, _lib.BH_NFEAT), dtype=torch.float64, device=dev)
+        final = torch.empty((n, S), dtype=y0.dtype, device=dev)
+        status = torch.empty((n,), dtype=torch.int32, device=dev)
+        nsteps = torch.empty((n,), dtype=torch.int64, device=dev)
+        ws = torch.empty((_lib.BH_WORKSPACE_BYTES,), dtype=torch.uint8, device=dev)
+        flags = 0
+        thr = 0.0
+        if self.stop_threshold is not None:
+            flags |= _lib.BH_FLAG_STOP
+            thr = float(self.stop_threshold)
+            if self.inf_fill:
+                flags |= _lib.BH_FLAG_INF_FILL
+        s = stream if stream is not None else torch.cuda.current_stream(dev)
+        with torch.cuda.device(dev):
+            rc = _lib.lib().bh_solve_batch(
+                int(sys_id), dtype_id, y0.data_ptr(), n, S, p.data_ptr(), p.shape[1], p_stride,
+                float(self.t_span[0]), st64.data_ptr(), n_save, float(self.rtol),
+                float(self.atol), self.max_steps, flags, thr, int(steady_idx),
+                _lib.ptr(traj), stats.data_ptr(), final.data_ptr(), status.data_ptr(),
+                nsteps.data_ptr(), ws.data_ptr(), ws.numel(), self.max_lanes, s.cuda_stream)
+        _lib.check(rc, "bh_solve_batch")
+        for t in (y0, p, st64, ws):
+            t.record_stream(s)
+        return FusedResult(t=save_ts, y=traj, final=final,
+                           stats=stats if steady_idx < n_save else None, status=status,
+                           nsteps=nsteps, steady_idx=int(steady_idx))

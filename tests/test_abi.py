@@ -1,0 +1,51 @@
+"""The C-ABI library loads and exports every symbol include/*.h declares (no GPU compute)."""
+
+import ctypes
+import glob
+import os
+import re
+
+import pytest
+
+from pybasin_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    names = set()
+    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+        src = open(h).read()
+        src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+        names |= set(re.findall(r"\b(bh_[a-z0-9_]+)\s*\(", src))
+    return sorted(names)
+
+
+def test_header_declares_entry_points():
+    syms = declared_symbols()
+    assert "bh_solve_batch" in syms and "bh_version" in syms
+    assert set(syms) == set(_lib.EXPORTS)
+
+
+def test_library_exports_every_declared_symbol():
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.fail(f"{_lib.LIB_PATH} missing: run python -m pybasin_amd.build")
+    L = ctypes.CDLL(_lib.LIB_PATH)
+    for s in declared_symbols():
+        assert hasattr(L, s), s
+
+
+def test_version_and_argument_errors_without_gpu():
+    L = _lib.lib()
+    assert L.bh_version() >= 10000
+    # invalid system id is rejected before any HIP call
+    rc = L.bh_solve_batch(99, 0, None, 1, 2, None, 3, 0, 0.0, None, 1, 1e-8, 1e-6, 10, 0, 0.0, 0,
+                          None, None, None, None, None, None, 0, 0, None)
+    assert rc == -1
+    assert b"system" in L.bh_last_error()
+    # wrong state dimension for the pendulum
+    rc = L.bh_solve_batch(2, 0, None, 1, 3, None, 3, 0, 0.0, None, 1, 1e-8, 1e-6, 10, 0, 0.0, 0,
+                          None, None, None, None, None, None, 0, 0, None)
+    assert rc == -5
+    with pytest.raises(ValueError):
+        _lib.check(rc, "bh_solve_batch")

@@ -1,0 +1,198 @@
+"""GPU parity of the per-lane HIP Dopri5 kernel (libbasinhip.so) against the CPU oracle.
+
+The oracle's per-lane torchdiffeq mode applies the same step rules to each
+IC; the kernel differs only in rounding (FMA contraction, device sin/cos,
+single-precision controller power), so trajectories agree to a small multiple
+of the tolerance on short horizons, and the fused statistics must equal the
+statistics of the kernel's own written trajectory.
+"""
+
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as O
+from pybasin_amd import _lib
+from pybasin_amd.solvers import HipDopri5Solver
+from pybasin_amd.systems import DuffingODE, ExponentialDecayODE, Linear2ODE, LorenzODE, PendulumODE
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+PEND = {"alpha": 0.1, "T": 0.5, "K": 1.0}
+DUFF = {"delta": 0.08, "k3": 1.0, "A": 0.2}
+LOR = {"sigma": 0.12, "r": 0.0, "b": -0.6}
+
+CASES = {
+    # name: (ode, oracle sys, lo, hi, t_end)
+    "pendulum": (PendulumODE(PEND), O.SYS_PENDULUM, [-math.pi + math.asin(0.5), -10.0],
+                 [math.pi + math.asin(0.5), 10.0], 20.0),
+    "duffing": (DuffingODE(DUFF), O.SYS_DUFFING, [-1.0, -0.5], [1.0, 1.0], 20.0),
+    "lorenz": (LorenzODE(LOR), O.SYS_LORENZ, [-10.0, -20.0, 0.0], [10.0, 20.0, 0.0], 10.0),
+    "linear2": (Linear2ODE({"m00": -0.1, "m01": 1.0, "m10": -1.0, "m11": -0.1}), O.SYS_LINEAR2,
+                [-1.0, -1.0], [1.0, 1.0], 20.0),
+}
+
+
+def _oracle_params(ode):
+    return np.array([float(v) for v in ode.params.values()])
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+@pytest.mark.parametrize("name", list(CASES))
+def test_trajectories_match_oracle_short_horizon(cuda_device, name, dtype):
+    ode, sysid, lo, hi, t_end = CASES[name]
+    print(f"[case] {name} {dtype}", flush=True)
+    y0 = torch.tensor(O.sample_uniform(64, lo, hi, seed=7), dtype=dtype)
+    solver = HipDopri5Solver(t_span=(0, t_end), t_steps=201, device="cuda", cache_dir=None)
+    t, y = solver.integrate(ode, y0.to(cuda_device))
+    assert y.shape == (201, 64, len(lo)) and y.dtype == dtype
+    bits = 32 if dtype == torch.float32 else 64
+    ref, st, _ = O.solve(sysid, y0.double().numpy(), _oracle_params(ode), t.double().cpu().numpy(),
+                         0.0, bits=bits)
+    assert (st == 0).all()
+    got = y.double().cpu().numpy()
+    scale = 1.0 + np.abs(ref)
+    # float64: agreement at a few times the local tolerance; float32: a few
+    # float32 ulps of the state magnitude times the step count
+    tol = 2e-6 if bits == 64 else 2e-4
+    err = (np.abs(got - ref) / scale).max()
+    print(f"[result] {name} {dtype} max scaled err {err:.3e}", flush=True)
+    assert err < tol
+    # save time t0 returns y0 exactly
+    assert torch.equal(y[0].cpu(), y0)
+
+
+def test_pendulum_golden_ics_float64(cuda_device):
+    with open(os.path.join(GOLD, "zigode_pendulum_golden.json")) as f:
+        gold = json.load(f)
+    ics = torch.tensor(gold["ics"], dtype=torch.float64)
+    solver = HipDopri5Solver(t_span=(0, 1000), t_steps=1000, device="cuda", cache_dir=None)
+    t, y = solver.integrate(PendulumODE(PEND), ics.to(cuda_device))
+    got = y.cpu().numpy()
+    ref, _, _ = O.solve(O.SYS_PENDULUM, ics.numpy(), _oracle_params(PendulumODE(PEND)),
+                        t.cpu().numpy(), 0.0, bits=64)
+    tol = gold["tolerances"]
+    # against the oracle's per-lane run: the same algorithm, rounding-level drift
+    assert np.abs(got - ref).max() < 1e-3
+    # against the reference's published final angles (zigode per IC and
+    # torchdiffeq shared-step) within the reference's own cross-solver bounds
+    for key in ("zig", "torchdiffeq"):
+        assert np.abs(got[-1, :, 0] - np.array(gold["final_theta"][key])).max() < tol["CROSS_SOLVER_ATOL"]
+    np.testing.assert_allclose(got[-1, [0, 4], 0], math.asin(0.5), atol=tol["FIXED_POINT_ATOL"])
+    np.testing.assert_allclose(got[-1, [1, 2, 3], 1], ref[-1, [1, 2, 3], 1],
+                               atol=tol["LIMIT_CYCLE_OMEGA_ATOL"])
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_fused_features_equal_statistics_of_written_window(cuda_device, dtype):
+    y0 = torch.tensor(O.sample_uniform(512, [-1.0, -0.5], [1.0, 1.0], seed=3), dtype=dtype)
+    solver = HipDopri5Solver(t_span=(0, 200), t_steps=500, t_eval=(150.0, 200.0), device="cuda",
+                             cache_dir=None)
+    res = solver.integrate_fused(DuffingODE(DUFF), y0.to(cuda_device), time_steady=170.0,
+                                 keep_trajectories=True)
+    si = res.steady_idx
+    assert si == O.filter_time_idx(res.t.cpu().numpy(), 170.0)
+    w = res.y[si:].double().cpu().numpy()
+    f = O.features(w)
+    stats = res.stats.cpu().numpy()
+    for j, key in enumerate(["mean", "standard_deviation", "minimum", "maximum"]):
+        np.testing.assert_allclose(stats[:, :, j], f[key], rtol=1e-9, atol=1e-9, err_msg=key)
+    assert torch.equal(res.final, res.y[-1])
+    assert (res.status.cpu() == 0).all()
+    assert (res.nsteps.cpu() > 0).all()
+
+
+def test_lane_refill_is_deterministic(cuda_device):
+    """The same ICs through 64 resident lanes (heavy refill) and through a
+    full device give bitwise identical results."""
+    y0 = torch.tensor(O.sample_uniform(1000, [-1.0, -0.5], [1.0, 1.0], seed=11)).to(cuda_device)
+    a = HipDopri5Solver(t_span=(0, 100), t_steps=100, device="cuda", cache_dir=None)
+    b = HipDopri5Solver(t_span=(0, 100), t_steps=100, device="cuda", cache_dir=None, max_lanes=64)
+    ra = a.integrate_fused(DuffingODE(DUFF), y0, keep_trajectories=True)
+    rb = b.integrate_fused(DuffingODE(DUFF), y0, keep_trajectories=True)
+    assert torch.equal(ra.y, rb.y) and torch.equal(ra.nsteps, rb.nsteps)
+    assert torch.equal(ra.stats, rb.stats)
+
+
+def test_lorenz_stop_freeze_matches_masked_oracle(cuda_device):
+    """Per-lane stop at |y|inf >= 200 with the frozen state filled in equals
+    integrating the masked RHS to the end (lorenz_ode.py:146-152)."""
+    y0 = torch.tensor(O.sample_uniform(128, [-10.0, -20.0, 0.0], [10.0, 20.0, 0.0], seed=5))
+    stop = HipDopri5Solver(t_span=(0, 60), t_steps=120, device="cuda", cache_dir=None,
+                           stop_threshold=200.0)
+    t, y = stop.integrate(LorenzODE(LOR), y0.to(cuda_device))
+    res = stop.integrate_fused(LorenzODE(LOR), y0.to(cuda_device), keep_trajectories=True)
+    status = res.status.cpu().numpy()
+    assert (status == _lib.BH_LANE_DIVERGED).any() and (status == 0).any()
+    ref, _, _ = O.solve(O.SYS_LORENZ, y0.double().numpy(), _oracle_params(LorenzODE(LOR)),
+                        t.double().cpu().numpy(), 0.0, bits=32)
+    got = y.double().cpu().numpy()
+    div = status == _lib.BH_LANE_DIVERGED
+    # diverged lanes: the frozen values beyond 200 agree with the masked run
+    assert (np.abs(ref[-1, div]).max(axis=1) >= 200).all()
+    agree = np.abs(got[-1] - ref[-1]).max(axis=1) <= 1e-3 * (1 + np.abs(ref[-1]).max(axis=1))
+    assert agree.mean() > 0.95
+    # Inf-fill variant (JaxSolver Event semantics)
+    inf = HipDopri5Solver(t_span=(0, 60), t_steps=120, device="cuda", cache_dir=None,
+                          stop_threshold=200.0, inf_fill=True)
+    r2 = inf.integrate_fused(LorenzODE(LOR), y0.to(cuda_device), keep_trajectories=True)
+    yy = r2.y.cpu()
+    d2 = r2.status.cpu().numpy() == _lib.BH_LANE_DIVERGED
+    assert torch.isinf(yy[-1, torch.from_numpy(d2)]).all()
+    assert torch.isfinite(yy[:, torch.from_numpy(~d2)]).all()
+
+
+def test_status_max_steps_and_nonfinite(cuda_device):
+    y0 = torch.tensor([[0.3, 0.2], [1.0, 0.0]], device=cuda_device)
+    s = HipDopri5Solver(t_span=(0, 100), t_steps=10, device="cuda", cache_dir=None, max_steps=5)
+    r = s.integrate_fused(PendulumODE(PEND), y0, keep_trajectories=True)
+    assert (r.status.cpu() == _lib.BH_LANE_MAX_STEPS).all()
+    assert (r.nsteps.cpu() == 5).all()
+    bad = torch.tensor([[float("nan"), 0.0]], device=cuda_device)
+    s2 = HipDopri5Solver(t_span=(0, 1), t_steps=5, t_eval=(0.5, 1.0), device="cuda", cache_dir=None)
+    r2 = s2.integrate_fused(PendulumODE(PEND), bad, keep_trajectories=True)
+    assert int(r2.status.cpu()[0]) == _lib.BH_LANE_NONFINITE
+
+
+# ---- the reference's solver contract (tests/unit/solvers/test_solver.py) on the HIP path ----
+
+def test_contract_decay(cuda_device):
+    solver = HipDopri5Solver(t_span=(0, 1), t_steps=10, device="cuda", cache_dir=None)
+    t, y = solver.integrate(ExponentialDecayODE({"decay": -1.0}), torch.tensor([[1.0]], device=cuda_device))
+    assert t.shape == (10,) and y.shape == (10, 1, 1)
+    assert y[0].item() == pytest.approx(1.0, abs=1e-5)
+    assert y[-1].item() == pytest.approx(math.exp(-1), abs=1e-6)
+
+
+def test_contract_t_eval_subrange(cuda_device):
+    solver = HipDopri5Solver(t_span=(0, 2), t_steps=10, t_eval=(1, 2), device="cuda", cache_dir=None)
+    t, y = solver.integrate(ExponentialDecayODE({"decay": -1.0}), torch.tensor([[1.0]], device=cuda_device))
+    assert t[0].item() == pytest.approx(1.0) and t[-1].item() == pytest.approx(2.0)
+    assert y[0].item() == pytest.approx(math.exp(-1), abs=1e-6)
+
+
+def test_contract_params_ic_major(cuda_device):
+    solver = HipDopri5Solver(t_span=(0, 1), t_steps=11, device="cuda", cache_dir=None)
+    y0 = torch.tensor([[1.0], [2.0]], device=cuda_device)
+    params = torch.tensor([[-1.0], [-2.0]])
+    _, y = solver.integrate(ExponentialDecayODE({"decay": -1.0}), y0, params=params)
+    assert y.shape == (11, 4, 1)
+    want = [math.exp(-1), math.exp(-2), 2 * math.exp(-1), 2 * math.exp(-2)]
+    np.testing.assert_allclose(y[-1, :, 0].cpu().numpy(), want, rtol=1e-5)
+    # linearity: trajectory of y0=2 is twice that of y0=1 for each parameter
+    np.testing.assert_allclose((y[:, 2] / y[:, 0]).cpu().numpy(), 2.0, rtol=1e-5)
+
+
+def test_contract_errors(cuda_device):
+    solver = HipDopri5Solver(t_span=(0, 1), t_steps=10, device="cuda", cache_dir=None)
+    with pytest.raises(ValueError):
+        solver.integrate(ExponentialDecayODE({"decay": -1.0}), torch.tensor([1.0], device=cuda_device))
+    with pytest.raises(ValueError, match="t_eval"):
+        HipDopri5Solver(t_span=(0, 1), t_steps=10, t_eval=(0.5, 1.5), device="cuda", cache_dir=None)
+    c = HipDopri5Solver(t_span=(0, 2), t_eval=(1, 2), device="cuda", cache_dir=None).clone(device="cpu")
+    assert c.t_eval == (1, 2) and c.device.type == "cpu"
