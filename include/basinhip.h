@@ -91,6 +91,32 @@ int32_t bh_solve_batch(int32_t system_id, int32_t dtype, const void *y0s, int64_
                        int32_t *out_status, int64_t *out_nsteps, void *workspace,
                        int64_t workspace_bytes, int32_t max_lanes, void *stream);
 
+/*
+ * bh_solve_rossler -- the 100-node Rossler network (BH_SYS_ROSSLER100, state
+ * layout [x_0..x_99, y_0..y_99, z_0..z_99]), 16 trajectories per workgroup,
+ * coupling K (L x) as a (16 x 100) . (100 x 100) FP32 matrix-core product per
+ * stage.  Replaces the integration of RosslerNetworkJaxODE
+ * (case_studies/rossler_network/rossler_network_jax_ode.py:53-79) under the
+ * reference's solver call (src/pybasin/solvers/base.py:129-183); only the
+ * state at t_end is produced (the synchronisation features use the final
+ * step, synchronization_feature_extractor.py:73-106).
+ *
+ *   dtype          BH_F32 only
+ *   y0s            (n, 300) float32
+ *   params         (n, 4) float64 [a, b, c, K] with row stride params_stride
+ *                  (0 = one shared vector)
+ *   laplacian      (100, 100) float32, row-major, symmetric
+ *   t0, t_end      integration start and the time of the returned state
+ *   out_final      (n, 300) float32; status / nsteps as in bh_solve_batch
+ *   max_tiles      cap on resident workgroups (0 = fill the device)
+ */
+int32_t bh_solve_rossler(int32_t dtype, const void *y0s, int64_t n, const double *params,
+                         int64_t params_stride, const float *laplacian, double t0, double t_end,
+                         double rtol, double atol, int64_t max_steps, uint32_t flags,
+                         double stop_threshold, void *out_final, int32_t *out_status,
+                         int64_t *out_nsteps, void *workspace, int64_t workspace_bytes,
+                         int32_t max_tiles, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -22,7 +22,7 @@ BH_NFEAT = 4
 BH_WORKSPACE_BYTES = 256
 
 # every symbol include/basinhip.h declares
-EXPORTS = ("bh_version", "bh_last_error", "bh_solve_batch")
+EXPORTS = ("bh_version", "bh_last_error", "bh_solve_batch", "bh_solve_rossler")
 
 _lock = threading.Lock()
 _lib: ctypes.CDLL | None = None

@@ -6,6 +6,7 @@
 
 #include "../../include/basinhip.h"
 #include "dopri5_lane.hip"
+#include "rossler_tile.hip"
 
 namespace {
 
@@ -153,6 +154,63 @@ int32_t bh_solve_batch(int32_t system_id, int32_t dtype, const void *y0s, int64_
     case BH_SYS_LORENZ: return dispatch_dtype<bh::Lorenz>(dtype, a, max_lanes, s);
     default: return fail(-1, "unknown system id %d", system_id);
     }
+}
+
+int32_t bh_solve_rossler(int32_t dtype, const void *y0s, int64_t n, const double *params,
+                         int64_t params_stride, const float *laplacian, double t0, double t_end,
+                         double rtol, double atol, int64_t max_steps, uint32_t flags,
+                         double stop_threshold, void *out_final, int32_t *out_status,
+                         int64_t *out_nsteps, void *workspace, int64_t workspace_bytes,
+                         int32_t max_tiles, void *stream) {
+    g_err[0] = 0;
+    namespace R = bh::ross;
+    if (dtype != BH_F32) return fail(-5, "the Rossler network kernel runs in float32 only");
+    if (n < 0) return fail(-5, "n must be >= 0");
+    if (n == 0) return 0;
+    if (!y0s || !params || !laplacian || !out_final)
+        return fail(-5, "y0s, params, laplacian and out_final are required");
+    if (params_stride != 0 && params_stride < 4) return fail(-5, "params_stride must be 0 or >= 4");
+    if (!(rtol >= 0.0) || !(atol >= 0.0) || rtol + atol <= 0.0)
+        return fail(-5, "rtol/atol must be >= 0 with rtol + atol > 0");
+    if (!(t_end >= t0)) return fail(-5, "t_end must be >= t0");
+    if (max_steps < 1) return fail(-5, "max_steps must be >= 1");
+    if (!workspace || workspace_bytes < BH_WORKSPACE_BYTES)
+        return fail(-5, "workspace of at least %d bytes is required", BH_WORKSPACE_BYTES);
+    if ((flags & BH_FLAG_STOP) && !(stop_threshold > 0.0))
+        return fail(-5, "BH_FLAG_STOP needs stop_threshold > 0");
+    if (max_tiles < 0) return fail(-5, "max_tiles must be >= 0");
+    R::Args a;
+    a.y0 = reinterpret_cast<const float *>(y0s);
+    a.params = params;
+    a.p_stride = params_stride;
+    a.n = n;
+    a.t0 = t0;
+    a.t_end = t_end;
+    a.rtol = rtol;
+    a.atol = atol;
+    a.max_steps = max_steps;
+    a.flags = flags;
+    a.stop_thr = stop_threshold;
+    a.lap = laplacian;
+    a.fin = reinterpret_cast<float *>(out_final);
+    a.status = out_status;
+    a.nsteps = out_nsteps;
+    a.counter = reinterpret_cast<unsigned long long *>(workspace);
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    int64_t lanes = 0;
+    int32_t rc = device_lanes(R::tile_kernel, R::THREADS, &lanes);
+    if (rc) return rc;
+    int64_t tiles = lanes / R::THREADS;
+    if (max_tiles > 0 && max_tiles < tiles) tiles = max_tiles;
+    const int64_t need = (n + R::TILE - 1) / R::TILE;
+    if (tiles > need) tiles = need;
+    if (tiles < 1) tiles = 1;
+    hipLaunchKernelGGL(bh::set_counter_kernel, dim3(1), dim3(64), 0, s, a.counter,
+                       (unsigned long long)(tiles * R::TILE));
+    BH_HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(R::tile_kernel, dim3((unsigned)tiles), dim3(R::THREADS), 0, s, a);
+    BH_HIP_CHECK(hipGetLastError());
+    return 0;
 }
 
 }  // extern "C"

@@ -1,0 +1,187 @@
+"""Steady-state features: filter_time, the streaming statistical set, fused extraction.
+
+* ``Solution``            data contract of src/pybasin/solution.py:24-91
+                          (y is (N, B, S), N == len(t)).
+* ``FeatureExtractor``    base with ``filter_time``
+                          (src/pybasin/feature_extractors/feature_extractor.py:85-130).
+* ``StatFeatureExtractor`` the streaming subset of TorchFeatureExtractor
+                          (src/pybasin/ts_torch/torch_feature_extractor.py:192-285):
+                          per-state feature dicts, columns ``state_{i}__{name}``
+                          state-major then dict order, impute_extreme (+-inf/NaN
+                          -> +-1e10, torch_feature_utilities.py:60-94) and an
+                          optional z-score fitted on the first call.
+* ``HipFusedFeatureExtractor`` the same columns taken from the statistics the
+                          HIP kernel accumulated during integration; falls back
+                          to the trajectory when the solution carries none
+                          (e.g. a CPU template clone).
+"""
+
+from __future__ import annotations
+
+import threading
+from typing import Any
+
+import torch
+
+from .solvers.hip_dopri5 import FUSED_RESULTS, FusedResult, steady_index
+
+DEFAULT_STEADY_FRACTION = 0.85
+
+# name -> (needs) ; every feature here is computable from mean / std / min / max
+STREAMING_FEATURES = ("mean", "standard_deviation", "variance", "minimum", "maximum",
+                      "absolute_maximum", "delta", "log_delta")
+DEFAULT_FEATURES = {"mean": None, "standard_deviation": None}
+
+
+class Solution:
+    def __init__(self, initial_condition: torch.Tensor, time: torch.Tensor, y: torch.Tensor,
+                 features: torch.Tensor | None = None, labels=None, model_params=None):
+        assert initial_condition.ndim == 2, "initial_condition must be 2D (B, S)"
+        assert time.ndim == 1, "time must be 1D (N,)"
+        assert y.ndim == 3, "y must be 3D (N, B, S)"
+        assert y.shape[0] == time.shape[0], "time dimension mismatch"
+        assert y.shape[1] == initial_condition.shape[0], "initial conditions dimension mismatch"
+        assert y.shape[2] == initial_condition.shape[1], "state dimension mismatch"
+        self.initial_condition = initial_condition
+        self.time = time
+        self.y = y
+        self.features = features
+        self.extracted_features = None
+        self.extracted_feature_names = None
+        self.feature_names = None
+        self.labels = labels
+        self.model_params = model_params
+
+
+class FeatureExtractor:
+    def __init__(self, time_steady: float | None = None):
+        self.time_steady = time_steady
+
+    def _resolve_time_steady(self, solution: Solution) -> float:
+        if self.time_steady is not None:
+            return self.time_steady
+        t0, t1 = float(solution.time[0]), float(solution.time[-1])
+        return t0 + DEFAULT_STEADY_FRACTION * (t1 - t0)
+
+    def filter_time(self, solution: Solution) -> torch.Tensor:
+        ts = self._resolve_time_steady(solution)
+        idx = torch.where(solution.time >= ts)[0]
+        start = int(idx[0]) if len(idx) > 0 else 0
+        return solution.y[start:, ...]
+
+    def extract_features(self, solution: Solution) -> torch.Tensor:  # pragma: no cover
+        raise NotImplementedError
+
+
+def impute_extreme(features: torch.Tensor, extreme_value: float = 1e10) -> torch.Tensor:
+    out = features.clone()
+    ext = torch.tensor(extreme_value, dtype=features.dtype, device=features.device)
+    out = torch.where(torch.isposinf(features), ext, out)
+    out = torch.where(torch.isneginf(features), -ext, out)
+    out = torch.where(torch.isnan(features), ext, out)
+    return out
+
+
+def stats_from_window(window: torch.Tensor) -> torch.Tensor:
+    """(N', B, S) -> (B, S, 4) float64: mean, std (ddof 0), min, max."""
+    w = window.to(torch.float64)
+    return torch.stack([w.mean(0), w.std(0, correction=0), w.min(0).values, w.max(0).values], -1)
+
+
+def feature_from_stats(name: str, stats: torch.Tensor) -> torch.Tensor:
+    """One streaming feature (torch_features_statistical.py:39-110) from (B, S, 4) stats."""
+    mean, std, mn, mx = stats.unbind(-1)
+    if name == "mean":
+        return mean
+    if name == "standard_deviation":
+        return std
+    if name == "variance":
+        return std * std
+    if name == "minimum":
+        return mn
+    if name == "maximum":
+        return mx
+    if name == "absolute_maximum":
+        return torch.maximum(mx.abs(), mn.abs())
+    if name == "delta":
+        return (mx - mean).abs()
+    if name == "log_delta":
+        return torch.log((mx - mean).abs() + 1e-10)
+    raise KeyError(f"feature {name!r} is not a streaming feature; available: {STREAMING_FEATURES}")
+
+
+class StatFeatureExtractor(FeatureExtractor):
+    """Streaming statistical features, column-compatible with TorchFeatureExtractor."""
+
+    def __init__(self, time_steady: float | None = None,
+                 features: dict[str, Any] | None = DEFAULT_FEATURES,  # type: ignore[assignment]
+                 features_per_state: dict[int, dict[str, Any] | None] | None = None,
+                 normalize: bool = False):
+        super().__init__(time_steady)
+        self.features = features
+        self.features_per_state = features_per_state or {}
+        self.normalize = normalize
+        self._mean: torch.Tensor | None = None
+        self._std: torch.Tensor | None = None
+        self._lock = threading.Lock()
+        self._names: list[str] = []
+
+    def _plan(self, n_states: int) -> list[tuple[int, str]]:
+        plan = []
+        for s in range(n_states):
+            if s in self.features_per_state:
+                fdict = self.features_per_state[s]
+                if fdict is None:
+                    continue
+            else:
+                fdict = self.features
+                if fdict is None:
+                    continue
+            for name in fdict:
+                plan.append((s, name))
+        return plan
+
+    @property
+    def feature_names(self) -> list[str]:
+        return list(self._names)
+
+    def _stats(self, solution: Solution) -> torch.Tensor:
+        return stats_from_window(self.filter_time(solution))
+
+    def extract_features(self, solution: Solution) -> torch.Tensor:
+        stats = self._stats(solution)
+        plan = self._plan(stats.shape[1])
+        cols = [feature_from_stats(name, stats)[:, s] for s, name in plan]
+        self._names = [f"state_{s}__{name}" for s, name in plan]
+        feats = torch.stack(cols, dim=1) if cols else stats.new_zeros((stats.shape[0], 0))
+        feats = impute_extreme(feats.to(torch.float32))
+        if self.normalize:
+            with self._lock:
+                if self._mean is None:
+                    self._mean = feats.mean(0, keepdim=True)
+                    self._std = feats.std(0, keepdim=True).clamp_min(1e-12)
+            feats = (feats - self._mean) / self._std
+        return feats
+
+
+class HipFusedFeatureExtractor(StatFeatureExtractor):
+    """StatFeatureExtractor fed by the kernel's fused window statistics."""
+
+    def _stats(self, solution: Solution) -> torch.Tensor:
+        res: FusedResult | None = FUSED_RESULTS.get(solution.y)
+        if res is not None and res.stats is not None:
+            want = steady_index(solution.time, self._resolve_time_steady(solution))
+            if want == res.steady_idx:
+                return res.stats
+            if res.y is not None:
+                return stats_from_window(res.y[want:])
+        return super()._stats(solution)
+
+    def extract_from_result(self, res: FusedResult) -> torch.Tensor:
+        """Features straight from a fused result (no Solution / trajectory)."""
+        if res.stats is None:
+            raise ValueError("the fused result carries no window statistics")
+        plan = self._plan(res.stats.shape[1])
+        self._names = [f"state_{s}__{name}" for s, name in plan]
+        cols = [feature_from_stats(name, res.stats)[:, s] for s, name in plan]
+        return impute_extreme(torch.stack(cols, dim=1).to(torch.float32))
