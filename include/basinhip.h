@@ -117,6 +117,38 @@ int32_t bh_solve_rossler(int32_t dtype, const void *y0s, int64_t n, const double
                          int64_t *out_nsteps, void *workspace, int64_t workspace_bytes,
                          int32_t max_tiles, void *stream);
 
+/* ---- learned surrogate (no reference counterpart: the reference has no
+ * learned classifier; these kernels train and run an MLP on labels produced
+ * by bh_solve_batch).  bf16 arguments are bfloat16 bit patterns. ---------- */
+
+/* C (M x N, float32, row stride ldc) (+)= A . B on the matrix cores, with
+   A(m, k) = A[m*sam + k*sak] and B(k, n) = B[k*sbk + n*sbn] in bf16 */
+int32_t bh_gemm_bf16(const void *A, int64_t sam, int64_t sak, const void *B, int64_t sbk,
+                     int64_t sbn, float *C, int64_t ldc, int64_t M, int64_t N, int64_t K,
+                     int32_t accumulate, void *stream);
+/* Z + bias per column; optional copy of that pre-activation (zsave), SiLU
+   (silu != 0) or identity written as bf16 (out, row stride ldo) and/or f32 */
+int32_t bh_mlp_bias_act(const float *Z, int64_t M, int64_t N, const float *bias, int32_t silu,
+                        float *zsave, void *out_bf16, int64_t ldo, float *out_f32, void *stream);
+/* dZ = dA * SiLU'(Z), written as float32 and bf16 */
+int32_t bh_mlp_silu_bwd(const float *dA, const float *Z, int64_t count, float *dZ,
+                        void *dZ_bf16, void *stream);
+/* two-class softmax cross-entropy: gradient of the mean loss w.r.t. the
+   logits (M x 2, float32 and bf16) and the per-row loss */
+int32_t bh_mlp_softmax_ce(const float *logits, const int32_t *labels, int64_t M, float *dlogits,
+                          void *dlogits_bf16, float *loss, void *stream);
+/* column sums of G (M x N float32) */
+int32_t bh_mlp_colsum(const float *G, int64_t M, int64_t N, float *out, void *stream);
+/* Adam step `step` (1-based) on float32 weights; refreshes the bf16 copy if given */
+int32_t bh_mlp_adam(float *W, const float *G, float *m, float *v, int64_t count, float lr,
+                    float beta1, float beta2, float eps, int64_t step, void *W_bf16, void *stream);
+/* (M x 3) float32 inputs -> (M x 16) bf16 rows [(x0-c0)s0, (x1-c1)s1, (x2-c2)s2, 0, ...] */
+int32_t bh_mlp_prep_input(const float *x, int64_t M, float c0, float s0, float c1, float s1,
+                          float c2, float s2, void *out_bf16, void *stream);
+/* logits (M x 2) + bias -> argmax label and max softmax probability */
+int32_t bh_mlp_classify(const float *Z, const float *bias, int64_t M, int32_t *label,
+                        float *conf, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

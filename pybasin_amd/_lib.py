@@ -22,7 +22,9 @@ BH_NFEAT = 4
 BH_WORKSPACE_BYTES = 256
 
 # every symbol include/basinhip.h declares
-EXPORTS = ("bh_version", "bh_last_error", "bh_solve_batch", "bh_solve_rossler")
+EXPORTS = ("bh_version", "bh_last_error", "bh_solve_batch", "bh_solve_rossler", "bh_gemm_bf16",
+           "bh_mlp_bias_act", "bh_mlp_silu_bwd", "bh_mlp_softmax_ce", "bh_mlp_colsum",
+           "bh_mlp_adam", "bh_mlp_prep_input", "bh_mlp_classify")
 
 _lock = threading.Lock()
 _lib: ctypes.CDLL | None = None
@@ -41,17 +43,25 @@ def _declare(L: ctypes.CDLL) -> None:
     L.bh_solve_batch.restype = i32
     L.bh_solve_batch.argtypes = [i32, i32, vp, i64, i32, vp, i32, i64, f64, vp, i32, f64, f64,
                                  i64, u32, f64, i32, vp, vp, vp, vp, vp, vp, i64, i32, vp]
-    if hasattr(L, "bh_solve_rossler"):
+    if True:
         L.bh_solve_rossler.restype = i32
         L.bh_solve_rossler.argtypes = [i32, vp, i64, vp, i64, vp, f64, f64, f64, f64, i64, u32,
                                        f64, vp, vp, vp, vp, i64, i32, vp]
-    if hasattr(L, "bh_mlp_forward"):
-        L.bh_mlp_forward.restype = i32
-        L.bh_mlp_forward.argtypes = [vp, i64, vp, vp, vp, vp, vp]
-    if hasattr(L, "bh_mlp_train_step"):
-        L.bh_mlp_train_step.restype = i32
-        L.bh_mlp_train_step.argtypes = [vp, vp, i64, vp, vp, vp, vp, i64, f64, f64, f64, f64,
-                                        vp, vp]
+    f32 = ctypes.c_float
+    sigs = {
+        "bh_gemm_bf16": [vp, i64, i64, vp, i64, i64, vp, i64, i64, i64, i64, i32, vp],
+        "bh_mlp_bias_act": [vp, i64, i64, vp, i32, vp, vp, i64, vp, vp],
+        "bh_mlp_silu_bwd": [vp, vp, i64, vp, vp, vp],
+        "bh_mlp_softmax_ce": [vp, vp, i64, vp, vp, vp, vp],
+        "bh_mlp_colsum": [vp, i64, i64, vp, vp],
+        "bh_mlp_adam": [vp, vp, vp, vp, i64, f32, f32, f32, f32, i64, vp, vp],
+        "bh_mlp_prep_input": [vp, i64, f32, f32, f32, f32, f32, f32, vp, vp],
+        "bh_mlp_classify": [vp, vp, i64, vp, vp, vp],
+    }
+    for name, args in sigs.items():
+        fn = getattr(L, name)
+        fn.restype = i32
+        fn.argtypes = args
 
 
 def lib() -> ctypes.CDLL:

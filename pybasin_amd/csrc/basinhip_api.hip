@@ -7,6 +7,7 @@
 #include "../../include/basinhip.h"
 #include "dopri5_lane.hip"
 #include "rossler_tile.hip"
+#include "mlp.hip"
 
 namespace {
 
@@ -209,6 +210,109 @@ int32_t bh_solve_rossler(int32_t dtype, const void *y0s, int64_t n, const double
                        (unsigned long long)(tiles * R::TILE));
     BH_HIP_CHECK(hipGetLastError());
     hipLaunchKernelGGL(R::tile_kernel, dim3((unsigned)tiles), dim3(R::THREADS), 0, s, a);
+    BH_HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
+// ---- surrogate MLP --------------------------------------------------------
+
+#define BH_GRID(count, block) dim3((unsigned)(((count) + (block) - 1) / (block)))
+
+int32_t bh_gemm_bf16(const void *A, int64_t sam, int64_t sak, const void *B, int64_t sbk,
+                     int64_t sbn, float *C, int64_t ldc, int64_t M, int64_t N, int64_t K,
+                     int32_t accumulate, void *stream) {
+    g_err[0] = 0;
+    if (M < 0 || N < 0 || K < 0) return fail(-5, "negative GEMM size");
+    if (M == 0 || N == 0) return 0;
+    if (!A || !B || !C) return fail(-5, "null GEMM operand");
+    if ((M + 63) / 64 > 65535) return fail(-5, "M too large for one launch (%lld)", (long long)M);
+    bh::mlp::GemmArgs g{reinterpret_cast<const unsigned short *>(A),
+                        reinterpret_cast<const unsigned short *>(B), C, M, N, K, sam, sak, sbk, sbn,
+                        ldc, accumulate};
+    hipLaunchKernelGGL(bh::mlp::gemm_bf16_kernel, dim3((unsigned)((N + 63) / 64), (unsigned)((M + 63) / 64)),
+                       dim3(256), 0, reinterpret_cast<hipStream_t>(stream), g);
+    BH_HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
+int32_t bh_mlp_bias_act(const float *Z, int64_t M, int64_t N, const float *bias, int32_t silu,
+                        float *zsave, void *out_bf16, int64_t ldo, float *out_f32, void *stream) {
+    g_err[0] = 0;
+    if (M <= 0 || N <= 0) return M == 0 ? 0 : fail(-5, "bad sizes");
+    if (!Z || !bias) return fail(-5, "null argument");
+    hipLaunchKernelGGL(bh::mlp::bias_act_kernel, BH_GRID(M * N, 256), dim3(256), 0,
+                       reinterpret_cast<hipStream_t>(stream), Z, M, N, bias, silu, zsave,
+                       reinterpret_cast<unsigned short *>(out_bf16), ldo, out_f32);
+    BH_HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
+int32_t bh_mlp_silu_bwd(const float *dA, const float *Z, int64_t count, float *dZ,
+                        void *dZ_bf16, void *stream) {
+    g_err[0] = 0;
+    if (count <= 0) return count == 0 ? 0 : fail(-5, "bad count");
+    if (!dA || !Z || !dZ || !dZ_bf16) return fail(-5, "null argument");
+    hipLaunchKernelGGL(bh::mlp::silu_bwd_kernel, BH_GRID(count, 256), dim3(256), 0,
+                       reinterpret_cast<hipStream_t>(stream), dA, Z, count, dZ,
+                       reinterpret_cast<unsigned short *>(dZ_bf16));
+    BH_HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
+int32_t bh_mlp_softmax_ce(const float *logits, const int32_t *labels, int64_t M, float *dlogits,
+                          void *dlogits_bf16, float *loss, void *stream) {
+    g_err[0] = 0;
+    if (M <= 0) return M == 0 ? 0 : fail(-5, "bad M");
+    if (!logits || !labels || !dlogits || !dlogits_bf16 || !loss) return fail(-5, "null argument");
+    hipLaunchKernelGGL(bh::mlp::softmax_ce_kernel, BH_GRID(M, 256), dim3(256), 0,
+                       reinterpret_cast<hipStream_t>(stream), logits, labels, M, dlogits,
+                       reinterpret_cast<unsigned short *>(dlogits_bf16), loss);
+    BH_HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
+int32_t bh_mlp_colsum(const float *G, int64_t M, int64_t N, float *out, void *stream) {
+    g_err[0] = 0;
+    if (M < 0 || N <= 0) return N == 0 ? 0 : fail(-5, "bad sizes");
+    if (!G || !out) return fail(-5, "null argument");
+    hipLaunchKernelGGL(bh::mlp::colsum_kernel, dim3((unsigned)N), dim3(256), 0,
+                       reinterpret_cast<hipStream_t>(stream), G, M, N, out);
+    BH_HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
+int32_t bh_mlp_adam(float *W, const float *G, float *m, float *v, int64_t count, float lr,
+                    float beta1, float beta2, float eps, int64_t step, void *W_bf16, void *stream) {
+    g_err[0] = 0;
+    if (count <= 0) return count == 0 ? 0 : fail(-5, "bad count");
+    if (!W || !G || !m || !v || step < 1) return fail(-5, "bad Adam arguments");
+    const float bc1 = 1.0f - powf(beta1, (float)step), bc2 = 1.0f - powf(beta2, (float)step);
+    hipLaunchKernelGGL(bh::mlp::adam_kernel, BH_GRID(count, 256), dim3(256), 0,
+                       reinterpret_cast<hipStream_t>(stream), W, G, m, v, count, lr, beta1, beta2,
+                       eps, bc1, bc2, reinterpret_cast<unsigned short *>(W_bf16));
+    BH_HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
+int32_t bh_mlp_prep_input(const float *x, int64_t M, float c0, float s0, float c1, float s1,
+                          float c2, float s2, void *out_bf16, void *stream) {
+    g_err[0] = 0;
+    if (M <= 0) return M == 0 ? 0 : fail(-5, "bad M");
+    if (!x || !out_bf16) return fail(-5, "null argument");
+    hipLaunchKernelGGL(bh::mlp::prep_input_kernel, BH_GRID(M * 16, 256), dim3(256), 0,
+                       reinterpret_cast<hipStream_t>(stream), x, M, c0, s0, c1, s1, c2, s2,
+                       reinterpret_cast<unsigned short *>(out_bf16));
+    BH_HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
+int32_t bh_mlp_classify(const float *Z, const float *bias, int64_t M, int32_t *label, float *conf,
+                        void *stream) {
+    g_err[0] = 0;
+    if (M <= 0) return M == 0 ? 0 : fail(-5, "bad M");
+    if (!Z || !bias || !label || !conf) return fail(-5, "null argument");
+    hipLaunchKernelGGL(bh::mlp::classify_kernel, BH_GRID(M, 256), dim3(256), 0,
+                       reinterpret_cast<hipStream_t>(stream), Z, bias, M, label, conf);
     BH_HIP_CHECK(hipGetLastError());
     return 0;
 }

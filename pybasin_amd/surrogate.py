@@ -1,0 +1,248 @@
+"""Learned surrogate: (theta, omega, T) -> attractor label, with Dopri5 fallback.
+
+The network (3 -> 256 -> 256 -> 256 -> 2, SiLU) is trained and evaluated by
+the HIP kernels of libbasinhip.so (bf16 weights and activations, float32
+accumulation on the matrix cores, Adam on float32 master weights).  ICs whose
+softmax confidence is below a gate are integrated by the per-lane Dopri5
+kernel; a random audit sample is integrated regardless and reports the
+surrogate's label error and basin-stability error against full integration.
+
+The surrogate has no reference counterpart; its labels come from the
+integrator path that replaces TorchDiffEqSolver (the pendulum of
+case_studies/pendulum/pendulum_ode.py:18-40 with the 1-NN template
+classifier of setup_pendulum_system.py:44-65).
+"""
+
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass
+
+import torch
+
+from . import _lib
+
+HIDDEN = 256
+IN_PAD = 16
+
+
+def _s(stream):
+    return stream.cuda_stream
+
+
+class SurrogateMLP:
+    """Parameters and kernels of the 4-layer MLP on one device."""
+
+    def __init__(self, device="cuda:0", hidden: int = HIDDEN, seed: int = 0,
+                 norm=(0.0, 1.0 / math.pi, 0.0, 0.1, 0.55, 1.0 / 0.45)):
+        self.device = torch.device(device)
+        self.hidden = hidden
+        self.norm = tuple(float(v) for v in norm)
+        g = torch.Generator().manual_seed(seed)
+        dims = [(IN_PAD, hidden, 3), (hidden, hidden, hidden), (hidden, hidden, hidden), (hidden, 2, hidden)]
+        self.W, self.b = [], []
+        for fin, fout, fan_in in dims:
+            bound = 1.0 / math.sqrt(fan_in)
+            w = (torch.rand(fin, fout, generator=g) * 2 - 1) * bound
+            if fin == IN_PAD:
+                w[3:] = 0.0  # padded input columns
+            b = (torch.rand(fout, generator=g) * 2 - 1) * bound
+            self.W.append(w.to(self.device).contiguous())
+            self.b.append(b.to(self.device).contiguous())
+        self.Wb = [w.to(torch.bfloat16).contiguous() for w in self.W]
+        self.mW = [torch.zeros_like(w) for w in self.W]
+        self.vW = [torch.zeros_like(w) for w in self.W]
+        self.mb = [torch.zeros_like(b) for b in self.b]
+        self.vb = [torch.zeros_like(b) for b in self.b]
+        self.step = 0
+
+    # ---- kernels ---------------------------------------------------------
+    def _gemm(self, A, sam, sak, B, sbk, sbn, C, M, N, K, stream, accumulate=0):
+        _lib.check(_lib.lib().bh_gemm_bf16(A.data_ptr(), sam, sak, B.data_ptr(), sbk, sbn,
+                                           C.data_ptr(), C.stride(0), M, N, K, accumulate,
+                                           _s(stream)), "bh_gemm_bf16")
+
+    def prep(self, x: torch.Tensor, stream) -> torch.Tensor:
+        M = x.shape[0]
+        out = torch.empty((M, IN_PAD), dtype=torch.bfloat16, device=self.device)
+        c0, s0, c1, s1, c2, s2 = self.norm
+        _lib.check(_lib.lib().bh_mlp_prep_input(x.data_ptr(), M, c0, s0, c1, s1, c2, s2,
+                                                out.data_ptr(), _s(stream)), "bh_mlp_prep_input")
+        return out
+
+    def forward(self, x: torch.Tensor, stream=None, keep: bool = False):
+        """x (M, 3) float32 on device -> logits (M, 2) float32 (bias added).
+        keep=True also returns the saved tensors for backward."""
+        stream = stream or torch.cuda.current_stream(self.device)
+        x = x.to(self.device, torch.float32).contiguous()
+        M = x.shape[0]
+        a = self.prep(x, stream)
+        acts, zs = [a], []
+        for layer in range(3):
+            W = self.Wb[layer]
+            K, N = W.shape
+            Z = torch.empty((M, N), dtype=torch.float32, device=self.device)
+            self._gemm(acts[-1], acts[-1].stride(0), 1, W, N, 1, Z, M, N, K, stream)
+            zsave = torch.empty_like(Z) if keep else None
+            A = torch.empty((M, N), dtype=torch.bfloat16, device=self.device)
+            _lib.check(_lib.lib().bh_mlp_bias_act(Z.data_ptr(), M, N, self.b[layer].data_ptr(), 1,
+                                                  _lib.ptr(zsave), A.data_ptr(), N, None,
+                                                  _s(stream)), "bh_mlp_bias_act")
+            zs.append(zsave)
+            acts.append(A)
+        Z4 = torch.empty((M, 2), dtype=torch.float32, device=self.device)
+        self._gemm(acts[-1], self.hidden, 1, self.Wb[3], 2, 1, Z4, M, 2, self.hidden, stream)
+        logits = torch.empty_like(Z4)
+        _lib.check(_lib.lib().bh_mlp_bias_act(Z4.data_ptr(), M, 2, self.b[3].data_ptr(), 0, None,
+                                              None, 0, logits.data_ptr(), _s(stream)), "bh_mlp_bias_act")
+        if keep:
+            return logits, (acts, zs)
+        return logits
+
+    def gradients(self, x: torch.Tensor, labels: torch.Tensor, stream=None):
+        """Mean cross-entropy loss and its gradients (dW list, db list)."""
+        stream = stream or torch.cuda.current_stream(self.device)
+        labels = labels.to(self.device, torch.int32).contiguous()
+        logits, (acts, zs) = self.forward(x, stream, keep=True)
+        M = logits.shape[0]
+        L = _lib.lib()
+        dlog = torch.empty_like(logits)
+        dlogb = torch.empty((M, 2), dtype=torch.bfloat16, device=self.device)
+        loss = torch.empty((M,), dtype=torch.float32, device=self.device)
+        _lib.check(L.bh_mlp_softmax_ce(logits.data_ptr(), labels.data_ptr(), M, dlog.data_ptr(),
+                                       dlogb.data_ptr(), loss.data_ptr(), _s(stream)), "bh_mlp_softmax_ce")
+        dW = [None] * 4
+        db = [None] * 4
+        gz, gzb = dlog, dlogb  # gradient w.r.t. this layer's pre-activation
+        for layer in range(3, -1, -1):
+            Ain = acts[layer]
+            K_in, N_out = self.Wb[layer].shape
+            # dW = Ain^T . gz  (K_in x N_out, reduction over M)
+            gw = torch.empty((K_in, N_out), dtype=torch.float32, device=self.device)
+            self._gemm(Ain, 1, Ain.stride(0), gzb, N_out, 1, gw, K_in, N_out, M, stream)
+            gb = torch.empty((N_out,), dtype=torch.float32, device=self.device)
+            _lib.check(L.bh_mlp_colsum(gz.data_ptr(), M, N_out, gb.data_ptr(), _s(stream)), "bh_mlp_colsum")
+            dW[layer], db[layer] = gw, gb
+            if layer == 0:
+                break
+            # dA_in = gz . W^T  (M x K_in)
+            dA = torch.empty((M, K_in), dtype=torch.float32, device=self.device)
+            self._gemm(gzb, N_out, 1, self.Wb[layer], 1, N_out, dA, M, K_in, N_out, stream)
+            nz = torch.empty_like(dA)
+            nzb = torch.empty((M, K_in), dtype=torch.bfloat16, device=self.device)
+            _lib.check(L.bh_mlp_silu_bwd(dA.data_ptr(), zs[layer - 1].data_ptr(), M * K_in,
+                                         nz.data_ptr(), nzb.data_ptr(), _s(stream)), "bh_mlp_silu_bwd")
+            gz, gzb = nz, nzb
+        return loss, dW, db
+
+    def adam_step(self, dW, db, lr=1e-3, b1=0.9, b2=0.999, eps=1e-8, stream=None):
+        stream = stream or torch.cuda.current_stream(self.device)
+        self.step += 1
+        L = _lib.lib()
+        for i in range(4):
+            _lib.check(L.bh_mlp_adam(self.W[i].data_ptr(), dW[i].data_ptr(), self.mW[i].data_ptr(),
+                                     self.vW[i].data_ptr(), self.W[i].numel(), lr, b1, b2, eps,
+                                     self.step, self.Wb[i].data_ptr(), _s(stream)), "bh_mlp_adam")
+            _lib.check(L.bh_mlp_adam(self.b[i].data_ptr(), db[i].data_ptr(), self.mb[i].data_ptr(),
+                                     self.vb[i].data_ptr(), self.b[i].numel(), lr, b1, b2, eps,
+                                     self.step, None, _s(stream)), "bh_mlp_adam")
+
+    def fit(self, x: torch.Tensor, labels: torch.Tensor, epochs: int = 40, batch: int = 1024,
+            lr: float = 2e-3, seed: int = 0) -> list[float]:
+        """Mini-batch Adam on (x, labels); returns the mean loss per epoch."""
+        x = x.to(self.device, torch.float32).contiguous()
+        labels = labels.to(self.device, torch.int32).contiguous()
+        n = x.shape[0]
+        g = torch.Generator(device=self.device).manual_seed(seed)
+        hist = []
+        for ep in range(epochs):
+            perm = torch.randperm(n, device=self.device, generator=g)
+            tot = torch.zeros((), dtype=torch.float64, device=self.device)
+            lr_ep = lr * 0.5 * (1 + math.cos(math.pi * ep / epochs))
+            for i in range(0, n - batch + 1, batch):
+                idx = perm[i:i + batch]
+                loss, dW, db = self.gradients(x[idx].contiguous(), labels[idx].contiguous())
+                self.adam_step(dW, db, lr=lr_ep)
+                tot += loss.double().sum()
+            hist.append(float(tot) / (n - n % batch))
+        return hist
+
+    def predict(self, x: torch.Tensor, chunk: int = 1 << 20, stream=None):
+        """labels (int32) and confidences (max softmax probability) for x (M, 3)."""
+        stream = stream or torch.cuda.current_stream(self.device)
+        x = x.to(self.device, torch.float32).contiguous()
+        M = x.shape[0]
+        lab = torch.empty((M,), dtype=torch.int32, device=self.device)
+        conf = torch.empty((M,), dtype=torch.float32, device=self.device)
+        zero_b = torch.zeros(2, dtype=torch.float32, device=self.device)
+        for s in range(0, M, chunk):
+            e = min(M, s + chunk)
+            logits = self.forward(x[s:e], stream)
+            _lib.check(_lib.lib().bh_mlp_classify(logits.data_ptr(), zero_b.data_ptr(), e - s,
+                                                  lab[s:].data_ptr(), conf[s:].data_ptr(),
+                                                  _s(stream)), "bh_mlp_classify")
+        return lab, conf
+
+
+@dataclass
+class GatedResult:
+    labels: torch.Tensor          # (n,) int32 final labels
+    routed: torch.Tensor          # (n,) bool: integrated by Dopri5
+    audit_idx: torch.Tensor       # indices of the audit sample
+    audit_surrogate: torch.Tensor  # surrogate labels on the audit sample
+    audit_integrated: torch.Tensor  # integrator labels on the audit sample
+
+
+def pendulum_labels_from_stats(stats: torch.Tensor) -> torch.Tensor:
+    """FP (0) / LC (1) from the window statistics of omega: FP when
+    |max - mean| < 0.01 (case_studies/pendulum/pendulum_feature_extractor.py:20-31)."""
+    delta = (stats[:, 1, 3] - stats[:, 1, 0]).abs()
+    lab = (delta >= 0.01).to(torch.int32)
+    return torch.where(torch.isfinite(delta), lab, torch.ones_like(lab))
+
+
+def integrate_labels(solver, ode, x: torch.Tensor, time_steady: float, stream=None) -> torch.Tensor:
+    """Plain path for rows (theta, omega, T): per-lane Dopri5 with the row's
+    torque as parameter, fused window statistics, threshold labels."""
+    dev = solver.device
+    x = x.to(dev, torch.float32)
+    n = x.shape[0]
+    if n == 0:
+        return torch.empty((0,), dtype=torch.int32, device=dev)
+    alpha, K = float(ode.params["alpha"]), float(ode.params["K"])
+    params = torch.stack([torch.full((n,), alpha, dtype=torch.float64, device=dev),
+                          x[:, 2].double(), torch.full((n,), K, dtype=torch.float64, device=dev)], 1)
+    y0 = x[:, :2].contiguous()
+    save_ts, y0 = solver._prepare_tensors(y0)
+    res = solver.run_fused(ode, y0, save_ts, params.contiguous(), keep_trajectories=False,
+                           time_steady=time_steady, stream=stream)
+    return pendulum_labels_from_stats(res.stats)
+
+
+def classify_gated(model: SurrogateMLP, solver, ode, x: torch.Tensor, gate: float = 0.95,
+                   audit_frac: float = 0.02, group: torch.Tensor | None = None,
+                   time_steady: float = 950.0, seed: int = 0) -> GatedResult:
+    """Surrogate labels for confident rows, Dopri5 for the rest, plus a random
+    audit sample (audit_frac of every group) integrated in the same launch."""
+    dev = solver.device
+    x = x.to(dev, torch.float32).contiguous()
+    n = x.shape[0]
+    lab, conf = model.predict(x)
+    routed = conf < gate
+    g = torch.Generator(device=dev).manual_seed(seed)
+    u = torch.rand((n,), generator=g, device=dev)
+    audit = u < audit_frac
+    if group is not None:  # at least one audit row per group
+        _, inv = torch.unique(group, return_inverse=True)
+        umin = torch.full((int(inv.max()) + 1,), 2.0, device=dev).scatter_reduce(0, inv, u, "amin")
+        audit |= u == umin[inv]
+    need = routed | audit
+    idx = torch.nonzero(need).squeeze(1)
+    ilab = integrate_labels(solver, ode, x[idx], time_steady)
+    final = lab.clone()
+    full = torch.full((n,), -1, dtype=torch.int32, device=dev)
+    full[idx] = ilab
+    final[routed] = full[routed]
+    aidx = torch.nonzero(audit).squeeze(1)
+    return GatedResult(labels=final, routed=routed, audit_idx=aidx, audit_surrogate=lab[aidx],
+                       audit_integrated=full[aidx])

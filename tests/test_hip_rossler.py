@@ -46,11 +46,17 @@ def test_final_state_matches_oracle_short_horizon(cuda_device, K):
     res = _run(y0.to(cuda_device), p, 0.5, 1e-6, 1e-6)
     ref, st, ns = O.solve(O.SYS_ROSSLER, y0.double().numpy(), np.array([0.2, 0.2, 7.0, K]),
                           np.array([0.5]), 0.0, rtol=1e-6, atol=1e-6, bits=32, threads=8)
-    assert (st == 0).all() and (res.status.cpu().numpy() == 0).all()
-    got = res.final.double().cpu().numpy()
-    err = np.abs(got - ref[-1]) / (1.0 + np.abs(ref[-1]))
-    print(f"[result] K={K} max scaled err {err.max():.3e}; steps hip {res.nsteps.float().mean():.1f} "
-          f"oracle {ns.mean():.1f}", flush=True)
+    hs = res.status.cpu().numpy()
+    # uncoupled nodes can blow up in finite time: both integrators must stop
+    # on exactly the same trajectories after the same number of steps
+    np.testing.assert_array_equal(hs != 0, st != 0)
+    np.testing.assert_array_equal(res.nsteps.cpu().numpy()[st != 0], ns[st != 0])
+    ok = st == 0
+    assert ok.sum() >= 32
+    got = res.final.double().cpu().numpy()[ok]
+    err = np.abs(got - ref[-1][ok]) / (1.0 + np.abs(ref[-1][ok]))
+    print(f"[result] K={K} max scaled err {err.max():.3e}; steps hip "
+          f"{res.nsteps.float().mean():.1f} oracle {ns.mean():.1f}", flush=True)
     assert err.max() < 5e-4
 
 
