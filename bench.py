@@ -35,6 +35,11 @@ PEAK_FP64_VALU = 78.6e12
 F_RHS = {"duffing": 8, "pendulum": 5, "lorenz": 9}
 STATE = {"duffing": 2, "pendulum": 2, "lorenz": 3}
 
+# HBM bytes per dispatch of the Duffing 100k float32 kernel, from rocprofv3
+# FETCH_SIZE + WRITE_SIZE scaled by a plain-copy calibration
+# (profiles/dopri5_lane_duffing_100k_f32_pmc.md)
+TRAFFIC_BYTES = {("duffing", "f32", 100_000): 9.79e6}
+
 DUFFING_TEMPLATES = [[-0.21, 0.02], [1.05, 0.77], [-0.67, 0.02], [-0.46, 0.30], [-0.43, 0.12]]
 
 
@@ -49,7 +54,20 @@ def parse():
     ap.add_argument("--max-lanes", type=int, default=0)
     ap.add_argument("--cpu-sample", type=int, default=20_000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 256 MiB device copy (calibrates the "
+                         "FETCH_SIZE / WRITE_SIZE counters) and exit")
     return ap.parse_args()
+
+
+def copy_calibration():
+    n = 64 * 1024 * 1024  # 64 Mi float32 = 256 MiB
+    src = torch.ones(n, dtype=torch.float32, device="cuda:0")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes_each_way": n * 4, "copies": 3}), flush=True)
 
 
 def cpu_baseline(y0_cpu, save64, n_sample):
@@ -72,6 +90,9 @@ def cpu_baseline(y0_cpu, save64, n_sample):
 
 def main():
     args = parse()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
               "(ICs shard trivially across GPUs; see DESIGN.md)", file=sys.stderr)
@@ -151,13 +172,15 @@ def main():
             "classifier": "1-NN on (max x, std x), 5 templates",
             "basin_stability": [round(v, 5) for v in bs],
             "mean_steps_per_ic": round(float(nsteps.mean()), 1),
+            "max_steps_per_ic": int(nsteps.max()),
             "lanes_failed": int((status != 0).sum()),
             "bs_error_vs_plain": 0.0,
         },
         "roofline": {
             "bound": "valu", "achieved": round(achieved / 1e12, 3),
             "peak": round(peak / 1e12, 1), "unit": "TFLOP/s",
-            "frac": round(achieved / peak, 4), "traffic": None,
+            "frac": round(achieved / peak, 4),
+            "traffic": TRAFFIC_BYTES.get(("duffing", args.dtype, n)),
             "kernel": "bh::dopri5_lane_kernel<Duffing,float>",
             "kernel_ms": round(kernel_ms, 3),
         },

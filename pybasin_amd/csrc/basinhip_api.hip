@@ -47,6 +47,9 @@ int32_t launch_lane(const bh::LaneArgs &args, int32_t max_lanes, hipStream_t str
     int64_t lanes = 0;
     int32_t rc = device_lanes(kernel, block, &lanes);
     if (rc) return rc;
+    // Resident lanes: every trajectory gets its own lane up to the occupancy
+    // limit (more resident waves hide the latency of the dependent step
+    // arithmetic); beyond that, finished lanes take the next trajectory.
     if (max_lanes > 0 && max_lanes < lanes) lanes = max_lanes;
     if (lanes > args.n) lanes = args.n;
     const int64_t blocks = (lanes + block - 1) / block;

@@ -85,13 +85,24 @@ __device__ constexpr double M5 = 187940372067.0 / 1594534317056.0 / 2.0;
 __device__ constexpr double M6 = -1776094331.0 / 19743644256.0 / 2.0;
 __device__ constexpr double M7 = 11237099.0 / 235043384.0 / 2.0;
 
+// previous representable value (towards -inf) of a finite x
 template <class T> __device__ __forceinline__ T prev_value(T x);
 template <> __device__ __forceinline__ float prev_value<float>(float x) {
-    return nextafterf(x, -INFINITY);
+    const int i = __float_as_int(x);
+    if (x == 0.0f) return -__int_as_float(1);
+    return __int_as_float(x > 0.0f ? i - 1 : i + 1);
 }
 template <> __device__ __forceinline__ double prev_value<double>(double x) {
-    return nextafter(x, -INFINITY);
+    const long long i = __double_as_longlong(x);
+    if (x == 0.0) return -__longlong_as_double(1);
+    return __longlong_as_double(x > 0.0 ? i - 1 : i + 1);
 }
+
+// reciprocal: hardware v_rcp_f32 (1 ulp) in single precision; the ratio only
+// steers the step size, so a 1-ulp quotient is as good as a rounded one
+template <class T> __device__ __forceinline__ T fast_rcp(T x);
+template <> __device__ __forceinline__ float fast_rcp<float>(float x) { return __builtin_amdgcn_rcpf(x); }
+template <> __device__ __forceinline__ double fast_rcp<double>(double x) { return 1.0 / x; }
 
 template <class T> __device__ __forceinline__ T tpow(T x, T e);
 template <> __device__ __forceinline__ float tpow<float>(float x, float e) { return powf(x, e); }
@@ -103,7 +114,8 @@ template <> __device__ __forceinline__ double tpow<double>(double x, double e) {
 __device__ __forceinline__ double next_dt(double dt, double ratio) {
     if (ratio == 0.0) return dt * 10.0;
     const float r = (float)ratio;
-    float f = 0.9f * exp2f(-0.2f * log2f(r));
+    // hardware log2 / exp2; ratios below the normal range give a factor >= 10
+    float f = 0.9f * __builtin_amdgcn_exp2f(-0.2f * __builtin_amdgcn_logf(r));
     const float dfac = ratio < 1.0 ? 1.0f : 0.2f;
     f = fminf(10.0f, fmaxf(f, dfac));
     return dt * (double)f;
@@ -116,6 +128,7 @@ struct Lane {
     T y[S], k1[S];
     double t, dt;
     int64_t steps;
+    double next_save;  // save_t[si], +inf once every save is emitted
     int32_t si;
     int32_t status;
     bool done;
@@ -211,9 +224,11 @@ __device__ __forceinline__ void lane_init(const LaneArgs &a, Lane<Sys, T> &L, in
         h1 = tpow<T>((T)0.01 / fmax(d1, d2), (T)0.2);
     L.dt = (double)fmin((T)100 * h0, fabs(h1));
     // save times at or before t0 return y0 exactly (torchdiffeq solution[0] = y0)
-    while (L.si < a.n_save && a.save_t[L.si] <= a.t0) {
+    L.next_save = a.save_t[0];
+    while (L.next_save <= a.t0) {
         emit<Sys, T>(a, L, ic, L.si, L.y);
         L.si++;
+        L.next_save = L.si < a.n_save ? a.save_t[L.si] : INFINITY;
     }
     if (L.si >= a.n_save) L.done = true;
 }
@@ -244,30 +259,33 @@ __device__ __forceinline__ void lane_step(const LaneArgs &a, Lane<Sys, T> &L, in
     }
     const T tf = (T)tA, h = (T)L.dt, t1p = prev_value<T>((T)tB);
     T k2[S], k3[S], k4[S], k5[S], k6[S], k7[S], ys[S];
+    const T tst[5] = {tf + (T)C2 * h, tf + (T)C3 * h, tf + (T)C4 * h, tf + (T)C5 * h, t1p};
+    T aux[5];
+    Sys::time_terms(tf, tst, aux);
     {
         const T c = (T)A21 * h;
 #pragma unroll
         for (int i = 0; i < S; i++) ys[i] = L.y[i] + L.k1[i] * c;
-        Sys::f(L.p, tf + (T)C2 * h, ys, k2);
+        rhs_aux<Sys, T>(L.p, aux[0], ys, k2);
     }
     {
         const T c1 = (T)A31 * h, c2 = (T)A32 * h;
 #pragma unroll
         for (int i = 0; i < S; i++) ys[i] = L.y[i] + (L.k1[i] * c1 + k2[i] * c2);
-        Sys::f(L.p, tf + (T)C3 * h, ys, k3);
+        rhs_aux<Sys, T>(L.p, aux[1], ys, k3);
     }
     {
         const T c1 = (T)A41 * h, c2 = (T)A42 * h, c3 = (T)A43 * h;
 #pragma unroll
         for (int i = 0; i < S; i++) ys[i] = L.y[i] + (L.k1[i] * c1 + k2[i] * c2 + k3[i] * c3);
-        Sys::f(L.p, tf + (T)C4 * h, ys, k4);
+        rhs_aux<Sys, T>(L.p, aux[2], ys, k4);
     }
     {
         const T c1 = (T)A51 * h, c2 = (T)A52 * h, c3 = (T)A53 * h, c4 = (T)A54 * h;
 #pragma unroll
         for (int i = 0; i < S; i++)
             ys[i] = L.y[i] + (L.k1[i] * c1 + k2[i] * c2 + k3[i] * c3 + k4[i] * c4);
-        Sys::f(L.p, tf + (T)C5 * h, ys, k5);
+        rhs_aux<Sys, T>(L.p, aux[3], ys, k5);
     }
     {
         const T c1 = (T)A61 * h, c2 = (T)A62 * h, c3 = (T)A63 * h, c4 = (T)A64 * h,
@@ -275,7 +293,7 @@ __device__ __forceinline__ void lane_step(const LaneArgs &a, Lane<Sys, T> &L, in
 #pragma unroll
         for (int i = 0; i < S; i++)
             ys[i] = L.y[i] + (L.k1[i] * c1 + k2[i] * c2 + k3[i] * c3 + k4[i] * c4 + k5[i] * c5);
-        Sys::f(L.p, t1p, ys, k6);
+        rhs_aux<Sys, T>(L.p, aux[4], ys, k6);
     }
     T y1[S];
     {
@@ -283,7 +301,7 @@ __device__ __forceinline__ void lane_step(const LaneArgs &a, Lane<Sys, T> &L, in
 #pragma unroll
         for (int i = 0; i < S; i++)
             y1[i] = L.y[i] + (L.k1[i] * c1 + k3[i] * c3 + k4[i] * c4 + k5[i] * c5 + k6[i] * c6);
-        Sys::f(L.p, t1p, y1, k7);
+        rhs_aux<Sys, T>(L.p, aux[4], y1, k7);
     }
     T acc = 0;
     {
@@ -295,11 +313,11 @@ __device__ __forceinline__ void lane_step(const LaneArgs &a, Lane<Sys, T> &L, in
             const T err = L.k1[i] * e1 + k3[i] * e3 + k4[i] * e4 + k5[i] * e5 + k6[i] * e6 +
                           k7[i] * e7;
             const T tol = ratol + rrtol * fmax(fabs(L.y[i]), fabs(y1[i]));
-            const T r = err / tol;
+            const T r = err * fast_rcp<T>(tol);
             acc += r * r;
         }
     }
-    const double ratio = (double)sqrt(acc / (T)S);
+    const double ratio = (double)sqrt(acc * (T)(1.0 / S));
     L.steps++;
     if (isnan(ratio)) {
         L.status = BH_LANE_NONFINITE;
@@ -308,7 +326,7 @@ __device__ __forceinline__ void lane_step(const LaneArgs &a, Lane<Sys, T> &L, in
         return;
     }
     if (ratio <= 1.0) {
-        if (L.si < a.n_save && a.save_t[L.si] <= tB) {
+        if (L.next_save <= tB) {
             // dense output on [tA, tB] (torchdiffeq _interp_fit / _interp_evaluate)
             T ca[S], cb[S], cc[S], cd[S];
             const T m1 = h * (T)M1, m3 = h * (T)M3, m4 = h * (T)M4, m5 = h * (T)M5,
@@ -325,7 +343,7 @@ __device__ __forceinline__ void lane_step(const LaneArgs &a, Lane<Sys, T> &L, in
             }
             const double span = tB - tA;
             do {
-                const T x = (T)((a.save_t[L.si] - tA) / span);
+                const T x = (T)((L.next_save - tA) / span);
                 const T x2 = x * x, x3 = x2 * x, x4 = x3 * x;
                 T v[S];
 #pragma unroll
@@ -337,7 +355,8 @@ __device__ __forceinline__ void lane_step(const LaneArgs &a, Lane<Sys, T> &L, in
                 }
                 emit<Sys, T>(a, L, ic, L.si, v);
                 L.si++;
-            } while (L.si < a.n_save && a.save_t[L.si] <= tB);
+                L.next_save = L.si < a.n_save ? a.save_t[L.si] : INFINITY;
+            } while (L.next_save <= tB);
         }
 #pragma unroll
         for (int i = 0; i < S; i++) {

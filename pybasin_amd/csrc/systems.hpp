@@ -16,8 +16,53 @@ template <> __device__ __forceinline__ double dsin<double>(double x) { return si
 template <> __device__ __forceinline__ float dcos<float>(float x) { return cosf(x); }
 template <> __device__ __forceinline__ double dcos<double>(double x) { return cos(x); }
 
+// sin and cos of a small argument |d| <= 0.5 by their Taylor series (the
+// truncation error is below 2e-11 for float and 1e-17 for double).
+template <class T> __device__ __forceinline__ void sincos_small(T d, T &s, T &c);
+template <> __device__ __forceinline__ void sincos_small<float>(float d, float &s, float &c) {
+    const float d2 = d * d;
+    s = d + d * d2 * (-1.0f / 6 + d2 * (1.0f / 120 + d2 * (-1.0f / 5040 + d2 * (1.0f / 362880))));
+    c = 1.0f + d2 * (-0.5f + d2 * (1.0f / 24 + d2 * (-1.0f / 720 + d2 * (1.0f / 40320 + d2 * (-1.0f / 3628800)))));
+}
+template <> __device__ __forceinline__ void sincos_small<double>(double d, double &s, double &c) {
+    const double d2 = d * d;
+    double ps = 1.0 / 1307674368000.0;  // 1/15!
+    ps = -1.0 / 6227020800.0 + d2 * ps;
+    ps = 1.0 / 39916800.0 + d2 * ps;
+    ps = -1.0 / 362880.0 + d2 * ps;
+    ps = -1.0 / 5040.0 + d2 * ps;
+    ps = 1.0 / 120.0 + d2 * ps;
+    ps = -1.0 / 6.0 + d2 * ps;
+    s = d + d * d2 * ps;
+    double pc = 1.0 / 20922789888000.0;  // 1/16!
+    pc = -1.0 / 87178291200.0 + d2 * pc;
+    pc = 1.0 / 479001600.0 + d2 * pc;
+    pc = -1.0 / 3628800.0 + d2 * pc;
+    pc = 1.0 / 40320.0 + d2 * pc;
+    pc = -1.0 / 720.0 + d2 * pc;
+    pc = 1.0 / 24.0 + d2 * pc;
+    pc = -0.5 + d2 * pc;
+    c = 1.0 + d2 * pc;
+}
+template <class T> __device__ __forceinline__ void dsincos(T x, T &s, T &c);
+template <> __device__ __forceinline__ void dsincos<float>(float x, float &s, float &c) { sincosf(x, &s, &c); }
+template <> __device__ __forceinline__ void dsincos<double>(double x, double &s, double &c) { sincos(x, &s, &c); }
+
+// Per-step time terms.  A system whose right-hand side depends on t gets the
+// five distinct stage times of a step (t0 + c_i h for c = 1/5, 3/10, 4/5,
+// 8/9, and the c = 1 time) and may precompute what it needs from them; the
+// others ignore them.
+struct NoTimeTerms {
+    static constexpr bool kTime = false;
+    template <class T>
+    __device__ __forceinline__ static void time_terms(T, const T (&)[5], T (&aux)[5]) {
+#pragma unroll
+        for (int i = 0; i < 5; i++) aux[i] = T(0);
+    }
+};
+
 // tests/unit/solvers/test_solver.py:16-18 -- dy/dt = p0 * y
-template <int S_> struct Decay {
+template <int S_> struct Decay : NoTimeTerms {
     static constexpr int S = S_, NP = 1;
     static constexpr int FLOPS = S_;
     template <class T>
@@ -28,7 +73,7 @@ template <int S_> struct Decay {
 };
 
 // y' = M y with M = [[p0, p1], [p2, p3]] (harmonic / damped oscillators in tests)
-struct Linear2 {
+struct Linear2 : NoTimeTerms {
     static constexpr int S = 2, NP = 4;
     static constexpr int FLOPS = 6;
     template <class T>
@@ -40,7 +85,7 @@ struct Linear2 {
 
 // case_studies/pendulum/pendulum_ode.py:32-40
 //   theta' = omega;  omega' = -alpha*omega + T - K*sin(theta);  p = [alpha, T, K]
-struct Pendulum {
+struct Pendulum : NoTimeTerms {
     static constexpr int S = 2, NP = 3;
     static constexpr int FLOPS = 5;
     template <class T>
@@ -52,19 +97,56 @@ struct Pendulum {
 
 // case_studies/duffing_oscillator/duffing_ode.py:81-88
 //   x' = v;  v' = -delta*v - k3*x^3 + A*cos(t);  p = [delta, k3, A]
+// Within a step the stage times t_i (values of the state dtype) differ from
+// t_0 by d_i = t_i - t_0, which the subtraction forms exactly once
+// t_i <= 2 t_0 (Sterbenz; every step after the first few); cos(t_i) is then
+// cos(t_0) cos(d_i) - sin(t_0) sin(d_i): one range-reduced sincos per step
+// instead of one cos per stage, for the same value of cos(t_i).
 struct Duffing {
     static constexpr int S = 2, NP = 3;
     static constexpr int FLOPS = 8;
+    static constexpr bool kTime = true;
     template <class T>
     __device__ __forceinline__ static void f(const T (&p)[NP], T t, const T (&y)[S], T (&dy)[S]) {
+        f_aux(p, dcos<T>(t), y, dy);
+    }
+    template <class T>
+    __device__ __forceinline__ static void f_aux(const T (&p)[NP], T cos_t, const T (&y)[S], T (&dy)[S]) {
         dy[0] = y[1];
-        dy[1] = -p[0] * y[1] - p[1] * (y[0] * y[0] * y[0]) + p[2] * dcos<T>(t);
+        dy[1] = -p[0] * y[1] - p[1] * (y[0] * y[0] * y[0]) + p[2] * cos_t;
+    }
+    template <class T>
+    __device__ __forceinline__ static void time_terms(T t0, const T (&ts)[5], T (&aux)[5]) {
+        T s0, c0;
+        dsincos<T>(t0, s0, c0);
+#pragma unroll
+        for (int i = 0; i < 5; i++) {
+            const T d = ts[i] - t0;
+            if (fabs(d) <= T(0.5)) {
+                T sd, cd;
+                sincos_small<T>(d, sd, cd);
+                aux[i] = c0 * cd - s0 * sd;
+            } else {
+                aux[i] = dcos<T>(ts[i]);
+            }
+        }
     }
 };
 
+// right-hand side with the per-step time terms (time-independent systems
+// ignore them)
+template <class Sys, class T>
+__device__ __forceinline__ void rhs_aux(const T (&p)[Sys::NP], T aux, const T (&y)[Sys::S],
+                                        T (&dy)[Sys::S]) {
+    if constexpr (Sys::kTime)
+        Sys::f_aux(p, aux, y, dy);
+    else
+        Sys::f(p, T(0), y, dy);
+}
+
 // case_studies/lorenz/lorenz_ode.py:128-152, including the freeze mask:
 // the derivative is multiplied by (max|y| < 200).
-struct Lorenz {
+struct Lorenz : NoTimeTerms {
     static constexpr int S = 3, NP = 3;
     static constexpr int FLOPS = 9;
     template <class T>
