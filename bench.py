@@ -54,6 +54,11 @@ def parse():
     ap.add_argument("--max-lanes", type=int, default=0)
     ap.add_argument("--cpu-sample", type=int, default=20_000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workload", choices=["duffing", "surrogate"], default="duffing",
+                    help="duffing: plain HIP Dopri5 on 100k Duffing ICs (default); surrogate: "
+                         "pendulum torque sweep through the MLP gate with Dopri5 fallback")
+    ap.add_argument("--gate", type=float, default=0.995)
+    ap.add_argument("--epochs", type=int, default=600)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 256 MiB device copy (calibrates the "
                          "FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -88,6 +93,76 @@ def cpu_baseline(y0_cpu, save64, n_sample):
                       f"float32, oracle per-lane Dopri5 (C, pthreads), {dt:.2f} s"}
 
 
+def surrogate_workload(args):
+    """Pendulum torque sweep: 46 T in [0.1, 1.0], uneven counts (~4.4M ICs);
+    the MLP (trained once on 50,000 integrator labels) classifies confident ICs,
+    Dopri5 the rest plus a 2 % audit per T.  A step = one gated pass over all ICs."""
+    import numpy as np
+
+    from pybasin_amd.solvers import HipDopri5Solver
+    from pybasin_amd.surrogate import (SurrogateMLP, basin_stability_by_group, classify_gated,
+                                       integrate_labels, torque_sweep_ics)
+    from pybasin_amd.systems import PendulumODE
+
+    T = np.linspace(0.1, 1.0, 46)
+    counts = np.random.default_rng(args.seed).integers(10_000, 200_001, 46)
+    x, group = torque_sweep_ics(T, counts, seed=args.seed)
+    n = x.shape[0]
+    solver = HipDopri5Solver(t_span=(0, 1000), t_steps=1000, t_eval=(900.0, 1000.0), device="cuda",
+                             rtol=1e-8, atol=1e-6, cache_dir=None)
+    ode = PendulumODE({"alpha": 0.1, "T": 0.5, "K": 1.0})
+    ts = 950.0
+    # one-time training on 50,000 integrator-labelled ICs
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    tidx = torch.randperm(n, generator=torch.Generator(device="cuda").manual_seed(1), device="cuda")[:50_000]
+    ylab = integrate_labels(solver, ode, x[tidx], ts)
+    model = SurrogateMLP(device="cuda:0", seed=args.seed)
+    hist = model.fit(x[tidx], ylab, epochs=args.epochs, batch=1024, lr=3e-3, seed=args.seed)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+    # plain path (reference fractions and its throughput)
+    for _ in range(max(1, args.warmup)):
+        plab = integrate_labels(solver, ode, x, ts)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        plab = integrate_labels(solver, ode, x, ts)
+    torch.cuda.synchronize()
+    ms_plain = 1e3 * (time.perf_counter() - t0) / args.steps
+    # gated path
+    for _ in range(args.warmup):
+        classify_gated(model, solver, ode, x, gate=args.gate, group=group, time_steady=ts)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        res = classify_gated(model, solver, ode, x, gate=args.gate, group=group, time_steady=ts)
+    torch.cuda.synchronize()
+    ms_gated = 1e3 * (time.perf_counter() - t0) / args.steps
+    bs_g = basin_stability_by_group(res.labels, group, 46)
+    bs_p = basin_stability_by_group(plab, group, 46)
+    max_err = float((bs_g - bs_p).abs().max())
+    audit_err = float((res.audit_surrogate != res.audit_integrated).float().mean())
+    out = {
+        "metric": "classified initial conditions/sec at <=0.5% basin-stability error vs plain",
+        "value": round(n / (ms_gated * 1e-3), 1), "unit": "ICs/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_gated, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+bf16",
+        "data": "synthetic: seeded-uniform ICs per torque value",
+        "config": {
+            "workload": "pendulum_torque_sweep_surrogate", "n_ics": n, "torques": 46,
+            "gate": args.gate, "routed_frac": round(float(res.routed.float().mean()), 4),
+            "audit_n": len(res.audit_idx), "audit_label_error": round(audit_err, 5),
+            "max_bs_error_vs_plain": round(max_err, 5),
+            "plain_ics_per_s": round(n / (ms_plain * 1e-3), 1), "plain_ms": round(ms_plain, 3),
+            "speedup_vs_plain": round(ms_plain / ms_gated, 3),
+            "train_n": 50_000, "train_epochs": args.epochs, "train_s": round(t_train, 2),
+            "final_train_loss": round(hist[-1], 4),
+        },
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     args = parse()
     if args.copy_calibration:
@@ -97,6 +172,9 @@ def main():
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
               "(ICs shard trivially across GPUs; see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "surrogate":
+        return surrogate_workload(args)
 
     from pybasin_amd.solvers import HipDopri5Solver
     from pybasin_amd.systems import DuffingODE

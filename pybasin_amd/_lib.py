@@ -24,7 +24,7 @@ BH_WORKSPACE_BYTES = 256
 # every symbol include/basinhip.h declares
 EXPORTS = ("bh_version", "bh_last_error", "bh_solve_batch", "bh_solve_rossler", "bh_gemm_bf16",
            "bh_mlp_bias_act", "bh_mlp_silu_bwd", "bh_mlp_softmax_ce", "bh_mlp_colsum",
-           "bh_mlp_adam", "bh_mlp_prep_input", "bh_mlp_classify")
+           "bh_mlp_adam", "bh_mlp_prep_input", "bh_mlp_classify", "bh_mlp_infer")
 
 _lock = threading.Lock()
 _lib: ctypes.CDLL | None = None
@@ -57,6 +57,7 @@ def _declare(L: ctypes.CDLL) -> None:
         "bh_mlp_adam": [vp, vp, vp, vp, i64, f32, f32, f32, f32, i64, vp, vp],
         "bh_mlp_prep_input": [vp, i64, f32, f32, f32, f32, f32, f32, vp, vp],
         "bh_mlp_classify": [vp, vp, i64, vp, vp, vp],
+        "bh_mlp_infer": [vp, i64, vp, vp, f32, f32, f32, f32, f32, f32, vp, vp, vp],
     }
     for name, args in sigs.items():
         fn = getattr(L, name)

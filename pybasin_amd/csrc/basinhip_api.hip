@@ -320,4 +320,34 @@ int32_t bh_mlp_classify(const float *Z, const float *bias, int64_t M, int32_t *l
     return 0;
 }
 
+int32_t bh_mlp_infer(const float *x, int64_t M, const void *w_packed, const float *biases,
+                     float c0, float s0, float c1, float s1, float c2, float s2, int32_t *label,
+                     float *conf, void *stream) {
+    g_err[0] = 0;
+    namespace P = bh::mlp;
+    if (M <= 0) return M == 0 ? 0 : fail(-5, "bad M");
+    if (!x || !w_packed || !biases || !label || !conf) return fail(-5, "null argument");
+    const int64_t tiles = (M + P::IT_ROWS - 1) / P::IT_ROWS;
+    if (tiles > 2147483647LL) return fail(-5, "M too large");
+    const P::bf16x4 *w = reinterpret_cast<const P::bf16x4 *>(w_packed);
+    P::InferArgs a;
+    a.x = x;
+    a.M = M;
+    a.w1 = w;                      // 1 kstep x 16 colblocks x 64 lanes
+    a.w2 = a.w1 + 1 * 16 * 64;     // 16 x 16 x 64
+    a.w3 = a.w2 + 16 * 16 * 64;
+    a.w4 = a.w3 + 16 * 16 * 64;    // 16 x 1 x 64
+    a.b1 = biases;
+    a.b2 = biases + 256;
+    a.b3 = biases + 512;
+    a.b4 = biases + 768;
+    a.c0 = c0; a.s0 = s0; a.c1 = c1; a.s1 = s1; a.c2 = c2; a.s2 = s2;
+    a.label = label;
+    a.conf = conf;
+    hipLaunchKernelGGL(P::mlp_infer_kernel, dim3((unsigned)tiles), dim3(256), 0,
+                       reinterpret_cast<hipStream_t>(stream), a);
+    BH_HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
 }  // extern "C"

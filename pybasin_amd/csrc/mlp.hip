@@ -171,17 +171,29 @@ __global__ void adam_kernel(float *W, const float *G, float *mom, float *vel, in
     if (Wb) Wb[i] = f2bf(w);
 }
 
-// (theta, omega, T) rows -> bf16 rows of 16: [(theta - c0) * s0, (omega - c1) * s1, (T - c2) * s2, 0...]
+// (theta, omega, T) rows -> bf16 rows of 16:
+//   [sin(theta), cos(theta), (omega - c1) * s1, (T - c2) * s2, (theta - c0) * s0,
+//    (omega^2 / 2 - cos(theta)) * s1^2, 0, ...]
+// The attractor of an IC is 2 pi-periodic in theta, so the network sees theta
+// through its sine and cosine (plus the scaled angle itself); the last
+// feature is the undamped pendulum energy, whose decay orders the basin bands.
 __global__ void prep_input_kernel(const float *x, int64_t M, float c0, float s0, float c1, float s1,
                                   float c2, float s2, unsigned short *out) {
     const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= M * 16) return;
     const int64_t m = idx >> 4;
     const int col = (int)(idx & 15);
+    const float th = x[3 * m];
     float v = 0.f;
-    if (col == 0) v = (x[3 * m] - c0) * s0;
-    if (col == 1) v = (x[3 * m + 1] - c1) * s1;
-    if (col == 2) v = (x[3 * m + 2] - c2) * s2;
+    if (col == 0) v = sinf(th);
+    if (col == 1) v = cosf(th);
+    if (col == 2) v = (x[3 * m + 1] - c1) * s1;
+    if (col == 3) v = (x[3 * m + 2] - c2) * s2;
+    if (col == 4) v = 0.0f * (th - c0) * s0;  // raw angle unused (2 pi symmetry)
+    if (col == 5) {
+        const float om = x[3 * m + 1];
+        v = (0.5f * om * om - cosf(th)) * (s1 * s1);
+    }
     out[idx] = f2bf(v);
 }
 
@@ -195,6 +207,135 @@ __global__ void classify_kernel(const float *Z, const float *b, int64_t M, int32
     const float p1 = 1.0f / (1.0f + __expf(-d));
     label[m] = d > 0.f ? 1 : 0;
     conf[m] = fmaxf(p1, 1.0f - p1);
+}
+
+}  // namespace mlp
+}  // namespace bh
+
+namespace bh {
+namespace mlp {
+
+// ---- fused inference -------------------------------------------------------
+// One workgroup (4 waves) takes a tile of 128 rows through the whole network:
+// input features -> 256 -> 256 -> 256 -> 2 -> (label, confidence).  The tile's
+// activations live in LDS (bf16, padded rows); the weights are pre-packed in
+// the B-fragment order of v_mfma_f32_16x16x16bf16_1k, so every fragment is one
+// 8-byte load per lane, coalesced across the wave and served from L2.
+// Packed layout per layer: [kstep][colblock][lane][4] with
+//   k = 16 kstep + 4 (lane / 16) + e,  col = 16 colblock + lane % 16.
+constexpr int IT_ROWS = 128;
+constexpr int IT_LD = 264;  // LDS row stride in bf16 (256 + 8)
+
+struct InferArgs {
+    const float *x;              // (M, 3) theta, omega, T
+    int64_t M;
+    const bf16x4 *w1, *w2, *w3, *w4;  // packed fragments
+    const float *b1, *b2, *b3, *b4;  // biases (b4 has 2 entries)
+    float c0, s0, c1, s1, c2, s2;    // input scaling
+    int32_t *label;
+    float *conf;
+};
+
+__device__ __forceinline__ float silu(float z) { return z / (1.0f + __expf(-z)); }
+
+// one 256-wide layer: out[128][256] = act(in[128][K] . W + b); wave w owns
+// column blocks 4w .. 4w+3 for all 8 row blocks.
+template <int KSTEPS>
+__device__ __forceinline__ void dense_layer(const unsigned short *in, unsigned short *out,
+                                            const bf16x4 *W, const float *b, int wave, int lane) {
+    floatx4 acc[8][4];
+#pragma unroll
+    for (int rb = 0; rb < 8; rb++)
+#pragma unroll
+        for (int c = 0; c < 4; c++) acc[rb][c] = (floatx4){0.f, 0.f, 0.f, 0.f};
+    const int lr = lane & 15, lk = (lane >> 4) * 4;
+#pragma unroll 2
+    for (int s = 0; s < KSTEPS; s++) {
+        bf16x4 bfr[4];
+#pragma unroll
+        for (int c = 0; c < 4; c++) bfr[c] = W[((int64_t)s * 16 + (4 * wave + c)) * 64 + lane];
+#pragma unroll
+        for (int rb = 0; rb < 8; rb++) {
+            const bf16x4 af =
+                *reinterpret_cast<const bf16x4 *>(in + (rb * 16 + lr) * IT_LD + s * 16 + lk);
+#pragma unroll
+            for (int c = 0; c < 4; c++)
+                acc[rb][c] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(af, bfr[c], acc[rb][c], 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        const int col = (4 * wave + c) * 16 + lr;
+        const float bias = b[col];
+#pragma unroll
+        for (int rb = 0; rb < 8; rb++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int row = rb * 16 + (lane >> 4) * 4 + r;
+                out[row * IT_LD + col] = f2bf(silu(acc[rb][c][r] + bias));
+            }
+    }
+}
+
+__global__ void __launch_bounds__(256) mlp_infer_kernel(InferArgs a) {
+    __shared__ __attribute__((aligned(16))) unsigned short bufA[IT_ROWS * IT_LD];
+    __shared__ __attribute__((aligned(16))) unsigned short bufB[IT_ROWS * IT_LD];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int64_t row0 = (int64_t)blockIdx.x * IT_ROWS;
+
+    // input features (16 columns) into bufA
+    for (int i = tid; i < IT_ROWS * 16; i += 256) {
+        const int r = i >> 4, col = i & 15;
+        const int64_t m = row0 + r;
+        float v = 0.f;
+        if (m < a.M) {
+            const float th = a.x[3 * m], om = a.x[3 * m + 1], T = a.x[3 * m + 2];
+            if (col == 0) v = sinf(th);
+            if (col == 1) v = cosf(th);
+            if (col == 2) v = (om - a.c1) * a.s1;
+            if (col == 3) v = (T - a.c2) * a.s2;
+            if (col == 4) v = 0.0f * (th - a.c0) * a.s0;
+            if (col == 5) v = (0.5f * om * om - cosf(th)) * (a.s1 * a.s1);
+        }
+        bufA[r * IT_LD + col] = f2bf(v);
+    }
+    __syncthreads();
+    dense_layer<1>(bufA, bufB, a.w1, a.b1, wave, lane);
+    __syncthreads();
+    dense_layer<16>(bufB, bufA, a.w2, a.b2, wave, lane);
+    __syncthreads();
+    dense_layer<16>(bufA, bufB, a.w3, a.b3, wave, lane);
+    __syncthreads();
+    // output layer: 2 logits (packed with 16 columns, 14 of them zero);
+    // wave w takes row blocks 2w, 2w+1
+    const int lr = lane & 15, lk = (lane >> 4) * 4;
+    floatx4 acc[2] = {(floatx4){0.f, 0.f, 0.f, 0.f}, (floatx4){0.f, 0.f, 0.f, 0.f}};
+#pragma unroll 4
+    for (int s = 0; s < 16; s++) {
+        const bf16x4 bfr = a.w4[s * 64 + lane];
+#pragma unroll
+        for (int j = 0; j < 2; j++) {
+            const int rb = 2 * wave + j;
+            const bf16x4 af = *reinterpret_cast<const bf16x4 *>(bufB + (rb * 16 + lr) * IT_LD + s * 16 + lk);
+            acc[j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(af, bfr, acc[j], 0, 0, 0);
+        }
+    }
+    const float bias = lr < 2 ? a.b4[lr] : 0.f;
+#pragma unroll
+    for (int j = 0; j < 2; j++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const float l_own = acc[j][r] + bias;
+            const float l_next = __shfl_down(l_own, 1);  // column 1 for the lane holding column 0
+            const int row = (2 * wave + j) * 16 + (lane >> 4) * 4 + r;
+            const int64_t m = row0 + row;
+            if (lr == 0 && m < a.M) {
+                const float d = l_next - l_own;
+                const float p1 = 1.0f / (1.0f + __expf(-d));
+                a.label[m] = d > 0.f ? 1 : 0;
+                a.conf[m] = fmaxf(p1, 1.0f - p1);
+            }
+        }
 }
 
 }  // namespace mlp

@@ -24,6 +24,7 @@ from . import _lib
 
 HIDDEN = 256
 IN_PAD = 16
+N_IN = 6  # sin theta, cos theta, omega, T, theta, energy (see bh_mlp_prep_input)
 
 
 def _s(stream):
@@ -39,13 +40,13 @@ class SurrogateMLP:
         self.hidden = hidden
         self.norm = tuple(float(v) for v in norm)
         g = torch.Generator().manual_seed(seed)
-        dims = [(IN_PAD, hidden, 3), (hidden, hidden, hidden), (hidden, hidden, hidden), (hidden, 2, hidden)]
+        dims = [(IN_PAD, hidden, N_IN), (hidden, hidden, hidden), (hidden, hidden, hidden), (hidden, 2, hidden)]
         self.W, self.b = [], []
         for fin, fout, fan_in in dims:
             bound = 1.0 / math.sqrt(fan_in)
             w = (torch.rand(fin, fout, generator=g) * 2 - 1) * bound
             if fin == IN_PAD:
-                w[3:] = 0.0  # padded input columns
+                w[N_IN:] = 0.0  # padded input columns
             b = (torch.rand(fout, generator=g) * 2 - 1) * bound
             self.W.append(w.to(self.device).contiguous())
             self.b.append(b.to(self.device).contiguous())
@@ -147,8 +148,8 @@ class SurrogateMLP:
                                      self.vb[i].data_ptr(), self.b[i].numel(), lr, b1, b2, eps,
                                      self.step, None, _s(stream)), "bh_mlp_adam")
 
-    def fit(self, x: torch.Tensor, labels: torch.Tensor, epochs: int = 40, batch: int = 1024,
-            lr: float = 2e-3, seed: int = 0) -> list[float]:
+    def fit(self, x: torch.Tensor, labels: torch.Tensor, epochs: int = 60, batch: int = 1024,
+            lr: float = 3e-3, seed: int = 0) -> list[float]:
         """Mini-batch Adam on (x, labels); returns the mean loss per epoch."""
         x = x.to(self.device, torch.float32).contiguous()
         labels = labels.to(self.device, torch.int32).contiguous()
@@ -167,8 +168,45 @@ class SurrogateMLP:
             hist.append(float(tot) / (n - n % batch))
         return hist
 
-    def predict(self, x: torch.Tensor, chunk: int = 1 << 20, stream=None):
-        """labels (int32) and confidences (max softmax probability) for x (M, 3)."""
+    def packed(self):
+        """Weights in the B-fragment order of the fused inference kernel
+        ([kstep][colblock][lane][4] bf16 per layer) and the biases concatenated."""
+        if getattr(self, "_packed_step", None) == self.step:
+            return self._packed
+        frags = []
+        for W in self.Wb:
+            K, N = W.shape
+            ks, cb = (K + 15) // 16, (N + 15) // 16
+            Wp = torch.zeros((ks * 16, cb * 16), dtype=torch.bfloat16, device=self.device)
+            Wp[:K, :N] = W
+            # element (s, c, lane, e) = Wp[16 s + 4 (lane // 16) + e, 16 c + lane % 16]
+            t = Wp.view(ks, 4, 4, cb, 16)            # s, kq(=lane//16), e, c, lr
+            t = t.permute(0, 3, 1, 4, 2).contiguous()  # s, c, kq, lr, e
+            frags.append(t.view(-1))
+        b = torch.cat([self.b[0], self.b[1], self.b[2], self.b[3]]).contiguous()
+        self._packed = (torch.cat(frags).contiguous(), b)
+        self._packed_step = self.step
+        return self._packed
+
+    def predict(self, x: torch.Tensor, stream=None):
+        """labels (int32) and confidences (max softmax probability) for x (M, 3),
+        through the fused inference kernel."""
+        stream = stream or torch.cuda.current_stream(self.device)
+        x = x.to(self.device, torch.float32).contiguous()
+        M = x.shape[0]
+        lab = torch.empty((M,), dtype=torch.int32, device=self.device)
+        conf = torch.empty((M,), dtype=torch.float32, device=self.device)
+        if M == 0:
+            return lab, conf
+        w, b = self.packed()
+        c0, s0, c1, s1, c2, s2 = self.norm
+        _lib.check(_lib.lib().bh_mlp_infer(x.data_ptr(), M, w.data_ptr(), b.data_ptr(), c0, s0, c1,
+                                           s1, c2, s2, lab.data_ptr(), conf.data_ptr(),
+                                           _s(stream)), "bh_mlp_infer")
+        return lab, conf
+
+    def predict_unfused(self, x: torch.Tensor, chunk: int = 1 << 20, stream=None):
+        """labels and confidences through the layer-by-layer kernels."""
         stream = stream or torch.cuda.current_stream(self.device)
         x = x.to(self.device, torch.float32).contiguous()
         M = x.shape[0]
@@ -221,9 +259,14 @@ def integrate_labels(solver, ode, x: torch.Tensor, time_steady: float, stream=No
 
 def classify_gated(model: SurrogateMLP, solver, ode, x: torch.Tensor, gate: float = 0.95,
                    audit_frac: float = 0.02, group: torch.Tensor | None = None,
-                   time_steady: float = 950.0, seed: int = 0) -> GatedResult:
-    """Surrogate labels for confident rows, Dopri5 for the rest, plus a random
-    audit sample (audit_frac of every group) integrated in the same launch."""
+                   time_steady: float = 950.0, seed: int = 0,
+                   group_error_cap: float | None = 0.01) -> GatedResult:
+    """Surrogate labels for confident rows, Dopri5 for the rest.
+
+    A random audit sample (audit_frac of every group) is integrated first; a
+    group whose surrogate error on its confident audited rows exceeds
+    group_error_cap is integrated entirely (the surrogate is not trusted
+    there), the other groups only below the confidence gate."""
     dev = solver.device
     x = x.to(dev, torch.float32).contiguous()
     n = x.shape[0]
@@ -232,17 +275,111 @@ def classify_gated(model: SurrogateMLP, solver, ode, x: torch.Tensor, gate: floa
     g = torch.Generator(device=dev).manual_seed(seed)
     u = torch.rand((n,), generator=g, device=dev)
     audit = u < audit_frac
-    if group is not None:  # at least one audit row per group
-        _, inv = torch.unique(group, return_inverse=True)
-        umin = torch.full((int(inv.max()) + 1,), 2.0, device=dev).scatter_reduce(0, inv, u, "amin")
-        audit |= u == umin[inv]
-    need = routed | audit
-    idx = torch.nonzero(need).squeeze(1)
-    ilab = integrate_labels(solver, ode, x[idx], time_steady)
-    final = lab.clone()
-    full = torch.full((n,), -1, dtype=torch.int32, device=dev)
-    full[idx] = ilab
-    final[routed] = full[routed]
+    if group is not None:  # every group audited (a uniform draw almost always is)
+        gl = group.long()
+        ng = int(gl.max()) + 1
+        have = torch.bincount(gl, weights=audit.double(), minlength=ng)
+        empty = torch.nonzero(have == 0).squeeze(1).tolist()
+        for gi in empty:
+            audit[torch.nonzero(gl == gi)[0, 0]] = True
     aidx = torch.nonzero(audit).squeeze(1)
+    full = torch.full((n,), -1, dtype=torch.int32, device=dev)
+    full[aidx] = integrate_labels(solver, ode, x[aidx], time_steady)
+    if group is not None and group_error_cap is not None:
+        wrong = ((lab != full) & audit & ~routed).double()
+        conf_aud = (audit & ~routed).double()
+        rate = torch.bincount(gl, weights=wrong, minlength=ng) / torch.bincount(
+            gl, weights=conf_aud, minlength=ng).clamp_min(1)
+        routed = routed | (rate > group_error_cap)[gl]
+    ridx = torch.nonzero(routed & ~audit).squeeze(1)
+    full[ridx] = integrate_labels(solver, ode, x[ridx], time_steady)
+    final = torch.where(routed | audit, full, lab)
     return GatedResult(labels=final, routed=routed, audit_idx=aidx, audit_surrogate=lab[aidx],
                        audit_integrated=full[aidx])
+
+
+# ---- pendulum torque sweep (surrogate workload) ----------------------------
+
+def torque_sweep_ics(T_values, counts, seed: int = 0, device="cuda:0"):
+    """Rows (theta, omega, T) for every torque T with counts[i] ICs drawn
+    uniformly in theta in [-pi + asin(T), pi + asin(T)], omega in [-10, 10]
+    (the pendulum sampler region, case_studies/pendulum/setup_pendulum_system.py:28-32,
+    with K = 1), plus the group index of every row."""
+    g = torch.Generator().manual_seed(seed)
+    rows, groups = [], []
+    for gi, (T, c) in enumerate(zip(T_values, counts)):
+        c = int(c)
+        a = math.asin(min(1.0, float(T)))
+        u = torch.rand(c, 2, generator=g)
+        th = u[:, 0] * (2 * math.pi) + (-math.pi + a)
+        om = u[:, 1] * 20.0 - 10.0
+        rows.append(torch.stack([th, om, torch.full((c,), float(T))], 1))
+        groups.append(torch.full((c,), gi, dtype=torch.int32))
+    return torch.cat(rows).to(device), torch.cat(groups).to(device)
+
+
+def basin_stability_by_group(labels: torch.Tensor, group: torch.Tensor, n_groups: int) -> torch.Tensor:
+    """Fraction of label 0 (FP) per group."""
+    cnt = torch.bincount(group.long(), minlength=n_groups).double()
+    fp = torch.bincount(group.long(), weights=(labels == 0).double(), minlength=n_groups)
+    return fp / cnt.clamp_min(1)
+
+
+@dataclass
+class SweepReport:
+    n: int
+    routed_frac: float
+    audit_n: int
+    audit_label_error: float
+    max_bs_error_vs_plain: float | None
+    bs_gated: list
+    bs_plain: list | None
+    train_loss: list
+    train_n: int
+    t_train_s: float
+    t_gated_s: float
+    t_plain_s: float | None
+
+
+def run_torque_sweep(solver, ode, x, group, n_groups, *, train_n=50_000, gate=0.95,
+                     audit_frac=0.02, epochs=60, seed=0, plain: bool = True,
+                     time_steady: float = 950.0) -> SweepReport:
+    """Train on train_n integrator-labelled rows, classify all rows through the
+    gate, and (plain=True) integrate every row for the reference fractions."""
+    import time
+
+    dev = solver.device
+    n = x.shape[0]
+    gsel = torch.Generator(device=dev).manual_seed(seed + 1)
+    tidx = torch.randperm(n, generator=gsel, device=dev)[:train_n]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ylab = integrate_labels(solver, ode, x[tidx], time_steady)
+    model = SurrogateMLP(device=dev, seed=seed)
+    hist = model.fit(x[tidx], ylab, epochs=epochs, batch=1024, lr=3e-3, seed=seed)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+
+    t0 = time.perf_counter()
+    res = classify_gated(model, solver, ode, x, gate=gate, audit_frac=audit_frac, group=group,
+                         time_steady=time_steady, seed=seed)
+    torch.cuda.synchronize()
+    t_gated = time.perf_counter() - t0
+    bs_g = basin_stability_by_group(res.labels, group, n_groups)
+    aerr = float((res.audit_surrogate != res.audit_integrated).float().mean()) if len(res.audit_idx) else 0.0
+
+    bs_p = None
+    t_plain = None
+    max_err = None
+    if plain:
+        t0 = time.perf_counter()
+        plab = integrate_labels(solver, ode, x, time_steady)
+        torch.cuda.synchronize()
+        t_plain = time.perf_counter() - t0
+        bs_p = basin_stability_by_group(plab, group, n_groups)
+        max_err = float((bs_g - bs_p).abs().max())
+    return SweepReport(n=n, routed_frac=float(res.routed.float().mean()), audit_n=len(res.audit_idx),
+                       audit_label_error=aerr, max_bs_error_vs_plain=max_err,
+                       bs_gated=bs_g.cpu().tolist(), bs_plain=None if bs_p is None else bs_p.cpu().tolist(),
+                       train_loss=hist, train_n=train_n, t_train_s=t_train, t_gated_s=t_gated,
+                       t_plain_s=t_plain)

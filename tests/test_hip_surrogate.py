@@ -26,9 +26,12 @@ def _cpu_forward(model, x, labels):
     bs = [b.detach().cpu().clone().requires_grad_(True) for b in model.b]
     c0, s0, c1, s1, c2, s2 = model.norm
     xin = torch.zeros(x.shape[0], 16)
-    xin[:, 0] = (x[:, 0] - c0) * s0
-    xin[:, 1] = (x[:, 1] - c1) * s1
-    xin[:, 2] = (x[:, 2] - c2) * s2
+    xin[:, 0] = torch.sin(x[:, 0])
+    xin[:, 1] = torch.cos(x[:, 0])
+    xin[:, 2] = (x[:, 1] - c1) * s1
+    xin[:, 3] = (x[:, 2] - c2) * s2
+    xin[:, 4] = 0.0
+    xin[:, 5] = (0.5 * x[:, 1] ** 2 - torch.cos(x[:, 0])) * s1 * s1
     a = _bf(xin)
     for i in range(3):
         z = a @ Ws[i] + bs[i]
@@ -100,3 +103,67 @@ def test_training_separates_a_nonlinear_boundary(cuda_device):
     print(f"[result] ring accuracy {acc:.4f} loss {hist[0]:.3f} -> {hist[-1]:.3f}", flush=True)
     assert acc > 0.97
     assert float(conf.min()) >= 0.5 and float(conf.max()) <= 1.0
+
+
+def test_fused_inference_matches_layerwise_path(cuda_device):
+    """The fused 128-row-tile kernel gives the labels and confidences of the
+    layer-by-layer GEMM path (same bf16 roundings, different summation order)."""
+    model = SurrogateMLP(device=cuda_device, seed=4)
+    g = torch.Generator().manual_seed(5)
+    n = 1000  # not a multiple of the 128-row tile
+    x = torch.stack([torch.rand(n, generator=g) * 6.28 - 2.6, torch.rand(n, generator=g) * 20 - 10,
+                     torch.rand(n, generator=g) * 0.9 + 0.1], 1).to(cuda_device)
+    # a few Adam steps so the weights are not the initial draw
+    labels = (x[:, 1] > 0).to(torch.int32)
+    model.fit(x, labels, epochs=3, batch=250, lr=1e-3)
+    lab_f, conf_f = model.predict(x)
+    lab_u, conf_u = model.predict_unfused(x)
+    torch.cuda.synchronize()
+    agree = float((lab_f == lab_u).float().mean())
+    assert agree > 0.995, agree
+    torch.testing.assert_close(conf_f, conf_u, rtol=0, atol=2e-2)
+    # the CPU float32 restatement agrees too
+    logits_ref, _, _, _ = _cpu_forward(model, x.cpu(), labels.cpu())
+    p1 = torch.softmax(logits_ref.detach(), 1)[:, 1]
+    conf_ref = torch.maximum(p1, 1 - p1)
+    torch.testing.assert_close(conf_f.cpu(), conf_ref, rtol=0, atol=2e-2)
+
+
+def test_surrogate_gain_on_torque_sweep(cuda_device):
+    """What the trained network gains over integrating everything: on a
+    6-torque pendulum sweep it labels most ICs directly, the gated labels keep
+    every basin-stability fraction within 1 % of full integration, and the
+    audit sample measures its error."""
+    import time
+
+    from pybasin_amd.solvers import HipDopri5Solver
+    from pybasin_amd.surrogate import basin_stability_by_group, classify_gated, integrate_labels, torque_sweep_ics
+    from pybasin_amd.systems import PendulumODE
+
+    T = np.linspace(0.3, 0.8, 6)
+    x, group = torque_sweep_ics(T, [20_000] * 6, seed=3, device=cuda_device)
+    solver = HipDopri5Solver(t_span=(0, 1000), t_steps=1000, t_eval=(900.0, 1000.0), device="cuda",
+                             cache_dir=None)
+    ode = PendulumODE({"alpha": 0.1, "T": 0.5, "K": 1.0})
+    tr = torch.randperm(x.shape[0], device=cuda_device)[:20_000]
+    model = SurrogateMLP(device=cuda_device, seed=0)
+    model.fit(x[tr], integrate_labels(solver, ode, x[tr], 950.0), epochs=150, batch=1024, lr=3e-3)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    plain = integrate_labels(solver, ode, x, 950.0)
+    torch.cuda.synchronize()
+    t_plain = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    res = classify_gated(model, solver, ode, x, gate=0.995, group=group)
+    torch.cuda.synchronize()
+    t_gated = time.perf_counter() - t0
+    routed = float(res.routed.float().mean())
+    err = float((basin_stability_by_group(res.labels, group, 6) -
+                 basin_stability_by_group(plain, group, 6)).abs().max())
+    audit = float((res.audit_surrogate != res.audit_integrated).float().mean())
+    print(f"[result] routed {routed:.3f} max BS err {err:.4f} audit err {audit:.4f} "
+          f"plain {t_plain * 1e3:.1f} ms gated {t_gated * 1e3:.1f} ms", flush=True)
+    assert routed < 0.6          # the network labels most ICs itself
+    assert err < 0.01
+    assert audit < 0.15
+    assert len(res.audit_idx) >= 6  # every torque audited
