@@ -1,0 +1,109 @@
+"""End-to-end labels of the HIP path against the reference's seeded ground truths.
+
+The ICs are regenerated from seed 42 (bit-exact with the reference CSVs,
+checked on the CPU in test_oracle_golden.py); the fixtures hold only the
+labels (tests/integration/*/ground_truths, produced by the reference's own
+pipeline with per-trajectory Dopri5 in float32).  Each case uses the
+reference case study's feature and template recipe:
+
+* pendulum case 1 (case_studies/pendulum/setup_pendulum_system.py:34-65):
+  t in [0, 1000], 1000 saves, time_steady 950, log_delta of omega, 1-NN on
+  templates [0.4, 0] FP / [2.7, 0] LC;
+* Duffing (setup_duffing_oscillator_system.py:98-139): save window
+  [900, 1000] with 5000 points, maximum and std of x, 1-NN on 5 templates;
+* Lorenz (setup_lorenz_system.py:192-227): save window [900, 1000], 4000
+  points, stop at |y|inf >= 200 with Inf fill (the Event semantics; such
+  trajectories are "unbounded", basin_stability_estimator.py:519-527), mean x
+  and 1-NN on the two bounded templates.
+"""
+
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as O
+from pybasin_amd import _lib
+from pybasin_amd.estimator import KNNTemplateClassifier, compute_bs
+from pybasin_amd.features import feature_from_stats
+from pybasin_amd.solvers import HipDopri5Solver
+from pybasin_amd.systems import DuffingODE, LorenzODE, PendulumODE
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+@pytest.fixture(scope="module")
+def fixtures():
+    with open(os.path.join(GOLD, "label_fixtures.json")) as f:
+        return json.load(f)
+
+
+def _report(name, got, want):
+    got, want = np.asarray(got), np.asarray(want)
+    agree = float((got == want).mean())
+    bs_g, bs_w = compute_bs(got), compute_bs(want)
+    n = len(want)
+    worst = max(abs(bs_g.get(k, 0.0) - bs_w.get(k, 0.0)) / max(math.sqrt(bs_w[k] * (1 - bs_w[k]) / n), 1e-12)
+                for k in bs_w)
+    print(f"[result] {name}: label agreement {agree:.4f}; BS hip {bs_g} ref {bs_w}; "
+          f"worst |dBS| / SE = {worst:.2f}", flush=True)
+    return agree, worst
+
+
+def _run(solver, ode, y0, templates, time_steady):
+    batch = torch.cat([y0, torch.tensor(templates, dtype=y0.dtype)]).to("cuda")
+    save_ts, batch = solver._prepare_tensors(batch)
+    return solver.run_fused(ode, batch, save_ts, None, keep_trajectories=False, time_steady=time_steady)
+
+
+def test_pendulum_case1_labels(cuda_device, fixtures):
+    fx = fixtures["pendulum_case1"]
+    y0 = torch.tensor(O.sample_uniform(fx["n"], fx["min_limits"], fx["max_limits"], fx["seed"]))
+    solver = HipDopri5Solver(t_span=(0, 1000), t_steps=1000, device="cuda", cache_dir=None)
+    res = _run(solver, PendulumODE({"alpha": 0.1, "T": 0.5, "K": 1.0}), y0, [[0.4, 0.0], [2.7, 0.0]], 950.0)
+    feats = feature_from_stats("log_delta", res.stats)[:, 1:2]
+    knn = KNNTemplateClassifier(["FP", "LC"]).fit(feats[-2:])
+    lab = knn.predict(feats[:-2])
+    agree, worst = _report("pendulum case1", lab, fx["labels"])
+    assert agree >= 0.995
+    assert worst <= 1.0  # every basin-stability fraction within its binomial SE
+
+
+def test_duffing_labels(cuda_device, fixtures):
+    fx = fixtures["duffing_main"]
+    y0 = torch.tensor(O.sample_uniform(fx["n"], fx["min_limits"], fx["max_limits"], fx["seed"]))
+    templates = [[-0.21, 0.02], [1.05, 0.77], [-0.67, 0.02], [-0.46, 0.30], [-0.43, 0.12]]
+    names = ["$\\bar{y}_1$", "$\\bar{y}_2$", "$\\bar{y}_3$", "$\\bar{y}_4$", "$\\bar{y}_5$"]
+    solver = HipDopri5Solver(t_span=(0, 1000), t_steps=5000, t_eval=(900.0, 1000.0), device="cuda",
+                             cache_dir=None)
+    res = _run(solver, DuffingODE({"delta": 0.08, "k3": 1.0, "A": 0.2}), y0, templates, 900.0)
+    feats = torch.stack([feature_from_stats("maximum", res.stats)[:, 0],
+                         feature_from_stats("standard_deviation", res.stats)[:, 0]], 1)
+    lab = KNNTemplateClassifier(names).fit(feats[-5:]).predict(feats[:-5])
+    agree, worst = _report("duffing", lab, fx["labels"])
+    assert agree >= 0.995
+    assert worst <= 1.0
+
+
+def test_lorenz_labels(cuda_device, fixtures):
+    fx = fixtures["lorenz_main"]
+    y0 = torch.tensor(O.sample_uniform(fx["n"], fx["min_limits"], fx["max_limits"], fx["seed"]))
+    solver = HipDopri5Solver(t_span=(0, 1000), t_steps=4000, t_eval=(900.0, 1000.0), device="cuda",
+                             cache_dir=None, stop_threshold=200.0, inf_fill=True)
+    res = _run(solver, LorenzODE({"sigma": 0.12, "r": 0.0, "b": -0.6}), y0,
+               [[0.8, -3.0, 0.0], [-0.8, 3.0, 0.0]], 900.0)
+    status = res.status.cpu().numpy()
+    assert (status[-2:] == _lib.BH_LANE_OK).all()
+    feats = feature_from_stats("mean", res.stats)[:, 0:1]
+    lab = KNNTemplateClassifier(["chaotic attractor 1", "chaotic attractor 2"]).fit(feats[-2:]).predict(feats[:-2])
+    lab = np.where(status[:-2] == _lib.BH_LANE_DIVERGED, "unbounded", lab)
+    agree, worst = _report("lorenz", lab, fx["labels"])
+    # bounded attractors are chaotic: their identity is robust, but unbounded
+    # vs bounded must match exactly
+    assert ((lab == "unbounded") == (np.asarray(fx["labels"]) == "unbounded")).mean() >= 0.999
+    assert agree >= 0.99
+    assert worst <= 2.0
