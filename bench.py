@@ -54,7 +54,7 @@ def parse():
     ap.add_argument("--max-lanes", type=int, default=0)
     ap.add_argument("--cpu-sample", type=int, default=20_000)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=["duffing", "surrogate"], default="duffing",
+    ap.add_argument("--workload", choices=["duffing", "surrogate", "rossler"], default="duffing",
                     help="duffing: plain HIP Dopri5 on 100k Duffing ICs (default); surrogate: "
                          "pendulum torque sweep through the MLP gate with Dopri5 fallback")
     ap.add_argument("--gate", type=float, default=0.995)
@@ -163,6 +163,78 @@ def surrogate_workload(args):
     print(json.dumps(out), flush=True)
 
 
+def rossler_workload(args):
+    """Rossler network, N = 100 nodes on the fixed Watts-Strogatz graph, 11
+    coupling strengths K in [0.119, 0.159] x 10,000 ICs each (IC-major, one
+    launch), t in [0, 1000], rtol 1e-3, atol 1e-6, stop at |y|inf >= 400;
+    labels synchronized / desynchronized / unbounded from the final state."""
+    import numpy as np
+
+    from pybasin_amd.solvers import HipDopri5Solver
+    from pybasin_amd.solvers.rossler import sync_labels
+    from pybasin_amd.systems import RosslerNetworkODE
+
+    B = min(args.n, 10_000)
+    Ks = np.linspace(0.119, 0.159, 11)
+    torch.manual_seed(args.seed)
+    lo = torch.tensor([-15.0] * 200 + [-5.0] * 100)
+    hi = torch.tensor([15.0] * 200 + [35.0] * 100)
+    y0 = (torch.rand(B, 300) * (hi - lo) + lo).to("cuda")
+    params = torch.tensor([[0.2, 0.2, 7.0, k] for k in Ks], dtype=torch.float64)
+    ode = RosslerNetworkODE({"a": 0.2, "b": 0.2, "c": 7.0, "K": 0.119})
+    solver = HipDopri5Solver(t_span=(0, 1000), t_steps=2, t_eval=(999.0, 1000.0), device="cuda",
+                             rtol=1e-3, atol=1e-6, cache_dir=None, stop_threshold=400.0)
+    stream = torch.cuda.current_stream()
+    kev = []
+
+    def step(record=False):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        res = solver.integrate_fused(ode, y0, params)
+        e1.record(stream)
+        if record:
+            kev.append((e0, e1))
+        return res
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        res = step(record=True)
+    lab = sync_labels(res.final, res.status)
+    torch.cuda.synchronize()
+    ms = 1e3 * (time.perf_counter() - t0) / args.steps
+    kernel_ms = sum(a.elapsed_time(b) for a, b in kev) / len(kev)
+    n = B * len(Ks)
+    nsteps = res.nsteps.double()
+    # FLOP per attempted step: 6 evaluations x (coupling product 2 N^2 on the
+    # matrix cores + 8 N local) + 63 S RK arithmetic + 16
+    mfma_flop = float(nsteps.sum()) * 6 * 2 * 100 * 100
+    valu_flop = float(nsteps.sum()) * (6 * 8 * 100 + 63 * 300 + 16)
+    per_k = {}
+    for i, k in enumerate(Ks):
+        li = lab[i::len(Ks)]
+        per_k[f"{k:.3f}"] = {c: round(float(np.mean(li == c)), 4) for c in ("synchronized", "desynchronized", "unbounded")}
+    out = {
+        "metric": "classified initial conditions/sec at <=0.5% basin-stability error vs plain",
+        "value": round(n / (ms * 1e-3), 1), "unit": "ICs/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic: seeded-uniform ICs, 11 coupling strengths in one launch",
+        "config": {"workload": "rossler100_k_sweep", "n_trajectories": n, "ics_per_k": B,
+                   "rtol": 1e-3, "atol": 1e-6, "t_span": [0, 1000],
+                   "mean_steps": round(float(nsteps.mean()), 1), "max_steps": int(nsteps.max()),
+                   "basin_stability": per_k},
+        "roofline": {"bound": "mfma", "achieved": round(mfma_flop / (kernel_ms * 1e-3) / 1e12, 3),
+                     "peak": 157.3, "unit": "TFLOP/s",
+                     "frac": round(mfma_flop / (kernel_ms * 1e-3) / 157.3e12, 4), "traffic": None,
+                     "valu_tflops": round(valu_flop / (kernel_ms * 1e-3) / 1e12, 3),
+                     "kernel": "bh::ross::tile_kernel", "kernel_ms": round(kernel_ms, 3)},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     args = parse()
     if args.copy_calibration:
@@ -175,6 +247,8 @@ def main():
 
     if args.workload == "surrogate":
         return surrogate_workload(args)
+    if args.workload == "rossler":
+        return rossler_workload(args)
 
     from pybasin_amd.solvers import HipDopri5Solver
     from pybasin_amd.systems import DuffingODE

@@ -112,3 +112,28 @@ def test_labels_match_oracle_long_horizon(cuda_device):
     # trajectories still converging at t = 1000
     assert ((lab == "unbounded") == (ref_lab == "unbounded")).all()
     assert agree >= 0.9
+
+
+def test_k_sweep_matches_reference_fractions(cuda_device):
+    """Basin stability of synchronisation for K = 0.119 / 0.139 / 0.159 against
+    the reference's own n = 500 runs (rossler_network_k_sweep_comparison.json)
+    with the reference's two-sample z-test and threshold 3
+    (tests/integration/rossler_network/test_rossler_network.py:99, :189)."""
+    import json as _json
+
+    with open(os.path.join(GOLD, "rossler_golden.json")) as f:
+        ref = {round(r["parameter_value"], 3): {a["label"]: a["python_bs"] for a in r["attractors"]}
+               for r in _json.load(f)["reference_run_n500"]["parameter_results"]}
+    B, Ks = 2000, [0.119, 0.139, 0.159]
+    y0 = torch.tensor(O.sample_uniform(B, LO, HI, seed=42)).to(cuda_device)
+    params = torch.tensor([[0.2, 0.2, 7.0, k] for k in Ks], dtype=torch.float64)
+    res = _run(y0, params, 1000.0, 1e-3, 1e-6, stop_threshold=400.0)
+    lab = sync_labels(res.final, res.status)
+    for i, k in enumerate(Ks):
+        li = lab[i::len(Ks)]
+        for name in ("synchronized", "unbounded"):
+            p1, p2 = float(np.mean(li == name)), ref[k][name]
+            se = np.sqrt(p1 * (1 - p1) / B + p2 * (1 - p2) / 500)
+            z = abs(p1 - p2) / max(se, 1e-12)
+            print(f"[result] K={k} {name}: hip {p1:.4f} (n={B}) ref {p2:.3f} (n=500) z={z:.2f}", flush=True)
+            assert z < 3.0
