@@ -23,7 +23,7 @@ import numpy as np
 import torch
 
 from .features import HipFusedFeatureExtractor, Solution, StatFeatureExtractor
-from .solvers.hip_dopri5 import FUSED_RESULTS
+from .solvers.hip_dopri5 import FUSED_RESULTS, functor_id
 
 
 def set_seed(seed: int) -> None:
@@ -121,7 +121,7 @@ class BasinStabilityEstimator:
         dev = self.solver.device
         batch = torch.cat([y0.to(dev), self.template_y0.to(dev, y0.dtype)])
         fx = self.feature_extractor
-        if dev.type == "cuda" and getattr(self.ode_system, "device_system_id", None) is not None:
+        if dev.type == "cuda" and functor_id(self.ode_system) is not None:
             save_ts, batch_p = self.solver._prepare_tensors(batch)
             steady = fx.time_steady
             res = self.solver.run_fused(self.ode_system, batch_p, save_ts, None,

@@ -51,16 +51,6 @@ class ODESystem(nn.Module, ABC):
             body = type(self).__name__
         return f"{type(self).__name__}:{body}:{dict(self.params)}"
 
-    def device_params(self, params: torch.Tensor | None) -> tuple[torch.Tensor, int]:
-        """Parameter block for the HIP kernel: float64, (rows, n_p) with row
-        stride n_p, or one shared row with stride 0."""
-        if params is None:
-            flat = torch.tensor([[float(v) for v in dict(self.params).values()]],
-                                dtype=torch.float64)
-            return flat, 0
-        p = params.to(torch.float64)
-        return p.contiguous(), p.shape[1]
-
 
 def ode_param_count(ode: ODESystem) -> int:
     return len(dict(ode.params))

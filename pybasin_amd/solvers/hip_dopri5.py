@@ -39,6 +39,40 @@ logger = logging.getLogger("pybasin_amd")
 FUSED_RESULTS: "weakref.WeakKeyDictionary[torch.Tensor, FusedResult]" = weakref.WeakKeyDictionary()
 
 
+# ODE classes that do not carry ``device_system_id`` themselves (e.g. the
+# reference's own case-study classes) opt in here; registration is explicit
+# so a class is never matched to a functor by name alone.
+_FUNCTORS: "dict[type, int]" = {}
+
+
+def register_functor(ode_class: type, system_id: int) -> None:
+    """Integrate instances of ode_class with the HIP functor system_id
+    (include/basinhip.h BH_SYS_*); the class's params must follow that
+    functor's parameter order."""
+    _FUNCTORS[ode_class] = int(system_id)
+
+
+def functor_id(ode_system) -> int | None:
+    sid = getattr(ode_system, "device_system_id", None)
+    if sid is not None:
+        return int(sid)
+    for cls in type(ode_system).__mro__:
+        if cls in _FUNCTORS:
+            return _FUNCTORS[cls]
+    return None
+
+
+def device_params(ode_system, params: torch.Tensor | None) -> tuple[torch.Tensor, int]:
+    """float64 parameter block for the kernels: (rows, n_p) with stride n_p,
+    or the system's own parameters as one row with stride 0 (built from the
+    params dict in declaration order, as params_to_array does)."""
+    if params is None:
+        vals = [float(v) for v in dict(ode_system.params).values()]
+        return torch.tensor([vals], dtype=torch.float64), 0
+    p = params.to(torch.float64).contiguous()
+    return p, p.shape[1]
+
+
 @dataclass
 class FusedResult:
     """Outputs of one fused HIP integration.
@@ -116,7 +150,7 @@ class HipDopri5Solver(Solver):
                                  cache_dir=None, t_eval=self.t_eval)
 
     def _integrate(self, ode_system, y0, save_ts, params=None):
-        sys_id = getattr(ode_system, "device_system_id", None)
+        sys_id = functor_id(ode_system)
         if self.device.type != "cuda" or sys_id is None:
             logger.info("[%s] %s on %s: torch shared-step Dopri5", type(self).__name__,
                         type(ode_system).__name__, self.device)
@@ -141,9 +175,10 @@ class HipDopri5Solver(Solver):
                   params: torch.Tensor | None, *, keep_trajectories: bool,
                   time_steady: float | None = None, steady_idx: int | None = None,
                   stream: torch.cuda.Stream | None = None) -> FusedResult:
-        sys_id = getattr(ode_system, "device_system_id", None)
+        sys_id = functor_id(ode_system)
         if sys_id is None:
-            raise ValueError(f"{type(ode_system).__name__} has no HIP device functor")
+            raise ValueError(f"{type(ode_system).__name__} has no HIP device functor "
+                             "(set device_system_id or call register_functor)")
         if sys_id == _lib.BH_SYS_ROSSLER100:
             from .rossler import run_rossler  # MFMA tile kernel
 
@@ -155,7 +190,7 @@ class HipDopri5Solver(Solver):
         dtype_id = _lib.BH_F32 if y0.dtype == torch.float32 else _lib.BH_F64
         y0 = y0.to(dev).contiguous()
         n, S = y0.shape
-        p, p_stride = ode_system.device_params(params)
+        p, p_stride = device_params(ode_system, params)
         p = p.to(dev).contiguous()
         if p_stride and p.shape[0] != n:
             raise ValueError(f"params rows {p.shape[0]} != trajectories {n}")

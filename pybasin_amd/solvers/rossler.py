@@ -17,7 +17,7 @@ from ..systems.rossler_network import N_NODES, rossler_laplacian
 def run_rossler(solver, ode_system, y0: torch.Tensor, save_ts: torch.Tensor,
                 params: torch.Tensor | None, *, keep_trajectories: bool,
                 stream: torch.cuda.Stream | None = None, max_tiles: int = 0):
-    from .hip_dopri5 import FusedResult
+    from .hip_dopri5 import FusedResult, device_params
 
     if keep_trajectories:
         raise ValueError("the Rossler network kernel returns the final state only; "
@@ -29,7 +29,7 @@ def run_rossler(solver, ode_system, y0: torch.Tensor, save_ts: torch.Tensor,
     n, S = y0.shape
     if S != 3 * N_NODES:
         raise ValueError(f"state dim must be {3 * N_NODES}, got {S}")
-    p, p_stride = ode_system.device_params(params)
+    p, p_stride = device_params(ode_system, params)
     p = p.to(dev).contiguous()
     if p_stride and p.shape[0] != n:
         raise ValueError(f"params rows {p.shape[0]} != trajectories {n}")
