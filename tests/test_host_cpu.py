@@ -1,0 +1,146 @@
+"""CPU tests of the host side: the torch shared-step Dopri5 (fallback and CPU
+clone), the Solver contract on it, feature extraction and the BS driver."""
+
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as O
+from pybasin_amd.estimator import (BasinStabilityEstimator, KNNTemplateClassifier,
+                                   UniformRandomSampler, compute_bs, get_errors)
+from pybasin_amd.features import (HipFusedFeatureExtractor, Solution, StatFeatureExtractor,
+                                  feature_from_stats, impute_extreme, stats_from_window)
+from pybasin_amd.solvers import HipDopri5Solver, TorchDopri5Solver, functor_id, register_functor
+from pybasin_amd.systems import DuffingODE, ExponentialDecayODE, LorenzODE, PendulumODE
+from pybasin_amd.ode_system import ODESystem
+
+
+@pytest.mark.parametrize("dtype,bits", [(torch.float32, 32), (torch.float64, 64)])
+def test_torch_solver_equals_oracle_shared_mode(dtype, bits):
+    """The torch fallback runs torchdiffeq's shared-dt algorithm: it must track
+    the oracle's shared mode on a small batch (same decisions, rounding only)."""
+    y0 = torch.tensor(O.sample_uniform(6, [-1.0, -0.5], [1.0, 1.0], 9), dtype=dtype)
+    s = TorchDopri5Solver(t_span=(0, 20), t_steps=41, device="cpu", cache_dir=None)
+    t, y = s.integrate(DuffingODE({"delta": 0.08, "k3": 1.0, "A": 0.2}), y0)
+    ref, st, _ = O.solve(O.SYS_DUFFING, y0.double().numpy(), np.array([0.08, 1.0, 0.2]),
+                         t.double().numpy(), 0.0, bits=bits, mode=O.MODE_TD_SHARED)
+    tol = 1e-4 if bits == 32 else 1e-9
+    np.testing.assert_allclose(y.double().numpy(), ref, atol=tol, rtol=0)
+
+
+def test_torch_solver_reproduces_reference_torchdiffeq_table():
+    """Shared-step float64 run of the 5 zigode test ICs: final theta equals the
+    reference's published torchdiffeq column (src/zigode/README.md:70-76)."""
+    ics = torch.tensor([[0.4, 0.0], [2.7, 0.0], [0.5, -5.0], [-2.0, 3.0], [1.0, 0.0]], dtype=torch.float64)
+    s = TorchDopri5Solver(t_span=(0, 1000), t_steps=1000, device="cpu", cache_dir=None)
+    _, y = s.integrate(PendulumODE({"alpha": 0.1, "T": 0.5, "K": 1.0}), ics)
+    want = [0.523599, 4947.703034, 4904.614031, 4975.223032, 0.523599]
+    np.testing.assert_allclose(y[-1, :, 0].numpy(), want, atol=2e-5)
+
+
+def test_solver_contract_on_cpu():
+    ode = ExponentialDecayODE({"decay": -1.0})
+    s = TorchDopri5Solver(t_span=(0, 1), t_steps=10, device="cpu", cache_dir=None)
+    t, y = s.integrate(ode, torch.tensor([[1.0]]))
+    assert t.shape == (10,) and y.shape == (10, 1, 1)
+    assert y[0].item() == pytest.approx(1.0, abs=1e-5)
+    assert y[-1].item() == pytest.approx(math.exp(-1), abs=1e-5)
+    s2 = TorchDopri5Solver(t_span=(0, 2), t_steps=10, t_eval=(1, 2), device="cpu", cache_dir=None)
+    _, y2 = s2.integrate(ode, torch.tensor([[1.0]]))
+    assert y2[0].item() == pytest.approx(math.exp(-1), abs=1e-5)
+    _, y3 = TorchDopri5Solver(t_span=(0, 1), t_steps=11, device="cpu", cache_dir=None).integrate(
+        ode, torch.tensor([[1.0]]), params=torch.tensor([[-1.0], [-2.0]]))
+    assert y3.shape == (11, 2, 1)
+    assert y3[-1, 1, 0].item() == pytest.approx(math.exp(-2), abs=1e-5)
+    with pytest.raises(ValueError):
+        s.integrate(ode, torch.tensor([1.0]))
+    with pytest.raises(ValueError, match="t_eval"):
+        TorchDopri5Solver(t_span=(0, 1), t_eval=(0.5, 1.5), device="cpu", cache_dir=None)
+
+
+def test_hip_solver_cpu_clone_uses_torch_path():
+    """A HIP solver on the CPU (TemplateIntegrator's clone) integrates with the
+    torch shared-step algorithm, no GPU needed."""
+    s = HipDopri5Solver(t_span=(0, 1), t_steps=5, device="cpu", cache_dir=None)
+    t, y = s.integrate(ExponentialDecayODE({"decay": -1.0}), torch.tensor([[1.0]]))
+    assert y[-1].item() == pytest.approx(math.exp(-1), abs=1e-5)
+    c = HipDopri5Solver(t_span=(0, 2), t_eval=(1, 2), device="cuda", cache_dir=None).clone(device="cpu")
+    assert c.device.type == "cpu" and c.t_eval == (1, 2)
+
+
+def test_functor_registry_is_explicit():
+    class MyPendulum(ODESystem):
+        def ode(self, t, y, p):
+            return torch.zeros_like(y)
+
+    assert functor_id(MyPendulum({"a": 1.0})) is None
+    register_functor(MyPendulum, 2)
+    assert functor_id(MyPendulum({"a": 1.0})) == 2
+    assert functor_id(PendulumODE({"alpha": 0.1, "T": 0.5, "K": 1.0})) == 2
+
+
+def test_cache_roundtrip(tmp_path):
+    s = TorchDopri5Solver(t_span=(0, 1), t_steps=5, device="cpu", cache_dir=str(tmp_path))
+    ode = ExponentialDecayODE({"decay": -1.0})
+    t1, y1 = s.integrate(ode, torch.tensor([[1.0]]))
+    files = list(tmp_path.iterdir())
+    assert len(files) == 1
+    t2, y2 = s.integrate(ode, torch.tensor([[1.0]]))
+    assert torch.equal(y1, y2)
+    files[0].write_bytes(b"corrupt")
+    t3, y3 = s.integrate(ode, torch.tensor([[1.0]]))
+    assert torch.allclose(y3, y1)
+
+
+def test_feature_columns_and_imputation():
+    t = torch.linspace(0, 10, 101)
+    y = torch.stack([torch.sin(t), t, torch.full_like(t, 2.0)], -1)[:, None, :].repeat(1, 3, 1)
+    sol = Solution(y[0], t, y)
+    fx = StatFeatureExtractor(time_steady=5.0, features=None,
+                              features_per_state={0: {"maximum": None, "standard_deviation": None},
+                                                  2: {"log_delta": None}})
+    f = fx.extract_features(sol)
+    assert fx.feature_names == ["state_0__maximum", "state_0__standard_deviation", "state_2__log_delta"]
+    w = O.features(y[50:].double().numpy())
+    np.testing.assert_allclose(f[:, 0].numpy(), w["maximum"][:, 0], rtol=1e-6)
+    np.testing.assert_allclose(f[:, 1].numpy(), w["standard_deviation"][:, 0], rtol=1e-5)
+    assert f[0, 2].item() == pytest.approx(math.log(1e-10), rel=1e-5)
+    # the fused extractor falls back to the trajectory when no kernel stats exist
+    f2 = HipFusedFeatureExtractor(time_steady=5.0, features=None,
+                                  features_per_state={0: {"maximum": None, "standard_deviation": None},
+                                                      2: {"log_delta": None}}).extract_features(sol)
+    assert torch.equal(f, f2)
+    imp = impute_extreme(torch.tensor([[float("inf"), -float("inf"), float("nan"), 3.0]]))
+    assert imp.tolist() == [[1e10, -1e10, 1e10, 3.0]]
+    st = stats_from_window(y[50:])
+    assert torch.allclose(feature_from_stats("delta", st), (st[..., 3] - st[..., 0]).abs())
+
+
+def test_bs_and_errors_match_reference_json():
+    labels = np.array(["FP"] * 602 + ["LC"] * 3398)
+    bs = compute_bs(labels)
+    err = get_errors(bs, len(labels))
+    assert bs["FP"] == pytest.approx(0.1505)
+    assert err["FP"]["e_abs"] == pytest.approx(0.005653533187308623)
+
+
+def test_estimator_on_cpu_with_templates():
+    """integrate -> features -> 1-NN on templates -> BS, on the CPU path."""
+    torch.manual_seed(0)
+    sampler = UniformRandomSampler([-1.0, -0.5], [1.0, 1.0], device="cpu")
+    solver = TorchDopri5Solver(t_span=(0, 20), t_steps=21, device="cpu", cache_dir=None, rtol=1e-6)
+    fx = StatFeatureExtractor(time_steady=10.0, features=None,
+                              features_per_state={0: {"maximum": None}})
+    bse = BasinStabilityEstimator(ExponentialDecayODE({"decay": -1.0}), sampler, 8, solver, fx,
+                                  template_y0=[[0.5, 0.5], [-0.5, -0.5]], template_labels=["pos", "neg"])
+    y0 = torch.tensor([[0.9, 0.1], [-0.9, 0.1], [0.4, -0.4], [-0.2, 0.3]])
+    res = bse.run(y0)
+    assert list(res.labels) == ["pos", "neg", "pos", "neg"]
+    assert res.basin_stability == {"neg": 0.5, "pos": 0.5}
+
+
+def test_knn_template_classifier():
+    knn = KNNTemplateClassifier(["a", "b"]).fit(torch.tensor([[0.0], [1.0]]))
+    assert list(knn.predict(torch.tensor([[0.2], [0.9], [-5.0]]))) == ["a", "b", "a"]
