@@ -248,15 +248,6 @@ template <class Sys, class T>
 __device__ __forceinline__ void lane_step(const LaneArgs &a, Lane<Sys, T> &L, int64_t ic) {
     constexpr int S = Sys::S;
     const double tA = L.t, tB = L.t + L.dt;
-    bool finite = true;
-#pragma unroll
-    for (int i = 0; i < S; i++) finite = finite && isfinite(L.y[i]);
-    if (!(tB > tA) || !finite) {
-        L.status = finite ? BH_LANE_DT_UNDERFLOW : BH_LANE_NONFINITE;
-        lane_fill_rest<Sys, T>(a, L, ic, false);
-        L.done = true;
-        return;
-    }
     const T tf = (T)tA, h = (T)L.dt, t1p = prev_value<T>((T)tB);
     T k2[S], k3[S], k4[S], k5[S], k6[S], k7[S], ys[S];
     const T tst[5] = {tf + (T)C2 * h, tf + (T)C3 * h, tf + (T)C4 * h, tf + (T)C5 * h, t1p};
@@ -318,72 +309,72 @@ __device__ __forceinline__ void lane_step(const LaneArgs &a, Lane<Sys, T> &L, in
         }
     }
     const double ratio = (double)sqrt(acc * (T)(1.0 / S));
-    L.steps++;
-    if (isnan(ratio)) {
-        L.status = BH_LANE_NONFINITE;
-        lane_fill_rest<Sys, T>(a, L, ic, false);
-        L.done = true;
-        return;
-    }
-    if (ratio <= 1.0) {
-        if (L.next_save <= tB) {
-            // dense output on [tA, tB] (torchdiffeq _interp_fit / _interp_evaluate)
-            T ca[S], cb[S], cc[S], cd[S];
-            const T m1 = h * (T)M1, m3 = h * (T)M3, m4 = h * (T)M4, m5 = h * (T)M5,
-                    m6 = h * (T)M6, m7 = h * (T)M7;
+    // Failure and acceptance are decided after the stages, so the common path
+    // has one divergent branch (the dense output) and one exit test; the
+    // state update is a select.
+    bool finite = true;
 #pragma unroll
-            for (int i = 0; i < S; i++) {
-                const T ymid = L.y[i] + (L.k1[i] * m1 + k3[i] * m3 + k4[i] * m4 + k5[i] * m5 +
-                                         k6[i] * m6 + k7[i] * m7);
-                const T f0 = L.k1[i], f1 = k7[i], a0 = L.y[i], a1 = y1[i];
-                ca[i] = (T)2 * h * (f1 - f0) - (T)8 * (a1 + a0) + (T)16 * ymid;
-                cb[i] = h * ((T)5 * f0 - (T)3 * f1) + (T)18 * a0 + (T)14 * a1 - (T)32 * ymid;
-                cc[i] = h * (f1 - (T)4 * f0) - (T)11 * a0 - (T)5 * a1 + (T)16 * ymid;
-                cd[i] = h * f0;
-            }
-            const double span = tB - tA;
-            do {
-                const T x = (T)((L.next_save - tA) / span);
-                const T x2 = x * x, x3 = x2 * x, x4 = x3 * x;
-                T v[S];
-#pragma unroll
-                for (int i = 0; i < S; i++) {
-                    T s = L.y[i] + x * cd[i];
-                    s = s + x2 * cc[i];
-                    s = s + x3 * cb[i];
-                    v[i] = s + x4 * ca[i];
-                }
-                emit<Sys, T>(a, L, ic, L.si, v);
-                L.si++;
-                L.next_save = L.si < a.n_save ? a.save_t[L.si] : INFINITY;
-            } while (L.next_save <= tB);
-        }
+    for (int i = 0; i < S; i++) finite = finite && isfinite(L.y[i]);
+    const bool bad = !(tB > tA) || !finite || isnan(ratio);
+    const bool accept = !bad && ratio <= 1.0;
+    L.steps += bad ? 0 : 1;
+    if (accept && L.next_save <= tB) {
+        // dense output on [tA, tB] (torchdiffeq _interp_fit / _interp_evaluate)
+        T ca[S], cb[S], cc[S], cd[S];
+        const T m1 = h * (T)M1, m3 = h * (T)M3, m4 = h * (T)M4, m5 = h * (T)M5,
+                m6 = h * (T)M6, m7 = h * (T)M7;
 #pragma unroll
         for (int i = 0; i < S; i++) {
-            L.y[i] = y1[i];
-            L.k1[i] = k7[i];
+            const T ymid = L.y[i] + (L.k1[i] * m1 + k3[i] * m3 + k4[i] * m4 + k5[i] * m5 +
+                                     k6[i] * m6 + k7[i] * m7);
+            const T f0 = L.k1[i], f1 = k7[i], a0 = L.y[i], a1 = y1[i];
+            ca[i] = (T)2 * h * (f1 - f0) - (T)8 * (a1 + a0) + (T)16 * ymid;
+            cb[i] = h * ((T)5 * f0 - (T)3 * f1) + (T)18 * a0 + (T)14 * a1 - (T)32 * ymid;
+            cc[i] = h * (f1 - (T)4 * f0) - (T)11 * a0 - (T)5 * a1 + (T)16 * ymid;
+            cd[i] = h * f0;
         }
-        L.t = tB;
-        if (L.si >= a.n_save) {
-            L.done = true;
-            return;
-        }
-        if (a.flags & BH_FLAG_STOP) {
-            T m = 0;
+        const double span = tB - tA;
+        do {
+            const T x = (T)((L.next_save - tA) / span);
+            const T x2 = x * x, x3 = x2 * x, x4 = x3 * x;
+            T v[S];
 #pragma unroll
-            for (int i = 0; i < S; i++) m = fmax(m, fabs(L.y[i]));
-            if (m >= (T)a.stop_thr) {
-                L.status = BH_LANE_DIVERGED;
-                lane_fill_rest<Sys, T>(a, L, ic, (a.flags & BH_FLAG_INF_FILL) != 0);
-                L.done = true;
-                return;
+            for (int i = 0; i < S; i++) {
+                T sv = L.y[i] + x * cd[i];
+                sv = sv + x2 * cc[i];
+                sv = sv + x3 * cb[i];
+                v[i] = sv + x4 * ca[i];
             }
-        }
+            emit<Sys, T>(a, L, ic, L.si, v);
+            L.si++;
+            L.next_save = L.si < a.n_save ? a.save_t[L.si] : INFINITY;
+        } while (L.next_save <= tB);
     }
+    T ymax = 0;
+#pragma unroll
+    for (int i = 0; i < S; i++) {
+        L.y[i] = accept ? y1[i] : L.y[i];
+        L.k1[i] = accept ? k7[i] : L.k1[i];
+        ymax = fmax(ymax, fabs(L.y[i]));
+    }
+    L.t = accept ? tB : L.t;
     L.dt = next_dt(L.dt, ratio);
-    if (L.steps >= a.max_steps) {
-        L.status = BH_LANE_MAX_STEPS;
-        lane_fill_rest<Sys, T>(a, L, ic, false);
+    const bool finished = accept && L.si >= a.n_save;
+    const bool stop = accept && !finished && (a.flags & BH_FLAG_STOP) && ymax >= (T)a.stop_thr;
+    const bool maxed = L.steps >= a.max_steps;
+    if (bad || finished || stop || maxed) {
+        if (!finished) {
+            bool inf = false;
+            if (bad) {
+                L.status = finite ? BH_LANE_DT_UNDERFLOW : BH_LANE_NONFINITE;
+            } else if (stop) {
+                L.status = BH_LANE_DIVERGED;
+                inf = (a.flags & BH_FLAG_INF_FILL) != 0;
+            } else {
+                L.status = BH_LANE_MAX_STEPS;
+            }
+            lane_fill_rest<Sys, T>(a, L, ic, inf);
+        }
         L.done = true;
     }
 }
@@ -426,7 +417,7 @@ __global__ void __launch_bounds__(256) dopri5_lane_kernel(LaneArgs a) {
     Lane<Sys, T> L;
     lane_init<Sys, T>(a, L, ic);
     for (;;) {
-        if (!L.done) lane_step<Sys, T>(a, L, ic);
+        if (!L.done) lane_step<Sys, T>(a, L, ic);  // not done unless init emitted every save
         if (L.done) {
             lane_finish<Sys, T>(a, L, ic);
             ic = (int64_t)atomicAdd(a.counter, 1ull);

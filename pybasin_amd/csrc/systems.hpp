@@ -117,18 +117,19 @@ struct Duffing {
     }
     template <class T>
     __device__ __forceinline__ static void time_terms(T t0, const T (&ts)[5], T (&aux)[5]) {
-        T s0, c0;
-        dsincos<T>(t0, s0, c0);
+        // one test per step: the c = 1 stage has the largest offset
+        if (fabs(ts[4] - t0) <= T(0.5)) {
+            T s0, c0;
+            dsincos<T>(t0, s0, c0);
 #pragma unroll
-        for (int i = 0; i < 5; i++) {
-            const T d = ts[i] - t0;
-            if (fabs(d) <= T(0.5)) {
+            for (int i = 0; i < 5; i++) {
                 T sd, cd;
-                sincos_small<T>(d, sd, cd);
+                sincos_small<T>(ts[i] - t0, sd, cd);
                 aux[i] = c0 * cd - s0 * sd;
-            } else {
-                aux[i] = dcos<T>(ts[i]);
             }
+        } else {
+#pragma unroll
+            for (int i = 0; i < 5; i++) aux[i] = dcos<T>(ts[i]);
         }
     }
 };
