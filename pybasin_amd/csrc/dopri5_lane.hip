@@ -7,10 +7,11 @@
 // batch tensor:
 //   * tableau, Shampine error weights and DPS_C_MID midpoint as restated in
 //     src/zigode/src/solver/dopri5_generic.zig:16-56;
-//   * stage arithmetic in the state dtype T, with the step coefficients
-//     (a_ij * dt) formed once per step; time-like values (t, dt, save times)
-//     in double; stage times cast to T, and the c = 1 stages evaluated at the
-//     previous representable value of (T)(t + dt) (torchdiffeq's Perturb.PREV);
+//   * stage arithmetic in the state dtype T (stage input y + dt sum_j a_ij k_j;
+//     torchdiffeq forms sum_j (a_ij dt) k_j, which differs in rounding only);
+//     time-like values (t, dt, save times) in double; stage times cast to T,
+//     and the c = 1 stages evaluated at the previous representable value of
+//     (T)(t + dt) (torchdiffeq's Perturb.PREV);
 //   * error ratio RMS(err / (atol + rtol * max(|y0|, |y1|))) <= 1 accepts;
 //     next dt = dt * clamp(0.9 * ratio^(-1/5), dfac, 10) with dfac = 1 on
 //     accept and 0.2 on reject (dopri5_generic.zig:228-235);
@@ -108,15 +109,15 @@ template <class T> __device__ __forceinline__ T tpow(T x, T e);
 template <> __device__ __forceinline__ float tpow<float>(float x, float e) { return powf(x, e); }
 template <> __device__ __forceinline__ double tpow<double>(double x, double e) { return pow(x, e); }
 
-// Controller factor 0.9 * ratio^(-1/5) clamped to [dfac, 10].  The power is
-// taken in single precision with the hardware log2/exp2 (relative error
-// ~1e-7 in the next dt, far below the step-size granularity that matters).
-__device__ __forceinline__ double next_dt(double dt, double ratio) {
-    if (ratio == 0.0) return dt * 10.0;
-    const float r = (float)ratio;
-    // hardware log2 / exp2; ratios below the normal range give a factor >= 10
-    float f = 0.9f * __builtin_amdgcn_exp2f(-0.2f * __builtin_amdgcn_logf(r));
-    const float dfac = ratio < 1.0 ? 1.0f : 0.2f;
+// Controller factor 0.9 * ratio^(-1/5) clamped to [dfac, 10], taken from the
+// squared ratio r2 as 0.9 * r2^(-1/10) with the hardware log2 / exp2 in
+// single precision (relative error ~1e-7 in the next dt, far below the
+// step-size granularity that matters; ratios below the normal range give a
+// factor >= 10).
+__device__ __forceinline__ double next_dt_sq(double dt, double r2) {
+    if (r2 == 0.0) return dt * 10.0;
+    float f = 0.9f * __builtin_amdgcn_exp2f(-0.1f * __builtin_amdgcn_logf((float)r2));
+    const float dfac = r2 < 1.0 ? 1.0f : 0.2f;
     f = fminf(10.0f, fmaxf(f, dfac));
     return dt * (double)f;
 }
@@ -253,62 +254,48 @@ __device__ __forceinline__ void lane_step(const LaneArgs &a, Lane<Sys, T> &L, in
     const T tst[5] = {tf + (T)C2 * h, tf + (T)C3 * h, tf + (T)C4 * h, tf + (T)C5 * h, t1p};
     T aux[5];
     Sys::time_terms(tf, tst, aux);
-    {
-        const T c = (T)A21 * h;
+    // Stage inputs y + h * sum_j a_ij k_j: the tableau constants are
+    // immediate operands and h is applied once per component.
 #pragma unroll
-        for (int i = 0; i < S; i++) ys[i] = L.y[i] + L.k1[i] * c;
-        rhs_aux<Sys, T>(L.p, aux[0], ys, k2);
-    }
-    {
-        const T c1 = (T)A31 * h, c2 = (T)A32 * h;
+    for (int i = 0; i < S; i++) ys[i] = fma(h, L.k1[i] * (T)A21, L.y[i]);
+    rhs_aux<Sys, T>(L.p, aux[0], ys, k2);
 #pragma unroll
-        for (int i = 0; i < S; i++) ys[i] = L.y[i] + (L.k1[i] * c1 + k2[i] * c2);
-        rhs_aux<Sys, T>(L.p, aux[1], ys, k3);
-    }
-    {
-        const T c1 = (T)A41 * h, c2 = (T)A42 * h, c3 = (T)A43 * h;
+    for (int i = 0; i < S; i++) ys[i] = fma(h, L.k1[i] * (T)A31 + k2[i] * (T)A32, L.y[i]);
+    rhs_aux<Sys, T>(L.p, aux[1], ys, k3);
 #pragma unroll
-        for (int i = 0; i < S; i++) ys[i] = L.y[i] + (L.k1[i] * c1 + k2[i] * c2 + k3[i] * c3);
-        rhs_aux<Sys, T>(L.p, aux[2], ys, k4);
-    }
-    {
-        const T c1 = (T)A51 * h, c2 = (T)A52 * h, c3 = (T)A53 * h, c4 = (T)A54 * h;
+    for (int i = 0; i < S; i++)
+        ys[i] = fma(h, L.k1[i] * (T)A41 + k2[i] * (T)A42 + k3[i] * (T)A43, L.y[i]);
+    rhs_aux<Sys, T>(L.p, aux[2], ys, k4);
 #pragma unroll
-        for (int i = 0; i < S; i++)
-            ys[i] = L.y[i] + (L.k1[i] * c1 + k2[i] * c2 + k3[i] * c3 + k4[i] * c4);
-        rhs_aux<Sys, T>(L.p, aux[3], ys, k5);
-    }
-    {
-        const T c1 = (T)A61 * h, c2 = (T)A62 * h, c3 = (T)A63 * h, c4 = (T)A64 * h,
-                c5 = (T)A65 * h;
+    for (int i = 0; i < S; i++)
+        ys[i] = fma(h, L.k1[i] * (T)A51 + k2[i] * (T)A52 + k3[i] * (T)A53 + k4[i] * (T)A54, L.y[i]);
+    rhs_aux<Sys, T>(L.p, aux[3], ys, k5);
 #pragma unroll
-        for (int i = 0; i < S; i++)
-            ys[i] = L.y[i] + (L.k1[i] * c1 + k2[i] * c2 + k3[i] * c3 + k4[i] * c4 + k5[i] * c5);
-        rhs_aux<Sys, T>(L.p, aux[4], ys, k6);
-    }
+    for (int i = 0; i < S; i++)
+        ys[i] = fma(h, L.k1[i] * (T)A61 + k2[i] * (T)A62 + k3[i] * (T)A63 + k4[i] * (T)A64 +
+                           k5[i] * (T)A65, L.y[i]);
+    rhs_aux<Sys, T>(L.p, aux[4], ys, k6);
     T y1[S];
-    {
-        const T c1 = (T)B1 * h, c3 = (T)B3 * h, c4 = (T)B4 * h, c5 = (T)B5 * h, c6 = (T)B6 * h;
 #pragma unroll
-        for (int i = 0; i < S; i++)
-            y1[i] = L.y[i] + (L.k1[i] * c1 + k3[i] * c3 + k4[i] * c4 + k5[i] * c5 + k6[i] * c6);
-        rhs_aux<Sys, T>(L.p, aux[4], y1, k7);
-    }
+    for (int i = 0; i < S; i++)
+        y1[i] = fma(h, L.k1[i] * (T)B1 + k3[i] * (T)B3 + k4[i] * (T)B4 + k5[i] * (T)B5 +
+                           k6[i] * (T)B6, L.y[i]);
+    rhs_aux<Sys, T>(L.p, aux[4], y1, k7);
+    // squared RMS error ratio (accept iff <= 1; the controller takes its
+    // -1/10 power, which is the -1/5 power of the ratio)
     T acc = 0;
     {
-        const T e1 = h * (T)E1, e3 = h * (T)E3, e4 = h * (T)E4, e5 = h * (T)E5, e6 = h * (T)E6,
-                e7 = h * (T)E7;
         const T ratol = (T)a.atol, rrtol = (T)a.rtol;
 #pragma unroll
         for (int i = 0; i < S; i++) {
-            const T err = L.k1[i] * e1 + k3[i] * e3 + k4[i] * e4 + k5[i] * e5 + k6[i] * e6 +
-                          k7[i] * e7;
+            const T err = h * (L.k1[i] * (T)E1 + k3[i] * (T)E3 + k4[i] * (T)E4 + k5[i] * (T)E5 +
+                               k6[i] * (T)E6 + k7[i] * (T)E7);
             const T tol = ratol + rrtol * fmax(fabs(L.y[i]), fabs(y1[i]));
             const T r = err * fast_rcp<T>(tol);
             acc += r * r;
         }
     }
-    const double ratio = (double)sqrt(acc * (T)(1.0 / S));
+    const double ratio = (double)(acc * (T)(1.0 / S));  // squared ratio
     // Failure and acceptance are decided after the stages, so the common path
     // has one divergent branch (the dense output) and one exit test; the
     // state update is a select.
@@ -358,7 +345,7 @@ __device__ __forceinline__ void lane_step(const LaneArgs &a, Lane<Sys, T> &L, in
         ymax = fmax(ymax, fabs(L.y[i]));
     }
     L.t = accept ? tB : L.t;
-    L.dt = next_dt(L.dt, ratio);
+    L.dt = next_dt_sq(L.dt, ratio);
     const bool finished = accept && L.si >= a.n_save;
     const bool stop = accept && !finished && (a.flags & BH_FLAG_STOP) && ymax >= (T)a.stop_thr;
     const bool maxed = L.steps >= a.max_steps;
