@@ -74,6 +74,19 @@ def device_params(ode_system, params: torch.Tensor | None) -> tuple[torch.Tensor
 
 
 @dataclass
+class LanePlan:
+    """Device-resident inputs of one bh_solve_batch launch."""
+
+    sys_id: int
+    y0: torch.Tensor
+    p: torch.Tensor
+    p_stride: int
+    save_ts: torch.Tensor
+    st64: torch.Tensor
+    steady_idx: int
+
+
+@dataclass
 class FusedResult:
     """Outputs of one fused HIP integration.
 
@@ -184,10 +197,22 @@ class HipDopri5Solver(Solver):
 
             return run_rossler(self, ode_system, y0, save_ts, params,
                                keep_trajectories=keep_trajectories, stream=stream)
+        plan = self.plan_fused(ode_system, y0, save_ts, params, time_steady=time_steady,
+                               steady_idx=steady_idx)
+        return self.launch_fused(plan, keep_trajectories=keep_trajectories, stream=stream)
+
+    def plan_fused(self, ode_system, y0: torch.Tensor, save_ts: torch.Tensor,
+                   params: torch.Tensor | None, *, time_steady: float | None = None,
+                   steady_idx: int | None = None) -> "LanePlan":
+        """Host-side preparation of a bh_solve_batch launch: validation and every
+        host-to-device copy, so that launch_fused issues device work only (and
+        can be captured into a graph)."""
+        sys_id = functor_id(ode_system)
+        if sys_id is None or sys_id == _lib.BH_SYS_ROSSLER100:
+            raise ValueError(f"{type(ode_system).__name__} has no per-lane device functor")
         dev = self.device
         if y0.dtype not in (torch.float32, torch.float64):
             raise ValueError(f"y0 dtype must be float32 or float64, got {y0.dtype}")
-        dtype_id = _lib.BH_F32 if y0.dtype == torch.float32 else _lib.BH_F64
         y0 = y0.to(dev).contiguous()
         n, S = y0.shape
         p, p_stride = device_params(ode_system, params)
@@ -195,9 +220,20 @@ class HipDopri5Solver(Solver):
         if p_stride and p.shape[0] != n:
             raise ValueError(f"params rows {p.shape[0]} != trajectories {n}")
         st64 = save_ts.to(device=dev, dtype=torch.float64).contiguous()
-        n_save = st64.shape[0]
         if steady_idx is None:
             steady_idx = steady_index(save_ts, time_steady)
+        return LanePlan(sys_id=int(sys_id), y0=y0, p=p, p_stride=p_stride, save_ts=save_ts,
+                        st64=st64, steady_idx=int(steady_idx))
+
+    def launch_fused(self, plan: "LanePlan", *, keep_trajectories: bool,
+                     stream: torch.cuda.Stream | None = None) -> FusedResult:
+        """Allocate the outputs and launch bh_solve_batch for a prepared plan."""
+        dev = self.device
+        y0, p, st64 = plan.y0, plan.p, plan.st64
+        dtype_id = _lib.BH_F32 if y0.dtype == torch.float32 else _lib.BH_F64
+        n, S = y0.shape
+        n_save = st64.shape[0]
+        steady_idx = plan.steady_idx
         traj = torch.empty((n_save, n, S), dtype=y0.dtype, device=dev) if keep_trajectories else None
         stats = torch.empty((n, S, _lib.BH_NFEAT), dtype=torch.float64, device=dev)
         final = torch.empty((n, S), dtype=y0.dtype, device=dev)
@@ -214,14 +250,15 @@ class HipDopri5Solver(Solver):
         s = stream if stream is not None else torch.cuda.current_stream(dev)
         with torch.cuda.device(dev):
             rc = _lib.lib().bh_solve_batch(
-                int(sys_id), dtype_id, y0.data_ptr(), n, S, p.data_ptr(), p.shape[1], p_stride,
-                float(self.t_span[0]), st64.data_ptr(), n_save, float(self.rtol),
-                float(self.atol), self.max_steps, flags, thr, int(steady_idx),
+                plan.sys_id, dtype_id, y0.data_ptr(), n, S, p.data_ptr(), p.shape[1],
+                plan.p_stride, float(self.t_span[0]), st64.data_ptr(), n_save, float(self.rtol),
+                float(self.atol), self.max_steps, flags, thr, steady_idx,
                 _lib.ptr(traj), stats.data_ptr(), final.data_ptr(), status.data_ptr(),
                 nsteps.data_ptr(), ws.data_ptr(), ws.numel(), self.max_lanes, s.cuda_stream)
         _lib.check(rc, "bh_solve_batch")
         for t in (y0, p, st64, ws):
             t.record_stream(s)
+        save_ts = plan.save_ts
         return FusedResult(t=save_ts, y=traj, final=final,
                            stats=stats if steady_idx < n_save else None, status=status,
                            nsteps=nsteps, steady_idx=int(steady_idx))

@@ -198,3 +198,37 @@ def test_contract_errors(cuda_device):
         HipDopri5Solver(t_span=(0, 1), t_steps=10, t_eval=(0.5, 1.5), device="cuda", cache_dir=None)
     c = HipDopri5Solver(t_span=(0, 2), t_eval=(1, 2), device="cuda", cache_dir=None).clone(device="cpu")
     assert c.t_eval == (1, 2) and c.device.type == "cpu"
+
+
+def test_graph_replay_matches_eager(cuda_device):
+    """The captured integration replayed several times, with eager work and
+    synchronisation between replays, equals the eager launch every time."""
+    from pybasin_amd.graphs import CapturedIntegration
+
+    ode = DuffingODE(DUFF)
+    solver = HipDopri5Solver(t_span=(0, 100), t_steps=200, t_eval=(50.0, 100.0), device="cuda",
+                             cache_dir=None)
+    ya = torch.tensor(O.sample_uniform(3000, [-1.0, -0.5], [1.0, 1.0], seed=21)).to(cuda_device)
+    yb = torch.tensor(O.sample_uniform(3000, [-1.0, -0.5], [1.0, 1.0], seed=22)).to(cuda_device)
+    cap = CapturedIntegration(solver, ode, ya, time_steady=60.0)
+    for y in (ya, yb, ya, yb):
+        eager = solver.integrate_fused(ode, y, time_steady=60.0)
+        torch.cuda.synchronize()
+        _ = torch.randn(1000, device=cuda_device).sum()  # unrelated eager work between replays
+        got = cap.replay(y)
+        torch.cuda.synchronize()
+        assert torch.equal(got.stats, eager.stats)
+        assert torch.equal(got.final, eager.final) and torch.equal(got.nsteps, eager.nsteps)
+
+
+def test_autotuner_picks_and_caches_a_lane_cap(cuda_device, tmp_path, monkeypatch):
+    from pybasin_amd import autotune
+
+    monkeypatch.setenv("PYBASIN_AMD_TUNE_CACHE", str(tmp_path / "tune.json"))
+    ode = DuffingODE(DUFF)
+    solver = HipDopri5Solver(t_span=(0, 1000), t_steps=10, device="cuda", cache_dir=None)
+    y0 = torch.tensor(O.sample_uniform(20000, [-1.0, -0.5], [1.0, 1.0], seed=2)).to(cuda_device)
+    best = autotune.tune_lane_cap(solver, ode, y0, probe_span=20.0)
+    assert best >= 0
+    assert (tmp_path / "tune.json").exists()
+    assert autotune.tune_lane_cap(solver, ode, y0, probe_span=20.0) == best  # cached
