@@ -23,6 +23,7 @@ import numpy as np
 import torch
 
 from .features import HipFusedFeatureExtractor, Solution, StatFeatureExtractor
+from . import _lib
 from .solvers.hip_dopri5 import FUSED_RESULTS, functor_id
 
 
@@ -97,14 +98,38 @@ class BSResult:
     feature_names: list[str]
     y0: torch.Tensor
     extra: dict = field(default_factory=dict)
+    study_label: dict = field(default_factory=lambda: {"baseline": True})
+
+
+def build_params_grid(ode_system, study: list[dict]) -> torch.Tensor:
+    """(P, n_params) float64 grid: the system's params in declaration order with
+    each study entry's overrides (basin_stability_estimator.py:469-486)."""
+    names = list(dict(ode_system.params).keys())
+    base = [float(v) for v in dict(ode_system.params).values()]
+    rows = []
+    for entry in study:
+        row = list(base)
+        for k, v in entry.items():
+            row[names.index(k)] = float(v)
+        rows.append(row)
+    return torch.tensor(rows, dtype=torch.float64)
 
 
 class BasinStabilityEstimator:
-    """integrate -> fused features -> 1-NN template labels -> basin stability."""
+    """integrate -> fused features -> 1-NN template labels -> basin stability.
+
+    The template ICs are integrated in the same launch as the sample, with the
+    system's own parameters (TemplateIntegrator integrates them once with its
+    fixed ode_params, template_integrator.py:59-105).  Unbounded trajectories
+    (stopped at the divergence threshold with Inf fill, or Inf in a kept
+    trajectory: _detect_unbounded_trajectories, basin_stability_estimator.py:519-527)
+    are labelled "unbounded" and left out of classification
+    (_process_study_label, 365-461).
+    """
 
     def __init__(self, ode_system, sampler: UniformRandomSampler, n: int, solver,
-                 feature_extractor: StatFeatureExtractor, template_y0, template_labels,
-                 keep_trajectories: bool = False):
+                 feature_extractor, template_y0, template_labels,
+                 keep_trajectories: bool = False, detect_unbounded: bool = True):
         self.ode_system = ode_system
         self.sampler = sampler
         self.n = int(n)
@@ -113,33 +138,85 @@ class BasinStabilityEstimator:
         self.template_y0 = torch.as_tensor(template_y0, dtype=torch.float32)
         self.classifier = KNNTemplateClassifier(template_labels)
         self.keep_trajectories = keep_trajectories
+        self.detect_unbounded = detect_unbounded
 
     def run(self, y0: torch.Tensor | None = None) -> BSResult:
+        return self._run(y0, None, [{"baseline": True}])[0]
+
+    def run_parameter_study(self, study: list[dict], y0: torch.Tensor | None = None
+                            ) -> list[BSResult]:
+        """One result per study entry ({param name: value}), all B x P
+        trajectories integrated in one launch, IC-major (row ic * P + p)."""
+        grid = build_params_grid(self.ode_system, study)
+        return self._run(y0, grid, list(study))
+
+    # ------------------------------------------------------------------ #
+    def _features(self, fx, res=None, solution=None) -> torch.Tensor:
+        if res is not None and hasattr(fx, "extract_from_result"):
+            return fx.extract_from_result(res)
+        return fx.extract_features(solution)
+
+    def _run(self, y0, grid, study_labels) -> list[BSResult]:
         if y0 is None:
             y0 = self.sampler.sample(self.n)
-        n = y0.shape[0]
+        B = y0.shape[0]
+        P = 1 if grid is None else grid.shape[0]
         dev = self.solver.device
-        batch = torch.cat([y0.to(dev), self.template_y0.to(dev, y0.dtype)])
+        main = y0.to(dev)
+        if P > 1:
+            main = torch.repeat_interleave(main, P, dim=0)
+        tmpl = self.template_y0.to(dev, y0.dtype)
+        batch = torch.cat([main, tmpl])
+        params = None
+        if grid is not None:
+            base = build_params_grid(self.ode_system, [{}])
+            params = torch.cat([grid.repeat(B, 1), base.expand(len(tmpl), -1)]).to(dev)
         fx = self.feature_extractor
+        extra: dict = {}
+        unbounded = None
         if dev.type == "cuda" and functor_id(self.ode_system) is not None:
             save_ts, batch_p = self.solver._prepare_tensors(batch)
-            steady = fx.time_steady
-            res = self.solver.run_fused(self.ode_system, batch_p, save_ts, None,
+            res = self.solver.run_fused(self.ode_system, batch_p, save_ts, params,
                                         keep_trajectories=self.keep_trajectories,
-                                        time_steady=steady)
-            if isinstance(fx, HipFusedFeatureExtractor):
-                feats = fx.extract_from_result(res)
+                                        time_steady=getattr(fx, "time_steady", None))
+            if res.y is not None:
+                FUSED_RESULTS[res.y] = res
+            fused_ok = res.stats is not None and (
+                isinstance(fx, HipFusedFeatureExtractor) or hasattr(fx, "extract_from_result"))
+            if fused_ok:
+                feats = self._features(fx, res=res)
             else:
                 if res.y is None:
                     raise ValueError("a non-fused extractor needs keep_trajectories=True")
-                FUSED_RESULTS[res.y] = res
                 feats = fx.extract_features(Solution(batch, save_ts, res.y))
+            if self.detect_unbounded and self.solver.stop_threshold is not None \
+                    and self.solver.inf_fill:
+                unbounded = (res.status == _lib.BH_LANE_DIVERGED).cpu()
             extra = {"status": res.status, "nsteps": res.nsteps}
         else:
-            t, y = self.solver.integrate(self.ode_system, batch)
+            t, y = self.solver.integrate(self.ode_system, batch, None) if params is None else \
+                self._integrate_rows(batch, params)
             feats = fx.extract_features(Solution(batch, t, y))
-            extra = {}
-        self.classifier.fit(feats[n:])
-        labels = self.classifier.predict(feats[:n])
-        bs = compute_bs(labels)
-        return BSResult(bs, get_errors(bs, n), labels, feats[:n], fx.feature_names, y0, extra)
+            if self.detect_unbounded:
+                unbounded = torch.isinf(y).any(dim=(0, 2)).cpu()
+        feats = feats.cpu() if feats.device.type != "cpu" else feats
+        n_main = B * P
+        self.classifier.fit(feats[n_main:])
+        results = []
+        for p, label in enumerate(study_labels):
+            f_p = feats[:n_main][p::P]
+            ub = unbounded[:n_main][p::P] if unbounded is not None else torch.zeros(B, dtype=torch.bool)
+            labels = np.empty(B, dtype=object)
+            labels[ub.numpy()] = "unbounded"
+            if int(ub.sum()) < B:
+                labels[(~ub).numpy()] = self.classifier.predict(f_p[~ub])
+            bs = compute_bs(labels)
+            names = list(getattr(fx, "feature_names", []))
+            results.append(BSResult(bs, get_errors(bs, B), labels, f_p, names, y0,
+                                    dict(extra, group=p), study_label=label))
+        return results
+
+    def _integrate_rows(self, batch, params):
+        """CPU path with per-row parameters (rows already expanded)."""
+        save_ts, b = self.solver._prepare_tensors(batch)
+        return save_ts, self.solver._integrate(self.ode_system, b, save_ts, params)[1]

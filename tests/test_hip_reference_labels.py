@@ -107,3 +107,88 @@ def test_lorenz_labels(cuda_device, fixtures):
     assert ((lab == "unbounded") == (np.asarray(fx["labels"]) == "unbounded")).mean() >= 0.999
     assert agree >= 0.99
     assert worst <= 2.0
+
+
+def _mcc(got, want):
+    """Matthews correlation over the label set (multi-class, Gorodkin's form)."""
+    labels = sorted(set(want) | set(got))
+    idx = {l: i for i, l in enumerate(labels)}
+    C = np.zeros((len(labels), len(labels)))
+    for g, w in zip(got, want):
+        C[idx[w], idx[g]] += 1
+    t, p, c, s = C.sum(1), C.sum(0), np.trace(C), C.sum()
+    den = math.sqrt((s * s - (p * p).sum()) * (s * s - (t * t).sum()))
+    return 1.0 if den == 0 else float((c * s - (t * p).sum()) / den)
+
+
+def test_pendulum_torque_sweep_parameter_study(cuda_device, fixtures):
+    """Pendulum case 2: 20 torques x 2000 seeded ICs in one launch through
+    run_parameter_study, against the reference's per-T label files
+    (tests/integration/pendulum/ground_truths/case2); pass criterion of the
+    reference's run_parameter_study_test (tests/integration/test_helpers.py:437-530):
+    MCC >= 0.95 per parameter value."""
+    from pybasin_amd.estimator import BasinStabilityEstimator
+    from pybasin_amd.features import HipFusedFeatureExtractor
+
+    fx = fixtures["pendulum_case2_T_sweep"]
+    y0 = torch.tensor(O.sample_uniform(fx["n"], fx["min_limits"], fx["max_limits"], fx["seed"]))
+    solver = HipDopri5Solver(t_span=(0, 1000), t_steps=1000, device="cuda", cache_dir=None)
+    ext = HipFusedFeatureExtractor(time_steady=950.0, features=None,
+                                   features_per_state={1: {"log_delta": None}})
+    bse = BasinStabilityEstimator(PendulumODE({"alpha": 0.1, "T": 0.5, "K": 1.0}), None, fx["n"],
+                                  solver, ext, template_y0=[[0.4, 0.0], [2.7, 0.0]],
+                                  template_labels=["FP", "LC"])
+    results = bse.run_parameter_study([{"T": e["T"]} for e in fx["sweep"]], y0)
+    worst_mcc, worst_agree = 1.0, 1.0
+    for r, e in zip(results, fx["sweep"]):
+        agree = float((np.asarray(r.labels) == np.asarray(e["labels"])).mean())
+        mcc = _mcc(list(r.labels), e["labels"])
+        print(f"[result] T={e['T']:.2f} agreement {agree:.4f} MCC {mcc:.4f} BS {r.basin_stability}",
+              flush=True)
+        worst_mcc, worst_agree = min(worst_mcc, mcc), min(worst_agree, agree)
+        assert mcc >= 0.95
+    print(f"[result] torque sweep: worst agreement {worst_agree:.4f}, worst MCC {worst_mcc:.4f}")
+
+
+def test_parameter_study_group_equals_single_launch(cuda_device):
+    """Per-lane integration is independent of the batch: group p of a study
+    equals a launch with that parameter set, bit for bit."""
+    solver = HipDopri5Solver(t_span=(0, 50), t_steps=101, t_eval=(25.0, 50.0), device="cuda",
+                             cache_dir=None)
+    y0 = torch.tensor(O.sample_uniform(500, [-2.6, -10.0], [3.7, 10.0], 5)).to(cuda_device)
+    grid = torch.tensor([[0.1, 0.3, 1.0], [0.1, 0.7, 1.0]], dtype=torch.float64)
+    ode = PendulumODE({"alpha": 0.1, "T": 0.5, "K": 1.0})
+    both = solver.integrate_fused(ode, y0, grid, time_steady=30.0)
+    for p in range(2):
+        one = solver.integrate_fused(PendulumODE({"alpha": 0.1, "T": float(grid[p, 1]), "K": 1.0}),
+                                     y0, time_steady=30.0)
+        assert torch.equal(both.final[p::2], one.final)
+        assert torch.equal(both.stats[p::2], one.stats)
+
+
+def test_case_extractors_fused_equal_trajectory(cuda_device):
+    """The case-study extractors give the same output from the kernel's fused
+    statistics as from the kept trajectory of the same launch."""
+    from pybasin_amd.case_features import (DuffingFeatureExtractor, LorenzFeatureExtractor,
+                                           PendulumFeatureExtractor)
+    from pybasin_amd.features import Solution
+    from pybasin_amd.solvers.hip_dopri5 import FUSED_RESULTS
+
+    cases = [
+        (PendulumODE({"alpha": 0.1, "T": 0.5, "K": 1.0}), PendulumFeatureExtractor(time_steady=950.0),
+         O.sample_uniform(400, [-2.6, -10.0], [3.7, 10.0], 1), {}),
+        (DuffingODE({"delta": 0.08, "k3": 1.0, "A": 0.2}), DuffingFeatureExtractor(time_steady=900.0),
+         O.sample_uniform(400, [-1.0, -0.5], [1.0, 1.0], 2), {}),
+        (LorenzODE({"sigma": 0.12, "r": 0.0, "b": -0.6}), LorenzFeatureExtractor(time_steady=900.0),
+         O.sample_uniform(400, [-10.0, -20.0, 0.0], [10.0, 20.0, 0.0], 3), {"stop_threshold": 200.0}),
+    ]
+    for ode, ext, ics, kw in cases:
+        solver = HipDopri5Solver(t_span=(0, 1000), t_steps=500, t_eval=(850.0, 1000.0),
+                                 device="cuda", cache_dir=None, **kw)
+        y0 = torch.tensor(ics).to(cuda_device)
+        res = solver.integrate_fused(ode, y0, time_steady=ext.time_steady, keep_trajectories=True)
+        fused = ext.extract_from_result(res)
+        FUSED_RESULTS.pop(res.y, None)
+        direct = ext.extract_features(Solution(y0, res.t, res.y))
+        assert fused.shape == direct.shape
+        torch.testing.assert_close(fused.cpu(), direct.cpu(), rtol=1e-5, atol=1e-6)

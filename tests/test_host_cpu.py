@@ -144,3 +144,77 @@ def test_estimator_on_cpu_with_templates():
 def test_knn_template_classifier():
     knn = KNNTemplateClassifier(["a", "b"]).fit(torch.tensor([[0.0], [1.0]]))
     assert list(knn.predict(torch.tensor([[0.2], [0.9], [-5.0]]))) == ["a", "b", "a"]
+
+
+def test_case_study_extractors_follow_reference_columns():
+    """Pendulum / Duffing / Lorenz extractors (case_studies/*_feature_extractor.py)
+    on synthetic windows, against direct torch reductions of the window."""
+    from pybasin_amd.case_features import (DuffingFeatureExtractor, LorenzFeatureExtractor,
+                                           PendulumFeatureExtractor)
+
+    t = torch.linspace(0, 10, 101)
+    # pendulum: omega constant (FP) / oscillating (LC)
+    om_fp = torch.full_like(t, 0.3)
+    om_lc = 1.0 + torch.sin(3 * t)
+    y = torch.stack([torch.stack([t, om_fp], -1), torch.stack([t, om_lc], -1)], 1)
+    sol = Solution(y[0], t, y)
+    f = PendulumFeatureExtractor(time_steady=5.0).extract_features(sol)
+    assert f.dtype == torch.float64 and f.tolist() == [[1.0, 0.0], [0.0, 1.0]]
+    # Duffing: max and ddof-1 std of x over the window
+    x = torch.stack([torch.sin(t) * 2, torch.cos(0.5 * t)], 1)
+    yd = torch.stack([x, torch.zeros_like(x)], -1)
+    fd = DuffingFeatureExtractor(time_steady=5.0).extract_features(Solution(yd[0], t, yd))
+    w = yd[50:, :, 0]
+    assert torch.allclose(fd[:, 0], w.max(0).values) and fd.dtype == torch.float32
+    assert torch.allclose(fd[:, 1], torch.std(w, dim=0), rtol=1e-5)
+    # Lorenz: sign of mean x, [0, 0] once any state exceeds 195 in the window
+    xl = torch.stack([torch.full_like(t, 2.0), torch.full_like(t, -2.0), torch.full_like(t, 2.0)], 1)
+    zl = torch.zeros_like(xl)
+    zl[80, 2] = -196.0
+    yl = torch.stack([xl, zl, zl], -1)
+    fl = LorenzFeatureExtractor(time_steady=5.0).extract_features(Solution(yl[0], t, yl))
+    assert fl.tolist() == [[1.0, 0.0], [0.0, 1.0], [0.0, 0.0]]
+
+
+def test_parameter_study_groups_match_single_runs_on_cpu():
+    """run_parameter_study integrates B x P IC-major rows in one call; every
+    group equals a separate run with that parameter set."""
+    from pybasin_amd.estimator import build_params_grid
+
+    ode = ExponentialDecayODE({"decay": -1.0})
+    g = build_params_grid(ode, [{"decay": -0.5}, {}, {"decay": 0.5}])
+    assert g.tolist() == [[-0.5], [-1.0], [0.5]]
+    solver = TorchDopri5Solver(t_span=(0, 4), t_steps=9, device="cpu", cache_dir=None)
+    fx = StatFeatureExtractor(time_steady=2.0, features=None,
+                              features_per_state={0: {"mean": None}})
+    y0 = torch.tensor([[0.9], [-0.9], [0.4], [-0.2]])
+    bse = BasinStabilityEstimator(ode, None, 4, solver, fx,
+                                  template_y0=[[0.5], [-0.5]], template_labels=["pos", "neg"])
+    study = bse.run_parameter_study([{"decay": -0.5}, {"decay": 0.5}], y0)
+    assert [r.study_label for r in study] == [{"decay": -0.5}, {"decay": 0.5}]
+    for r, d in zip(study, (-0.5, 0.5)):
+        single = BasinStabilityEstimator(ExponentialDecayODE({"decay": d}), None, 4, solver, fx,
+                                         template_y0=[[0.5], [-0.5]],
+                                         template_labels=["pos", "neg"])
+        # templates use the base parameters in the study, so compare features only
+        assert torch.allclose(r.features, single.run(y0).features, rtol=1e-6)
+        assert list(r.labels) == ["pos", "neg", "pos", "neg"]
+
+
+def test_unbounded_rows_are_labelled_and_excluded_on_cpu():
+    """Inf in a trajectory marks the row unbounded (basin_stability_estimator.py:519-527);
+    unbounded rows are left out of classification and counted in BS."""
+    class InfAfterEscape(TorchDopri5Solver):
+        def _integrate(self, ode_system, y0, save_ts, params=None):
+            t, y = super()._integrate(ode_system, y0, save_ts, params)
+            y = y.clone()
+            y[2:, y0[:, 0].abs() > 5] = float("inf")  # Event semantics: Inf after the stop
+            return t, y
+
+    solver = InfAfterEscape(t_span=(0, 1), t_steps=5, device="cpu", cache_dir=None)
+    fx = StatFeatureExtractor(time_steady=0.5, features=None, features_per_state={0: {"mean": None}})
+    bse = BasinStabilityEstimator(ExponentialDecayODE({"decay": -1.0}), None, 4, solver, fx,
+                                  template_y0=[[0.5], [-0.5]], template_labels=["pos", "neg"])
+    res = bse.run(torch.tensor([[0.9], [9.0], [-0.4], [-9.0]]))
+    assert list(res.labels) == ["pos", "unbounded", "neg", "unbounded"]
+    assert res.basin_stability == {"neg": 0.25, "pos": 0.25, "unbounded": 0.5}
