@@ -92,6 +92,32 @@ int32_t bh_solve_batch(int32_t system_id, int32_t dtype, const void *y0s, int64_
                        int64_t workspace_bytes, int32_t max_lanes, void *stream);
 
 /*
+ * bh_solve_batch_peaks -- bh_solve_batch plus the orbit peaks of every state
+ * over the feature window [steady_idx, n_save): strict local maxima with
+ * support 1 (a value greater than both neighbours; the first and last window
+ * points are never peaks), in time order.  Replaces extract_orbit_data /
+ * extract_peak_values on the saved trajectories (src/pybasin/utils.py:254-303,
+ * src/pybasin/ts_torch/calculators/torch_features_pattern.py:64-150) with
+ * peak_support = 1.
+ *
+ *   peak_cap         rows of out_peak_vals
+ *   out_peak_vals    (peak_cap, n, dim) in dtype: peak k of (ic, state) at
+ *                    [k][ic][state] for k < min(count, peak_cap); other
+ *                    entries are not written
+ *   out_peak_counts  (n, dim) int32: number of peaks (may exceed peak_cap;
+ *                    the caller re-runs with a larger cap to get them all)
+ */
+int32_t bh_solve_batch_peaks(int32_t system_id, int32_t dtype, const void *y0s, int64_t n,
+                             int32_t dim, const double *params, int32_t n_params,
+                             int64_t params_stride, double t0, const double *save_ts,
+                             int32_t n_save, double rtol, double atol, int64_t max_steps,
+                             uint32_t flags, double stop_threshold, int32_t steady_idx,
+                             void *out_traj, double *out_feats, void *out_final,
+                             int32_t *out_status, int64_t *out_nsteps, int32_t peak_cap,
+                             void *out_peak_vals, int32_t *out_peak_counts, void *workspace,
+                             int64_t workspace_bytes, int32_t max_lanes, void *stream);
+
+/*
  * bh_solve_rossler -- the 100-node Rossler network (BH_SYS_ROSSLER100, state
  * layout [x_0..x_99, y_0..y_99, z_0..z_99]), 16 trajectories per workgroup,
  * coupling K (L x) as a (16 x 100) . (100 x 100) FP32 matrix-core product per

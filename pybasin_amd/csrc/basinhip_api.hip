@@ -40,10 +40,10 @@ int32_t device_lanes(K kernel, int block, int64_t *lanes) {
     return 0;
 }
 
-template <class Sys, class T>
+template <class Sys, class T, bool kPeaks>
 int32_t launch_lane(const bh::LaneArgs &args, int32_t max_lanes, hipStream_t stream) {
     constexpr int block = 256;
-    auto kernel = bh::dopri5_lane_kernel<Sys, T>;
+    auto kernel = bh::dopri5_lane_kernel<Sys, T, kPeaks>;
     int64_t lanes = 0;
     int32_t rc = device_lanes(kernel, block, &lanes);
     if (rc) return rc;
@@ -65,8 +65,13 @@ int32_t launch_lane(const bh::LaneArgs &args, int32_t max_lanes, hipStream_t str
 template <class Sys>
 int32_t dispatch_dtype(int32_t dtype, const bh::LaneArgs &args, int32_t max_lanes,
                        hipStream_t s) {
-    if (dtype == BH_F32) return launch_lane<Sys, float>(args, max_lanes, s);
-    if (dtype == BH_F64) return launch_lane<Sys, double>(args, max_lanes, s);
+    const bool peaks = args.peak_counts != nullptr;
+    if (dtype == BH_F32)
+        return peaks ? launch_lane<Sys, float, true>(args, max_lanes, s)
+                     : launch_lane<Sys, float, false>(args, max_lanes, s);
+    if (dtype == BH_F64)
+        return peaks ? launch_lane<Sys, double, true>(args, max_lanes, s)
+                     : launch_lane<Sys, double, false>(args, max_lanes, s);
     return fail(-5, "dtype must be BH_F32 or BH_F64, got %d", dtype);
 }
 
@@ -93,21 +98,14 @@ int32_t check_system(int32_t sys, int32_t dim, int32_t n_params) {
     return 0;
 }
 
-}  // namespace
-
-extern "C" {
-
-int32_t bh_version(void) { return 10000; }
-
-const char *bh_last_error(void) { return g_err; }
-
-int32_t bh_solve_batch(int32_t system_id, int32_t dtype, const void *y0s, int64_t n, int32_t dim,
-                       const double *params, int32_t n_params, int64_t params_stride, double t0,
-                       const double *save_ts, int32_t n_save, double rtol, double atol,
-                       int64_t max_steps, uint32_t flags, double stop_threshold,
-                       int32_t steady_idx, void *out_traj, double *out_feats, void *out_final,
-                       int32_t *out_status, int64_t *out_nsteps, void *workspace,
-                       int64_t workspace_bytes, int32_t max_lanes, void *stream) {
+int32_t solve_batch_impl(int32_t system_id, int32_t dtype, const void *y0s, int64_t n, int32_t dim,
+                         const double *params, int32_t n_params, int64_t params_stride, double t0,
+                         const double *save_ts, int32_t n_save, double rtol, double atol,
+                         int64_t max_steps, uint32_t flags, double stop_threshold,
+                         int32_t steady_idx, void *out_traj, double *out_feats, void *out_final,
+                         int32_t *out_status, int64_t *out_nsteps, int32_t peak_cap,
+                         void *out_peak_vals, int32_t *out_peak_counts, void *workspace,
+                         int64_t workspace_bytes, int32_t max_lanes, void *stream) {
     g_err[0] = 0;
     int32_t rc = check_system(system_id, dim, n_params);
     if (rc) return rc;
@@ -146,6 +144,9 @@ int32_t bh_solve_batch(int32_t system_id, int32_t dtype, const void *y0s, int64_
     a.status = out_status;
     a.nsteps = out_nsteps;
     a.counter = reinterpret_cast<unsigned long long *>(workspace);
+    a.peak_vals = out_peak_vals;
+    a.peak_counts = out_peak_counts;
+    a.peak_cap = peak_cap;
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
 
     switch (system_id) {
@@ -158,6 +159,47 @@ int32_t bh_solve_batch(int32_t system_id, int32_t dtype, const void *y0s, int64_
     case BH_SYS_LORENZ: return dispatch_dtype<bh::Lorenz>(dtype, a, max_lanes, s);
     default: return fail(-1, "unknown system id %d", system_id);
     }
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t bh_version(void) { return 10000; }
+
+const char *bh_last_error(void) { return g_err; }
+
+int32_t bh_solve_batch(int32_t system_id, int32_t dtype, const void *y0s, int64_t n, int32_t dim,
+                       const double *params, int32_t n_params, int64_t params_stride, double t0,
+                       const double *save_ts, int32_t n_save, double rtol, double atol,
+                       int64_t max_steps, uint32_t flags, double stop_threshold,
+                       int32_t steady_idx, void *out_traj, double *out_feats, void *out_final,
+                       int32_t *out_status, int64_t *out_nsteps, void *workspace,
+                       int64_t workspace_bytes, int32_t max_lanes, void *stream) {
+    return solve_batch_impl(system_id, dtype, y0s, n, dim, params, n_params, params_stride, t0,
+                            save_ts, n_save, rtol, atol, max_steps, flags, stop_threshold,
+                            steady_idx, out_traj, out_feats, out_final, out_status, out_nsteps, 0,
+                            nullptr, nullptr, workspace, workspace_bytes, max_lanes, stream);
+}
+
+int32_t bh_solve_batch_peaks(int32_t system_id, int32_t dtype, const void *y0s, int64_t n,
+                             int32_t dim, const double *params, int32_t n_params,
+                             int64_t params_stride, double t0, const double *save_ts,
+                             int32_t n_save, double rtol, double atol, int64_t max_steps,
+                             uint32_t flags, double stop_threshold, int32_t steady_idx,
+                             void *out_traj, double *out_feats, void *out_final,
+                             int32_t *out_status, int64_t *out_nsteps, int32_t peak_cap,
+                             void *out_peak_vals, int32_t *out_peak_counts, void *workspace,
+                             int64_t workspace_bytes, int32_t max_lanes, void *stream) {
+    g_err[0] = 0;
+    if (!out_peak_counts) return fail(-5, "out_peak_counts is required");
+    if (peak_cap < 0) return fail(-5, "peak_cap must be >= 0, got %d", peak_cap);
+    if (peak_cap > 0 && !out_peak_vals) return fail(-5, "peak_cap > 0 needs out_peak_vals");
+    return solve_batch_impl(system_id, dtype, y0s, n, dim, params, n_params, params_stride, t0,
+                            save_ts, n_save, rtol, atol, max_steps, flags, stop_threshold,
+                            steady_idx, out_traj, out_feats, out_final, out_status, out_nsteps,
+                            peak_cap, out_peak_vals, out_peak_counts, workspace, workspace_bytes,
+                            max_lanes, stream);
 }
 
 int32_t bh_solve_rossler(int32_t dtype, const void *y0s, int64_t n, const double *params,

@@ -28,6 +28,12 @@
 // max (src/pybasin/feature_extractors/feature_extractor.py:102-130,
 // src/pybasin/ts_torch/calculators/torch_features_statistical.py:39-99).
 //
+// Orbit peaks (optional, kPeaks): strict local maxima with support 1 of each
+// state over the same window, i.e. the find_peak_mask / extract_peak_values
+// pair (src/pybasin/ts_torch/calculators/torch_features_pattern.py:64-150)
+// behind extract_orbit_data (src/pybasin/utils.py:254-303), found from the
+// last two saved values as each new one arrives.
+//
 // Work distribution: a resident set of lanes; when a lane's trajectory ends
 // (all saves emitted, divergence stop, or failure) it takes the next index
 // from a device counter, so uneven lengths do not idle the lane.
@@ -59,6 +65,10 @@ struct LaneArgs {
     int32_t *status;
     int64_t *nsteps;
     unsigned long long *counter;
+    // orbit peaks over the feature window (kernels instantiated with kPeaks)
+    void *peak_vals;       // (peak_cap, n, S), values of strict local maxima in time order
+    int32_t *peak_counts;  // (n, S), number of maxima (may exceed peak_cap)
+    int32_t peak_cap;
 };
 
 // Dormand-Prince 5(4) coefficients (Dormand & Prince 1980; Shampine's error
@@ -122,9 +132,18 @@ __device__ __forceinline__ double next_dt_sq(double dt, double r2) {
     return dt * (double)f;
 }
 
-template <class Sys, class T>
+// previous two window values and the maxima count per state
+template <int S, class T, bool kOn> struct PeakState {
+    T p1[S], p2[S];
+    int32_t count[S];
+};
+template <int S, class T> struct PeakState<S, T, false> {};
+
+template <class Sys, class T, bool kPeaks = false>
 struct Lane {
     static constexpr int S = Sys::S;
+    static constexpr bool kP = kPeaks;
+    PeakState<Sys::S, T, kPeaks> pk;
     T p[Sys::NP];
     T y[S], k1[S];
     double t, dt;
@@ -140,8 +159,8 @@ struct Lane {
     T last[S];
 };
 
-template <class Sys, class T>
-__device__ __forceinline__ void emit(const LaneArgs &a, Lane<Sys, T> &L, int64_t ic, int32_t si,
+template <class Sys, class T, class LaneT>
+__device__ __forceinline__ void emit(const LaneArgs &a, LaneT &L, int64_t ic, int32_t si,
                                      const T (&v)[Sys::S]) {
     constexpr int S = Sys::S;
     if (a.traj) {
@@ -166,14 +185,34 @@ __device__ __forceinline__ void emit(const LaneArgs &a, Lane<Sys, T> &L, int64_t
             L.mn[i] = fmin(L.mn[i], v[i]);
             L.mx[i] = fmax(L.mx[i], v[i]);
         }
+        if constexpr (LaneT::kP) {
+            // window position L.cnt; the value before it is a maximum when it
+            // exceeds both neighbours (NaN compares false: no maximum)
+            if (L.cnt >= 2) {
+#pragma unroll
+                for (int i = 0; i < S; i++) {
+                    if (L.pk.p1[i] > L.pk.p2[i] && L.pk.p1[i] > v[i]) {
+                        if (L.pk.count[i] < a.peak_cap)
+                            reinterpret_cast<T *>(a.peak_vals)[((int64_t)L.pk.count[i] * a.n + ic) * S + i] =
+                                L.pk.p1[i];
+                        L.pk.count[i]++;
+                    }
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < S; i++) {
+                L.pk.p2[i] = L.pk.p1[i];
+                L.pk.p1[i] = v[i];
+            }
+        }
         L.cnt++;
     }
 #pragma unroll
     for (int i = 0; i < S; i++) L.last[i] = v[i];
 }
 
-template <class Sys, class T>
-__device__ __forceinline__ void lane_init(const LaneArgs &a, Lane<Sys, T> &L, int64_t ic) {
+template <class Sys, class T, class LaneT>
+__device__ __forceinline__ void lane_init(const LaneArgs &a, LaneT &L, int64_t ic) {
     constexpr int S = Sys::S;
     const T *y0 = reinterpret_cast<const T *>(a.y0) + ic * S;
     const double *pp = a.params + ic * a.p_stride;
@@ -185,6 +224,10 @@ __device__ __forceinline__ void lane_init(const LaneArgs &a, Lane<Sys, T> &L, in
         L.s1[i] = 0.0;
         L.s2[i] = 0.0;
         L.shift[i] = 0.0;
+    }
+    if constexpr (LaneT::kP) {
+#pragma unroll
+        for (int i = 0; i < S; i++) L.pk.count[i] = 0;
     }
     L.t = a.t0;
     L.steps = 0;
@@ -234,9 +277,8 @@ __device__ __forceinline__ void lane_init(const LaneArgs &a, Lane<Sys, T> &L, in
     if (L.si >= a.n_save) L.done = true;
 }
 
-template <class Sys, class T>
-__device__ __forceinline__ void lane_fill_rest(const LaneArgs &a, Lane<Sys, T> &L, int64_t ic,
-                                               bool inf) {
+template <class Sys, class T, class LaneT>
+__device__ __forceinline__ void lane_fill_rest(const LaneArgs &a, LaneT &L, int64_t ic, bool inf) {
     constexpr int S = Sys::S;
     T v[S];
 #pragma unroll
@@ -245,8 +287,8 @@ __device__ __forceinline__ void lane_fill_rest(const LaneArgs &a, Lane<Sys, T> &
 }
 
 // One attempted step.  Sets L.done when the lane has finished.
-template <class Sys, class T>
-__device__ __forceinline__ void lane_step(const LaneArgs &a, Lane<Sys, T> &L, int64_t ic) {
+template <class Sys, class T, class LaneT>
+__device__ __forceinline__ void lane_step(const LaneArgs &a, LaneT &L, int64_t ic) {
     constexpr int S = Sys::S;
     const double tA = L.t, tB = L.t + L.dt;
     const T tf = (T)tA, h = (T)L.dt, t1p = prev_value<T>((T)tB);
@@ -366,8 +408,8 @@ __device__ __forceinline__ void lane_step(const LaneArgs &a, Lane<Sys, T> &L, in
     }
 }
 
-template <class Sys, class T>
-__device__ __forceinline__ void lane_finish(const LaneArgs &a, Lane<Sys, T> &L, int64_t ic) {
+template <class Sys, class T, class LaneT>
+__device__ __forceinline__ void lane_finish(const LaneArgs &a, LaneT &L, int64_t ic) {
     constexpr int S = Sys::S;
     if (a.feats) {
         double *o = a.feats + ic * S * BH_NFEAT;
@@ -395,13 +437,17 @@ __device__ __forceinline__ void lane_finish(const LaneArgs &a, Lane<Sys, T> &L, 
     }
     if (a.status) a.status[ic] = L.status;
     if (a.nsteps) a.nsteps[ic] = L.steps;
+    if constexpr (LaneT::kP) {
+#pragma unroll
+        for (int i = 0; i < S; i++) a.peak_counts[ic * S + i] = L.pk.count[i];
+    }
 }
 
-template <class Sys, class T>
+template <class Sys, class T, bool kPeaks>
 __global__ void __launch_bounds__(256) dopri5_lane_kernel(LaneArgs a) {
     int64_t ic = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (ic >= a.n) return;
-    Lane<Sys, T> L;
+    Lane<Sys, T, kPeaks> L;
     lane_init<Sys, T>(a, L, ic);
     for (;;) {
         if (!L.done) lane_step<Sys, T>(a, L, ic);  // not done unless init emitted every save

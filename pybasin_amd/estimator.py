@@ -129,7 +129,8 @@ class BasinStabilityEstimator:
 
     def __init__(self, ode_system, sampler: UniformRandomSampler, n: int, solver,
                  feature_extractor, template_y0, template_labels,
-                 keep_trajectories: bool = False, detect_unbounded: bool = True):
+                 keep_trajectories: bool = False, detect_unbounded: bool = True,
+                 compute_orbit_data: list[int] | bool = False):
         self.ode_system = ode_system
         self.sampler = sampler
         self.n = int(n)
@@ -139,6 +140,7 @@ class BasinStabilityEstimator:
         self.classifier = KNNTemplateClassifier(template_labels)
         self.keep_trajectories = keep_trajectories
         self.detect_unbounded = detect_unbounded
+        self.compute_orbit_data = compute_orbit_data
 
     def run(self, y0: torch.Tensor | None = None) -> BSResult:
         return self._run(y0, None, [{"baseline": True}])[0]
@@ -176,9 +178,14 @@ class BasinStabilityEstimator:
         unbounded = None
         if dev.type == "cuda" and functor_id(self.ode_system) is not None:
             save_ts, batch_p = self.solver._prepare_tensors(batch)
-            res = self.solver.run_fused(self.ode_system, batch_p, save_ts, params,
-                                        keep_trajectories=self.keep_trajectories,
-                                        time_steady=getattr(fx, "time_steady", None))
+            steady = getattr(fx, "time_steady", None)
+            if self.compute_orbit_data:
+                res, extra_orbit = self._orbit_rows(batch_p, save_ts, params, steady)
+            else:
+                res = self.solver.run_fused(self.ode_system, batch_p, save_ts, params,
+                                            keep_trajectories=self.keep_trajectories,
+                                            time_steady=steady)
+                extra_orbit = None
             if res.y is not None:
                 FUSED_RESULTS[res.y] = res
             fused_ok = res.stats is not None and (
@@ -199,6 +206,12 @@ class BasinStabilityEstimator:
             feats = fx.extract_features(Solution(batch, t, y))
             if self.detect_unbounded:
                 unbounded = torch.isinf(y).any(dim=(0, 2)).cpu()
+            extra_orbit = None
+            if self.compute_orbit_data:
+                from .orbit import extract_orbit_data
+
+                extra_orbit = extract_orbit_data(t, y, self._orbit_dof(batch.shape[1]),
+                                                 time_steady=getattr(fx, "time_steady", None))
         feats = feats.cpu() if feats.device.type != "cpu" else feats
         n_main = B * P
         self.classifier.fit(feats[n_main:])
@@ -212,9 +225,35 @@ class BasinStabilityEstimator:
                 labels[(~ub).numpy()] = self.classifier.predict(f_p[~ub])
             bs = compute_bs(labels)
             names = list(getattr(fx, "feature_names", []))
-            results.append(BSResult(bs, get_errors(bs, B), labels, f_p, names, y0,
-                                    dict(extra, group=p), study_label=label))
+            ex = dict(extra, group=p)
+            if extra_orbit is not None:
+                from .orbit import OrbitData
+
+                ex["orbit_data"] = OrbitData(extra_orbit.peak_values[:, :n_main][:, p::P],
+                                             extra_orbit.peak_counts[:n_main][p::P],
+                                             extra_orbit.dof_indices, extra_orbit.time_steady)
+            results.append(BSResult(bs, get_errors(bs, B), labels, f_p, names, y0, ex,
+                                    study_label=label))
         return results
+
+    def _orbit_dof(self, S: int) -> list[int]:
+        c = self.compute_orbit_data
+        return list(range(S)) if c is True else list(c)
+
+    def _orbit_rows(self, batch_p, save_ts, params, steady):
+        """Fused launch with peaks for already expanded per-row parameters."""
+        from .orbit import orbit_data_from_result
+
+        plan = self.solver.plan_fused(self.ode_system, batch_p, save_ts, params,
+                                      time_steady=steady)
+        cap = max(0, min(64, (len(save_ts) - plan.steady_idx - 1) // 2))
+        res = self.solver.launch_fused(plan, keep_trajectories=self.keep_trajectories,
+                                       peak_cap=cap)
+        need = int(res.peak_counts.max()) if res.peak_counts.numel() else 0
+        if need > cap:
+            res = self.solver.launch_fused(plan, keep_trajectories=self.keep_trajectories,
+                                           peak_cap=need)
+        return res, orbit_data_from_result(res, self._orbit_dof(batch_p.shape[1]), steady)
 
     def _integrate_rows(self, batch, params):
         """CPU path with per-row parameters (rows already expanded)."""

@@ -94,6 +94,7 @@ class FusedResult:
     y       (n_save, n, S) trajectories, or None when not kept
     final   (n, S) state at the last save time
     stats   (n, S, 4) float64: mean, std (ddof 0), min, max over the window
+    peak_values / peak_counts  orbit peaks over the window when requested
     status  (n,) int32 lane status (BH_LANE_*)
     nsteps  (n,) int64 attempted steps per trajectory
     steady_idx  first save index of the feature window
@@ -106,6 +107,8 @@ class FusedResult:
     status: torch.Tensor
     nsteps: torch.Tensor
     steady_idx: int
+    peak_values: torch.Tensor | None = None   # (peak_cap, n, S), see bh_solve_batch_peaks
+    peak_counts: torch.Tensor | None = None   # (n, S) int32
 
 
 def steady_index(save_ts: torch.Tensor, time_steady: float | None) -> int:
@@ -226,8 +229,11 @@ class HipDopri5Solver(Solver):
                         st64=st64, steady_idx=int(steady_idx))
 
     def launch_fused(self, plan: "LanePlan", *, keep_trajectories: bool,
-                     stream: torch.cuda.Stream | None = None) -> FusedResult:
-        """Allocate the outputs and launch bh_solve_batch for a prepared plan."""
+                     stream: torch.cuda.Stream | None = None,
+                     peak_cap: int | None = None) -> FusedResult:
+        """Allocate the outputs and launch bh_solve_batch for a prepared plan
+        (bh_solve_batch_peaks with a peak buffer of peak_cap rows when
+        peak_cap is given)."""
         dev = self.device
         y0, p, st64 = plan.y0, plan.p, plan.st64
         dtype_id = _lib.BH_F32 if y0.dtype == torch.float32 else _lib.BH_F64
@@ -248,17 +254,28 @@ class HipDopri5Solver(Solver):
             if self.inf_fill:
                 flags |= _lib.BH_FLAG_INF_FILL
         s = stream if stream is not None else torch.cuda.current_stream(dev)
+        common = (plan.sys_id, dtype_id, y0.data_ptr(), n, S, p.data_ptr(), p.shape[1],
+                  plan.p_stride, float(self.t_span[0]), st64.data_ptr(), n_save,
+                  float(self.rtol), float(self.atol), self.max_steps, flags, thr, steady_idx,
+                  _lib.ptr(traj), stats.data_ptr(), final.data_ptr(), status.data_ptr(),
+                  nsteps.data_ptr())
+        pk_vals = pk_counts = None
         with torch.cuda.device(dev):
-            rc = _lib.lib().bh_solve_batch(
-                plan.sys_id, dtype_id, y0.data_ptr(), n, S, p.data_ptr(), p.shape[1],
-                plan.p_stride, float(self.t_span[0]), st64.data_ptr(), n_save, float(self.rtol),
-                float(self.atol), self.max_steps, flags, thr, steady_idx,
-                _lib.ptr(traj), stats.data_ptr(), final.data_ptr(), status.data_ptr(),
-                nsteps.data_ptr(), ws.data_ptr(), ws.numel(), self.max_lanes, s.cuda_stream)
+            if peak_cap is None:
+                rc = _lib.lib().bh_solve_batch(*common, ws.data_ptr(), ws.numel(),
+                                               self.max_lanes, s.cuda_stream)
+            else:
+                cap = max(int(peak_cap), 0)
+                pk_vals = torch.full((cap, n, S), float("nan"), dtype=y0.dtype, device=dev)
+                pk_counts = torch.empty((n, S), dtype=torch.int32, device=dev)
+                rc = _lib.lib().bh_solve_batch_peaks(*common, cap, _lib.ptr(pk_vals) if cap else None,
+                                                     pk_counts.data_ptr(), ws.data_ptr(),
+                                                     ws.numel(), self.max_lanes, s.cuda_stream)
         _lib.check(rc, "bh_solve_batch")
         for t in (y0, p, st64, ws):
             t.record_stream(s)
         save_ts = plan.save_ts
         return FusedResult(t=save_ts, y=traj, final=final,
                            stats=stats if steady_idx < n_save else None, status=status,
-                           nsteps=nsteps, steady_idx=int(steady_idx))
+                           nsteps=nsteps, steady_idx=int(steady_idx), peak_values=pk_vals,
+                           peak_counts=pk_counts)

@@ -232,3 +232,49 @@ def test_autotuner_picks_and_caches_a_lane_cap(cuda_device, tmp_path, monkeypatc
     assert best >= 0
     assert (tmp_path / "tune.json").exists()
     assert autotune.tune_lane_cap(solver, ode, y0, probe_span=20.0) == best  # cached
+
+
+@pytest.mark.parametrize("name,dtype,kw", [
+    ("duffing", torch.float32, {}), ("duffing", torch.float64, {}), ("pendulum", torch.float32, {}),
+    ("lorenz", torch.float32, {"stop_threshold": 200.0}),
+    ("lorenz", torch.float32, {"stop_threshold": 200.0, "inf_fill": True}),
+])
+def test_fused_orbit_peaks_equal_trajectory_peaks(cuda_device, name, dtype, kw):
+    """bh_solve_batch_peaks: the kernel's peaks equal extract_orbit_data on the
+    trajectory the same launch wrote (same saved values, so bit for bit), and
+    the other outputs equal a launch without peaks."""
+    from pybasin_amd.orbit import extract_orbit_data, orbit_data_from_result
+
+    ode, _, lo, hi, _ = CASES[name]
+    y0 = torch.tensor(O.sample_uniform(700, lo, hi, 31), dtype=dtype).to(cuda_device)
+    solver = HipDopri5Solver(t_span=(0, 400), t_steps=800, t_eval=(300.0, 400.0), device="cuda",
+                             cache_dir=None, **kw)
+    save_ts, y0p = solver._prepare_tensors(y0)
+    plan = solver.plan_fused(ode, y0p, save_ts, None, time_steady=320.0)
+    res = solver.launch_fused(plan, keep_trajectories=True, peak_cap=399)
+    base = solver.launch_fused(plan, keep_trajectories=True)
+    torch.cuda.synchronize()
+    assert torch.equal(res.y, base.y) and torch.equal(res.stats.nan_to_num(7.0), base.stats.nan_to_num(7.0))
+    dof = list(range(y0.shape[1]))
+    got = orbit_data_from_result(res, dof, 320.0)
+    want = extract_orbit_data(res.t, res.y, dof, time_steady=320.0)
+    assert torch.equal(got.peak_counts, want.peak_counts)
+    assert got.peak_values.shape == want.peak_values.shape
+    assert torch.equal(got.peak_values.nan_to_num(-7.0), want.peak_values.nan_to_num(-7.0))
+    assert int(got.peak_counts.max()) > 0
+    print(f"[result] {name} {dtype}: max peaks {int(got.peak_counts.max())}", flush=True)
+
+
+def test_orbit_peak_buffer_overflow_reruns(cuda_device):
+    """A first buffer smaller than the peak count triggers one exact re-run."""
+    from pybasin_amd.orbit import extract_orbit_data, integrate_orbit
+
+    y0 = torch.tensor(O.sample_uniform(300, [-1.0, -0.5], [1.0, 1.0], 8)).to(cuda_device)
+    solver = HipDopri5Solver(t_span=(0, 300), t_steps=600, t_eval=(200.0, 300.0), device="cuda",
+                             cache_dir=None)
+    res, od = integrate_orbit(solver, DuffingODE(DUFF), y0, dof=[0], time_steady=220.0, peak_cap=2)
+    assert res.peak_values.shape[0] == int(res.peak_counts.max()) > 2
+    full = solver.integrate_fused(DuffingODE(DUFF), y0, time_steady=220.0, keep_trajectories=True)
+    want = extract_orbit_data(full.t, full.y, [0], time_steady=220.0)
+    assert torch.equal(od.peak_counts, want.peak_counts)
+    assert torch.equal(od.peak_values.nan_to_num(-7.0), want.peak_values.nan_to_num(-7.0))

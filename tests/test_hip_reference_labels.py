@@ -192,3 +192,28 @@ def test_case_extractors_fused_equal_trajectory(cuda_device):
         direct = ext.extract_features(Solution(y0, res.t, res.y))
         assert fused.shape == direct.shape
         torch.testing.assert_close(fused.cpu(), direct.cpu(), rtol=1e-5, atol=1e-6)
+
+
+def test_estimator_orbit_data_per_group(cuda_device):
+    """compute_orbit_data in a parameter study: group p's orbit data equals the
+    trajectory peaks of that group's rows."""
+    from pybasin_amd.estimator import BasinStabilityEstimator
+    from pybasin_amd.features import HipFusedFeatureExtractor
+    from pybasin_amd.orbit import extract_orbit_data
+
+    y0 = torch.tensor(O.sample_uniform(300, [-2.6, -10.0], [3.7, 10.0], 4))
+    solver = HipDopri5Solver(t_span=(0, 200), t_steps=400, t_eval=(100.0, 200.0), device="cuda",
+                             cache_dir=None)
+    ext = HipFusedFeatureExtractor(time_steady=150.0, features=None,
+                                   features_per_state={1: {"log_delta": None}})
+    ode = PendulumODE({"alpha": 0.1, "T": 0.5, "K": 1.0})
+    bse = BasinStabilityEstimator(ode, None, 300, solver, ext, template_y0=[[0.4, 0.0], [2.7, 0.0]],
+                                  template_labels=["FP", "LC"], compute_orbit_data=[1])
+    study = bse.run_parameter_study([{"T": 0.3}, {"T": 0.8}], y0)
+    for r, T in zip(study, (0.3, 0.8)):
+        t, y = solver.integrate(PendulumODE({"alpha": 0.1, "T": T, "K": 1.0}), y0.to("cuda"))
+        want = extract_orbit_data(t, y, [1], time_steady=150.0)
+        od = r.extra["orbit_data"]
+        m = int(want.peak_counts.max())
+        assert torch.equal(od.peak_counts, want.peak_counts)
+        assert torch.equal(od.peak_values[:m].nan_to_num(-7.0), want.peak_values.nan_to_num(-7.0))

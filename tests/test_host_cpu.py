@@ -218,3 +218,38 @@ def test_unbounded_rows_are_labelled_and_excluded_on_cpu():
     res = bse.run(torch.tensor([[0.9], [9.0], [-0.4], [-9.0]]))
     assert list(res.labels) == ["pos", "unbounded", "neg", "unbounded"]
     assert res.basin_stability == {"neg": 0.25, "pos": 0.25, "unbounded": 0.5}
+
+
+def test_peak_mask_matches_scipy_argrelmax():
+    """find_peak_mask is strict local maxima with support n (scipy.signal.argrelmax,
+    which the reference names as its equivalent, torch_features_pattern.py:64-78)
+    on the interior; plateaus give no peak and the first and last n points are
+    excluded (argrelmax's default clip mode can report those)."""
+    from scipy.signal import argrelmax
+
+    from pybasin_amd.orbit import extract_orbit_data, extract_peak_values, find_peak_mask
+
+    g = torch.Generator().manual_seed(0)
+    x = torch.randint(0, 6, (64, 5, 3), generator=g).float()  # many ties
+    for n in (1, 2, 3):
+        m = find_peak_mask(x, n)
+        for b in range(5):
+            for s in range(3):
+                want = np.zeros(64, bool)
+                want[argrelmax(x[:, b, s].numpy(), order=n)[0]] = True
+                want[:n] = False
+                want[64 - n:] = False
+                assert (m[:, b, s].numpy() == want).all()
+    vals, counts = extract_peak_values(x, 1)
+    for b in range(5):
+        for s in range(3):
+            pk = x[1:-1, b, s][find_peak_mask(x, 1)[1:-1, b, s]]
+            assert counts[b, s] == len(pk)
+            assert torch.equal(vals[: len(pk), b, s], pk)
+            assert torch.isnan(vals[len(pk):, b, s]).all()
+    t = torch.linspace(0, 20, 201)
+    y = torch.stack([torch.sin(t), torch.cos(2 * t)], -1)[:, None, :]
+    od = extract_orbit_data(t, y, dof=[1], time_steady=10.0)
+    assert od.peak_counts.tolist() == [[3]] and od.time_steady == 10.0
+    # samples every 0.1: a sampled maximum of cos(2t) is within 1 - cos(0.1) of 1
+    assert torch.allclose(od.peak_values[:, 0, 0], torch.ones(3), atol=1 - math.cos(0.1))
