@@ -16,33 +16,59 @@ template <> __device__ __forceinline__ double dsin<double>(double x) { return si
 template <> __device__ __forceinline__ float dcos<float>(float x) { return cosf(x); }
 template <> __device__ __forceinline__ double dcos<double>(double x) { return cos(x); }
 
-// sin and cos of a small argument |d| <= 0.5 by their Taylor series (the
-// truncation error is below 6e-9 for float and 1e-17 for double).
-template <class T> __device__ __forceinline__ void sincos_small(T d, T &s, T &c);
-template <> __device__ __forceinline__ void sincos_small<float>(float d, float &s, float &c) {
+// sin and cos of a small argument by their Taylor series: kWide covers
+// |d| <= 1, the narrow form |d| <= 0.5 (truncation error below 2e-10 /
+// 6e-9 in float and 1e-17 in double).
+template <class T, bool kWide> __device__ __forceinline__ void sincos_small(T d, T &s, T &c);
+template <> __device__ __forceinline__ void sincos_small<float, true>(float d, float &s, float &c) {
+    const float d2 = d * d;
+    float ps = 1.0f / 39916800;  // 1/11!
+    ps = -1.0f / 362880 + d2 * ps;
+    ps = 1.0f / 5040 + d2 * ps;
+    ps = -1.0f / 120 + d2 * ps;
+    ps = 1.0f / 6 + d2 * ps;
+    s = d - d * d2 * ps;
+    float pc = 1.0f / 3628800;  // 1/10!
+    pc = -1.0f / 40320 + d2 * pc;
+    pc = 1.0f / 720 + d2 * pc;
+    pc = -1.0f / 24 + d2 * pc;
+    pc = 0.5f + d2 * pc;
+    c = 1.0f - d2 * pc;
+}
+template <> __device__ __forceinline__ void sincos_small<float, false>(float d, float &s, float &c) {
     const float d2 = d * d;
     s = d + d * d2 * (-1.0f / 6 + d2 * (1.0f / 120 + d2 * (-1.0f / 5040)));
     c = 1.0f + d2 * (-0.5f + d2 * (1.0f / 24 + d2 * (-1.0f / 720 + d2 * (1.0f / 40320))));
 }
-template <> __device__ __forceinline__ void sincos_small<double>(double d, double &s, double &c) {
+template <bool kWide>
+__device__ __forceinline__ void sincos_small_f64(double d, double &s, double &c) {
     const double d2 = d * d;
-    double ps = 1.0 / 1307674368000.0;  // 1/15!
+    double ps = kWide ? 1.0 / 121645100408832000.0 : 0.0;  // 1/19!
+    ps = (kWide ? -1.0 / 355687428096000.0 : 0.0) + d2 * ps;
+    ps = 1.0 / 1307674368000.0 + d2 * ps;
     ps = -1.0 / 6227020800.0 + d2 * ps;
     ps = 1.0 / 39916800.0 + d2 * ps;
     ps = -1.0 / 362880.0 + d2 * ps;
-    ps = -1.0 / 5040.0 + d2 * ps;
-    ps = 1.0 / 120.0 + d2 * ps;
-    ps = -1.0 / 6.0 + d2 * ps;
-    s = d + d * d2 * ps;
-    double pc = 1.0 / 20922789888000.0;  // 1/16!
-    pc = -1.0 / 87178291200.0 + d2 * pc;
-    pc = 1.0 / 479001600.0 + d2 * pc;
-    pc = -1.0 / 3628800.0 + d2 * pc;
-    pc = 1.0 / 40320.0 + d2 * pc;
-    pc = -1.0 / 720.0 + d2 * pc;
-    pc = 1.0 / 24.0 + d2 * pc;
-    pc = -0.5 + d2 * pc;
-    c = 1.0 + d2 * pc;
+    ps = 1.0 / 5040.0 + d2 * ps;
+    ps = -1.0 / 120.0 + d2 * ps;
+    ps = 1.0 / 6.0 + d2 * ps;
+    s = d - d * d2 * ps;
+    double pc = kWide ? 1.0 / 6402373705728000.0 : 0.0;  // 1/18!
+    pc = -1.0 / 20922789888000.0 + d2 * pc;
+    pc = 1.0 / 87178291200.0 + d2 * pc;
+    pc = -1.0 / 479001600.0 + d2 * pc;
+    pc = 1.0 / 3628800.0 + d2 * pc;
+    pc = -1.0 / 40320.0 + d2 * pc;
+    pc = 1.0 / 720.0 + d2 * pc;
+    pc = -1.0 / 24.0 + d2 * pc;
+    pc = 0.5 + d2 * pc;
+    c = 1.0 - d2 * pc;
+}
+template <> __device__ __forceinline__ void sincos_small<double, true>(double d, double &s, double &c) {
+    sincos_small_f64<true>(d, s, c);
+}
+template <> __device__ __forceinline__ void sincos_small<double, false>(double d, double &s, double &c) {
+    sincos_small_f64<false>(d, s, c);
 }
 template <class T> __device__ __forceinline__ void dsincos(T x, T &s, T &c);
 template <> __device__ __forceinline__ void dsincos<float>(float x, float &s, float &c) { sincosf(x, &s, &c); }
@@ -117,15 +143,28 @@ struct Duffing {
     }
     template <class T>
     __device__ __forceinline__ static void time_terms(T t0, const T (&ts)[5], T (&aux)[5]) {
-        // one test per step: the c = 1 stage has the largest offset
-        if (fabs(ts[4] - t0) <= T(0.5)) {
+        // cos(t0 + d_i) = cos t0 cos d_i - sin t0 sin d_i with d_i = t_i - t0
+        // exact (Sterbenz) and sin/cos of d_i by short series.  The c = 1
+        // stage has the largest offset, so one test per step decides; the
+        // series length is chosen per wave (a branch every lane takes).
+        const T dmax = fabs(ts[4] - t0);
+        if (dmax <= T(1)) {
             T s0, c0;
             dsincos<T>(t0, s0, c0);
+            if (__any(dmax > T(0.5))) {
 #pragma unroll
-            for (int i = 0; i < 5; i++) {
-                T sd, cd;
-                sincos_small<T>(ts[i] - t0, sd, cd);
-                aux[i] = c0 * cd - s0 * sd;
+                for (int i = 0; i < 5; i++) {
+                    T sd, cd;
+                    sincos_small<T, true>(ts[i] - t0, sd, cd);
+                    aux[i] = c0 * cd - s0 * sd;
+                }
+            } else {
+#pragma unroll
+                for (int i = 0; i < 5; i++) {
+                    T sd, cd;
+                    sincos_small<T, false>(ts[i] - t0, sd, cd);
+                    aux[i] = c0 * cd - s0 * sd;
+                }
             }
         } else {
 #pragma unroll
