@@ -135,7 +135,7 @@ int32_t solve_batch_impl(int32_t system_id, int32_t dtype, const void *y0s, int6
     a.t0 = t0;
     a.rtol = rtol;
     a.atol = atol;
-    a.max_steps = max_steps;
+    a.max_steps = max_steps < INT32_MAX ? max_steps : INT32_MAX;  // per-lane counter is 32-bit
     a.flags = flags;
     a.stop_thr = stop_threshold;
     a.traj = out_traj;

@@ -124,11 +124,10 @@ template <> __device__ __forceinline__ double tpow<double>(double x, double e) {
 // single precision (relative error ~1e-7 in the next dt, far below the
 // step-size granularity that matters; ratios below the normal range give a
 // factor >= 10).
-__device__ __forceinline__ double next_dt_sq(double dt, double r2) {
-    if (r2 == 0.0) return dt * 10.0;
-    float f = 0.9f * __builtin_amdgcn_exp2f(-0.1f * __builtin_amdgcn_logf((float)r2));
-    const float dfac = r2 < 1.0 ? 1.0f : 0.2f;
-    f = fminf(10.0f, fmaxf(f, dfac));
+__device__ __forceinline__ double next_dt_sq(double dt, float r2, bool below_one) {
+    // r2 == 0 gives log2 = -inf and a factor clamped to 10
+    float f = 0.9f * __builtin_amdgcn_exp2f(-0.1f * __builtin_amdgcn_logf(r2));
+    f = fminf(10.0f, fmaxf(f, below_one ? 1.0f : 0.2f));
     return dt * (double)f;
 }
 
@@ -146,8 +145,9 @@ struct Lane {
     PeakState<Sys::S, T, kPeaks> pk;
     T p[Sys::NP];
     T y[S], k1[S];
+    T s0, c0;  // sin / cos of (T)t for systems with time terms
     double t, dt;
-    int64_t steps;
+    int32_t steps;  // attempted steps (max_steps is capped at 2^31 - 1)
     double next_save;  // save_t[si], +inf once every save is emitted
     int32_t si;
     int32_t status;
@@ -230,6 +230,7 @@ __device__ __forceinline__ void lane_init(const LaneArgs &a, LaneT &L, int64_t i
         for (int i = 0; i < S; i++) L.pk.count[i] = 0;
     }
     L.t = a.t0;
+    if constexpr (Sys::kTime) dsincos<T>((T)a.t0, L.s0, L.c0);
     L.steps = 0;
     L.si = 0;
     L.cnt = 0;
@@ -291,11 +292,15 @@ template <class Sys, class T, class LaneT>
 __device__ __forceinline__ void lane_step(const LaneArgs &a, LaneT &L, int64_t ic) {
     constexpr int S = Sys::S;
     const double tA = L.t, tB = L.t + L.dt;
-    const T tf = (T)tA, h = (T)L.dt, t1p = prev_value<T>((T)tB);
+    const T tf = (T)tA, h = (T)L.dt, tBf = (T)tB, t1p = prev_value<T>(tBf);
     T k2[S], k3[S], k4[S], k5[S], k6[S], k7[S], ys[S];
-    const T tst[5] = {tf + (T)C2 * h, tf + (T)C3 * h, tf + (T)C4 * h, tf + (T)C5 * h, t1p};
+    const T tst[4] = {tf + (T)C2 * h, tf + (T)C3 * h, tf + (T)C4 * h, tf + (T)C5 * h};
     T aux[5];
-    Sys::time_terms(tf, tst, aux);
+    // sin / cos of the end time: the c = 1 stages here, the start of the next
+    // step after an accept
+    T sB = 0, cB = 0;
+    if constexpr (Sys::kTime) dsincos<T>(tBf, sB, cB);
+    Sys::time_terms(tf, tst, L.s0, L.c0, tBf, sB, cB, t1p, aux);
     // Stage inputs y + h * sum_j a_ij k_j: the tableau constants are
     // immediate operands and h is applied once per component.
 #pragma unroll
@@ -337,7 +342,7 @@ __device__ __forceinline__ void lane_step(const LaneArgs &a, LaneT &L, int64_t i
             acc += r * r;
         }
     }
-    const double ratio = (double)(acc * (T)(1.0 / S));  // squared ratio
+    const T ratio = acc * (T)(1.0 / S);  // squared ratio
     // Failure and acceptance are decided after the stages, so the common path
     // has one divergent branch (the dense output) and one exit test; the
     // state update is a select.
@@ -345,7 +350,7 @@ __device__ __forceinline__ void lane_step(const LaneArgs &a, LaneT &L, int64_t i
 #pragma unroll
     for (int i = 0; i < S; i++) finite = finite && isfinite(L.y[i]);
     const bool bad = !(tB > tA) || !finite || isnan(ratio);
-    const bool accept = !bad && ratio <= 1.0;
+    const bool accept = !bad && ratio <= (T)1;
     L.steps += bad ? 0 : 1;
     if (accept && L.next_save <= tB) {
         // dense output on [tA, tB] (torchdiffeq _interp_fit / _interp_evaluate)
@@ -387,10 +392,14 @@ __device__ __forceinline__ void lane_step(const LaneArgs &a, LaneT &L, int64_t i
         ymax = fmax(ymax, fabs(L.y[i]));
     }
     L.t = accept ? tB : L.t;
-    L.dt = next_dt_sq(L.dt, ratio);
+    if constexpr (Sys::kTime) {
+        L.s0 = accept ? sB : L.s0;
+        L.c0 = accept ? cB : L.c0;
+    }
+    L.dt = next_dt_sq(L.dt, (float)ratio, ratio < (T)1);
     const bool finished = accept && L.si >= a.n_save;
     const bool stop = accept && !finished && (a.flags & BH_FLAG_STOP) && ymax >= (T)a.stop_thr;
-    const bool maxed = L.steps >= a.max_steps;
+    const bool maxed = L.steps >= (int32_t)a.max_steps;
     if (bad || finished || stop || maxed) {
         if (!finished) {
             bool inf = false;

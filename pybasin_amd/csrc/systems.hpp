@@ -75,13 +75,16 @@ template <> __device__ __forceinline__ void dsincos<float>(float x, float &s, fl
 template <> __device__ __forceinline__ void dsincos<double>(double x, double &s, double &c) { sincos(x, &s, &c); }
 
 // Per-step time terms.  A system whose right-hand side depends on t gets the
-// five distinct stage times of a step (t0 + c_i h for c = 1/5, 3/10, 4/5,
-// 8/9, and the c = 1 time) and may precompute what it needs from them; the
+// step's stage times (t0 + c_i h for c = 1/5, 3/10, 4/5, 8/9, in the state
+// dtype), the sine and cosine of t0 carried from the previous step, those of
+// the end time tB = (T)(t + dt) and the c = 1 stage time t1 (the previous
+// representable value of tB), and fills one auxiliary value per stage; the
 // others ignore them.
 struct NoTimeTerms {
     static constexpr bool kTime = false;
     template <class T>
-    __device__ __forceinline__ static void time_terms(T, const T (&)[5], T (&aux)[5]) {
+    __device__ __forceinline__ static void time_terms(T, const T (&)[4], T, T, T, T, T, T,
+                                                      T (&aux)[5]) {
 #pragma unroll
         for (int i = 0; i < 5; i++) aux[i] = T(0);
     }
@@ -142,25 +145,24 @@ struct Duffing {
         dy[1] = -p[0] * y[1] - p[1] * (y[0] * y[0] * y[0]) + p[2] * cos_t;
     }
     template <class T>
-    __device__ __forceinline__ static void time_terms(T t0, const T (&ts)[5], T (&aux)[5]) {
+    __device__ __forceinline__ static void time_terms(T t0, const T (&ts)[4], T s0, T c0, T tB,
+                                                      T sB, T cB, T t1, T (&aux)[5]) {
         // cos(t0 + d_i) = cos t0 cos d_i - sin t0 sin d_i with d_i = t_i - t0
-        // exact (Sterbenz) and sin/cos of d_i by short series.  The c = 1
+        // exact (Sterbenz) and sin/cos of d_i by short series.  The c = 8/9
         // stage has the largest offset, so one test per step decides; the
         // series length is chosen per wave (a branch every lane takes).
-        const T dmax = fabs(ts[4] - t0);
+        const T dmax = fabs(ts[3] - t0);
         if (dmax <= T(1)) {
-            T s0, c0;
-            dsincos<T>(t0, s0, c0);
             if (__any(dmax > T(0.5))) {
 #pragma unroll
-                for (int i = 0; i < 5; i++) {
+                for (int i = 0; i < 4; i++) {
                     T sd, cd;
                     sincos_small<T, true>(ts[i] - t0, sd, cd);
                     aux[i] = c0 * cd - s0 * sd;
                 }
             } else {
 #pragma unroll
-                for (int i = 0; i < 5; i++) {
+                for (int i = 0; i < 4; i++) {
                     T sd, cd;
                     sincos_small<T, false>(ts[i] - t0, sd, cd);
                     aux[i] = c0 * cd - s0 * sd;
@@ -168,8 +170,11 @@ struct Duffing {
             }
         } else {
 #pragma unroll
-            for (int i = 0; i < 5; i++) aux[i] = dcos<T>(ts[i]);
+            for (int i = 0; i < 4; i++) aux[i] = dcos<T>(ts[i]);
         }
+        // t1 = tB - e with e one unit in the last place: cos e rounds to 1 and
+        // sin e to e, so cos t1 = cos tB + sin tB * e
+        aux[4] = fma(sB, tB - t1, cB);
     }
 };
 
