@@ -52,13 +52,16 @@ def parse():
     ap.add_argument("--dtype", choices=["f32", "f64"], default="f32")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--max-lanes", type=int, default=0)
-    ap.add_argument("--cpu-sample", type=int, default=20_000)
+    ap.add_argument("--cpu-sample", type=int, default=100_000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", choices=["duffing", "surrogate", "rossler"], default="duffing",
                     help="duffing: plain HIP Dopri5 on 100k Duffing ICs (default); surrogate: "
                          "pendulum torque sweep through the MLP gate with Dopri5 fallback")
     ap.add_argument("--gate", type=float, default=0.995)
     ap.add_argument("--epochs", type=int, default=600)
+    ap.add_argument("--evolve-t", type=float, default=100.0,
+                    help="surrogate: below-gate ICs are integrated to this time and classified "
+                         "again before a full integration (0 disables)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 256 MiB device copy (calibrates the "
                          "FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -131,12 +134,15 @@ def surrogate_workload(args):
     torch.cuda.synchronize()
     ms_plain = 1e3 * (time.perf_counter() - t0) / args.steps
     # gated path
+    ev = args.evolve_t if args.evolve_t > 0 else None
     for _ in range(args.warmup):
-        classify_gated(model, solver, ode, x, gate=args.gate, group=group, time_steady=ts)
+        classify_gated(model, solver, ode, x, gate=args.gate, group=group, time_steady=ts,
+                       evolve_t=ev)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        res = classify_gated(model, solver, ode, x, gate=args.gate, group=group, time_steady=ts)
+        res = classify_gated(model, solver, ode, x, gate=args.gate, group=group, time_steady=ts,
+                             evolve_t=ev)
     torch.cuda.synchronize()
     ms_gated = 1e3 * (time.perf_counter() - t0) / args.steps
     bs_g = basin_stability_by_group(res.labels, group, 46)
@@ -152,6 +158,11 @@ def surrogate_workload(args):
         "config": {
             "workload": "pendulum_torque_sweep_surrogate", "n_ics": n, "torques": 46,
             "gate": args.gate, "routed_frac": round(float(res.routed.float().mean()), 4),
+            "group_routed_frac": round(float(res.group_routed.float().mean()), 4),
+            "evolve_t": ev,
+            "evolved_frac": round(float(res.evolved.float().mean()), 4),
+            "integrated_frac": round(float(res.integrated.float().mean()), 4),
+            "audit_evolved_label_error": res.audit_evolved_error,
             "audit_n": len(res.audit_idx), "audit_label_error": round(audit_err, 5),
             "max_bs_error_vs_plain": round(max_err, 5),
             "plain_ics_per_s": round(n / (ms_plain * 1e-3), 1), "plain_ms": round(ms_plain, 3),

@@ -169,14 +169,17 @@ int32_t bh_mlp_colsum(const float *G, int64_t M, int64_t N, float *out, void *st
 int32_t bh_mlp_adam(float *W, const float *G, float *m, float *v, int64_t count, float lr,
                     float beta1, float beta2, float eps, int64_t step, void *W_bf16, void *stream);
 /* (M x 3) float32 inputs (theta, omega, T) -> (M x 16) bf16 rows
-   [sin x0, cos x0, (x1-c1)s1, (x2-c2)s2, (x0-c0)s0, (x1^2/2 - cos x0)s1^2, 0, ...] */
+   [sin x0, cos x0, (x1-c1)s1, (x2-c2)s2, 0 * (x0-c0)s0, (x1^2/2 - cos x0)s1^2, 0, ...]
+   (the raw angle column is kept at zero: the label is 2 pi periodic in x0) */
 int32_t bh_mlp_prep_input(const float *x, int64_t M, float c0, float s0, float c1, float s1,
                           float c2, float s2, void *out_bf16, void *stream);
 /* fused inference: (M x 3) float32 rows (theta, omega, T) -> label and max
-   softmax probability.  w_packed: the four weight matrices in MFMA
-   B-fragment order ([kstep][colblock][lane][4] bf16: W1 1x16, W2 16x16,
-   W3 16x16, W4 16x1 fragments of 64 lanes); biases: b1, b2, b3 (256 each),
-   b4 (2) float32, concatenated; input scaling as bh_mlp_prep_input */
+   softmax probability.  w_packed: the four weight matrices (zero-padded to
+   multiples of 32 rows and 16 columns) in fragment order [kstep][colblock]
+   [lane][8] bf16, element (s, c, lane, e) = W[32 s + 8 (lane / 16) + e]
+   [16 c + lane % 16]: W1 1x16, W2 8x16, W3 8x16, W4 8x1 fragments of 64
+   lanes; biases: b1, b2, b3 (256 each), b4 (2) float32, concatenated; input
+   scaling as bh_mlp_prep_input */
 int32_t bh_mlp_infer(const float *x, int64_t M, const void *w_packed, const float *biases,
                      float c0, float s0, float c1, float s1, float c2, float s2, int32_t *label,
                      float *conf, void *stream);

@@ -371,14 +371,14 @@ int32_t bh_mlp_infer(const float *x, int64_t M, const void *w_packed, const floa
     if (!x || !w_packed || !biases || !label || !conf) return fail(-5, "null argument");
     const int64_t tiles = (M + P::IT_ROWS - 1) / P::IT_ROWS;
     if (tiles > 2147483647LL) return fail(-5, "M too large");
-    const P::bf16x4 *w = reinterpret_cast<const P::bf16x4 *>(w_packed);
+    const P::bf16x8v *w = reinterpret_cast<const P::bf16x8v *>(w_packed);
     P::InferArgs a;
     a.x = x;
     a.M = M;
-    a.w1 = w;                      // 1 kstep x 16 colblocks x 64 lanes
-    a.w2 = a.w1 + 1 * 16 * 64;     // 16 x 16 x 64
-    a.w3 = a.w2 + 16 * 16 * 64;
-    a.w4 = a.w3 + 16 * 16 * 64;    // 16 x 1 x 64
+    a.w1 = w;                      // 1 kstep x 16 colblocks x 64 lanes (x 8 bf16)
+    a.w2 = a.w1 + 1 * 16 * 64;     // 8 x 16 x 64
+    a.w3 = a.w2 + 8 * 16 * 64;
+    a.w4 = a.w3 + 8 * 16 * 64;     // 8 x 1 x 64
     a.b1 = biases;
     a.b2 = biases + 256;
     a.b3 = biases + 512;

@@ -217,125 +217,165 @@ namespace mlp {
 
 // ---- fused inference -------------------------------------------------------
 // One workgroup (4 waves) takes a tile of 128 rows through the whole network:
-// input features -> 256 -> 256 -> 256 -> 2 -> (label, confidence).  The tile's
-// activations live in LDS (bf16, padded rows); the weights are pre-packed in
-// the B-fragment order of v_mfma_f32_16x16x16bf16_1k, so every fragment is one
-// 8-byte load per lane, coalesced across the wave and served from L2.
-// Packed layout per layer: [kstep][colblock][lane][4] with
-//   k = 16 kstep + 4 (lane / 16) + e,  col = 16 colblock + lane % 16.
+// input features -> 256 -> 256 -> 256 -> 2 -> (label, confidence).  The
+// tile's activations live in one LDS buffer (bf16, padded rows, 66 KB, so
+// two workgroups share a CU); a layer reads the whole buffer, waits at a
+// barrier and writes its output over it.  Products use the 32-deep bf16
+// matrix instruction with the weights as the first operand, so each lane
+// ends with four adjacent output columns of one row (one 8-byte LDS store).
+// Weights are pre-packed per layer as [kstep][colblock][lane][8] with
+//   k = 32 kstep + 8 (lane / 16) + e,  col = 16 colblock + lane % 16,
+// one 16-byte load per lane, coalesced across the wave and served from L2.
 constexpr int IT_ROWS = 128;
 constexpr int IT_LD = 264;  // LDS row stride in bf16 (256 + 8)
+
+typedef __bf16 bf16x8v __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x4v __attribute__((ext_vector_type(4)));
 
 struct InferArgs {
     const float *x;              // (M, 3) theta, omega, T
     int64_t M;
-    const bf16x4 *w1, *w2, *w3, *w4;  // packed fragments
-    const float *b1, *b2, *b3, *b4;  // biases (b4 has 2 entries)
-    float c0, s0, c1, s1, c2, s2;    // input scaling
+    const bf16x8v *w1, *w2, *w3, *w4;  // packed fragments
+    const float *b1, *b2, *b3, *b4;    // biases (b4 has 2 entries)
+    float c0, s0, c1, s1, c2, s2;      // input scaling
     int32_t *label;
     float *conf;
 };
 
 __device__ __forceinline__ float silu(float z) { return z / (1.0f + __expf(-z)); }
+// SiLU with a hardware reciprocal (1 ulp); the result is rounded to bf16
+__device__ __forceinline__ float silu_fast(float z) {
+    return z * __builtin_amdgcn_rcpf(1.0f + __expf(-z));
+}
 
-// one 256-wide layer: out[128][256] = act(in[128][K] . W + b); wave w owns
-// column blocks 4w .. 4w+3 for all 8 row blocks.
+// out^T blocks of one 256-wide layer: wave w owns column blocks 4w .. 4w+3
+// for all 8 row blocks; acc[rb][c][r] = column 16 (4w + c) + 4 (lane / 16) + r
+// of row 16 rb + lane % 16.
 template <int KSTEPS>
-__device__ __forceinline__ void dense_layer(const unsigned short *in, unsigned short *out,
-                                            const bf16x4 *W, const float *b, int wave, int lane) {
-    floatx4 acc[8][4];
+__device__ __forceinline__ void layer_products(const unsigned short *buf, const bf16x8v *W,
+                                               int wave, int lane, floatx4 (&acc)[8][4]) {
 #pragma unroll
     for (int rb = 0; rb < 8; rb++)
 #pragma unroll
         for (int c = 0; c < 4; c++) acc[rb][c] = (floatx4){0.f, 0.f, 0.f, 0.f};
-    const int lr = lane & 15, lk = (lane >> 4) * 4;
-#pragma unroll 2
-    for (int s = 0; s < KSTEPS; s++) {
-        bf16x4 bfr[4];
+    const int lr = lane & 15, lk = (lane >> 4) * 8;
+    bf16x8v wf[4];
 #pragma unroll
-        for (int c = 0; c < 4; c++) bfr[c] = W[((int64_t)s * 16 + (4 * wave + c)) * 64 + lane];
+    for (int c = 0; c < 4; c++) wf[c] = W[(4 * wave + c) * 64 + lane];
+#pragma unroll 1
+    for (int s = 0; s < KSTEPS; s++) {
+        bf16x8v wn[4];
+        if (s + 1 < KSTEPS) {  // next step's fragments in flight during this step
+#pragma unroll
+            for (int c = 0; c < 4; c++) wn[c] = W[((int64_t)(s + 1) * 16 + 4 * wave + c) * 64 + lane];
+        }
 #pragma unroll
         for (int rb = 0; rb < 8; rb++) {
-            const bf16x4 af =
-                *reinterpret_cast<const bf16x4 *>(in + (rb * 16 + lr) * IT_LD + s * 16 + lk);
+            const bf16x8v xf =
+                *reinterpret_cast<const bf16x8v *>(buf + (rb * 16 + lr) * IT_LD + s * 32 + lk);
 #pragma unroll
             for (int c = 0; c < 4; c++)
-                acc[rb][c] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(af, bfr[c], acc[rb][c], 0, 0, 0);
+                acc[rb][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wf[c], xf, acc[rb][c], 0, 0, 0);
         }
-    }
+        if (s + 1 < KSTEPS) {
 #pragma unroll
-    for (int c = 0; c < 4; c++) {
-        const int col = (4 * wave + c) * 16 + lr;
-        const float bias = b[col];
-#pragma unroll
-        for (int rb = 0; rb < 8; rb++)
-#pragma unroll
-            for (int r = 0; r < 4; r++) {
-                const int row = rb * 16 + (lane >> 4) * 4 + r;
-                out[row * IT_LD + col] = f2bf(silu(acc[rb][c][r] + bias));
-            }
+            for (int c = 0; c < 4; c++) wf[c] = wn[c];
+        }
     }
 }
 
-__global__ void __launch_bounds__(256) mlp_infer_kernel(InferArgs a) {
-    __shared__ __attribute__((aligned(16))) unsigned short bufA[IT_ROWS * IT_LD];
-    __shared__ __attribute__((aligned(16))) unsigned short bufB[IT_ROWS * IT_LD];
+__device__ __forceinline__ void layer_store(unsigned short *buf, const float *b, int wave,
+                                            int lane, const floatx4 (&acc)[8][4]) {
+    const int lr = lane & 15, q = lane >> 4;
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        const int col = (4 * wave + c) * 16 + 4 * q;
+        const float4 bias = *reinterpret_cast<const float4 *>(b + col);
+#pragma unroll
+        for (int rb = 0; rb < 8; rb++) {
+            bf16x4v v;
+            v[0] = (__bf16)silu_fast(acc[rb][c][0] + bias.x);
+            v[1] = (__bf16)silu_fast(acc[rb][c][1] + bias.y);
+            v[2] = (__bf16)silu_fast(acc[rb][c][2] + bias.z);
+            v[3] = (__bf16)silu_fast(acc[rb][c][3] + bias.w);
+            *reinterpret_cast<bf16x4v *>(buf + (rb * 16 + lr) * IT_LD + col) = v;
+        }
+    }
+}
+
+template <int KSTEPS>
+__device__ __forceinline__ void dense_layer(unsigned short *buf, const bf16x8v *W, const float *b,
+                                            int wave, int lane) {
+    floatx4 acc[8][4];
+    layer_products<KSTEPS>(buf, W, wave, lane, acc);
+    __syncthreads();  // every wave has read the layer input
+    layer_store(buf, b, wave, lane, acc);
+    __syncthreads();
+}
+
+__global__ void __launch_bounds__(256, 2) mlp_infer_kernel(InferArgs a) {
+    __shared__ __attribute__((aligned(16))) unsigned short buf[IT_ROWS * IT_LD];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int64_t row0 = (int64_t)blockIdx.x * IT_ROWS;
 
-    // input features (16 columns) into bufA
-    for (int i = tid; i < IT_ROWS * 16; i += 256) {
-        const int r = i >> 4, col = i & 15;
-        const int64_t m = row0 + r;
-        float v = 0.f;
+    // input features of one row per thread: columns 0..31 (6 used, rest 0)
+    if (tid < IT_ROWS) {
+        const int64_t m = row0 + tid;
+        float f[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
         if (m < a.M) {
             const float th = a.x[3 * m], om = a.x[3 * m + 1], T = a.x[3 * m + 2];
-            if (col == 0) v = sinf(th);
-            if (col == 1) v = cosf(th);
-            if (col == 2) v = (om - a.c1) * a.s1;
-            if (col == 3) v = (T - a.c2) * a.s2;
-            if (col == 4) v = 0.0f * (th - a.c0) * a.s0;
-            if (col == 5) v = (0.5f * om * om - cosf(th)) * (a.s1 * a.s1);
+            float sn, cs;
+            sincosf(th, &sn, &cs);
+            f[0] = sn;
+            f[1] = cs;
+            f[2] = (om - a.c1) * a.s1;
+            f[3] = (T - a.c2) * a.s2;
+            f[4] = 0.0f * (th - a.c0) * a.s0;
+            f[5] = (0.5f * om * om - cs) * (a.s1 * a.s1);
         }
-        bufA[r * IT_LD + col] = f2bf(v);
+        bf16x8v v0, z;
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+            v0[e] = (__bf16)(e < 6 ? f[e] : 0.f);
+            z[e] = (__bf16)0.f;
+        }
+        bf16x8v *dst = reinterpret_cast<bf16x8v *>(buf + tid * IT_LD);
+        dst[0] = v0;
+        dst[1] = z;
+        dst[2] = z;
+        dst[3] = z;
     }
     __syncthreads();
-    dense_layer<1>(bufA, bufB, a.w1, a.b1, wave, lane);
-    __syncthreads();
-    dense_layer<16>(bufB, bufA, a.w2, a.b2, wave, lane);
-    __syncthreads();
-    dense_layer<16>(bufA, bufB, a.w3, a.b3, wave, lane);
-    __syncthreads();
-    // output layer: 2 logits (packed with 16 columns, 14 of them zero);
-    // wave w takes row blocks 2w, 2w+1
-    const int lr = lane & 15, lk = (lane >> 4) * 4;
+    dense_layer<1>(buf, a.w1, a.b1, wave, lane);
+    dense_layer<8>(buf, a.w2, a.b2, wave, lane);
+    dense_layer<8>(buf, a.w3, a.b3, wave, lane);
+    // output layer: 2 logits in a 16-column block; wave w takes row blocks
+    // 2w, 2w+1; lanes 0..15 hold logits 0 and 1 of row 16 rb + lane
+    const int lr = lane & 15, lk = (lane >> 4) * 8;
     floatx4 acc[2] = {(floatx4){0.f, 0.f, 0.f, 0.f}, (floatx4){0.f, 0.f, 0.f, 0.f}};
-#pragma unroll 4
-    for (int s = 0; s < 16; s++) {
-        const bf16x4 bfr = a.w4[s * 64 + lane];
+#pragma unroll
+    for (int s = 0; s < 8; s++) {
+        const bf16x8v wf = a.w4[s * 64 + lane];
 #pragma unroll
         for (int j = 0; j < 2; j++) {
             const int rb = 2 * wave + j;
-            const bf16x4 af = *reinterpret_cast<const bf16x4 *>(bufB + (rb * 16 + lr) * IT_LD + s * 16 + lk);
-            acc[j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(af, bfr, acc[j], 0, 0, 0);
+            const bf16x8v xf = *reinterpret_cast<const bf16x8v *>(buf + (rb * 16 + lr) * IT_LD + s * 32 + lk);
+            acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wf, xf, acc[j], 0, 0, 0);
         }
     }
-    const float bias = lr < 2 ? a.b4[lr] : 0.f;
+    if (lane < 16) {
+        const float b0 = a.b4[0], b1 = a.b4[1];
 #pragma unroll
-    for (int j = 0; j < 2; j++)
-#pragma unroll
-        for (int r = 0; r < 4; r++) {
-            const float l_own = acc[j][r] + bias;
-            const float l_next = __shfl_down(l_own, 1);  // column 1 for the lane holding column 0
-            const int row = (2 * wave + j) * 16 + (lane >> 4) * 4 + r;
-            const int64_t m = row0 + row;
-            if (lr == 0 && m < a.M) {
-                const float d = l_next - l_own;
+        for (int j = 0; j < 2; j++) {
+            const int64_t m = row0 + (2 * wave + j) * 16 + lane;
+            if (m < a.M) {
+                const float d = (acc[j][1] + b1) - (acc[j][0] + b0);
                 const float p1 = 1.0f / (1.0f + __expf(-d));
                 a.label[m] = d > 0.f ? 1 : 0;
                 a.conf[m] = fmaxf(p1, 1.0f - p1);
             }
         }
+    }
 }
 
 }  // namespace mlp

@@ -169,18 +169,19 @@ class SurrogateMLP:
         return hist
 
     def packed(self):
-        """Weights in the B-fragment order of the fused inference kernel
-        ([kstep][colblock][lane][4] bf16 per layer) and the biases concatenated."""
+        """Weights in the fragment order of the fused inference kernel
+        ([kstep][colblock][lane][8] bf16 per layer, 32-deep steps) and the
+        biases concatenated."""
         if getattr(self, "_packed_step", None) == self.step:
             return self._packed
         frags = []
         for W in self.Wb:
             K, N = W.shape
-            ks, cb = (K + 15) // 16, (N + 15) // 16
-            Wp = torch.zeros((ks * 16, cb * 16), dtype=torch.bfloat16, device=self.device)
+            ks, cb = (K + 31) // 32, (N + 15) // 16
+            Wp = torch.zeros((ks * 32, cb * 16), dtype=torch.bfloat16, device=self.device)
             Wp[:K, :N] = W
-            # element (s, c, lane, e) = Wp[16 s + 4 (lane // 16) + e, 16 c + lane % 16]
-            t = Wp.view(ks, 4, 4, cb, 16)            # s, kq(=lane//16), e, c, lr
+            # element (s, c, lane, e) = Wp[32 s + 8 (lane // 16) + e, 16 c + lane % 16]
+            t = Wp.view(ks, 4, 8, cb, 16)            # s, kq(=lane//16), e, c, lr
             t = t.permute(0, 3, 1, 4, 2).contiguous()  # s, c, kq, lr, e
             frags.append(t.view(-1))
         b = torch.cat([self.b[0], self.b[1], self.b[2], self.b[3]]).contiguous()
@@ -225,10 +226,14 @@ class SurrogateMLP:
 @dataclass
 class GatedResult:
     labels: torch.Tensor          # (n,) int32 final labels
-    routed: torch.Tensor          # (n,) bool: integrated by Dopri5
+    routed: torch.Tensor          # (n,) bool: below the gate or in a distrusted group
     audit_idx: torch.Tensor       # indices of the audit sample
     audit_surrogate: torch.Tensor  # surrogate labels on the audit sample
     audit_integrated: torch.Tensor  # integrator labels on the audit sample
+    integrated: torch.Tensor | None = None  # (n,) bool: integrated over the full span
+    group_routed: torch.Tensor | None = None  # (n,) bool: routed by the group audit
+    evolved: torch.Tensor | None = None      # (n,) bool: labelled after a short integration
+    audit_evolved_error: float | None = None  # evolved-label error on routed audit rows
 
 
 def pendulum_labels_from_stats(stats: torch.Tensor) -> torch.Tensor:
@@ -257,45 +262,149 @@ def integrate_labels(solver, ode, x: torch.Tensor, time_steady: float, stream=No
     return pendulum_labels_from_stats(res.stats)
 
 
+def evolve_states(solver, ode, x: torch.Tensor, t_probe: float, stream=None) -> torch.Tensor:
+    """Rows (theta, omega, T) advanced by Dopri5 from t_span[0] to t_probe
+    (same tolerances, per-row torque), as new (theta, omega, T) rows."""
+    dev = solver.device
+    n = x.shape[0]
+    if n == 0:
+        return x
+    alpha, K = float(ode.params["alpha"]), float(ode.params["K"])
+    params = torch.stack([torch.full((n,), alpha, dtype=torch.float64, device=dev),
+                          x[:, 2].double(), torch.full((n,), K, dtype=torch.float64, device=dev)], 1)
+    save = torch.tensor([float(t_probe)], dtype=torch.float32, device=dev)
+    res = solver.run_fused(ode, x[:, :2].contiguous(), save, params.contiguous(),
+                           keep_trajectories=False, steady_idx=1, stream=stream)
+    return torch.cat([res.final, x[:, 2:3]], 1).contiguous()
+
+
 def classify_gated(model: SurrogateMLP, solver, ode, x: torch.Tensor, gate: float = 0.95,
                    audit_frac: float = 0.02, group: torch.Tensor | None = None,
                    time_steady: float = 950.0, seed: int = 0,
-                   group_error_cap: float | None = 0.01) -> GatedResult:
+                   group_error_cap: float | None = 0.01,
+                   evolve_t: float | None = None) -> GatedResult:
     """Surrogate labels for confident rows, Dopri5 for the rest.
 
-    A random audit sample (audit_frac of every group) is integrated first; a
-    group whose surrogate error on its confident audited rows exceeds
-    group_error_cap is integrated entirely (the surrogate is not trusted
-    there), the other groups only below the confidence gate."""
+    A random audit sample (audit_frac of every group) is integrated over the
+    full span on a second stream while the network runs; a group whose
+    surrogate error on its confident audited rows exceeds group_error_cap is
+    distrusted (the surrogate's first answer is not used there), the other
+    groups keep the confident labels.
+
+    evolve_t: rows below the gate are first integrated to t = evolve_t only
+    and classified again at the state they reached: the label is a property
+    of the trajectory, and after the transient most states lie far from the
+    basin boundary.  Rows still below the gate are integrated over the full
+    span (on a third stream, while the audit finishes).  Distrusted groups go
+    the same way, and a distrusted group whose evolved labels disagree with
+    its audit by more than group_error_cap is integrated entirely.  Without
+    evolve_t every routed row is integrated over the full span."""
     dev = solver.device
     x = x.to(dev, torch.float32).contiguous()
     n = x.shape[0]
-    lab, conf = model.predict(x)
-    routed = conf < gate
+    main = torch.cuda.current_stream(dev)
     g = torch.Generator(device=dev).manual_seed(seed)
     u = torch.rand((n,), generator=g, device=dev)
     audit = u < audit_frac
     if group is not None:  # every group audited (a uniform draw almost always is)
         gl = group.long()
         ng = int(gl.max()) + 1
-        have = torch.bincount(gl, weights=audit.double(), minlength=ng)
-        empty = torch.nonzero(have == 0).squeeze(1).tolist()
-        for gi in empty:
+        have = torch.zeros((ng,), dtype=torch.bool, device=dev)
+        have[gl[audit]] = True
+        for gi in torch.nonzero(~have).squeeze(1).tolist():
             audit[torch.nonzero(gl == gi)[0, 0]] = True
     aidx = torch.nonzero(audit).squeeze(1)
+    xa = x[aidx]
+    # the audit's full integrations overlap the network and the short integrations
+    side = torch.cuda.Stream(device=dev)
+    side.wait_stream(main)
+    with torch.cuda.stream(side):
+        lab_audit = integrate_labels(solver, ode, xa, time_steady)
+    lab, conf = model.predict(x)
+    routed = conf < gate
+    evolved = torch.zeros_like(routed)
+    lab_e = torch.full((n,), -1, dtype=torch.int32, device=dev)
     full = torch.full((n,), -1, dtype=torch.int32, device=dev)
-    full[aidx] = integrate_labels(solver, ode, x[aidx], time_steady)
+
+    def evolve(rows: torch.Tensor) -> None:
+        idx = torch.nonzero(rows).squeeze(1)
+        if idx.numel() == 0:
+            return
+        lab2, conf2 = model.predict(evolve_states(solver, ode, x[idx], evolve_t))
+        ok = conf2 >= gate
+        evolved[idx[ok]] = True
+        lab_e[idx[ok]] = lab2[ok]
+
+    pending = []  # (stream, row indices, labels) of full integrations in flight
+
+    def integrate_async(rows: torch.Tensor) -> None:
+        idx = torch.nonzero(rows).squeeze(1)
+        if idx.numel() == 0:
+            return
+        xs = x[idx]
+        st = torch.cuda.Stream(device=dev)
+        st.wait_stream(main)
+        with torch.cuda.stream(st):
+            lab_f = integrate_labels(solver, ode, xs, time_steady)
+        pending.append((st, idx, xs, lab_f))
+
+    done_full = torch.zeros_like(routed)
+    if evolve_t is not None:
+        evolve(routed)
+        first = routed & ~evolved & ~audit
+        integrate_async(first)  # still below the gate: full span, now
+        done_full |= first
+    else:
+        integrate_async(routed & ~audit)
+        done_full |= routed & ~audit
+    main.wait_stream(side)
+    lab_audit.record_stream(main)
+    xa.record_stream(side)
+    full[aidx] = lab_audit
+    bad_group = None
+    distrusted = torch.zeros_like(routed)
     if group is not None and group_error_cap is not None:
-        wrong = ((lab != full) & audit & ~routed).double()
-        conf_aud = (audit & ~routed).double()
-        rate = torch.bincount(gl, weights=wrong, minlength=ng) / torch.bincount(
-            gl, weights=conf_aud, minlength=ng).clamp_min(1)
-        routed = routed | (rate > group_error_cap)[gl]
-    ridx = torch.nonzero(routed & ~audit).squeeze(1)
-    full[ridx] = integrate_labels(solver, ode, x[ridx], time_steady)
-    final = torch.where(routed | audit, full, lab)
+        ga = gl[aidx]
+
+        def group_rate(pred, valid):
+            # error per group over its audited rows where valid
+            v = valid[aidx]
+            wrong = ((pred[aidx] != lab_audit) & v).double()
+            return torch.bincount(ga, weights=wrong, minlength=ng) / torch.bincount(
+                ga, weights=v.double(), minlength=ng).clamp_min(1)
+
+        bad_group = group_rate(lab, ~routed) > group_error_cap
+        distrusted = bad_group[gl]
+        if bool(bad_group.any()):
+            if evolve_t is not None:
+                evolve(distrusted & ~routed)
+                still_bad = group_rate(lab_e, evolved) > group_error_cap
+                distrusted = (bad_group & still_bad)[gl]
+    group_routed = (bad_group[gl] & ~routed) if bad_group is not None else torch.zeros_like(routed)
+    routed = routed | group_routed
+    if evolve_t is not None:
+        use_e = evolved & ~distrusted
+        final = torch.where(use_e, lab_e, lab)
+        need_full = ((routed & ~use_e) | distrusted) & ~audit & ~done_full
+        ea = use_e & audit
+        audit_evolved_error = float((lab_e[ea] != full[ea]).float().mean()) if bool(ea.any()) else None
+    else:
+        use_e = evolved
+        final = lab
+        need_full = group_routed & ~audit & ~done_full
+        audit_evolved_error = None
+    integrate_async(need_full)
+    for st, idx, xs, lab_f in pending:
+        main.wait_stream(st)
+        lab_f.record_stream(main)
+        xs.record_stream(st)
+        full[idx] = lab_f
+    integrated = need_full | done_full | audit
+    final = torch.where(integrated, full, final)
     return GatedResult(labels=final, routed=routed, audit_idx=aidx, audit_surrogate=lab[aidx],
-                       audit_integrated=full[aidx])
+                       audit_integrated=full[aidx], integrated=integrated,
+                       group_routed=group_routed, evolved=use_e & ~integrated,
+                       audit_evolved_error=audit_evolved_error)
 
 
 # ---- pendulum torque sweep (surrogate workload) ----------------------------

@@ -153,17 +153,26 @@ def test_surrogate_gain_on_torque_sweep(cuda_device):
     plain = integrate_labels(solver, ode, x, 950.0)
     torch.cuda.synchronize()
     t_plain = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    res = classify_gated(model, solver, ode, x, gate=0.995, group=group)
-    torch.cuda.synchronize()
-    t_gated = time.perf_counter() - t0
-    routed = float(res.routed.float().mean())
-    err = float((basin_stability_by_group(res.labels, group, 6) -
-                 basin_stability_by_group(plain, group, 6)).abs().max())
-    audit = float((res.audit_surrogate != res.audit_integrated).float().mean())
-    print(f"[result] routed {routed:.3f} max BS err {err:.4f} audit err {audit:.4f} "
-          f"plain {t_plain * 1e3:.1f} ms gated {t_gated * 1e3:.1f} ms", flush=True)
-    assert routed < 0.6          # the network labels most ICs itself
-    assert err < 0.01
-    assert audit < 0.15
-    assert len(res.audit_idx) >= 6  # every torque audited
+    for evolve_t in (None, 50.0):
+        t0 = time.perf_counter()
+        res = classify_gated(model, solver, ode, x, gate=0.995, group=group, evolve_t=evolve_t)
+        torch.cuda.synchronize()
+        t_gated = time.perf_counter() - t0
+        routed = float(res.routed.float().mean())
+        integrated = float(res.integrated.float().mean())
+        err = float((basin_stability_by_group(res.labels, group, 6) -
+                     basin_stability_by_group(plain, group, 6)).abs().max())
+        audit = float((res.audit_surrogate != res.audit_integrated).float().mean())
+        print(f"[result] evolve {evolve_t}: routed {routed:.3f} integrated {integrated:.3f} "
+              f"max BS err {err:.4f} audit err {audit:.4f} evolved-label audit err "
+              f"{res.audit_evolved_error} plain {t_plain * 1e3:.1f} ms gated {t_gated * 1e3:.1f} ms",
+              flush=True)
+        assert routed < 0.6          # the network labels most ICs itself
+        assert err < 0.01
+        assert audit < 0.15
+        assert len(res.audit_idx) >= 6  # every torque audited
+        # every row's label is the surrogate's, an evolved state's, or the integrator's
+        assert bool(((res.labels == 0) | (res.labels == 1)).all())
+        if evolve_t is not None:
+            assert integrated < routed  # the short integrations resolve routed rows
+            assert res.audit_evolved_error is None or res.audit_evolved_error < 0.02
