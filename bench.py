@@ -57,9 +57,11 @@ def parse():
     ap.add_argument("--workload", choices=["duffing", "surrogate", "rossler"], default="duffing",
                     help="duffing: plain HIP Dopri5 on 100k Duffing ICs (default); surrogate: "
                          "pendulum torque sweep through the MLP gate with Dopri5 fallback")
-    ap.add_argument("--gate", type=float, default=0.995)
+    ap.add_argument("--gate", type=float, default=0.999)
     ap.add_argument("--epochs", type=int, default=600)
-    ap.add_argument("--evolve-t", type=float, default=100.0,
+    ap.add_argument("--early-frac", type=float, default=0.0,
+                    help="surrogate: least confident fraction integrated with the audit")
+    ap.add_argument("--evolve-t", type=float, default=50.0,
                     help="surrogate: below-gate ICs are integrated to this time and classified "
                          "again before a full integration (0 disables)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -137,14 +139,17 @@ def surrogate_workload(args):
     ev = args.evolve_t if args.evolve_t > 0 else None
     for _ in range(args.warmup):
         classify_gated(model, solver, ode, x, gate=args.gate, group=group, time_steady=ts,
-                       evolve_t=ev)
+                       evolve_t=ev, early_frac=args.early_frac)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
         res = classify_gated(model, solver, ode, x, gate=args.gate, group=group, time_steady=ts,
-                             evolve_t=ev)
+                             evolve_t=ev, early_frac=args.early_frac)
     torch.cuda.synchronize()
     ms_gated = 1e3 * (time.perf_counter() - t0) / args.steps
+    stages = {}
+    classify_gated(model, solver, ode, x, gate=args.gate, group=group, time_steady=ts, evolve_t=ev,
+                   early_frac=args.early_frac, stage_times=stages)
     bs_g = basin_stability_by_group(res.labels, group, 46)
     bs_p = basin_stability_by_group(plab, group, 46)
     max_err = float((bs_g - bs_p).abs().max())
@@ -159,7 +164,8 @@ def surrogate_workload(args):
             "workload": "pendulum_torque_sweep_surrogate", "n_ics": n, "torques": 46,
             "gate": args.gate, "routed_frac": round(float(res.routed.float().mean()), 4),
             "group_routed_frac": round(float(res.group_routed.float().mean()), 4),
-            "evolve_t": ev,
+            "evolve_t": ev, "early_frac": args.early_frac, "late_integrated": res.n_late,
+            "stage_ms": {k: round(v, 3) for k, v in stages.items()},
             "evolved_frac": round(float(res.evolved.float().mean()), 4),
             "integrated_frac": round(float(res.integrated.float().mean()), 4),
             "audit_evolved_label_error": res.audit_evolved_error,

@@ -234,6 +234,7 @@ class GatedResult:
     group_routed: torch.Tensor | None = None  # (n,) bool: routed by the group audit
     evolved: torch.Tensor | None = None      # (n,) bool: labelled after a short integration
     audit_evolved_error: float | None = None  # evolved-label error on routed audit rows
+    n_late: int = 0                           # rows integrated after the network passes
 
 
 def pendulum_labels_from_stats(stats: torch.Tensor) -> torch.Tensor:
@@ -282,7 +283,8 @@ def classify_gated(model: SurrogateMLP, solver, ode, x: torch.Tensor, gate: floa
                    audit_frac: float = 0.02, group: torch.Tensor | None = None,
                    time_steady: float = 950.0, seed: int = 0,
                    group_error_cap: float | None = 0.01,
-                   evolve_t: float | None = None) -> GatedResult:
+                   evolve_t: float | None = None, early_frac: float = 0.0,
+                   stage_times: dict | None = None) -> GatedResult:
     """Surrogate labels for confident rows, Dopri5 for the rest.
 
     A random audit sample (audit_frac of every group) is integrated over the
@@ -295,11 +297,29 @@ def classify_gated(model: SurrogateMLP, solver, ode, x: torch.Tensor, gate: floa
     and classified again at the state they reached: the label is a property
     of the trajectory, and after the transient most states lie far from the
     basin boundary.  Rows still below the gate are integrated over the full
-    span (on a third stream, while the audit finishes).  Distrusted groups go
+    span (on the main stream, while the audit finishes on its own).  Distrusted groups go
     the same way, and a distrusted group whose evolved labels disagree with
     its audit by more than group_error_cap is integrated entirely.  Without
-    evolve_t every routed row is integrated over the full span."""
+    evolve_t every routed row is integrated over the full span.
+
+    early_frac: the early_frac * n least confident rows are integrated over
+    the full span in the audit's launch, so the rows a short integration
+    leaves undecided (near-boundary ICs, mostly among them) do not need a
+    full-span launch of their own afterwards.  A scheduling choice only:
+    integrated rows always take the integrator's label.
+
+    stage_times: if a dict is given, it receives the main stream's elapsed
+    milliseconds at each stage boundary (HIP events; read after the call)."""
     dev = solver.device
+    marks = []
+
+    def mark(name):
+        if stage_times is not None:
+            ev = torch.cuda.Event(enable_timing=True)
+            ev.record(torch.cuda.current_stream(dev))
+            marks.append((name, ev))
+
+    mark("start")
     x = x.to(dev, torch.float32).contiguous()
     n = x.shape[0]
     main = torch.cuda.current_stream(dev)
@@ -314,13 +334,24 @@ def classify_gated(model: SurrogateMLP, solver, ode, x: torch.Tensor, gate: floa
         for gi in torch.nonzero(~have).squeeze(1).tolist():
             audit[torch.nonzero(gl == gi)[0, 0]] = True
     aidx = torch.nonzero(audit).squeeze(1)
-    xa = x[aidx]
-    # the audit's full integrations overlap the network and the short integrations
     side = torch.cuda.Stream(device=dev)
+    early = torch.zeros_like(audit)
+    k = int(early_frac * n)
+    if k > 0:
+        lab, conf = model.predict(x)
+        mark("infer")
+        early[torch.topk(conf, k, largest=False).indices] = True
+    side_rows = audit | early
+    sidx = torch.nonzero(side_rows).squeeze(1)
+    xs = x[sidx]
+    # the audit's (and the earliest) full integrations overlap the network
+    # passes and the short integrations
     side.wait_stream(main)
     with torch.cuda.stream(side):
-        lab_audit = integrate_labels(solver, ode, xa, time_steady)
-    lab, conf = model.predict(x)
+        lab_side = integrate_labels(solver, ode, xs, time_steady)
+    if k == 0:
+        lab, conf = model.predict(x)
+        mark("infer")
     routed = conf < gate
     evolved = torch.zeros_like(routed)
     lab_e = torch.full((n,), -1, dtype=torch.int32, device=dev)
@@ -335,32 +366,33 @@ def classify_gated(model: SurrogateMLP, solver, ode, x: torch.Tensor, gate: floa
         evolved[idx[ok]] = True
         lab_e[idx[ok]] = lab2[ok]
 
-    pending = []  # (stream, row indices, labels) of full integrations in flight
+    pending = []  # (row indices, labels) of full integrations issued on the main stream
 
     def integrate_async(rows: torch.Tensor) -> None:
+        # on the main stream, so it overlaps the audit's stream rather than
+        # queueing behind it
         idx = torch.nonzero(rows).squeeze(1)
         if idx.numel() == 0:
             return
-        xs = x[idx]
-        st = torch.cuda.Stream(device=dev)
-        st.wait_stream(main)
-        with torch.cuda.stream(st):
-            lab_f = integrate_labels(solver, ode, xs, time_steady)
-        pending.append((st, idx, xs, lab_f))
+        pending.append((idx, integrate_labels(solver, ode, x[idx], time_steady)))
 
-    done_full = torch.zeros_like(routed)
+    done_full = early.clone()
     if evolve_t is not None:
-        evolve(routed)
-        first = routed & ~evolved & ~audit
+        evolve(routed & ~early)
+        mark("evolve")
+        first = routed & ~evolved & ~side_rows
         integrate_async(first)  # still below the gate: full span, now
+        mark("late1")
         done_full |= first
     else:
-        integrate_async(routed & ~audit)
+        integrate_async(routed & ~side_rows)
         done_full |= routed & ~audit
     main.wait_stream(side)
-    lab_audit.record_stream(main)
-    xa.record_stream(side)
-    full[aidx] = lab_audit
+    mark("audit")
+    lab_side.record_stream(main)
+    xs.record_stream(side)
+    full[sidx] = lab_side
+    lab_audit = full[aidx]
     bad_group = None
     distrusted = torch.zeros_like(routed)
     if group is not None and group_error_cap is not None:
@@ -377,7 +409,7 @@ def classify_gated(model: SurrogateMLP, solver, ode, x: torch.Tensor, gate: floa
         distrusted = bad_group[gl]
         if bool(bad_group.any()):
             if evolve_t is not None:
-                evolve(distrusted & ~routed)
+                evolve(distrusted & ~routed & ~early)
                 still_bad = group_rate(lab_e, evolved) > group_error_cap
                 distrusted = (bad_group & still_bad)[gl]
     group_routed = (bad_group[gl] & ~routed) if bad_group is not None else torch.zeros_like(routed)
@@ -393,18 +425,24 @@ def classify_gated(model: SurrogateMLP, solver, ode, x: torch.Tensor, gate: floa
         final = lab
         need_full = group_routed & ~audit & ~done_full
         audit_evolved_error = None
+    mark("group")
     integrate_async(need_full)
-    for st, idx, xs, lab_f in pending:
-        main.wait_stream(st)
-        lab_f.record_stream(main)
-        xs.record_stream(st)
+    mark("late2")
+    n_late = sum(int(idx.numel()) for idx, _ in pending)
+    for idx, lab_f in pending:
         full[idx] = lab_f
     integrated = need_full | done_full | audit
     final = torch.where(integrated, full, final)
+    mark("end")
+    if stage_times is not None:
+        torch.cuda.synchronize(dev)
+        for name, ev in marks[1:]:
+            stage_times[name] = marks[0][1].elapsed_time(ev)
+        stage_times["late_launches"] = len(pending)
     return GatedResult(labels=final, routed=routed, audit_idx=aidx, audit_surrogate=lab[aidx],
                        audit_integrated=full[aidx], integrated=integrated,
                        group_routed=group_routed, evolved=use_e & ~integrated,
-                       audit_evolved_error=audit_evolved_error)
+                       audit_evolved_error=audit_evolved_error, n_late=n_late)
 
 
 # ---- pendulum torque sweep (surrogate workload) ----------------------------
