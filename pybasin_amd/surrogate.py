@@ -288,19 +288,22 @@ def classify_gated(model: SurrogateMLP, solver, ode, x: torch.Tensor, gate: floa
     """Surrogate labels for confident rows, Dopri5 for the rest.
 
     A random audit sample (audit_frac of every group) is integrated over the
-    full span on a second stream while the network runs; a group whose
-    surrogate error on its confident audited rows exceeds group_error_cap is
-    distrusted (the surrogate's first answer is not used there), the other
-    groups keep the confident labels.
+    full span on a second stream while the network runs.  When it ends, a
+    group whose labels disagree with these integrations on more than
+    group_error_cap of its audited rows (rows that would have been
+    integrated anyway excluded) is integrated entirely.
 
     evolve_t: rows below the gate are first integrated to t = evolve_t only
     and classified again at the state they reached: the label is a property
     of the trajectory, and after the transient most states lie far from the
-    basin boundary.  Rows still below the gate are integrated over the full
-    span (on the main stream, while the audit finishes on its own).  Distrusted groups go
-    the same way, and a distrusted group whose evolved labels disagree with
-    its audit by more than group_error_cap is integrated entirely.  Without
-    evolve_t every routed row is integrated over the full span.
+    basin boundary.  The audit sample is evolved too, and a group whose first
+    answers disagree with the answers at the evolved states on more than
+    group_error_cap of its confident audited rows is treated as below the
+    gate (its confident rows are evolved as well), a decision available long
+    before the audit's full integrations end.  Rows still below the gate
+    after evolving are integrated over the full span on the main stream,
+    while the audit finishes.  Without evolve_t every below-gate row is
+    integrated over the full span.
 
     early_frac: the early_frac * n least confident rows are integrated over
     the full span in the audit's launch, so the rows a short integration
@@ -376,56 +379,62 @@ def classify_gated(model: SurrogateMLP, solver, ode, x: torch.Tensor, gate: floa
             return
         pending.append((idx, integrate_labels(solver, ode, x[idx], time_steady)))
 
+    has_groups = group is not None and group_error_cap is not None
+
+    def group_rate(pred, ref, valid):
+        # disagreement of pred with ref per group, over audited rows where valid
+        v = valid[aidx]
+        wrong = ((pred[aidx] != ref[aidx]) & v).double()
+        return torch.bincount(gl[aidx], weights=wrong, minlength=ng) / torch.bincount(
+            gl[aidx], weights=v.double(), minlength=ng).clamp_min(1)
+
     done_full = early.clone()
+    group_routed = torch.zeros_like(routed)
+    distrusted = torch.zeros_like(routed)
+    audit_evolved_error = None
     if evolve_t is not None:
-        evolve(routed & ~early)
+        # short integrations of the below-gate rows and of the audit sample;
+        # on the audit sample the first answer is checked against the answer
+        # at the evolved state (self-consistency, known long before the
+        # audit's full integrations end)
+        evolve((routed | audit) & ~early)
         mark("evolve")
-        first = routed & ~evolved & ~side_rows
+        if has_groups:
+            bad_group = group_rate(lab, lab_e, audit & ~routed & evolved) > group_error_cap
+            if bool(bad_group.any()):
+                group_routed = bad_group[gl] & ~routed
+                evolve(group_routed & ~audit & ~early)
+        routed = routed | group_routed
+        use_e = evolved & routed
+        final = torch.where(use_e, lab_e, lab)
+        first = routed & ~use_e & ~side_rows
         integrate_async(first)  # still below the gate: full span, now
-        mark("late1")
         done_full |= first
+        mark("late1")
     else:
+        final = lab
         integrate_async(routed & ~side_rows)
         done_full |= routed & ~audit
+        use_e = torch.zeros_like(routed)
     main.wait_stream(side)
     mark("audit")
     lab_side.record_stream(main)
     xs.record_stream(side)
     full[sidx] = lab_side
-    lab_audit = full[aidx]
-    bad_group = None
-    distrusted = torch.zeros_like(routed)
-    if group is not None and group_error_cap is not None:
-        ga = gl[aidx]
-
-        def group_rate(pred, valid):
-            # error per group over its audited rows where valid
-            v = valid[aidx]
-            wrong = ((pred[aidx] != lab_audit) & v).double()
-            return torch.bincount(ga, weights=wrong, minlength=ng) / torch.bincount(
-                ga, weights=v.double(), minlength=ng).clamp_min(1)
-
-        bad_group = group_rate(lab, ~routed) > group_error_cap
-        distrusted = bad_group[gl]
-        if bool(bad_group.any()):
-            if evolve_t is not None:
-                evolve(distrusted & ~routed & ~early)
-                still_bad = group_rate(lab_e, evolved) > group_error_cap
-                distrusted = (bad_group & still_bad)[gl]
-    group_routed = (bad_group[gl] & ~routed) if bad_group is not None else torch.zeros_like(routed)
-    routed = routed | group_routed
     if evolve_t is not None:
-        use_e = evolved & ~distrusted
-        final = torch.where(use_e, lab_e, lab)
-        need_full = ((routed & ~use_e) | distrusted) & ~audit & ~done_full
         ea = use_e & audit
         audit_evolved_error = float((lab_e[ea] != full[ea]).float().mean()) if bool(ea.any()) else None
-    else:
-        use_e = evolved
-        final = lab
-        need_full = group_routed & ~audit & ~done_full
-        audit_evolved_error = None
+    if has_groups:
+        # the audit proper: a group whose labels so far disagree with its full
+        # integrations on more than group_error_cap of the audited rows that
+        # were not integrated is integrated entirely
+        would_integrate = early | (routed & ~use_e)
+        failed = group_rate(final, full, ~would_integrate) > group_error_cap
+        distrusted = failed[gl]
+        group_routed = group_routed | (distrusted & ~routed)
+        routed = routed | distrusted
     mark("group")
+    need_full = distrusted & ~audit & ~done_full
     integrate_async(need_full)
     mark("late2")
     n_late = sum(int(idx.numel()) for idx, _ in pending)
