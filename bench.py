@@ -38,7 +38,7 @@ STATE = {"duffing": 2, "pendulum": 2, "lorenz": 3}
 # HBM bytes per dispatch of the Duffing 100k float32 kernel, from rocprofv3
 # FETCH_SIZE + WRITE_SIZE scaled by a plain-copy calibration
 # (profiles/dopri5_lane_duffing_100k_f32_pmc.md)
-TRAFFIC_BYTES = {("duffing", "f32", 100_000): 9.79e6}
+TRAFFIC_BYTES = {("duffing", "f32", 100_000): 9.73e6}
 
 DUFFING_TEMPLATES = [[-0.21, 0.02], [1.05, 0.77], [-0.67, 0.02], [-0.46, 0.30], [-0.43, 0.12]]
 
@@ -54,7 +54,8 @@ def parse():
     ap.add_argument("--max-lanes", type=int, default=0)
     ap.add_argument("--cpu-sample", type=int, default=100_000)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=["duffing", "surrogate", "rossler"], default="duffing",
+    ap.add_argument("--workload", choices=["duffing", "surrogate", "rossler", "mlp_infer"],
+                    default="duffing",
                     help="duffing: plain HIP Dopri5 on 100k Duffing ICs (default); surrogate: "
                          "pendulum torque sweep through the MLP gate with Dopri5 fallback")
     ap.add_argument("--gate", type=float, default=0.999)
@@ -252,6 +253,48 @@ def rossler_workload(args):
     print(json.dumps(out), flush=True)
 
 
+def mlp_infer_workload(args):
+    """The fused surrogate inference kernel alone on the torque-sweep rows
+    (4.43M (theta, omega, T) rows, random-init weights): rows/s and MFMA
+    roofline (133,120 multiply-adds per row, bf16 dense peak 2.5 PFLOP/s)."""
+    import numpy as np
+
+    from pybasin_amd.surrogate import SurrogateMLP, torque_sweep_ics
+
+    T = np.linspace(0.1, 1.0, 46)
+    counts = np.random.default_rng(args.seed).integers(10_000, 200_001, 46)
+    x, _ = torque_sweep_ics(T, counts, seed=args.seed)
+    n = x.shape[0]
+    model = SurrogateMLP(device="cuda:0", seed=args.seed)
+    stream = torch.cuda.current_stream()
+    for _ in range(args.warmup):
+        model.predict(x)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record(stream)
+    for _ in range(args.steps):
+        model.predict(x)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = 1e3 * (time.perf_counter() - t0) / args.steps
+    kernel_ms = e0.elapsed_time(e1) / args.steps
+    macs = 6 * 256 + 256 * 256 * 2 + 256 * 2  # multiply-adds per row (unpadded layers)
+    flop = 2.0 * macs * n
+    out = {
+        "metric": "surrogate inference rows/sec", "value": round(n / (ms * 1e-3), 1),
+        "unit": "rows/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(ms, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "bf16", "data": "synthetic: torque-sweep rows, random-init weights",
+        "config": {"workload": "mlp_infer_torque_sweep", "rows": n, "net": "6(3)-256-256-256-2 SiLU"},
+        "roofline": {"bound": "mfma", "achieved": round(flop / (kernel_ms * 1e-3) / 1e12, 1),
+                     "peak": 2500.0, "unit": "TFLOP/s",
+                     "frac": round(flop / (kernel_ms * 1e-3) / 2500e12, 4), "traffic": None,
+                     "kernel": "bh::mlp::mlp_infer_kernel", "kernel_ms": round(kernel_ms, 3)},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     args = parse()
     if args.copy_calibration:
@@ -264,6 +307,8 @@ def main():
 
     if args.workload == "surrogate":
         return surrogate_workload(args)
+    if args.workload == "mlp_infer":
+        return mlp_infer_workload(args)
     if args.workload == "rossler":
         return rossler_workload(args)
 
