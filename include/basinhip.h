@@ -187,6 +187,46 @@ int32_t bh_mlp_infer(const float *x, int64_t M, const void *w_packed, const floa
 int32_t bh_mlp_classify(const float *Z, const float *bias, int64_t M, int32_t *label,
                         float *conf, void *stream);
 
+/* ---- label layer (replaces the sklearn calls of the reference's
+   HDBSCANClusterer, src/pybasin/predictors/hdbscan_clusterer.py:16-166, and
+   the 1-NN template classifiers of the case studies) --------------------- */
+
+/* k nearest reference rows of every query row (Euclidean, double; rows of F
+   <= 16 features, row-major): out_d (nq, k) distances ascending, out_i (nq, k)
+   reference indices; equal distances keep the lower index.  1 <= k <= 32.
+   sklearn NearestNeighbors.kneighbors / KNeighborsClassifier (brute force). */
+int32_t bh_knn(const double *Q, int64_t nq, const double *R, int64_t nr, int32_t F, int32_t k,
+               double *out_d, int64_t *out_i, void *stream);
+
+/* device workspace bh_hdbscan_mst needs for n rows */
+int64_t bh_hdbscan_workspace_bytes(int64_t n);
+
+/* Minimum spanning tree of the HDBSCAN mutual-reachability graph
+   max(core_a, core_b, d(a, b)) of X (n, F) given the core distances (the
+   distance to the min_samples-th nearest row counting the row itself, e.g.
+   column min_samples - 1 of bh_knn(X, X)): n - 1 edges (out_a, out_b,
+   out_w) in device memory.  Boruvka rounds; SYNCHRONOUS (the host merges
+   components between rounds).  Replaces the spanning tree inside sklearn
+   HDBSCAN.fit (mst_from_data_matrix, sklearn/cluster/_hdbscan/_linkage.pyx). */
+int32_t bh_hdbscan_mst(const double *X, int64_t n, int32_t F, const double *core, int64_t *out_a,
+                       int64_t *out_b, double *out_w, void *workspace, int64_t workspace_bytes,
+                       void *stream);
+
+/* HOST arrays: flat HDBSCAN labels from the spanning tree -- single-linkage
+   hierarchy, condensed with min_cluster_size, excess-of-mass selection
+   without the root (allow_single_cluster = False, cluster_selection_epsilon
+   = 0): labels (n) int32, -1 noise, clusters 0.. in increasing condensed-tree
+   id; *n_clusters the number selected. */
+int32_t bh_hdbscan_labels(const int64_t *edge_a, const int64_t *edge_b, const double *edge_w,
+                          int64_t n, int32_t min_cluster_size, int32_t *labels,
+                          int32_t *n_clusters);
+
+/* Silhouette value of every row (rows sorted by label 0..n_labels-1, label
+   c occupying [starts[c], starts[c+1])): s = (b - a) / max(a, b), 0 for a
+   label of one row.  sklearn.metrics.silhouette_score = mean of s_out. */
+int32_t bh_silhouette(const double *X_sorted, int64_t n, int32_t F, const int32_t *labels_sorted,
+                      const int64_t *starts, int32_t n_labels, double *s_out, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

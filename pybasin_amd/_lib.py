@@ -25,7 +25,8 @@ BH_WORKSPACE_BYTES = 256
 EXPORTS = ("bh_version", "bh_last_error", "bh_solve_batch", "bh_solve_batch_peaks",
            "bh_solve_rossler", "bh_gemm_bf16",
            "bh_mlp_bias_act", "bh_mlp_silu_bwd", "bh_mlp_softmax_ce", "bh_mlp_colsum",
-           "bh_mlp_adam", "bh_mlp_prep_input", "bh_mlp_classify", "bh_mlp_infer")
+           "bh_mlp_adam", "bh_mlp_prep_input", "bh_mlp_classify", "bh_mlp_infer", "bh_knn",
+           "bh_hdbscan_workspace_bytes", "bh_hdbscan_mst", "bh_hdbscan_labels", "bh_silhouette")
 
 _lock = threading.Lock()
 _lib: ctypes.CDLL | None = None
@@ -48,10 +49,11 @@ def _declare(L: ctypes.CDLL) -> None:
     L.bh_solve_batch_peaks.argtypes = [i32, i32, vp, i64, i32, vp, i32, i64, f64, vp, i32, f64,
                                        f64, i64, u32, f64, i32, vp, vp, vp, vp, vp, i32, vp, vp,
                                        vp, i64, i32, vp]
-    if True:
-        L.bh_solve_rossler.restype = i32
-        L.bh_solve_rossler.argtypes = [i32, vp, i64, vp, i64, vp, f64, f64, f64, f64, i64, u32,
-                                       f64, vp, vp, vp, vp, i64, i32, vp]
+    L.bh_solve_rossler.restype = i32
+    L.bh_solve_rossler.argtypes = [i32, vp, i64, vp, i64, vp, f64, f64, f64, f64, i64, u32,
+                                   f64, vp, vp, vp, vp, i64, i32, vp]
+    L.bh_hdbscan_workspace_bytes.restype = i64
+    L.bh_hdbscan_workspace_bytes.argtypes = [i64]
     f32 = ctypes.c_float
     sigs = {
         "bh_gemm_bf16": [vp, i64, i64, vp, i64, i64, vp, i64, i64, i64, i64, i32, vp],
@@ -63,6 +65,10 @@ def _declare(L: ctypes.CDLL) -> None:
         "bh_mlp_prep_input": [vp, i64, f32, f32, f32, f32, f32, f32, vp, vp],
         "bh_mlp_classify": [vp, vp, i64, vp, vp, vp],
         "bh_mlp_infer": [vp, i64, vp, vp, f32, f32, f32, f32, f32, f32, vp, vp, vp],
+        "bh_knn": [vp, i64, vp, i64, i32, i32, vp, vp, vp],
+        "bh_hdbscan_mst": [vp, i64, i32, vp, vp, vp, vp, vp, i64, vp],
+        "bh_hdbscan_labels": [vp, vp, vp, i64, i32, vp, vp],
+        "bh_silhouette": [vp, i64, i32, vp, vp, i32, vp, vp],
     }
     for name, args in sigs.items():
         fn = getattr(L, name)

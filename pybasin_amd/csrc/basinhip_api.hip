@@ -8,6 +8,7 @@
 #include "dopri5_lane.hip"
 #include "rossler_tile.hip"
 #include "mlp.hip"
+#include "cluster.hip"
 
 namespace {
 
@@ -160,6 +161,68 @@ int32_t solve_batch_impl(int32_t system_id, int32_t dtype, const void *y0s, int6
     default: return fail(-1, "unknown system id %d", system_id);
     }
 }
+
+// feature slots for F features: 2, 4, 8 or 16
+inline int feature_slots(int F) { return F <= 2 ? 2 : F <= 4 ? 4 : F <= 8 ? 8 : 16; }
+
+template <int K, int FC>
+int32_t launch_knn_fc(const double *Q, int64_t nq, const double *R, int64_t nr, int32_t F,
+                      int32_t k, double *out_d, int64_t *out_i, hipStream_t s) {
+    namespace C = bh::clu;
+    const int64_t blocks = (nq + C::TILE - 1) / C::TILE;
+    hipLaunchKernelGGL((C::knn_kernel<K, FC>), dim3((unsigned)blocks), dim3(C::TILE), 0, s, Q, nq,
+                       R, nr, F, k, out_d, out_i);
+    BH_HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
+template <int K>
+int32_t launch_knn(const double *Q, int64_t nq, const double *R, int64_t nr, int32_t F, int32_t k,
+                   double *out_d, int64_t *out_i, hipStream_t s) {
+    switch (feature_slots(F)) {
+    case 2: return launch_knn_fc<K, 2>(Q, nq, R, nr, F, k, out_d, out_i, s);
+    case 4: return launch_knn_fc<K, 4>(Q, nq, R, nr, F, k, out_d, out_i, s);
+    case 8: return launch_knn_fc<K, 8>(Q, nq, R, nr, F, k, out_d, out_i, s);
+    default: return launch_knn_fc<K, 16>(Q, nq, R, nr, F, k, out_d, out_i, s);
+    }
+}
+
+template <int FC>
+int32_t boruvka_round(const double *X, int64_t n, int32_t F, const double *core_sq,
+                      const int32_t *comp, double *best_key, int64_t *best_j, hipStream_t s) {
+    namespace C = bh::clu;
+    hipLaunchKernelGGL(C::boruvka_kernel<FC>, dim3((unsigned)((n + C::TILE - 1) / C::TILE)),
+                       dim3(C::TILE), 0, s, X, n, F, core_sq, comp, best_key, best_j);
+    BH_HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
+template <int FC>
+int32_t edge_weights(const double *X, int32_t F, const double *core, const int64_t *ea,
+                     const int64_t *eb, int64_t m, double *w, hipStream_t s) {
+    hipLaunchKernelGGL(bh::clu::edge_weight_kernel<FC>, dim3((unsigned)((m + 255) / 256)), dim3(256),
+                       0, s, X, F, core, ea, eb, m, w);
+    BH_HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
+template <int FC>
+int32_t silhouette_launch(const double *X, int64_t n, int32_t F, const int32_t *lab,
+                          const int64_t *starts, int32_t C_, double *s_out, hipStream_t s) {
+    namespace C = bh::clu;
+    hipLaunchKernelGGL(C::silhouette_kernel<FC>, dim3((unsigned)((n + C::TILE - 1) / C::TILE)),
+                       dim3(C::TILE), 0, s, X, n, F, lab, starts, C_, s_out);
+    BH_HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
+#define BH_BY_SLOTS(F, call)                      \
+    switch (feature_slots(F)) {                   \
+    case 2: { constexpr int FCV = 2; rc = call; } break; \
+    case 4: { constexpr int FCV = 4; rc = call; } break; \
+    case 8: { constexpr int FCV = 8; rc = call; } break; \
+    default: { constexpr int FCV = 16; rc = call; } break; \
+    }
 
 }  // namespace
 
@@ -390,6 +453,138 @@ int32_t bh_mlp_infer(const float *x, int64_t M, const void *w_packed, const floa
                        reinterpret_cast<hipStream_t>(stream), a);
     BH_HIP_CHECK(hipGetLastError());
     return 0;
+}
+
+
+// ---- label layer ------------------------------------------------------------
+
+int32_t bh_knn(const double *Q, int64_t nq, const double *R, int64_t nr, int32_t F, int32_t k,
+               double *out_d, int64_t *out_i, void *stream) {
+    g_err[0] = 0;
+    namespace C = bh::clu;
+    if (nq < 0 || nr < 1) return fail(-5, "bh_knn: nq >= 0 and nr >= 1 required");
+    if (F < 1 || F > C::MAXF) return fail(-5, "bh_knn: 1 <= F <= %d, got %d", C::MAXF, F);
+    if (k < 1 || k > C::MAXK) return fail(-5, "bh_knn: 1 <= k <= %d, got %d", C::MAXK, k);
+    if (nq == 0) return 0;
+    if (!Q || !R || !out_d || !out_i) return fail(-5, "bh_knn: null argument");
+    if ((nq + C::TILE - 1) / C::TILE > 2147483647LL) return fail(-5, "bh_knn: nq too large");
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    if (k <= 8) return launch_knn<8>(Q, nq, R, nr, F, k, out_d, out_i, s);
+    if (k <= 16) return launch_knn<16>(Q, nq, R, nr, F, k, out_d, out_i, s);
+    return launch_knn<32>(Q, nq, R, nr, F, k, out_d, out_i, s);
+}
+
+int64_t bh_hdbscan_workspace_bytes(int64_t n) { return n * 32 + 1024; }
+
+int32_t bh_hdbscan_mst(const double *X, int64_t n, int32_t F, const double *core, int64_t *out_a,
+                       int64_t *out_b, double *out_w, void *workspace, int64_t workspace_bytes,
+                       void *stream) {
+    g_err[0] = 0;
+    namespace C = bh::clu;
+    if (n < 2) return fail(-5, "bh_hdbscan_mst: n >= 2 required");
+    if (F < 1 || F > C::MAXF) return fail(-5, "bh_hdbscan_mst: 1 <= F <= %d", C::MAXF);
+    if (!X || !core || !out_a || !out_b || !out_w || !workspace)
+        return fail(-5, "bh_hdbscan_mst: null argument");
+    if (workspace_bytes < bh_hdbscan_workspace_bytes(n))
+        return fail(-5, "bh_hdbscan_mst: workspace of %lld bytes required",
+                    (long long)bh_hdbscan_workspace_bytes(n));
+    if (n > 2147483647LL) return fail(-5, "bh_hdbscan_mst: n too large");
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    char *ws = reinterpret_cast<char *>(workspace);
+    double *core_sq = reinterpret_cast<double *>(ws);
+    double *best_key = core_sq + n;
+    int64_t *best_j = reinterpret_cast<int64_t *>(best_key + n);
+    int32_t *comp_d = reinterpret_cast<int32_t *>(best_j + n);
+    // squared core distances for the comparison key
+    std::vector<double> core_h(n);
+    BH_HIP_CHECK(hipMemcpyAsync(core_h.data(), core, n * sizeof(double), hipMemcpyDeviceToHost, s));
+    BH_HIP_CHECK(hipStreamSynchronize(s));
+    std::vector<double> csq(n);
+    for (int64_t i = 0; i < n; i++) csq[i] = core_h[i] * core_h[i];
+    BH_HIP_CHECK(hipMemcpyAsync(core_sq, csq.data(), n * sizeof(double), hipMemcpyHostToDevice, s));
+    C::UnionFind uf(n);
+    std::vector<int32_t> comp(n);
+    std::iota(comp.begin(), comp.end(), 0);
+    std::vector<double> bk(n);
+    std::vector<int64_t> bj(n), ea, eb;
+    ea.reserve(n - 1);
+    eb.reserve(n - 1);
+    int64_t n_comp = n;
+    while (n_comp > 1) {
+        BH_HIP_CHECK(hipMemcpyAsync(comp_d, comp.data(), n * sizeof(int32_t), hipMemcpyHostToDevice, s));
+        int32_t rc = 0;
+        BH_BY_SLOTS(F, (boruvka_round<FCV>(X, n, F, core_sq, comp_d, best_key, best_j, s)));
+        if (rc) return rc;
+        BH_HIP_CHECK(hipMemcpyAsync(bk.data(), best_key, n * sizeof(double), hipMemcpyDeviceToHost, s));
+        BH_HIP_CHECK(hipMemcpyAsync(bj.data(), best_j, n * sizeof(int64_t), hipMemcpyDeviceToHost, s));
+        BH_HIP_CHECK(hipStreamSynchronize(s));
+        // cheapest edge per component (ties: lower key, then lower point, then lower partner)
+        std::vector<int64_t> pick(n, -1);
+        for (int64_t i = 0; i < n; i++) {
+            if (bj[i] < 0) continue;
+            const int32_t c = comp[i];
+            const int64_t p = pick[c];
+            if (p < 0 || bk[i] < bk[p] || (bk[i] == bk[p] && (i < p || (i == p && bj[i] < bj[p]))))
+                pick[c] = i;
+        }
+        int64_t merged = 0;
+        for (int64_t c = 0; c < n; c++) {
+            const int64_t i = pick[c];
+            if (i < 0) continue;
+            const int64_t a = uf.find(i), b = uf.find(bj[i]);
+            if (a == b) continue;  // the partner component chose the same edge
+            int64_t big = a, small = b;
+            if (uf.size[big] < uf.size[small]) std::swap(big, small);
+            uf.parent[small] = big;
+            uf.size[big] += uf.size[small];
+            ea.push_back(i);
+            eb.push_back(bj[i]);
+            merged++;
+        }
+        if (merged == 0) return fail(-3, "bh_hdbscan_mst: no edge between components");
+        n_comp -= merged;
+        for (int64_t i = 0; i < n; i++) comp[i] = (int32_t)uf.find(i);
+    }
+    const int64_t m = (int64_t)ea.size();
+    BH_HIP_CHECK(hipMemcpyAsync(out_a, ea.data(), m * sizeof(int64_t), hipMemcpyHostToDevice, s));
+    BH_HIP_CHECK(hipMemcpyAsync(out_b, eb.data(), m * sizeof(int64_t), hipMemcpyHostToDevice, s));
+    int32_t rc = 0;
+    BH_BY_SLOTS(F, (edge_weights<FCV>(X, F, core, out_a, out_b, m, out_w, s)));
+    if (rc) return rc;
+    BH_HIP_CHECK(hipStreamSynchronize(s));
+    return 0;
+}
+
+int32_t bh_hdbscan_labels(const int64_t *edge_a, const int64_t *edge_b, const double *edge_w,
+                          int64_t n, int32_t min_cluster_size, int32_t *labels,
+                          int32_t *n_clusters) {
+    g_err[0] = 0;
+    namespace C = bh::clu;
+    if (n < 2) return fail(-5, "bh_hdbscan_labels: n >= 2 required");
+    if (min_cluster_size < 2) return fail(-5, "bh_hdbscan_labels: min_cluster_size >= 2 required");
+    if (!edge_a || !edge_b || !edge_w || !labels) return fail(-5, "bh_hdbscan_labels: null argument");
+    for (int64_t e = 0; e < n - 1; e++)
+        if (edge_a[e] < 0 || edge_a[e] >= n || edge_b[e] < 0 || edge_b[e] >= n || !(edge_w[e] >= 0.0))
+            return fail(-5, "bh_hdbscan_labels: edge %lld out of range", (long long)e);
+    const C::Hierarchy h = C::single_linkage(n, edge_a, edge_b, edge_w);
+    const C::Condensed c = C::condense(h, n, min_cluster_size);
+    const int32_t k = C::select_and_label(c, n, labels);
+    if (n_clusters) *n_clusters = k;
+    return 0;
+}
+
+int32_t bh_silhouette(const double *X_sorted, int64_t n, int32_t F, const int32_t *labels_sorted,
+                      const int64_t *starts, int32_t n_labels, double *s_out, void *stream) {
+    g_err[0] = 0;
+    namespace C = bh::clu;
+    if (n < 1 || F < 1 || F > C::MAXF || n_labels < 1)
+        return fail(-5, "bh_silhouette: n >= 1, 1 <= F <= %d, n_labels >= 1 required", C::MAXF);
+    if (!X_sorted || !labels_sorted || !starts || !s_out) return fail(-5, "bh_silhouette: null argument");
+    if ((n + C::TILE - 1) / C::TILE > 2147483647LL) return fail(-5, "bh_silhouette: n too large");
+    int32_t rc = 0;
+    BH_BY_SLOTS(F, (silhouette_launch<FCV>(X_sorted, n, F, labels_sorted, starts, n_labels, s_out,
+                                            reinterpret_cast<hipStream_t>(stream))));
+    return rc;
 }
 
 }  // extern "C"

@@ -65,6 +65,11 @@ class KNNTemplateClassifier:
 
     def predict_index(self, X: torch.Tensor) -> torch.Tensor:
         assert self._X is not None, "fit() first"
+        if X.device.type == "cuda":
+            from .cluster import knn  # HIP kernel (bh_knn)
+
+            _, idx = knn(X.to(torch.float64).contiguous(), self._X.to(X.device).contiguous(), 1)
+            return idx[:, 0]
         d = torch.cdist(X.to(torch.float64), self._X.to(X.device))
         return torch.argmin(d, dim=1)
 
@@ -128,16 +133,26 @@ class BasinStabilityEstimator:
     """
 
     def __init__(self, ode_system, sampler: UniformRandomSampler, n: int, solver,
-                 feature_extractor, template_y0, template_labels,
+                 feature_extractor, template_y0=None, template_labels=None,
                  keep_trajectories: bool = False, detect_unbounded: bool = True,
-                 compute_orbit_data: list[int] | bool = False):
+                 compute_orbit_data: list[int] | bool = False, predictor=None):
         self.ode_system = ode_system
         self.sampler = sampler
         self.n = int(n)
         self.solver = solver
         self.feature_extractor = feature_extractor
-        self.template_y0 = torch.as_tensor(template_y0, dtype=torch.float32)
-        self.classifier = KNNTemplateClassifier(template_labels)
+        # supervised (templates + 1-NN, or a classifier with fit/predict) or
+        # unsupervised (a clusterer with fit_predict, e.g. HipHDBSCANClusterer)
+        if template_y0 is None:
+            if predictor is None or not hasattr(predictor, "fit_predict"):
+                raise ValueError("without templates the predictor must be a clusterer (fit_predict)")
+            self.template_y0 = torch.zeros((0, 0), dtype=torch.float32)
+            self.template_labels = None
+        else:
+            self.template_y0 = torch.as_tensor(template_y0, dtype=torch.float32)
+            self.template_labels = list(template_labels)
+        self.predictor = predictor
+        self.classifier = KNNTemplateClassifier(self.template_labels or [])
         self.keep_trajectories = keep_trajectories
         self.detect_unbounded = detect_unbounded
         self.compute_orbit_data = compute_orbit_data
@@ -168,6 +183,8 @@ class BasinStabilityEstimator:
         if P > 1:
             main = torch.repeat_interleave(main, P, dim=0)
         tmpl = self.template_y0.to(dev, y0.dtype)
+        if tmpl.numel() == 0:
+            tmpl = tmpl.reshape(0, main.shape[1])
         batch = torch.cat([main, tmpl])
         params = None
         if grid is not None:
@@ -214,7 +231,11 @@ class BasinStabilityEstimator:
                                                  time_steady=getattr(fx, "time_steady", None))
         feats = feats.cpu() if feats.device.type != "cpu" else feats
         n_main = B * P
-        self.classifier.fit(feats[n_main:])
+        supervised = self.template_labels is not None
+        if supervised and self.predictor is not None:
+            self.predictor.fit(feats[n_main:].numpy(), np.asarray(self.template_labels))
+        elif supervised:
+            self.classifier.fit(feats[n_main:])
         results = []
         for p, label in enumerate(study_labels):
             f_p = feats[:n_main][p::P]
@@ -222,7 +243,14 @@ class BasinStabilityEstimator:
             labels = np.empty(B, dtype=object)
             labels[ub.numpy()] = "unbounded"
             if int(ub.sum()) < B:
-                labels[(~ub).numpy()] = self.classifier.predict(f_p[~ub])
+                fb = f_p[~ub]
+                if not supervised:
+                    pred = self.predictor.fit_predict(fb.numpy())  # per group, as the reference
+                elif self.predictor is not None:
+                    pred = self.predictor.predict(fb.numpy())
+                else:
+                    pred = self.classifier.predict(fb.to(dev) if dev.type == "cuda" else fb)
+                labels[(~ub).numpy()] = np.asarray(pred, dtype=object)
             bs = compute_bs(labels)
             names = list(getattr(fx, "feature_names", []))
             ex = dict(extra, group=p)
