@@ -213,19 +213,22 @@ int32_t bh_hdbscan_mst(const double *X, int64_t n, int32_t F, const double *core
                        void *stream);
 
 /* HOST arrays: flat HDBSCAN labels from the spanning tree -- single-linkage
-   hierarchy, condensed with min_cluster_size, excess-of-mass selection
-   without the root (allow_single_cluster = False, cluster_selection_epsilon
-   = 0): labels (n) int32, -1 noise, clusters 0.. in increasing condensed-tree
-   id; *n_clusters the number selected. */
+   hierarchy (built once), condensed with each of the n_sizes
+   min_cluster_sizes, excess-of-mass selection without the root
+   (allow_single_cluster = False, cluster_selection_epsilon = 0): labels
+   (n_sizes, n) int32, -1 noise, clusters 0.. in increasing condensed-tree
+   id; n_clusters (n_sizes) the numbers selected (or NULL). */
 int32_t bh_hdbscan_labels(const int64_t *edge_a, const int64_t *edge_b, const double *edge_w,
-                          int64_t n, int32_t min_cluster_size, int32_t *labels,
-                          int32_t *n_clusters);
+                          int64_t n, const int32_t *min_cluster_sizes, int32_t n_sizes,
+                          int32_t *labels, int32_t *n_clusters);
 
 /* Silhouette value of every row (rows sorted by label 0..n_labels-1, label
    c occupying [starts[c], starts[c+1])): s = (b - a) / max(a, b), 0 for a
-   label of one row.  sklearn.metrics.silhouette_score = mean of s_out. */
+   label of one row.  sklearn.metrics.silhouette_score = mean of s_out.
+   sums: (n_labels, n) float64 device scratch for the per-label distance sums. */
 int32_t bh_silhouette(const double *X_sorted, int64_t n, int32_t F, const int32_t *labels_sorted,
-                      const int64_t *starts, int32_t n_labels, double *s_out, void *stream);
+                      const int64_t *starts, int32_t n_labels, double *sums, double *s_out,
+                      void *stream);
 
 #ifdef __cplusplus
 }

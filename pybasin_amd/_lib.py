@@ -67,8 +67,8 @@ def _declare(L: ctypes.CDLL) -> None:
         "bh_mlp_infer": [vp, i64, vp, vp, f32, f32, f32, f32, f32, f32, vp, vp, vp],
         "bh_knn": [vp, i64, vp, i64, i32, i32, vp, vp, vp],
         "bh_hdbscan_mst": [vp, i64, i32, vp, vp, vp, vp, vp, i64, vp],
-        "bh_hdbscan_labels": [vp, vp, vp, i64, i32, vp, vp],
-        "bh_silhouette": [vp, i64, i32, vp, vp, i32, vp, vp],
+        "bh_hdbscan_labels": [vp, vp, vp, i64, vp, i32, vp, vp],
+        "bh_silhouette": [vp, i64, i32, vp, vp, i32, vp, vp, vp],
     }
     for name, args in sigs.items():
         fn = getattr(L, name)

@@ -75,17 +75,20 @@ def mutual_reachability_mst(X: torch.Tensor, core: torch.Tensor):
 
 
 def labels_from_mst(a: np.ndarray, b: np.ndarray, w: np.ndarray, n: int,
-                    min_cluster_size: int) -> np.ndarray:
-    """Flat HDBSCAN labels (-1 noise) from a spanning tree given on the host."""
+                    min_cluster_size) -> np.ndarray:
+    """Flat HDBSCAN labels (-1 noise) from a spanning tree given on the host;
+    min_cluster_size may be a list (one label row per size, one hierarchy)."""
     a = np.ascontiguousarray(a, dtype=np.int64)
     b = np.ascontiguousarray(b, dtype=np.int64)
     w = np.ascontiguousarray(w, dtype=np.float64)
-    lab = np.empty(n, dtype=np.int32)
-    nc = np.zeros(1, dtype=np.int32)
+    sizes = np.ascontiguousarray(np.atleast_1d(min_cluster_size), dtype=np.int32)
+    lab = np.empty((len(sizes), n), dtype=np.int32)
+    nc = np.zeros(len(sizes), dtype=np.int32)
     _lib.check(_lib.lib().bh_hdbscan_labels(a.ctypes.data, b.ctypes.data, w.ctypes.data, n,
-                                            int(min_cluster_size), lab.ctypes.data, nc.ctypes.data),
-               "bh_hdbscan_labels")
-    return lab.astype(np.int64)
+                                            sizes.ctypes.data, len(sizes), lab.ctypes.data,
+                                            nc.ctypes.data), "bh_hdbscan_labels")
+    lab = lab.astype(np.int64)
+    return lab if np.ndim(min_cluster_size) else lab[0]
 
 
 def silhouette_score(X: torch.Tensor, labels) -> float:
@@ -102,11 +105,12 @@ def silhouette_score(X: torch.Tensor, labels) -> float:
     starts = torch.zeros(len(uniq) + 1, dtype=torch.int64, device=X.device)
     starts[1:] = torch.cumsum(counts, 0)
     s = torch.empty((X.shape[0],), dtype=torch.float64, device=X.device)
+    sums = torch.empty((len(uniq), X.shape[0]), dtype=torch.float64, device=X.device)
     st = torch.cuda.current_stream(X.device)
     with torch.cuda.device(X.device):
         _lib.check(_lib.lib().bh_silhouette(Xs.data_ptr(), X.shape[0], X.shape[1], ls.data_ptr(),
-                                            starts.data_ptr(), len(uniq), s.data_ptr(),
-                                            st.cuda_stream), "bh_silhouette")
+                                            starts.data_ptr(), len(uniq), sums.data_ptr(),
+                                            s.data_ptr(), st.cuda_stream), "bh_silhouette")
     return float(s.mean())
 
 
@@ -145,11 +149,24 @@ class HipHDBSCANClusterer:
                  max(100, int(0.03 * n)), max(150, int(0.05 * n))]
         best_score, best = -1.0, sizes[0]
         self.scores_ = {}
+        # candidates sharing min_samples share a tree: label them together
+        by_ms: dict[int, list[int]] = {}
         for size in sizes:
-            lab = self._hdbscan(X, size, min(10, size // 5))
+            by_ms.setdefault(min(10, size // 5), []).append(size)
+        labels = {}
+        for ms, group in by_ms.items():
+            a, b, w = self._tree(X, ms)
+            for size, lab in zip(group, labels_from_mst(a, b, w, X.shape[0], group)):
+                labels[size] = lab
+        seen: dict[bytes, float] = {}  # candidate sizes often give the same clustering
+        for size in sizes:
+            lab = labels[size]
             keep = lab != -1
             if len(np.unique(lab[keep])) >= 2 and keep.sum() > 1:
-                score = silhouette_score(X[torch.as_tensor(keep, device=X.device)], lab[keep])
+                key = lab.tobytes()
+                if key not in seen:
+                    seen[key] = silhouette_score(X[torch.as_tensor(keep, device=X.device)], lab[keep])
+                score = seen[key]
                 self.scores_[size] = score
                 if score > best_score:
                     best_score, best = score, size

@@ -191,8 +191,11 @@ template <int FC>
 int32_t boruvka_round(const double *X, int64_t n, int32_t F, const double *core_sq,
                       const int32_t *comp, double *best_key, int64_t *best_j, hipStream_t s) {
     namespace C = bh::clu;
-    hipLaunchKernelGGL(C::boruvka_kernel<FC>, dim3((unsigned)((n + C::TILE - 1) / C::TILE)),
-                       dim3(C::TILE), 0, s, X, n, F, core_sq, comp, best_key, best_j);
+    int64_t chunk = (n + C::MST_CHUNKS - 1) / C::MST_CHUNKS;
+    chunk = (chunk + C::TILE - 1) / C::TILE * C::TILE;
+    hipLaunchKernelGGL(C::boruvka_kernel<FC>,
+                       dim3((unsigned)((n + C::TILE - 1) / C::TILE), C::MST_CHUNKS), dim3(C::TILE), 0,
+                       s, X, n, F, core_sq, comp, chunk, best_key, best_j);
     BH_HIP_CHECK(hipGetLastError());
     return 0;
 }
@@ -208,10 +211,15 @@ int32_t edge_weights(const double *X, int32_t F, const double *core, const int64
 
 template <int FC>
 int32_t silhouette_launch(const double *X, int64_t n, int32_t F, const int32_t *lab,
-                          const int64_t *starts, int32_t C_, double *s_out, hipStream_t s) {
+                          const int64_t *starts, int32_t C_, double *sums, double *s_out,
+                          hipStream_t s) {
     namespace C = bh::clu;
-    hipLaunchKernelGGL(C::silhouette_kernel<FC>, dim3((unsigned)((n + C::TILE - 1) / C::TILE)),
-                       dim3(C::TILE), 0, s, X, n, F, lab, starts, C_, s_out);
+    hipLaunchKernelGGL(C::silhouette_sums_kernel<FC>,
+                       dim3((unsigned)((n + C::TILE - 1) / C::TILE), (unsigned)C_), dim3(C::TILE), 0,
+                       s, X, n, F, starts, sums);
+    BH_HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(C::silhouette_value_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
+                       sums, n, lab, starts, C_, s_out);
     BH_HIP_CHECK(hipGetLastError());
     return 0;
 }
@@ -474,7 +482,9 @@ int32_t bh_knn(const double *Q, int64_t nq, const double *R, int64_t nr, int32_t
     return launch_knn<32>(Q, nq, R, nr, F, k, out_d, out_i, s);
 }
 
-int64_t bh_hdbscan_workspace_bytes(int64_t n) { return n * 32 + 1024; }
+int64_t bh_hdbscan_workspace_bytes(int64_t n) {
+    return n * (8 + 4 + 16 * bh::clu::MST_CHUNKS) + 1024;
+}
 
 int32_t bh_hdbscan_mst(const double *X, int64_t n, int32_t F, const double *core, int64_t *out_a,
                        int64_t *out_b, double *out_w, void *workspace, int64_t workspace_bytes,
@@ -491,10 +501,11 @@ int32_t bh_hdbscan_mst(const double *X, int64_t n, int32_t F, const double *core
     if (n > 2147483647LL) return fail(-5, "bh_hdbscan_mst: n too large");
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     char *ws = reinterpret_cast<char *>(workspace);
+    const int64_t nch = C::MST_CHUNKS;
     double *core_sq = reinterpret_cast<double *>(ws);
-    double *best_key = core_sq + n;
-    int64_t *best_j = reinterpret_cast<int64_t *>(best_key + n);
-    int32_t *comp_d = reinterpret_cast<int32_t *>(best_j + n);
+    double *best_key = core_sq + n;                                   // (chunks, n)
+    int64_t *best_j = reinterpret_cast<int64_t *>(best_key + nch * n);  // (chunks, n)
+    int32_t *comp_d = reinterpret_cast<int32_t *>(best_j + nch * n);
     // squared core distances for the comparison key
     std::vector<double> core_h(n);
     BH_HIP_CHECK(hipMemcpyAsync(core_h.data(), core, n * sizeof(double), hipMemcpyDeviceToHost, s));
@@ -505,8 +516,8 @@ int32_t bh_hdbscan_mst(const double *X, int64_t n, int32_t F, const double *core
     C::UnionFind uf(n);
     std::vector<int32_t> comp(n);
     std::iota(comp.begin(), comp.end(), 0);
-    std::vector<double> bk(n);
-    std::vector<int64_t> bj(n), ea, eb;
+    std::vector<double> bkc(nch * n), bk(n);
+    std::vector<int64_t> bjc(nch * n), bj(n), ea, eb;
     ea.reserve(n - 1);
     eb.reserve(n - 1);
     int64_t n_comp = n;
@@ -515,9 +526,26 @@ int32_t bh_hdbscan_mst(const double *X, int64_t n, int32_t F, const double *core
         int32_t rc = 0;
         BH_BY_SLOTS(F, (boruvka_round<FCV>(X, n, F, core_sq, comp_d, best_key, best_j, s)));
         if (rc) return rc;
-        BH_HIP_CHECK(hipMemcpyAsync(bk.data(), best_key, n * sizeof(double), hipMemcpyDeviceToHost, s));
-        BH_HIP_CHECK(hipMemcpyAsync(bj.data(), best_j, n * sizeof(int64_t), hipMemcpyDeviceToHost, s));
+        BH_HIP_CHECK(hipMemcpyAsync(bkc.data(), best_key, nch * n * sizeof(double),
+                                    hipMemcpyDeviceToHost, s));
+        BH_HIP_CHECK(hipMemcpyAsync(bjc.data(), best_j, nch * n * sizeof(int64_t),
+                                    hipMemcpyDeviceToHost, s));
         BH_HIP_CHECK(hipStreamSynchronize(s));
+        // best over chunks (chunks hold increasing row ranges: the first of
+        // equal keys has the lower partner index)
+        for (int64_t i = 0; i < n; i++) {
+            double k = INFINITY;
+            int64_t j = -1;
+            for (int64_t ch = 0; ch < nch; ch++) {
+                const int64_t cj = bjc[ch * n + i];
+                if (cj >= 0 && bkc[ch * n + i] < k) {
+                    k = bkc[ch * n + i];
+                    j = cj;
+                }
+            }
+            bk[i] = k;
+            bj[i] = j;
+        }
         // cheapest edge per component (ties: lower key, then lower point, then lower partner)
         std::vector<int64_t> pick(n, -1);
         for (int64_t i = 0; i < n; i++) {
@@ -556,34 +584,43 @@ int32_t bh_hdbscan_mst(const double *X, int64_t n, int32_t F, const double *core
 }
 
 int32_t bh_hdbscan_labels(const int64_t *edge_a, const int64_t *edge_b, const double *edge_w,
-                          int64_t n, int32_t min_cluster_size, int32_t *labels,
-                          int32_t *n_clusters) {
+                          int64_t n, const int32_t *min_cluster_sizes, int32_t n_sizes,
+                          int32_t *labels, int32_t *n_clusters) {
     g_err[0] = 0;
     namespace C = bh::clu;
     if (n < 2) return fail(-5, "bh_hdbscan_labels: n >= 2 required");
-    if (min_cluster_size < 2) return fail(-5, "bh_hdbscan_labels: min_cluster_size >= 2 required");
+    if (n_sizes < 1 || !min_cluster_sizes) return fail(-5, "bh_hdbscan_labels: no min_cluster_size");
+    for (int32_t k = 0; k < n_sizes; k++)
+        if (min_cluster_sizes[k] < 2)
+            return fail(-5, "bh_hdbscan_labels: min_cluster_size >= 2 required");
     if (!edge_a || !edge_b || !edge_w || !labels) return fail(-5, "bh_hdbscan_labels: null argument");
     for (int64_t e = 0; e < n - 1; e++)
         if (edge_a[e] < 0 || edge_a[e] >= n || edge_b[e] < 0 || edge_b[e] >= n || !(edge_w[e] >= 0.0))
             return fail(-5, "bh_hdbscan_labels: edge %lld out of range", (long long)e);
+    // one hierarchy, condensed once per requested min_cluster_size
     const C::Hierarchy h = C::single_linkage(n, edge_a, edge_b, edge_w);
-    const C::Condensed c = C::condense(h, n, min_cluster_size);
-    const int32_t k = C::select_and_label(c, n, labels);
-    if (n_clusters) *n_clusters = k;
+    for (int32_t k = 0; k < n_sizes; k++) {
+        const C::Condensed c = C::condense(h, n, min_cluster_sizes[k]);
+        const int32_t nc = C::select_and_label(c, n, labels + (int64_t)k * n);
+        if (n_clusters) n_clusters[k] = nc;
+    }
     return 0;
 }
 
 int32_t bh_silhouette(const double *X_sorted, int64_t n, int32_t F, const int32_t *labels_sorted,
-                      const int64_t *starts, int32_t n_labels, double *s_out, void *stream) {
+                      const int64_t *starts, int32_t n_labels, double *sums, double *s_out,
+                      void *stream) {
     g_err[0] = 0;
     namespace C = bh::clu;
     if (n < 1 || F < 1 || F > C::MAXF || n_labels < 1)
         return fail(-5, "bh_silhouette: n >= 1, 1 <= F <= %d, n_labels >= 1 required", C::MAXF);
-    if (!X_sorted || !labels_sorted || !starts || !s_out) return fail(-5, "bh_silhouette: null argument");
+    if (!X_sorted || !labels_sorted || !starts || !sums || !s_out)
+        return fail(-5, "bh_silhouette: null argument");
+    if (n_labels > 65535) return fail(-5, "bh_silhouette: at most 65535 labels");
     if ((n + C::TILE - 1) / C::TILE > 2147483647LL) return fail(-5, "bh_silhouette: n too large");
     int32_t rc = 0;
-    BH_BY_SLOTS(F, (silhouette_launch<FCV>(X_sorted, n, F, labels_sorted, starts, n_labels, s_out,
-                                            reinterpret_cast<hipStream_t>(stream))));
+    BH_BY_SLOTS(F, (silhouette_launch<FCV>(X_sorted, n, F, labels_sorted, starts, n_labels, sums,
+                                            s_out, reinterpret_cast<hipStream_t>(stream))));
     return rc;
 }
 

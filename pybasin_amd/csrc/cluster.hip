@@ -33,6 +33,7 @@ namespace clu {
 constexpr int TILE = 256;   // reference points staged per LDS tile
 constexpr int MAXF = 16;    // features (rows are padded to 2, 4, 8 or 16 slots)
 constexpr int MAXK = 32;    // neighbours
+constexpr int MST_CHUNKS = 8;  // reference-row chunks per Boruvka round
 
 // Kernels are instantiated for FC = 2, 4, 8, 16 feature slots; rows hold F
 // <= FC features and the slots beyond F are zero (adding (0 - 0)^2 = +0 to a
@@ -123,23 +124,28 @@ __global__ void __launch_bounds__(TILE) knn_kernel(const double *Q, int64_t nq, 
 // One Boruvka round: for each point the cheapest mutual-reachability edge
 // to a point of another component.  The comparison key is
 // max(core_a^2, core_b^2, d^2) (monotone in the edge weight); ties go to the
-// lower index.  best_j = -1 when every point shares the component.
+// lower index.  The reference rows are split into gridDim.y chunks (more
+// waves in flight than one row per lane gives); chunk y writes its best to
+// best_key / best_j[y * n + i] (best_j = -1: no row of another component in
+// the chunk), and the host takes the minimum over chunks.
 template <int FC>
 __global__ void __launch_bounds__(TILE) boruvka_kernel(const double *X, int64_t n, int F,
                                                        const double *core_sq, const int32_t *comp,
-                                                       double *best_key, int64_t *best_j) {
+                                                       int64_t chunk, double *best_key,
+                                                       int64_t *best_j) {
     __shared__ double tile[TILE * FC];
     __shared__ double tcore[TILE];
     __shared__ int32_t tcomp[TILE];
     const int64_t i = (int64_t)blockIdx.x * TILE + threadIdx.x;
+    const int64_t j0 = (int64_t)blockIdx.y * chunk, j1 = min(n, j0 + chunk);
     double x[FC];
     load_row<FC>(x, X + i * F, F, i < n);
     const double ci_sq = i < n ? core_sq[i] : 0.0;
     const int32_t ci = i < n ? comp[i] : -1;
-    double bk = INFINITY;
+    double bk = i < n ? INFINITY : -1.0;  // an idle lane never updates
     int64_t bj = -1;
-    for (int64_t r0 = 0; r0 < n; r0 += TILE) {
-        const int cnt = (int)min((int64_t)TILE, n - r0);
+    for (int64_t r0 = j0; r0 < j1; r0 += TILE) {
+        const int cnt = (int)min((int64_t)TILE, j1 - r0);
         __syncthreads();
         stage_tile<FC>(tile, X, r0, cnt, F);
         for (int e = threadIdx.x; e < cnt; e += TILE) {
@@ -147,7 +153,6 @@ __global__ void __launch_bounds__(TILE) boruvka_kernel(const double *X, int64_t 
             tcomp[e] = comp[r0 + e];
         }
         __syncthreads();
-        if (i >= n) continue;
         for (int j = 0; j < cnt; j++) {
             if (tcomp[j] == ci) continue;
             const double lo = fmax(ci_sq, tcore[j]);
@@ -160,8 +165,8 @@ __global__ void __launch_bounds__(TILE) boruvka_kernel(const double *X, int64_t 
         }
     }
     if (i < n) {
-        best_key[i] = bk;
-        best_j[i] = bj;
+        best_key[blockIdx.y * n + i] = bk;
+        best_j[blockIdx.y * n + i] = bj;
     }
 }
 
@@ -179,42 +184,50 @@ __global__ void edge_weight_kernel(const double *X, int F, const double *core, c
     w[e] = fmax(fmax(core[a], core[b]), d);
 }
 
-// Silhouette terms: rows sorted by label, labels 0..C-1 in contiguous
-// blocks [start[c], start[c+1]).  For each row: sums of distances to every
-// block, then a = own block mean (excluding itself), b = smallest other
-// block mean, s = (b - a) / max(a, b) (0 for a block of one).
+// Silhouette: rows sorted by label, labels 0..C-1 in contiguous blocks
+// [start[c], start[c+1]).  silhouette_sums_kernel: block (x, c) sums the
+// distances from its rows to every row of label c into sums[c * n + i];
+// silhouette_value_kernel: a = own-label mean (excluding the row itself),
+// b = smallest other-label mean, s = (b - a) / max(a, b) (0 for a label of
+// one row).
 template <int FC>
-__global__ void __launch_bounds__(TILE) silhouette_kernel(const double *X, int64_t n, int F,
-                                                          const int32_t *label,
-                                                          const int64_t *start, int C,
-                                                          double *s_out) {
+__global__ void __launch_bounds__(TILE) silhouette_sums_kernel(const double *X, int64_t n, int F,
+                                                               const int64_t *start,
+                                                               double *sums) {
     __shared__ double tile[TILE * FC];
     const int64_t i = (int64_t)blockIdx.x * TILE + threadIdx.x;
+    const int c = blockIdx.y;
     double x[FC];
     load_row<FC>(x, X + i * F, F, i < n);
-    const int li = i < n ? label[i] : 0;
+    const int64_t c0 = start[c], c1 = start[c + 1];
+    double sum = 0.0;
+    for (int64_t r0 = c0; r0 < c1; r0 += TILE) {
+        const int cnt = (int)min((int64_t)TILE, c1 - r0);
+        __syncthreads();
+        stage_tile<FC>(tile, X, r0, cnt, F);
+        __syncthreads();
+        if (i < n)
+            for (int j = 0; j < cnt; j++) sum += sqrt(sqdist<FC>(x, tile + j * FC));
+    }
+    if (i < n) sums[(int64_t)c * n + i] = sum;
+}
+
+__global__ void silhouette_value_kernel(const double *sums, int64_t n, const int32_t *label,
+                                        const int64_t *start, int C, double *s_out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int li = label[i];
     double a = 0.0, b = INFINITY;
     for (int c = 0; c < C; c++) {
-        const int64_t c0 = start[c], c1 = start[c + 1];
-        double sum = 0.0;
-        for (int64_t r0 = c0; r0 < c1; r0 += TILE) {
-            const int cnt = (int)min((int64_t)TILE, c1 - r0);
-            __syncthreads();
-            stage_tile<FC>(tile, X, r0, cnt, F);
-            __syncthreads();
-            if (i < n)
-                for (int j = 0; j < cnt; j++) sum += sqrt(sqdist<FC>(x, tile + j * FC));
-        }
-        const int64_t size = c1 - c0;
+        const int64_t size = start[c + 1] - start[c];
+        const double sum = sums[(int64_t)c * n + i];
         if (c == li)
             a = size > 1 ? sum / (double)(size - 1) : 0.0;
         else if (size > 0)
             b = fmin(b, sum / (double)size);
     }
-    if (i < n) {
-        const int64_t own = start[li + 1] - start[li];
-        s_out[i] = (own > 1 && isfinite(b)) ? (b - a) / fmax(a, b) : 0.0;
-    }
+    const int64_t own = start[li + 1] - start[li];
+    s_out[i] = (own > 1 && isfinite(b)) ? (b - a) / fmax(a, b) : 0.0;
 }
 
 // ---- host side: spanning tree rounds and the cluster hierarchy -------------
