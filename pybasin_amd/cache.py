@@ -3,6 +3,12 @@
 Key = md5 over the solver name, the ODE description, the solver config,
 the bytes of y0 / save_ts / params; value = a safetensors file holding t and y.
 A corrupt file is deleted and the result recomputed.
+
+Fused results (``HipDopri5Solver.integrate_fused``) are cached the same way
+under a key that also covers the fused options (feature window, kept
+trajectories, orbit-peak capacity): the file holds the window statistics,
+final states, status and step counts (and the trajectories / peaks when
+kept), so a cached run does not touch the GPU.
 """
 
 from __future__ import annotations
@@ -55,6 +61,41 @@ class CacheManager:
         os.makedirs(self.cache_dir, exist_ok=True)
         save_file({"t": t.detach().cpu().contiguous(), "y": y.detach().cpu().contiguous()},
                   self._path(key))
+
+    # ---- fused results ------------------------------------------------------
+    _FUSED_FIELDS = ("t", "y", "final", "stats", "status", "nsteps", "peak_values", "peak_counts")
+
+    def save_fused(self, key: str, res) -> None:
+        from safetensors.torch import save_file
+
+        os.makedirs(self.cache_dir, exist_ok=True)
+        tensors = {f: getattr(res, f).detach().cpu().contiguous() for f in self._FUSED_FIELDS
+                   if getattr(res, f) is not None}
+        save_file(tensors, self._path(key), metadata={"kind": "fused",
+                                                      "steady_idx": str(int(res.steady_idx))})
+
+    def load_fused(self, key: str, device):
+        from safetensors import safe_open
+
+        from .solvers.hip_dopri5 import FusedResult
+
+        path = self._path(key)
+        if not os.path.exists(path):
+            return None
+        try:
+            with safe_open(path, framework="pt", device="cpu") as f:
+                meta = f.metadata() or {}
+                if meta.get("kind") != "fused":
+                    raise ValueError("not a fused-result file")
+                d = {k: f.get_tensor(k).to(device) for k in f.keys()}
+            return FusedResult(t=d["t"], y=d.get("y"), final=d["final"], stats=d.get("stats"),
+                               status=d["status"], nsteps=d["nsteps"],
+                               steady_idx=int(meta["steady_idx"]),
+                               peak_values=d.get("peak_values"), peak_counts=d.get("peak_counts"))
+        except Exception as exc:
+            logger.warning("removing corrupt cache file %s (%s)", path, exc)
+            os.remove(path)
+            return None
 
     def cached_call(self, solver_name, ode_system, y0, save_ts, solver_config, device,
                     compute_fn, params=None):

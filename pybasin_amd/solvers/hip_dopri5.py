@@ -179,13 +179,28 @@ class HipDopri5Solver(Solver):
                         *, time_steady: float | None = None,
                         keep_trajectories: bool = False) -> FusedResult:
         """integrate() with the feature window fused into the kernel; trajectories
-        reach HBM only when keep_trajectories is set."""
+        reach HBM only when keep_trajectories is set.  With a cache_dir the
+        result is read from / written to the on-disk cache (a hit skips the
+        GPU)."""
         if self.device.type != "cuda":
             raise RuntimeError("integrate_fused runs on a HIP device; use integrate() on CPU")
         y0, params = self._expand(y0, params)
         save_ts, y0 = self._prepare_tensors(y0)
-        return self.run_fused(ode_system, y0, save_ts, params,
-                              keep_trajectories=keep_trajectories, time_steady=time_steady)
+        if self._cache_manager is None:
+            return self.run_fused(ode_system, y0, save_ts, params,
+                                  keep_trajectories=keep_trajectories, time_steady=time_steady)
+        cfg = dict(self._get_cache_config(), fused=True,
+                   steady_idx=steady_index(save_ts, time_steady),
+                   keep_trajectories=bool(keep_trajectories))
+        key = self._cache_manager.build_key(type(self).__name__, ode_system, y0, save_ts, cfg,
+                                            params)
+        hit = self._cache_manager.load_fused(key, self.device)
+        if hit is not None:
+            return hit
+        res = self.run_fused(ode_system, y0, save_ts, params, keep_trajectories=keep_trajectories,
+                             time_steady=time_steady)
+        self._cache_manager.save_fused(key, res)
+        return res
 
     def run_fused(self, ode_system, y0: torch.Tensor, save_ts: torch.Tensor,
                   params: torch.Tensor | None, *, keep_trajectories: bool,

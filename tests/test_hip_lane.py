@@ -278,3 +278,31 @@ def test_orbit_peak_buffer_overflow_reruns(cuda_device):
     want = extract_orbit_data(full.t, full.y, [0], time_steady=220.0)
     assert torch.equal(od.peak_counts, want.peak_counts)
     assert torch.equal(od.peak_values.nan_to_num(-7.0), want.peak_values.nan_to_num(-7.0))
+
+
+def test_fused_results_cache_round_trip(cuda_device, tmp_path, monkeypatch):
+    """integrate_fused with a cache_dir writes the fused outputs to safetensors;
+    a second call with the same inputs reads them back without launching, a
+    different feature window is a different entry, a corrupt file is
+    recomputed (cache_manager.py:27-128 semantics)."""
+    ode = DuffingODE(DUFF)
+    solver = HipDopri5Solver(t_span=(0, 100), t_steps=200, t_eval=(50.0, 100.0), device="cuda",
+                             cache_dir=str(tmp_path))
+    y0 = torch.tensor(O.sample_uniform(500, [-1.0, -0.5], [1.0, 1.0], seed=3)).to(cuda_device)
+    first = solver.integrate_fused(ode, y0, time_steady=60.0, keep_trajectories=True)
+    files = list(tmp_path.iterdir())
+    assert len(files) == 1
+    calls = []
+    real = HipDopri5Solver.run_fused
+    monkeypatch.setattr(HipDopri5Solver, "run_fused",
+                        lambda self, *a, **k: calls.append(1) or real(self, *a, **k))
+    again = solver.integrate_fused(ode, y0, time_steady=60.0, keep_trajectories=True)
+    assert not calls
+    for f in ("t", "y", "final", "stats", "status", "nsteps"):
+        assert torch.equal(getattr(again, f).cpu(), getattr(first, f).cpu()), f
+    assert again.steady_idx == first.steady_idx and again.final.device == first.final.device
+    solver.integrate_fused(ode, y0, time_steady=70.0, keep_trajectories=True)
+    assert len(calls) == 1 and len(list(tmp_path.iterdir())) == 2
+    files[0].write_bytes(b"corrupt")
+    fixed = solver.integrate_fused(ode, y0, time_steady=60.0, keep_trajectories=True)
+    assert len(calls) == 2 and torch.equal(fixed.stats.cpu(), first.stats.cpu())
