@@ -187,6 +187,18 @@ int32_t bh_mlp_infer(const float *x, int64_t M, const void *w_packed, const floa
 int32_t bh_mlp_classify(const float *Z, const float *bias, int64_t M, int32_t *label,
                         float *conf, void *stream);
 
+/* Order statistics of the feature window of a kept trajectory buffer
+   (n_save, n, dim) in dtype: out (n_ranks, n, dim) in dtype holds, per
+   (trajectory, state), the value of 0-based rank ranks[r] (device int64,
+   each in [0, n_save - start)) among rows [start, n_save); NaN if the window
+   holds a NaN.  Median = rank (N - 1) / 2 (torch.median, lower middle);
+   quantiles interpolate two ranks (torch.quantile).  Replaces the windowed
+   calculators of src/pybasin/ts_torch/calculators/torch_features_statistical.py
+   (median 30-38, quantile 154-157). */
+int32_t bh_window_order_stats(int32_t dtype, const void *traj, int64_t n_save, int64_t n,
+                              int32_t dim, int64_t start, const int64_t *ranks, int32_t n_ranks,
+                              void *out, void *stream);
+
 /* ---- label layer (replaces the sklearn calls of the reference's
    HDBSCANClusterer, src/pybasin/predictors/hdbscan_clusterer.py:16-166, and
    the 1-NN template classifiers of the case studies) --------------------- */

@@ -26,7 +26,8 @@ EXPORTS = ("bh_version", "bh_last_error", "bh_solve_batch", "bh_solve_batch_peak
            "bh_solve_rossler", "bh_gemm_bf16",
            "bh_mlp_bias_act", "bh_mlp_silu_bwd", "bh_mlp_softmax_ce", "bh_mlp_colsum",
            "bh_mlp_adam", "bh_mlp_prep_input", "bh_mlp_classify", "bh_mlp_infer", "bh_knn",
-           "bh_hdbscan_workspace_bytes", "bh_hdbscan_mst", "bh_hdbscan_labels", "bh_silhouette")
+           "bh_hdbscan_workspace_bytes", "bh_hdbscan_mst", "bh_hdbscan_labels", "bh_silhouette",
+           "bh_window_order_stats")
 
 _lock = threading.Lock()
 _lib: ctypes.CDLL | None = None
@@ -69,6 +70,7 @@ def _declare(L: ctypes.CDLL) -> None:
         "bh_hdbscan_mst": [vp, i64, i32, vp, vp, vp, vp, vp, i64, vp],
         "bh_hdbscan_labels": [vp, vp, vp, i64, vp, i32, vp, vp],
         "bh_silhouette": [vp, i64, i32, vp, vp, i32, vp, vp, vp],
+        "bh_window_order_stats": [i32, vp, i64, i64, i32, i64, vp, i32, vp, vp],
     }
     for name, args in sigs.items():
         fn = getattr(L, name)

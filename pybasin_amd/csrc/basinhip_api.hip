@@ -9,6 +9,7 @@
 #include "rossler_tile.hip"
 #include "mlp.hip"
 #include "cluster.hip"
+#include "window.hip"
 
 namespace {
 
@@ -622,6 +623,35 @@ int32_t bh_silhouette(const double *X_sorted, int64_t n, int32_t F, const int32_
     BH_BY_SLOTS(F, (silhouette_launch<FCV>(X_sorted, n, F, labels_sorted, starts, n_labels, sums,
                                             s_out, reinterpret_cast<hipStream_t>(stream))));
     return rc;
+}
+
+// ---- windowed order statistics ---------------------------------------------
+
+int32_t bh_window_order_stats(int32_t dtype, const void *traj, int64_t n_save, int64_t n,
+                              int32_t dim, int64_t start, const int64_t *ranks, int32_t n_ranks,
+                              void *out, void *stream) {
+    g_err[0] = 0;
+    if (n < 0 || dim < 1 || n_save < 1) return fail(-5, "bh_window_order_stats: bad sizes");
+    if (start < 0 || start >= n_save) return fail(-5, "bh_window_order_stats: empty window");
+    if (n_ranks < 1 || !ranks) return fail(-5, "bh_window_order_stats: no ranks");
+    if (n == 0) return 0;
+    if (!traj || !out) return fail(-5, "bh_window_order_stats: null argument");
+    const int64_t np = n * dim;
+    if ((np + 255) / 256 > 2147483647LL) return fail(-5, "bh_window_order_stats: too many rows");
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    const dim3 grid((unsigned)((np + 255) / 256));
+    if (dtype == BH_F32)
+        hipLaunchKernelGGL(bh::win::order_stat_kernel<float>, grid, dim3(256), 0, s,
+                           reinterpret_cast<const float *>(traj), n_save, n, dim, start, ranks,
+                           n_ranks, reinterpret_cast<float *>(out));
+    else if (dtype == BH_F64)
+        hipLaunchKernelGGL(bh::win::order_stat_kernel<double>, grid, dim3(256), 0, s,
+                           reinterpret_cast<const double *>(traj), n_save, n, dim, start, ranks,
+                           n_ranks, reinterpret_cast<double *>(out));
+    else
+        return fail(-5, "bh_window_order_stats: dtype must be BH_F32 or BH_F64");
+    BH_HIP_CHECK(hipGetLastError());
+    return 0;
 }
 
 }  // extern "C"

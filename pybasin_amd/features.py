@@ -27,10 +27,16 @@ from .solvers.hip_dopri5 import FUSED_RESULTS, FusedResult, steady_index
 
 DEFAULT_STEADY_FRACTION = 0.85
 
-# name -> (needs) ; every feature here is computable from mean / std / min / max
-STREAMING_FEATURES = ("mean", "standard_deviation", "variance", "minimum", "maximum",
-                      "absolute_maximum", "delta", "log_delta")
+# every feature here is computable from mean / std / min / max
+STREAMING_FEATURES = ("mean", "standard_deviation", "variance", "root_mean_square", "minimum",
+                      "maximum", "absolute_maximum", "delta", "log_delta")
+# order statistics of the window (need the saved window: keep_trajectories)
+WINDOW_FEATURES = ("median", "quantile")
 DEFAULT_FEATURES = {"mean": None, "standard_deviation": None}
+# the reference's default set (settings.py:376-395: minimal features + delta, log_delta)
+MINIMAL_FEATURES = dict.fromkeys(["median", "mean", "standard_deviation", "variance",
+                                  "root_mean_square", "maximum", "absolute_maximum", "minimum",
+                                  "delta", "log_delta"])
 
 
 class Solution:
@@ -97,6 +103,8 @@ def feature_from_stats(name: str, stats: torch.Tensor) -> torch.Tensor:
         return std
     if name == "variance":
         return std * std
+    if name == "root_mean_square":
+        return torch.sqrt(std * std + mean * mean)
     if name == "minimum":
         return mn
     if name == "maximum":
@@ -108,6 +116,58 @@ def feature_from_stats(name: str, stats: torch.Tensor) -> torch.Tensor:
     if name == "log_delta":
         return torch.log((mx - mean).abs() + 1e-10)
     raise KeyError(f"feature {name!r} is not a streaming feature; available: {STREAMING_FEATURES}")
+
+
+def feature_name(state: int, name: str, params: dict | None) -> str:
+    """state_{i}__{name}[__{key}_{value} ...] with parameters sorted by key
+    (src/pybasin/feature_extractors/utils.py:34-80)."""
+    out = f"state_{state}__{name}"
+    for k in sorted(params or {}):
+        out += f"__{k}_{params[k]}"
+    return out
+
+
+def window_order_stats(y: torch.Tensor, start: int, ranks: list[int]) -> torch.Tensor:
+    """(n_ranks, B, S): the values of the given 0-based ranks among rows
+    [start, N) of a (N, B, S) trajectory tensor; NaN where the window holds
+    a NaN.  HIP kernel (bh_window_order_stats) for device tensors, torch sort
+    on the CPU."""
+    if y.device.type == "cuda":
+        from . import _lib
+
+        y = y.contiguous()
+        N, B, S = y.shape
+        r = torch.tensor(ranks, dtype=torch.int64, device=y.device)
+        out = torch.empty((len(ranks), B, S), dtype=y.dtype, device=y.device)
+        dt = _lib.BH_F32 if y.dtype == torch.float32 else _lib.BH_F64
+        s = torch.cuda.current_stream(y.device)
+        with torch.cuda.device(y.device):
+            _lib.check(_lib.lib().bh_window_order_stats(dt, y.data_ptr(), N, B, S, int(start),
+                                                         r.data_ptr(), len(ranks), out.data_ptr(),
+                                                         s.cuda_stream), "bh_window_order_stats")
+        return out
+    w = y[start:]
+    srt = torch.sort(w, dim=0).values
+    out = srt[ranks]
+    return torch.where(torch.isnan(w).any(0, keepdim=True), torch.full_like(out, float("nan")), out)
+
+
+def window_feature(name: str, params: dict | None, y: torch.Tensor, start: int) -> torch.Tensor:
+    """median / quantile of the window rows [start, N) of y (N, B, S) -> (B, S),
+    as x.median(dim=0).values and torch.quantile(x, q, dim=0)
+    (torch_features_statistical.py:30-38, 154-157)."""
+    n = y.shape[0] - start
+    if n < 1:
+        raise ValueError("empty feature window")
+    if name == "median":
+        return window_order_stats(y, start, [(n - 1) // 2])[0]
+    if name == "quantile":
+        q = float(params["q"])
+        pos = q * (n - 1)
+        lo, hi = int(torch.floor(torch.tensor(pos))), int(torch.ceil(torch.tensor(pos)))
+        v = window_order_stats(y, start, [lo, hi])
+        return torch.lerp(v[0], v[1], torch.tensor(pos - lo, dtype=v.dtype, device=v.device))
+    raise KeyError(f"feature {name!r} is not a window feature; available: {WINDOW_FEATURES}")
 
 
 class StatFeatureExtractor(FeatureExtractor):
@@ -126,7 +186,7 @@ class StatFeatureExtractor(FeatureExtractor):
         self._lock = threading.Lock()
         self._names: list[str] = []
 
-    def _plan(self, n_states: int) -> list[tuple[int, str]]:
+    def _plan(self, n_states: int) -> list[tuple[int, str, dict | None]]:
         plan = []
         for s in range(n_states):
             if s in self.features_per_state:
@@ -137,9 +197,29 @@ class StatFeatureExtractor(FeatureExtractor):
                 fdict = self.features
                 if fdict is None:
                     continue
-            for name in fdict:
-                plan.append((s, name))
+            for name, plist in fdict.items():
+                if name not in STREAMING_FEATURES and name not in WINDOW_FEATURES:
+                    raise KeyError(f"feature {name!r} is not supported; available: "
+                                   f"{STREAMING_FEATURES + WINDOW_FEATURES}")
+                for params in (plist if plist else [None]):
+                    plan.append((s, name, params))
         return plan
+
+    def _columns(self, plan, stats: torch.Tensor, y: torch.Tensor | None,
+                 start: int) -> torch.Tensor:
+        cols = []
+        for s, name, params in plan:
+            if name in WINDOW_FEATURES:
+                if y is None:
+                    raise ValueError(f"{name} needs the saved window: integrate with "
+                                     "keep_trajectories=True")
+                cols.append(window_feature(name, params, y, start)[:, s].to(stats.device,
+                                                                             torch.float64))
+            else:
+                cols.append(feature_from_stats(name, stats)[:, s])
+        self._names = [feature_name(s, name, params) for s, name, params in plan]
+        feats = torch.stack(cols, dim=1) if cols else stats.new_zeros((stats.shape[0], 0))
+        return impute_extreme(feats.to(torch.float32))
 
     @property
     def feature_names(self) -> list[str]:
@@ -151,10 +231,8 @@ class StatFeatureExtractor(FeatureExtractor):
     def extract_features(self, solution: Solution) -> torch.Tensor:
         stats = self._stats(solution)
         plan = self._plan(stats.shape[1])
-        cols = [feature_from_stats(name, stats)[:, s] for s, name in plan]
-        self._names = [f"state_{s}__{name}" for s, name in plan]
-        feats = torch.stack(cols, dim=1) if cols else stats.new_zeros((stats.shape[0], 0))
-        feats = impute_extreme(feats.to(torch.float32))
+        start = steady_index(solution.time, self._resolve_time_steady(solution))
+        feats = self._columns(plan, stats, solution.y, start)
         if self.normalize:
             with self._lock:
                 if self._mean is None:
@@ -182,6 +260,4 @@ class HipFusedFeatureExtractor(StatFeatureExtractor):
         if res.stats is None:
             raise ValueError("the fused result carries no window statistics")
         plan = self._plan(res.stats.shape[1])
-        self._names = [f"state_{s}__{name}" for s, name in plan]
-        cols = [feature_from_stats(name, res.stats)[:, s] for s, name in plan]
-        return impute_extreme(torch.stack(cols, dim=1).to(torch.float32))
+        return self._columns(plan, res.stats, res.y, res.steady_idx)

@@ -306,3 +306,62 @@ def test_fused_results_cache_round_trip(cuda_device, tmp_path, monkeypatch):
     files[0].write_bytes(b"corrupt")
     fixed = solver.integrate_fused(ode, y0, time_steady=60.0, keep_trajectories=True)
     assert len(calls) == 2 and torch.equal(fixed.stats.cpu(), first.stats.cpu())
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_window_order_stats_match_torch(cuda_device, dtype):
+    """bh_window_order_stats = torch's median / quantile order statistics on
+    windows with ties, signed zeros, infinities and NaN."""
+    from pybasin_amd.features import window_feature, window_order_stats
+
+    g = torch.Generator().manual_seed(5)
+    for N in (1, 2, 9, 1000):
+        y = torch.randint(-50, 50, (N, 300, 3), generator=g).to(dtype) / 7
+        y[:, 0, 0] = 0.0
+        y[0, 0, 0] = -0.0
+        y[N // 2, 1, 1] = float("inf")
+        y[0, 2, 2] = -float("inf")
+        y[N - 1, 3, 0] = float("nan")
+        yd = y.to(cuda_device)
+        start = N // 3
+        n = N - start
+        ranks = [0, (n - 1) // 2, n - 1]
+        got = window_order_stats(yd, start, ranks).cpu()
+        srt = torch.sort(y[start:], dim=0).values
+        want = srt[ranks]
+        nan = torch.isnan(y[start:]).any(0)
+        want = torch.where(nan[None], torch.full_like(want, float("nan")), want)
+        assert torch.equal(got.nan_to_num(123.0), want.nan_to_num(123.0)), N
+        med = window_feature("median", None, yd, start).cpu()
+        assert torch.equal(med.nan_to_num(123.0), y[start:].median(dim=0).values.nan_to_num(123.0))
+        q = window_feature("quantile", {"q": 0.3}, yd, start).cpu()
+        assert torch.allclose(q.nan_to_num(123.0), torch.quantile(y[start:], 0.3, dim=0).nan_to_num(123.0),
+                              rtol=1e-6, atol=1e-6, equal_nan=True)
+
+
+def test_minimal_feature_set_on_fused_result(cuda_device):
+    """The reference's default feature set (median, mean, std, var, rms, max,
+    abs max, min, delta, log_delta) from a fused launch with kept
+    trajectories equals the torch calculators on that trajectory window."""
+    from pybasin_amd.features import MINIMAL_FEATURES, HipFusedFeatureExtractor
+
+    ode = DuffingODE(DUFF)
+    solver = HipDopri5Solver(t_span=(0, 200), t_steps=400, t_eval=(100.0, 200.0), device="cuda",
+                             cache_dir=None)
+    y0 = torch.tensor(O.sample_uniform(2000, [-1.0, -0.5], [1.0, 1.0], seed=9)).to(cuda_device)
+    res = solver.integrate_fused(ode, y0, time_steady=150.0, keep_trajectories=True)
+    fx = HipFusedFeatureExtractor(time_steady=150.0, features=MINIMAL_FEATURES)
+    f = fx.extract_from_result(res).cpu()
+    w = res.y[res.steady_idx:].double().cpu()
+    ref = {"median": w.median(dim=0).values, "mean": w.mean(0),
+           "standard_deviation": w.std(0, correction=0), "variance": w.var(0, correction=0),
+           "root_mean_square": torch.sqrt((w ** 2).mean(0)), "maximum": w.max(0).values,
+           "absolute_maximum": w.abs().max(0).values, "minimum": w.min(0).values,
+           "delta": (w.max(0).values - w.mean(0)).abs(),
+           "log_delta": torch.log((w.max(0).values - w.mean(0)).abs() + 1e-10)}
+    for j, name in enumerate(fx.feature_names):
+        s, feat = int(name.split("__")[0][6:]), name.split("__")[1]
+        np.testing.assert_allclose(f[:, j].numpy(), ref[feat][:, s].to(torch.float32).numpy(),
+                                   rtol=2e-6, atol=2e-6, err_msg=name)
+    with pytest.raises(ValueError, match="keep_trajectories"):
+        fx.extract_from_result(solver.integrate_fused(ode, y0, time_steady=150.0))

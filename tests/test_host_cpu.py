@@ -253,3 +253,37 @@ def test_peak_mask_matches_scipy_argrelmax():
     assert od.peak_counts.tolist() == [[3]] and od.time_steady == 10.0
     # samples every 0.1: a sampled maximum of cos(2t) is within 1 - cos(0.1) of 1
     assert torch.allclose(od.peak_values[:, 0, 0], torch.ones(3), atol=1 - math.cos(0.1))
+
+
+def test_window_features_follow_torch_calculators_on_cpu():
+    """median = x.median(dim=0).values (lower middle), quantile =
+    torch.quantile, root_mean_square = sqrt(mean(x^2)) (torch_features_statistical.py),
+    names state_{i}__quantile__q_0.1 (feature_extractors/utils.py:34-80)."""
+    from pybasin_amd.features import MINIMAL_FEATURES, feature_name, window_feature
+
+    g = torch.Generator().manual_seed(1)
+    for N in (7, 8):
+        y = torch.randint(-3, 4, (N, 5, 2), generator=g).to(torch.float32)  # ties
+        y[0, 0, 0] = float("nan")
+        got = window_feature("median", None, y, 2)
+        want = y[2:].median(dim=0).values
+        assert torch.equal(got.nan_to_num(99), want.nan_to_num(99))
+        for q in (0.1, 0.5, 0.9):
+            got = window_feature("quantile", {"q": q}, y, 1)
+            want = torch.quantile(y[1:], q, dim=0)
+            assert torch.allclose(got.nan_to_num(99), want.nan_to_num(99), atol=1e-6)
+    assert feature_name(1, "quantile", {"q": 0.1}) == "state_1__quantile__q_0.1"
+    t = torch.linspace(0, 10, 101)
+    yy = torch.stack([torch.sin(t), t, torch.cos(3 * t)], -1)[:, None, :].repeat(1, 2, 1)
+    fx = StatFeatureExtractor(time_steady=5.0, features=MINIMAL_FEATURES)
+    f = fx.extract_features(Solution(yy[0], t, yy))
+    w = yy[50:].double()
+    ref = {"median": w.median(dim=0).values, "mean": w.mean(0), "standard_deviation": w.std(0, correction=0),
+           "variance": w.var(0, correction=0), "root_mean_square": torch.sqrt((w ** 2).mean(0)),
+           "maximum": w.max(0).values, "absolute_maximum": w.abs().max(0).values,
+           "minimum": w.min(0).values, "delta": (w.max(0).values - w.mean(0)).abs(),
+           "log_delta": torch.log((w.max(0).values - w.mean(0)).abs() + 1e-10)}
+    assert fx.feature_names[:10] == [f"state_0__{k}" for k in MINIMAL_FEATURES]
+    for j, name in enumerate(fx.feature_names):
+        s, feat = int(name.split("__")[0][6:]), name.split("__")[1]
+        assert f[0, j].item() == pytest.approx(ref[feat][0, s].item(), rel=1e-5, abs=1e-6), name
