@@ -54,7 +54,8 @@ def parse():
     ap.add_argument("--max-lanes", type=int, default=0)
     ap.add_argument("--cpu-sample", type=int, default=100_000)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=["duffing", "surrogate", "rossler", "mlp_infer", "hdbscan"],
+    ap.add_argument("--workload", choices=["duffing", "lorenz", "surrogate", "rossler", "mlp_infer",
+                                           "hdbscan"],
                     default="duffing",
                     help="duffing: plain HIP Dopri5 on 100k Duffing ICs (default); surrogate: "
                          "pendulum torque sweep through the MLP gate with Dopri5 fallback")
@@ -365,6 +366,84 @@ def hdbscan_workload(args):
     print(json.dumps(out), flush=True)
 
 
+def lorenz_workload(args):
+    """Lorenz with escape: 200,000 seeded ICs (x in [-10, 10], y in [-20, 20],
+    z = 0), sigma 0.12, r 0, b -0.6, t in [0, 1000], float32, rtol 1e-8,
+    atol 1e-6, per-lane stop at |y|inf >= 200 with Inf fill, 1,000 saves on
+    [900, 1000]; the resident-lane cap (refill granularity) chosen by the
+    autotuner; labels: unbounded by status, the two attractors by 1-NN on
+    mean x (setup_lorenz_system.py:54-70)."""
+    from pybasin_amd import autotune
+    from pybasin_amd.cluster import knn
+    from pybasin_amd.solvers import HipDopri5Solver
+    from pybasin_amd.systems import LorenzODE
+
+    n = 200_000 if args.n == 100_000 else args.n
+    torch.manual_seed(args.seed)
+    lo = torch.tensor([-10.0, -20.0, 0.0])
+    hi = torch.tensor([10.0, 20.0, 0.0])
+    y0 = (torch.rand(n, 3) * (hi - lo) + lo)
+    templates = torch.tensor([[0.8, -3.0, 0.0], [-0.8, 3.0, 0.0]])
+    y0_all = torch.cat([y0, templates]).to("cuda")
+    ode = LorenzODE({"sigma": 0.12, "r": 0.0, "b": -0.6})
+    solver = HipDopri5Solver(t_span=(0, 1000), t_steps=1000, t_eval=(900.0, 1000.0), device="cuda",
+                             rtol=1e-8, atol=1e-6, cache_dir=None, stop_threshold=200.0,
+                             inf_fill=True)
+    os.environ.setdefault("PYBASIN_AMD_TUNE_CACHE", os.path.join("/tmp", "pybasin_amd_tune.json"))
+    t0 = time.perf_counter()
+    cap = autotune.tune_lane_cap(solver, ode, y0_all, probe_span=50.0, use_cache=False)
+    t_tune = time.perf_counter() - t0
+    solver.max_lanes = cap
+    save_ts = torch.linspace(900.0, 1000.0, 1000, dtype=torch.float32, device="cuda")
+    stream = torch.cuda.current_stream()
+    kev = []
+
+    def step(record=False):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        res = solver.run_fused(ode, y0_all, save_ts, None, keep_trajectories=False, steady_idx=0,
+                               stream=stream)
+        e1.record(stream)
+        if record:
+            kev.append((e0, e1))
+        mean_x = res.stats[:, 0, 0:1].contiguous()
+        lab = knn(mean_x[:n], mean_x[n:], 1)[1][:, 0]
+        lab = torch.where(res.status[:n] == 4, torch.full_like(lab, 2), lab)  # BH_LANE_DIVERGED
+        return res, torch.bincount(lab, minlength=3)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        res, counts = step(record=True)
+    torch.cuda.synchronize()
+    ms = 1e3 * (time.perf_counter() - t0) / args.steps
+    kernel_ms = sum(a.elapsed_time(b) for a, b in kev) / len(kev)
+    nsteps = res.nsteps.double()
+    flops = float(nsteps.sum()) * (6 * F_RHS["lorenz"] + 63 * 3 + 16) + \
+        float((res.status == 0).sum()) * 1000 * (36 + 9 + 6) * 3
+    out = {
+        "metric": "classified initial conditions/sec at <=0.5% basin-stability error vs plain",
+        "value": round(n / (ms * 1e-3), 1), "unit": "ICs/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic: seeded-uniform ICs (torch.manual_seed, CPU float32 draw)",
+        "config": {"workload": "lorenz_escape_hip_dopri5", "n_ics": n, "stop": "|y|inf >= 200, Inf fill",
+                   "lane_cap": cap, "tuner_timings_ms": autotune.last_timings,
+                   "tune_s": round(t_tune, 2),
+                   "basin_stability": {k: round(float(v) / n, 5) for k, v in
+                                       zip(("attractor 1", "attractor 2", "unbounded"), counts.tolist())},
+                   "mean_steps_per_ic": round(float(nsteps.mean()), 1),
+                   "max_steps_per_ic": int(nsteps.max())},
+        "roofline": {"bound": "valu", "achieved": round(flops / (kernel_ms * 1e-3) / 1e12, 3),
+                     "peak": 157.3, "unit": "TFLOP/s",
+                     "frac": round(flops / (kernel_ms * 1e-3) / PEAK_FP32_VALU, 4), "traffic": None,
+                     "kernel": "bh::dopri5_lane_kernel<Lorenz,float>", "kernel_ms": round(kernel_ms, 3)},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     args = parse()
     if args.copy_calibration:
@@ -377,6 +456,8 @@ def main():
 
     if args.workload == "surrogate":
         return surrogate_workload(args)
+    if args.workload == "lorenz":
+        return lorenz_workload(args)
     if args.workload == "hdbscan":
         return hdbscan_workload(args)
     if args.workload == "mlp_infer":
@@ -384,6 +465,7 @@ def main():
     if args.workload == "rossler":
         return rossler_workload(args)
 
+    from pybasin_amd.cluster import knn
     from pybasin_amd.solvers import HipDopri5Solver
     from pybasin_amd.systems import DuffingODE
 
@@ -411,10 +493,10 @@ def main():
         e1.record(stream)
         if record:
             kev.append((e0, e1))
-        # reference Duffing classifier features: state 0 maximum and std
+        # reference Duffing classifier features: state 0 maximum and std; 1-NN
+        # on the templates by the HIP k-NN kernel
         feats = torch.stack([res.stats[:, 0, 3], res.stats[:, 0, 1]], dim=1)
-        d = torch.cdist(feats[:n], feats[n:])
-        labels = torch.argmin(d, dim=1)
+        labels = knn(feats[:n].contiguous(), feats[n:].contiguous(), 1)[1][:, 0]
         counts = torch.bincount(labels, minlength=len(DUFFING_TEMPLATES))
         return res, counts
 
@@ -467,7 +549,7 @@ def main():
             "peak": round(peak / 1e12, 1), "unit": "TFLOP/s",
             "frac": round(achieved / peak, 4),
             "traffic": TRAFFIC_BYTES.get(("duffing", args.dtype, n)),
-            "kernel": "bh::dopri5_lane_kernel<Duffing,float>",
+            "kernel": f"bh::dopri5_lane_kernel<Duffing,{'float' if dtype == torch.float32 else 'double'}>",
             "kernel_ms": round(kernel_ms, 3),
         },
     }

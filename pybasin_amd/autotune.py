@@ -22,6 +22,7 @@ from . import _lib
 
 _CACHE_ENV = "PYBASIN_AMD_TUNE_CACHE"
 _lock = threading.Lock()
+last_timings: dict = {}  # candidate -> ms of the most recent tuning run
 
 
 def _cache_path() -> str:
@@ -94,6 +95,8 @@ def tune_lane_cap(solver, ode_system, y0: torch.Tensor, params: torch.Tensor | N
                              max_lanes=lanes)
         results[lanes] = _time(lambda: probe.integrate_fused(ode_system, y0, params))
     best = min(results, key=results.get)
+    last_timings.clear()
+    last_timings.update({str(k): round(v, 3) for k, v in results.items()})
     with _lock:
         cache = _load() if use_cache else {}
         cache[key] = {"max_lanes": best, "timings_ms": {str(k): v for k, v in results.items()}}
