@@ -302,14 +302,16 @@ def hdbscan_workload(args):
     features (max x, std x) of the n default ICs, integrated once outside the
     timed region; a step = one auto-tuned clustering with noise assignment.
     sklearn's own pipeline is timed on a bounded sample for comparison."""
-    from oracle import oracle as O  # ICs only (the seeded uniform draw)
     from pybasin_amd.cluster import HipHDBSCANClusterer
     from pybasin_amd.features import feature_from_stats
     from pybasin_amd.solvers import HipDopri5Solver
     from pybasin_amd.systems import DuffingODE
 
     n = args.n
-    y0 = torch.tensor(O.sample_uniform(n, [-1.0, -0.5], [1.0, 1.0], 42)).to("cuda")
+    torch.manual_seed(args.seed)
+    lo = torch.tensor([-1.0, -0.5], dtype=torch.float64)
+    hi = torch.tensor([1.0, 1.0], dtype=torch.float64)
+    y0 = (torch.rand(n, 2, dtype=torch.float64) * (hi - lo) + lo).to("cuda")
     solver = HipDopri5Solver(t_span=(0, 1000), t_steps=1000, t_eval=(900.0, 1000.0), device="cuda",
                              cache_dir=None)
     res = solver.integrate_fused(DuffingODE({"delta": 0.08, "k3": 1.0, "A": 0.2}), y0,
@@ -391,7 +393,7 @@ def lorenz_workload(args):
                              inf_fill=True)
     os.environ.setdefault("PYBASIN_AMD_TUNE_CACHE", os.path.join("/tmp", "pybasin_amd_tune.json"))
     t0 = time.perf_counter()
-    cap = autotune.tune_lane_cap(solver, ode, y0_all, probe_span=50.0, use_cache=False)
+    cap = autotune.tune_lane_cap(solver, ode, y0_all, use_cache=False)
     t_tune = time.perf_counter() - t0
     solver.max_lanes = cap
     save_ts = torch.linspace(900.0, 1000.0, 1000, dtype=torch.float32, device="cuda")

@@ -159,56 +159,66 @@ struct Lane {
     T last[S];
 };
 
-template <class Sys, class T, class LaneT>
-__device__ __forceinline__ void emit(const LaneArgs &a, LaneT &L, int64_t ic, int32_t si,
-                                     const T (&v)[Sys::S]) {
-    constexpr int S = Sys::S;
+template <class Sys, class T>
+__device__ __forceinline__ void write_row(const LaneArgs &a, int64_t ic, int32_t si, const T (&v)[Sys::S]) {
     if (a.traj) {
-        T *o = reinterpret_cast<T *>(a.traj) + ((int64_t)si * a.n + ic) * S;
+        T *o = reinterpret_cast<T *>(a.traj) + ((int64_t)si * a.n + ic) * Sys::S;
 #pragma unroll
-        for (int i = 0; i < S; i++) o[i] = v[i];
+        for (int i = 0; i < Sys::S; i++) o[i] = v[i];
     }
-    if (si >= a.steady_idx) {
-        if (L.cnt == 0) {
+}
+
+// One sample of the steady window: running statistics and the peak test.
+template <class Sys, class T, class LaneT>
+__device__ __forceinline__ void window_sample(const LaneArgs &a, LaneT &L, int64_t ic,
+                                              const T (&v)[Sys::S]) {
+    constexpr int S = Sys::S;
+    if (L.cnt == 0) {
+#pragma unroll
+        for (int i = 0; i < S; i++) {
+            L.shift[i] = (double)v[i];
+            L.mn[i] = v[i];
+            L.mx[i] = v[i];
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < S; i++) {
+        const double d = (double)v[i] - L.shift[i];
+        L.s1[i] += d;
+        L.s2[i] = fma(d, d, L.s2[i]);
+        L.mn[i] = fmin(L.mn[i], v[i]);
+        L.mx[i] = fmax(L.mx[i], v[i]);
+    }
+    if constexpr (LaneT::kP) {
+        // window position L.cnt; the value before it is a maximum when it
+        // exceeds both neighbours (NaN compares false: no maximum)
+        if (L.cnt >= 2) {
 #pragma unroll
             for (int i = 0; i < S; i++) {
-                L.shift[i] = (double)v[i];
-                L.mn[i] = v[i];
-                L.mx[i] = v[i];
+                if (L.pk.p1[i] > L.pk.p2[i] && L.pk.p1[i] > v[i]) {
+                    if (L.pk.count[i] < a.peak_cap)
+                        reinterpret_cast<T *>(a.peak_vals)[((int64_t)L.pk.count[i] * a.n + ic) * S + i] =
+                            L.pk.p1[i];
+                    L.pk.count[i]++;
+                }
             }
         }
 #pragma unroll
         for (int i = 0; i < S; i++) {
-            const double d = (double)v[i] - L.shift[i];
-            L.s1[i] += d;
-            L.s2[i] = fma(d, d, L.s2[i]);
-            L.mn[i] = fmin(L.mn[i], v[i]);
-            L.mx[i] = fmax(L.mx[i], v[i]);
+            L.pk.p2[i] = L.pk.p1[i];
+            L.pk.p1[i] = v[i];
         }
-        if constexpr (LaneT::kP) {
-            // window position L.cnt; the value before it is a maximum when it
-            // exceeds both neighbours (NaN compares false: no maximum)
-            if (L.cnt >= 2) {
-#pragma unroll
-                for (int i = 0; i < S; i++) {
-                    if (L.pk.p1[i] > L.pk.p2[i] && L.pk.p1[i] > v[i]) {
-                        if (L.pk.count[i] < a.peak_cap)
-                            reinterpret_cast<T *>(a.peak_vals)[((int64_t)L.pk.count[i] * a.n + ic) * S + i] =
-                                L.pk.p1[i];
-                        L.pk.count[i]++;
-                    }
-                }
-            }
-#pragma unroll
-            for (int i = 0; i < S; i++) {
-                L.pk.p2[i] = L.pk.p1[i];
-                L.pk.p1[i] = v[i];
-            }
-        }
-        L.cnt++;
     }
+    L.cnt++;
+}
+
+template <class Sys, class T, class LaneT>
+__device__ __forceinline__ void emit(const LaneArgs &a, LaneT &L, int64_t ic, int32_t si,
+                                     const T (&v)[Sys::S]) {
+    write_row<Sys, T>(a, ic, si, v);
+    if (si >= a.steady_idx) window_sample<Sys, T>(a, L, ic, v);
 #pragma unroll
-    for (int i = 0; i < S; i++) L.last[i] = v[i];
+    for (int i = 0; i < Sys::S; i++) L.last[i] = v[i];
 }
 
 template <class Sys, class T, class LaneT>
@@ -284,7 +294,34 @@ __device__ __forceinline__ void lane_fill_rest(const LaneArgs &a, LaneT &L, int6
     T v[S];
 #pragma unroll
     for (int i = 0; i < S; i++) v[i] = inf ? (T)INFINITY : L.y[i];
-    for (; L.si < a.n_save; L.si++) emit<Sys, T>(a, L, ic, L.si, v);
+    if (L.si >= a.n_save) return;
+    for (int32_t si = L.si; si < a.n_save; si++) write_row<Sys, T>(a, ic, si, v);
+#pragma unroll
+    for (int i = 0; i < S; i++) L.last[i] = v[i];
+    const int32_t w0 = max(L.si, a.steady_idx);
+    L.si = a.n_save;
+    if (w0 >= a.n_save) return;
+    // The first window sample goes through the full update (it may open the
+    // window, and it decides whether the sample before it is a peak).  The
+    // k others equal it: no peak among them, extremes unchanged, and their
+    // sums are added at once, so a lane stopped early costs O(1) here
+    // instead of one update per remaining save.
+    window_sample<Sys, T>(a, L, ic, v);
+    const int32_t k = a.n_save - w0 - 1;
+    if (k > 0) {
+#pragma unroll
+        for (int i = 0; i < S; i++) {
+            const double d = (double)v[i] - L.shift[i];
+            const double kd = (double)k * d;
+            L.s1[i] += kd;
+            L.s2[i] = fma(kd, d, L.s2[i]);
+        }
+        if constexpr (LaneT::kP) {
+#pragma unroll
+            for (int i = 0; i < S; i++) L.pk.p2[i] = v[i];
+        }
+        L.cnt += k;
+    }
 }
 
 // One attempted step.  Sets L.done when the lane has finished.
@@ -458,15 +495,22 @@ __global__ void __launch_bounds__(256) dopri5_lane_kernel(LaneArgs a) {
     if (ic >= a.n) return;
     Lane<Sys, T, kPeaks> L;
     lane_init<Sys, T>(a, L, ic);
-    for (;;) {
+    bool live = true;
+    // Every pass runs one step for each lane still integrating, and in the
+    // same pass a lane that finished writes its results and takes the next
+    // IC from the counter.  The loop ends by a wave-wide vote: had it ended
+    // per lane, the finished lanes would leave the loop and wait there until
+    // the last lane of the wave finished, and refills would happen once per
+    // wave instead of once per lane.
+    do {
         if (!L.done) lane_step<Sys, T>(a, L, ic);  // not done unless init emitted every save
-        if (L.done) {
+        if (L.done && live) {
             lane_finish<Sys, T>(a, L, ic);
             ic = (int64_t)atomicAdd(a.counter, 1ull);
-            if (ic >= a.n) break;
-            lane_init<Sys, T>(a, L, ic);
+            live = ic < a.n;
+            if (live) lane_init<Sys, T>(a, L, ic);
         }
-    }
+    } while (__any(live));
 }
 
 __global__ void set_counter_kernel(unsigned long long *c, unsigned long long v) {

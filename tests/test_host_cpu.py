@@ -287,3 +287,38 @@ def test_window_features_follow_torch_calculators_on_cpu():
     for j, name in enumerate(fx.feature_names):
         s, feat = int(name.split("__")[0][6:]), name.split("__")[1]
         assert f[0, j].item() == pytest.approx(ref[feat][0, s].item(), rel=1e-5, abs=1e-6), name
+
+
+def test_duffing_stage_sincos_coefficients_are_within_bound():
+    """The float polynomial that the Duffing functor uses for sin/cos of a
+    stage offset |d| <= 1 (coefficients read from systems.hpp), evaluated
+    in float32 in the kernel's order, stays within 1.3 ulp of sin/cos."""
+    import os
+    import re
+
+    src = open(os.path.join(os.path.dirname(__file__), "..", "pybasin_amd", "csrc",
+                            "systems.hpp")).read()
+    body = src[src.index("sincos_unit<float>(float d"):]
+    body = body[:body.index("}")]
+    lit = [np.float32(v) for v in re.findall(r"(-?\d\.\d+e[-+]\d+)f", body)]
+    assert len(lit) == 7, lit
+    p, q = lit[:3], lit[3:]
+    f = np.float32
+    d = np.linspace(-1, 1, 400001).astype(f)
+    x = (d * d).astype(f)
+
+    def horner(cs):
+        r = cs[-1]
+        for c in cs[-2::-1]:
+            r = (c + x * r).astype(f)
+        return r
+
+    s = (d + (d * x).astype(f) * horner(p)).astype(f)
+    c = (f(1) - x * horner(q)).astype(f)
+    dd = d.astype(np.float64)
+    err_s = np.abs(s - np.sin(dd)) / np.spacing(np.abs(np.sin(dd)).astype(f))
+    err_c = np.abs(c - np.cos(dd)) / np.spacing(np.cos(dd).astype(f))
+    assert err_s.max() <= 1.3 and err_c.max() <= 1.3, (err_s.max(), err_c.max())
+    # a dropped or sign-flipped term is far outside the bound
+    bad = (d + (d * x).astype(f) * horner([p[0], p[1], -p[2]])).astype(f)
+    assert (np.abs(bad - np.sin(dd)) / np.spacing(np.abs(np.sin(dd)).astype(f))).max() > 10
