@@ -372,7 +372,7 @@ def lorenz_workload(args):
     """Lorenz with escape: 200,000 seeded ICs (x in [-10, 10], y in [-20, 20],
     z = 0), sigma 0.12, r 0, b -0.6, t in [0, 1000], float32, rtol 1e-8,
     atol 1e-6, per-lane stop at |y|inf >= 200 with Inf fill, 1,000 saves on
-    [900, 1000]; the resident-lane cap (refill granularity) chosen by the
+    [900, 1000]; the resident-lane cap and refill quorum chosen by the
     autotuner; labels: unbounded by status, the two attractors by 1-NN on
     mean x (setup_lorenz_system.py:54-70)."""
     from pybasin_amd import autotune
@@ -393,9 +393,9 @@ def lorenz_workload(args):
                              inf_fill=True)
     os.environ.setdefault("PYBASIN_AMD_TUNE_CACHE", os.path.join("/tmp", "pybasin_amd_tune.json"))
     t0 = time.perf_counter()
-    cap = autotune.tune_lane_cap(solver, ode, y0_all, use_cache=False)
+    cap, quorum = autotune.tune_lane_launch(solver, ode, y0_all, use_cache=False)
     t_tune = time.perf_counter() - t0
-    solver.max_lanes = cap
+    solver.max_lanes, solver.refill_quorum = cap, quorum
     save_ts = torch.linspace(900.0, 1000.0, 1000, dtype=torch.float32, device="cuda")
     stream = torch.cuda.current_stream()
     kev = []
@@ -432,7 +432,8 @@ def lorenz_workload(args):
         "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic: seeded-uniform ICs (torch.manual_seed, CPU float32 draw)",
         "config": {"workload": "lorenz_escape_hip_dopri5", "n_ics": n, "stop": "|y|inf >= 200, Inf fill",
-                   "lane_cap": cap, "tuner_timings_ms": autotune.last_timings,
+                   "lane_cap": cap, "refill_quorum": quorum,
+                   "tuner_timings_ms": autotune.last_timings,
                    "tune_s": round(t_tune, 2),
                    "basin_stability": {k: round(float(v) / n, 5) for k, v in
                                        zip(("attractor 1", "attractor 2", "unbounded"), counts.tolist())},

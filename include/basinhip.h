@@ -47,6 +47,14 @@ enum {
 #define BH_FLAG_STOP 0x1u     /* enable the per-lane divergence stop at stop_threshold    */
 #define BH_FLAG_INF_FILL 0x2u /* after a stop, remaining saves are +Inf (JaxSolver Event
                                  semantics) instead of the frozen state (masked-RHS semantics) */
+/* Refill quorum, bits 8-14 of flags (a launch-shape choice; results do not
+   depend on it): finished lanes of a wave take new trajectories once at
+   least q of them wait (or every lane still holding one has finished).
+   q = 1 refills each lane at once (best when trajectories end at very
+   different times, e.g. divergence stops); q = 64 refills whole waves (keeps
+   a wave's lanes in step, so save-point work is shared).  0 = the default. */
+#define BH_REFILL_QUORUM_SHIFT 8u
+#define BH_REFILL_QUORUM(q) (((uint32_t)(q) & 0x7fu) << BH_REFILL_QUORUM_SHIFT)
 
 /* number of statistics per state written to out_feats: mean, std (ddof 0),
    min, max over save indices [steady_idx, n_save) */

@@ -1,12 +1,15 @@
 """Launch-shape autotuner for the integration kernels.
 
 The tunables are the resident-lane cap of the per-lane kernel (how many
-waves per SIMD stay resident; finished lanes refill from the counter) and
-the resident-workgroup cap of the Rossler tile kernel.  The tuner times each
-candidate on a short-horizon probe of the caller's own problem (same system,
-dtype, parameters and a sample of its ICs), keeps the fastest, and caches
-the choice per (device, system, dtype, size bucket) in a JSON file so later
-runs reuse it without timing.
+waves per SIMD stay resident; finished lanes refill from the counter), its
+refill quorum (how many finished lanes of a wave wait before they refill
+together), and the resident-workgroup cap of the Rossler tile kernel.  The tuner times each
+candidate on a probe of the caller's own problem (same system, dtype,
+parameters and ICs; by default the whole run with its save grid, since
+early exits such as divergence stops decide whether refilling pays and may
+come late; no trajectory output), keeps the fastest, and caches the choice per (device,
+system, dtype, size bucket) in a JSON file so later runs reuse it without
+timing.
 """
 
 from __future__ import annotations
@@ -66,11 +69,15 @@ def _time(fn, reps: int = 2) -> float:
     return best
 
 
-def tune_lane_cap(solver, ode_system, y0: torch.Tensor, params: torch.Tensor | None = None,
-                  probe_span: float | None = None, candidates=(0, 3, 2, 1),
-                  use_cache: bool = True) -> int:
-    """Best resident-lane cap (bh_solve_batch max_lanes; 0 = occupancy limit)
-    for this problem.  candidates other than 0 are waves per SIMD."""
+def tune_lane_launch(solver, ode_system, y0: torch.Tensor, params: torch.Tensor | None = None,
+                     probe_span: float | None = None, caps=(0, 3, 2, 1), quorums=(64, 1),
+                     use_cache: bool = True) -> tuple[int, int]:
+    """Best (resident-lane cap, refill quorum) pair for this problem
+    (bh_solve_batch max_lanes, 0 = occupancy limit; refill quorum as in
+    BH_REFILL_QUORUM).  caps other than 0 are waves per SIMD.  Without
+    probe_span each candidate runs the solver's own span and save grid
+    (without trajectory output); with it, a shorter run over [t0, t0 +
+    probe_span] with two saves."""
     from .solvers.hip_dopri5 import functor_id
 
     sid = functor_id(ode_system)
@@ -78,31 +85,47 @@ def tune_lane_cap(solver, ode_system, y0: torch.Tensor, params: torch.Tensor | N
         raise ValueError("no device functor for this ODE system")
     dev = solver.device
     n = y0.shape[0] * (1 if params is None else params.shape[0])
-    key = _key(dev, sid, y0.dtype, n)
+    key = _key(dev, sid, y0.dtype, n) + f"|q{','.join(str(int(q)) for q in quorums)}"
     with _lock:
         cache = _load() if use_cache else {}
         if key in cache:
-            return int(cache[key]["max_lanes"])
+            return int(cache[key]["max_lanes"]), int(cache[key]["refill_quorum"])
     props = torch.cuda.get_device_properties(dev)
     wave_set = props.multi_processor_count * 4 * 64
-    span = probe_span if probe_span is not None else (solver.t_span[1] - solver.t_span[0]) / 20
     results = {}
-    for c in candidates:
+    for c in caps:
         lanes = 0 if c == 0 else int(c * wave_set)
-        probe = type(solver)(t_span=(solver.t_span[0], solver.t_span[0] + span), t_steps=2,
-                             device=str(dev), rtol=solver.rtol, atol=solver.atol, cache_dir=None,
-                             stop_threshold=solver.stop_threshold, inf_fill=solver.inf_fill,
-                             max_lanes=lanes)
-        results[lanes] = _time(lambda: probe.integrate_fused(ode_system, y0, params))
+        for q in quorums:
+            if probe_span is None:
+                # the caller's own run: same span and save grid, no trajectories
+                probe = solver.clone(cache_dir=None)
+                probe.max_lanes = lanes
+                probe.refill_quorum = int(q)
+            else:
+                probe = type(solver)(t_span=(solver.t_span[0], solver.t_span[0] + probe_span),
+                                     t_steps=2, device=str(dev), rtol=solver.rtol, atol=solver.atol,
+                                     cache_dir=None, stop_threshold=solver.stop_threshold,
+                                     inf_fill=solver.inf_fill, max_lanes=lanes, refill_quorum=int(q))
+            results[(lanes, int(q))] = _time(lambda: probe.integrate_fused(ode_system, y0, params))
     best = min(results, key=results.get)
     last_timings.clear()
-    last_timings.update({str(k): round(v, 3) for k, v in results.items()})
+    last_timings.update({f"{k[0]}/q{k[1]}": round(v, 3) for k, v in results.items()})
     with _lock:
         cache = _load() if use_cache else {}
-        cache[key] = {"max_lanes": best, "timings_ms": {str(k): v for k, v in results.items()}}
+        cache[key] = {"max_lanes": best[0], "refill_quorum": best[1],
+                      "timings_ms": {f"{k[0]}/q{k[1]}": v for k, v in results.items()}}
         if use_cache:
             _store(cache)
     return best
+
+
+def tune_lane_cap(solver, ode_system, y0: torch.Tensor, params: torch.Tensor | None = None,
+                  probe_span: float | None = None, candidates=(0, 3, 2, 1),
+                  use_cache: bool = True) -> int:
+    """Best resident-lane cap at the solver's own refill quorum (see
+    tune_lane_launch)."""
+    return tune_lane_launch(solver, ode_system, y0, params, probe_span, caps=candidates,
+                            quorums=(solver.refill_quorum,), use_cache=use_cache)[0]
 
 
 def tune_rossler_tiles(solver, ode_system, y0: torch.Tensor, params: torch.Tensor | None = None,

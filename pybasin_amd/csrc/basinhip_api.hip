@@ -45,9 +45,8 @@ int32_t device_lanes(K kernel, int block, int64_t *lanes) {
 template <class Sys, class T, bool kPeaks>
 int32_t launch_lane(const bh::LaneArgs &args, int32_t max_lanes, hipStream_t stream) {
     constexpr int block = 256;
-    auto kernel = bh::dopri5_lane_kernel<Sys, T, kPeaks>;
     int64_t lanes = 0;
-    int32_t rc = device_lanes(kernel, block, &lanes);
+    int32_t rc = device_lanes(bh::dopri5_lane_kernel<Sys, T, kPeaks, true>, block, &lanes);
     if (rc) return rc;
     // Resident lanes: every trajectory gets its own lane up to the occupancy
     // limit (more resident waves hide the latency of the dependent step
@@ -55,11 +54,18 @@ int32_t launch_lane(const bh::LaneArgs &args, int32_t max_lanes, hipStream_t str
     if (max_lanes > 0 && max_lanes < lanes) lanes = max_lanes;
     if (lanes > args.n) lanes = args.n;
     const int64_t blocks = (lanes + block - 1) / block;
-    // lanes [0, blocks*block) start on their own index; the counter hands out the rest
-    hipLaunchKernelGGL(bh::set_counter_kernel, dim3(1), dim3(64), 0, stream, args.counter,
-                       (unsigned long long)(blocks * block));
-    BH_HIP_CHECK(hipGetLastError());
-    hipLaunchKernelGGL(kernel, dim3((unsigned)blocks), dim3(block), 0, stream, args);
+    if (blocks * block >= args.n) {
+        // every trajectory has a lane from the start: no refills
+        hipLaunchKernelGGL((bh::dopri5_lane_kernel<Sys, T, kPeaks, false>), dim3((unsigned)blocks),
+                           dim3(block), 0, stream, args);
+    } else {
+        // lanes [0, blocks*block) start on their own index; the counter hands out the rest
+        hipLaunchKernelGGL(bh::set_counter_kernel, dim3(1), dim3(64), 0, stream, args.counter,
+                           (unsigned long long)(blocks * block));
+        BH_HIP_CHECK(hipGetLastError());
+        hipLaunchKernelGGL((bh::dopri5_lane_kernel<Sys, T, kPeaks, true>), dim3((unsigned)blocks),
+                           dim3(block), 0, stream, args);
+    }
     BH_HIP_CHECK(hipGetLastError());
     return 0;
 }

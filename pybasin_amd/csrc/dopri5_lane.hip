@@ -36,7 +36,9 @@
 //
 // Work distribution: a resident set of lanes; when a lane's trajectory ends
 // (all saves emitted, divergence stop, or failure) it takes the next index
-// from a device counter, so uneven lengths do not idle the lane.
+// from a device counter, so uneven lengths do not idle the lane.  Finished
+// lanes of a wave refill together once a quorum of them waits (flags bits
+// 8-14), which trades idle lanes for lanes that stay in step.
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <stdint.h>
@@ -45,6 +47,10 @@
 #include "systems.hpp"
 
 namespace bh {
+
+// refill quorum when the flags leave it at 0 (chosen by measurement on the
+// case-study workloads)
+constexpr int kDefaultRefillQuorum = 64;
 
 struct LaneArgs {
     const void *y0;
@@ -489,28 +495,47 @@ __device__ __forceinline__ void lane_finish(const LaneArgs &a, LaneT &L, int64_t
     }
 }
 
-template <class Sys, class T, bool kPeaks>
+// kRefill = false: every trajectory has its own lane (the launch covers n),
+// so a lane steps until its trajectory ends and writes its results; the loop
+// carries no refill bookkeeping.  kRefill = true: a resident set smaller
+// than n, refilled from a.counter.
+template <class Sys, class T, bool kPeaks, bool kRefill>
 __global__ void __launch_bounds__(256) dopri5_lane_kernel(LaneArgs a) {
     int64_t ic = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (ic >= a.n) return;
     Lane<Sys, T, kPeaks> L;
     lane_init<Sys, T>(a, L, ic);
-    bool live = true;
-    // Every pass runs one step for each lane still integrating, and in the
-    // same pass a lane that finished writes its results and takes the next
-    // IC from the counter.  The loop ends by a wave-wide vote: had it ended
-    // per lane, the finished lanes would leave the loop and wait there until
-    // the last lane of the wave finished, and refills would happen once per
-    // wave instead of once per lane.
-    do {
-        if (!L.done) lane_step<Sys, T>(a, L, ic);  // not done unless init emitted every save
-        if (L.done && live) {
-            lane_finish<Sys, T>(a, L, ic);
-            ic = (int64_t)atomicAdd(a.counter, 1ull);
-            live = ic < a.n;
-            if (live) lane_init<Sys, T>(a, L, ic);
-        }
-    } while (__any(live));
+    if constexpr (!kRefill) {
+        while (!L.done) lane_step<Sys, T>(a, L, ic);  // not done unless init emitted every save
+        lane_finish<Sys, T>(a, L, ic);
+    } else {
+        bool live = true;
+        int quorum = (int)((a.flags >> BH_REFILL_QUORUM_SHIFT) & 0x7fu);
+        quorum = quorum == 0 ? kDefaultRefillQuorum : (quorum > 64 ? 64 : quorum);
+        // Every pass runs one step for each lane still integrating; in the
+        // same pass finished lanes write their results and take the next ICs
+        // from the counter once the refill quorum is met.  The loop ends by
+        // a wave-wide vote: had it ended per lane, the finished lanes would
+        // leave the loop and wait there until the last lane of the wave
+        // finished, and refills could only happen once per wave.
+        do {
+            if (!L.done) lane_step<Sys, T>(a, L, ic);
+            bool go = L.done && live;
+            if (quorum > 1) {
+                // wave-uniform counts: finished lanes wait for the quorum, or
+                // for the last lane still integrating
+                const int waiting = __popcll(__ballot(go));
+                const int holding = __popcll(__ballot(live));
+                go = go && waiting >= (quorum < holding ? quorum : holding);
+            }
+            if (go) {
+                lane_finish<Sys, T>(a, L, ic);
+                ic = (int64_t)atomicAdd(a.counter, 1ull);
+                live = ic < a.n;
+                if (live) lane_init<Sys, T>(a, L, ic);
+            }
+        } while (__any(live));
+    }
 }
 
 __global__ void set_counter_kernel(unsigned long long *c, unsigned long long v) {

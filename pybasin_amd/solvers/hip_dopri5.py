@@ -138,12 +138,16 @@ class HipDopri5Solver(Solver):
         stop_threshold: float | None = None,
         inf_fill: bool = False,
         max_lanes: int = 0,
+        refill_quorum: int = 0,
     ):
         super().__init__(t_span, t_steps, device, rtol, atol, cache_dir, t_eval)
         self.max_steps = int(max_steps)
         self.stop_threshold = stop_threshold
         self.inf_fill = inf_fill
         self.max_lanes = int(max_lanes)
+        if not 0 <= int(refill_quorum) <= 64:
+            raise ValueError(f"refill_quorum must be in [0, 64], got {refill_quorum}")
+        self.refill_quorum = int(refill_quorum)
 
     def _get_cache_config(self) -> dict[str, Any]:
         cfg = super()._get_cache_config()
@@ -157,7 +161,7 @@ class HipDopri5Solver(Solver):
             device=device or self._device_str, rtol=self.rtol, atol=self.atol,
             cache_dir=self._cache_dir if cache_dir is UNSET else cache_dir,  # type: ignore[arg-type]
             t_eval=self.t_eval, max_steps=self.max_steps, stop_threshold=self.stop_threshold,
-            inf_fill=self.inf_fill, max_lanes=self.max_lanes)
+            inf_fill=self.inf_fill, max_lanes=self.max_lanes, refill_quorum=self.refill_quorum)
 
     # ------------------------------------------------------------------ #
     def _torch_solver(self) -> TorchDopri5Solver:
@@ -261,7 +265,7 @@ class HipDopri5Solver(Solver):
         status = torch.empty((n,), dtype=torch.int32, device=dev)
         nsteps = torch.empty((n,), dtype=torch.int64, device=dev)
         ws = torch.empty((_lib.BH_WORKSPACE_BYTES,), dtype=torch.uint8, device=dev)
-        flags = 0
+        flags = (self.refill_quorum & 0x7F) << _lib.BH_REFILL_QUORUM_SHIFT
         thr = 0.0
         if self.stop_threshold is not None:
             flags |= _lib.BH_FLAG_STOP

@@ -232,6 +232,30 @@ def test_autotuner_picks_and_caches_a_lane_cap(cuda_device, tmp_path, monkeypatc
     assert best >= 0
     assert (tmp_path / "tune.json").exists()
     assert autotune.tune_lane_cap(solver, ode, y0, probe_span=20.0) == best  # cached
+    cap, q = autotune.tune_lane_launch(solver, ode, y0, probe_span=20.0, caps=(0, 1), quorums=(64, 1))
+    assert cap >= 0 and q in (64, 1)
+    assert autotune.tune_lane_launch(solver, ode, y0, probe_span=20.0, caps=(0, 1),
+                                     quorums=(64, 1)) == (cap, q)  # cached
+
+
+def test_refill_quorum_does_not_change_results(cuda_device):
+    """Per-lane and whole-wave refills (quorum 1 and 64) with divergence
+    stops give bitwise identical fused results."""
+    y0 = torch.tensor(O.sample_uniform(3000, [-10.0, -20.0, 0.0], [10.0, 20.0, 0.0], seed=5),
+                      dtype=torch.float32).to(cuda_device)
+    ode = LorenzODE({"sigma": 0.12, "r": 0.0, "b": -0.6})
+    out = []
+    for q in (1, 7, 64):
+        s = HipDopri5Solver(t_span=(0, 100), t_steps=200, device="cuda", cache_dir=None,
+                            stop_threshold=200.0, inf_fill=True, max_lanes=256, refill_quorum=q)
+        out.append(s.integrate_fused(ode, y0, keep_trajectories=True))
+    for r in out[1:]:
+        assert torch.equal(r.y.nan_to_num(7.0), out[0].y.nan_to_num(7.0))
+        assert torch.equal(r.stats.nan_to_num(7.0), out[0].stats.nan_to_num(7.0))
+        assert torch.equal(r.status, out[0].status) and torch.equal(r.nsteps, out[0].nsteps)
+    assert (out[0].status == 4).any() and (out[0].status == 0).any()
+    with pytest.raises(ValueError):
+        HipDopri5Solver(refill_quorum=65)
 
 
 @pytest.mark.parametrize("name,dtype,kw", [
