@@ -17,6 +17,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -38,7 +39,7 @@ STATE = {"duffing": 2, "pendulum": 2, "lorenz": 3}
 # HBM bytes per dispatch of the Duffing 100k float32 kernel, from rocprofv3
 # FETCH_SIZE + WRITE_SIZE scaled by a plain-copy calibration
 # (profiles/dopri5_lane_duffing_100k_f32_pmc.md)
-TRAFFIC_BYTES = {("duffing", "f32", 100_000): 9.73e6}
+TRAFFIC_BYTES = {("duffing", "f32", 100_000): 9.60e6}
 
 DUFFING_TEMPLATES = [[-0.21, 0.02], [1.05, 0.77], [-0.67, 0.02], [-0.46, 0.30], [-0.43, 0.12]]
 
@@ -54,6 +55,8 @@ def parse():
     ap.add_argument("--max-lanes", type=int, default=0)
     ap.add_argument("--cpu-sample", type=int, default=100_000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--ref-sample", type=int, default=64,
+                    help="ICs for the scipy path (64x for the torch shared-step path); 0 skips them")
     ap.add_argument("--workload", choices=["duffing", "lorenz", "surrogate", "rossler", "mlp_infer",
                                            "hdbscan"],
                     default="duffing",
@@ -98,6 +101,63 @@ def cpu_baseline(y0_cpu, save64, n_sample):
     return {"value": round(n_sample / dt, 1), "unit": "ICs/s", "cores": threads, "kind": "port",
             "sample": f"first {n_sample} of the same seeded Duffing ICs, full t in [0,1000], "
                       f"float32, oracle per-lane Dopri5 (C, pthreads), {dt:.2f} s"}
+
+
+def _duffing_rhs(t, y):
+    return [y[1], -0.08 * y[1] - y[0] ** 3 + 0.2 * math.cos(t)]
+
+
+def _scipy_one(y0, save):
+    from scipy.integrate import solve_ivp
+
+    r = solve_ivp(_duffing_rhs, (0.0, 1000.0), y0, method="RK45", rtol=1e-8, atol=1e-6,
+                  max_step=10.0, t_eval=save)
+    return r.y[:, -1]
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_reference_paths(y0_cpu, save64, n_scipy: int, n_torch: int):
+    """The reference's two CPU solver paths, restated here (the reference
+    itself is not importable), timed on samples of the same ICs before the
+    GPU is touched: ScipyParallelSolver (solve_ivp RK45 per IC over a process
+    pool, max_step = (t1 - t0) / 100, scipy_solver.py:141-151) and
+    TorchDiffEqSolver (one shared-step Dopri5 over the batch in torch on the
+    CPU, torchdiffeq_solver.py:97-134)."""
+    import multiprocessing as mp
+
+    from pybasin_amd.solvers import TorchDopri5Solver
+    from pybasin_amd.systems import DuffingODE
+
+    threads = max(1, min(os.cpu_count() or 1, 16))
+    out = []
+    ys = y0_cpu[:n_scipy].double().numpy()
+    t0 = time.perf_counter()
+    with mp.get_context("fork").Pool(threads) as pool:  # no GPU state exists yet
+        pool.starmap(_scipy_one, [(y, save64) for y in ys])
+    dt = time.perf_counter() - t0
+    out.append({"path": "ScipyParallelSolver counterpart (solve_ivp RK45 per IC, process pool)",
+                "value": round(n_scipy / dt, 1), "unit": "ICs/s", "cores": threads,
+                "sample": f"first {n_scipy} ICs, float64, full t in [0,1000], {dt:.2f} s"})
+    torch.set_num_threads(threads)
+    solver = TorchDopri5Solver(t_span=(0, 1000), t_steps=1000, t_eval=(900.0, 1000.0), device="cpu",
+                               rtol=1e-8, atol=1e-6, cache_dir=None)
+    t0 = time.perf_counter()
+    solver.integrate(DuffingODE({"delta": 0.08, "k3": 1.0, "A": 0.2}), y0_cpu[:n_torch].clone())
+    dt = time.perf_counter() - t0
+    out.append({"path": "TorchDiffEqSolver counterpart (shared-step Dopri5, torch CPU)",
+                "value": round(n_torch / dt, 1), "unit": "ICs/s", "cores": threads,
+                "sample": f"first {n_torch} ICs, float32, full t in [0,1000], {dt:.2f} s"})
+    return out
 
 
 def surrogate_workload(args):
@@ -479,6 +539,12 @@ def main():
     lo = torch.tensor([-1.0, -0.5])
     hi = torch.tensor([1.0, 1.0])
     y0_cpu = torch.rand(n, 2) * (hi - lo) + lo
+    ref_paths = None
+    if not args.no_cpu and args.ref_sample > 0:
+        import numpy as np
+
+        ref_paths = cpu_reference_paths(y0_cpu, np.linspace(900.0, 1000.0, 1000), args.ref_sample,
+                                        64 * args.ref_sample)
     y0_all = torch.cat([y0_cpu, torch.tensor(DUFFING_TEMPLATES)]).to(dtype).to(dev)
     ode = DuffingODE({"delta": 0.08, "k3": 1.0, "A": 0.2})
     solver = HipDopri5Solver(t_span=(0, 1000), t_steps=1000, t_eval=(900.0, 1000.0), device="cuda",
@@ -561,6 +627,9 @@ def main():
 
         save64 = save_ts.double().cpu().numpy().astype(np.float64)
         out["cpu_baseline"] = cpu_baseline(y0_cpu, save64, min(args.cpu_sample, n))
+        out["cpu_baseline"]["cpu_model"] = cpu_model()
+        if ref_paths is not None:
+            out["cpu_reference_paths"] = ref_paths
     print(json.dumps(out), flush=True)
 
 
