@@ -55,6 +55,8 @@ def parse():
     ap.add_argument("--max-lanes", type=int, default=0)
     ap.add_argument("--cpu-sample", type=int, default=100_000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--keep-traj", action="store_true",
+                    help="duffing: also write the full trajectories to HBM (reports write GB/s)")
     ap.add_argument("--ref-sample", type=int, default=64,
                     help="ICs for the scipy path (64x for the torch shared-step path); 0 skips them")
     ap.add_argument("--workload", choices=["duffing", "lorenz", "surrogate", "rossler", "mlp_infer",
@@ -557,8 +559,8 @@ def main():
         e0 = torch.cuda.Event(enable_timing=True)
         e1 = torch.cuda.Event(enable_timing=True)
         e0.record(stream)
-        res = solver.run_fused(ode, y0_all, save_ts, None, keep_trajectories=False, steady_idx=0,
-                               stream=stream)
+        res = solver.run_fused(ode, y0_all, save_ts, None, keep_trajectories=args.keep_traj,
+                               steady_idx=0, stream=stream)
         e1.record(stream)
         if record:
             kev.append((e0, e1))
@@ -617,11 +619,17 @@ def main():
             "bound": "valu", "achieved": round(achieved / 1e12, 3),
             "peak": round(peak / 1e12, 1), "unit": "TFLOP/s",
             "frac": round(achieved / peak, 4),
-            "traffic": TRAFFIC_BYTES.get(("duffing", args.dtype, n)),
+            "traffic": None if args.keep_traj else TRAFFIC_BYTES.get(("duffing", args.dtype, n)),
             "kernel": f"bh::dopri5_lane_kernel<Duffing,{'float' if dtype == torch.float32 else 'double'}>",
             "kernel_ms": round(kernel_ms, 3),
         },
     }
+    if args.keep_traj:
+        # trajectories (1000, n + 5, 2) written to HBM inside the kernel
+        traj_bytes = float(res.y.numel() * res.y.element_size())
+        out["config"]["trajectories"] = "kept (n_save, n, S) in HBM"
+        out["traj_write"] = {"bytes": traj_bytes, "GBps": round(traj_bytes / (kernel_ms * 1e-3) / 1e9, 1),
+                             "hbm_peak_GBps": 8000.0}
     if not args.no_cpu:
         import numpy as np
 
