@@ -16,12 +16,18 @@ ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "libbasinhip.so")
 
-SOURCES = ["basinhip_api.hip"]
+# translation units and their extra flags: the per-lane solver is scheduled
+# for instruction-level parallelism (a wave alone on its SIMD waits on its
+# dependent arithmetic; measured 3 % faster on the Duffing launch)
+SOURCES = {
+    "basinhip_api.hip": [],
+    "lane_api.hip": ["-mllvm", "-amdgpu-sched-strategy=max-ilp"],
+}
 
 # Packed FP32 (v_pk_*_f32) is disabled for every HIP object: some of its
 # forms have produced silently wrong results on these devices.
 HIPCC_FLAGS = [
-    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-result", "-Wno-unused-command-line-argument",
 ]
@@ -35,13 +41,28 @@ def hipcc() -> str:
 
 
 def build(verbose: bool = False) -> str:
-    srcs = [os.path.join(CSRC, s) for s in SOURCES]
+    """Compile every translation unit (in parallel) and link the library."""
+    tmpdir = LIB + ".objs"
+    os.makedirs(tmpdir, exist_ok=True)
+    procs, objs = [], []
+    for src, extra in SOURCES.items():
+        obj = os.path.join(tmpdir, src.replace(".hip", ".o"))
+        cmd = [hipcc(), *HIPCC_FLAGS, *extra, "-I", os.path.join(ROOT, "include"), "-c", "-o", obj,
+               os.path.join(CSRC, src)]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        procs.append((cmd, subprocess.Popen(cmd)))
+        objs.append(obj)
+    for cmd, pr in procs:
+        if pr.wait() != 0:
+            raise subprocess.CalledProcessError(pr.returncode, cmd)
     tmp = LIB + ".tmp"
-    cmd = [hipcc(), *HIPCC_FLAGS, "-I", os.path.join(ROOT, "include"), "-o", tmp, *srcs]
+    cmd = [hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
+    shutil.rmtree(tmpdir, ignore_errors=True)
     return LIB
 
 

@@ -449,6 +449,12 @@ __global__ void __launch_bounds__(THREADS) tile_kernel(Args a) {
     }
 }
 
+// the refill counter's start value, written by a kernel on the launch's
+// stream (no memset, so the pair can be captured in a graph)
+__global__ void set_counter_kernel(unsigned long long *c, unsigned long long v) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *c = v;
+}
+
 }  // namespace ross
 
 }  // namespace bh
