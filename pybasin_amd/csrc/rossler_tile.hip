@@ -10,8 +10,9 @@
 // whole launch, and read their A operand from the stage inputs staged in LDS.
 //
 // Step control is per trajectory with the torchdiffeq rules used by the
-// per-lane kernel (error RMS over the 300 components, I-controller, Hairer
-// initial step, dense output at the last save time without clamping the step).
+// per-lane kernel (error RMS over the 300 components with the quotient by
+// the hardware reciprocal, as there; I-controller, Hairer initial step,
+// dense output at the last save time without clamping the step).
 // Per-trajectory scalars are replicated in the 32 threads that own the
 // trajectory; the reductions they need (initial-step norms, error norm,
 // max |y| for the divergence stop) go through LDS in a fixed order, so every
@@ -216,6 +217,10 @@ __global__ void __launch_bounds__(THREADS) tile_kernel(Args a) {
 
     for (;;) {
         if (!__syncthreads_or(mode != IDLE)) break;
+        // the initial-step norms are needed only in iterations where some
+        // slot of the tile starts a trajectory (a tile-uniform test, so the
+        // reductions inside stay reached by every thread or by none)
+        const bool any_init = __syncthreads_or(mode == INIT);
         const bool stepping = mode == STEP, init = mode == INIT;
         const double tB = t + dt;
         const float h = stepping ? (float)dt : 0.0f;
@@ -226,7 +231,7 @@ __global__ void __launch_bounds__(THREADS) tile_kernel(Args a) {
 #pragma unroll
             for (int r = 0; r < NPT; r++) ys[c][r] = y[c][r] + k1[c][r] * (A21 * h);
         rhs(k2);
-        {
+        if (any_init) {
             // initial-step norms d0, d1 (used by INIT slots only)
             float s0 = 0.f, s1 = 0.f;
 #pragma unroll
@@ -249,8 +254,8 @@ __global__ void __launch_bounds__(THREADS) tile_kernel(Args a) {
                 // stash d1 in dt until the second evaluation
                 dt = (double)d1;
             }
+            __syncthreads();
         }
-        __syncthreads();
 
         // ---- round 2: k3 = f(y + h(a31 k1 + a32 k2)); INIT: f1 = f(y0 + h0 f0) ----
         {
@@ -262,7 +267,7 @@ __global__ void __launch_bounds__(THREADS) tile_kernel(Args a) {
                 for (int r = 0; r < NPT; r++) ys[c][r] = y[c][r] + (k1[c][r] * c1 + k2[c][r] * c2);
         }
         rhs(k3);
-        {
+        if (any_init) {
             float s2 = 0.f;
 #pragma unroll
             for (int c = 0; c < 3; c++)
@@ -285,8 +290,8 @@ __global__ void __launch_bounds__(THREADS) tile_kernel(Args a) {
                     h1 = powf(0.01f / fmaxf(d1, d2), 0.2f);
                 dt = (double)fminf(100.0f * h0, fabsf(h1));
             }
+            __syncthreads();
         }
-        __syncthreads();
 
         // ---- rounds 3..6 (stage inputs; zero coefficients for non-stepping slots) ----
         {
@@ -340,7 +345,7 @@ __global__ void __launch_bounds__(THREADS) tile_kernel(Args a) {
                         const float err = k1[c][r] * e1 + k3[c][r] * e3 + k4[c][r] * e4 +
                                           k5[c][r] * e5 + k6[c][r] * e6 + k7[c][r] * e7;
                         const float tol = ratol + rrtol * fmaxf(fabsf(y[c][r]), fabsf(ys[c][r]));
-                        const float v = err / tol;
+                        const float v = err * __builtin_amdgcn_rcpf(tol);
                         se += v * v;
                         ymax = fmaxf(ymax, fabsf(ys[c][r]));
                     }
