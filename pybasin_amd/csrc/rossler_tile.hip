@@ -112,6 +112,29 @@ __device__ __forceinline__ float slot_reduce(float v, float (*buf)[TILE], int wa
     return r;
 }
 
+// The sum of one value and the maximum of another over the threads of each
+// slot, with one barrier.
+__device__ __forceinline__ void slot_sum_max(float &sum, float &mx, float (*bs)[TILE],
+                                             float (*bm)[TILE], int wave, int lane, int m) {
+    sum += __shfl_xor(sum, 16);
+    mx = fmaxf(mx, __shfl_xor(mx, 16));
+    sum += __shfl_xor(sum, 32);
+    mx = fmaxf(mx, __shfl_xor(mx, 32));
+    if (lane < TILE) {
+        bs[wave][lane] = sum;
+        bm[wave][lane] = mx;
+    }
+    __syncthreads();
+    float rs = bs[0][m], rm = bm[0][m];
+#pragma unroll
+    for (int w = 1; w < THREADS / 64; w++) {
+        rs += bs[w][m];
+        rm = fmaxf(rm, bm[w][m]);
+    }
+    sum = rs;
+    mx = rm;
+}
+
 __global__ void __launch_bounds__(THREADS) tile_kernel(Args a) {
     __shared__ float Xs[TILE * XS];
     __shared__ float Cs[TILE * XS];
@@ -351,8 +374,7 @@ __global__ void __launch_bounds__(THREADS) tile_kernel(Args a) {
                     }
                 }
         }
-        se = slot_reduce<false>(se, red0, wave, lane, m);
-        ymax = slot_reduce<true>(ymax, red1, wave, lane, m);
+        slot_sum_max(se, ymax, red0, red1, wave, lane, m);
 
         bool finished = false;
         int32_t st = BH_LANE_OK;
