@@ -6,8 +6,9 @@
 // with L the 100 x 100 graph Laplacian (rossler_network_topology.py:143-156).
 // The coupling for the 16 trajectories of a tile is one (16 x 100) . (100 x 100)
 // product per stage on the matrix cores (v_mfma_f32_16x16x4f32, FP32 in and
-// out): seven waves each own a 16-column block of L, held in registers for the
-// whole launch, and read their A operand from the stage inputs staged in LDS.
+// out): each of the eight waves owns a 16-column block of L (seven cover
+// the 100 nodes, the eighth is zero padding), held in registers for the
+// whole launch, and reads its A operand from the stage inputs staged in LDS.
 //
 // Step control is per trajectory with the torchdiffeq rules used by the
 // per-lane kernel (error RMS over the 300 components with the quotient by
@@ -42,7 +43,6 @@ constexpr int THREADS = 512;  // 8 waves
 constexpr int NPT = 4;        // node slots per thread (last one real for q < 4)
 constexpr int XS = 101;       // LDS row stride (odd: spreads the A-operand reads over banks)
 constexpr int KB = NN / 4;    // k-blocks of the 16x16x4 product
-constexpr int NB = 7;         // 16-column blocks covering 100 nodes
 
 enum Mode : int { IDLE = 0, INIT = 1, STEP = 2 };
 
@@ -141,19 +141,21 @@ __global__ void __launch_bounds__(THREADS) tile_kernel(Args a) {
     __shared__ float red0[THREADS / 64][TILE];
     __shared__ float red1[THREADS / 64][TILE];
     __shared__ long long slot_ic[TILE];
+    __shared__ int slot_new[TILE];  // the slot took a new trajectory at the last refill
 
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int m = tid & 15, q = tid >> 4;
     const bool has4 = q < NN - 3 * 32;  // fourth node slot (q + 96) is real
 
-    // B operand: this wave's 16-column block of L, all 25 k-blocks
+    // B operand: this wave's 16-column block of L, all 25 k-blocks (the
+    // eighth wave's block lies past column 99 and is zero: it runs the same
+    // products as the others, which keeps the stage code free of branches
+    // so the local terms can issue while the matrix cores work)
     float Bop[KB];
-    if (wave < NB) {
 #pragma unroll
-        for (int kb = 0; kb < KB; kb++) {
-            const int k = 4 * kb + (lane >> 4), j = 16 * wave + (lane & 15);
-            Bop[kb] = (j < NN) ? a.lap[k * NN + j] : 0.0f;
-        }
+    for (int kb = 0; kb < KB; kb++) {
+        const int k = 4 * kb + (lane >> 4), j = 16 * wave + (lane & 15);
+        Bop[kb] = (j < NN) ? a.lap[k * NN + j] : 0.0f;
     }
 
     // per-thread state for its node slots: x, y, z of nodes q + 32 r
@@ -215,35 +217,41 @@ __global__ void __launch_bounds__(THREADS) tile_kernel(Args a) {
         for (int r = 0; r < NPT; r++)
             if (r < 3 || has4) Xs[m * XS + q + 32 * r] = ys[0][r];
         __syncthreads();
-        if (wave < NB) {
-            floatx4 acc = {0.f, 0.f, 0.f, 0.f};
-            const float *xrow = Xs + (lane & 15) * XS + (lane >> 4);
+        // local terms (no coupling) next to the coupling product
+        float part[NPT];
 #pragma unroll
-            for (int kb = 0; kb < KB; kb++)
-                acc = __builtin_amdgcn_mfma_f32_16x16x4f32(xrow[4 * kb], Bop[kb], acc, 0, 0, 0);
-            const int col = 16 * wave + (lane & 15);
-            if (col < NN) {
+        for (int r = 0; r < NPT; r++) {
+            const float xx = ys[0][r], yy = ys[1][r], zz = ys[2][r];
+            part[r] = -yy - zz;
+            out[1][r] = xx + pa * yy;
+            out[2][r] = pb + zz * (xx - pc);
+        }
+        floatx4 acc = {0.f, 0.f, 0.f, 0.f};
+        const float *xrow = Xs + (lane & 15) * XS + (lane >> 4);
 #pragma unroll
-                for (int it = 0; it < 4; it++) Cs[(4 * (lane >> 4) + it) * XS + col] = acc[it];
-            }
+        for (int kb = 0; kb < KB; kb++)
+            acc = __builtin_amdgcn_mfma_f32_16x16x4f32(xrow[4 * kb], Bop[kb], acc, 0, 0, 0);
+        const int col = 16 * wave + (lane & 15);
+        if (col < NN) {
+#pragma unroll
+            for (int it = 0; it < 4; it++) Cs[(4 * (lane >> 4) + it) * XS + col] = acc[it];
         }
         __syncthreads();
 #pragma unroll
         for (int r = 0; r < NPT; r++) {
             const float cpl = (r < 3 || has4) ? Cs[m * XS + q + 32 * r] : 0.0f;
-            const float xx = ys[0][r], yy = ys[1][r], zz = ys[2][r];
-            out[0][r] = -yy - zz - pK * cpl;
-            out[1][r] = xx + pa * yy;
-            out[2][r] = pb + zz * (xx - pc);
+            out[0][r] = part[r] - pK * cpl;
         }
     };
 
+    // The initial-step norms are needed only in iterations where some slot of
+    // the tile starts a trajectory: every slot at the first iteration, later
+    // the slots refilled at the end of the previous one.  any_init is the
+    // same in every thread, so the reductions it guards are reached by every
+    // thread or by none.
+    bool any_init = true;
     for (;;) {
         if (!__syncthreads_or(mode != IDLE)) break;
-        // the initial-step norms are needed only in iterations where some
-        // slot of the tile starts a trajectory (a tile-uniform test, so the
-        // reductions inside stay reached by every thread or by none)
-        const bool any_init = __syncthreads_or(mode == INIT);
         const bool stepping = mode == STEP, init = mode == INIT;
         const double tB = t + dt;
         const float h = stepping ? (float)dt : 0.0f;
@@ -458,12 +466,21 @@ __global__ void __launch_bounds__(THREADS) tile_kernel(Args a) {
         }
 
         // ---- finished slots record their result and take the next trajectory ----
-        if (finished && q == 0) {
-            if (a.status) a.status[ic] = st;
-            if (a.nsteps) a.nsteps[ic] = steps;
-            slot_ic[m] = (long long)atomicAdd(a.counter, 1ull);
+        if (q == 0) {
+            int renewed = 0;
+            if (finished) {
+                if (a.status) a.status[ic] = st;
+                if (a.nsteps) a.nsteps[ic] = steps;
+                const long long next = (long long)atomicAdd(a.counter, 1ull);
+                slot_ic[m] = next;
+                renewed = next < a.n;
+            }
+            slot_new[m] = renewed;
         }
         __syncthreads();
+        any_init = false;
+#pragma unroll
+        for (int j = 0; j < TILE; j++) any_init = any_init || slot_new[j] != 0;
         if (finished) {
             ic = slot_ic[m];
             if (ic < a.n) {
