@@ -59,7 +59,8 @@ def core_distances(X: torch.Tensor, min_samples: int) -> torch.Tensor:
 
 
 def mutual_reachability_mst(X: torch.Tensor, core: torch.Tensor):
-    """Spanning-tree edges (a, b, w) of the mutual-reachability graph (device)."""
+    """Spanning-tree edges (a, b, w) of the mutual-reachability graph (device),
+    in increasing weight (equal weights in the order the rounds found them)."""
     n, F = X.shape
     a = torch.empty((n - 1,), dtype=torch.int64, device=X.device)
     b = torch.empty_like(a)
@@ -71,7 +72,9 @@ def mutual_reachability_mst(X: torch.Tensor, core: torch.Tensor):
         _lib.check(_lib.lib().bh_hdbscan_mst(X.data_ptr(), n, F, core.data_ptr(), a.data_ptr(),
                                              b.data_ptr(), w.data_ptr(), ws.data_ptr(), ws.numel(),
                                              s.cuda_stream), "bh_hdbscan_mst")
-    return a, b, w
+    # increasing weight (stable), so the host hierarchy build skips its sort
+    w, order = torch.sort(w, stable=True)
+    return a[order], b[order], w
 
 
 def labels_from_mst(a: np.ndarray, b: np.ndarray, w: np.ndarray, n: int,
@@ -158,6 +161,7 @@ class HipHDBSCANClusterer:
             a, b, w = self._tree(X, ms)
             for size, lab in zip(group, labels_from_mst(a, b, w, X.shape[0], group)):
                 labels[size] = lab
+        self._tuned_labels = labels
         seen: dict[bytes, float] = {}  # candidate sizes often give the same clustering
         for size in sizes:
             lab = labels[size]
@@ -196,7 +200,9 @@ class HipHDBSCANClusterer:
             size = self._find_optimal_min_cluster_size(Xd)
             self.min_cluster_size = size
             self.min_samples = min(10, size // 5)
-        lab = self._hdbscan(Xd, self.min_cluster_size, self.min_samples)
+            lab = self._tuned_labels[size]  # same tree, same size: the labels are these
+        else:
+            lab = self._hdbscan(Xd, self.min_cluster_size, self.min_samples)
         if self.assign_noise:
             lab = self._assign_noise(Xd, lab)
         self.labels_ = lab

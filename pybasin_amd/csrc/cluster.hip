@@ -255,9 +255,18 @@ struct Hierarchy {
 };
 
 inline Hierarchy single_linkage(int64_t n, const int64_t *ea, const int64_t *eb, const double *w) {
+    // increasing weight, equal weights in edge order (a stable sort), sorted
+    // as contiguous (weight, index) pairs; edges that arrive sorted (the
+    // device sorts them) are taken as they are
     std::vector<int64_t> order(n - 1);
-    std::iota(order.begin(), order.end(), 0);
-    std::stable_sort(order.begin(), order.end(), [&](int64_t p, int64_t q) { return w[p] < w[q]; });
+    if (std::is_sorted(w, w + (n - 1))) {
+        std::iota(order.begin(), order.end(), 0);
+    } else {
+        std::vector<std::pair<double, int64_t>> keyed(n - 1);
+        for (int64_t e = 0; e < n - 1; e++) keyed[e] = {w[e], e};
+        std::sort(keyed.begin(), keyed.end());
+        for (int64_t e = 0; e < n - 1; e++) order[e] = keyed[e].second;
+    }
     Hierarchy h;
     h.left.resize(n - 1);
     h.right.resize(n - 1);
@@ -293,14 +302,21 @@ struct Condensed {
 // first from the root; cluster ids n, n + 1, ... in that order).
 inline Condensed condense(const Hierarchy &h, int64_t n, int64_t mcs) {
     Condensed c;
+    c.parent.reserve(n + 64);
+    c.child.reserve(n + 64);
+    c.csize.reserve(n + 64);
+    c.lambda.reserve(n + 64);
     const int64_t root = 2 * n - 2;
     auto count = [&](int64_t node) { return node >= n ? h.size[node - n] : (int64_t)1; };
     std::vector<int64_t> relabel(2 * n - 1, -1);
     relabel[root] = n;
     int64_t next_label = n + 1;
     // points below a pruned node fall out at the pruning lambda
+    std::vector<int64_t> stack;
+    stack.reserve(64);
     auto fall_out = [&](int64_t top, int64_t cluster, double lam) {
-        std::vector<int64_t> stack{top};
+        stack.clear();
+        stack.push_back(top);
         while (!stack.empty()) {
             const int64_t v = stack.back();
             stack.pop_back();
@@ -400,8 +416,9 @@ inline int32_t select_and_label(const Condensed &c, int64_t n, int32_t *labels) 
     for (size_t e = 0; e < c.child.size(); e++)
         if (c.child[e] < n) point_parent[c.child[e]] = c.parent[e] - n;
     std::vector<int32_t> resolved(nc, -2);
+    std::vector<int64_t> path;
     auto cluster_label = [&](int64_t k) {
-        std::vector<int64_t> path;
+        path.clear();
         int32_t lab = -1;
         while (k >= 0) {
             if (resolved[k] != -2) {
