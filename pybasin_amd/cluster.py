@@ -58,9 +58,30 @@ def core_distances(X: torch.Tensor, min_samples: int) -> torch.Tensor:
     return d[:, -1].contiguous()
 
 
+def spatial_order(X: torch.Tensor) -> torch.Tensor:
+    """Row order along a Z-order (Morton) curve of the first one to three
+    features, each quantised to 21 bits over its range: rows close in the
+    order are close in space, which lets the spanning-tree rounds skip
+    whole tiles of rows."""
+    d = min(X.shape[1], 3)
+    lo = X[:, :d].min(0).values
+    span = (X[:, :d].max(0).values - lo).clamp_min(1e-300)
+    u = ((X[:, :d] - lo) / span * (2**21 - 1)).clamp(0, 2**21 - 1).to(torch.int64)
+    code = torch.zeros(X.shape[0], dtype=torch.int64, device=X.device)
+    for b in range(21):
+        for f in range(d):
+            code |= ((u[:, f] >> b) & 1) << (b * d + f)
+    return torch.argsort(code, stable=True)
+
+
 def mutual_reachability_mst(X: torch.Tensor, core: torch.Tensor):
     """Spanning-tree edges (a, b, w) of the mutual-reachability graph (device),
-    in increasing weight (equal weights in the order the rounds found them)."""
+    in increasing weight (equal weights in the order the rounds found them).
+    The rounds run on the rows in spatial order; edges come back in the
+    caller's row indices."""
+    perm = spatial_order(X)
+    X = X[perm].contiguous()
+    core = core[perm].contiguous()
     n, F = X.shape
     a = torch.empty((n - 1,), dtype=torch.int64, device=X.device)
     b = torch.empty_like(a)
@@ -74,7 +95,7 @@ def mutual_reachability_mst(X: torch.Tensor, core: torch.Tensor):
                                              s.cuda_stream), "bh_hdbscan_mst")
     # increasing weight (stable), so the host hierarchy build skips its sort
     w, order = torch.sort(w, stable=True)
-    return a[order], b[order], w
+    return perm[a[order]], perm[b[order]], w
 
 
 def labels_from_mst(a: np.ndarray, b: np.ndarray, w: np.ndarray, n: int,

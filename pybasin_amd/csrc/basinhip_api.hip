@@ -40,14 +40,25 @@ int32_t launch_knn(const double *Q, int64_t nq, const double *R, int64_t nr, int
 }
 
 template <int FC>
+int32_t tile_bounds(const double *X, int64_t n, int32_t F, const double *core_sq, int64_t n_tiles,
+                    double *lo, double *hi, double *mincore, hipStream_t s) {
+    hipLaunchKernelGGL(bh::clu::tile_bounds_kernel<FC>, dim3((unsigned)((n_tiles + 255) / 256)),
+                       dim3(256), 0, s, X, n, F, core_sq, n_tiles, lo, hi, mincore);
+    BH_HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
+template <int FC>
 int32_t boruvka_round(const double *X, int64_t n, int32_t F, const double *core_sq,
-                      const int32_t *comp, double *best_key, int64_t *best_j, hipStream_t s) {
+                      const int32_t *comp, const double *lo, const double *hi, const double *mincore,
+                      int32_t *tcomp, int64_t n_tiles, double *best_key, int64_t *best_j,
+                      hipStream_t s) {
     namespace C = bh::clu;
-    int64_t chunk = (n + C::MST_CHUNKS - 1) / C::MST_CHUNKS;
-    chunk = (chunk + C::TILE - 1) / C::TILE * C::TILE;
-    hipLaunchKernelGGL(C::boruvka_kernel<FC>,
-                       dim3((unsigned)((n + C::TILE - 1) / C::TILE), C::MST_CHUNKS), dim3(C::TILE), 0,
-                       s, X, n, F, core_sq, comp, chunk, best_key, best_j);
+    hipLaunchKernelGGL(C::tile_comp_kernel, dim3((unsigned)((n_tiles + 255) / 256)), dim3(256), 0, s,
+                       comp, n, n_tiles, tcomp);
+    BH_HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(C::boruvka_kernel<FC>, dim3((unsigned)n_tiles), dim3(C::QT), 0, s, X, n, F,
+                       core_sq, comp, lo, hi, mincore, tcomp, n_tiles, best_key, best_j);
     BH_HIP_CHECK(hipGetLastError());
     return 0;
 }
@@ -302,7 +313,11 @@ int32_t bh_knn(const double *Q, int64_t nq, const double *R, int64_t nr, int32_t
 }
 
 int64_t bh_hdbscan_workspace_bytes(int64_t n) {
-    return n * (8 + 4 + 16 * bh::clu::MST_CHUNKS) + 1024;
+    namespace C = bh::clu;
+    const int64_t tiles = (n + C::QT - 1) / C::QT;
+    // squared cores, best keys, best partners, components; per tile a box
+    // of 2 x MAXF slots, the smallest core, the shared component; alignment
+    return n * (8 + 8 + 8 + 4) + tiles * ((2 * C::MAXF + 1) * 8 + 4) + 1024;
 }
 
 int32_t bh_hdbscan_mst(const double *X, int64_t n, int32_t F, const double *core, int64_t *out_a,
@@ -320,11 +335,15 @@ int32_t bh_hdbscan_mst(const double *X, int64_t n, int32_t F, const double *core
     if (n > 2147483647LL) return fail(-5, "bh_hdbscan_mst: n too large");
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     char *ws = reinterpret_cast<char *>(workspace);
-    const int64_t nch = C::MST_CHUNKS;
+    const int64_t n_tiles = (n + C::QT - 1) / C::QT;
     double *core_sq = reinterpret_cast<double *>(ws);
-    double *best_key = core_sq + n;                                   // (chunks, n)
-    int64_t *best_j = reinterpret_cast<int64_t *>(best_key + nch * n);  // (chunks, n)
-    int32_t *comp_d = reinterpret_cast<int32_t *>(best_j + nch * n);
+    double *best_key = core_sq + n;
+    int64_t *best_j = reinterpret_cast<int64_t *>(best_key + n);
+    double *tlo = reinterpret_cast<double *>(best_j + n);
+    double *thi = tlo + n_tiles * C::MAXF;
+    double *tmin = thi + n_tiles * C::MAXF;
+    int32_t *comp_d = reinterpret_cast<int32_t *>(tmin + n_tiles);
+    int32_t *tcomp = comp_d + n;
     // squared core distances for the comparison key
     std::vector<double> core_h(n);
     BH_HIP_CHECK(hipMemcpyAsync(core_h.data(), core, n * sizeof(double), hipMemcpyDeviceToHost, s));
@@ -332,39 +351,25 @@ int32_t bh_hdbscan_mst(const double *X, int64_t n, int32_t F, const double *core
     std::vector<double> csq(n);
     for (int64_t i = 0; i < n; i++) csq[i] = core_h[i] * core_h[i];
     BH_HIP_CHECK(hipMemcpyAsync(core_sq, csq.data(), n * sizeof(double), hipMemcpyHostToDevice, s));
+    int32_t rc = 0;
+    BH_BY_SLOTS(F, (tile_bounds<FCV>(X, n, F, core_sq, n_tiles, tlo, thi, tmin, s)));
+    if (rc) return rc;
     C::UnionFind uf(n);
     std::vector<int32_t> comp(n);
     std::iota(comp.begin(), comp.end(), 0);
-    std::vector<double> bkc(nch * n), bk(n);
-    std::vector<int64_t> bjc(nch * n), bj(n), ea, eb;
+    std::vector<double> bk(n);
+    std::vector<int64_t> bj(n), ea, eb;
     ea.reserve(n - 1);
     eb.reserve(n - 1);
     int64_t n_comp = n;
     while (n_comp > 1) {
         BH_HIP_CHECK(hipMemcpyAsync(comp_d, comp.data(), n * sizeof(int32_t), hipMemcpyHostToDevice, s));
-        int32_t rc = 0;
-        BH_BY_SLOTS(F, (boruvka_round<FCV>(X, n, F, core_sq, comp_d, best_key, best_j, s)));
+        BH_BY_SLOTS(F, (boruvka_round<FCV>(X, n, F, core_sq, comp_d, tlo, thi, tmin, tcomp, n_tiles,
+                                           best_key, best_j, s)));
         if (rc) return rc;
-        BH_HIP_CHECK(hipMemcpyAsync(bkc.data(), best_key, nch * n * sizeof(double),
-                                    hipMemcpyDeviceToHost, s));
-        BH_HIP_CHECK(hipMemcpyAsync(bjc.data(), best_j, nch * n * sizeof(int64_t),
-                                    hipMemcpyDeviceToHost, s));
+        BH_HIP_CHECK(hipMemcpyAsync(bk.data(), best_key, n * sizeof(double), hipMemcpyDeviceToHost, s));
+        BH_HIP_CHECK(hipMemcpyAsync(bj.data(), best_j, n * sizeof(int64_t), hipMemcpyDeviceToHost, s));
         BH_HIP_CHECK(hipStreamSynchronize(s));
-        // best over chunks (chunks hold increasing row ranges: the first of
-        // equal keys has the lower partner index)
-        for (int64_t i = 0; i < n; i++) {
-            double k = INFINITY;
-            int64_t j = -1;
-            for (int64_t ch = 0; ch < nch; ch++) {
-                const int64_t cj = bjc[ch * n + i];
-                if (cj >= 0 && bkc[ch * n + i] < k) {
-                    k = bkc[ch * n + i];
-                    j = cj;
-                }
-            }
-            bk[i] = k;
-            bj[i] = j;
-        }
         // cheapest edge per component (ties: lower key, then lower point, then lower partner)
         std::vector<int64_t> pick(n, -1);
         for (int64_t i = 0; i < n; i++) {
@@ -395,7 +400,6 @@ int32_t bh_hdbscan_mst(const double *X, int64_t n, int32_t F, const double *core
     const int64_t m = (int64_t)ea.size();
     BH_HIP_CHECK(hipMemcpyAsync(out_a, ea.data(), m * sizeof(int64_t), hipMemcpyHostToDevice, s));
     BH_HIP_CHECK(hipMemcpyAsync(out_b, eb.data(), m * sizeof(int64_t), hipMemcpyHostToDevice, s));
-    int32_t rc = 0;
     BH_BY_SLOTS(F, (edge_weights<FCV>(X, F, core, out_a, out_b, m, out_w, s)));
     if (rc) return rc;
     BH_HIP_CHECK(hipStreamSynchronize(s));

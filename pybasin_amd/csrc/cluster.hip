@@ -33,7 +33,6 @@ namespace clu {
 constexpr int TILE = 256;   // reference points staged per LDS tile
 constexpr int MAXF = 16;    // features (rows are padded to 2, 4, 8 or 16 slots)
 constexpr int MAXK = 32;    // neighbours
-constexpr int MST_CHUNKS = 8;  // reference-row chunks per Boruvka round
 
 // Kernels are instantiated for FC = 2, 4, 8, 16 feature slots; rows hold F
 // <= FC features and the slots beyond F are zero (adding (0 - 0)^2 = +0 to a
@@ -121,52 +120,117 @@ __global__ void __launch_bounds__(TILE) knn_kernel(const double *Q, int64_t nq, 
     }
 }
 
-// One Boruvka round: for each point the cheapest mutual-reachability edge
-// to a point of another component.  The comparison key is
-// max(core_a^2, core_b^2, d^2) (monotone in the edge weight); ties go to the
-// lower index.  The reference rows are split into gridDim.y chunks (more
-// waves in flight than one row per lane gives); chunk y writes its best to
-// best_key / best_j[y * n + i] (best_j = -1: no row of another component in
-// the chunk), and the host takes the minimum over chunks.
+// Boruvka rounds over tiles of QT consecutive rows.  Rows should arrive in
+// a spatially coherent order (the caller sorts them along a space-filling
+// curve): then a tile's bounding box is small, and a query can skip every
+// tile whose lower bound max(core_a^2, min core^2 of the tile, squared
+// distance to the box) already loses to its best edge, or whose rows all
+// lie in the query's own component.
+constexpr int QT = 64;
+
+// per-tile bounding box (FC slots; padded slots are 0) and the smallest
+// squared core distance, once per spanning tree
 template <int FC>
-__global__ void __launch_bounds__(TILE) boruvka_kernel(const double *X, int64_t n, int F,
-                                                       const double *core_sq, const int32_t *comp,
-                                                       int64_t chunk, double *best_key,
-                                                       int64_t *best_j) {
-    __shared__ double tile[TILE * FC];
-    __shared__ double tcore[TILE];
-    __shared__ int32_t tcomp[TILE];
-    const int64_t i = (int64_t)blockIdx.x * TILE + threadIdx.x;
-    const int64_t j0 = (int64_t)blockIdx.y * chunk, j1 = min(n, j0 + chunk);
-    double x[FC];
-    load_row<FC>(x, X + i * F, F, i < n);
-    const double ci_sq = i < n ? core_sq[i] : 0.0;
-    const int32_t ci = i < n ? comp[i] : -1;
-    double bk = i < n ? INFINITY : -1.0;  // an idle lane never updates
-    int64_t bj = -1;
-    for (int64_t r0 = j0; r0 < j1; r0 += TILE) {
-        const int cnt = (int)min((int64_t)TILE, j1 - r0);
-        __syncthreads();
-        stage_tile<FC>(tile, X, r0, cnt, F);
-        for (int e = threadIdx.x; e < cnt; e += TILE) {
-            tcore[e] = core_sq[r0 + e];
-            tcomp[e] = comp[r0 + e];
+__global__ void tile_bounds_kernel(const double *X, int64_t n, int F, const double *core_sq,
+                                   int64_t n_tiles, double *lo, double *hi, double *mincore) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n_tiles) return;
+    double l[FC], h[FC], x[FC];
+#pragma unroll
+    for (int f = 0; f < FC; f++) {
+        l[f] = INFINITY;
+        h[f] = -INFINITY;
+    }
+    double mc = INFINITY;
+    const int64_t r1 = min(n, (t + 1) * QT);
+    for (int64_t r = t * QT; r < r1; r++) {
+        load_row<FC>(x, X + r * F, F, true);
+#pragma unroll
+        for (int f = 0; f < FC; f++) {
+            l[f] = fmin(l[f], x[f]);
+            h[f] = fmax(h[f], x[f]);
         }
-        __syncthreads();
-        for (int j = 0; j < cnt; j++) {
-            if (tcomp[j] == ci) continue;
-            const double lo = fmax(ci_sq, tcore[j]);
-            if (lo > bk) continue;  // cannot beat the current best
-            const double key = fmax(lo, sqdist<FC>(x, tile + j * FC));
-            if (key < bk) {
+        mc = fmin(mc, core_sq[r]);
+    }
+#pragma unroll
+    for (int f = 0; f < FC; f++) {
+        lo[t * FC + f] = l[f];
+        hi[t * FC + f] = h[f];
+    }
+    mincore[t] = mc;
+}
+
+// the component all rows of a tile share, or -1, once per round
+__global__ void tile_comp_kernel(const int32_t *comp, int64_t n, int64_t n_tiles, int32_t *tcomp) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n_tiles) return;
+    int32_t c = comp[t * QT];
+    const int64_t r1 = min(n, (t + 1) * QT);
+    for (int64_t r = t * QT + 1; r < r1 && c >= 0; r++)
+        if (comp[r] != c) c = -1;
+    tcomp[t] = c;
+}
+
+// One Boruvka round: for each row the cheapest mutual-reachability edge to
+// a row of another component, key max(core_a^2, core_b^2, d^2) (monotone in
+// the edge weight); among equal keys the partner nearest in row order, then
+// the lower index (dense regions have many equal keys, and this order lets
+// a tie found nearby prune every farther tile).  One wave per query tile;
+// reference tiles are visited outward from the query's own, so good edges
+// are found early.  The box bound is a true lower bound of the computed key:
+// each difference to the box is at most the difference to any row in it,
+// and rounding to nearest is monotone.
+template <int FC>
+__global__ void __launch_bounds__(QT) boruvka_kernel(const double *X, int64_t n, int F,
+                                                     const double *core_sq, const int32_t *comp,
+                                                     const double *lo, const double *hi,
+                                                     const double *mincore, const int32_t *tcomp,
+                                                     int64_t n_tiles, double *best_key,
+                                                     int64_t *best_j) {
+    const int64_t qt = blockIdx.x;
+    const int64_t i = qt * QT + threadIdx.x;
+    const bool valid = i < n;
+    double x[FC];
+    load_row<FC>(x, X + (valid ? i : 0) * F, F, valid);
+    const double ci_sq = valid ? core_sq[i] : 0.0;
+    const int32_t ci = valid ? comp[i] : -2;  // -2 matches no tile component
+    double bk = valid ? INFINITY : -1.0;        // an idle lane never needs a tile
+    int64_t bj = -1, bd = INT64_MAX;           // partner, its distance in row order
+    for (int64_t s = 0; s < 2 * n_tiles; s++) {
+        // tiles qt, qt + 1, qt - 1, qt + 2, qt - 2, ...
+        const int64_t d = (s + 1) / 2;
+        const int64_t t = (s & 1) ? qt + d : qt - d;
+        if (t < 0 || t >= n_tiles) continue;
+        double g = 0.0;
+#pragma unroll
+        for (int f = 0; f < FC; f++) {
+            const double v = fmax(fmax(__dsub_rn(lo[t * FC + f], x[f]), __dsub_rn(x[f], hi[t * FC + f])), 0.0);
+            g = __dadd_rn(g, __dmul_rn(v, v));
+        }
+        const double lb = fmax(fmax(ci_sq, mincore[t]), g);
+        const int64_t t0 = t * QT, t1 = min(n, t0 + QT) - 1;
+        const int64_t near = i < t0 ? t0 - i : (i > t1 ? i - t1 : 0);  // nearest row index distance
+        const bool need = tcomp[t] != ci && !(lb > bk) && !(lb == bk && near > bd);
+        if (!__any(need)) continue;
+        const int64_t r1 = min(n, (t + 1) * QT);
+        for (int64_t j = t * QT; j < r1; j++) {
+            if (comp[j] == ci) continue;
+            const double lo2 = fmax(ci_sq, core_sq[j]);
+            if (lo2 > bk) continue;
+            double y[FC];
+            load_row<FC>(y, X + j * F, F, true);
+            const double key = fmax(lo2, sqdist<FC>(x, y));
+            const int64_t dj = j > i ? j - i : i - j;
+            if (key < bk || (key == bk && (dj < bd || (dj == bd && j < bj)))) {
                 bk = key;
-                bj = r0 + j;
+                bj = j;
+                bd = dj;
             }
         }
     }
-    if (i < n) {
-        best_key[blockIdx.y * n + i] = bk;
-        best_j[blockIdx.y * n + i] = bj;
+    if (valid) {
+        best_key[i] = bk;
+        best_j[i] = bj;
     }
 }
 
