@@ -218,8 +218,20 @@ int32_t bh_window_order_stats(int32_t dtype, const void *traj, int64_t n_save, i
 int32_t bh_knn(const double *Q, int64_t nq, const double *R, int64_t nr, int32_t F, int32_t k,
                double *out_d, int64_t *out_i, void *stream);
 
-/* device workspace bh_hdbscan_mst needs for n rows */
+/* device workspace bh_hdbscan_mst and bh_core_distances need for n rows */
 int64_t bh_hdbscan_workspace_bytes(int64_t n);
+
+/* HDBSCAN core distances: out_core[i] = distance from row i of X (n, F) to
+   its k-th nearest row, the row itself included (the value equals column
+   k - 1 of bh_knn(X, X)).  Rows should be in spatial order (e.g. along a
+   Z-order curve): tiles of 64 consecutive rows are skipped by their bounding
+   boxes, so the cost falls far below n^2 when consecutive rows are close.
+   1 <= k <= min(32, n).  Replaces the core-distance query of sklearn
+   HDBSCAN.fit (sklearn/cluster/_hdbscan/hdbscan.py, _hdbscan_brute /
+   _hdbscan_prims: NearestNeighbors(min_samples).kneighbors), reached from
+   HDBSCANClusterer.fit_predict (src/pybasin/predictors/hdbscan_clusterer.py:150-166). */
+int32_t bh_core_distances(const double *X, int64_t n, int32_t F, int32_t k, double *out_core,
+                          void *workspace, int64_t workspace_bytes, void *stream);
 
 /* Minimum spanning tree of the HDBSCAN mutual-reachability graph
    max(core_a, core_b, d(a, b)) of X (n, F) given the core distances (the

@@ -27,7 +27,8 @@ EXPORTS = ("bh_version", "bh_last_error", "bh_solve_batch", "bh_solve_batch_peak
            "bh_solve_rossler", "bh_gemm_bf16",
            "bh_mlp_bias_act", "bh_mlp_silu_bwd", "bh_mlp_softmax_ce", "bh_mlp_colsum",
            "bh_mlp_adam", "bh_mlp_prep_input", "bh_mlp_classify", "bh_mlp_infer", "bh_knn",
-           "bh_hdbscan_workspace_bytes", "bh_hdbscan_mst", "bh_hdbscan_labels", "bh_silhouette",
+           "bh_hdbscan_workspace_bytes", "bh_core_distances", "bh_hdbscan_mst", "bh_hdbscan_labels",
+           "bh_silhouette",
            "bh_window_order_stats")
 
 _lock = threading.Lock()
@@ -68,6 +69,7 @@ def _declare(L: ctypes.CDLL) -> None:
         "bh_mlp_classify": [vp, vp, i64, vp, vp, vp],
         "bh_mlp_infer": [vp, i64, vp, vp, f32, f32, f32, f32, f32, f32, vp, vp, vp],
         "bh_knn": [vp, i64, vp, i64, i32, i32, vp, vp, vp],
+        "bh_core_distances": [vp, i64, i32, i32, vp, vp, i64, vp],
         "bh_hdbscan_mst": [vp, i64, i32, vp, vp, vp, vp, vp, i64, vp],
         "bh_hdbscan_labels": [vp, vp, vp, i64, vp, i32, vp, vp],
         "bh_silhouette": [vp, i64, i32, vp, vp, i32, vp, vp, vp],

@@ -51,11 +51,31 @@ def knn(Q: torch.Tensor, R: torch.Tensor, k: int, stream=None):
     return d, i
 
 
-def core_distances(X: torch.Tensor, min_samples: int) -> torch.Tensor:
+def _workspace(n: int, device) -> torch.Tensor:
+    return torch.empty((int(_lib.lib().bh_hdbscan_workspace_bytes(n)),), dtype=torch.uint8,
+                       device=device)
+
+
+def core_distances(X: torch.Tensor, min_samples: int, ordered: bool = False) -> torch.Tensor:
     """Distance to the min_samples-th nearest row, the row itself included
-    (sklearn HDBSCAN's core distance)."""
-    d, _ = knn(X, X, min_samples)
-    return d[:, -1].contiguous()
+    (sklearn HDBSCAN's core distance).  The kernel prunes by tiles of
+    consecutive rows; unless ordered is set, the rows are put in spatial
+    order first (the result is in the caller's row order either way)."""
+    n, F = X.shape
+    perm = None if ordered else spatial_order(X)
+    Xs = X if perm is None else X[perm].contiguous()
+    core = torch.empty((n,), dtype=torch.float64, device=X.device)
+    ws = _workspace(n, X.device)
+    s = torch.cuda.current_stream(X.device)
+    with torch.cuda.device(X.device):
+        _lib.check(_lib.lib().bh_core_distances(Xs.data_ptr(), n, F, int(min_samples), core.data_ptr(),
+                                                ws.data_ptr(), ws.numel(), s.cuda_stream),
+                   "bh_core_distances")
+    if perm is None:
+        return core
+    out = torch.empty_like(core)
+    out[perm] = core
+    return out
 
 
 def spatial_order(X: torch.Tensor) -> torch.Tensor:
@@ -74,20 +94,22 @@ def spatial_order(X: torch.Tensor) -> torch.Tensor:
     return torch.argsort(code, stable=True)
 
 
-def mutual_reachability_mst(X: torch.Tensor, core: torch.Tensor):
+def mutual_reachability_mst(X: torch.Tensor, core: torch.Tensor, ordered: bool = False):
     """Spanning-tree edges (a, b, w) of the mutual-reachability graph (device),
     in increasing weight (equal weights in the order the rounds found them).
-    The rounds run on the rows in spatial order; edges come back in the
-    caller's row indices."""
-    perm = spatial_order(X)
-    X = X[perm].contiguous()
-    core = core[perm].contiguous()
+    The rounds run on the rows in spatial order (put so here unless ordered
+    is set); edges come back in the caller's row indices."""
     n, F = X.shape
+    if ordered:
+        perm = torch.arange(n, device=X.device)
+    else:
+        perm = spatial_order(X)
+        X = X[perm].contiguous()
+        core = core[perm].contiguous()
     a = torch.empty((n - 1,), dtype=torch.int64, device=X.device)
     b = torch.empty_like(a)
     w = torch.empty((n - 1,), dtype=torch.float64, device=X.device)
-    ws = torch.empty((int(_lib.lib().bh_hdbscan_workspace_bytes(n)),), dtype=torch.uint8,
-                     device=X.device)
+    ws = _workspace(n, X.device)
     s = torch.cuda.current_stream(X.device)
     with torch.cuda.device(X.device):
         _lib.check(_lib.lib().bh_hdbscan_mst(X.data_ptr(), n, F, core.data_ptr(), a.data_ptr(),
@@ -158,8 +180,12 @@ class HipHDBSCANClusterer:
             if min_samples > X.shape[0]:
                 raise ValueError(f"min_samples ({min_samples}) must be at most the number of "
                                  f"samples ({X.shape[0]})")
-            a, b, w = mutual_reachability_mst(X, core_distances(X, min_samples))
-            self._trees[min_samples] = (a.cpu().numpy(), b.cpu().numpy(), w.cpu().numpy())
+            # rows in spatial order once for both the core distances and the tree
+            perm = spatial_order(X)
+            Xs = X[perm].contiguous()
+            a, b, w = mutual_reachability_mst(Xs, core_distances(Xs, min_samples, ordered=True),
+                                              ordered=True)
+            self._trees[min_samples] = (perm[a].cpu().numpy(), perm[b].cpu().numpy(), w.cpu().numpy())
         return self._trees[min_samples]
 
     def _hdbscan(self, X: torch.Tensor, min_cluster_size: int, min_samples: int | None) -> np.ndarray:

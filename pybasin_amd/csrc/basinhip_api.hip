@@ -49,6 +49,21 @@ int32_t tile_bounds(const double *X, int64_t n, int32_t F, const double *core_sq
 }
 
 template <int FC>
+int32_t core_dist_launch(const double *X, int64_t n, int32_t F, int32_t k, const double *lo,
+                         const double *hi, int64_t n_tiles, double *core, hipStream_t s) {
+    namespace C = bh::clu;
+    const dim3 grid((unsigned)n_tiles), block(C::QT);
+    if (k <= 8)
+        hipLaunchKernelGGL((C::core_distance_kernel<8, FC>), grid, block, 0, s, X, n, F, k, lo, hi, n_tiles, core);
+    else if (k <= 16)
+        hipLaunchKernelGGL((C::core_distance_kernel<16, FC>), grid, block, 0, s, X, n, F, k, lo, hi, n_tiles, core);
+    else
+        hipLaunchKernelGGL((C::core_distance_kernel<32, FC>), grid, block, 0, s, X, n, F, k, lo, hi, n_tiles, core);
+    BH_HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
+template <int FC>
 int32_t boruvka_round(const double *X, int64_t n, int32_t F, const double *core_sq,
                       const int32_t *comp, const double *lo, const double *hi, const double *mincore,
                       int32_t *tcomp, int64_t n_tiles, double *best_key, int64_t *best_j,
@@ -310,6 +325,29 @@ int32_t bh_knn(const double *Q, int64_t nq, const double *R, int64_t nr, int32_t
     if (k <= 8) return launch_knn<8>(Q, nq, R, nr, F, k, out_d, out_i, s);
     if (k <= 16) return launch_knn<16>(Q, nq, R, nr, F, k, out_d, out_i, s);
     return launch_knn<32>(Q, nq, R, nr, F, k, out_d, out_i, s);
+}
+
+int32_t bh_core_distances(const double *X, int64_t n, int32_t F, int32_t k, double *out_core,
+                          void *workspace, int64_t workspace_bytes, void *stream) {
+    g_err[0] = 0;
+    namespace C = bh::clu;
+    if (n < 1) return fail(-5, "bh_core_distances: n >= 1 required");
+    if (F < 1 || F > C::MAXF) return fail(-5, "bh_core_distances: 1 <= F <= %d, got %d", C::MAXF, F);
+    if (k < 1 || k > C::MAXK || k > n) return fail(-5, "bh_core_distances: 1 <= k <= min(%d, n), got %d", C::MAXK, k);
+    if (!X || !out_core || !workspace) return fail(-5, "bh_core_distances: null argument");
+    if (workspace_bytes < bh_hdbscan_workspace_bytes(n))
+        return fail(-5, "bh_core_distances: workspace of %lld bytes required",
+                    (long long)bh_hdbscan_workspace_bytes(n));
+    if (n > 2147483647LL) return fail(-5, "bh_core_distances: n too large");
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    const int64_t n_tiles = (n + C::QT - 1) / C::QT;
+    double *tlo = reinterpret_cast<double *>(workspace);
+    double *thi = tlo + n_tiles * C::MAXF;
+    int32_t rc = 0;
+    BH_BY_SLOTS(F, (tile_bounds<FCV>(X, n, F, nullptr, n_tiles, tlo, thi, nullptr, s)));
+    if (rc) return rc;
+    BH_BY_SLOTS(F, (core_dist_launch<FCV>(X, n, F, k, tlo, thi, n_tiles, out_core, s)));
+    return rc;
 }
 
 int64_t bh_hdbscan_workspace_bytes(int64_t n) {

@@ -128,8 +128,8 @@ __global__ void __launch_bounds__(TILE) knn_kernel(const double *Q, int64_t nq, 
 // lie in the query's own component.
 constexpr int QT = 64;
 
-// per-tile bounding box (FC slots; padded slots are 0) and the smallest
-// squared core distance, once per spanning tree
+// per-tile bounding box (FC slots; padded slots are 0) and, when core_sq is
+// given, the smallest squared core distance, once per spanning tree
 template <int FC>
 __global__ void tile_bounds_kernel(const double *X, int64_t n, int F, const double *core_sq,
                                    int64_t n_tiles, double *lo, double *hi, double *mincore) {
@@ -150,14 +150,14 @@ __global__ void tile_bounds_kernel(const double *X, int64_t n, int F, const doub
             l[f] = fmin(l[f], x[f]);
             h[f] = fmax(h[f], x[f]);
         }
-        mc = fmin(mc, core_sq[r]);
+        if (core_sq) mc = fmin(mc, core_sq[r]);
     }
 #pragma unroll
     for (int f = 0; f < FC; f++) {
         lo[t * FC + f] = l[f];
         hi[t * FC + f] = h[f];
     }
-    mincore[t] = mc;
+    if (mincore) mincore[t] = mc;
 }
 
 // the component all rows of a tile share, or -1, once per round
@@ -232,6 +232,59 @@ __global__ void __launch_bounds__(QT) boruvka_kernel(const double *X, int64_t n,
         best_key[i] = bk;
         best_j[i] = bj;
     }
+}
+
+// Core distances: the k-th smallest distance from each row to the rows of X
+// (itself included), for rows in spatial order.  One wave per query tile,
+// reference tiles outward from the query's own; a tile is skipped when its
+// box is no closer than every lane's current k-th distance.  The k smallest
+// squared distances are kept ascending in K registers (slots past k hold
+// the next ones and are not read); only the k-th value leaves the kernel,
+// and it does not depend on the visiting order.
+template <int K, int FC>
+__global__ void __launch_bounds__(QT) core_distance_kernel(const double *X, int64_t n, int F, int k,
+                                                           const double *lo, const double *hi,
+                                                           int64_t n_tiles, double *core) {
+    const int64_t qt = blockIdx.x;
+    const int64_t i = qt * QT + threadIdx.x;
+    const bool valid = i < n;
+    double x[FC];
+    load_row<FC>(x, X + (valid ? i : 0) * F, F, valid);
+    double bd[K];
+#pragma unroll
+    for (int j = 0; j < K; j++) bd[j] = INFINITY;
+    auto kth = [&]() {
+        double v = bd[0];
+#pragma unroll
+        for (int j = 1; j < K; j++) v = (j == k - 1) ? bd[j] : v;
+        return v;
+    };
+    for (int64_t s = 0; s < 2 * n_tiles; s++) {
+        const int64_t d = (s + 1) / 2;
+        const int64_t t = (s & 1) ? qt + d : qt - d;
+        if (t < 0 || t >= n_tiles) continue;
+        double g = 0.0;
+#pragma unroll
+        for (int f = 0; f < FC; f++) {
+            const double v = fmax(fmax(__dsub_rn(lo[t * FC + f], x[f]), __dsub_rn(x[f], hi[t * FC + f])), 0.0);
+            g = __dadd_rn(g, __dmul_rn(v, v));
+        }
+        const bool need = valid && g < kth();
+        if (!__any(need)) continue;
+        const int64_t r1 = min(n, (t + 1) * QT);
+        for (int64_t j = t * QT; j < r1; j++) {
+            double y[FC];
+            load_row<FC>(y, X + j * F, F, true);
+            const double v = sqdist<FC>(x, y);
+            if (v < bd[K - 1]) {
+                // insert into the ascending list
+#pragma unroll
+                for (int m = K - 1; m > 0; m--) bd[m] = (v < bd[m - 1]) ? bd[m - 1] : fmin(bd[m], v);
+                bd[0] = fmin(bd[0], v);
+            }
+        }
+    }
+    if (valid) core[i] = sqrt(kth());
 }
 
 // exact weights max(core_a, core_b, d(a, b)) of the chosen edges

@@ -64,6 +64,28 @@ def test_knn_matches_sklearn(cuda_device, F, k):
     assert np.array_equal(i.cpu().numpy(), ir)
 
 
+@pytest.mark.parametrize("F,k,clustered", [(2, 10, True), (3, 5, False), (8, 16, True), (2, 32, False)])
+def test_pruned_core_distances_equal_knn(cuda_device, F, k, clustered):
+    """The tile-pruned core distances equal column k - 1 of the brute-force
+    k-NN bit for bit, in spatial and in arbitrary row order (duplicates
+    included, as dense attractor features have them)."""
+    from pybasin_amd.cluster import core_distances, knn, spatial_order
+
+    rng = np.random.default_rng(F + k)
+    if clustered:
+        centers = rng.normal(size=(5, F)) * 3
+        X = centers[rng.integers(0, 5, 5000)] + rng.normal(size=(5000, F)) * 0.05
+        X[:300] = X[300:600]  # exact duplicates
+    else:
+        X = rng.normal(size=(4321, F))
+    Xd = torch.tensor(X, device=cuda_device)
+    want = knn(Xd, Xd, k)[0][:, -1]
+    assert torch.equal(core_distances(Xd, k), want)
+    perm = spatial_order(Xd)
+    got = core_distances(Xd[perm].contiguous(), k, ordered=True)
+    assert torch.equal(got, want[perm])
+
+
 def test_mst_weights_equal_dense_reference(cuda_device):
     """The Boruvka tree has the weight multiset of a minimum spanning tree of
     the dense mutual-reachability matrix (scipy)."""
