@@ -60,7 +60,7 @@ def parse():
     ap.add_argument("--ref-sample", type=int, default=64,
                     help="ICs for the scipy path (64x for the torch shared-step path); 0 skips them")
     ap.add_argument("--workload", choices=["duffing", "lorenz", "surrogate", "rossler", "mlp_infer",
-                                           "hdbscan"],
+                                           "hdbscan", "estimator"],
                     default="duffing",
                     help="duffing: plain HIP Dopri5 on 100k Duffing ICs (default); surrogate: "
                          "pendulum torque sweep through the MLP gate with Dopri5 fallback")
@@ -430,6 +430,61 @@ def hdbscan_workload(args):
     print(json.dumps(out), flush=True)
 
 
+def estimator_workload(args):
+    """The reference's end-to-end benchmark
+    (benchmarks/end_to_end/benchmark_basin_stability_estimator.py): the
+    pendulum (alpha 0.1, T 0.5, K 1) through BasinStabilityEstimator with
+    n = 100,000 sampled ICs, t in [0, 1000] with 1,000 saves over the whole
+    span, rtol 1e-8, atol 1e-6, PendulumFeatureExtractor(time_steady=950),
+    1-NN on the templates (0.5, 0) FP and (2.7, 0) LC, no unbounded
+    detection.  A step = one estimator run (sampling, integration, features,
+    classification, basin stability) on the GPU."""
+    import math
+
+    from pybasin_amd.case_features import PendulumFeatureExtractor
+    from pybasin_amd.estimator import BasinStabilityEstimator, UniformRandomSampler
+    from pybasin_amd.solvers import HipDopri5Solver
+    from pybasin_amd.systems import PendulumODE
+
+    n = args.n
+    T = 0.5
+    sampler = UniformRandomSampler([-math.pi + math.asin(T), -10.0], [math.pi + math.asin(T), 10.0],
+                                   device="cuda")
+    solver = HipDopri5Solver(t_span=(0, 1000), t_steps=1000, device="cuda", rtol=1e-8, atol=1e-6,
+                             cache_dir=None)
+    bse = BasinStabilityEstimator(PendulumODE({"alpha": 0.1, "T": T, "K": 1.0}), sampler, n=n,
+                                  solver=solver,
+                                  feature_extractor=PendulumFeatureExtractor(time_steady=950.0),
+                                  template_y0=[[0.5, 0.0], [2.7, 0.0]], template_labels=["FP", "LC"],
+                                  detect_unbounded=False)
+    torch.manual_seed(args.seed)
+    for _ in range(args.warmup):
+        bse.run()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        res = bse.run()
+    torch.cuda.synchronize()
+    ms = 1e3 * (time.perf_counter() - t0) / args.steps
+    published = 100_000 / 12.5678  # ICs/s, RTX 5070 Ti Laptop, JAX solver
+    value = n / (ms * 1e-3)
+    out = {
+        "metric": "classified initial conditions/sec (end-to-end BasinStabilityEstimator)",
+        "value": round(value, 1), "unit": "ICs/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms, 3), "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": round(value / published, 2) if n == 100_000 else None,
+        "dtype": "f32", "data": "synthetic: seeded-uniform ICs (the estimator's sampler on the GPU)",
+        "config": {"workload": "pendulum_end_to_end_estimator", "n_ics": n,
+                   "basin_stability": {k: round(v, 5) for k, v in res.basin_stability.items()},
+                   "reference_published": {"value": round(published, 1), "unit": "ICs/s",
+                                           "hardware": "RTX 5070 Ti Laptop, JAX/Diffrax solver",
+                                           "source": "benchmarks/end_to_end/results/"
+                                                     "end_to_end_comparison.csv:31"}},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def lorenz_workload(args):
     """Lorenz with escape: 200,000 seeded ICs (x in [-10, 10], y in [-20, 20],
     z = 0), sigma 0.12, r 0, b -0.6, t in [0, 1000], float32, rtol 1e-8,
@@ -523,6 +578,8 @@ def main():
         return surrogate_workload(args)
     if args.workload == "lorenz":
         return lorenz_workload(args)
+    if args.workload == "estimator":
+        return estimator_workload(args)
     if args.workload == "hdbscan":
         return hdbscan_workload(args)
     if args.workload == "mlp_infer":

@@ -212,7 +212,7 @@ int32_t bh_window_order_stats(int32_t dtype, const void *traj, int64_t n_save, i
    the 1-NN template classifiers of the case studies) --------------------- */
 
 /* k nearest reference rows of every query row (Euclidean, double; rows of F
-   <= 16 features, row-major): out_d (nq, k) distances ascending, out_i (nq, k)
+   <= 32 features, row-major): out_d (nq, k) distances ascending, out_i (nq, k)
    reference indices; equal distances keep the lower index.  1 <= k <= 32.
    sklearn NearestNeighbors.kneighbors / KNeighborsClassifier (brute force). */
 int32_t bh_knn(const double *Q, int64_t nq, const double *R, int64_t nr, int32_t F, int32_t k,

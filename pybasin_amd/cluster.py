@@ -23,7 +23,7 @@ import torch
 
 from . import _lib
 
-MAX_FEATURES = 16
+MAX_FEATURES = 32
 
 
 def _as_rows(X, device) -> torch.Tensor:

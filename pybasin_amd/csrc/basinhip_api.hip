@@ -15,7 +15,7 @@ using bhapi::fail;
 using bhapi::g_err;
 
 // feature slots for F features: 2, 4, 8 or 16
-inline int feature_slots(int F) { return F <= 2 ? 2 : F <= 4 ? 4 : F <= 8 ? 8 : 16; }
+inline int feature_slots(int F) { return F <= 2 ? 2 : F <= 4 ? 4 : F <= 8 ? 8 : F <= 16 ? 16 : 32; }
 
 template <int K, int FC>
 int32_t launch_knn_fc(const double *Q, int64_t nq, const double *R, int64_t nr, int32_t F,
@@ -35,7 +35,8 @@ int32_t launch_knn(const double *Q, int64_t nq, const double *R, int64_t nr, int
     case 2: return launch_knn_fc<K, 2>(Q, nq, R, nr, F, k, out_d, out_i, s);
     case 4: return launch_knn_fc<K, 4>(Q, nq, R, nr, F, k, out_d, out_i, s);
     case 8: return launch_knn_fc<K, 8>(Q, nq, R, nr, F, k, out_d, out_i, s);
-    default: return launch_knn_fc<K, 16>(Q, nq, R, nr, F, k, out_d, out_i, s);
+    case 16: return launch_knn_fc<K, 16>(Q, nq, R, nr, F, k, out_d, out_i, s);
+    default: return launch_knn_fc<K, 32>(Q, nq, R, nr, F, k, out_d, out_i, s);
     }
 }
 
@@ -107,7 +108,8 @@ int32_t silhouette_launch(const double *X, int64_t n, int32_t F, const int32_t *
     case 2: { constexpr int FCV = 2; rc = call; } break; \
     case 4: { constexpr int FCV = 4; rc = call; } break; \
     case 8: { constexpr int FCV = 8; rc = call; } break; \
-    default: { constexpr int FCV = 16; rc = call; } break; \
+    case 16: { constexpr int FCV = 16; rc = call; } break; \
+    default: { constexpr int FCV = 32; rc = call; } break; \
     }
 
 }  // namespace

@@ -31,10 +31,10 @@ namespace bh {
 namespace clu {
 
 constexpr int TILE = 256;   // reference points staged per LDS tile
-constexpr int MAXF = 16;    // features (rows are padded to 2, 4, 8 or 16 slots)
+constexpr int MAXF = 32;    // features (rows are padded to 2, 4, 8, 16 or 32 slots)
 constexpr int MAXK = 32;    // neighbours
 
-// Kernels are instantiated for FC = 2, 4, 8, 16 feature slots; rows hold F
+// Kernels are instantiated for FC = 2, 4, 8, 16, 32 feature slots; rows hold F
 // <= FC features and the slots beyond F are zero (adding (0 - 0)^2 = +0 to a
 // sum leaves it unchanged, so padded distances equal unpadded ones).
 template <int FC>
@@ -253,12 +253,7 @@ __global__ void __launch_bounds__(QT) core_distance_kernel(const double *X, int6
     double bd[K];
 #pragma unroll
     for (int j = 0; j < K; j++) bd[j] = INFINITY;
-    auto kth = [&]() {
-        double v = bd[0];
-#pragma unroll
-        for (int j = 1; j < K; j++) v = (j == k - 1) ? bd[j] : v;
-        return v;
-    };
+    double kd = INFINITY;  // bd[k - 1], kept beside the list (no runtime index into it)
     for (int64_t s = 0; s < 2 * n_tiles; s++) {
         const int64_t d = (s + 1) / 2;
         const int64_t t = (s & 1) ? qt + d : qt - d;
@@ -269,7 +264,7 @@ __global__ void __launch_bounds__(QT) core_distance_kernel(const double *X, int6
             const double v = fmax(fmax(__dsub_rn(lo[t * FC + f], x[f]), __dsub_rn(x[f], hi[t * FC + f])), 0.0);
             g = __dadd_rn(g, __dmul_rn(v, v));
         }
-        const bool need = valid && g < kth();
+        const bool need = valid && g < kd;
         if (!__any(need)) continue;
         const int64_t r1 = min(n, (t + 1) * QT);
         for (int64_t j = t * QT; j < r1; j++) {
@@ -281,10 +276,14 @@ __global__ void __launch_bounds__(QT) core_distance_kernel(const double *X, int6
 #pragma unroll
                 for (int m = K - 1; m > 0; m--) bd[m] = (v < bd[m - 1]) ? bd[m - 1] : fmin(bd[m], v);
                 bd[0] = fmin(bd[0], v);
+                double nk = bd[0];
+#pragma unroll
+                for (int m = 1; m < K; m++) nk = (m == k - 1) ? bd[m] : nk;
+                kd = nk;
             }
         }
     }
-    if (valid) core[i] = sqrt(kth());
+    if (valid) core[i] = sqrt(kd);
 }
 
 // exact weights max(core_a, core_b, d(a, b)) of the chosen edges

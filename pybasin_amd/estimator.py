@@ -22,9 +22,9 @@ from dataclasses import dataclass, field
 import numpy as np
 import torch
 
-from .features import HipFusedFeatureExtractor, Solution, StatFeatureExtractor
+from .features import MINIMAL_FEATURES, HipFusedFeatureExtractor, Solution, StatFeatureExtractor
 from . import _lib
-from .solvers.hip_dopri5 import FUSED_RESULTS, functor_id
+from .solvers.hip_dopri5 import FUSED_RESULTS, HipDopri5Solver, functor_id
 
 
 def set_seed(seed: int) -> None:
@@ -132,10 +132,26 @@ class BasinStabilityEstimator:
     (_process_study_label, 365-461).
     """
 
-    def __init__(self, ode_system, sampler: UniformRandomSampler, n: int, solver,
-                 feature_extractor, template_y0=None, template_labels=None,
+    def __init__(self, ode_system, sampler: UniformRandomSampler, n: int = 10_000, solver=None,
+                 feature_extractor=None, template_y0=None, template_labels=None,
                  keep_trajectories: bool = False, detect_unbounded: bool = True,
                  compute_orbit_data: list[int] | bool = False, predictor=None):
+        # defaults as the reference's (basin_stability_estimator.py:73-181):
+        # Dopri5 over t in [0, 1000] saving 1,000 points of [850, 1000] on the
+        # sampler's device, the minimal feature set z-scored, and without
+        # templates HDBSCAN with auto-tuned min_cluster_size and noise assignment
+        if solver is None:
+            solver = HipDopri5Solver(t_span=(0, 1000), t_steps=1000, t_eval=(850.0, 1000.0),
+                                     device=str(sampler.device))
+        if feature_extractor is None:
+            feature_extractor = HipFusedFeatureExtractor(features=MINIMAL_FEATURES, normalize=True)
+        if getattr(feature_extractor, "needs_window", False):
+            keep_trajectories = True  # order statistics read the saved window
+        if template_y0 is None and predictor is None:
+            from .cluster import HipHDBSCANClusterer
+
+            predictor = HipHDBSCANClusterer(auto_tune=True, assign_noise=True,
+                                            device=str(solver.device))
         self.ode_system = ode_system
         self.sampler = sampler
         self.n = int(n)

@@ -228,18 +228,28 @@ class StatFeatureExtractor(FeatureExtractor):
     def _stats(self, solution: Solution) -> torch.Tensor:
         return stats_from_window(self.filter_time(solution))
 
+    @property
+    def needs_window(self) -> bool:
+        """Whether some requested feature is a window order statistic (the
+        saved window must be kept)."""
+        dicts = [self.features] + list(self.features_per_state.values())
+        return any(name in WINDOW_FEATURES for d in dicts if d for name in d)
+
+    def _normalized(self, feats: torch.Tensor) -> torch.Tensor:
+        # z-score with the mean / std of the first batch seen (then fixed)
+        if not self.normalize:
+            return feats
+        with self._lock:
+            if self._mean is None:
+                self._mean = feats.mean(0, keepdim=True)
+                self._std = feats.std(0, keepdim=True).clamp_min(1e-12)
+        return (feats - self._mean) / self._std
+
     def extract_features(self, solution: Solution) -> torch.Tensor:
         stats = self._stats(solution)
         plan = self._plan(stats.shape[1])
         start = steady_index(solution.time, self._resolve_time_steady(solution))
-        feats = self._columns(plan, stats, solution.y, start)
-        if self.normalize:
-            with self._lock:
-                if self._mean is None:
-                    self._mean = feats.mean(0, keepdim=True)
-                    self._std = feats.std(0, keepdim=True).clamp_min(1e-12)
-            feats = (feats - self._mean) / self._std
-        return feats
+        return self._normalized(self._columns(plan, stats, solution.y, start))
 
 
 class HipFusedFeatureExtractor(StatFeatureExtractor):
@@ -260,4 +270,4 @@ class HipFusedFeatureExtractor(StatFeatureExtractor):
         if res.stats is None:
             raise ValueError("the fused result carries no window statistics")
         plan = self._plan(res.stats.shape[1])
-        return self._columns(plan, res.stats, res.y, res.steady_idx)
+        return self._normalized(self._columns(plan, res.stats, res.y, res.steady_idx))

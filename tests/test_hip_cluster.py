@@ -49,7 +49,7 @@ def _mismatch(a, b):
     return len(a) - int(sum(max(C[i, j], 0) for i, j in zip(r, c)))
 
 
-@pytest.mark.parametrize("F,k", [(2, 1), (3, 5), (8, 10), (16, 32)])
+@pytest.mark.parametrize("F,k", [(2, 1), (3, 5), (8, 10), (16, 32), (20, 10), (32, 4)])
 def test_knn_matches_sklearn(cuda_device, F, k):
     from sklearn.neighbors import NearestNeighbors
 
@@ -64,7 +64,8 @@ def test_knn_matches_sklearn(cuda_device, F, k):
     assert np.array_equal(i.cpu().numpy(), ir)
 
 
-@pytest.mark.parametrize("F,k,clustered", [(2, 10, True), (3, 5, False), (8, 16, True), (2, 32, False)])
+@pytest.mark.parametrize("F,k,clustered", [(2, 10, True), (3, 5, False), (8, 16, True), (2, 32, False),
+                                            (20, 10, True)])
 def test_pruned_core_distances_equal_knn(cuda_device, F, k, clustered):
     """The tile-pruned core distances equal column k - 1 of the brute-force
     k-NN bit for bit, in spatial and in arbitrary row order (duplicates
