@@ -60,7 +60,7 @@ def parse():
     ap.add_argument("--ref-sample", type=int, default=64,
                     help="ICs for the scipy path (64x for the torch shared-step path); 0 skips them")
     ap.add_argument("--workload", choices=["duffing", "lorenz", "surrogate", "rossler", "mlp_infer",
-                                           "hdbscan", "estimator"],
+                                           "hdbscan", "estimator", "estimator_defaults"],
                     default="duffing",
                     help="duffing: plain HIP Dopri5 on 100k Duffing ICs (default); surrogate: "
                          "pendulum torque sweep through the MLP gate with Dopri5 fallback")
@@ -485,6 +485,49 @@ def estimator_workload(args):
     print(json.dumps(out), flush=True)
 
 
+def estimator_defaults_workload(args):
+    """The reference's example run with pybasin's defaults
+    (docs/benchmarks/basin-stability-estimator.md:9-24): the pendulum case
+    study with n = 10,000 ICs where only the ODE system and the sampler
+    region are given; solver, feature extractor (minimal set, z-scored) and
+    predictor (auto-tuned HDBSCAN with noise assignment) take their
+    defaults.  A step = one estimator run."""
+    import math
+
+    from pybasin_amd.estimator import BasinStabilityEstimator, UniformRandomSampler
+    from pybasin_amd.systems import PendulumODE
+
+    n = 10_000 if args.n == 100_000 else args.n
+    T = 0.5
+    sampler = UniformRandomSampler([-math.pi + math.asin(T), -10.0], [math.pi + math.asin(T), 10.0],
+                                   device="cuda")
+    torch.manual_seed(args.seed)
+    bse = BasinStabilityEstimator(PendulumODE({"alpha": 0.1, "T": T, "K": 1.0}), sampler, n=n)
+    for _ in range(args.warmup):
+        bse.run()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        res = bse.run()
+    torch.cuda.synchronize()
+    ms = 1e3 * (time.perf_counter() - t0) / args.steps
+    value = n / (ms * 1e-3)
+    published = 10_000 / 17.0062
+    out = {
+        "metric": "classified initial conditions/sec (end-to-end BasinStabilityEstimator, defaults)",
+        "value": round(value, 1), "unit": "ICs/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": round(value / published, 1) if n == 10_000 else None,
+        "dtype": "f32", "data": "synthetic: seeded-uniform ICs (the estimator's sampler on the GPU)",
+        "config": {"workload": "pendulum_estimator_defaults", "n_ics": n,
+                   "basin_stability": {str(k): round(v, 5) for k, v in res.basin_stability.items()},
+                   "reference_published": {"value": round(published, 1), "unit": "ICs/s",
+                                           "total_s": 17.0062,
+                                           "source": "docs/benchmarks/basin-stability-estimator.md:13-24"}},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def lorenz_workload(args):
     """Lorenz with escape: 200,000 seeded ICs (x in [-10, 10], y in [-20, 20],
     z = 0), sigma 0.12, r 0, b -0.6, t in [0, 1000], float32, rtol 1e-8,
@@ -580,6 +623,8 @@ def main():
         return lorenz_workload(args)
     if args.workload == "estimator":
         return estimator_workload(args)
+    if args.workload == "estimator_defaults":
+        return estimator_defaults_workload(args)
     if args.workload == "hdbscan":
         return hdbscan_workload(args)
     if args.workload == "mlp_infer":
