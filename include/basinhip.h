@@ -256,11 +256,16 @@ int32_t bh_hdbscan_labels(const int64_t *edge_a, const int64_t *edge_b, const do
 
 /* Silhouette value of every row (rows sorted by label 0..n_labels-1, label
    c occupying [starts[c], starts[c+1])): s = (b - a) / max(a, b), 0 for a
-   label of one row.  sklearn.metrics.silhouette_score = mean of s_out.
+   label of one row.  sklearn.metrics.silhouette_score = mean of s_out
+   (replaces the silhouette_score call of _find_optimal_min_cluster_size,
+   hdbscan_clusterer.py:121-166).  Optional weights (n) with label_counts
+   (n_labels) = their sums per label: row i then stands for weights[i] equal
+   rows (collapse duplicates to cut the pair count); the score is the
+   weights-weighted mean of s_out.  Both NULL: every row counts once.
    sums: (n_labels, n) float64 device scratch for the per-label distance sums. */
 int32_t bh_silhouette(const double *X_sorted, int64_t n, int32_t F, const int32_t *labels_sorted,
-                      const int64_t *starts, int32_t n_labels, double *sums, double *s_out,
-                      void *stream);
+                      const int64_t *starts, int32_t n_labels, const double *weights,
+                      const double *label_counts, double *sums, double *s_out, void *stream);
 
 #ifdef __cplusplus
 }

@@ -72,7 +72,7 @@ def _declare(L: ctypes.CDLL) -> None:
         "bh_core_distances": [vp, i64, i32, i32, vp, vp, i64, vp],
         "bh_hdbscan_mst": [vp, i64, i32, vp, vp, vp, vp, vp, i64, vp],
         "bh_hdbscan_labels": [vp, vp, vp, i64, vp, i32, vp, vp],
-        "bh_silhouette": [vp, i64, i32, vp, vp, i32, vp, vp, vp],
+        "bh_silhouette": [vp, i64, i32, vp, vp, i32, vp, vp, vp, vp, vp],
         "bh_window_order_stats": [i32, vp, i64, i64, i32, i64, vp, i32, vp, vp],
     }
     for name, args in sigs.items():

@@ -139,25 +139,33 @@ def labels_from_mst(a: np.ndarray, b: np.ndarray, w: np.ndarray, n: int,
 
 def silhouette_score(X: torch.Tensor, labels) -> float:
     """Mean silhouette value over the rows (sklearn.metrics.silhouette_score,
-    Euclidean; labels any integers, at least two distinct)."""
+    Euclidean; labels any integers, at least two distinct).  Equal rows with
+    equal labels are collapsed into one weighted row first (dense attractor
+    features repeat exactly), which cuts the O(n^2) pair sums without
+    changing the score beyond rounding."""
     lab = torch.as_tensor(np.asarray(labels), device=X.device)
     uniq, inv = torch.unique(lab, return_inverse=True)
     if len(uniq) < 2:
         raise ValueError("silhouette needs at least two labels")
-    order = torch.argsort(inv, stable=True)
-    Xs = X[order].contiguous()
-    ls = inv[order].to(torch.int32).contiguous()
-    counts = torch.bincount(inv, minlength=len(uniq))
+    # (label, row) pairs, sorted by label first: contiguous label blocks
+    keyed = torch.cat([inv.to(torch.float64)[:, None], X.to(torch.float64)], 1)
+    rows, counts = torch.unique(keyed, dim=0, return_counts=True)
+    ls = rows[:, 0].to(torch.int32).contiguous()
+    Xs = rows[:, 1:].contiguous()
+    w = counts.to(torch.float64).contiguous()
+    m = Xs.shape[0]
+    lc = torch.zeros(len(uniq), dtype=torch.float64, device=X.device).index_add_(0, ls.long(), w)
     starts = torch.zeros(len(uniq) + 1, dtype=torch.int64, device=X.device)
-    starts[1:] = torch.cumsum(counts, 0)
-    s = torch.empty((X.shape[0],), dtype=torch.float64, device=X.device)
-    sums = torch.empty((len(uniq), X.shape[0]), dtype=torch.float64, device=X.device)
+    starts[1:] = torch.cumsum(torch.bincount(ls.long(), minlength=len(uniq)), 0)
+    s = torch.empty((m,), dtype=torch.float64, device=X.device)
+    sums = torch.empty((len(uniq), m), dtype=torch.float64, device=X.device)
     st = torch.cuda.current_stream(X.device)
     with torch.cuda.device(X.device):
-        _lib.check(_lib.lib().bh_silhouette(Xs.data_ptr(), X.shape[0], X.shape[1], ls.data_ptr(),
-                                            starts.data_ptr(), len(uniq), sums.data_ptr(),
-                                            s.data_ptr(), st.cuda_stream), "bh_silhouette")
-    return float(s.mean())
+        _lib.check(_lib.lib().bh_silhouette(Xs.data_ptr(), m, Xs.shape[1], ls.data_ptr(),
+                                            starts.data_ptr(), len(uniq), w.data_ptr(), lc.data_ptr(),
+                                            sums.data_ptr(), s.data_ptr(), st.cuda_stream),
+                   "bh_silhouette")
+    return float((s * w).sum() / w.sum())
 
 
 class HipHDBSCANClusterer:

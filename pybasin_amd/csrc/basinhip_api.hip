@@ -89,16 +89,16 @@ int32_t edge_weights(const double *X, int32_t F, const double *core, const int64
 }
 
 template <int FC>
-int32_t silhouette_launch(const double *X, int64_t n, int32_t F, const int32_t *lab,
-                          const int64_t *starts, int32_t C_, double *sums, double *s_out,
-                          hipStream_t s) {
+int32_t silhouette_launch(const double *X, const double *w, int64_t n, int32_t F, const int32_t *lab,
+                          const int64_t *starts, const double *counts, int32_t C_, double *sums,
+                          double *s_out, hipStream_t s) {
     namespace C = bh::clu;
     hipLaunchKernelGGL(C::silhouette_sums_kernel<FC>,
                        dim3((unsigned)((n + C::TILE - 1) / C::TILE), (unsigned)C_), dim3(C::TILE), 0,
-                       s, X, n, F, starts, sums);
+                       s, X, w, n, F, starts, sums);
     BH_HIP_CHECK(hipGetLastError());
     hipLaunchKernelGGL(C::silhouette_value_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
-                       sums, n, lab, starts, C_, s_out);
+                       sums, n, lab, starts, counts, C_, s_out);
     BH_HIP_CHECK(hipGetLastError());
     return 0;
 }
@@ -471,19 +471,22 @@ int32_t bh_hdbscan_labels(const int64_t *edge_a, const int64_t *edge_b, const do
 }
 
 int32_t bh_silhouette(const double *X_sorted, int64_t n, int32_t F, const int32_t *labels_sorted,
-                      const int64_t *starts, int32_t n_labels, double *sums, double *s_out,
-                      void *stream) {
+                      const int64_t *starts, int32_t n_labels, const double *weights,
+                      const double *label_counts, double *sums, double *s_out, void *stream) {
     g_err[0] = 0;
     namespace C = bh::clu;
     if (n < 1 || F < 1 || F > C::MAXF || n_labels < 1)
         return fail(-5, "bh_silhouette: n >= 1, 1 <= F <= %d, n_labels >= 1 required", C::MAXF);
     if (!X_sorted || !labels_sorted || !starts || !sums || !s_out)
         return fail(-5, "bh_silhouette: null argument");
+    if ((weights == nullptr) != (label_counts == nullptr))
+        return fail(-5, "bh_silhouette: weights and label_counts go together");
     if (n_labels > 65535) return fail(-5, "bh_silhouette: at most 65535 labels");
     if ((n + C::TILE - 1) / C::TILE > 2147483647LL) return fail(-5, "bh_silhouette: n too large");
     int32_t rc = 0;
-    BH_BY_SLOTS(F, (silhouette_launch<FCV>(X_sorted, n, F, labels_sorted, starts, n_labels, sums,
-                                            s_out, reinterpret_cast<hipStream_t>(stream))));
+    BH_BY_SLOTS(F, (silhouette_launch<FCV>(X_sorted, weights, n, F, labels_sorted, starts,
+                                            label_counts, n_labels, sums, s_out,
+                                            reinterpret_cast<hipStream_t>(stream))));
     return rc;
 }
 

@@ -18,7 +18,8 @@
 // sum((a_f - b_f)^2) with separate multiplies and adds; neighbour searches
 // compare squared distances.  The spanning tree is built by Boruvka rounds:
 // every point finds its cheapest edge to another component on the GPU
-// (O(n^2) per round, O(log n) rounds), the host merges components.
+// (tiles of rows in spatial order skipped by bounding boxes; O(log n)
+// rounds), the host merges components.
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <stdint.h>
@@ -301,16 +302,20 @@ __global__ void edge_weight_kernel(const double *X, int F, const double *core, c
 }
 
 // Silhouette: rows sorted by label, labels 0..C-1 in contiguous blocks
-// [start[c], start[c+1]).  silhouette_sums_kernel: block (x, c) sums the
-// distances from its rows to every row of label c into sums[c * n + i];
-// silhouette_value_kernel: a = own-label mean (excluding the row itself),
-// b = smallest other-label mean, s = (b - a) / max(a, b) (0 for a label of
-// one row).
+// [start[c], start[c+1]), each row standing for w rows (w = 1 when no
+// weights are given: equal rows collapse into one weighted row).
+// silhouette_sums_kernel: block (x, c) sums the weighted distances from its
+// rows to every row of label c into sums[c * n + i];
+// silhouette_value_kernel: with N_c the weighted size of label c, a = own
+// sum / (N_own - 1) (the row's own copies add distance 0), b = smallest
+// other sum / N_c, s = (b - a) / max(a, b) (0 for a label of one row).
 template <int FC>
-__global__ void __launch_bounds__(TILE) silhouette_sums_kernel(const double *X, int64_t n, int F,
+__global__ void __launch_bounds__(TILE) silhouette_sums_kernel(const double *X, const double *w,
+                                                               int64_t n, int F,
                                                                const int64_t *start,
                                                                double *sums) {
     __shared__ double tile[TILE * FC];
+    __shared__ double tile_w[TILE];
     const int64_t i = (int64_t)blockIdx.x * TILE + threadIdx.x;
     const int c = blockIdx.y;
     double x[FC];
@@ -321,29 +326,36 @@ __global__ void __launch_bounds__(TILE) silhouette_sums_kernel(const double *X, 
         const int cnt = (int)min((int64_t)TILE, c1 - r0);
         __syncthreads();
         stage_tile<FC>(tile, X, r0, cnt, F);
+        if (w)
+            for (int e = threadIdx.x; e < cnt; e += TILE) tile_w[e] = w[r0 + e];
         __syncthreads();
-        if (i < n)
-            for (int j = 0; j < cnt; j++) sum += sqrt(sqdist<FC>(x, tile + j * FC));
+        if (i < n) {
+            if (w)
+                for (int j = 0; j < cnt; j++) sum += tile_w[j] * sqrt(sqdist<FC>(x, tile + j * FC));
+            else
+                for (int j = 0; j < cnt; j++) sum += sqrt(sqdist<FC>(x, tile + j * FC));
+        }
     }
     if (i < n) sums[(int64_t)c * n + i] = sum;
 }
 
 __global__ void silhouette_value_kernel(const double *sums, int64_t n, const int32_t *label,
-                                        const int64_t *start, int C, double *s_out) {
+                                        const int64_t *start, const double *counts, int C,
+                                        double *s_out) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int li = label[i];
     double a = 0.0, b = INFINITY;
     for (int c = 0; c < C; c++) {
-        const int64_t size = start[c + 1] - start[c];
+        const double size = counts ? counts[c] : (double)(start[c + 1] - start[c]);
         const double sum = sums[(int64_t)c * n + i];
         if (c == li)
-            a = size > 1 ? sum / (double)(size - 1) : 0.0;
-        else if (size > 0)
-            b = fmin(b, sum / (double)size);
+            a = size > 1.0 ? sum / (size - 1.0) : 0.0;
+        else if (size > 0.0)
+            b = fmin(b, sum / size);
     }
-    const int64_t own = start[li + 1] - start[li];
-    s_out[i] = (own > 1 && isfinite(b)) ? (b - a) / fmax(a, b) : 0.0;
+    const double own = counts ? counts[li] : (double)(start[li + 1] - start[li]);
+    s_out[i] = (own > 1.0 && isfinite(b)) ? (b - a) / fmax(a, b) : 0.0;
 }
 
 // ---- host side: spanning tree rounds and the cluster hierarchy -------------

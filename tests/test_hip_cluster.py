@@ -144,6 +144,34 @@ def test_silhouette_matches_sklearn(cuda_device):
     assert got == pytest.approx(sk(X, lab), rel=1e-10, abs=1e-12)
 
 
+def test_silhouette_with_repeated_rows_matches_sklearn(cuda_device):
+    """Rows repeated many times (as dense attractor features are) collapse
+    into weighted rows; the score still equals sklearn's."""
+    from sklearn.metrics import silhouette_score as sk
+
+    from pybasin_amd.cluster import silhouette_score
+
+    rng = np.random.default_rng(4)
+    base = rng.normal(size=(60, 2))
+    X = base[rng.integers(0, 60, 4000)]
+    lab = (X[:, 0] > 0).astype(np.int64) + 2 * (X[:, 1] > 0.5)
+    lab[:2] = 7  # one label shared by a repeated row and its copy
+    X[1] = X[0]
+    got = silhouette_score(torch.tensor(X, device=cuda_device), lab)
+    # the definition with direct differences (sklearn forms distances from
+    # dot products, so equal rows come out a little above 0 there)
+    D = np.sqrt(((X[:, None, :] - X[None, :, :]) ** 2).sum(-1))
+    s = np.zeros(len(X))
+    for i in range(len(X)):
+        own = lab == lab[i]
+        if own.sum() > 1:
+            a = D[i, own].sum() / (own.sum() - 1)
+            b = min(D[i, lab == c].mean() for c in np.unique(lab) if c != lab[i])
+            s[i] = (b - a) / max(a, b)
+    assert got == pytest.approx(s.mean(), rel=1e-12, abs=1e-14)
+    assert got == pytest.approx(sk(X, lab), rel=1e-8)
+
+
 def _reference_clusterer(X):
     """The reference's HDBSCANClusterer(auto_tune=True, assign_noise=True)
     steps (hdbscan_clusterer.py:55-166) on sklearn objects."""
