@@ -6,9 +6,11 @@
 // with L the 100 x 100 graph Laplacian (rossler_network_topology.py:143-156).
 // The coupling for the 16 trajectories of a tile is one (16 x 100) . (100 x 100)
 // product per stage on the matrix cores (v_mfma_f32_16x16x4f32, FP32 in and
-// out): each of the eight waves owns a 16-column block of L (seven cover
-// the 100 nodes, the eighth is zero padding), held in registers for the
-// whole launch, and reads its A operand from the stage inputs staged in LDS.
+// out): seven waves each own a 16-column block of L for the first 22 of its
+// 25 k-blocks, the eighth wave the last 3 k-blocks of all seven blocks, so
+// every SIMD runs the same 43-44 products per stage; the B operands stay in
+// registers for the whole launch, the A operand comes from the stage inputs
+// staged in LDS, and the two partial products meet in the consumer.
 //
 // Step control is per trajectory with the torchdiffeq rules used by the
 // per-lane kernel (error RMS over the 300 components with the quotient by
@@ -43,6 +45,9 @@ constexpr int THREADS = 512;  // 8 waves
 constexpr int NPT = 4;        // node slots per thread (last one real for q < 4)
 constexpr int XS = 101;       // LDS row stride (odd: spreads the A-operand reads over banks)
 constexpr int KB = NN / 4;    // k-blocks of the 16x16x4 product
+constexpr int NB = 7;         // 16-column blocks covering 100 nodes
+constexpr int KS = 22;        // k-blocks of each block done by its own wave; the
+                              // eighth wave does the last KB - KS of all seven
 
 enum Mode : int { IDLE = 0, INIT = 1, STEP = 2 };
 
@@ -138,6 +143,7 @@ __device__ __forceinline__ void slot_sum_max(float &sum, float &mx, float (*bs)[
 __global__ void __launch_bounds__(THREADS) tile_kernel(Args a) {
     __shared__ float Xs[TILE * XS];
     __shared__ float Cs[TILE * XS];
+    __shared__ float Cs2[TILE * XS];  // the eighth wave's partial products
     __shared__ float red0[THREADS / 64][TILE];
     __shared__ float red1[THREADS / 64][TILE];
     __shared__ long long slot_ic[TILE];
@@ -147,15 +153,24 @@ __global__ void __launch_bounds__(THREADS) tile_kernel(Args a) {
     const int m = tid & 15, q = tid >> 4;
     const bool has4 = q < NN - 3 * 32;  // fourth node slot (q + 96) is real
 
-    // B operand: this wave's 16-column block of L, all 25 k-blocks (the
-    // eighth wave's block lies past column 99 and is zero: it runs the same
-    // products as the others, which keeps the stage code free of branches
-    // so the local terms can issue while the matrix cores work)
-    float Bop[KB];
+    // B operands: L[4 kb .. 4 kb + 3][16-column block] for this wave's share
+    // (columns past 99 are zero)
+    float Bop[KS];
+    if (wave < NB) {
 #pragma unroll
-    for (int kb = 0; kb < KB; kb++) {
-        const int k = 4 * kb + (lane >> 4), j = 16 * wave + (lane & 15);
-        Bop[kb] = (j < NN) ? a.lap[k * NN + j] : 0.0f;
+        for (int kb = 0; kb < KS; kb++) {
+            const int k = 4 * kb + (lane >> 4), j = 16 * wave + (lane & 15);
+            Bop[kb] = (j < NN) ? a.lap[k * NN + j] : 0.0f;
+        }
+    } else {
+        // blocks b = 0..6, k-blocks KS..KB-1: Bop[(KB - KS) * b + t]
+#pragma unroll
+        for (int b = 0; b < NB; b++)
+#pragma unroll
+            for (int t = 0; t < KB - KS; t++) {
+                const int k = 4 * (KS + t) + (lane >> 4), j = 16 * b + (lane & 15);
+                Bop[(KB - KS) * b + t] = (j < NN) ? a.lap[k * NN + j] : 0.0f;
+            }
     }
 
     // per-thread state for its node slots: x, y, z of nodes q + 32 r
@@ -226,20 +241,37 @@ __global__ void __launch_bounds__(THREADS) tile_kernel(Args a) {
             out[1][r] = xx + pa * yy;
             out[2][r] = pb + zz * (xx - pc);
         }
-        floatx4 acc = {0.f, 0.f, 0.f, 0.f};
         const float *xrow = Xs + (lane & 15) * XS + (lane >> 4);
+        if (wave < NB) {
+            floatx4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-        for (int kb = 0; kb < KB; kb++)
-            acc = __builtin_amdgcn_mfma_f32_16x16x4f32(xrow[4 * kb], Bop[kb], acc, 0, 0, 0);
-        const int col = 16 * wave + (lane & 15);
-        if (col < NN) {
+            for (int kb = 0; kb < KS; kb++)
+                acc = __builtin_amdgcn_mfma_f32_16x16x4f32(xrow[4 * kb], Bop[kb], acc, 0, 0, 0);
+            const int col = 16 * wave + (lane & 15);
+            if (col < NN) {
 #pragma unroll
-            for (int it = 0; it < 4; it++) Cs[(4 * (lane >> 4) + it) * XS + col] = acc[it];
+                for (int it = 0; it < 4; it++) Cs[(4 * (lane >> 4) + it) * XS + col] = acc[it];
+            }
+        } else {
+#pragma unroll
+            for (int b = 0; b < NB; b++) {
+                floatx4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                for (int t = 0; t < KB - KS; t++)
+                    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(xrow[4 * (KS + t)],
+                                                               Bop[(KB - KS) * b + t], acc, 0, 0, 0);
+                const int col = 16 * b + (lane & 15);
+                if (col < NN) {
+#pragma unroll
+                    for (int it = 0; it < 4; it++) Cs2[(4 * (lane >> 4) + it) * XS + col] = acc[it];
+                }
+            }
         }
         __syncthreads();
 #pragma unroll
         for (int r = 0; r < NPT; r++) {
-            const float cpl = (r < 3 || has4) ? Cs[m * XS + q + 32 * r] : 0.0f;
+            const int e = m * XS + q + 32 * r;
+            const float cpl = (r < 3 || has4) ? Cs[e] + Cs2[e] : 0.0f;
             out[0][r] = part[r] - pK * cpl;
         }
     };
