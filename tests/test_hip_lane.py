@@ -262,6 +262,10 @@ def test_refill_quorum_does_not_change_results(cuda_device):
     ("duffing", torch.float32, {}), ("duffing", torch.float64, {}), ("pendulum", torch.float32, {}),
     ("lorenz", torch.float32, {"stop_threshold": 200.0}),
     ("lorenz", torch.float32, {"stop_threshold": 200.0, "inf_fill": True}),
+    # refilled lanes (a resident set smaller than the batch)
+    ("duffing", torch.float32, {"max_lanes": 64}),
+    ("lorenz", torch.float32, {"stop_threshold": 200.0, "inf_fill": True, "max_lanes": 128,
+                               "refill_quorum": 1}),
 ])
 def test_fused_orbit_peaks_equal_trajectory_peaks(cuda_device, name, dtype, kw):
     """bh_solve_batch_peaks: the kernel's peaks equal extract_orbit_data on the
