@@ -342,7 +342,20 @@ __device__ __forceinline__ void lane_step(const LaneArgs &a, LaneT &L, int64_t i
     // sin / cos of the end time: the c = 1 stages here, the start of the next
     // step after an accept
     T sB = 0, cB = 0;
-    if constexpr (Sys::kTime) dsincos<T>(tBf, sB, cB);
+    if constexpr (Sys::kTime) {
+        // rotated from the step's start by d = tB - t0 (exact, as the stage
+        // offsets), with a fresh range-reduced sincos every 8th attempted
+        // step of the lane, so at most 7 rotations accumulate into (s0, c0)
+        const T dB = tBf - tf;
+        if ((L.steps & 7) == 0 || !(fabs(dB) <= T(1))) {
+            dsincos<T>(tBf, sB, cB);
+        } else {
+            T sd, cd;
+            sincos_unit<T>(dB, sd, cd);
+            sB = fma(L.s0, cd, L.c0 * sd);
+            cB = fma(L.c0, cd, -(L.s0 * sd));
+        }
+    }
     Sys::time_terms(tf, tst, L.s0, L.c0, tBf, sB, cB, t1p, aux);
     // Stage inputs y + h * sum_j a_ij k_j: the tableau constants are
     // immediate operands and h is applied once per component.

@@ -123,8 +123,10 @@ struct Duffing {
     }
     template <class T>
     __device__ __forceinline__ static void f_aux(const T (&p)[NP], T cos_t, const T (&y)[S], T (&dy)[S]) {
+        // -delta v - k3 x^3 + A cos t grouped so that three dependent
+        // operations follow the stage input (k3 x, x^2 and delta v at once)
         dy[0] = y[1];
-        dy[1] = -p[0] * y[1] - p[1] * (y[0] * y[0] * y[0]) + p[2] * cos_t;
+        dy[1] = fma(-(p[1] * y[0]), y[0] * y[0], fma(p[2], cos_t, -(p[0] * y[1])));
     }
     template <class T>
     __device__ __forceinline__ static void time_terms(T t0, const T (&ts)[4], T s0, T c0, T tB,
