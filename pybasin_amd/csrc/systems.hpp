@@ -3,7 +3,9 @@
 // Each functor has a compile-time state size S and parameter count NP and
 // evaluates dy = f(t, y; p) entirely in registers.  The arithmetic follows
 // the reference's torch RHS term by term (same operand order; the compiler
-// may contract a multiply-add into one FMA).
+// may contract a multiply-add into one FMA), except Duffing's v', whose three
+// terms are regrouped to shorten its dependency chain (same terms, another
+// rounding order).
 #pragma once
 #include <hip/hip_runtime.h>
 
