@@ -39,7 +39,7 @@ STATE = {"duffing": 2, "pendulum": 2, "lorenz": 3}
 # HBM bytes per dispatch of the Duffing 100k float32 kernel, from rocprofv3
 # FETCH_SIZE + WRITE_SIZE scaled by a plain-copy calibration
 # (profiles/dopri5_lane_duffing_100k_f32_pmc.md)
-TRAFFIC_BYTES = {("duffing", "f32", 100_000): 9.60e6}
+TRAFFIC_BYTES = {("duffing", "f32", 100_000): 9.50e6}
 
 DUFFING_TEMPLATES = [[-0.21, 0.02], [1.05, 0.77], [-0.67, 0.02], [-0.46, 0.30], [-0.43, 0.12]]
 
