@@ -126,6 +126,35 @@ int32_t bh_solve_batch_peaks(int32_t system_id, int32_t dtype, const void *y0s, 
                              int64_t workspace_bytes, int32_t max_lanes, void *stream);
 
 /*
+ * bh_autotune_lane -- the launch-shape tuner of bh_solve_batch as one call:
+ * solves the caller's problem once per (resident-lane cap, refill quorum)
+ * candidate -- one untimed launch, then `reps` launches timed with HIP events
+ * on `stream` -- and returns the fastest pair.  The results of bh_solve_batch
+ * do not depend on the launch shape, so on return the outputs hold the
+ * problem's solution.  Blocks until the timed launches end.  The reference
+ * has no counterpart (its solvers have no launch shape to choose); the
+ * arguments up to workspace_bytes are those of bh_solve_batch, except that
+ * the refill-quorum bits of flags are replaced by each candidate's.
+ *
+ *   caps        (n_caps) host array of max_lanes values (0 = fill the device)
+ *   quorums     (n_quorums) host array of refill quorums, each 1..64
+ *   reps        timed launches per candidate (>= 1); a candidate's time is
+ *               the fastest of them
+ *   best_cap, best_quorum   out: the fastest candidate
+ *   timings_ms  (n_caps * n_quorums) host floats [cap][quorum], or NULL
+ */
+int32_t bh_autotune_lane(int32_t system_id, int32_t dtype, const void *y0s, int64_t n,
+                         int32_t dim, const double *params, int32_t n_params,
+                         int64_t params_stride, double t0, const double *save_ts, int32_t n_save,
+                         double rtol, double atol, int64_t max_steps, uint32_t flags,
+                         double stop_threshold, int32_t steady_idx, void *out_traj,
+                         double *out_feats, void *out_final, int32_t *out_status,
+                         int64_t *out_nsteps, void *workspace, int64_t workspace_bytes,
+                         const int32_t *caps, int32_t n_caps, const int32_t *quorums,
+                         int32_t n_quorums, int32_t reps, int32_t *best_cap, int32_t *best_quorum,
+                         float *timings_ms, void *stream);
+
+/*
  * bh_solve_rossler -- the 100-node Rossler network (BH_SYS_ROSSLER100, state
  * layout [x_0..x_99, y_0..y_99, z_0..z_99]), 16 trajectories per workgroup,
  * coupling K (L x) as a (16 x 100) . (100 x 100) FP32 matrix-core product per

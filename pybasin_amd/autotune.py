@@ -3,8 +3,9 @@
 The tunables are the resident-lane cap of the per-lane kernel (how many
 waves per SIMD stay resident; finished lanes refill from the counter), its
 refill quorum (how many finished lanes of a wave wait before they refill
-together), and the resident-workgroup cap of the Rossler tile kernel.  The tuner times each
-candidate on a probe of the caller's own problem (same system, dtype,
+together), and the resident-workgroup cap of the Rossler tile kernel.  The per-lane tuner is
+the library's bh_autotune_lane, which times each candidate (HIP events on
+the launch stream) on a probe of the caller's own problem (same system, dtype,
 parameters and ICs; by default the whole run with its save grid, since
 early exits such as divergence stops decide whether refilling pays and may
 come late; no trajectory output), keeps the fastest, and caches the choice per (device,
@@ -92,21 +93,20 @@ def tune_lane_launch(solver, ode_system, y0: torch.Tensor, params: torch.Tensor 
             return int(cache[key]["max_lanes"]), int(cache[key]["refill_quorum"])
     props = torch.cuda.get_device_properties(dev)
     wave_set = props.multi_processor_count * 4 * 64
-    results = {}
-    for c in caps:
-        lanes = 0 if c == 0 else int(c * wave_set)
-        for q in quorums:
-            if probe_span is None:
-                # the caller's own run: same span and save grid, no trajectories
-                probe = solver.clone(cache_dir=None)
-                probe.max_lanes = lanes
-                probe.refill_quorum = int(q)
-            else:
-                probe = type(solver)(t_span=(solver.t_span[0], solver.t_span[0] + probe_span),
-                                     t_steps=2, device=str(dev), rtol=solver.rtol, atol=solver.atol,
-                                     cache_dir=None, stop_threshold=solver.stop_threshold,
-                                     inf_fill=solver.inf_fill, max_lanes=lanes, refill_quorum=int(q))
-            results[(lanes, int(q))] = _time(lambda: probe.integrate_fused(ode_system, y0, params))
+    lanes = [0 if c == 0 else int(c * wave_set) for c in caps]
+    if probe_span is None:
+        # the caller's own run: same span and save grid, no trajectories
+        probe = solver.clone(cache_dir=None)
+    else:
+        probe = type(solver)(t_span=(solver.t_span[0], solver.t_span[0] + probe_span), t_steps=2,
+                             device=str(dev), rtol=solver.rtol, atol=solver.atol, cache_dir=None,
+                             stop_threshold=solver.stop_threshold, inf_fill=solver.inf_fill)
+    y0e, pe = probe._expand(y0, params)
+    save_ts, y0e = probe._prepare_tensors(y0e)
+    plan = probe.plan_fused(ode_system, y0e, save_ts, pe)
+    # bh_autotune_lane: one untimed and two timed launches per candidate
+    # (quorum 0 is the library default, 64)
+    _, _, results, _ = probe.tune_fused(plan, lanes, [int(q) or 64 for q in quorums], reps=2)
     best = min(results, key=results.get)
     last_timings.clear()
     last_timings.update({f"{k[0]}/q{k[1]}": round(v, 3) for k, v in results.items()})

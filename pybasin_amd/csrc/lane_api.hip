@@ -1,5 +1,6 @@
 // lane_api.hip -- C-ABI entry points of the per-lane Dopri5 solver
-// (bh_solve_batch, bh_solve_batch_peaks; declared in include/basinhip.h).
+// (bh_solve_batch, bh_solve_batch_peaks, bh_autotune_lane; declared in
+// include/basinhip.h).
 // Its own translation unit so that it can be built with the instruction
 // scheduler that favours independent instructions: the step loop of a
 // wave alone on its SIMD is bound by the latency of its dependent
@@ -175,6 +176,78 @@ int32_t bh_solve_batch_peaks(int32_t system_id, int32_t dtype, const void *y0s, 
                             steady_idx, out_traj, out_feats, out_final, out_status, out_nsteps,
                             peak_cap, out_peak_vals, out_peak_counts, workspace, workspace_bytes,
                             max_lanes, stream);
+}
+
+int32_t bh_autotune_lane(int32_t system_id, int32_t dtype, const void *y0s, int64_t n,
+                         int32_t dim, const double *params, int32_t n_params,
+                         int64_t params_stride, double t0, const double *save_ts, int32_t n_save,
+                         double rtol, double atol, int64_t max_steps, uint32_t flags,
+                         double stop_threshold, int32_t steady_idx, void *out_traj,
+                         double *out_feats, void *out_final, int32_t *out_status,
+                         int64_t *out_nsteps, void *workspace, int64_t workspace_bytes,
+                         const int32_t *caps, int32_t n_caps, const int32_t *quorums,
+                         int32_t n_quorums, int32_t reps, int32_t *best_cap, int32_t *best_quorum,
+                         float *timings_ms, void *stream) {
+    g_err[0] = 0;
+    if (!caps || !quorums || n_caps < 1 || n_quorums < 1)
+        return fail(-5, "at least one cap and one quorum are required");
+    if (!best_cap || !best_quorum) return fail(-5, "best_cap and best_quorum are required");
+    if (reps < 1) return fail(-5, "reps must be >= 1, got %d", reps);
+    for (int32_t i = 0; i < n_caps; i++)
+        if (caps[i] < 0) return fail(-5, "caps[%d] = %d: caps are max_lanes values >= 0", i, caps[i]);
+    for (int32_t j = 0; j < n_quorums; j++)
+        if (quorums[j] < 1 || quorums[j] > 64)
+            return fail(-5, "quorums[%d] = %d: a refill quorum is 1..64", j, quorums[j]);
+    *best_cap = caps[0];
+    *best_quorum = quorums[0];
+    if (timings_ms)
+        for (int32_t k = 0; k < n_caps * n_quorums; k++) timings_ms[k] = 0.0f;
+    if (n == 0) return 0;
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    int32_t rc = 0;
+    float best = INFINITY;
+    if (hipEventCreate(&ev[0]) != hipSuccess || hipEventCreate(&ev[1]) != hipSuccess) {
+        rc = fail(-6, "hipEventCreate failed");
+    }
+    const uint32_t base = flags & ~(0x7fu << BH_REFILL_QUORUM_SHIFT);
+    for (int32_t i = 0; i < n_caps && rc == 0; i++) {
+        for (int32_t j = 0; j < n_quorums && rc == 0; j++) {
+            const uint32_t f = base | BH_REFILL_QUORUM(quorums[j]);
+            float t_min = INFINITY;
+            // one untimed launch, then reps timed ones; every launch solves the
+            // whole problem (the outputs do not depend on the launch shape)
+            for (int32_t r = 0; r <= reps && rc == 0; r++) {
+                if (r > 0 && hipEventRecord(ev[0], s) != hipSuccess) {
+                    rc = fail(-6, "hipEventRecord failed");
+                    break;
+                }
+                rc = solve_batch_impl(system_id, dtype, y0s, n, dim, params, n_params,
+                                      params_stride, t0, save_ts, n_save, rtol, atol, max_steps, f,
+                                      stop_threshold, steady_idx, out_traj, out_feats, out_final,
+                                      out_status, out_nsteps, 0, nullptr, nullptr, workspace,
+                                      workspace_bytes, caps[i], stream);
+                if (rc || r == 0) continue;
+                float ms = 0.0f;
+                if (hipEventRecord(ev[1], s) != hipSuccess || hipEventSynchronize(ev[1]) != hipSuccess ||
+                    hipEventElapsedTime(&ms, ev[0], ev[1]) != hipSuccess) {
+                    rc = fail(-6, "timing the launch with HIP events failed");
+                    break;
+                }
+                t_min = fminf(t_min, ms);
+            }
+            if (rc) break;
+            if (timings_ms) timings_ms[i * n_quorums + j] = t_min;
+            if (t_min < best) {
+                best = t_min;
+                *best_cap = caps[i];
+                *best_quorum = quorums[j];
+            }
+        }
+    }
+    for (hipEvent_t e : ev)
+        if (e) (void)hipEventDestroy(e);
+    return rc;
 }
 
 }  // extern "C"

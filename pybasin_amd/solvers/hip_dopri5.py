@@ -247,6 +247,43 @@ class HipDopri5Solver(Solver):
         return LanePlan(sys_id=int(sys_id), y0=y0, p=p, p_stride=p_stride, save_ts=save_ts,
                         st64=st64, steady_idx=int(steady_idx))
 
+    def _lane_call(self, plan: "LanePlan", keep_trajectories: bool):
+        """Output tensors and the bh_solve_batch arguments shared by every
+        lane entry point (system id through out_nsteps), for a prepared plan."""
+        dev = self.device
+        y0, p, st64 = plan.y0, plan.p, plan.st64
+        dtype_id = _lib.BH_F32 if y0.dtype == torch.float32 else _lib.BH_F64
+        n, S = y0.shape
+        n_save = st64.shape[0]
+        out = dict(
+            traj=torch.empty((n_save, n, S), dtype=y0.dtype, device=dev) if keep_trajectories else None,
+            stats=torch.empty((n, S, _lib.BH_NFEAT), dtype=torch.float64, device=dev),
+            final=torch.empty((n, S), dtype=y0.dtype, device=dev),
+            status=torch.empty((n,), dtype=torch.int32, device=dev),
+            nsteps=torch.empty((n,), dtype=torch.int64, device=dev),
+            ws=torch.empty((_lib.BH_WORKSPACE_BYTES,), dtype=torch.uint8, device=dev))
+        flags = (self.refill_quorum & 0x7F) << _lib.BH_REFILL_QUORUM_SHIFT
+        thr = 0.0
+        if self.stop_threshold is not None:
+            flags |= _lib.BH_FLAG_STOP
+            thr = float(self.stop_threshold)
+            if self.inf_fill:
+                flags |= _lib.BH_FLAG_INF_FILL
+        common = (plan.sys_id, dtype_id, y0.data_ptr(), n, S, p.data_ptr(), p.shape[1],
+                  plan.p_stride, float(self.t_span[0]), st64.data_ptr(), n_save,
+                  float(self.rtol), float(self.atol), self.max_steps, flags, thr, plan.steady_idx,
+                  _lib.ptr(out["traj"]), out["stats"].data_ptr(), out["final"].data_ptr(),
+                  out["status"].data_ptr(), out["nsteps"].data_ptr())
+        return out, common
+
+    def _result(self, plan: "LanePlan", out: dict, pk_vals=None, pk_counts=None) -> FusedResult:
+        n_save = plan.st64.shape[0]
+        return FusedResult(t=plan.save_ts, y=out["traj"], final=out["final"],
+                           stats=out["stats"] if plan.steady_idx < n_save else None,
+                           status=out["status"], nsteps=out["nsteps"],
+                           steady_idx=int(plan.steady_idx), peak_values=pk_vals,
+                           peak_counts=pk_counts)
+
     def launch_fused(self, plan: "LanePlan", *, keep_trajectories: bool,
                      stream: torch.cuda.Stream | None = None,
                      peak_cap: int | None = None) -> FusedResult:
@@ -255,29 +292,10 @@ class HipDopri5Solver(Solver):
         peak_cap is given)."""
         dev = self.device
         y0, p, st64 = plan.y0, plan.p, plan.st64
-        dtype_id = _lib.BH_F32 if y0.dtype == torch.float32 else _lib.BH_F64
         n, S = y0.shape
-        n_save = st64.shape[0]
-        steady_idx = plan.steady_idx
-        traj = torch.empty((n_save, n, S), dtype=y0.dtype, device=dev) if keep_trajectories else None
-        stats = torch.empty((n, S, _lib.BH_NFEAT), dtype=torch.float64, device=dev)
-        final = torch.empty((n, S), dtype=y0.dtype, device=dev)
-        status = torch.empty((n,), dtype=torch.int32, device=dev)
-        nsteps = torch.empty((n,), dtype=torch.int64, device=dev)
-        ws = torch.empty((_lib.BH_WORKSPACE_BYTES,), dtype=torch.uint8, device=dev)
-        flags = (self.refill_quorum & 0x7F) << _lib.BH_REFILL_QUORUM_SHIFT
-        thr = 0.0
-        if self.stop_threshold is not None:
-            flags |= _lib.BH_FLAG_STOP
-            thr = float(self.stop_threshold)
-            if self.inf_fill:
-                flags |= _lib.BH_FLAG_INF_FILL
+        out, common = self._lane_call(plan, keep_trajectories)
+        ws = out["ws"]
         s = stream if stream is not None else torch.cuda.current_stream(dev)
-        common = (plan.sys_id, dtype_id, y0.data_ptr(), n, S, p.data_ptr(), p.shape[1],
-                  plan.p_stride, float(self.t_span[0]), st64.data_ptr(), n_save,
-                  float(self.rtol), float(self.atol), self.max_steps, flags, thr, steady_idx,
-                  _lib.ptr(traj), stats.data_ptr(), final.data_ptr(), status.data_ptr(),
-                  nsteps.data_ptr())
         pk_vals = pk_counts = None
         with torch.cuda.device(dev):
             if peak_cap is None:
@@ -293,8 +311,31 @@ class HipDopri5Solver(Solver):
         _lib.check(rc, "bh_solve_batch")
         for t in (y0, p, st64, ws):
             t.record_stream(s)
-        save_ts = plan.save_ts
-        return FusedResult(t=save_ts, y=traj, final=final,
-                           stats=stats if steady_idx < n_save else None, status=status,
-                           nsteps=nsteps, steady_idx=int(steady_idx), peak_values=pk_vals,
-                           peak_counts=pk_counts)
+        return self._result(plan, out, pk_vals, pk_counts)
+
+    def tune_fused(self, plan: "LanePlan", caps=(0,), quorums=(64,), reps: int = 2,
+                   stream: torch.cuda.Stream | None = None):
+        """bh_autotune_lane on a prepared plan: solve it once per (max_lanes cap,
+        refill quorum) candidate, timing each launch, and return (best cap,
+        best quorum, {(cap, quorum): ms}, FusedResult of the problem).  Blocks
+        until the timed launches end; no trajectories are written."""
+        import ctypes
+
+        dev = self.device
+        out, common = self._lane_call(plan, False)
+        ws = out["ws"]
+        caps = [int(c) for c in caps]
+        quorums = [int(q) for q in quorums]
+        c_caps = (ctypes.c_int32 * len(caps))(*caps)
+        c_q = (ctypes.c_int32 * len(quorums))(*quorums)
+        c_t = (ctypes.c_float * (len(caps) * len(quorums)))()
+        best_cap, best_q = ctypes.c_int32(), ctypes.c_int32()
+        s = stream if stream is not None else torch.cuda.current_stream(dev)
+        with torch.cuda.device(dev):
+            rc = _lib.lib().bh_autotune_lane(*common, ws.data_ptr(), ws.numel(), c_caps, len(caps),
+                                             c_q, len(quorums), int(reps), ctypes.byref(best_cap),
+                                             ctypes.byref(best_q), c_t, s.cuda_stream)
+        _lib.check(rc, "bh_autotune_lane")
+        timings = {(c, q): float(c_t[i * len(quorums) + j])
+                   for i, c in enumerate(caps) for j, q in enumerate(quorums)}
+        return best_cap.value, best_q.value, timings, self._result(plan, out)

@@ -49,3 +49,8 @@ def test_version_and_argument_errors_without_gpu():
     assert rc == -5
     with pytest.raises(ValueError):
         _lib.check(rc, "bh_solve_batch")
+    # the tuner checks its candidate lists before any HIP call
+    rc = L.bh_autotune_lane(4, 0, None, 1, 3, None, 3, 0, 0.0, None, 1, 1e-8, 1e-6, 10, 0, 0.0, 0,
+                            None, None, None, None, None, None, 0, None, 0, None, 0, 1, None, None,
+                            None, None)
+    assert rc == -5 and b"cap" in L.bh_last_error()

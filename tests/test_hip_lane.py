@@ -238,6 +238,31 @@ def test_autotuner_picks_and_caches_a_lane_cap(cuda_device, tmp_path, monkeypatc
                                      quorums=(64, 1)) == (cap, q)  # cached
 
 
+def test_native_autotuner_solves_and_picks_the_fastest(cuda_device):
+    """bh_autotune_lane: every (cap, quorum) candidate is timed, the fastest is
+    returned, and the outputs it leaves equal a plain bh_solve_batch launch
+    (the results do not depend on the launch shape)."""
+    y0 = torch.tensor(O.sample_uniform(3000, [-10.0, -20.0, 0.0], [10.0, 20.0, 0.0], seed=6),
+                      dtype=torch.float32).to(cuda_device)
+    ode = LorenzODE({"sigma": 0.12, "r": 0.0, "b": -0.6})
+    s = HipDopri5Solver(t_span=(0, 100), t_steps=200, t_eval=(50.0, 100.0), device="cuda",
+                        cache_dir=None, stop_threshold=200.0, inf_fill=True)
+    save_ts, y0p = s._prepare_tensors(y0)
+    plan = s.plan_fused(ode, y0p, save_ts, None)
+    caps, quorums = [0, 128, 512], [1, 64]
+    cap, q, timings, res = s.tune_fused(plan, caps, quorums, reps=1)
+    assert set(timings) == {(c, k) for c in caps for k in quorums}
+    assert all(t > 0 for t in timings.values())
+    assert timings[(cap, q)] == min(timings.values())
+    base = s.launch_fused(plan, keep_trajectories=False)
+    assert torch.equal(res.stats.nan_to_num(7.0), base.stats.nan_to_num(7.0))
+    assert torch.equal(res.final.nan_to_num(7.0), base.final.nan_to_num(7.0))
+    assert torch.equal(res.status, base.status) and torch.equal(res.nsteps, base.nsteps)
+    assert (base.status == 4).any() and (base.status == 0).any()
+    with pytest.raises(ValueError, match="quorum"):
+        s.tune_fused(plan, [0], [65])
+
+
 def test_refill_quorum_does_not_change_results(cuda_device):
     """Per-lane and whole-wave refills (quorum 1 and 64) with divergence
     stops give bitwise identical fused results."""
