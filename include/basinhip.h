@@ -224,6 +224,28 @@ int32_t bh_mlp_infer(const float *x, int64_t M, const void *w_packed, const floa
 int32_t bh_mlp_classify(const float *Z, const float *bias, int64_t M, int32_t *label,
                         float *conf, void *stream);
 
+/* One training step of the surrogate on a batch, as one call: forward
+   (bh_mlp_prep_input, three GEMM + bias + SiLU layers, GEMM + bias to
+   logits), mean softmax cross-entropy, backward, and Adam on every weight
+   and bias -- the same kernels, in the same arithmetic, as the separate
+   entry points above.  No counterpart in the reference (learned surrogate).
+     x (M, 3) float32, labels (M) int32 in {0, 1}; norm: 6 host floats
+     (c0, s0, c1, s1, c2, s2 of bh_mlp_prep_input); hidden: width H
+     W, W_bf16, b, mW, vW, mb, vb: host arrays of 4 device pointers, one per
+       layer -- weights (K_in x N_out) float32 row-major with K_in 16, H, H,
+       H and N_out H, H, H, 2; their bfloat16 copies (rewritten by the
+       update); biases (N_out); Adam moments shaped like each
+     step: Adam step number (>= 1, for the bias correction)
+     loss_out (M) float32 per-row loss, or NULL
+     workspace: >= bh_mlp_train_workspace_bytes(M, hidden) device bytes */
+int64_t bh_mlp_train_workspace_bytes(int64_t M, int32_t hidden);
+int32_t bh_mlp_train_step(const float *x, const int32_t *labels, int64_t M, const float *norm,
+                          int32_t hidden, float *const *W, void *const *W_bf16, float *const *b,
+                          float *const *mW, float *const *vW, float *const *mb, float *const *vb,
+                          float lr, float beta1, float beta2, float eps, int64_t step,
+                          float *loss_out, void *workspace, int64_t workspace_bytes,
+                          void *stream);
+
 /* Order statistics of the feature window of a kept trajectory buffer
    (n_save, n, dim) in dtype: out (n_ranks, n, dim) in dtype holds, per
    (trajectory, state), the value of 0-based rank ranks[r] (device int64,

@@ -26,7 +26,8 @@ BH_WORKSPACE_BYTES = 256
 EXPORTS = ("bh_version", "bh_last_error", "bh_solve_batch", "bh_solve_batch_peaks",
            "bh_autotune_lane", "bh_solve_rossler", "bh_gemm_bf16",
            "bh_mlp_bias_act", "bh_mlp_silu_bwd", "bh_mlp_softmax_ce", "bh_mlp_colsum",
-           "bh_mlp_adam", "bh_mlp_prep_input", "bh_mlp_classify", "bh_mlp_infer", "bh_knn",
+           "bh_mlp_adam", "bh_mlp_prep_input", "bh_mlp_classify", "bh_mlp_infer",
+           "bh_mlp_train_workspace_bytes", "bh_mlp_train_step", "bh_knn",
            "bh_hdbscan_workspace_bytes", "bh_core_distances", "bh_hdbscan_mst", "bh_hdbscan_labels",
            "bh_silhouette",
            "bh_window_order_stats")
@@ -61,6 +62,8 @@ def _declare(L: ctypes.CDLL) -> None:
                                    f64, vp, vp, vp, vp, i64, i32, vp]
     L.bh_hdbscan_workspace_bytes.restype = i64
     L.bh_hdbscan_workspace_bytes.argtypes = [i64]
+    L.bh_mlp_train_workspace_bytes.restype = i64
+    L.bh_mlp_train_workspace_bytes.argtypes = [i64, i32]
     f32 = ctypes.c_float
     sigs = {
         "bh_gemm_bf16": [vp, i64, i64, vp, i64, i64, vp, i64, i64, i64, i64, i32, vp],
@@ -72,6 +75,8 @@ def _declare(L: ctypes.CDLL) -> None:
         "bh_mlp_prep_input": [vp, i64, f32, f32, f32, f32, f32, f32, vp, vp],
         "bh_mlp_classify": [vp, vp, i64, vp, vp, vp],
         "bh_mlp_infer": [vp, i64, vp, vp, f32, f32, f32, f32, f32, f32, vp, vp, vp],
+        "bh_mlp_train_step": [vp, vp, i64, vp, i32, vp, vp, vp, vp, vp, vp, vp, f32, f32, f32, f32,
+                              i64, vp, vp, i64, vp],
         "bh_knn": [vp, i64, vp, i64, i32, i32, vp, vp, vp],
         "bh_core_distances": [vp, i64, i32, i32, vp, vp, i64, vp],
         "bh_hdbscan_mst": [vp, i64, i32, vp, vp, vp, vp, vp, i64, vp],

@@ -310,6 +310,122 @@ int32_t bh_mlp_infer(const float *x, int64_t M, const void *w_packed, const floa
     return 0;
 }
 
+// Workspace of bh_mlp_train_step, carved in this order (each piece aligned to
+// 256 bytes): input activations (M, 16) bf16; hidden activations 3 x (M, H)
+// bf16; saved pre-activations 3 x (M, H) f32; GEMM output (M, H) f32; output
+// logits and their gradient (M, 2) f32 x 2 + (M, 2) bf16; per-row loss (M)
+// f32; back-propagated gradients 2 x (M, H) f32 + 2 x (M, H) bf16 (ping-pong)
+// and dA (M, H) f32; one layer's weight and bias gradients max(16, H) x H f32
+// + H f32.
+namespace {
+constexpr int64_t kMlpIn = 16;  // padded input width (bh_mlp_prep_input)
+
+struct TrainLayout {
+    int64_t a0, act[3], z[3], zbuf, logits, dlog, dlogb, loss, g[2], gb16[2], dA, gw, gbias, total;
+};
+
+TrainLayout train_layout(int64_t M, int64_t H) {
+    TrainLayout L{};
+    int64_t off = 0;
+    auto take = [&off](int64_t bytes) {
+        const int64_t at = off;
+        off += (bytes + 255) / 256 * 256;
+        return at;
+    };
+    L.a0 = take(M * kMlpIn * 2);
+    for (int i = 0; i < 3; i++) L.act[i] = take(M * H * 2);
+    for (int i = 0; i < 3; i++) L.z[i] = take(M * H * 4);
+    L.zbuf = take(M * H * 4);
+    L.logits = take(M * 2 * 4);
+    L.dlog = take(M * 2 * 4);
+    L.dlogb = take(M * 2 * 2);
+    L.loss = take(M * 4);
+    for (int i = 0; i < 2; i++) L.g[i] = take(M * H * 4);
+    for (int i = 0; i < 2; i++) L.gb16[i] = take(M * H * 2);
+    L.dA = take(M * H * 4);
+    L.gw = take((H > kMlpIn ? H : kMlpIn) * H * 4);
+    L.gbias = take(H * 4);
+    L.total = off;
+    return L;
+}
+}  // namespace
+
+int64_t bh_mlp_train_workspace_bytes(int64_t M, int32_t hidden) {
+    if (M < 0 || hidden < 2) return -5;
+    return train_layout(M, hidden).total;
+}
+
+int32_t bh_mlp_train_step(const float *x, const int32_t *labels, int64_t M, const float *norm,
+                          int32_t hidden, float *const *W, void *const *W_bf16, float *const *b,
+                          float *const *mW, float *const *vW, float *const *mb, float *const *vb,
+                          float lr, float beta1, float beta2, float eps, int64_t step,
+                          float *loss_out, void *workspace, int64_t workspace_bytes,
+                          void *stream) {
+    g_err[0] = 0;
+    if (M < 0) return fail(-5, "M must be >= 0, got %lld", (long long)M);
+    if (hidden < 2) return fail(-5, "hidden must be >= 2, got %d", hidden);
+    if (!W || !W_bf16 || !b || !mW || !vW || !mb || !vb || !norm)
+        return fail(-5, "the parameter, moment and norm arrays are required");
+    for (int l = 0; l < 4; l++)
+        if (!W[l] || !W_bf16[l] || !b[l] || !mW[l] || !vW[l] || !mb[l] || !vb[l])
+            return fail(-5, "null pointer in the arrays of layer %d", l);
+    if (step < 1) return fail(-5, "step must be >= 1 (Adam's bias correction)");
+    if (M == 0) return 0;
+    if (!x || !labels) return fail(-5, "x and labels are required");
+    const int64_t H = hidden;
+    const TrainLayout L = train_layout(M, H);
+    if (!workspace || workspace_bytes < L.total)
+        return fail(-5, "workspace of %lld bytes is required", (long long)L.total);
+    char *ws = reinterpret_cast<char *>(workspace);
+    auto f32 = [ws](int64_t off) { return reinterpret_cast<float *>(ws + off); };
+    auto b16 = [ws](int64_t off) { return reinterpret_cast<void *>(ws + off); };
+    int32_t rc;
+#define BH_TRY(call)          \
+    do {                      \
+        rc = (call);          \
+        if (rc) return rc;    \
+    } while (0)
+    // forward: input features, three SiLU layers, logits (pre-activations kept)
+    BH_TRY(bh_mlp_prep_input(x, M, norm[0], norm[1], norm[2], norm[3], norm[4], norm[5],
+                             b16(L.a0), stream));
+    const int64_t K_in[4] = {kMlpIn, H, H, H}, N_out[4] = {H, H, H, 2};
+    const void *act_in[4] = {b16(L.a0), b16(L.act[0]), b16(L.act[1]), b16(L.act[2])};
+    for (int l = 0; l < 3; l++) {
+        BH_TRY(bh_gemm_bf16(act_in[l], K_in[l], 1, W_bf16[l], N_out[l], 1, f32(L.zbuf), N_out[l], M,
+                            N_out[l], K_in[l], 0, stream));
+        BH_TRY(bh_mlp_bias_act(f32(L.zbuf), M, N_out[l], b[l], 1, f32(L.z[l]), b16(L.act[l]),
+                               N_out[l], nullptr, stream));
+    }
+    BH_TRY(bh_gemm_bf16(act_in[3], H, 1, W_bf16[3], 2, 1, f32(L.zbuf), 2, M, 2, H, 0, stream));
+    BH_TRY(bh_mlp_bias_act(f32(L.zbuf), M, 2, b[3], 0, nullptr, nullptr, 0, f32(L.logits), stream));
+    float *loss = loss_out ? loss_out : f32(L.loss);
+    BH_TRY(bh_mlp_softmax_ce(f32(L.logits), labels, M, f32(L.dlog), b16(L.dlogb), loss, stream));
+    // backward, last layer first; each layer's Adam update runs once its weights
+    // have served the gradient of the layer below
+    const float *gz = f32(L.dlog);
+    const void *gzb = b16(L.dlogb);
+    for (int l = 3; l >= 0; l--) {
+        BH_TRY(bh_gemm_bf16(act_in[l], 1, K_in[l], gzb, N_out[l], 1, f32(L.gw), N_out[l], K_in[l],
+                            N_out[l], M, 0, stream));
+        BH_TRY(bh_mlp_colsum(gz, M, N_out[l], f32(L.gbias), stream));
+        if (l > 0) {
+            const int nxt = l & 1;
+            BH_TRY(bh_gemm_bf16(gzb, N_out[l], 1, W_bf16[l], 1, N_out[l], f32(L.dA), K_in[l], M,
+                                K_in[l], N_out[l], 0, stream));
+            BH_TRY(bh_mlp_silu_bwd(f32(L.dA), f32(L.z[l - 1]), M * K_in[l], f32(L.g[nxt]),
+                                   b16(L.gb16[nxt]), stream));
+            gz = f32(L.g[nxt]);
+            gzb = b16(L.gb16[nxt]);
+        }
+        BH_TRY(bh_mlp_adam(W[l], f32(L.gw), mW[l], vW[l], K_in[l] * N_out[l], lr, beta1, beta2, eps,
+                           step, W_bf16[l], stream));
+        BH_TRY(bh_mlp_adam(b[l], f32(L.gbias), mb[l], vb[l], N_out[l], lr, beta1, beta2, eps, step,
+                           nullptr, stream));
+    }
+#undef BH_TRY
+    return 0;
+}
+
 
 // ---- label layer ------------------------------------------------------------
 

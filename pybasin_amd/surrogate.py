@@ -148,6 +148,34 @@ class SurrogateMLP:
                                      self.vb[i].data_ptr(), self.b[i].numel(), lr, b1, b2, eps,
                                      self.step, None, _s(stream)), "bh_mlp_adam")
 
+    def train_step(self, x: torch.Tensor, labels: torch.Tensor, lr=1e-3, b1=0.9, b2=0.999,
+                   eps=1e-8, stream=None) -> torch.Tensor:
+        """One native training step (bh_mlp_train_step: forward, loss, backward
+        and Adam in one call, the same kernels as gradients() + adam_step());
+        returns the per-row loss.  The workspace is kept between calls, so
+        calls on different streams must not overlap."""
+        import ctypes
+
+        stream = stream or torch.cuda.current_stream(self.device)
+        x = x.to(self.device, torch.float32).contiguous()
+        labels = labels.to(self.device, torch.int32).contiguous()
+        M = x.shape[0]
+        L = _lib.lib()
+        need = L.bh_mlp_train_workspace_bytes(M, self.hidden)
+        ws = getattr(self, "_train_ws", None)
+        if ws is None or ws.numel() < need:
+            ws = self._train_ws = torch.empty((max(need, 1),), dtype=torch.uint8, device=self.device)
+        loss = torch.empty((M,), dtype=torch.float32, device=self.device)
+        arr = lambda ts: (ctypes.c_void_p * 4)(*[t.data_ptr() for t in ts])  # noqa: E731
+        norm = (ctypes.c_float * 6)(*self.norm)
+        self.step += 1
+        _lib.check(L.bh_mlp_train_step(x.data_ptr(), labels.data_ptr(), M, norm, self.hidden,
+                                       arr(self.W), arr(self.Wb), arr(self.b), arr(self.mW),
+                                       arr(self.vW), arr(self.mb), arr(self.vb), lr, b1, b2, eps,
+                                       self.step, loss.data_ptr(), ws.data_ptr(), ws.numel(),
+                                       _s(stream)), "bh_mlp_train_step")
+        return loss
+
     def fit(self, x: torch.Tensor, labels: torch.Tensor, epochs: int = 60, batch: int = 1024,
             lr: float = 3e-3, seed: int = 0) -> list[float]:
         """Mini-batch Adam on (x, labels); returns the mean loss per epoch."""
@@ -162,8 +190,7 @@ class SurrogateMLP:
             lr_ep = lr * 0.5 * (1 + math.cos(math.pi * ep / epochs))
             for i in range(0, n - batch + 1, batch):
                 idx = perm[i:i + batch]
-                loss, dW, db = self.gradients(x[idx].contiguous(), labels[idx].contiguous())
-                self.adam_step(dW, db, lr=lr_ep)
+                loss = self.train_step(x[idx], labels[idx], lr=lr_ep)
                 tot += loss.double().sum()
             hist.append(float(tot) / (n - n % batch))
         return hist

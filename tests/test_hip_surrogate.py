@@ -87,6 +87,28 @@ def test_forward_and_gradients_match_cpu_autograd(cuda_device):
         assert float((gb - gb_ref).norm() / (gb_ref.norm() + 1e-12)) < 3e-2
 
 
+def test_native_train_step_equals_the_composed_step(cuda_device):
+    """bh_mlp_train_step (one call) leaves the weights, their bf16 copies, the
+    Adam moments and the loss bit for bit where gradients() + adam_step()
+    (the separate kernels) leave them, over two steps on different batch sizes."""
+    a = SurrogateMLP(device=cuda_device, seed=4)
+    b = SurrogateMLP(device=cuda_device, seed=4)
+    g = torch.Generator().manual_seed(5)
+    for n, lr in ((700, 3e-3), (257, 1e-3)):
+        x = torch.stack([torch.rand(n, generator=g) * 6.28 - 2.6, torch.rand(n, generator=g) * 20 - 10,
+                         torch.rand(n, generator=g) * 0.9 + 0.1], 1).to(cuda_device)
+        labels = (torch.rand(n, generator=g) > 0.5).to(torch.int32).to(cuda_device)
+        loss_a = a.train_step(x, labels, lr=lr)
+        loss_b, dW, db = b.gradients(x, labels)
+        b.adam_step(dW, db, lr=lr)
+        torch.cuda.synchronize()
+        assert a.step == b.step
+        assert torch.equal(loss_a, loss_b)
+        for name in ("W", "Wb", "b", "mW", "vW", "mb", "vb"):
+            for i in range(4):
+                assert torch.equal(getattr(a, name)[i], getattr(b, name)[i]), (name, i, n)
+
+
 def test_training_separates_a_nonlinear_boundary(cuda_device):
     """Adam on the kernels learns a ring-shaped two-class boundary."""
     model = SurrogateMLP(device=cuda_device, seed=2)
