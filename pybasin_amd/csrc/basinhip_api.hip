@@ -6,6 +6,10 @@
 #include "cluster.hip"
 #include "window.hip"
 
+#include <system_error>
+#include <thread>
+#include <vector>
+
 thread_local char bhapi::g_err[512] = "";
 
 namespace {
@@ -576,13 +580,28 @@ int32_t bh_hdbscan_labels(const int64_t *edge_a, const int64_t *edge_b, const do
     for (int64_t e = 0; e < n - 1; e++)
         if (edge_a[e] < 0 || edge_a[e] >= n || edge_b[e] < 0 || edge_b[e] >= n || !(edge_w[e] >= 0.0))
             return fail(-5, "bh_hdbscan_labels: edge %lld out of range", (long long)e);
-    // one hierarchy, condensed once per requested min_cluster_size
+    // one hierarchy, condensed once per requested min_cluster_size; the sizes
+    // are independent and run on host threads of their own
     const C::Hierarchy h = C::single_linkage(n, edge_a, edge_b, edge_w);
-    for (int32_t k = 0; k < n_sizes; k++) {
+    auto one = [&](int32_t k) {
         const C::Condensed c = C::condense(h, n, min_cluster_sizes[k]);
         const int32_t nc = C::select_and_label(c, n, labels + (int64_t)k * n);
         if (n_clusters) n_clusters[k] = nc;
+    };
+    if (n_sizes == 1) {
+        one(0);
+        return 0;
     }
+    std::vector<std::thread> pool;
+    pool.reserve(n_sizes - 1);
+    try {
+        for (int32_t k = 1; k < n_sizes; k++) pool.emplace_back(one, k);
+    } catch (const std::system_error &) {
+        // no thread to spare: the remaining sizes run here
+        for (int32_t k = (int32_t)pool.size() + 1; k < n_sizes; k++) one(k);
+    }
+    one(0);
+    for (std::thread &t : pool) t.join();
     return 0;
 }
 
