@@ -7,7 +7,10 @@ seeded-uniform in x in [-1, 1], v in [-0.5, 1]; t in [0, 1000], rtol 1e-8,
 atol 1e-6, float32; save grid 1,000 points on [900, 1000]; fused
 mean / std / min / max per state over that window; 1-NN classification
 against the 5 template ICs of the reference's Duffing setup, integrated in
-the same launch.  Inputs are resident in HBM before timing starts.
+the same launch.  Inputs are resident in HBM before timing starts.  The
+launch shape is chosen once by the project's autotuner before the timed
+steps (reported with its timings; --no-tune keeps one launch with every
+trajectory on its own lane).
 
 Prints one JSON line (rank 0).  --gpus above 1 is not supported: this build
 targets one GPU per process and runs no collectives.
@@ -39,9 +42,20 @@ STATE = {"duffing": 2, "pendulum": 2, "lorenz": 3}
 # HBM bytes per dispatch of the Duffing 100k float32 kernel, from rocprofv3
 # FETCH_SIZE + WRITE_SIZE scaled by a plain-copy calibration
 # (profiles/dopri5_lane_duffing_100k_f32_pmc.md)
-TRAFFIC_BYTES = {("duffing", "f32", 100_000): 9.50e6}
+TRAFFIC_BYTES = {("duffing", "f32", 100_000, False): 9.50e6}
+# the two-phase launch adds the round trip of each suspended trajectory's
+# state record and index through HBM, per suspended trajectory (same
+# counters, phase budget 3,239: profiles/dopri5_phase_duffing_100k_f32.md)
+TRAFFIC_PER_SUSPENDED = {("duffing", "f32", 100_000): 390.0}
 
 DUFFING_TEMPLATES = [[-0.21, 0.02], [1.05, 0.77], [-0.67, 0.02], [-0.46, 0.30], [-0.43, 0.12]]
+
+
+def _label_counts(labels: torch.Tensor, k: int) -> torch.Tensor:
+    """Counts per label 0..k-1 on the device without a host round trip
+    (torch.bincount reads the maximum label back to size its output)."""
+    return torch.zeros(k, dtype=torch.int64, device=labels.device).scatter_add_(
+        0, labels.long(), torch.ones_like(labels, dtype=torch.int64))
 
 
 def parse():
@@ -53,6 +67,11 @@ def parse():
     ap.add_argument("--dtype", choices=["f32", "f64"], default="f32")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--max-lanes", type=int, default=0)
+    ap.add_argument("--phase-iters", type=int, default=0,
+                    help="duffing: two-phase launch with this step budget instead of the tuner")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="duffing: skip the launch-shape tuner (every trajectory on its own lane, "
+                         "one launch)")
     ap.add_argument("--cpu-sample", type=int, default=100_000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--keep-traj", action="store_true",
@@ -532,8 +551,8 @@ def lorenz_workload(args):
     """Lorenz with escape: 200,000 seeded ICs (x in [-10, 10], y in [-20, 20],
     z = 0), sigma 0.12, r 0, b -0.6, t in [0, 1000], float32, rtol 1e-8,
     atol 1e-6, per-lane stop at |y|inf >= 200 with Inf fill, 1,000 saves on
-    [900, 1000]; the resident-lane cap and refill quorum chosen by the
-    autotuner; labels: unbounded by status, the two attractors by 1-NN on
+    [900, 1000]; the launch shape (resident-lane cap and refill quorum, or
+    the two-phase launch and its step budget) chosen by the autotuner; labels: unbounded by status, the two attractors by 1-NN on
     mean x (setup_lorenz_system.py:54-70)."""
     from pybasin_amd import autotune
     from pybasin_amd.cluster import knn
@@ -553,9 +572,9 @@ def lorenz_workload(args):
                              inf_fill=True)
     os.environ.setdefault("PYBASIN_AMD_TUNE_CACHE", os.path.join("/tmp", "pybasin_amd_tune.json"))
     t0 = time.perf_counter()
-    cap, quorum = autotune.tune_lane_launch(solver, ode, y0_all, use_cache=False)
+    shape = autotune.tune_launch_shape(solver, ode, y0_all, use_cache=False)
     t_tune = time.perf_counter() - t0
-    solver.max_lanes, solver.refill_quorum = cap, quorum
+    autotune.apply_launch_shape(solver, shape)
     save_ts = torch.linspace(900.0, 1000.0, 1000, dtype=torch.float32, device="cuda")
     stream = torch.cuda.current_stream()
     kev = []
@@ -571,7 +590,7 @@ def lorenz_workload(args):
         mean_x = res.stats[:, 0, 0:1].contiguous()
         lab = knn(mean_x[:n], mean_x[n:], 1)[1][:, 0]
         lab = torch.where(res.status[:n] == 4, torch.full_like(lab, 2), lab)  # BH_LANE_DIVERGED
-        return res, torch.bincount(lab, minlength=3)
+        return res, _label_counts(lab, 3)
 
     for _ in range(args.warmup):
         step()
@@ -592,7 +611,8 @@ def lorenz_workload(args):
         "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic: seeded-uniform ICs (torch.manual_seed, CPU float32 draw)",
         "config": {"workload": "lorenz_escape_hip_dopri5", "n_ics": n, "stop": "|y|inf >= 200, Inf fill",
-                   "lane_cap": cap, "refill_quorum": quorum,
+                   "lane_cap": shape["max_lanes"], "refill_quorum": shape["refill_quorum"],
+                   "phase_iters": shape["phase_iters"],
                    "tuner_timings_ms": autotune.last_timings,
                    "tune_s": round(t_tune, 2),
                    "basin_stability": {k: round(float(v) / n, 5) for k, v in
@@ -602,7 +622,9 @@ def lorenz_workload(args):
         "roofline": {"bound": "valu", "achieved": round(flops / (kernel_ms * 1e-3) / 1e12, 3),
                      "peak": 157.3, "unit": "TFLOP/s",
                      "frac": round(flops / (kernel_ms * 1e-3) / PEAK_FP32_VALU, 4), "traffic": None,
-                     "kernel": "bh::dopri5_lane_kernel<Lorenz,float>", "kernel_ms": round(kernel_ms, 3)},
+                     "kernel": ("bh::dopri5_phase_kernel<Lorenz,float> phases 1 + 2" if solver.phase_iters
+                                else "bh::dopri5_lane_kernel<Lorenz,float>"),
+                     "kernel_ms": round(kernel_ms, 3)},
     }
     print(json.dumps(out), flush=True)
 
@@ -653,6 +675,18 @@ def main():
     ode = DuffingODE({"delta": 0.08, "k3": 1.0, "A": 0.2})
     solver = HipDopri5Solver(t_span=(0, 1000), t_steps=1000, t_eval=(900.0, 1000.0), device="cuda",
                              rtol=1e-8, atol=1e-6, cache_dir=None, max_lanes=args.max_lanes)
+    launch = {"max_lanes": args.max_lanes, "refill_quorum": 0, "phase_iters": args.phase_iters}
+    solver.phase_iters = args.phase_iters
+    if not args.no_tune and args.max_lanes == 0 and args.phase_iters == 0 and not args.keep_traj:
+        # one-time launch-shape choice on this problem, outside the timed steps
+        from pybasin_amd import autotune
+
+        os.environ.setdefault("PYBASIN_AMD_TUNE_CACHE", os.path.join("/tmp", "pybasin_amd_tune.json"))
+        t0 = time.perf_counter()
+        launch = autotune.tune_launch_shape(solver, ode, y0_all, use_cache=False)
+        launch = dict(launch, tune_s=round(time.perf_counter() - t0, 2),
+                      tuner_timings_ms=dict(autotune.last_timings))
+        autotune.apply_launch_shape(solver, launch)
     save_ts = torch.linspace(900.0, 1000.0, 1000, dtype=dtype, device=dev)
     stream = torch.cuda.current_stream(dev)
     kev = []
@@ -670,7 +704,7 @@ def main():
         # on the templates by the HIP k-NN kernel
         feats = torch.stack([res.stats[:, 0, 3], res.stats[:, 0, 1]], dim=1)
         labels = knn(feats[:n].contiguous(), feats[n:].contiguous(), 1)[1][:, 0]
-        counts = torch.bincount(labels, minlength=len(DUFFING_TEMPLATES))
+        counts = _label_counts(labels, len(DUFFING_TEMPLATES))
         return res, counts
 
     for _ in range(args.warmup):
@@ -691,7 +725,16 @@ def main():
     flops = float(nsteps.sum()) * (6 * F_RHS["duffing"] + 63 * S + 16) + n_traj * 1000 * (36 + 9 + 6) * S
     achieved = flops / (kernel_ms * 1e-3)
     peak = PEAK_FP32_VALU if dtype == torch.float32 else PEAK_FP64_VALU
+    tname = "float" if dtype == torch.float32 else "double"
     bs = (counts.double() / n).cpu().tolist()
+    traffic = None
+    base_traffic = TRAFFIC_BYTES.get(("duffing", args.dtype, n, False))
+    if not args.keep_traj and base_traffic is not None:
+        if not solver.phase_iters:
+            traffic = base_traffic
+        elif TRAFFIC_PER_SUSPENDED.get(("duffing", args.dtype, n)) is not None:
+            traffic = base_traffic + TRAFFIC_PER_SUSPENDED[("duffing", args.dtype, n)] * float(
+                (nsteps > solver.phase_iters).sum())
 
     out = {
         "metric": "classified initial conditions/sec at <=0.5% basin-stability error vs plain",
@@ -716,13 +759,16 @@ def main():
             "max_steps_per_ic": int(nsteps.max()),
             "lanes_failed": int((status != 0).sum()),
             "bs_error_vs_plain": 0.0,
+            "launch": dict(launch, suspended=int((nsteps > solver.phase_iters).sum())
+                           if solver.phase_iters else 0),
         },
         "roofline": {
             "bound": "valu", "achieved": round(achieved / 1e12, 3),
             "peak": round(peak / 1e12, 1), "unit": "TFLOP/s",
             "frac": round(achieved / peak, 4),
-            "traffic": None if args.keep_traj else TRAFFIC_BYTES.get(("duffing", args.dtype, n)),
-            "kernel": f"bh::dopri5_lane_kernel<Duffing,{'float' if dtype == torch.float32 else 'double'}>",
+            "traffic": traffic,
+            "kernel": (f"bh::dopri5_phase_kernel<Duffing,{tname}> phases 1 + 2" if solver.phase_iters
+                       else f"bh::dopri5_lane_kernel<Duffing,{tname}>"),
             "kernel_ms": round(kernel_ms, 3),
         },
     }

@@ -126,22 +126,57 @@ int32_t bh_solve_batch_peaks(int32_t system_id, int32_t dtype, const void *y0s, 
                              int64_t workspace_bytes, int32_t max_lanes, void *stream);
 
 /*
- * bh_autotune_lane -- the launch-shape tuner of bh_solve_batch as one call:
- * solves the caller's problem once per (resident-lane cap, refill quorum)
- * candidate -- one untimed launch, then `reps` launches timed with HIP events
- * on `stream` -- and returns the fastest pair.  The results of bh_solve_batch
- * do not depend on the launch shape, so on return the outputs hold the
- * problem's solution.  Blocks until the timed launches end.  The reference
- * has no counterpart (its solvers have no launch shape to choose); the
- * arguments up to workspace_bytes are those of bh_solve_batch, except that
- * the refill-quorum bits of flags are replaced by each candidate's.
+ * bh_solve_batch_phased -- bh_solve_batch as two launches with compaction in
+ * between, for batches whose trajectories end at very different times:
+ * phase 1 gives every trajectory its own lane for at most phase_iters
+ * attempted steps; trajectories still integrating then store their whole
+ * lane state in the workspace, packed densely, and phase 2 resumes them on
+ * consecutive lanes, so the long trajectories run in full waves instead of
+ * a few live lanes per wave.  Results equal bh_solve_batch bit for bit (a
+ * trajectory's arithmetic does not depend on the lane or launch that runs
+ * it).  Arguments as bh_solve_batch, with phase_iters (>= 1) in place of
+ * max_lanes; no orbit peaks; asynchronous (no host synchronisation).  Same
+ * replaced interface as bh_solve_batch.
+ *
+ *   workspace   >= bh_solve_batch_phased_workspace_bytes(system_id, dtype,
+ *               dim, n) device bytes (counter, and index + state record of
+ *               every trajectory that may be suspended)
+ */
+int64_t bh_solve_batch_phased_workspace_bytes(int32_t system_id, int32_t dtype, int32_t dim,
+                                              int64_t n);
+int32_t bh_solve_batch_phased(int32_t system_id, int32_t dtype, const void *y0s, int64_t n,
+                              int32_t dim, const double *params, int32_t n_params,
+                              int64_t params_stride, double t0, const double *save_ts,
+                              int32_t n_save, double rtol, double atol, int64_t max_steps,
+                              uint32_t flags, double stop_threshold, int32_t steady_idx,
+                              void *out_traj, double *out_feats, void *out_final,
+                              int32_t *out_status, int64_t *out_nsteps, int32_t phase_iters,
+                              void *workspace, int64_t workspace_bytes, void *stream);
+
+/*
+ * bh_autotune_lane -- the launch-shape tuner of the per-lane solver as one
+ * call: solves the caller's problem once per candidate -- one untimed
+ * launch, then `reps` launches timed with HIP events on `stream` -- and
+ * returns the fastest.  Candidates: every (resident-lane cap, refill quorum)
+ * pair of bh_solve_batch, and every phase budget of bh_solve_batch_phased.
+ * The results do not depend on the launch shape, so on return the outputs
+ * hold the problem's solution.  Blocks until the timed launches end.  The
+ * reference has no counterpart (its solvers have no launch shape to
+ * choose); the arguments up to workspace_bytes are those of bh_solve_batch,
+ * except that the refill-quorum bits of flags are replaced by each
+ * candidate's (with phase candidates the workspace must hold
+ * bh_solve_batch_phased_workspace_bytes).
  *
  *   caps        (n_caps) host array of max_lanes values (0 = fill the device)
  *   quorums     (n_quorums) host array of refill quorums, each 1..64
+ *   phases      (n_phases) host array of phase_iters values (>= 1), or NULL
+ *               with n_phases = 0
  *   reps        timed launches per candidate (>= 1); a candidate's time is
  *               the fastest of them
- *   best_cap, best_quorum   out: the fastest candidate
- *   timings_ms  (n_caps * n_quorums) host floats [cap][quorum], or NULL
+ *   best_cap, best_quorum, best_phase   out: the fastest candidate (a
+ *               phase winner has cap and quorum 0; a refill winner phase 0)
+ *   timings_ms  (n_caps * n_quorums + n_phases) host floats: the refill
+ *               pairs [cap][quorum], then the phases; or NULL
  */
 int32_t bh_autotune_lane(int32_t system_id, int32_t dtype, const void *y0s, int64_t n,
                          int32_t dim, const double *params, int32_t n_params,
@@ -151,7 +186,8 @@ int32_t bh_autotune_lane(int32_t system_id, int32_t dtype, const void *y0s, int6
                          double *out_feats, void *out_final, int32_t *out_status,
                          int64_t *out_nsteps, void *workspace, int64_t workspace_bytes,
                          const int32_t *caps, int32_t n_caps, const int32_t *quorums,
-                         int32_t n_quorums, int32_t reps, int32_t *best_cap, int32_t *best_quorum,
+                         int32_t n_quorums, const int32_t *phases, int32_t n_phases, int32_t reps,
+                         int32_t *best_cap, int32_t *best_quorum, int32_t *best_phase,
                          float *timings_ms, void *stream);
 
 /*

@@ -3,7 +3,9 @@
 The tunables are the resident-lane cap of the per-lane kernel (how many
 waves per SIMD stay resident; finished lanes refill from the counter), its
 refill quorum (how many finished lanes of a wave wait before they refill
-together), and the resident-workgroup cap of the Rossler tile kernel.  The per-lane tuner is
+together), the alternative two-phase launch with its step budget (long
+trajectories packed into dense waves after the budget), and the
+resident-workgroup cap of the Rossler tile kernel.  The per-lane tuner is
 the library's bh_autotune_lane, which times each candidate (HIP events on
 the launch stream) on a probe of the caller's own problem (same system, dtype,
 parameters and ICs; by default the whole run with its save grid, since
@@ -70,15 +72,19 @@ def _time(fn, reps: int = 2) -> float:
     return best
 
 
-def tune_lane_launch(solver, ode_system, y0: torch.Tensor, params: torch.Tensor | None = None,
-                     probe_span: float | None = None, caps=(0, 3, 2, 1), quorums=(64, 1),
-                     use_cache: bool = True) -> tuple[int, int]:
-    """Best (resident-lane cap, refill quorum) pair for this problem
-    (bh_solve_batch max_lanes, 0 = occupancy limit; refill quorum as in
-    BH_REFILL_QUORUM).  caps other than 0 are waves per SIMD.  Without
-    probe_span each candidate runs the solver's own span and save grid
-    (without trajectory output); with it, a shorter run over [t0, t0 +
-    probe_span] with two saves."""
+def tune_launch_shape(solver, ode_system, y0: torch.Tensor, params: torch.Tensor | None = None,
+                      probe_span: float | None = None, caps=(0, 3, 2, 1), quorums=(64, 1),
+                      phase_quantiles=(0.25, 0.35, 0.45, 0.55, 0.65, 0.75, 0.8, 0.85),
+                      use_cache: bool = True) -> dict:
+    """Best launch shape for this problem: {"max_lanes", "refill_quorum",
+    "phase_iters"} (set them on the solver; phase_iters > 0 selects the
+    two-phase launch, else the refilling launch with max_lanes and
+    refill_quorum).  Refill candidates: caps (waves per SIMD, 0 = occupancy
+    limit) x quorums.  Phase candidates: the step budgets at which the given
+    fractions of the trajectories have ended, read from the attempted-step
+    counts of the first solve.  Without probe_span each candidate runs the
+    solver's own span and save grid (without trajectory output); with it, a
+    shorter run over [t0, t0 + probe_span] with two saves."""
     from .solvers.hip_dopri5 import functor_id
 
     sid = functor_id(ode_system)
@@ -86,11 +92,13 @@ def tune_lane_launch(solver, ode_system, y0: torch.Tensor, params: torch.Tensor 
         raise ValueError("no device functor for this ODE system")
     dev = solver.device
     n = y0.shape[0] * (1 if params is None else params.shape[0])
-    key = _key(dev, sid, y0.dtype, n) + f"|q{','.join(str(int(q)) for q in quorums)}"
+    key = (_key(dev, sid, y0.dtype, n) + f"|q{','.join(str(int(q)) for q in quorums)}"
+           + f"|c{','.join(str(c) for c in caps)}|p{','.join(str(q) for q in phase_quantiles)}")
     with _lock:
         cache = _load() if use_cache else {}
         if key in cache:
-            return int(cache[key]["max_lanes"]), int(cache[key]["refill_quorum"])
+            hit = cache[key]
+            return {k: int(hit[k]) for k in ("max_lanes", "refill_quorum", "phase_iters")}
     props = torch.cuda.get_device_properties(dev)
     wave_set = props.multi_processor_count * 4 * 64
     lanes = [0 if c == 0 else int(c * wave_set) for c in caps]
@@ -106,17 +114,44 @@ def tune_lane_launch(solver, ode_system, y0: torch.Tensor, params: torch.Tensor 
     plan = probe.plan_fused(ode_system, y0e, save_ts, pe)
     # bh_autotune_lane: one untimed and two timed launches per candidate
     # (quorum 0 is the library default, 64)
-    _, _, results, _ = probe.tune_fused(plan, lanes, [int(q) or 64 for q in quorums], reps=2)
+    _, _, _, results, res = probe.tune_fused(plan, lanes, [int(q) or 64 for q in quorums], reps=2)
+    if phase_quantiles and res.nsteps.numel():
+        steps = res.nsteps.double()
+        top = int(steps.max())
+        qs = torch.quantile(steps, torch.tensor(list(phase_quantiles), dtype=torch.float64,
+                                                device=steps.device))
+        budgets = sorted({int(v) + 1 for v in qs.cpu().tolist() if 0 < int(v) + 1 < top})
+        if budgets:
+            results.update(probe.tune_fused(plan, [], [], budgets, reps=2)[3])
     best = min(results, key=results.get)
+    shape = ({"max_lanes": 0, "refill_quorum": 0, "phase_iters": best[1]} if best[0] == "phased"
+             else {"max_lanes": best[1], "refill_quorum": best[2], "phase_iters": 0})
+    names = {k: (f"phase{k[1]}" if k[0] == "phased" else f"{k[1]}/q{k[2]}") for k in results}
     last_timings.clear()
-    last_timings.update({f"{k[0]}/q{k[1]}": round(v, 3) for k, v in results.items()})
+    last_timings.update({names[k]: round(v, 3) for k, v in results.items()})
     with _lock:
         cache = _load() if use_cache else {}
-        cache[key] = {"max_lanes": best[0], "refill_quorum": best[1],
-                      "timings_ms": {f"{k[0]}/q{k[1]}": v for k, v in results.items()}}
+        cache[key] = dict(shape, timings_ms={names[k]: v for k, v in results.items()})
         if use_cache:
             _store(cache)
-    return best
+    return shape
+
+
+def apply_launch_shape(solver, shape: dict) -> None:
+    """Set a tuned launch shape (tune_launch_shape) on a HipDopri5Solver."""
+    solver.max_lanes = int(shape["max_lanes"])
+    solver.refill_quorum = int(shape["refill_quorum"])
+    solver.phase_iters = int(shape["phase_iters"])
+
+
+def tune_lane_launch(solver, ode_system, y0: torch.Tensor, params: torch.Tensor | None = None,
+                     probe_span: float | None = None, caps=(0, 3, 2, 1), quorums=(64, 1),
+                     use_cache: bool = True) -> tuple[int, int]:
+    """Best (resident-lane cap, refill quorum) pair of the refilling launch
+    (tune_launch_shape without phase candidates)."""
+    shape = tune_launch_shape(solver, ode_system, y0, params, probe_span, caps, quorums, (),
+                              use_cache)
+    return shape["max_lanes"], shape["refill_quorum"]
 
 
 def tune_lane_cap(solver, ode_system, y0: torch.Tensor, params: torch.Tensor | None = None,

@@ -75,6 +75,11 @@ struct LaneArgs {
     void *peak_vals;       // (peak_cap, n, S), values of strict local maxima in time order
     int32_t *peak_counts;  // (n, S), number of maxima (may exceed peak_cap)
     int32_t peak_cap;
+    // two-phase launches: step budget of the first phase and the compact
+    // buffer of suspended lane states with their trajectory indices
+    int32_t phase_iters;
+    void *state;
+    int64_t *state_ic;
 };
 
 // Dormand-Prince 5(4) coefficients (Dormand & Prince 1980; Shampine's error
@@ -549,6 +554,47 @@ __global__ void __launch_bounds__(256) dopri5_lane_kernel(LaneArgs a) {
             }
         } while (__any(live));
     }
+}
+
+// Two-phase launch.  Phase 1 gives every trajectory its own lane for at most
+// phase_iters attempted steps; a lane still integrating then appends its
+// whole state (the Lane record) to a compact buffer at a slot taken from the
+// counter, and leaves.  Phase 2 resumes slot r on lane r, so the long
+// trajectories continue in dense waves instead of a few live lanes per
+// wave.  A trajectory's arithmetic does not depend on the lane, the slot or
+// the launch that runs it, so the results equal a single-phase launch.
+template <class Sys, class T, int kPhase>
+__global__ void __launch_bounds__(256) dopri5_phase_kernel(LaneArgs a) {
+    using LaneT = Lane<Sys, T, false>;
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    LaneT *state = reinterpret_cast<LaneT *>(a.state);
+    LaneT L;
+    int64_t ic;
+    if constexpr (kPhase == 1) {
+        ic = r;
+        if (ic >= a.n) return;
+        lane_init<Sys, T>(a, L, ic);
+        for (int32_t it = 0; !L.done && it < a.phase_iters; it++) lane_step<Sys, T>(a, L, ic);
+        if (!L.done) {
+            const unsigned long long slot = atomicAdd(a.counter, 1ull);
+            state[slot] = L;
+            a.state_ic[slot] = ic;
+            return;
+        }
+    } else {
+        // the counter holds the number of suspended states (kernel boundary
+        // after phase 1)
+        if (r >= (int64_t)*a.counter) return;
+        L = state[r];
+        ic = a.state_ic[r];
+        while (!L.done) lane_step<Sys, T>(a, L, ic);
+    }
+    lane_finish<Sys, T>(a, L, ic);
+}
+
+template <class Sys, class T>
+constexpr int64_t lane_state_bytes() {
+    return (int64_t)sizeof(Lane<Sys, T, false>);
 }
 
 __global__ void set_counter_kernel(unsigned long long *c, unsigned long long v) {

@@ -1,6 +1,6 @@
 // lane_api.hip -- C-ABI entry points of the per-lane Dopri5 solver
-// (bh_solve_batch, bh_solve_batch_peaks, bh_autotune_lane; declared in
-// include/basinhip.h).
+// (bh_solve_batch, bh_solve_batch_peaks, bh_solve_batch_phased,
+// bh_autotune_lane; declared in include/basinhip.h).
 // Its own translation unit so that it can be built with the instruction
 // scheduler that favours independent instructions: the step loop of a
 // wave alone on its SIMD is bound by the latency of its dependent
@@ -42,10 +42,34 @@ int32_t launch_lane(const bh::LaneArgs &args, int32_t max_lanes, hipStream_t str
     return 0;
 }
 
+// two-phase launch (args.phase_iters > 0): counter reset, phase 1 over every
+// trajectory, phase 2 over at most n suspended states (lanes past the
+// counter leave at once), no host synchronisation
+template <class Sys, class T>
+int32_t launch_phased(const bh::LaneArgs &args, hipStream_t stream) {
+    constexpr int block = 256;
+    const int64_t blocks = (args.n + block - 1) / block;
+    if (blocks > 2147483647LL) return fail(-5, "n too large for a two-phase launch");
+    hipLaunchKernelGGL(bh::set_counter_kernel, dim3(1), dim3(64), 0, stream, args.counter, 0ull);
+    BH_HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL((bh::dopri5_phase_kernel<Sys, T, 1>), dim3((unsigned)blocks), dim3(block), 0,
+                       stream, args);
+    BH_HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL((bh::dopri5_phase_kernel<Sys, T, 2>), dim3((unsigned)blocks), dim3(block), 0,
+                       stream, args);
+    BH_HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
 template <class Sys>
 int32_t dispatch_dtype(int32_t dtype, const bh::LaneArgs &args, int32_t max_lanes,
                        hipStream_t s) {
     const bool peaks = args.peak_counts != nullptr;
+    if (args.phase_iters > 0) {
+        if (dtype == BH_F32) return launch_phased<Sys, float>(args, s);
+        if (dtype == BH_F64) return launch_phased<Sys, double>(args, s);
+        return fail(-5, "dtype must be BH_F32 or BH_F64, got %d", dtype);
+    }
     if (dtype == BH_F32)
         return peaks ? launch_lane<Sys, float, true>(args, max_lanes, s)
                      : launch_lane<Sys, float, false>(args, max_lanes, s);
@@ -58,6 +82,29 @@ int32_t dispatch_dtype(int32_t dtype, const bh::LaneArgs &args, int32_t max_lane
 struct SysInfo {
     int s, np;
 };
+
+template <class Sys>
+int64_t state_bytes_of(int32_t dtype) {
+    return dtype == BH_F64 ? bh::lane_state_bytes<Sys, double>() : bh::lane_state_bytes<Sys, float>();
+}
+
+// workspace of a two-phase launch: counter, trajectory index and state record
+// per possibly suspended lane (-1 for an unknown system)
+int64_t phased_workspace(int32_t sys, int32_t dtype, int32_t dim, int64_t n) {
+    int64_t sb;
+    switch (sys) {
+    case BH_SYS_DECAY:
+        sb = dim == 1 ? state_bytes_of<bh::Decay<1>>(dtype) : state_bytes_of<bh::Decay<2>>(dtype);
+        break;
+    case BH_SYS_LINEAR2: sb = state_bytes_of<bh::Linear2>(dtype); break;
+    case BH_SYS_PENDULUM: sb = state_bytes_of<bh::Pendulum>(dtype); break;
+    case BH_SYS_DUFFING: sb = state_bytes_of<bh::Duffing>(dtype); break;
+    case BH_SYS_LORENZ: sb = state_bytes_of<bh::Lorenz>(dtype); break;
+    default: return -1;
+    }
+    const int64_t ic_bytes = (n * 8 + 255) / 256 * 256;
+    return BH_WORKSPACE_BYTES + ic_bytes + n * sb;
+}
 
 int32_t check_system(int32_t sys, int32_t dim, int32_t n_params) {
     SysInfo info;
@@ -85,7 +132,8 @@ int32_t solve_batch_impl(int32_t system_id, int32_t dtype, const void *y0s, int6
                          int32_t steady_idx, void *out_traj, double *out_feats, void *out_final,
                          int32_t *out_status, int64_t *out_nsteps, int32_t peak_cap,
                          void *out_peak_vals, int32_t *out_peak_counts, void *workspace,
-                         int64_t workspace_bytes, int32_t max_lanes, void *stream) {
+                         int64_t workspace_bytes, int32_t max_lanes, void *stream,
+                         int32_t phase_iters = 0) {
     g_err[0] = 0;
     int32_t rc = check_system(system_id, dim, n_params);
     if (rc) return rc;
@@ -103,6 +151,13 @@ int32_t solve_batch_impl(int32_t system_id, int32_t dtype, const void *y0s, int6
     if ((flags & BH_FLAG_STOP) && !(stop_threshold > 0.0))
         return fail(-5, "BH_FLAG_STOP needs stop_threshold > 0");
     if (max_lanes < 0) return fail(-5, "max_lanes must be >= 0");
+    if (phase_iters < 0) return fail(-5, "phase_iters must be >= 0");
+    if (phase_iters > 0) {
+        if (out_peak_counts) return fail(-5, "orbit peaks are not produced by two-phase launches");
+        const int64_t need = phased_workspace(system_id, dtype, dim, n);
+        if (workspace_bytes < need)
+            return fail(-5, "a two-phase launch needs %lld workspace bytes", (long long)need);
+    }
 
     bh::LaneArgs a;
     a.y0 = y0s;
@@ -127,6 +182,14 @@ int32_t solve_batch_impl(int32_t system_id, int32_t dtype, const void *y0s, int6
     a.peak_vals = out_peak_vals;
     a.peak_counts = out_peak_counts;
     a.peak_cap = peak_cap;
+    a.phase_iters = phase_iters;
+    a.state_ic = nullptr;
+    a.state = nullptr;
+    if (phase_iters > 0) {
+        char *w = reinterpret_cast<char *>(workspace);
+        a.state_ic = reinterpret_cast<int64_t *>(w + BH_WORKSPACE_BYTES);
+        a.state = w + BH_WORKSPACE_BYTES + (n * 8 + 255) / 256 * 256;
+    }
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
 
     switch (system_id) {
@@ -178,6 +241,36 @@ int32_t bh_solve_batch_peaks(int32_t system_id, int32_t dtype, const void *y0s, 
                             max_lanes, stream);
 }
 
+int64_t bh_solve_batch_phased_workspace_bytes(int32_t system_id, int32_t dtype, int32_t dim,
+                                              int64_t n) {
+    g_err[0] = 0;
+    if (dtype != BH_F32 && dtype != BH_F64) return fail(-5, "dtype must be BH_F32 or BH_F64");
+    if (n < 0) return fail(-5, "n must be >= 0");
+    if (system_id == BH_SYS_DECAY && dim != 1 && dim != 2)
+        return fail(-5, "decay system supports dim 1 or 2, got %d", dim);
+    const int64_t bytes = phased_workspace(system_id, dtype, dim, n);
+    if (bytes < 0) return fail(-1, "system id %d has no per-lane device functor", system_id);
+    return bytes;
+}
+
+int32_t bh_solve_batch_phased(int32_t system_id, int32_t dtype, const void *y0s, int64_t n,
+                              int32_t dim, const double *params, int32_t n_params,
+                              int64_t params_stride, double t0, const double *save_ts,
+                              int32_t n_save, double rtol, double atol, int64_t max_steps,
+                              uint32_t flags, double stop_threshold, int32_t steady_idx,
+                              void *out_traj, double *out_feats, void *out_final,
+                              int32_t *out_status, int64_t *out_nsteps, int32_t phase_iters,
+                              void *workspace, int64_t workspace_bytes, void *stream) {
+    if (phase_iters < 1) {
+        g_err[0] = 0;
+        return fail(-5, "phase_iters must be >= 1, got %d", phase_iters);
+    }
+    return solve_batch_impl(system_id, dtype, y0s, n, dim, params, n_params, params_stride, t0,
+                            save_ts, n_save, rtol, atol, max_steps, flags, stop_threshold,
+                            steady_idx, out_traj, out_feats, out_final, out_status, out_nsteps, 0,
+                            nullptr, nullptr, workspace, workspace_bytes, 0, stream, phase_iters);
+}
+
 int32_t bh_autotune_lane(int32_t system_id, int32_t dtype, const void *y0s, int64_t n,
                          int32_t dim, const double *params, int32_t n_params,
                          int64_t params_stride, double t0, const double *save_ts, int32_t n_save,
@@ -186,67 +279,90 @@ int32_t bh_autotune_lane(int32_t system_id, int32_t dtype, const void *y0s, int6
                          double *out_feats, void *out_final, int32_t *out_status,
                          int64_t *out_nsteps, void *workspace, int64_t workspace_bytes,
                          const int32_t *caps, int32_t n_caps, const int32_t *quorums,
-                         int32_t n_quorums, int32_t reps, int32_t *best_cap, int32_t *best_quorum,
+                         int32_t n_quorums, const int32_t *phases, int32_t n_phases, int32_t reps,
+                         int32_t *best_cap, int32_t *best_quorum, int32_t *best_phase,
                          float *timings_ms, void *stream) {
     g_err[0] = 0;
-    if (!caps || !quorums || n_caps < 1 || n_quorums < 1)
-        return fail(-5, "at least one cap and one quorum are required");
-    if (!best_cap || !best_quorum) return fail(-5, "best_cap and best_quorum are required");
+    if (n_caps < 0 || n_quorums < 0 || n_phases < 0) return fail(-5, "negative candidate count");
+    if ((n_caps > 0 && !caps) || (n_quorums > 0 && !quorums) || (n_phases > 0 && !phases))
+        return fail(-5, "null candidate array");
+    const int32_t n_refill = n_caps * n_quorums;
+    if (n_refill + n_phases < 1)
+        return fail(-5, "no candidate: give caps and quorums, or phases");
+    if (!best_cap || !best_quorum || !best_phase)
+        return fail(-5, "best_cap, best_quorum and best_phase are required");
     if (reps < 1) return fail(-5, "reps must be >= 1, got %d", reps);
     for (int32_t i = 0; i < n_caps; i++)
         if (caps[i] < 0) return fail(-5, "caps[%d] = %d: caps are max_lanes values >= 0", i, caps[i]);
     for (int32_t j = 0; j < n_quorums; j++)
         if (quorums[j] < 1 || quorums[j] > 64)
             return fail(-5, "quorums[%d] = %d: a refill quorum is 1..64", j, quorums[j]);
-    *best_cap = caps[0];
-    *best_quorum = quorums[0];
+    for (int32_t k = 0; k < n_phases; k++)
+        if (phases[k] < 1) return fail(-5, "phases[%d] = %d: a phase budget is >= 1", k, phases[k]);
+    *best_cap = n_refill ? caps[0] : 0;
+    *best_quorum = n_refill ? quorums[0] : 0;
+    *best_phase = n_refill ? 0 : phases[0];
     if (timings_ms)
-        for (int32_t k = 0; k < n_caps * n_quorums; k++) timings_ms[k] = 0.0f;
+        for (int32_t k = 0; k < n_refill + n_phases; k++) timings_ms[k] = 0.0f;
     if (n == 0) return 0;
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     hipEvent_t ev[2] = {nullptr, nullptr};
-    int32_t rc = 0;
-    float best = INFINITY;
     if (hipEventCreate(&ev[0]) != hipSuccess || hipEventCreate(&ev[1]) != hipSuccess) {
-        rc = fail(-6, "hipEventCreate failed");
+        for (hipEvent_t e : ev)
+            if (e) (void)hipEventDestroy(e);
+        return fail(-6, "hipEventCreate failed");
     }
     const uint32_t base = flags & ~(0x7fu << BH_REFILL_QUORUM_SHIFT);
+    // one untimed launch, then reps timed ones; every launch solves the whole
+    // problem (the outputs do not depend on the launch shape)
+    auto time_one = [&](int32_t cap, uint32_t f, int32_t phase, float *t_min) -> int32_t {
+        *t_min = INFINITY;
+        for (int32_t r = 0; r <= reps; r++) {
+            if (r > 0 && hipEventRecord(ev[0], s) != hipSuccess) return fail(-6, "hipEventRecord failed");
+            const int32_t rc = solve_batch_impl(
+                system_id, dtype, y0s, n, dim, params, n_params, params_stride, t0, save_ts, n_save,
+                rtol, atol, max_steps, f, stop_threshold, steady_idx, out_traj, out_feats, out_final,
+                out_status, out_nsteps, 0, nullptr, nullptr, workspace, workspace_bytes, cap, stream,
+                phase);
+            if (rc) return rc;
+            if (r == 0) continue;
+            float ms = 0.0f;
+            if (hipEventRecord(ev[1], s) != hipSuccess || hipEventSynchronize(ev[1]) != hipSuccess ||
+                hipEventElapsedTime(&ms, ev[0], ev[1]) != hipSuccess)
+                return fail(-6, "timing the launch with HIP events failed");
+            *t_min = fminf(*t_min, ms);
+        }
+        return 0;
+    };
+    int32_t rc = 0;
+    float best = INFINITY;
     for (int32_t i = 0; i < n_caps && rc == 0; i++) {
         for (int32_t j = 0; j < n_quorums && rc == 0; j++) {
-            const uint32_t f = base | BH_REFILL_QUORUM(quorums[j]);
-            float t_min = INFINITY;
-            // one untimed launch, then reps timed ones; every launch solves the
-            // whole problem (the outputs do not depend on the launch shape)
-            for (int32_t r = 0; r <= reps && rc == 0; r++) {
-                if (r > 0 && hipEventRecord(ev[0], s) != hipSuccess) {
-                    rc = fail(-6, "hipEventRecord failed");
-                    break;
-                }
-                rc = solve_batch_impl(system_id, dtype, y0s, n, dim, params, n_params,
-                                      params_stride, t0, save_ts, n_save, rtol, atol, max_steps, f,
-                                      stop_threshold, steady_idx, out_traj, out_feats, out_final,
-                                      out_status, out_nsteps, 0, nullptr, nullptr, workspace,
-                                      workspace_bytes, caps[i], stream);
-                if (rc || r == 0) continue;
-                float ms = 0.0f;
-                if (hipEventRecord(ev[1], s) != hipSuccess || hipEventSynchronize(ev[1]) != hipSuccess ||
-                    hipEventElapsedTime(&ms, ev[0], ev[1]) != hipSuccess) {
-                    rc = fail(-6, "timing the launch with HIP events failed");
-                    break;
-                }
-                t_min = fminf(t_min, ms);
-            }
+            float t = 0.0f;
+            rc = time_one(caps[i], base | BH_REFILL_QUORUM(quorums[j]), 0, &t);
             if (rc) break;
-            if (timings_ms) timings_ms[i * n_quorums + j] = t_min;
-            if (t_min < best) {
-                best = t_min;
+            if (timings_ms) timings_ms[i * n_quorums + j] = t;
+            if (t < best) {
+                best = t;
                 *best_cap = caps[i];
                 *best_quorum = quorums[j];
+                *best_phase = 0;
             }
         }
     }
-    for (hipEvent_t e : ev)
-        if (e) (void)hipEventDestroy(e);
+    for (int32_t k = 0; k < n_phases && rc == 0; k++) {
+        float t = 0.0f;
+        rc = time_one(0, flags, phases[k], &t);
+        if (rc) break;
+        if (timings_ms) timings_ms[n_refill + k] = t;
+        if (t < best) {
+            best = t;
+            *best_cap = 0;
+            *best_quorum = 0;
+            *best_phase = phases[k];
+        }
+    }
+    for (hipEvent_t e : ev) (void)hipEventDestroy(e);
     return rc;
 }
 

@@ -22,6 +22,7 @@ during the launch.
 from __future__ import annotations
 
 import logging
+import threading
 import weakref
 from dataclasses import dataclass
 from typing import Any
@@ -71,6 +72,24 @@ def device_params(ode_system, params: torch.Tensor | None) -> tuple[torch.Tensor
         return torch.tensor([vals], dtype=torch.float64), 0
     p = params.to(torch.float64).contiguous()
     return p, p.shape[1]
+
+
+# Device copies of shared parameter vectors, by (device, values): a fresh
+# host-to-device copy from pageable memory would synchronise the stream on
+# every launch.  The cached tensors are only read by the kernels.
+_PARAM_CACHE: dict = {}
+_PARAM_CACHE_LOCK = threading.Lock()
+
+
+def _shared_params_on(p: torch.Tensor, dev: torch.device) -> torch.Tensor:
+    key = (str(dev), tuple(p.flatten().tolist()))
+    with _PARAM_CACHE_LOCK:
+        hit = _PARAM_CACHE.get(key)
+        if hit is None:
+            if len(_PARAM_CACHE) >= 256:
+                _PARAM_CACHE.clear()
+            hit = _PARAM_CACHE[key] = p.to(dev).contiguous()
+    return hit
 
 
 @dataclass
@@ -139,6 +158,7 @@ class HipDopri5Solver(Solver):
         inf_fill: bool = False,
         max_lanes: int = 0,
         refill_quorum: int = 0,
+        phase_iters: int = 0,
     ):
         super().__init__(t_span, t_steps, device, rtol, atol, cache_dir, t_eval)
         self.max_steps = int(max_steps)
@@ -148,6 +168,9 @@ class HipDopri5Solver(Solver):
         if not 0 <= int(refill_quorum) <= 64:
             raise ValueError(f"refill_quorum must be in [0, 64], got {refill_quorum}")
         self.refill_quorum = int(refill_quorum)
+        if int(phase_iters) < 0:
+            raise ValueError(f"phase_iters must be >= 0, got {phase_iters}")
+        self.phase_iters = int(phase_iters)
 
     def _get_cache_config(self) -> dict[str, Any]:
         cfg = super()._get_cache_config()
@@ -161,7 +184,8 @@ class HipDopri5Solver(Solver):
             device=device or self._device_str, rtol=self.rtol, atol=self.atol,
             cache_dir=self._cache_dir if cache_dir is UNSET else cache_dir,  # type: ignore[arg-type]
             t_eval=self.t_eval, max_steps=self.max_steps, stop_threshold=self.stop_threshold,
-            inf_fill=self.inf_fill, max_lanes=self.max_lanes, refill_quorum=self.refill_quorum)
+            inf_fill=self.inf_fill, max_lanes=self.max_lanes, refill_quorum=self.refill_quorum,
+            phase_iters=self.phase_iters)
 
     # ------------------------------------------------------------------ #
     def _torch_solver(self) -> TorchDopri5Solver:
@@ -238,7 +262,7 @@ class HipDopri5Solver(Solver):
         y0 = y0.to(dev).contiguous()
         n, S = y0.shape
         p, p_stride = device_params(ode_system, params)
-        p = p.to(dev).contiguous()
+        p = _shared_params_on(p, dev) if p_stride == 0 else p.to(dev).contiguous()
         if p_stride and p.shape[0] != n:
             raise ValueError(f"params rows {p.shape[0]} != trajectories {n}")
         st64 = save_ts.to(device=dev, dtype=torch.float64).contiguous()
@@ -298,7 +322,14 @@ class HipDopri5Solver(Solver):
         s = stream if stream is not None else torch.cuda.current_stream(dev)
         pk_vals = pk_counts = None
         with torch.cuda.device(dev):
-            if peak_cap is None:
+            if peak_cap is None and self.phase_iters > 0:
+                # two launches with the long trajectories packed in between
+                need = _lib.lib().bh_solve_batch_phased_workspace_bytes(common[0], common[1], S, n)
+                _lib.check(-5 if need < 0 else 0, "bh_solve_batch_phased_workspace_bytes")
+                ws = out["ws"] = torch.empty((need,), dtype=torch.uint8, device=dev)
+                rc = _lib.lib().bh_solve_batch_phased(*common, self.phase_iters, ws.data_ptr(),
+                                                      ws.numel(), s.cuda_stream)
+            elif peak_cap is None:
                 rc = _lib.lib().bh_solve_batch(*common, ws.data_ptr(), ws.numel(),
                                                self.max_lanes, s.cuda_stream)
             else:
@@ -313,29 +344,41 @@ class HipDopri5Solver(Solver):
             t.record_stream(s)
         return self._result(plan, out, pk_vals, pk_counts)
 
-    def tune_fused(self, plan: "LanePlan", caps=(0,), quorums=(64,), reps: int = 2,
+    def tune_fused(self, plan: "LanePlan", caps=(0,), quorums=(64,), phases=(), reps: int = 2,
                    stream: torch.cuda.Stream | None = None):
-        """bh_autotune_lane on a prepared plan: solve it once per (max_lanes cap,
-        refill quorum) candidate, timing each launch, and return (best cap,
-        best quorum, {(cap, quorum): ms}, FusedResult of the problem).  Blocks
+        """bh_autotune_lane on a prepared plan: solve it once per candidate --
+        each (max_lanes cap, refill quorum) pair and each phase budget of the
+        two-phase launch -- timing each launch, and return (best cap, best
+        quorum, best phase_iters, {candidate: ms}, FusedResult of the problem);
+        candidates are keyed ("refill", cap, quorum) and ("phased", K).  Blocks
         until the timed launches end; no trajectories are written."""
         import ctypes
 
         dev = self.device
         out, common = self._lane_call(plan, False)
-        ws = out["ws"]
         caps = [int(c) for c in caps]
         quorums = [int(q) for q in quorums]
-        c_caps = (ctypes.c_int32 * len(caps))(*caps)
-        c_q = (ctypes.c_int32 * len(quorums))(*quorums)
-        c_t = (ctypes.c_float * (len(caps) * len(quorums)))()
-        best_cap, best_q = ctypes.c_int32(), ctypes.c_int32()
+        phases = [int(k) for k in phases]
+        ws = out["ws"]
+        if phases:
+            n, S = plan.y0.shape
+            need = _lib.lib().bh_solve_batch_phased_workspace_bytes(common[0], common[1], S, n)
+            _lib.check(-5 if need < 0 else 0, "bh_solve_batch_phased_workspace_bytes")
+            ws = out["ws"] = torch.empty((max(need, _lib.BH_WORKSPACE_BYTES),), dtype=torch.uint8,
+                                         device=dev)
+        arr = lambda v: (ctypes.c_int32 * max(len(v), 1))(*v)  # noqa: E731
+        n_t = len(caps) * len(quorums) + len(phases)
+        c_t = (ctypes.c_float * max(n_t, 1))()
+        best_cap, best_q, best_k = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
         s = stream if stream is not None else torch.cuda.current_stream(dev)
         with torch.cuda.device(dev):
-            rc = _lib.lib().bh_autotune_lane(*common, ws.data_ptr(), ws.numel(), c_caps, len(caps),
-                                             c_q, len(quorums), int(reps), ctypes.byref(best_cap),
-                                             ctypes.byref(best_q), c_t, s.cuda_stream)
+            rc = _lib.lib().bh_autotune_lane(*common, ws.data_ptr(), ws.numel(), arr(caps), len(caps),
+                                             arr(quorums), len(quorums), arr(phases), len(phases),
+                                             int(reps), ctypes.byref(best_cap), ctypes.byref(best_q),
+                                             ctypes.byref(best_k), c_t, s.cuda_stream)
         _lib.check(rc, "bh_autotune_lane")
-        timings = {(c, q): float(c_t[i * len(quorums) + j])
+        timings = {("refill", c, q): float(c_t[i * len(quorums) + j])
                    for i, c in enumerate(caps) for j, q in enumerate(quorums)}
-        return best_cap.value, best_q.value, timings, self._result(plan, out)
+        base = len(caps) * len(quorums)
+        timings.update({("phased", k): float(c_t[base + i]) for i, k in enumerate(phases)})
+        return best_cap.value, best_q.value, best_k.value, timings, self._result(plan, out)

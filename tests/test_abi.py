@@ -51,6 +51,12 @@ def test_version_and_argument_errors_without_gpu():
         _lib.check(rc, "bh_solve_batch")
     # the tuner checks its candidate lists before any HIP call
     rc = L.bh_autotune_lane(4, 0, None, 1, 3, None, 3, 0, 0.0, None, 1, 1e-8, 1e-6, 10, 0, 0.0, 0,
-                            None, None, None, None, None, None, 0, None, 0, None, 0, 1, None, None,
-                            None, None)
-    assert rc == -5 and b"cap" in L.bh_last_error()
+                            None, None, None, None, None, None, 0, None, 0, None, 0, None, 0, 1,
+                            None, None, None, None, None)
+    assert rc == -5 and b"candidate" in L.bh_last_error()
+    # the two-phase entry checks its step budget and the workspace size query its system
+    rc = L.bh_solve_batch_phased(4, 0, None, 1, 3, None, 3, 0, 0.0, None, 1, 1e-8, 1e-6, 10, 0, 0.0,
+                                 0, None, None, None, None, None, 0, None, 0, None)
+    assert rc == -5 and b"phase_iters" in L.bh_last_error()
+    assert L.bh_solve_batch_phased_workspace_bytes(99, 0, 2, 10) < 0
+    assert L.bh_solve_batch_phased_workspace_bytes(4, 0, 3, 1000) > 256 + 8 * 1000

@@ -7,7 +7,7 @@ float32 and float64 runs agree on the labels away from basin boundaries, and
 the basin-stability fractions agree with the reference's own 4,000-sample
 runs (tests/integration/duffing/main_duffing_supervised.json,
 tests/integration/lorenz/main_lorenz.json) within their combined binomial
-errors.  Small-size edge cases (empty batch, one IC, ragged counts with
+errors.  Launch shapes include the two-phase launch.  Small-size edge cases (empty batch, one IC, ragged counts with
 refills) close the file.
 """
 
@@ -77,6 +77,9 @@ def test_duffing_100k_permutation_invariance_and_reference_fractions(cuda_device
     inv = torch.argsort(perm).cuda()
     assert torch.equal(shuffled.stats[:n][inv], base.stats[:n])
     assert torch.equal(shuffled.nsteps[:n][inv], base.nsteps[:n])
+    # the two-phase launch (long trajectories packed after 2,800 steps)
+    phased = _duffing(y0, torch.float32, phase_iters=2800)
+    assert torch.equal(phased.stats, base.stats) and torch.equal(phased.nsteps, base.nsteps)
     assert (base.status == _lib.BH_LANE_OK).all()
     lab = _duffing_labels(base, n)
     bs = (torch.bincount(lab, minlength=5).double() / n).cpu().tolist()
@@ -106,10 +109,10 @@ def test_lorenz_200k_launch_shapes_agree_and_reference_fractions(cuda_device):
     y0 = _uniform(n, [-10.0, -20.0, 0.0], [10.0, 20.0, 0.0], 0)
     batch = torch.cat([y0, torch.tensor(LOR_TEMPLATES)]).cuda()
     out = []
-    for cap, q in ((0, 0), (131072, 1), (65536, 64)):
+    for cap, q, k in ((0, 0, 0), (131072, 1, 0), (65536, 64, 0), (0, 0, 600)):
         solver = HipDopri5Solver(t_span=(0, 1000), t_steps=1000, t_eval=(900.0, 1000.0), device="cuda",
                                  cache_dir=None, stop_threshold=200.0, inf_fill=True, max_lanes=cap,
-                                 refill_quorum=q)
+                                 refill_quorum=q, phase_iters=k)
         save_ts, b = solver._prepare_tensors(batch)
         out.append(solver.run_fused(LorenzODE(LOR), b, save_ts, None, keep_trajectories=False,
                                     steady_idx=0))

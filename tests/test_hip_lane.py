@@ -236,6 +236,14 @@ def test_autotuner_picks_and_caches_a_lane_cap(cuda_device, tmp_path, monkeypatc
     assert cap >= 0 and q in (64, 1)
     assert autotune.tune_lane_launch(solver, ode, y0, probe_span=20.0, caps=(0, 1),
                                      quorums=(64, 1)) == (cap, q)  # cached
+    # with the two-phase launch among the candidates
+    shape = autotune.tune_launch_shape(solver, ode, y0, probe_span=20.0, caps=(0,), quorums=(64,))
+    assert set(shape) == {"max_lanes", "refill_quorum", "phase_iters"}
+    assert any(k.startswith("phase") for k in autotune.last_timings)
+    assert autotune.tune_launch_shape(solver, ode, y0, probe_span=20.0, caps=(0,),
+                                      quorums=(64,)) == shape  # cached
+    autotune.apply_launch_shape(solver, shape)
+    assert solver.phase_iters == shape["phase_iters"]
 
 
 def test_native_autotuner_solves_and_picks_the_fastest(cuda_device):
@@ -249,11 +257,13 @@ def test_native_autotuner_solves_and_picks_the_fastest(cuda_device):
                         cache_dir=None, stop_threshold=200.0, inf_fill=True)
     save_ts, y0p = s._prepare_tensors(y0)
     plan = s.plan_fused(ode, y0p, save_ts, None)
-    caps, quorums = [0, 128, 512], [1, 64]
-    cap, q, timings, res = s.tune_fused(plan, caps, quorums, reps=1)
-    assert set(timings) == {(c, k) for c in caps for k in quorums}
+    caps, quorums, phases = [0, 128, 512], [1, 64], [20, 200]
+    cap, q, k, timings, res = s.tune_fused(plan, caps, quorums, phases, reps=1)
+    assert set(timings) == ({("refill", c, j) for c in caps for j in quorums}
+                            | {("phased", j) for j in phases})
     assert all(t > 0 for t in timings.values())
-    assert timings[(cap, q)] == min(timings.values())
+    won = ("phased", k) if k else ("refill", cap, q)
+    assert timings[won] == min(timings.values())
     base = s.launch_fused(plan, keep_trajectories=False)
     assert torch.equal(res.stats.nan_to_num(7.0), base.stats.nan_to_num(7.0))
     assert torch.equal(res.final.nan_to_num(7.0), base.final.nan_to_num(7.0))
@@ -261,6 +271,34 @@ def test_native_autotuner_solves_and_picks_the_fastest(cuda_device):
     assert (base.status == 4).any() and (base.status == 0).any()
     with pytest.raises(ValueError, match="quorum"):
         s.tune_fused(plan, [0], [65])
+    # only phase candidates
+    cap, q, k, timings, res = s.tune_fused(plan, [], [], [50], reps=1)
+    assert (cap, q, k) == (0, 0, 50) and torch.equal(res.nsteps, base.nsteps)
+
+
+@pytest.mark.parametrize("name,dtype,kw,K", [
+    ("duffing", torch.float32, {}, 300), ("duffing", torch.float64, {}, 1),
+    ("pendulum", torch.float32, {}, 250),
+    ("lorenz", torch.float32, {"stop_threshold": 200.0}, 150),
+    ("lorenz", torch.float32, {"stop_threshold": 200.0, "inf_fill": True}, 40),
+    ("duffing", torch.float32, {}, 10**6),  # budget beyond every trajectory: nothing suspended
+])
+def test_two_phase_launch_equals_one_launch(cuda_device, name, dtype, kw, K):
+    """bh_solve_batch_phased (phase 1 capped at K attempted steps, the rest
+    packed and resumed) gives the trajectories, statistics and counters of a
+    single launch bit for bit, on a ragged batch size."""
+    ode, _, lo, hi, _ = CASES[name]
+    n = 1000 + 37
+    y0 = torch.tensor(O.sample_uniform(n, lo, hi, seed=11), dtype=dtype).to(cuda_device)
+    common = dict(t_span=(0, 60), t_steps=120, t_eval=(30.0, 60.0), device="cuda", cache_dir=None, **kw)
+    one = HipDopri5Solver(**common).integrate_fused(ode, y0, keep_trajectories=True)
+    two = HipDopri5Solver(**common, phase_iters=K).integrate_fused(ode, y0, keep_trajectories=True)
+    assert torch.equal(two.y.nan_to_num(7.0), one.y.nan_to_num(7.0))
+    assert torch.equal(two.stats.nan_to_num(7.0), one.stats.nan_to_num(7.0))
+    assert torch.equal(two.final.nan_to_num(7.0), one.final.nan_to_num(7.0))
+    assert torch.equal(two.status, one.status) and torch.equal(two.nsteps, one.nsteps)
+    if K < 10**6:
+        assert int((one.nsteps > K).sum()) > 0  # some trajectories were suspended and resumed
 
 
 def test_refill_quorum_does_not_change_results(cuda_device):
