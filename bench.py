@@ -68,7 +68,7 @@ def parse():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--max-lanes", type=int, default=0)
     ap.add_argument("--phase-iters", type=int, default=0,
-                    help="duffing: two-phase launch with this step budget instead of the tuner")
+                    help="duffing, lorenz: two-phase launch with this step budget instead of the tuner")
     ap.add_argument("--no-tune", action="store_true",
                     help="duffing: skip the launch-shape tuner (every trajectory on its own lane, "
                          "one launch)")
@@ -572,7 +572,11 @@ def lorenz_workload(args):
                              inf_fill=True)
     os.environ.setdefault("PYBASIN_AMD_TUNE_CACHE", os.path.join("/tmp", "pybasin_amd_tune.json"))
     t0 = time.perf_counter()
-    shape = autotune.tune_launch_shape(solver, ode, y0_all, use_cache=False)
+    if args.phase_iters > 0:
+        shape = {"max_lanes": 0, "refill_quorum": 0, "phase_iters": args.phase_iters}
+        autotune.last_timings.clear()
+    else:
+        shape = autotune.tune_launch_shape(solver, ode, y0_all, use_cache=False)
     t_tune = time.perf_counter() - t0
     autotune.apply_launch_shape(solver, shape)
     save_ts = torch.linspace(900.0, 1000.0, 1000, dtype=torch.float32, device="cuda")
