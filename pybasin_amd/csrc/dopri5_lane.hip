@@ -306,7 +306,8 @@ __device__ __forceinline__ void lane_fill_rest(const LaneArgs &a, LaneT &L, int6
 #pragma unroll
     for (int i = 0; i < S; i++) v[i] = inf ? (T)INFINITY : L.y[i];
     if (L.si >= a.n_save) return;
-    for (int32_t si = L.si; si < a.n_save; si++) write_row<Sys, T>(a, ic, si, v);
+    if (a.traj)
+        for (int32_t si = L.si; si < a.n_save; si++) write_row<Sys, T>(a, ic, si, v);
 #pragma unroll
     for (int i = 0; i < S; i++) L.last[i] = v[i];
     const int32_t w0 = max(L.si, a.steady_idx);
