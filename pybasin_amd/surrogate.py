@@ -290,6 +290,38 @@ def integrate_labels(solver, ode, x: torch.Tensor, time_steady: float, stream=No
     return pendulum_labels_from_stats(res.stats)
 
 
+def tune_plain_launch(solver, ode, x: torch.Tensor, time_steady: float,
+                      quantiles=(0.1, 0.25, 0.5), reps: int = 2) -> dict:
+    """Launch shape for the plain path on rows (theta, omega, T), set on the
+    solver: the whole-device refilling launch or a two-phase launch whose
+    step budget is the attempted-step count at which the given fractions of
+    the rows have ended (bh_autotune_lane on the rows themselves).  Returns
+    the shape with the candidates' times."""
+    dev = solver.device
+    x = x.to(dev, torch.float32)
+    n = x.shape[0]
+    alpha, K = float(ode.params["alpha"]), float(ode.params["K"])
+    params = torch.stack([torch.full((n,), alpha, dtype=torch.float64, device=dev),
+                          x[:, 2].double(), torch.full((n,), K, dtype=torch.float64, device=dev)], 1)
+    save_ts, y0 = solver._prepare_tensors(x[:, :2].contiguous())
+    plan = solver.plan_fused(ode, y0, save_ts, params.contiguous(), time_steady=time_steady)
+    _, _, _, timings, res = solver.tune_fused(plan, [0], [64], reps=reps)
+    steps = res.nsteps.double()
+    qs = torch.quantile(steps[:: max(1, n // 1_000_000)].contiguous(),
+                        torch.tensor(list(quantiles), dtype=torch.float64, device=dev))
+    top = int(steps.max())
+    budgets = sorted({int(v) + 1 for v in qs.cpu().tolist() if 0 < int(v) + 1 < top})
+    if budgets:
+        timings.update(solver.tune_fused(plan, [], [], budgets, reps=reps)[3])
+    best = min(timings, key=timings.get)
+    shape = ({"max_lanes": 0, "refill_quorum": 0, "phase_iters": best[1]} if best[0] == "phased"
+             else {"max_lanes": best[1], "refill_quorum": best[2], "phase_iters": 0})
+    solver.max_lanes, solver.refill_quorum, solver.phase_iters = (
+        shape["max_lanes"], shape["refill_quorum"], shape["phase_iters"])
+    names = {k: (f"phase{k[1]}" if k[0] == "phased" else f"{k[1]}/q{k[2]}") for k in timings}
+    return dict(shape, timings_ms={names[k]: round(v, 3) for k, v in timings.items()})
+
+
 def evolve_states(solver, ode, x: torch.Tensor, t_probe: float, stream=None) -> torch.Tensor:
     """Rows (theta, omega, T) advanced by Dopri5 from t_span[0] to t_probe
     (same tolerances, per-row torque), as new (theta, omega, T) rows."""
@@ -300,7 +332,7 @@ def evolve_states(solver, ode, x: torch.Tensor, t_probe: float, stream=None) -> 
     alpha, K = float(ode.params["alpha"]), float(ode.params["K"])
     params = torch.stack([torch.full((n,), alpha, dtype=torch.float64, device=dev),
                           x[:, 2].double(), torch.full((n,), K, dtype=torch.float64, device=dev)], 1)
-    save = torch.tensor([float(t_probe)], dtype=torch.float32, device=dev)
+    save = torch.full((1,), float(t_probe), dtype=torch.float32, device=dev)  # no host copy
     res = solver.run_fused(ode, x[:, :2].contiguous(), save, params.contiguous(),
                            keep_trajectories=False, steady_idx=1, stream=stream)
     return torch.cat([res.final, x[:, 2:3]], 1).contiguous()
@@ -311,7 +343,7 @@ def classify_gated(model: SurrogateMLP, solver, ode, x: torch.Tensor, gate: floa
                    time_steady: float = 950.0, seed: int = 0,
                    group_error_cap: float | None = 0.01,
                    evolve_t: float | None = None, early_frac: float = 0.0,
-                   stage_times: dict | None = None) -> GatedResult:
+                   stage_times: dict | None = None, n_groups: int | None = None) -> GatedResult:
     """Surrogate labels for confident rows, Dopri5 for the rest.
 
     A random audit sample (audit_frac of every group) is integrated over the
@@ -339,7 +371,10 @@ def classify_gated(model: SurrogateMLP, solver, ode, x: torch.Tensor, gate: floa
     integrated rows always take the integrator's label.
 
     stage_times: if a dict is given, it receives the main stream's elapsed
-    milliseconds at each stage boundary (HIP events; read after the call)."""
+    milliseconds at each stage boundary (HIP events; read after the call).
+
+    n_groups: the number of groups (max(group) + 1), if the caller knows it
+    (saves reading it back from the device)."""
     dev = solver.device
     marks = []
 
@@ -356,14 +391,17 @@ def classify_gated(model: SurrogateMLP, solver, ode, x: torch.Tensor, gate: floa
     g = torch.Generator(device=dev).manual_seed(seed)
     u = torch.rand((n,), generator=g, device=dev)
     audit = u < audit_frac
+    aidx = torch.nonzero(audit).squeeze(1)
     if group is not None:  # every group audited (a uniform draw almost always is)
         gl = group.long()
-        ng = int(gl.max()) + 1
+        ng = int(n_groups) if n_groups is not None else int(gl.max()) + 1
         have = torch.zeros((ng,), dtype=torch.bool, device=dev)
-        have[gl[audit]] = True
-        for gi in torch.nonzero(~have).squeeze(1).tolist():
-            audit[torch.nonzero(gl == gi)[0, 0]] = True
-    aidx = torch.nonzero(audit).squeeze(1)
+        have[gl[aidx]] = True
+        missing = torch.nonzero(~have).squeeze(1).tolist()
+        if missing:
+            for gi in missing:
+                audit[torch.nonzero(gl == gi)[0, 0]] = True
+            aidx = torch.nonzero(audit).squeeze(1)
     side = torch.cuda.Stream(device=dev)
     early = torch.zeros_like(audit)
     k = int(early_frac * n)
@@ -372,7 +410,7 @@ def classify_gated(model: SurrogateMLP, solver, ode, x: torch.Tensor, gate: floa
         mark("infer")
         early[torch.topk(conf, k, largest=False).indices] = True
     side_rows = audit | early
-    sidx = torch.nonzero(side_rows).squeeze(1)
+    sidx = aidx if k == 0 else torch.nonzero(side_rows).squeeze(1)
     xs = x[sidx]
     # the audit's (and the earliest) full integrations overlap the network
     # passes and the short integrations
@@ -393,8 +431,10 @@ def classify_gated(model: SurrogateMLP, solver, ode, x: torch.Tensor, gate: floa
             return
         lab2, conf2 = model.predict(evolve_states(solver, ode, x[idx], evolve_t))
         ok = conf2 >= gate
-        evolved[idx[ok]] = True
-        lab_e[idx[ok]] = lab2[ok]
+        # rows of one call are not evolved yet; index updates without a
+        # device-to-host read
+        evolved[idx] = ok
+        lab_e[idx] = torch.where(ok, lab2, lab_e[idx])
 
     pending = []  # (row indices, labels) of full integrations issued on the main stream
 
@@ -410,10 +450,13 @@ def classify_gated(model: SurrogateMLP, solver, ode, x: torch.Tensor, gate: floa
 
     def group_rate(pred, ref, valid):
         # disagreement of pred with ref per group, over audited rows where valid
+        # (sums by index_add: bincount would read the largest index back)
         v = valid[aidx]
         wrong = ((pred[aidx] != ref[aidx]) & v).double()
-        return torch.bincount(gl[aidx], weights=wrong, minlength=ng) / torch.bincount(
-            gl[aidx], weights=v.double(), minlength=ng).clamp_min(1)
+        ga = gl[aidx]
+        num = torch.zeros((ng,), dtype=torch.float64, device=dev).index_add_(0, ga, wrong)
+        den = torch.zeros((ng,), dtype=torch.float64, device=dev).index_add_(0, ga, v.double())
+        return num / den.clamp_min(1)
 
     done_full = early.clone()
     group_routed = torch.zeros_like(routed)
