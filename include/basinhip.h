@@ -135,8 +135,9 @@ int32_t bh_solve_batch_peaks(int32_t system_id, int32_t dtype, const void *y0s, 
  * a few live lanes per wave.  Results equal bh_solve_batch bit for bit (a
  * trajectory's arithmetic does not depend on the lane or launch that runs
  * it).  Arguments as bh_solve_batch, with phase_iters (>= 1) in place of
- * max_lanes; no orbit peaks; asynchronous (no host synchronisation).  Same
- * replaced interface as bh_solve_batch.
+ * max_lanes; no orbit peaks; asynchronous (no host synchronisation).
+ * Replaces the same reference interface as bh_solve_batch (the integration
+ * inside TorchDiffEqSolver._integrate, src/pybasin/solvers/torchdiffeq_solver.py:97-134).
  *
  *   workspace   >= bh_solve_batch_phased_workspace_bytes(system_id, dtype,
  *               dim, n) device bytes (counter, and index + state record of
