@@ -198,3 +198,28 @@ def test_surrogate_gain_on_torque_sweep(cuda_device):
         if evolve_t is not None:
             assert integrated < routed  # the short integrations resolve routed rows
             assert res.audit_evolved_error is None or res.audit_evolved_error < 0.02
+            # the group count given by the caller changes nothing
+            again = classify_gated(model, solver, ode, x, gate=0.995, group=group, evolve_t=evolve_t,
+                                   n_groups=6)
+            assert torch.equal(again.labels, res.labels) and torch.equal(again.audit_idx, res.audit_idx)
+
+
+def test_tuned_plain_launch_keeps_the_labels(cuda_device):
+    """tune_plain_launch picks a launch shape for the plain path on the rows
+    themselves (refilling or two-phase); the labels stay those of the default
+    launch."""
+    from pybasin_amd.solvers import HipDopri5Solver
+    from pybasin_amd.surrogate import integrate_labels, torque_sweep_ics, tune_plain_launch
+    from pybasin_amd.systems import PendulumODE
+
+    x, _ = torque_sweep_ics(np.linspace(0.2, 0.9, 4), [3_000] * 4, seed=5, device=cuda_device)
+    ode = PendulumODE({"alpha": 0.1, "T": 0.5, "K": 1.0})
+    solver = HipDopri5Solver(t_span=(0, 200), t_steps=200, t_eval=(150.0, 200.0), device="cuda",
+                             cache_dir=None)
+    want = integrate_labels(solver, ode, x, 175.0)
+    tuned = solver.clone(cache_dir=None)
+    shape = tune_plain_launch(tuned, ode, x, 175.0, reps=1)
+    assert {"max_lanes", "refill_quorum", "phase_iters", "timings_ms"} <= set(shape)
+    assert any(k.startswith("phase") for k in shape["timings_ms"])
+    assert tuned.phase_iters == shape["phase_iters"]
+    assert torch.equal(integrate_labels(tuned, ode, x, 175.0), want)
