@@ -3,9 +3,10 @@
 A basin-stability sweep launches the same integration shape many times (new
 ICs, same sizes).  ``CapturedIntegration`` does the host work of a launch once
 (validation, parameter and save-grid uploads), records the refill-counter
-reset and the integration kernel into a graph through torch's graph API, and
-replays it after copying new ICs into the captured input buffer.  The counter
-is reset by a kernel, never by a memset, so every replay starts from the same
+reset and the integration kernel (both phases and the compaction counter
+of a two-phase launch) into a graph through torch's graph API, and replays
+it after copying new ICs into the captured input buffer.  The counter is
+reset by a kernel, never by a memset, so every replay starts from the same
 state; outputs are overwritten on every replay.
 """
 

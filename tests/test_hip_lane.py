@@ -200,14 +200,17 @@ def test_contract_errors(cuda_device):
     assert c.t_eval == (1, 2) and c.device.type == "cpu"
 
 
-def test_graph_replay_matches_eager(cuda_device):
+@pytest.mark.parametrize("phase_iters", [0, 150])
+def test_graph_replay_matches_eager(cuda_device, phase_iters):
     """The captured integration replayed several times, with eager work and
-    synchronisation between replays, equals the eager launch every time."""
+    synchronisation between replays, equals the eager launch every time
+    (single launch, and the two-phase launch with its counter reset and
+    compaction inside the graph)."""
     from pybasin_amd.graphs import CapturedIntegration
 
     ode = DuffingODE(DUFF)
     solver = HipDopri5Solver(t_span=(0, 100), t_steps=200, t_eval=(50.0, 100.0), device="cuda",
-                             cache_dir=None)
+                             cache_dir=None, phase_iters=phase_iters)
     ya = torch.tensor(O.sample_uniform(3000, [-1.0, -0.5], [1.0, 1.0], seed=21)).to(cuda_device)
     yb = torch.tensor(O.sample_uniform(3000, [-1.0, -0.5], [1.0, 1.0], seed=22)).to(cuda_device)
     cap = CapturedIntegration(solver, ode, ya, time_steady=60.0)
