@@ -178,8 +178,12 @@ def test_contract_t_eval_subrange(cuda_device):
     assert y[0].item() == pytest.approx(math.exp(-1), abs=1e-6)
 
 
-def test_contract_params_ic_major(cuda_device):
-    solver = HipDopri5Solver(t_span=(0, 1), t_steps=11, device="cuda", cache_dir=None)
+@pytest.mark.parametrize("phase_iters", [0, 3])
+def test_contract_params_ic_major(cuda_device, phase_iters):
+    """IC-major parameter rows (base.py:137-159); also through the two-phase
+    launch, whose resumed lanes keep their own parameter row."""
+    solver = HipDopri5Solver(t_span=(0, 1), t_steps=11, device="cuda", cache_dir=None,
+                             phase_iters=phase_iters)
     y0 = torch.tensor([[1.0], [2.0]], device=cuda_device)
     params = torch.tensor([[-1.0], [-2.0]])
     _, y = solver.integrate(ExponentialDecayODE({"decay": -1.0}), y0, params=params)
