@@ -429,9 +429,11 @@ __device__ __forceinline__ void lane_step(const LaneArgs &a, LaneT &L, int64_t i
             cc[i] = h * (f1 - (T)4 * f0) - (T)11 * a0 - (T)5 * a1 + (T)16 * ymid;
             cd[i] = h * f0;
         }
-        const double span = tB - tA;
+        // x = (s - tA) / (tB - tA) with one reciprocal per step (a save at tB
+        // itself takes x = 1 exactly)
+        const double inv_span = 1.0 / (tB - tA);
         do {
-            const T x = (T)((L.next_save - tA) / span);
+            const T x = L.next_save == tB ? T(1) : (T)((L.next_save - tA) * inv_span);
             const T x2 = x * x, x3 = x2 * x, x4 = x3 * x;
             T v[S];
 #pragma unroll
