@@ -160,6 +160,7 @@ struct Lane {
     double t, dt;
     int32_t steps;  // attempted steps (max_steps is capped at 2^31 - 1)
     double next_save;  // save_t[si], +inf once every save is emitted
+    double after_save;  // save_t[si + 1] (+inf past the end), loaded a save ahead
     int32_t si;
     int32_t status;
     bool done;
@@ -291,10 +292,12 @@ __device__ __forceinline__ void lane_init(const LaneArgs &a, LaneT &L, int64_t i
     L.dt = (double)fmin((T)100 * h0, fabs(h1));
     // save times at or before t0 return y0 exactly (torchdiffeq solution[0] = y0)
     L.next_save = a.save_t[0];
+    L.after_save = a.n_save > 1 ? a.save_t[1] : INFINITY;
     while (L.next_save <= a.t0) {
         emit<Sys, T>(a, L, ic, L.si, L.y);
         L.si++;
-        L.next_save = L.si < a.n_save ? a.save_t[L.si] : INFINITY;
+        L.next_save = L.after_save;
+        L.after_save = L.si + 1 < a.n_save ? a.save_t[L.si + 1] : INFINITY;
     }
     if (L.si >= a.n_save) L.done = true;
 }
@@ -445,7 +448,10 @@ __device__ __forceinline__ void lane_step(const LaneArgs &a, LaneT &L, int64_t i
             }
             emit<Sys, T>(a, L, ic, L.si, v);
             L.si++;
-            L.next_save = L.si < a.n_save ? a.save_t[L.si] : INFINITY;
+            // the next save time was loaded one save ahead, so the loop test
+            // does not wait on memory
+            L.next_save = L.after_save;
+            L.after_save = L.si + 1 < a.n_save ? a.save_t[L.si + 1] : INFINITY;
         } while (L.next_save <= tB);
     }
     T ymax = 0;
