@@ -59,4 +59,11 @@ def test_version_and_argument_errors_without_gpu():
                                  0, None, None, None, None, None, 0, None, 0, None)
     assert rc == -5 and b"phase_iters" in L.bh_last_error()
     assert L.bh_solve_batch_phased_workspace_bytes(99, 0, 2, 10) < 0
-    assert L.bh_solve_batch_phased_workspace_bytes(4, 0, 3, 1000) > 256 + 8 * 1000
+    need = L.bh_solve_batch_phased_workspace_bytes(4, 0, 3, 1000)
+    assert need > 256 + 8 * 1000
+    # a workspace below that size is refused before any device work (the
+    # pointers are never dereferenced on this path)
+    fake = 4096
+    rc = L.bh_solve_batch_phased(4, 0, fake, 1000, 3, fake, 3, 0, 0.0, fake, 10, 1e-8, 1e-6, 10, 0,
+                                 0.0, 0, None, None, None, None, None, 5, fake, need - 1, None)
+    assert rc == -5 and b"workspace" in L.bh_last_error()
