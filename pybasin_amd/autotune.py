@@ -118,8 +118,10 @@ def tune_launch_shape(solver, ode_system, y0: torch.Tensor, params: torch.Tensor
     if phase_quantiles and res.nsteps.numel():
         steps = res.nsteps.double()
         top = int(steps.max())
-        qs = torch.quantile(steps, torch.tensor(list(phase_quantiles), dtype=torch.float64,
-                                                device=steps.device))
+        # torch.quantile takes at most 2^24 values: a strided sample of large batches
+        sample = steps[:: max(1, steps.numel() // 1_000_000)].contiguous()
+        qs = torch.quantile(sample, torch.tensor(list(phase_quantiles), dtype=torch.float64,
+                                                 device=steps.device))
         budgets = sorted({int(v) + 1 for v in qs.cpu().tolist() if 0 < int(v) + 1 < top})
         if budgets:
             results.update(probe.tune_fused(plan, [], [], budgets, reps=2)[3])
