@@ -141,7 +141,9 @@ int32_t bh_solve_batch_peaks(int32_t system_id, int32_t dtype, const void *y0s, 
  *
  *   workspace   >= bh_solve_batch_phased_workspace_bytes(system_id, dtype,
  *               dim, n) device bytes (counter, and index + state record of
- *               every trajectory that may be suspended)
+ *               every trajectory that may be suspended); that query returns
+ *               a negative error code (bh_last_error) for an unknown system,
+ *               a dtype other than BH_F32 / BH_F64 or n < 0
  */
 int64_t bh_solve_batch_phased_workspace_bytes(int32_t system_id, int32_t dtype, int32_t dim,
                                               int64_t n);
