@@ -662,7 +662,7 @@ def main():
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
-              "(ICs shard trivially across GPUs; see DESIGN.md)", file=sys.stderr)
+              "(the ICs would shard as contiguous blocks, one per GPU, with one count reduction)", file=sys.stderr)
         sys.exit(2)
 
     if args.workload == "surrogate":
