@@ -401,7 +401,7 @@ def mlp_infer_workload(args):
 
 def hdbscan_workload(args):
     """The reference's default predictor (HDBSCANClusterer(auto_tune=True,
-    assign_noise=True), hdbscan_clusterer.py:16-166) on the GPU: Duffing
+    assign_noise=True), hdbscan_clusterer.py:12-148) on the GPU: Duffing
     features (max x, std x) of the n default ICs, integrated once outside the
     timed region; a step = one auto-tuned clustering with noise assignment.
     sklearn's own pipeline is timed on a bounded sample for comparison."""

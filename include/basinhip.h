@@ -27,9 +27,9 @@ enum {
     BH_SYS_DECAY = 0,    /* y' = p0*y, S = 1 or 2  (tests/unit/solvers/test_solver.py:16-18) */
     BH_SYS_LINEAR2 = 1,  /* y' = [[p0,p1],[p2,p3]] y, S = 2 (test system)              */
     BH_SYS_PENDULUM = 2, /* [alpha, T, K], S = 2  (case_studies/pendulum/pendulum_ode.py:23-40) */
-    BH_SYS_DUFFING = 3,  /* [delta, k3, A], S = 2 (case_studies/duffing_oscillator/duffing_ode.py:72-88) */
+    BH_SYS_DUFFING = 3,  /* [delta, k3, A], S = 2 (case_studies/duffing_oscillator/duffing_ode.py:26-42) */
     BH_SYS_LORENZ = 4,   /* [sigma, r, b], S = 3, freeze mask |y|inf >= 200
-                            (case_studies/lorenz/lorenz_ode.py:116-152) */
+                            (case_studies/lorenz/lorenz_ode.py:28-64) */
     BH_SYS_ROSSLER100 = 5 /* [a, b, c, K], S = 300; see bh_solve_rossler */
 };
 enum { BH_F32 = 0, BH_F64 = 1 };
@@ -288,8 +288,8 @@ int32_t bh_mlp_train_step(const float *x, const int32_t *labels, int64_t M, cons
 /* Order statistics of the feature window of a kept trajectory buffer
    (n_save, n, dim) in dtype: out (n_ranks, n, dim) in dtype holds, per
    (trajectory, state), the value of 0-based rank ranks[r] (device int64,
-   each in [0, n_save - start)) among rows [start, n_save); NaN if the window
-   holds a NaN.  Median = rank (N - 1) / 2 (torch.median, lower middle);
+   valid in [0, n_save - start)) among rows [start, n_save); NaN if the window
+   holds a NaN or if ranks[r] lies outside the valid range.  Median = rank (N - 1) / 2 (torch.median, lower middle);
    quantiles interpolate two ranks (torch.quantile).  Replaces the windowed
    calculators of src/pybasin/ts_torch/calculators/torch_features_statistical.py
    (median 30-38, quantile 154-157). */
@@ -298,7 +298,7 @@ int32_t bh_window_order_stats(int32_t dtype, const void *traj, int64_t n_save, i
                               void *out, void *stream);
 
 /* ---- label layer (replaces the sklearn calls of the reference's
-   HDBSCANClusterer, src/pybasin/predictors/hdbscan_clusterer.py:16-166, and
+   HDBSCANClusterer, src/pybasin/predictors/hdbscan_clusterer.py:12-148, and
    the 1-NN template classifiers of the case studies) --------------------- */
 
 /* k nearest reference rows of every query row (Euclidean, double; rows of F
@@ -319,7 +319,7 @@ int64_t bh_hdbscan_workspace_bytes(int64_t n);
    1 <= k <= min(32, n).  Replaces the core-distance query of sklearn
    HDBSCAN.fit (sklearn/cluster/_hdbscan/hdbscan.py, _hdbscan_brute /
    _hdbscan_prims: NearestNeighbors(min_samples).kneighbors), reached from
-   HDBSCANClusterer.fit_predict (src/pybasin/predictors/hdbscan_clusterer.py:150-166). */
+   HDBSCANClusterer.fit_predict (src/pybasin/predictors/hdbscan_clusterer.py:50-69). */
 int32_t bh_core_distances(const double *X, int64_t n, int32_t F, int32_t k, double *out_core,
                           void *workspace, int64_t workspace_bytes, void *stream);
 
@@ -348,7 +348,7 @@ int32_t bh_hdbscan_labels(const int64_t *edge_a, const int64_t *edge_b, const do
    c occupying [starts[c], starts[c+1])): s = (b - a) / max(a, b), 0 for a
    label of one row.  sklearn.metrics.silhouette_score = mean of s_out
    (replaces the silhouette_score call of _find_optimal_min_cluster_size,
-   hdbscan_clusterer.py:121-166).  Optional weights (n) with label_counts
+   hdbscan_clusterer.py:112-148).  Optional weights (n) with label_counts
    (n_labels) = their sums per label: row i then stands for weights[i] equal
    rows (collapse duplicates to cut the pair count); the score is the
    weights-weighted mean of s_out.  Both NULL: every row counts once.

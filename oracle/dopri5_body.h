@@ -46,11 +46,11 @@ static void FN(rhs)(int sys, REAL t, const REAL *y, REAL *dy, const REAL *p, int
         dy[0] = y[1];
         dy[1] = -p[0] * y[1] + p[1] - p[2] * SIN(y[0]);
         break;
-    case OR_SYS_DUFFING: /* case_studies/duffing_oscillator/duffing_ode.py:81-88 */
+    case OR_SYS_DUFFING: /* case_studies/duffing_oscillator/duffing_ode.py:35-42 */
         dy[0] = y[1];
         dy[1] = -p[0] * y[1] - p[1] * (y[0] * y[0] * y[0]) + p[2] * COS(t);
         break;
-    case OR_SYS_LORENZ: { /* case_studies/lorenz/lorenz_ode.py:128-152 (freeze mask) */
+    case OR_SYS_LORENZ: { /* case_studies/lorenz/lorenz_ode.py:40-64 (freeze mask) */
         REAL m = FABS(y[0]);
         if (FABS(y[1]) > m) m = FABS(y[1]);
         if (FABS(y[2]) > m) m = FABS(y[2]);

@@ -50,7 +50,7 @@ enum { OR_MODE_TD_LANE = 0, OR_MODE_TD_SHARED = 1, OR_MODE_ZIG = 2 };
 #define OR_ROSSLER_E 400
 static int g_edges_i[2 * OR_ROSSLER_E], g_edges_j[2 * OR_ROSSLER_E];
 
-/* rossler_network_topology.py:105-127: both directions of every edge, in
+/* rossler_network_topology.py:26-48: both directions of every edge, in
    list order (i->j, then j->i). */
 int or_set_rossler_edges(const int32_t *ei, const int32_t *ej, int n_edges)
 {

@@ -9,10 +9,6 @@ Contents
 * ``solve``            ctypes front end of ``liboracle.so`` (dopri5_oracle.c):
                        torchdiffeq semantics per lane or with one shared dt,
                        or zigode semantics.
-* ``save_grid``        ``Solver._prepare_tensors`` + TorchDiffEqSolver anchor
-                       (src/pybasin/solvers/base.py:113-127,
-                       src/pybasin/solvers/torchdiffeq_solver.py:122-131).
-* ``expand_params``    IC-major expansion (src/pybasin/solvers/base.py:155-159).
 * ``sample_uniform``   UniformRandomSampler (src/pybasin/sampler.py:61-65).
 * ``filter_time_idx``  FeatureExtractor.filter_time (feature_extractor.py:85-130).
 * ``features``         mean / std(ddof=0) / min / max / delta / log_delta
@@ -20,8 +16,15 @@ Contents
                        with ``impute_extreme`` (torch_feature_utilities.py:60-94).
 * ``sync_features``    SynchronizationFeatureExtractor._compute_sync_features
                        (case_studies/rossler_network/synchronization_feature_extractor.py:73-106).
+* ``sync_labels``      SynchronizationClassifier.fit_predict
+                       (case_studies/rossler_network/synchronization_classifier.py:36-54)
+                       plus the stop-event "unbounded" label.
 * ``basin_stability``  _compute_bs / get_errors (basin_stability_estimator.py:586-625).
-* ``knn1``             1-nearest-neighbour label (sklearn KNeighborsClassifier(1)).
+
+Every function is pinned in tests/test_oracle_golden.py: the solver by the
+reference's published tables and closed forms, the helpers by known answers
+a plausible slip (a strict comparison, a wrong axis or slice, a wrong
+default fraction) would fail.
 """
 
 from __future__ import annotations
@@ -86,18 +89,6 @@ def set_rossler_edges(edges: np.ndarray) -> None:
         raise ValueError("edge list must hold 400 edges")
 
 
-def save_grid(t_span, t_steps, t_eval=None, dtype=np.float32):
-    """Save times exactly as the reference builds them: torch.linspace in the
-    state dtype over t_eval (or t_span), then promoted to float64 by
-    torchdiffeq.  Returns (save_ts_in_dtype, save_ts_f64, t0)."""
-    import torch
-
-    start, end = t_eval if t_eval is not None else t_span
-    tdt = torch.float32 if dtype == np.float32 else torch.float64
-    ts = torch.linspace(start, end, t_steps, dtype=tdt)
-    return ts.numpy(), ts.numpy().astype(np.float64), float(t_span[0])
-
-
 def solve(sys: int, y0: np.ndarray, params: np.ndarray, save_ts64: np.ndarray, t0: float,
           rtol: float = 1e-8, atol: float = 1e-6, bits: int = 32, mode: int = MODE_TD_LANE,
           t1: float | None = None, max_steps: int = 10**9, threads: int = 1):
@@ -139,12 +130,6 @@ def rhs(sys: int, t: float, y, p, bits: int = 64) -> np.ndarray:
     return dy
 
 
-def expand_params(y0: np.ndarray, params: np.ndarray):
-    """IC-major: trajectory ic*P + p carries (y0[ic], params[p])."""
-    B, P = y0.shape[0], params.shape[0]
-    return np.repeat(y0, P, axis=0), np.tile(params, (B, 1))
-
-
 def sample_uniform(n: int, lo, hi, seed: int) -> np.ndarray:
     """torch.manual_seed(seed); torch.rand(n, S) * (hi - lo) + lo in float32 on CPU."""
     import torch
@@ -156,12 +141,17 @@ def sample_uniform(n: int, lo, hi, seed: int) -> np.ndarray:
 
 
 def filter_time_idx(save_ts: np.ndarray, time_steady: float | None) -> int:
-    """Index of the first save time >= time_steady (0 if none); default
-    time_steady = t0 + 0.85 (t1 - t0) of the save grid."""
+    """FeatureExtractor.filter_time's start index (feature_extractor.py:85-130):
+    the first save time >= time_steady, or 0 when no save time qualifies
+    (the whole window is kept, as the reference does).  time_steady None
+    means t0 + 0.85 (t1 - t0) over the save grid itself (feature_extractor.py:96-100,
+    DEFAULT_STEADY_FRACTION = 0.85 in src/pybasin/constants.py:6).  The comparison runs in the grid's dtype:
+    torch compares a float32 tensor with a Python float in float32."""
+    ts = np.asarray(save_ts)
     if time_steady is None:
-        t0, t1 = float(save_ts[0]), float(save_ts[-1])
+        t0, t1 = float(ts[0]), float(ts[-1])
         time_steady = t0 + 0.85 * (t1 - t0)
-    idx = np.nonzero(save_ts >= time_steady)[0]
+    idx = np.nonzero(ts >= ts.dtype.type(time_steady))[0]
     return int(idx[0]) if len(idx) else 0
 
 
@@ -188,18 +178,25 @@ def features(window: np.ndarray) -> dict[str, np.ndarray]:
 
 
 def sync_features(final_state: np.ndarray, n_nodes: int) -> np.ndarray:
-    x = final_state[:, :n_nodes]
-    y = final_state[:, n_nodes:2 * n_nodes]
-    z = final_state[:, 2 * n_nodes:]
-    dx = x.max(1) - x.min(1)
-    dy = y.max(1) - y.min(1)
-    dz = z.max(1) - z.min(1)
-    return np.stack([dx, dy, dz, np.maximum(np.maximum(dx, dy), dz)], axis=1)
+    """[max_dev_x, max_dev_y, max_dev_z, max_dev_all] per row of the final
+    state (n, 3 N), laid out [x_0..x_{N-1}, y_0.., z_0..]: the spread
+    max - min over the nodes of each state block, and the largest of the
+    three (synchronization_feature_extractor.py:73-106)."""
+    f = np.asarray(final_state, dtype=np.float64)
+    N = n_nodes
+    blocks = [f[:, 0:N], f[:, N:2 * N], f[:, 2 * N:3 * N]]
+    dev = [blk.max(axis=1) - blk.min(axis=1) for blk in blocks]
+    return np.stack(dev + [np.maximum(np.maximum(dev[0], dev[1]), dev[2])], axis=1)
 
 
-def knn1(feat: np.ndarray, templates: np.ndarray, labels) -> np.ndarray:
-    d = ((feat[:, None, :] - templates[None, :, :]) ** 2).sum(-1)
-    return np.asarray(labels)[np.argmin(d, axis=1)]
+def sync_labels(features: np.ndarray, unbounded: np.ndarray, epsilon: float = 1.5) -> np.ndarray:
+    """'synchronized' where max_dev_all < epsilon, else 'desynchronized'
+    (synchronization_classifier.py:36-54; epsilon 1.5 from
+    setup_rossler_network_system.py:103-105); rows flagged unbounded (the
+    stop event fired) are 'unbounded'."""
+    lab = np.where(np.asarray(features)[:, 3] < epsilon, "synchronized", "desynchronized").astype(object)
+    lab[np.asarray(unbounded, dtype=bool)] = "unbounded"
+    return lab
 
 
 def basin_stability(labels: np.ndarray) -> dict:

@@ -1,7 +1,7 @@
 """On-disk cache of integration results (src/pybasin/cache_manager.py:27-128).
 
-Key = md5 over the solver name, the ODE description, the solver config,
-the bytes of y0 / save_ts / params; value = a safetensors file holding t and y.
+Key = md5 over the solver name, the ODE description, the ODE's parameter
+values, the solver config, the bytes of y0 / save_ts / params; value = a safetensors file holding t and y.
 A corrupt file is deleted and the result recomputed.
 
 Fused results (``HipDopri5Solver.integrate_fused``) are cached the same way
@@ -23,6 +23,15 @@ import torch
 logger = logging.getLogger("pybasin_amd")
 
 
+def _plain(v):
+    """A parameter value as a hashable plain value (tensors by their values)."""
+    if isinstance(v, torch.Tensor):
+        return (str(v.dtype), tuple(v.detach().cpu().reshape(-1).tolist()))
+    if isinstance(v, (list, tuple)):
+        return tuple(_plain(x) for x in v)
+    return v
+
+
 class CacheManager:
     def __init__(self, cache_dir: str):
         self.cache_dir = cache_dir
@@ -31,6 +40,10 @@ class CacheManager:
         h = hashlib.md5()
         h.update(solver_name.encode())
         h.update(getattr(ode_system, "get_str", lambda: type(ode_system).__name__)().encode())
+        # the parameter values, sorted by name (cache_manager.py:46-48): an
+        # ODE's description may carry no parameters (utils.py:56-89)
+        params_dict = dict(getattr(ode_system, "params", None) or {})
+        h.update(repr(tuple(sorted((str(k), _plain(v)) for k, v in params_dict.items()))).encode())
         h.update(json.dumps(solver_config, sort_keys=True, default=str).encode())
         for t in (y0, save_ts, params):
             if t is not None:

@@ -1,7 +1,7 @@
 """The label layer on the GPU: nearest neighbours and HDBSCAN.
 
 ``HipHDBSCANClusterer`` mirrors the reference's HDBSCANClusterer
-(src/pybasin/predictors/hdbscan_clusterer.py:16-166): sklearn HDBSCAN
+(src/pybasin/predictors/hdbscan_clusterer.py:12-148): sklearn HDBSCAN
 semantics (min_cluster_size, min_samples counting the point itself,
 excess-of-mass selection, no single cluster, Euclidean metric), the
 silhouette-based min_cluster_size auto-tuning over the same five candidate
@@ -169,7 +169,7 @@ def silhouette_score(X: torch.Tensor, labels) -> float:
 
 
 class HipHDBSCANClusterer:
-    """HDBSCANClusterer on the GPU (hdbscan_clusterer.py:16-166)."""
+    """HDBSCANClusterer on the GPU (hdbscan_clusterer.py:12-148)."""
 
     def __init__(self, min_cluster_size: int = 50, min_samples: int | None = 10,
                  assign_noise: bool = False, n_neighbors: int = 5, auto_tune: bool = False,

@@ -3,7 +3,7 @@
 // condensed tree, excess-of-mass selection) and silhouette scores.
 //
 // Replaces the sklearn calls behind the reference's HDBSCANClusterer
-// (src/pybasin/predictors/hdbscan_clusterer.py:16-166): HDBSCAN.fit_predict,
+// (src/pybasin/predictors/hdbscan_clusterer.py:12-148): HDBSCAN.fit_predict,
 // NearestNeighbors.kneighbors (noise assignment) and silhouette_score
 // (min_cluster_size auto-tuning).  The algorithm is HDBSCAN* as published
 // (Campello, Moulavi, Sander 2013; McInnes, Healy 2017): core distance =

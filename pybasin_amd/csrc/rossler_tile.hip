@@ -3,7 +3,7 @@
 //
 // Right-hand side (case_studies/rossler_network/rossler_network_jax_ode.py:63-79):
 //   x_i' = -y_i - z_i - K (L x)_i,  y_i' = x_i + a y_i,  z_i' = b + z_i (x_i - c)
-// with L the 100 x 100 graph Laplacian (rossler_network_topology.py:143-156).
+// with L the 100 x 100 graph Laplacian (rossler_network_topology.py:64-77).
 // The coupling for the 16 trajectories of a tile is one (16 x 100) . (100 x 100)
 // product per stage on the matrix cores (v_mfma_f32_16x16x4f32, FP32 in and
 // out): seven waves each own a 16-column block of L for the first 22 of its

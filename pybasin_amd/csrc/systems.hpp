@@ -108,7 +108,7 @@ struct Pendulum : NoTimeTerms {
     }
 };
 
-// case_studies/duffing_oscillator/duffing_ode.py:81-88
+// case_studies/duffing_oscillator/duffing_ode.py:35-42
 //   x' = v;  v' = -delta*v - k3*x^3 + A*cos(t);  p = [delta, k3, A]
 // Within a step the stage times t_i (values of the state dtype) differ from
 // t_0 by d_i = t_i - t_0, which the subtraction forms exactly once
@@ -167,7 +167,7 @@ __device__ __forceinline__ void rhs_aux(const T (&p)[Sys::NP], T aux, const T (&
         Sys::f(p, T(0), y, dy);
 }
 
-// case_studies/lorenz/lorenz_ode.py:128-152, including the freeze mask:
+// case_studies/lorenz/lorenz_ode.py:40-64, including the freeze mask:
 // the derivative is multiplied by (max|y| < 200).
 struct Lorenz : NoTimeTerms {
     static constexpr int S = 3, NP = 3;

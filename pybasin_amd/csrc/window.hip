@@ -9,7 +9,8 @@
 // addresses) and finds the value of each requested rank by radix selection
 // on the order-preserving integer image of the floats, 4 bits per pass
 // (8 passes for float, 16 for double), 16 counters in registers.  A window
-// holding a NaN yields NaN (torch.median and torch.quantile propagate NaN).
+// holding a NaN yields NaN (torch.median and torch.quantile propagate NaN);
+// so does a rank outside [0, window length).
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -59,6 +60,10 @@ __global__ void __launch_bounds__(256) order_stat_kernel(const T *traj, int64_t 
             continue;
         }
         int64_t k = ranks[r];  // rank within the window, 0-based
+        if (k < 0 || k >= n_save - start) {  // no such order statistic
+            out[(int64_t)r * np + p] = (T)NAN;
+            continue;
+        }
         U prefix = 0, mask = 0;
         for (int shift = Key<T>::BITS - 4; shift >= 0; shift -= 4) {
             int64_t cnt[16];

@@ -242,7 +242,9 @@ class StatFeatureExtractor(FeatureExtractor):
         with self._lock:
             if self._mean is None:
                 self._mean = feats.mean(0, keepdim=True)
-                self._std = feats.std(0, keepdim=True).clamp_min(1e-12)
+                std = feats.std(0, keepdim=True)
+                # a constant column keeps its scale (torch_feature_extractor.py:273-281)
+                self._std = torch.where(std == 0, torch.ones_like(std), std)
         return (feats - self._mean) / self._std
 
     def extract_features(self, solution: Solution) -> torch.Tensor:

@@ -1,7 +1,7 @@
 """HipDopri5Solver: per-lane adaptive Dormand-Prince 5(4) on MI355X.
 
 Drop-in ``Solver`` (src/pybasin/solvers/base.py:16-213, protocol in
-src/pybasin/protocols.py:195-266) whose ``_integrate`` replaces
+src/pybasin/protocols.py:117-187) whose ``_integrate`` replaces
 TorchDiffEqSolver._integrate (src/pybasin/solvers/torchdiffeq_solver.py:97-134).
 
 * Systems with a device functor (``ode_system.device_system_id``) run in the
@@ -23,11 +23,11 @@ from __future__ import annotations
 
 import logging
 import threading
-import weakref
 from dataclasses import dataclass
 from typing import Any
 
 import torch
+from torch.utils.weak import WeakTensorKeyDictionary
 
 from .. import _lib
 from .base import DEFAULT_CACHE_DIR, UNSET, Solver
@@ -36,8 +36,10 @@ from .torch_dopri5 import TorchDopri5Solver
 logger = logging.getLogger("pybasin_amd")
 
 # fused results attached to returned trajectory tensors (read by
-# HipFusedFeatureExtractor); keyed weakly so nothing outlives the tensor
-FUSED_RESULTS: "weakref.WeakKeyDictionary[torch.Tensor, FusedResult]" = weakref.WeakKeyDictionary()
+# HipFusedFeatureExtractor); keyed weakly so nothing outlives the tensor, and
+# by tensor identity: a plain WeakKeyDictionary compares keys with
+# Tensor.__eq__, which is elementwise and cannot be a dictionary test
+FUSED_RESULTS: "WeakTensorKeyDictionary" = WeakTensorKeyDictionary()
 
 
 # ODE classes that do not carry ``device_system_id`` themselves (e.g. the

@@ -266,7 +266,7 @@ class GatedResult:
 
 def pendulum_labels_from_stats(stats: torch.Tensor) -> torch.Tensor:
     """FP (0) / LC (1) from the window statistics of omega: FP when
-    |max - mean| < 0.01 (case_studies/pendulum/pendulum_feature_extractor.py:20-31)."""
+    |max - mean| < 0.01 (case_studies/pendulum/pendulum_feature_extractor.py:17-29)."""
     delta = (stats[:, 1, 3] - stats[:, 1, 0]).abs()
     lab = (delta >= 0.01).to(torch.int32)
     return torch.where(torch.isfinite(delta), lab, torch.ones_like(lab))

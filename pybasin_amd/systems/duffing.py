@@ -1,4 +1,4 @@
-"""Duffing oscillator (case_studies/duffing_oscillator/duffing_ode.py:54-88)."""
+"""Duffing oscillator (case_studies/duffing_oscillator/duffing_ode.py:8-42)."""
 
 from __future__ import annotations
 

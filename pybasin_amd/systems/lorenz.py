@@ -1,4 +1,4 @@
-"""Lorenz system with the freeze mask (case_studies/lorenz/lorenz_ode.py:96-152)."""
+"""Lorenz system with the freeze mask (case_studies/lorenz/lorenz_ode.py:8-64)."""
 
 from __future__ import annotations
 

@@ -6,7 +6,7 @@ Torch restatement of RosslerNetworkJaxODE
 list is kept as data in ``data/rossler_ws100_edges.txt``).  State layout
 [x_0..x_{N-1}, y_0..y_{N-1}, z_0..z_{N-1}]; per node
     x_i' = -y_i - z_i - K (L x)_i,   y_i' = x_i + a y_i,   z_i' = b + z_i (x_i - c)
-with L = D - A the graph Laplacian (rossler_network_topology.py:143-156).
+with L = D - A the graph Laplacian (rossler_network_topology.py:64-77).
 """
 
 from __future__ import annotations

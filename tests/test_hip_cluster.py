@@ -1,6 +1,6 @@
 """GPU label layer (libbasinhip.so bh_knn / bh_hdbscan_* / bh_silhouette)
 against scikit-learn, the library the reference's HDBSCANClusterer and
-KNeighborsClassifier call (src/pybasin/predictors/hdbscan_clusterer.py:16-166).
+KNeighborsClassifier call (src/pybasin/predictors/hdbscan_clusterer.py:12-148).
 
 sklearn 1.7.2 is installed here (the reference pins 1.8.0; the HDBSCAN,
 NearestNeighbors and silhouette code paths used are unchanged between them).
@@ -174,7 +174,7 @@ def test_silhouette_with_repeated_rows_matches_sklearn(cuda_device):
 
 def _reference_clusterer(X):
     """The reference's HDBSCANClusterer(auto_tune=True, assign_noise=True)
-    steps (hdbscan_clusterer.py:55-166) on sklearn objects."""
+    steps (hdbscan_clusterer.py:50-148) on sklearn objects."""
     from sklearn.cluster import HDBSCAN
     from sklearn.metrics import silhouette_score
     from sklearn.neighbors import NearestNeighbors

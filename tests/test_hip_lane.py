@@ -122,7 +122,7 @@ def test_lane_refill_is_deterministic(cuda_device):
 
 def test_lorenz_stop_freeze_matches_masked_oracle(cuda_device):
     """Per-lane stop at |y|inf >= 200 with the frozen state filled in equals
-    integrating the masked RHS to the end (lorenz_ode.py:146-152)."""
+    integrating the masked RHS to the end (lorenz_ode.py:58-64)."""
     y0 = torch.tensor(O.sample_uniform(128, [-10.0, -20.0, 0.0], [10.0, 20.0, 0.0], seed=5))
     stop = HipDopri5Solver(t_span=(0, 60), t_steps=120, device="cuda", cache_dir=None,
                            stop_threshold=200.0)

@@ -9,9 +9,9 @@ reference case study's feature and template recipe:
 * pendulum case 1 (case_studies/pendulum/setup_pendulum_system.py:34-65):
   t in [0, 1000], 1000 saves, time_steady 950, log_delta of omega, 1-NN on
   templates [0.4, 0] FP / [2.7, 0] LC;
-* Duffing (setup_duffing_oscillator_system.py:98-139): save window
+* Duffing (setup_duffing_oscillator_system.py:12-74): save window
   [900, 1000] with 5000 points, maximum and std of x, 1-NN on 5 templates;
-* Lorenz (setup_lorenz_system.py:192-227): save window [900, 1000], 4000
+* Lorenz (setup_lorenz_system.py:29-88): save window [900, 1000], 4000
   points, stop at |y|inf >= 200 with Inf fill (the Event semantics; such
   trajectories are "unbounded", basin_stability_estimator.py:519-527), mean x
   and 1-NN on the two bounded templates.

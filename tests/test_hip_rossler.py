@@ -103,8 +103,7 @@ def test_labels_match_oracle_long_horizon(cuda_device):
                          np.array([1000.0]), 0.0, rtol=1e-3, atol=1e-6, bits=32, threads=8)
     fin = ref[-1]
     bounded = (st == 0) & np.isfinite(fin).all(1) & (np.abs(fin).max(1) < 400)
-    dev = O.sync_features(fin, 100)[:, 3]
-    ref_lab = np.where(~bounded, "unbounded", np.where(dev < 1.5, "synchronized", "desynchronized"))
+    ref_lab = O.sync_labels(O.sync_features(np.nan_to_num(fin), 100), ~bounded)
     agree = (lab == ref_lab).mean()
     print(f"[result] rossler labels agree {agree:.3f}; hip {dict(zip(*np.unique(lab, return_counts=True)))} "
           f"oracle {dict(zip(*np.unique(ref_lab, return_counts=True)))}", flush=True)

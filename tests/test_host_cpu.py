@@ -322,3 +322,68 @@ def test_duffing_stage_sincos_coefficients_are_within_bound():
     # a dropped or sign-flipped term is far outside the bound
     bad = (d + (d * x).astype(f) * horner([p[0], p[1], -p[2]])).astype(f)
     assert (np.abs(bad - np.sin(dd)) / np.spacing(np.abs(np.sin(dd)).astype(f))).max() > 10
+
+
+def test_fused_result_lookup_by_tensor_identity():
+    """A fused result registered for a trajectory tensor is found again from a
+    Solution holding that same tensor (the reference estimator passes y on
+    unchanged, basin_stability_estimator.py:282,307), and a tensor with equal
+    values is a different key."""
+    from pybasin_amd.case_features import PendulumFeatureExtractor
+    from pybasin_amd.solvers import FUSED_RESULTS, FusedResult
+
+    t = torch.linspace(0, 10, 101)
+    y = torch.stack([torch.sin(t), torch.cos(t)], -1)[:, None, :].repeat(1, 4, 1).contiguous()
+    stats = stats_from_window(y[50:]).clone()
+    stats[:, 0, 3] = 7.0  # a marker the trajectory itself does not give
+    res = FusedResult(t=t, y=y, final=y[-1], stats=stats, status=torch.zeros(4, dtype=torch.int32),
+                      nsteps=torch.ones(4, dtype=torch.int64), steady_idx=50)
+    FUSED_RESULTS[y] = res
+    other = y.clone()
+    fx = HipFusedFeatureExtractor(time_steady=5.0, features=None,
+                                  features_per_state={0: {"maximum": None}})
+    f = fx.extract_features(Solution(y[0], t, y))
+    assert torch.equal(f[:, 0], torch.full((4,), 7.0))
+    f_other = fx.extract_features(Solution(other[0], t, other))
+    assert torch.allclose(f_other[:, 0], y[50:, :, 0].max(0).values)
+    assert FUSED_RESULTS.get(other) is None
+    # the case-study extractors look their results up the same way
+    PendulumFeatureExtractor(time_steady=5.0).extract_features(Solution(y[0], t, y))
+    del FUSED_RESULTS[y]
+
+
+class _NoParamStrODE(ODESystem):
+    """An ODE whose description carries no parameter values (as the
+    reference's LorenzODE.get_str, lorenz_ode.py:66-72)."""
+
+    def ode(self, t, y, p):
+        return p[..., 0:1] * y
+
+    def get_str(self):
+        return "decay without parameters in its description"
+
+
+def test_cache_key_covers_ode_parameters(tmp_path):
+    s = TorchDopri5Solver(t_span=(0, 1), t_steps=5, device="cpu", cache_dir=str(tmp_path))
+    y0 = torch.tensor([[1.0]])
+    _, ya = s.integrate(_NoParamStrODE({"decay": -1.0}), y0)
+    _, yb = s.integrate(_NoParamStrODE({"decay": -2.0}), y0)
+    assert len(list(tmp_path.iterdir())) == 2
+    assert not torch.allclose(ya, yb)
+    _, ya2 = s.integrate(_NoParamStrODE({"decay": -1.0}), y0)
+    assert torch.equal(ya, ya2) and len(list(tmp_path.iterdir())) == 2
+
+
+def test_zscore_keeps_constant_columns_at_unit_scale():
+    """A column constant in the first batch keeps std 1 (the reference's
+    torch.where(std == 0, 1, std), torch_feature_extractor.py:273-281), so a
+    later batch that differs there is not scaled up."""
+    t = torch.linspace(0, 10, 101)
+    y = torch.stack([torch.sin(t), torch.full_like(t, 2.0)], -1)[:, None, :].repeat(1, 3, 1)
+    y[:, 1, 0] *= 2
+    fx = StatFeatureExtractor(time_steady=5.0, features={"mean": None}, normalize=True)
+    fx.extract_features(Solution(y[0], t, y))
+    y2 = y.clone()
+    y2[:, :, 1] = 3.0
+    f2 = fx.extract_features(Solution(y2[0], t, y2))
+    assert torch.allclose(f2[:, 1], torch.full((3,), 1.0))

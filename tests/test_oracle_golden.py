@@ -240,3 +240,100 @@ def test_label_fixture_ics_regenerate_from_seed():
     assert counts == {"FP": 602, "LC": 3398}
     bs = O.basin_stability(np.array(pc["labels"]))
     assert bs["FP"][0] == pytest.approx(0.1505) and bs["FP"][1] == pytest.approx(0.005653533187308623)
+
+
+# ---- helper pins: known answers a plausible slip would miss ------------------
+
+def _grid(a, b, n, dtype=np.float32):
+    import torch
+
+    return torch.linspace(a, b, n, dtype=torch.float32 if dtype == np.float32 else torch.float64).numpy()
+
+
+def test_filter_time_idx_known_answers():
+    from pybasin_amd.solvers.hip_dopri5 import steady_index
+
+    g11 = _grid(0.0, 10.0, 11)  # 0, 1, ..., 10 exactly
+    g100 = _grid(0.0, 10.0, 100)  # step 10/99
+    g21 = _grid(900.0, 1000.0, 21)  # step 5
+    cases = [
+        # the reference's docstring example (feature_extractor.py:110-114): a
+        # save time exactly at time_steady is kept (>=, not >)
+        (g11, 9.0, 9),
+        # the reference's unit test (tests/unit/test_feature_extractor.py:72-80):
+        # i * 10 / 99 >= 5 first at i = 50, 50 steps kept
+        (g100, 5.0, 50),
+        # default: t0 + 0.85 (t1 - t0) of the save grid (not of t_span):
+        # 900 + 85 = 985 lies on the grid at index 17
+        (g21, None, 17),
+        (g11, None, 9),  # 8.5 -> first kept 9
+        # past the end: no index qualifies, the whole window is kept
+        (g11, 20.0, 0),
+        (g11, 0.0, 0),
+        (g11, -3.0, 0),
+        # the comparison runs in float32: 9.0000001 rounds to 9.0f and keeps 9
+        (g11, 9.0000001, 9),
+    ]
+    import torch
+
+    for grid, ts, want in cases:
+        assert O.filter_time_idx(grid, ts) == want, (grid[:3], ts, want)
+        assert steady_index(torch.from_numpy(grid), ts) == want, (grid[:3], ts, want)
+    # a float64 grid compares in float64: 9.0000001 > 9.0
+    assert O.filter_time_idx(_grid(0.0, 10.0, 11, np.float64), 9.0000001) == 10
+
+
+def _sync_rows():
+    N = 100
+    rows, want = [], []
+    # row 0: x = 0.00 .. 0.99 (spread 0.99), y constant, z constant except the
+    # last z node (index 3N - 1) raised by 3.5
+    r = np.zeros(3 * N)
+    r[:N] = np.arange(N) * 0.01
+    r[N:2 * N] = 2.0
+    r[2 * N:] = 5.0
+    r[3 * N - 1] = 8.5
+    rows.append(r)
+    want.append([0.99, 0.0, 3.5, 3.5])
+    # row 1: fully synchronised
+    r = np.concatenate([np.full(N, 1.0), np.full(N, -2.0), np.full(N, 0.3)])
+    rows.append(r)
+    want.append([0.0, 0.0, 0.0, 0.0])
+    # row 2: outliers on the block edges: last x node, first y node
+    r = np.zeros(3 * N)
+    r[N - 1] = 0.7
+    r[N] = -1.2
+    rows.append(r)
+    want.append([0.7, 1.2, 0.0, 1.2])
+    # row 3: spread exactly epsilon (1.5) -> desynchronised (strict <)
+    r = np.zeros(3 * N)
+    r[5] = 1.5
+    rows.append(r)
+    want.append([1.5, 0.0, 0.0, 1.5])
+    return np.array(rows), np.array(want)
+
+
+def test_sync_features_closed_forms():
+    rows, want = _sync_rows()
+    np.testing.assert_allclose(O.sync_features(rows, 100), want, rtol=0, atol=1e-12)
+    lab = O.sync_labels(O.sync_features(rows, 100), np.array([False, False, False, False]))
+    assert lab.tolist() == ["desynchronized", "synchronized", "synchronized", "desynchronized"]
+    lab = O.sync_labels(O.sync_features(rows, 100), np.array([False, True, False, False]))
+    assert lab[1] == "unbounded"
+
+
+def test_product_sync_features_equal_closed_forms():
+    """The product's restatement (pybasin_amd.solvers.rossler) on the same rows."""
+    import torch
+
+    from pybasin_amd.solvers.rossler import sync_features
+
+    rows, want = _sync_rows()
+    got = sync_features(torch.from_numpy(rows)).numpy()
+    np.testing.assert_allclose(got, want, rtol=0, atol=1e-12)
+
+
+def test_basin_stability_closed_form():
+    bs = O.basin_stability(np.array(["a"] * 3 + ["b"] * 1))
+    assert bs["a"] == (0.75, pytest.approx(math.sqrt(0.75 * 0.25 / 4)))
+    assert bs["b"] == (0.25, pytest.approx(math.sqrt(0.75 * 0.25 / 4)))
