@@ -1,7 +1,7 @@
 """Full-size workloads on the GPU, checked through size-independent properties.
 
-At 100,000 Duffing and 200,000 Lorenz ICs the oracle would take minutes, so
-these tests check what must hold at any size: each IC's result does not
+IC-by-IC comparisons with the oracle at these sizes are in
+test_hip_parity_full.py; these tests check what must hold at any size: each IC's result does not
 depend on where it sits in the batch or on the launch shape (bitwise), the
 float32 and float64 runs agree on the labels away from basin boundaries, and
 the basin-stability fractions agree with the reference's own 4,000-sample
@@ -101,7 +101,7 @@ def test_duffing_100k_float32_and_float64_labels_agree(cuda_device):
     print(f"[result] duffing 100k float32 vs float64 label agreement {agree:.5f}", flush=True)
     # ICs whose label flips with the working precision sit on basin
     # boundaries of this multistable system; they are a small minority
-    assert agree >= 0.99
+    assert agree >= 0.995
 
 
 def test_lorenz_200k_launch_shapes_agree_and_reference_fractions(cuda_device):

@@ -102,11 +102,11 @@ def test_lorenz_labels(cuda_device, fixtures):
     lab = KNNTemplateClassifier(["chaotic attractor 1", "chaotic attractor 2"]).fit(feats[-2:]).predict(feats[:-2])
     lab = np.where(status[:-2] == _lib.BH_LANE_DIVERGED, "unbounded", lab)
     agree, worst = _report("lorenz", lab, fx["labels"])
-    # bounded attractors are chaotic: their identity is robust, but unbounded
-    # vs bounded must match exactly
+    # the bounded attractors are chaotic, yet which one an IC reaches is
+    # robust (boundary ICs are 0.1 % of this set: tests/test_hip_parity_full.py)
     assert ((lab == "unbounded") == (np.asarray(fx["labels"]) == "unbounded")).mean() >= 0.999
-    assert agree >= 0.99
-    assert worst <= 2.0
+    assert agree >= 0.995
+    assert worst <= 1.0
 
 
 def _mcc(got, want):

@@ -91,26 +91,120 @@ def test_ic_major_sweep_and_stop(cuda_device):
     assert f.shape == (B * len(Ks), 4)
 
 
-def test_labels_match_oracle_long_horizon(cuda_device):
-    """t in [0, 1000], rtol 1e-3 (the reference setup): synchronized /
-    desynchronized / unbounded labels against the oracle on the same ICs."""
-    B, K = 48, 0.159
-    y0 = torch.tensor(O.sample_uniform(B, LO, HI, seed=42))
-    res = _run(y0.to(cuda_device), torch.tensor([[0.2, 0.2, 7.0, K]], dtype=torch.float64), 1000.0,
-               1e-3, 1e-6, stop_threshold=400.0)
-    lab = sync_labels(res.final, res.status)
-    ref, st, _ = O.solve(O.SYS_ROSSLER, y0.double().numpy(), np.array([0.2, 0.2, 7.0, K]),
-                         np.array([1000.0]), 0.0, rtol=1e-3, atol=1e-6, bits=32, threads=8)
+# rounding- and tolerance-level perturbations of the oracle run that decide
+# which ICs lie on a chaotic boundary (bits, rtol, atol); the first is the
+# configured run
+PERTURBED = [(64, 1e-3, 1e-6), (64, 1e-4, 1e-7), (32, 1e-3, 1e-6)] + [
+    (bits, 1e-3 * (1 + d), 1e-6) for d in (-0.002, 0.002, -0.004, 0.004, -0.006, 0.006)
+    for bits in (32, 64)] + [(32, 1e-3, 1e-6 * (1 + d)) for d in (-0.002, 0.002, -0.004, 0.004)]
+
+
+def _oracle_labels(y0, params, bits, rtol, atol):
+    ref, st, _ = O.solve(O.SYS_ROSSLER, y0, params, np.array([1000.0]), 0.0, rtol=rtol, atol=atol,
+                         bits=bits, threads=max(1, min(os.cpu_count() or 1, 16)))
     fin = ref[-1]
-    bounded = (st == 0) & np.isfinite(fin).all(1) & (np.abs(fin).max(1) < 400)
-    ref_lab = O.sync_labels(O.sync_features(np.nan_to_num(fin), 100), ~bounded)
-    agree = (lab == ref_lab).mean()
-    print(f"[result] rossler labels agree {agree:.3f}; hip {dict(zip(*np.unique(lab, return_counts=True)))} "
-          f"oracle {dict(zip(*np.unique(ref_lab, return_counts=True)))}", flush=True)
-    # unbounded vs bounded must agree exactly; sync vs desync may flip only for
-    # trajectories still converging at t = 1000
-    assert ((lab == "unbounded") == (ref_lab == "unbounded")).all()
-    assert agree >= 0.9
+    unbounded = (st != 0) | ~np.isfinite(fin).all(1) | (np.abs(fin).max(1) >= 400)
+    return O.sync_labels(O.sync_features(np.nan_to_num(fin), 100), unbounded)
+
+
+def test_labels_match_oracle_long_horizon(cuda_device):
+    """t in [0, 1000], rtol 1e-3 (the reference setup), 100 ICs x the 11
+    coupling strengths of the sweep: synchronized / desynchronized /
+    unbounded labels against the oracle on the same ICs.
+
+    Whether a trajectory has synchronised by t = 1000 depends, for about a
+    tenth of these ICs, on how long a chaotic transient lasts, which any
+    rounding-level change decides anew: these are the chaotic-boundary ICs
+    of this system.  An IC counts as one when its label is not the same in
+    all of the perturbed oracle runs (float32 / float64, rtol 1e-3 / 1e-4,
+    tolerances moved by up to 0.6 %).  Off the boundary the kernel's labels
+    must agree on >= 99.5 % of the ICs.  Each basin-stability fraction must
+    lie within its binomial standard error of the perturbed runs' mean, and
+    within the range those runs span widened by one standard error: on this
+    system a label's noise under rounding is as large as the sampling error,
+    so any single run, the oracle's included, lands about one standard error
+    from another, and the ensemble mean is the fraction to compare with."""
+    import math
+
+    B = 100
+    Ks = np.linspace(0.119, 0.159, 11)
+    y0 = O.sample_uniform(B, LO, HI, seed=42)
+    params = np.array([[0.2, 0.2, 7.0, k] for k in Ks])
+    res = _run(torch.tensor(y0).to(cuda_device), torch.tensor(params), 1000.0, 1e-3, 1e-6,
+               stop_threshold=400.0)
+    lab = sync_labels(res.final, res.status)
+    rows = np.repeat(y0.astype(np.float64), len(Ks), axis=0)  # IC-major (base.py:155-159)
+    prow = np.tile(params, (B, 1))
+    runs = np.array([_oracle_labels(rows, prow, *p) for p in PERTURBED])
+    ref = runs[0]
+    boundary = (runs != ref).any(0)
+    off = ~boundary
+    agree = float((lab[off] == ref[off]).mean())
+    classes = ("synchronized", "desynchronized", "unbounded")
+    n = len(ref)
+    print(f"[result] rossler labels: boundary {int(boundary.sum())}/{n}; agreement off the boundary "
+          f"{agree:.4f} (all {float((lab == ref).mean()):.4f}); hip "
+          f"{dict(zip(*np.unique(lab, return_counts=True)))} oracle "
+          f"{dict(zip(*np.unique(ref, return_counts=True)))}", flush=True)
+    for c in classes:
+        fr = (runs == c).mean(1)
+        got = float(np.mean(lab == c))
+        se = math.sqrt(max(got * (1 - got), 1e-300) / n)
+        print(f"[result] rossler BS {c}: hip {got:.4f}, perturbed oracle runs mean {fr.mean():.4f} "
+              f"range {fr.min():.4f}..{fr.max():.4f}, SE {se:.4f}", flush=True)
+        assert abs(got - fr.mean()) <= se
+        assert fr.min() - se <= got <= fr.max() + se
+    # unbounded vs bounded is not a boundary question
+    assert ((lab == "unbounded") == (ref == "unbounded")).all()
+    assert agree >= 0.995
+
+
+def test_k_sweep_as_configured_matches_paper_and_reference(cuda_device):
+    """The configured sweep: 11 coupling strengths K = linspace(0.119, 0.159,
+    11) x 10,000 ICs in one launch (110,000 trajectories, IC-major), t in
+    [0, 1000], rtol 1e-3, atol 1e-6, stop at |y|inf >= 400.  Three of the K
+    values are on the paper's grid (setup_rossler_network_system.py:25-31):
+    the bounded fraction (synchronized + desynchronized, as the reference's
+    test_rossler_network.py:213 sums them) against the paper's S_B with the
+    reference's z-test and threshold 3 (paper SE at the reference's n = 500,
+    test_rossler_network.py:78-80,99,189), and synchronized / unbounded
+    against the reference's own n = 500 runs
+    (artifacts/results/rossler_network_k_sweep_comparison.json)."""
+    import json as _json
+    import math
+
+    with open(os.path.join(GOLD, "rossler_golden.json")) as f:
+        gold = _json.load(f)
+    paper = dict(zip([round(k, 3) for k in gold["K"]], gold["S_B_paper"]))
+    ref = {round(r["parameter_value"], 3): {a["label"]: a["python_bs"] for a in r["attractors"]}
+           for r in gold["reference_run_n500"]["parameter_results"]}
+    B = 10_000
+    Ks = np.linspace(0.119, 0.159, 11)
+    torch.manual_seed(0)
+    lo, hi = torch.tensor(LO), torch.tensor(HI)
+    y0 = (torch.rand(B, 300) * (hi - lo) + lo).to(cuda_device)
+    params = torch.tensor([[0.2, 0.2, 7.0, k] for k in Ks], dtype=torch.float64)
+    print("[case] rossler sweep 11 K x 10,000 ICs", flush=True)
+    res = _run(y0, params, 1000.0, 1e-3, 1e-6, stop_threshold=400.0)
+    lab = sync_labels(res.final, res.status)
+    assert res.final.shape == (B * len(Ks), 300)
+    for i, k in enumerate(Ks):
+        kk = round(float(k), 3)
+        li = lab[i::len(Ks)]
+        bounded = float(np.mean(li != "unbounded"))
+        line = f"[result] K={kk}: sync {np.mean(li == 'synchronized'):.4f} bounded {bounded:.4f}"
+        if kk in paper:
+            sb = paper[kk]
+            z = abs(bounded - sb) / math.sqrt(bounded * (1 - bounded) / B + sb * (1 - sb) / 500)
+            line += f"; paper S_B {sb:.3f} z={z:.2f}"
+            assert z < 3.0
+        if kk in ref:
+            for name in ("synchronized", "unbounded"):
+                p1, p2 = float(np.mean(li == name)), ref[kk][name]
+                z = abs(p1 - p2) / max(math.sqrt(p1 * (1 - p1) / B + p2 * (1 - p2) / 500), 1e-12)
+                line += f"; {name} vs reference n=500 {p2:.3f} z={z:.2f}"
+                assert z < 3.0
+        print(line, flush=True)
 
 
 def test_k_sweep_matches_reference_fractions(cuda_device):
