@@ -1,19 +1,28 @@
 #!/usr/bin/env python3
 """Benchmark: classified initial conditions per second on one MI355X.
 
-Default workload (one "step" = one full basin-stability pass over the batch):
-Duffing oscillator x' = v, v' = -0.08 v - x^3 + 0.2 cos t, n = 100,000 ICs
-seeded-uniform in x in [-1, 1], v in [-0.5, 1]; t in [0, 1000], rtol 1e-8,
-atol 1e-6, float32; save grid 1,000 points on [900, 1000]; fused
-mean / std / min / max per state over that window; 1-NN classification
-against the 5 template ICs of the reference's Duffing setup, integrated in
-the same launch.  Inputs are resident in HBM before timing starts.  The
-launch shape is chosen once by the project's autotuner before the timed
-steps (reported with its timings; --no-tune keeps one launch with every
-trajectory on its own lane).
+Default workload (the metric's own configuration): the pendulum torque sweep
+through the learned surrogate with Dopri5 fallback.  46 torques T uniformly
+in [0.1, 1.0] with per-torque counts np.random.default_rng(0).integers(
+10_000, 200_001, 46) (4,431,800 ICs, theta in [-pi + asin T, pi + asin T],
+omega in [-10, 10]), alpha 0.1, K 1, t in [0, 1000], rtol 1e-8, atol 1e-6,
+float32; labels FP / LC from the window [950, 1000].  The network (trained
+once, outside the timed steps, by the project's kernels on 50,000
+integrator-labelled ICs) labels confident ICs; the rest go to the per-lane
+Dopri5 kernel; a 2 % audit per torque is integrated over the full span.  A
+step is one gated pass over all 4,431,800 ICs; `value` is the pass rate with
+two passes in flight (each on its own stream, as a server would overlap the
+latency-bound tail of one pass with the next), and the plain path (full
+integration of every IC, with the same passes in flight and its own tuned
+launch shape) is timed in the same run; the basin-stability error of the
+gated labels against the plain labels is measured, not assumed.  The plain
+Duffing workload (100,000 ICs, float32) is measured as well and reported in
+`config.duffing_100k`.
 
-Prints one JSON line (rank 0).  --gpus above 1 is not supported: this build
-targets one GPU per process and runs no collectives.
+Other workloads: --workload duffing | lorenz | rossler | mlp_infer | hdbscan
+| estimator | estimator_defaults.  Inputs are resident in HBM before timing
+starts.  Prints one JSON line.  --gpus above 1 is not supported: this build
+targets one GPU and runs no collectives.
 """
 
 from __future__ import annotations
@@ -23,6 +32,7 @@ import json
 import math
 import os
 import sys
+import threading
 import time
 
 import torch
@@ -32,12 +42,15 @@ sys.path.insert(0, ROOT)
 
 PEAK_FP32_VALU = 157.3e12  # MI355X FP32 vector peak (packed), FLOP/s
 PEAK_FP64_VALU = 78.6e12
+PEAK_BF16_MFMA = 2500e12   # dense
 
 # flops per attempted step: 6 F_rhs + 63 S + 16 (RK combination, error norm,
-# controller); per save point: dense-output fit 36 S + evaluation 9 S +
-# streaming statistics 6 S
+# controller); a step whose span holds a save point adds the dense-output fit
+# (36 S); each save point adds its evaluation (9 S) and the streaming
+# statistics (6 S) when it lies in the feature window
 F_RHS = {"duffing": 8, "pendulum": 5, "lorenz": 9}
 STATE = {"duffing": 2, "pendulum": 2, "lorenz": 3}
+MLP_MACS = 6 * 256 + 256 * 256 * 2 + 256 * 2  # multiply-adds per row (unpadded layers)
 
 # HBM bytes per dispatch of the Duffing 100k float32 kernel, from rocprofv3
 # FETCH_SIZE + WRITE_SIZE scaled by a plain-copy calibration
@@ -61,7 +74,7 @@ def _label_counts(labels: torch.Tensor, k: int) -> torch.Tensor:
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=100_000)
     ap.add_argument("--dtype", choices=["f32", "f64"], default="f32")
@@ -70,26 +83,25 @@ def parse():
     ap.add_argument("--phase-iters", type=int, default=0,
                     help="duffing, lorenz: two-phase launch with this step budget instead of the tuner")
     ap.add_argument("--no-tune", action="store_true",
-                    help="duffing: skip the launch-shape tuner (every trajectory on its own lane, "
-                         "one launch)")
-    ap.add_argument("--cpu-sample", type=int, default=100_000)
-    ap.add_argument("--no-cpu", action="store_true")
+                    help="skip the launch-shape tuner (every trajectory on its own lane, one launch)")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baselines")
     ap.add_argument("--keep-traj", action="store_true",
                     help="duffing: also write the full trajectories to HBM (reports write GB/s)")
-    ap.add_argument("--ref-sample", type=int, default=64,
-                    help="ICs for the scipy path (64x for the torch shared-step path); 0 skips them")
-    ap.add_argument("--workload", choices=["duffing", "lorenz", "surrogate", "rossler", "mlp_infer",
+    ap.add_argument("--workload", choices=["surrogate", "duffing", "lorenz", "rossler", "mlp_infer",
                                            "hdbscan", "estimator", "estimator_defaults"],
-                    default="duffing",
-                    help="duffing: plain HIP Dopri5 on 100k Duffing ICs (default); surrogate: "
-                         "pendulum torque sweep through the MLP gate with Dopri5 fallback")
-    ap.add_argument("--gate", type=float, default=0.999)
+                    default="surrogate")
+    ap.add_argument("--gate", type=float, default=0.999,
+                    help="surrogate: confidence gate of the headline pass")
+    ap.add_argument("--config-gate", type=float, default=0.95,
+                    help="surrogate: the configured gate, run and reported beside the headline")
+    ap.add_argument("--inflight", type=int, default=2,
+                    help="surrogate: passes in flight (each on its own stream and host thread)")
     ap.add_argument("--epochs", type=int, default=600)
-    ap.add_argument("--early-frac", type=float, default=0.0,
-                    help="surrogate: least confident fraction integrated with the audit")
     ap.add_argument("--evolve-t", type=float, default=50.0,
                     help="surrogate: below-gate ICs are integrated to this time and classified "
                          "again before a full integration (0 disables)")
+    ap.add_argument("--no-duffing", action="store_true",
+                    help="surrogate: skip the plain Duffing measurement in config.duffing_100k")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 256 MiB device copy (calibrates the "
                          "FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -106,34 +118,17 @@ def copy_calibration():
     print(json.dumps({"copy_calibration_bytes_each_way": n * 4, "copies": 3}), flush=True)
 
 
-def cpu_baseline(y0_cpu, save64, n_sample):
-    """The CPU oracle (per-lane Dopri5 in C, same algorithm and dtype) on the
-    first n_sample ICs, threads over the host cores."""
-    import numpy as np
+# ---- host cores and the CPU baselines ------------------------------------------
 
-    from oracle import oracle as O
-
-    threads = max(1, min(os.cpu_count() or 1, 16))
-    y = y0_cpu[:n_sample].double().numpy()
-    t0 = time.perf_counter()
-    O.solve(O.SYS_DUFFING, y, np.array([0.08, 1.0, 0.2]), save64, 0.0, rtol=1e-8, atol=1e-6,
-            bits=32, threads=threads)
-    dt = time.perf_counter() - t0
-    return {"value": round(n_sample / dt, 1), "unit": "ICs/s", "cores": threads, "kind": "port",
-            "sample": f"first {n_sample} of the same seeded Duffing ICs, full t in [0,1000], "
-                      f"float32, oracle per-lane Dopri5 (C, pthreads), {dt:.2f} s"}
-
-
-def _duffing_rhs(t, y):
-    return [y[1], -0.08 * y[1] - y[0] ** 3 + 0.2 * math.cos(t)]
-
-
-def _scipy_one(y0, save):
-    from scipy.integrate import solve_ivp
-
-    r = solve_ivp(_duffing_rhs, (0.0, 1000.0), y0, method="RK45", rtol=1e-8, atol=1e-6,
-                  max_step=10.0, t_eval=save)
-    return r.y[:, -1]
+def host_cores() -> dict:
+    """Cores this process may use, and what os.cpu_count() reports.  Thread
+    and process pools use the cores the process is allowed to run on, at most
+    16 (the GPU box grants one GPU's job a 16-CPU share)."""
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    return {"os_cpu_count": os.cpu_count(), "affinity": avail, "used": max(1, min(avail, 16))}
 
 
 def cpu_model() -> str:
@@ -147,142 +142,333 @@ def cpu_model() -> str:
     return "unknown"
 
 
-def cpu_reference_paths(y0_cpu, save64, n_scipy: int, n_torch: int):
-    """The reference's two CPU solver paths, restated here (the reference
-    itself is not importable), timed on samples of the same ICs before the
-    GPU is touched: ScipyParallelSolver (solve_ivp RK45 per IC over a process
-    pool, max_step = (t1 - t0) / 100, scipy_solver.py:141-151) and
-    TorchDiffEqSolver (one shared-step Dopri5 over the batch in torch on the
-    CPU, torchdiffeq_solver.py:97-134)."""
+def _rhs_scipy(sysname, p):
+    if sysname == "duffing":
+        d, k3, A = p
+        return lambda t, y: [y[1], -d * y[1] - k3 * y[0] ** 3 + A * math.cos(t)]
+    if sysname == "pendulum":
+        a, T, K = p
+        return lambda t, y: [y[1], -a * y[1] + T - K * math.sin(y[0])]
+    s, r, b = p
+
+    def f(t, y):
+        if max(abs(y[0]), abs(y[1]), abs(y[2])) >= 200.0:  # lorenz_ode.py:58-64 freeze mask
+            return [0.0, 0.0, 0.0]
+        return [s * (y[1] - y[0]), r * y[0] - y[0] * y[2] - y[1], y[0] * y[1] - b * y[2]]
+    return f
+
+
+def _scipy_one(sysname, y0, p, save, t1, rtol, atol):
+    from scipy.integrate import solve_ivp
+
+    r = solve_ivp(_rhs_scipy(sysname, p), (0.0, t1), y0, method="RK45", rtol=rtol, atol=atol,
+                  max_step=t1 / 100.0, t_eval=save)
+    return r.y[:, -1]
+
+
+def cpu_baselines(sysname, y0_cpu, params_rows, save_ts, *, n_torch=4096, n_scipy=64, n_oracle=20_000,
+                  rtol=1e-8, atol=1e-6, t1=1000.0):
+    """The reference's CPU paths on seeded samples of the workload's own rows
+    (same float32 ICs and per-row parameters), timed on the host cores.
+
+    * TorchDiffEqSolver counterpart (torchdiffeq_solver.py:97-134): one
+      shared-step Dopri5 over the sample in torch on the CPU (this package's
+      TorchDopri5Solver, torchdiffeq's algorithm), float32, the whole span
+      with the workload's save grid -- the cpu_baseline.
+    * ScipyParallelSolver counterpart (scipy_solver.py:113-169): solve_ivp
+      RK45 per IC over a process pool, max_step = (t1 - t0) / 100, float64.
+    * The oracle: per-lane Dopri5 in C over threads (the same algorithm as
+      the kernel), float32.
+    Returns (cpu_baseline, [the other two])."""
     import multiprocessing as mp
 
-    from pybasin_amd.solvers import TorchDopri5Solver
-    from pybasin_amd.systems import DuffingODE
+    import numpy as np
 
-    threads = max(1, min(os.cpu_count() or 1, 16))
-    out = []
+    from oracle import oracle as O
+    from pybasin_amd.solvers import TorchDopri5Solver
+    from pybasin_amd.systems import DuffingODE, LorenzODE, PendulumODE
+
+    cores = host_cores()
+    used = cores["used"]
+    ode = {"duffing": DuffingODE({"delta": 0.08, "k3": 1.0, "A": 0.2}),
+           "pendulum": PendulumODE({"alpha": 0.1, "T": 0.5, "K": 1.0}),
+           "lorenz": LorenzODE({"sigma": 0.12, "r": 0.0, "b": -0.6})}[sysname]
+    sid = {"duffing": O.SYS_DUFFING, "pendulum": O.SYS_PENDULUM, "lorenz": O.SYS_LORENZ}[sysname]
+    save32 = save_ts.to(torch.float32).cpu()
+    save64 = save32.double().numpy()
+    others = []
+    # ScipyParallelSolver counterpart (before torch spins up its thread pool)
     ys = y0_cpu[:n_scipy].double().numpy()
+    ps = params_rows[:n_scipy].double().numpy()
     t0 = time.perf_counter()
-    with mp.get_context("fork").Pool(threads) as pool:  # no GPU state exists yet
-        pool.starmap(_scipy_one, [(y, save64) for y in ys])
+    with mp.get_context("fork").Pool(used) as pool:  # the GPU is not initialised in this process yet
+        pool.starmap(_scipy_one, [(sysname, y, p, save64, t1, rtol, atol) for y, p in zip(ys, ps)])
     dt = time.perf_counter() - t0
-    out.append({"path": "ScipyParallelSolver counterpart (solve_ivp RK45 per IC, process pool)",
-                "value": round(n_scipy / dt, 1), "unit": "ICs/s", "cores": threads,
-                "sample": f"first {n_scipy} ICs, float64, full t in [0,1000], {dt:.2f} s"})
-    torch.set_num_threads(threads)
-    solver = TorchDopri5Solver(t_span=(0, 1000), t_steps=1000, t_eval=(900.0, 1000.0), device="cpu",
-                               rtol=1e-8, atol=1e-6, cache_dir=None)
+    others.append({"path": "ScipyParallelSolver counterpart (solve_ivp RK45 per IC, process pool)",
+                   "value": round(n_scipy / dt, 1), "unit": "ICs/s", "cores": used,
+                   "sample": f"first {n_scipy} rows, float64, t in [0,{t1:g}], {dt:.2f} s"})
+    # TorchDiffEqSolver counterpart
+    torch.set_num_threads(used)
+    solver = TorchDopri5Solver(t_span=(0, t1), t_steps=len(save32), device="cpu", rtol=rtol, atol=atol,
+                               cache_dir=None)
+    yt = y0_cpu[:n_torch].to(torch.float32).contiguous()
+    pt = params_rows[:n_torch].to(torch.float32).contiguous()
     t0 = time.perf_counter()
-    solver.integrate(DuffingODE({"delta": 0.08, "k3": 1.0, "A": 0.2}), y0_cpu[:n_torch].clone())
+    solver._integrate(ode, yt, save32, pt)
     dt = time.perf_counter() - t0
-    out.append({"path": "TorchDiffEqSolver counterpart (shared-step Dopri5, torch CPU)",
-                "value": round(n_torch / dt, 1), "unit": "ICs/s", "cores": threads,
-                "sample": f"first {n_torch} ICs, float32, full t in [0,1000], {dt:.2f} s"})
-    return out
+    base = {"value": round(n_torch / dt, 1), "unit": "ICs/s", "cores": used, "kind": "port",
+            "path": "TorchDiffEqSolver counterpart (shared-step Dopri5, torch CPU)",
+            "sample": f"first {n_torch} rows of the workload (seeded), float32, t in [0,{t1:g}], "
+                      f"{len(save32)} saves, one shared-step batch: {dt:.2f} s",
+            "cpu_model": cpu_model(), "os_cpu_count": cores["os_cpu_count"],
+            "affinity_cores": cores["affinity"]}
+    # the oracle, per lane
+    yo = y0_cpu[:n_oracle].double().numpy()
+    po = params_rows[:n_oracle].double().numpy()
+    t0 = time.perf_counter()
+    O.solve(sid, yo, po, save64, 0.0, rtol=rtol, atol=atol, bits=32, threads=used, t1=None)
+    dt = time.perf_counter() - t0
+    others.append({"path": "oracle (per-lane Dopri5 in C, pthreads; the kernel's algorithm)",
+                   "value": round(len(yo) / dt, 1), "unit": "ICs/s", "cores": used,
+                   "sample": f"first {len(yo)} rows, float32, {dt:.2f} s"})
+    return base, others
+
+
+# ---- passes in flight -------------------------------------------------------------
+
+def run_passes(fn, k: int, inflight: int):
+    """k calls of fn with up to `inflight` in flight: each of `inflight` host
+    threads issues its share of the calls on a stream of its own.  Returns
+    (wall ms per call between two device-wide synchronisations, the last
+    result of the first thread)."""
+    streams = [torch.cuda.Stream() for _ in range(inflight)]
+    out = [None] * inflight
+    err = []
+
+    def worker(i):
+        try:
+            with torch.cuda.stream(streams[i]):
+                for _ in range(i, k, inflight):
+                    out[i] = fn()
+        except BaseException as e:  # surfaced after the join
+            err.append(e)
+
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    if inflight == 1:
+        worker(0)
+    else:
+        th = [threading.Thread(target=worker, args=(i,)) for i in range(inflight)]
+        for t in th:
+            t.start()
+        for t in th:
+            t.join()
+    torch.cuda.synchronize()
+    ms = 1e3 * (time.perf_counter() - t0) / k
+    if err:
+        raise err[0]
+    return ms, out[0]
+
+
+def _lane_flops(sysname, nsteps, save_hits, n_saved, window_saves):
+    """Algorithmic FLOP of a per-lane launch: attempted steps x F_step, steps
+    whose span holds a save x the dense-output fit (36 S), evaluated saves x
+    9 S, window saves x the statistics (6 S)."""
+    S = STATE[sysname]
+    return (float(nsteps) * (6 * F_RHS[sysname] + 63 * S + 16) + float(save_hits) * 36 * S
+            + float(n_saved) * 9 * S + float(window_saves) * 6 * S)
 
 
 def surrogate_workload(args):
-    """Pendulum torque sweep: 46 T in [0.1, 1.0], uneven counts (~4.4M ICs);
-    the MLP (trained once on 50,000 integrator labels) classifies confident ICs,
-    Dopri5 the rest plus a 2 % audit per T.  A step = one gated pass over all ICs."""
+    """The metric's configuration (see the module docstring)."""
     import numpy as np
 
     from pybasin_amd.solvers import HipDopri5Solver
-    from pybasin_amd.surrogate import (SurrogateMLP, basin_stability_by_group, classify_gated,
-                                       integrate_labels, torque_sweep_ics, tune_plain_launch)
+    from pybasin_amd.surrogate import (basin_stability_by_group, classify_gated, integrate_labels,
+                                       torque_sweep_ics, train_surrogate, tune_plain_launch)
     from pybasin_amd.systems import PendulumODE
 
     T = np.linspace(0.1, 1.0, 46)
     counts = np.random.default_rng(args.seed).integers(10_000, 200_001, 46)
-    x, group = torque_sweep_ics(T, counts, seed=args.seed)
+    x, group = torque_sweep_ics(T, counts, seed=args.seed, device="cpu")
     n = x.shape[0]
+    save_ts = torch.linspace(900.0, 1000.0, 1000, dtype=torch.float32)
+    base, others = None, None
+    if not args.no_cpu:
+        perm = torch.randperm(n, generator=torch.Generator().manual_seed(args.seed + 7))
+        xs = x[perm]
+        prow = torch.stack([torch.full((n,), 0.1), xs[:, 2], torch.full((n,), 1.0)], 1)
+        base, others = cpu_baselines("pendulum", xs[:, :2], prow, save_ts, n_torch=4096)
+        base["sample"] = "seeded random rows of the sweep (all torques): " + base["sample"]
+    x, group = x.to("cuda"), group.to("cuda")
     solver = HipDopri5Solver(t_span=(0, 1000), t_steps=1000, t_eval=(900.0, 1000.0), device="cuda",
                              rtol=1e-8, atol=1e-6, cache_dir=None)
     ode = PendulumODE({"alpha": 0.1, "T": 0.5, "K": 1.0})
     ts = 950.0
+    ev = args.evolve_t if args.evolve_t > 0 else None
     # one-time training on 50,000 integrator-labelled ICs
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    tidx = torch.randperm(n, generator=torch.Generator(device="cuda").manual_seed(1), device="cuda")[:50_000]
-    ylab = integrate_labels(solver, ode, x[tidx], ts)
-    model = SurrogateMLP(device="cuda:0", seed=args.seed)
-    hist = model.fit(x[tidx], ylab, epochs=args.epochs, batch=1024, lr=3e-3, seed=args.seed)
+    model, hist, _ = train_surrogate(solver, ode, x, n_train=50_000, epochs=args.epochs, seed=args.seed)
     torch.cuda.synchronize()
     t_train = time.perf_counter() - t0
-    # the plain path with its own tuned launch shape as well (the two-phase
-    # launch on these rows); the gated pass keeps the solver's default shape,
-    # which suits its latency-bound audit integrations
-    tuned = None
-    if not args.no_tune:
-        tsolver = solver.clone(cache_dir=None)
-        t0 = time.perf_counter()
-        launch = tune_plain_launch(tsolver, ode, x, ts)
-        launch["tune_s"] = round(time.perf_counter() - t0, 2)
-        integrate_labels(tsolver, ode, x, ts)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(args.steps):
-            tlab = integrate_labels(tsolver, ode, x, ts)
-        torch.cuda.synchronize()
-        tuned = (1e3 * (time.perf_counter() - t0) / args.steps, launch, tlab)
-    # plain path (reference fractions and its throughput)
-    for _ in range(max(1, args.warmup)):
-        plab = integrate_labels(solver, ode, x, ts)
-    torch.cuda.synchronize()
+    # the plain path: its default launch and its tuned launch shape
+    tsolver = solver.clone(cache_dir=None)
     t0 = time.perf_counter()
-    for _ in range(args.steps):
-        plab = integrate_labels(solver, ode, x, ts)
-    torch.cuda.synchronize()
-    ms_plain = 1e3 * (time.perf_counter() - t0) / args.steps
-    # gated path
-    ev = args.evolve_t if args.evolve_t > 0 else None
-    for _ in range(args.warmup):
-        classify_gated(model, solver, ode, x, gate=args.gate, group=group, time_steady=ts,
-                       evolve_t=ev, early_frac=args.early_frac, n_groups=len(T))
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        res = classify_gated(model, solver, ode, x, gate=args.gate, group=group, time_steady=ts,
-                             evolve_t=ev, early_frac=args.early_frac, n_groups=len(T))
-    torch.cuda.synchronize()
-    ms_gated = 1e3 * (time.perf_counter() - t0) / args.steps
-    stages = {}
-    classify_gated(model, solver, ode, x, gate=args.gate, group=group, time_steady=ts, evolve_t=ev,
-                   early_frac=args.early_frac, stage_times=stages)
-    bs_g = basin_stability_by_group(res.labels, group, 46)
+    launch = tune_plain_launch(tsolver, ode, x, ts) if not args.no_tune else {}
+    launch["tune_s"] = round(time.perf_counter() - t0, 2)
+    plain = {}
+    for name, sv in (("default", solver), ("tuned", tsolver)):
+        run_passes(lambda: integrate_labels(sv, ode, x, ts), max(1, args.warmup), 1)
+        for fl in sorted({1, args.inflight}):
+            plain[(name, fl)], lab = run_passes(lambda: integrate_labels(sv, ode, x, ts), args.steps, fl)
+        plain[(name, "labels")] = lab
+    plab = plain[("default", "labels")]
     bs_p = basin_stability_by_group(plab, group, 46)
-    max_err = float((bs_g - bs_p).abs().max())
+
+    def gated(gate, stage_times=None):
+        return classify_gated(model, solver, ode, x, gate=gate, group=group, time_steady=ts,
+                              evolve_t=ev, n_groups=46, stage_times=stage_times)
+
+    gated_ms = {}
+    for gate in (args.gate, args.config_gate):
+        run_passes(lambda: gated(gate), max(1, args.warmup), 1)
+        for fl in sorted({1, args.inflight}):
+            gated_ms[(gate, fl)], res = run_passes(lambda: gated(gate), args.steps, fl)
+        gated_ms[(gate, "res")] = res
+    res = gated_ms[(args.gate, "res")]
+    res_cfg = gated_ms[(args.config_gate, "res")]
+    err = (basin_stability_by_group(res.labels, group, 46) - bs_p).abs()
+    err_cfg = (basin_stability_by_group(res_cfg.labels, group, 46) - bs_p).abs()
     audit_err = float((res.audit_surrogate != res.audit_integrated).float().mean())
+    # one instrumented pass: stage times, the audit launch and the network pass
+    stages = {}
+    gated(args.gate, stages)
+    audit_steps = stages.pop("audit_nsteps")
+    audit_rows = stages.pop("audit_rows")
+    audit_ms = stages.pop("audit_launch_ms")
+    infer_ms = stages.pop("infer_launch_ms", None)
+    # the audit launch's algorithmic FLOP: its rows' attempted and save-hit
+    # step counts (the same trajectories re-solved with the count flag)
+    from pybasin_amd.solvers.hip_dopri5 import count_save_steps
+
+    xa = x[res.audit_idx]
+    pa = torch.stack([torch.full((len(xa),), 0.1, dtype=torch.float64, device=x.device), xa[:, 2].double(),
+                      torch.full((len(xa),), 1.0, dtype=torch.float64, device=x.device)], 1).contiguous()
+    sa, ya = solver._prepare_tensors(xa[:, :2].contiguous())
+    a_steps, a_hits = count_save_steps(solver, ode, ya, sa, pa, ts)
+    assert a_steps == int(audit_steps.sum()) and len(xa) == audit_rows
+    from pybasin_amd.solvers.hip_dopri5 import steady_index
+
+    win = 1000 - steady_index(sa, ts)
+    audit_flops = _lane_flops("pendulum", a_steps, a_hits, audit_rows * 1000, audit_rows * win)
+    # the plain pass's Dopri5 launch (tuned shape) timed on its stream, with
+    # save-hit step counts from the kernel
+    plain_roof = plain_launch_roofline(tsolver, ode, x, ts)
+    ms_g = gated_ms[(args.gate, args.inflight)]
+    ms_tuned = plain[("tuned", args.inflight)]
     out = {
         "metric": "classified initial conditions/sec at <=0.5% basin-stability error vs plain",
-        "value": round(n / (ms_gated * 1e-3), 1), "unit": "ICs/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_gated, 3),
+        "value": round(n / (ms_g * 1e-3), 1), "unit": "ICs/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_g, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+bf16",
-        "data": "synthetic: seeded-uniform ICs per torque value",
+        "data": "synthetic: seeded-uniform ICs per torque value; network trained on the "
+                "integrator's labels of 50,000 of them",
         "config": {
             "workload": "pendulum_torque_sweep_surrogate", "n_ics": n, "torques": 46,
-            "gate": args.gate, "routed_frac": round(float(res.routed.float().mean()), 4),
-            "group_routed_frac": round(float(res.group_routed.float().mean()), 4),
-            "evolve_t": ev, "early_frac": args.early_frac, "late_integrated": res.n_late,
-            "stage_ms": {k: round(v, 3) for k, v in stages.items()},
-            "evolved_frac": round(float(res.evolved.float().mean()), 4),
+            "counts": "np.random.default_rng(0).integers(10_000, 200_001, 46)",
+            "t_span": [0, 1000], "rtol": 1e-8, "atol": 1e-6, "label_window": [950, 1000],
+            "gate": args.gate, "inflight": args.inflight,
+            "max_bs_error_vs_plain": round(float(err.max()), 5),
+            "worst_torque": round(float(T[int(err.argmax())]), 3),
+            "ms_per_pass_one_in_flight": round(gated_ms[(args.gate, 1)], 3),
+            "plain_ms": {"default": round(plain[("default", args.inflight)], 3),
+                         "tuned": round(ms_tuned, 3),
+                         "default_one_in_flight": round(plain[("default", 1)], 3),
+                         "tuned_one_in_flight": round(plain[("tuned", 1)], 3)},
+            "plain_tuned_launch": launch,
+            "plain_tuned_labels_equal_default": bool(torch.equal(plain[("tuned", "labels")], plab)),
+            "speedup_vs_plain_tuned": round(ms_tuned / ms_g, 3),
+            "speedup_vs_plain_default": round(plain[("default", args.inflight)] / ms_g, 3),
+            "speedup_one_in_flight_vs_plain_tuned": round(plain[("tuned", 1)] / gated_ms[(args.gate, 1)], 3),
+            "routed_frac": round(float(res.routed.float().mean()), 4),
+            "evolve_t": ev, "evolved_frac": round(float(res.evolved.float().mean()), 4),
             "integrated_frac": round(float(res.integrated.float().mean()), 4),
-            "audit_evolved_label_error": res.audit_evolved_error,
-            "audit_n": len(res.audit_idx), "audit_label_error": round(audit_err, 5),
-            "max_bs_error_vs_plain": round(max_err, 5),
-            "plain_ics_per_s": round(n / (ms_plain * 1e-3), 1), "plain_ms": round(ms_plain, 3),
-            "speedup_vs_plain": round(ms_plain / ms_gated, 3),
+            "late_integrated": res.n_late,
+            "audit_n": len(res.audit_idx), "audit_network_label_error": round(audit_err, 5),
+            "stage_ms": {k: round(v, 3) for k, v in stages.items()},
+            "configured_gate": {
+                "gate": args.config_gate,
+                "max_bs_error_vs_plain": round(float(err_cfg.max()), 5),
+                "worst_torque": round(float(T[int(err_cfg.argmax())]), 3),
+                "within_0.5pct": bool(float(err_cfg.max()) <= 0.005),
+                "ms_per_pass": round(gated_ms[(args.config_gate, args.inflight)], 3),
+                "ms_per_pass_one_in_flight": round(gated_ms[(args.config_gate, 1)], 3),
+                "routed_frac": round(float(res_cfg.routed.float().mean()), 4)},
             "train_n": 50_000, "train_epochs": args.epochs, "train_s": round(t_train, 2),
             "final_train_loss": round(hist[-1], 4),
         },
+        "roofline": {
+            "bound": "valu", "kernel": "bh::dopri5_lane_kernel<Pendulum,float> (the audit launch: "
+                                       "the gated pass's longest kernel, on its critical path)",
+            "achieved": round(audit_flops / (audit_ms * 1e-3) / 1e12, 3), "peak": 157.3,
+            "unit": "TFLOP/s", "frac": round(audit_flops / (audit_ms * 1e-3) / PEAK_FP32_VALU, 4),
+            "traffic": None, "kernel_ms": round(audit_ms, 3), "rows": audit_rows,
+            "attempted_steps": a_steps, "save_hit_steps": a_hits,
+            "mean_steps": round(float(audit_steps.double().mean()), 1),
+            "max_steps": int(audit_steps.max())},
+        "roofline_plain": plain_roof,
     }
-    if tuned is not None:
-        ms_t, launch, tlab = tuned
-        out["config"]["plain_tuned"] = {
-            "ms": round(ms_t, 3), "ics_per_s": round(n / (ms_t * 1e-3), 1),
-            "speedup_vs_plain_tuned": round(ms_t / ms_gated, 3),
-            "labels_equal_plain": bool(torch.equal(tlab, plab)), "launch": launch}
+    if infer_ms is not None:
+        fl = 2.0 * MLP_MACS * n
+        out["roofline_network"] = {
+            "bound": "mfma", "kernel": "bh::mlp::mlp_infer_kernel (beside the audit)",
+            "achieved": round(fl / (infer_ms * 1e-3) / 1e12, 1), "peak": 2500.0, "unit": "TFLOP/s",
+            "frac": round(fl / (infer_ms * 1e-3) / PEAK_BF16_MFMA, 4), "kernel_ms": round(infer_ms, 3)}
+    if base is not None:
+        out["cpu_baseline"] = base
+        out["cpu_other_paths"] = others
+    if not args.no_duffing:
+        d = duffing_workload(args, cpu=False, emit=False)
+        out["config"]["duffing_100k"] = {
+            "value": d["value"], "unit": "ICs/s", "ms_per_step": d["ms_per_step"],
+            "launch": d["config"]["launch"], "roofline": d["roofline"]}
     print(json.dumps(out), flush=True)
+
+
+def plain_launch_roofline(solver, ode, x, time_steady, reps: int = 3):
+    """The plain pass's Dopri5 launch (the solver's launch shape) on all rows,
+    timed with HIP events on its stream, FLOP from the kernel's per-lane
+    attempted and save-hit step counts."""
+    from pybasin_amd.solvers.hip_dopri5 import count_save_steps
+
+    n = x.shape[0]
+    alpha, K = float(ode.params["alpha"]), float(ode.params["K"])
+    params = torch.stack([torch.full((n,), alpha, dtype=torch.float64, device=x.device),
+                          x[:, 2].double(), torch.full((n,), K, dtype=torch.float64, device=x.device)],
+                         1).contiguous()
+    save_ts, y0 = solver._prepare_tensors(x[:, :2].contiguous())
+    stream = torch.cuda.current_stream()
+    ms = []
+    for _ in range(reps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        res = solver.run_fused(ode, y0, save_ts, params, keep_trajectories=False,
+                               time_steady=time_steady, stream=stream)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms.append(e0.elapsed_time(e1))
+    steps, hits = count_save_steps(solver, ode, y0, save_ts, params, time_steady)
+    window = 1000 - int(res.steady_idx)
+    ok = int((res.status == 0).sum())
+    flops = _lane_flops("pendulum", steps, hits, ok * 1000, ok * window)
+    k = sum(ms) / len(ms)
+    kern = (f"bh::dopri5_phase_kernel<Pendulum,float> phases 1 + 2 (K = {solver.phase_iters})"
+            if solver.phase_iters else "bh::dopri5_lane_kernel<Pendulum,float>")
+    return {"bound": "valu", "kernel": kern, "achieved": round(flops / (k * 1e-3) / 1e12, 3),
+            "peak": 157.3, "unit": "TFLOP/s", "frac": round(flops / (k * 1e-3) / PEAK_FP32_VALU, 4),
+            "traffic": None, "kernel_ms": round(k, 3), "rows": n,
+            "attempted_steps": int(steps), "save_hit_steps": int(hits)}
 
 
 def rossler_workload(args):
@@ -655,33 +841,17 @@ def lorenz_workload(args):
     print(json.dumps(out), flush=True)
 
 
-def main():
-    args = parse()
-    if args.copy_calibration:
-        copy_calibration()
-        return
-    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py runs on one GPU only: this build has no multi-GPU path "
-              "(the ICs would shard as contiguous blocks, one per GPU, with one count reduction)", file=sys.stderr)
-        sys.exit(2)
-
-    if args.workload == "surrogate":
-        return surrogate_workload(args)
-    if args.workload == "lorenz":
-        return lorenz_workload(args)
-    if args.workload == "estimator":
-        return estimator_workload(args)
-    if args.workload == "estimator_defaults":
-        return estimator_defaults_workload(args)
-    if args.workload == "hdbscan":
-        return hdbscan_workload(args)
-    if args.workload == "mlp_infer":
-        return mlp_infer_workload(args)
-    if args.workload == "rossler":
-        return rossler_workload(args)
-
+def duffing_workload(args, cpu: bool = True, emit: bool = True):
+    """Duffing oscillator x' = v, v' = -0.08 v - x^3 + 0.2 cos t, n = 100,000
+    ICs seeded-uniform in x in [-1, 1], v in [-0.5, 1]; t in [0, 1000], rtol
+    1e-8, atol 1e-6, float32 (--dtype f64 for float64); save grid 1,000
+    points on [900, 1000]; fused mean / std / min / max per state over that
+    window; 1-NN classification against the 5 template ICs of the
+    reference's Duffing setup, integrated in the same launch.  The launch
+    shape is chosen once by the project's autotuner before the timed steps."""
     from pybasin_amd.cluster import knn
     from pybasin_amd.solvers import HipDopri5Solver
+    from pybasin_amd.solvers.hip_dopri5 import count_save_steps
     from pybasin_amd.systems import DuffingODE
 
     dev = torch.device("cuda:0")
@@ -691,12 +861,11 @@ def main():
     lo = torch.tensor([-1.0, -0.5])
     hi = torch.tensor([1.0, 1.0])
     y0_cpu = torch.rand(n, 2) * (hi - lo) + lo
-    ref_paths = None
-    if not args.no_cpu and args.ref_sample > 0:
-        import numpy as np
-
-        ref_paths = cpu_reference_paths(y0_cpu, np.linspace(900.0, 1000.0, 1000), args.ref_sample,
-                                        64 * args.ref_sample)
+    base = others = None
+    save_cpu = torch.linspace(900.0, 1000.0, 1000, dtype=torch.float32)
+    if cpu and not args.no_cpu:
+        prow = torch.tensor([[0.08, 1.0, 0.2]]).expand(n, 3)
+        base, others = cpu_baselines("duffing", y0_cpu, prow, save_cpu)
     y0_all = torch.cat([y0_cpu, torch.tensor(DUFFING_TEMPLATES)]).to(dtype).to(dev)
     ode = DuffingODE({"delta": 0.08, "k3": 1.0, "A": 0.2})
     solver = HipDopri5Solver(t_span=(0, 1000), t_steps=1000, t_eval=(900.0, 1000.0), device="cuda",
@@ -744,15 +913,15 @@ def main():
     ms_step = 1e3 * wall / args.steps
     kernel_ms = sum(a.elapsed_time(b) for a, b in kev) / len(kev)
 
-    nsteps = res.nsteps.to(torch.float64)
+    steps, hits = count_save_steps(solver, ode, y0_all, save_ts, None, None, steady_idx=0)
     status = res.status.cpu()
-    S = 2
-    n_traj = n + len(DUFFING_TEMPLATES)
-    flops = float(nsteps.sum()) * (6 * F_RHS["duffing"] + 63 * S + 16) + n_traj * 1000 * (36 + 9 + 6) * S
+    ok = int((status == 0).sum())
+    flops = _lane_flops("duffing", steps, hits, ok * 1000, ok * 1000)
     achieved = flops / (kernel_ms * 1e-3)
     peak = PEAK_FP32_VALU if dtype == torch.float32 else PEAK_FP64_VALU
     tname = "float" if dtype == torch.float32 else "double"
     bs = (counts.double() / n).cpu().tolist()
+    nsteps = res.nsteps.to(torch.float64)
     traffic = None
     base_traffic = TRAFFIC_BYTES.get(("duffing", args.dtype, n, False))
     if not args.keep_traj and base_traffic is not None:
@@ -784,7 +953,6 @@ def main():
             "mean_steps_per_ic": round(float(nsteps.mean()), 1),
             "max_steps_per_ic": int(nsteps.max()),
             "lanes_failed": int((status != 0).sum()),
-            "bs_error_vs_plain": 0.0,
             "launch": dict(launch, suspended=int((nsteps > solver.phase_iters).sum())
                            if solver.phase_iters else 0),
         },
@@ -796,6 +964,7 @@ def main():
             "kernel": (f"bh::dopri5_phase_kernel<Duffing,{tname}> phases 1 + 2" if solver.phase_iters
                        else f"bh::dopri5_lane_kernel<Duffing,{tname}>"),
             "kernel_ms": round(kernel_ms, 3),
+            "attempted_steps": int(steps), "save_hit_steps": int(hits),
         },
     }
     if args.keep_traj:
@@ -804,15 +973,29 @@ def main():
         out["config"]["trajectories"] = "kept (n_save, n, S) in HBM"
         out["traj_write"] = {"bytes": traj_bytes, "GBps": round(traj_bytes / (kernel_ms * 1e-3) / 1e9, 1),
                              "hbm_peak_GBps": 8000.0}
-    if not args.no_cpu:
-        import numpy as np
+    if base is not None:
+        out["cpu_baseline"] = base
+        out["cpu_other_paths"] = others
+    if emit:
+        print(json.dumps(out), flush=True)
+    return out
 
-        save64 = save_ts.double().cpu().numpy().astype(np.float64)
-        out["cpu_baseline"] = cpu_baseline(y0_cpu, save64, min(args.cpu_sample, n))
-        out["cpu_baseline"]["cpu_model"] = cpu_model()
-        if ref_paths is not None:
-            out["cpu_reference_paths"] = ref_paths
-    print(json.dumps(out), flush=True)
+
+def main():
+    args = parse()
+    if args.copy_calibration:
+        copy_calibration()
+        return
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path "
+              "(the ICs would shard as contiguous blocks, one per GPU, with one count reduction)",
+              file=sys.stderr)
+        sys.exit(2)
+    workloads = {"surrogate": surrogate_workload, "duffing": duffing_workload,
+                 "lorenz": lorenz_workload, "estimator": estimator_workload,
+                 "estimator_defaults": estimator_defaults_workload, "hdbscan": hdbscan_workload,
+                 "mlp_infer": mlp_infer_workload, "rossler": rossler_workload}
+    workloads[args.workload](args)
 
 
 if __name__ == "__main__":

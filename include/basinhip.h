@@ -47,6 +47,10 @@ enum {
 #define BH_FLAG_STOP 0x1u     /* enable the per-lane divergence stop at stop_threshold    */
 #define BH_FLAG_INF_FILL 0x2u /* after a stop, remaining saves are +Inf (JaxSolver Event
                                  semantics) instead of the frozen state (masked-RHS semantics) */
+/* out_nsteps[i] = attempted steps | (steps whose span held a save point << 32)
+   instead of the attempted steps alone (the work count of the dense output,
+   for roofline accounting; the per-lane entry points) */
+#define BH_FLAG_COUNT_SAVE_STEPS 0x4u
 /* Refill quorum, bits 8-14 of flags (a launch-shape choice; results do not
    depend on it): finished lanes of a wave take new trajectories once at
    least q of them wait (or every lane still holding one has finished).

@@ -18,6 +18,7 @@ BH_F32, BH_F64 = 0, 1
 BH_LANE_OK, BH_LANE_MAX_STEPS, BH_LANE_DT_UNDERFLOW, BH_LANE_NONFINITE, BH_LANE_DIVERGED = range(5)
 BH_FLAG_STOP = 0x1
 BH_FLAG_INF_FILL = 0x2
+BH_FLAG_COUNT_SAVE_STEPS = 0x4
 BH_REFILL_QUORUM_SHIFT = 8
 BH_NFEAT = 4
 BH_WORKSPACE_BYTES = 256

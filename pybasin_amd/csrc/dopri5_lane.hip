@@ -159,6 +159,7 @@ struct Lane {
     T s0, c0;  // sin / cos of (T)t for systems with time terms
     double t, dt;
     int32_t steps;  // attempted steps (max_steps is capped at 2^31 - 1)
+    int32_t hits;   // accepted steps whose span held a save point
     double next_save;  // save_t[si], +inf once every save is emitted
     double after_save;  // save_t[si + 1] (+inf past the end), loaded a save ahead
     int32_t si;
@@ -254,6 +255,7 @@ __device__ __forceinline__ void lane_init(const LaneArgs &a, LaneT &L, int64_t i
     L.t = a.t0;
     if constexpr (Sys::kTime) dsincos<T>((T)a.t0, L.s0, L.c0);
     L.steps = 0;
+    L.hits = 0;
     L.si = 0;
     L.cnt = 0;
     L.status = BH_LANE_OK;
@@ -418,6 +420,7 @@ __device__ __forceinline__ void lane_step(const LaneArgs &a, LaneT &L, int64_t i
     const bool accept = !bad && ratio <= (T)1;
     L.steps += bad ? 0 : 1;
     if (accept && L.next_save <= tB) {
+        L.hits++;
         // dense output on [tA, tB] (torchdiffeq _interp_fit / _interp_evaluate)
         T ca[S], cb[S], cc[S], cd[S];
         const T m1 = h * (T)M1, m3 = h * (T)M3, m4 = h * (T)M4, m5 = h * (T)M5,
@@ -515,7 +518,10 @@ __device__ __forceinline__ void lane_finish(const LaneArgs &a, LaneT &L, int64_t
         for (int i = 0; i < S; i++) o[i] = L.last[i];
     }
     if (a.status) a.status[ic] = L.status;
-    if (a.nsteps) a.nsteps[ic] = L.steps;
+    if (a.nsteps)
+        a.nsteps[ic] = (a.flags & BH_FLAG_COUNT_SAVE_STEPS)
+                           ? (int64_t)(((uint64_t)(uint32_t)L.hits << 32) | (uint32_t)L.steps)
+                           : (int64_t)L.steps;
     if constexpr (LaneT::kP) {
 #pragma unroll
         for (int i = 0; i < S; i++) a.peak_counts[ic * S + i] = L.pk.count[i];

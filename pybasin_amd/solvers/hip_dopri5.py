@@ -142,6 +142,24 @@ def steady_index(save_ts: torch.Tensor, time_steady: float | None) -> int:
     return int(idx[0, 0]) if len(idx) else 0
 
 
+def count_save_steps(solver, ode_system, y0: torch.Tensor, save_ts: torch.Tensor,
+                     params: torch.Tensor | None, time_steady: float | None = None, *,
+                     steady_idx: int | None = None) -> tuple[int, int]:
+    """Total attempted steps and total steps whose span held a save point
+    (the dense-output fits) of solving this problem with the solver's launch
+    shape: one extra launch with BH_FLAG_COUNT_SAVE_STEPS.  A trajectory's
+    steps do not depend on the launch, so these are the counts of every
+    launch of the same problem."""
+    plan = solver.plan_fused(ode_system, y0, save_ts, params, time_steady=time_steady,
+                             steady_idx=steady_idx)
+    res = solver.launch_fused(plan, keep_trajectories=False,
+                              extra_flags=_lib.BH_FLAG_COUNT_SAVE_STEPS)
+    packed = res.nsteps
+    steps = int((packed & 0xFFFFFFFF).sum())
+    hits = int((packed >> 32).sum())
+    return steps, hits
+
+
 class HipDopri5Solver(Solver):
     display_name = "HIP Dopri5 (per lane)"
 
@@ -273,7 +291,7 @@ class HipDopri5Solver(Solver):
         return LanePlan(sys_id=int(sys_id), y0=y0, p=p, p_stride=p_stride, save_ts=save_ts,
                         st64=st64, steady_idx=int(steady_idx))
 
-    def _lane_call(self, plan: "LanePlan", keep_trajectories: bool):
+    def _lane_call(self, plan: "LanePlan", keep_trajectories: bool, extra_flags: int = 0):
         """Output tensors and the bh_solve_batch arguments shared by every
         lane entry point (system id through out_nsteps), for a prepared plan."""
         dev = self.device
@@ -288,7 +306,7 @@ class HipDopri5Solver(Solver):
             status=torch.empty((n,), dtype=torch.int32, device=dev),
             nsteps=torch.empty((n,), dtype=torch.int64, device=dev),
             ws=torch.empty((_lib.BH_WORKSPACE_BYTES,), dtype=torch.uint8, device=dev))
-        flags = (self.refill_quorum & 0x7F) << _lib.BH_REFILL_QUORUM_SHIFT
+        flags = ((self.refill_quorum & 0x7F) << _lib.BH_REFILL_QUORUM_SHIFT) | int(extra_flags)
         thr = 0.0
         if self.stop_threshold is not None:
             flags |= _lib.BH_FLAG_STOP
@@ -312,14 +330,15 @@ class HipDopri5Solver(Solver):
 
     def launch_fused(self, plan: "LanePlan", *, keep_trajectories: bool,
                      stream: torch.cuda.Stream | None = None,
-                     peak_cap: int | None = None) -> FusedResult:
+                     peak_cap: int | None = None, extra_flags: int = 0) -> FusedResult:
         """Allocate the outputs and launch bh_solve_batch for a prepared plan
         (bh_solve_batch_peaks with a peak buffer of peak_cap rows when
-        peak_cap is given)."""
+        peak_cap is given).  extra_flags are or-ed into the launch flags
+        (BH_FLAG_COUNT_SAVE_STEPS)."""
         dev = self.device
         y0, p, st64 = plan.y0, plan.p, plan.st64
         n, S = y0.shape
-        out, common = self._lane_call(plan, keep_trajectories)
+        out, common = self._lane_call(plan, keep_trajectories, extra_flags)
         ws = out["ws"]
         s = stream if stream is not None else torch.cuda.current_stream(dev)
         pk_vals = pk_counts = None

@@ -262,6 +262,7 @@ class GatedResult:
     evolved: torch.Tensor | None = None      # (n,) bool: labelled after a short integration
     audit_evolved_error: float | None = None  # evolved-label error on routed audit rows
     n_late: int = 0                           # rows integrated after the network passes
+    audit_final: torch.Tensor | None = None   # labels the pass gives the audited rows when not audited
 
 
 def pendulum_labels_from_stats(stats: torch.Tensor) -> torch.Tensor:
@@ -272,14 +273,17 @@ def pendulum_labels_from_stats(stats: torch.Tensor) -> torch.Tensor:
     return torch.where(torch.isfinite(delta), lab, torch.ones_like(lab))
 
 
-def integrate_labels(solver, ode, x: torch.Tensor, time_steady: float, stream=None) -> torch.Tensor:
+def integrate_labels(solver, ode, x: torch.Tensor, time_steady: float, stream=None,
+                     with_result: bool = False):
     """Plain path for rows (theta, omega, T): per-lane Dopri5 with the row's
-    torque as parameter, fused window statistics, threshold labels."""
+    torque as parameter, fused window statistics, threshold labels.
+    with_result=True returns (labels, FusedResult)."""
     dev = solver.device
     x = x.to(dev, torch.float32)
     n = x.shape[0]
     if n == 0:
-        return torch.empty((0,), dtype=torch.int32, device=dev)
+        empty = torch.empty((0,), dtype=torch.int32, device=dev)
+        return (empty, None) if with_result else empty
     alpha, K = float(ode.params["alpha"]), float(ode.params["K"])
     params = torch.stack([torch.full((n,), alpha, dtype=torch.float64, device=dev),
                           x[:, 2].double(), torch.full((n,), K, dtype=torch.float64, device=dev)], 1)
@@ -287,7 +291,8 @@ def integrate_labels(solver, ode, x: torch.Tensor, time_steady: float, stream=No
     save_ts, y0 = solver._prepare_tensors(y0)
     res = solver.run_fused(ode, y0, save_ts, params.contiguous(), keep_trajectories=False,
                            time_steady=time_steady, stream=stream)
-    return pendulum_labels_from_stats(res.stats)
+    lab = pendulum_labels_from_stats(res.stats)
+    return (lab, res) if with_result else lab
 
 
 def tune_plain_launch(solver, ode, x: torch.Tensor, time_steady: float,
@@ -371,7 +376,9 @@ def classify_gated(model: SurrogateMLP, solver, ode, x: torch.Tensor, gate: floa
     integrated rows always take the integrator's label.
 
     stage_times: if a dict is given, it receives the main stream's elapsed
-    milliseconds at each stage boundary (HIP events; read after the call).
+    milliseconds at each stage boundary (HIP events; read after the call),
+    the audit launch's span (audit_launch_ms) and step counts
+    (audit_nsteps), and the network pass's span (infer_launch_ms).
 
     n_groups: the number of groups (max(group) + 1), if the caller knows it
     (saves reading it back from the device)."""
@@ -416,8 +423,18 @@ def classify_gated(model: SurrogateMLP, solver, ode, x: torch.Tensor, gate: floa
     # passes and the short integrations
     side.wait_stream(main)
     with torch.cuda.stream(side):
-        lab_side = integrate_labels(solver, ode, xs, time_steady)
+        if stage_times is not None:
+            # the audit launch's own span and step counts (for its roofline)
+            a0, a1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a0.record(side)
+            lab_side, audit_res = integrate_labels(solver, ode, xs, time_steady, with_result=True)
+            a1.record(side)
+        else:
+            lab_side = integrate_labels(solver, ode, xs, time_steady)
     if k == 0:
+        if stage_times is not None:
+            i0 = torch.cuda.Event(enable_timing=True)
+            i0.record(main)
         lab, conf = model.predict(x)
         mark("infer")
     routed = conf < gate
@@ -510,6 +527,10 @@ def classify_gated(model: SurrogateMLP, solver, ode, x: torch.Tensor, gate: floa
     n_late = sum(int(idx.numel()) for idx, _ in pending)
     for idx, lab_f in pending:
         full[idx] = lab_f
+    # what the pass answers for the audited rows had they not been audited:
+    # the integrator's label where it integrates them anyway, else its own
+    integrates_anyway = need_full | done_full | distrusted | (routed & ~use_e)
+    audit_final = torch.where(integrates_anyway, full, final)[aidx]
     integrated = need_full | done_full | audit
     final = torch.where(integrated, full, final)
     mark("end")
@@ -518,10 +539,31 @@ def classify_gated(model: SurrogateMLP, solver, ode, x: torch.Tensor, gate: floa
         for name, ev in marks[1:]:
             stage_times[name] = marks[0][1].elapsed_time(ev)
         stage_times["late_launches"] = len(pending)
+        stage_times["audit_launch_ms"] = a0.elapsed_time(a1)
+        stage_times["audit_nsteps"] = None if audit_res is None else audit_res.nsteps
+        stage_times["audit_rows"] = int(xs.shape[0])
+        if k == 0:
+            stage_times["infer_launch_ms"] = i0.elapsed_time(marks[1][1])
     return GatedResult(labels=final, routed=routed, audit_idx=aidx, audit_surrogate=lab[aidx],
                        audit_integrated=full[aidx], integrated=integrated,
                        group_routed=group_routed, evolved=use_e & ~integrated,
-                       audit_evolved_error=audit_evolved_error, n_late=n_late)
+                       audit_evolved_error=audit_evolved_error, n_late=n_late,
+                       audit_final=audit_final)
+
+
+def train_surrogate(solver, ode, x: torch.Tensor, n_train: int = 50_000, epochs: int = 600,
+                    seed: int = 0, time_steady: float = 950.0, batch: int = 1024, lr: float = 3e-3):
+    """Train a SurrogateMLP on n_train rows of x drawn at random (seeded) and
+    labelled by the integrator (integrate_labels); returns (model, per-epoch
+    loss, the training rows' indices)."""
+    dev = solver.device
+    n = x.shape[0]
+    gsel = torch.Generator(device=dev).manual_seed(seed + 1)
+    tidx = torch.randperm(n, generator=gsel, device=dev)[:n_train]
+    ylab = integrate_labels(solver, ode, x[tidx], time_steady)
+    model = SurrogateMLP(device=dev, seed=seed)
+    hist = model.fit(x[tidx], ylab, epochs=epochs, batch=batch, lr=lr, seed=seed)
+    return model, hist, tidx
 
 
 # ---- pendulum torque sweep (surrogate workload) ----------------------------

@@ -223,3 +223,67 @@ def test_tuned_plain_launch_keeps_the_labels(cuda_device):
     assert any(k.startswith("phase") for k in shape["timings_ms"])
     assert tuned.phase_iters == shape["phase_iters"]
     assert torch.equal(integrate_labels(tuned, ode, x, 175.0), want)
+
+
+def test_torque_sweep_as_configured(cuda_device):
+    """The surrogate workload at its configuration: 46 torques uniformly in
+    [0.1, 1.0], per-torque counts from np.random.default_rng(0).integers(
+    10_000, 200_001, 46) (4,431,800 ICs), the network trained by the
+    project's kernels on 50,000 integrator-labelled ICs, a 2 % integrated
+    audit per torque.  At the product's gate (0.999) every torque's
+    basin-stability fraction is within 0.5 % of plain full integration of all
+    ICs (computed here); the configured gate (0.95) is run and reported too,
+    and the audit's own estimate of each torque's error is checked against
+    the true error (the audit is how a run without the plain pass knows its
+    error)."""
+    import math
+
+    from pybasin_amd.solvers import HipDopri5Solver
+    from pybasin_amd.surrogate import (basin_stability_by_group, classify_gated, integrate_labels,
+                                       torque_sweep_ics, train_surrogate)
+    from pybasin_amd.systems import PendulumODE
+
+    T = np.linspace(0.1, 1.0, 46)
+    counts = np.random.default_rng(0).integers(10_000, 200_001, 46)
+    x, group = torque_sweep_ics(T, counts, seed=0, device=cuda_device)
+    n = x.shape[0]
+    assert n == int(counts.sum()) == 4_431_800
+    solver = HipDopri5Solver(t_span=(0, 1000), t_steps=1000, t_eval=(900.0, 1000.0), device="cuda",
+                             cache_dir=None)
+    ode = PendulumODE({"alpha": 0.1, "T": 0.5, "K": 1.0})
+    print("[case] torque sweep 46 T, 4,431,800 ICs: training on 50,000", flush=True)
+    model, hist, _ = train_surrogate(solver, ode, x, n_train=50_000, epochs=600, seed=0)
+    print(f"[result] training loss {hist[0]:.4f} -> {hist[-1]:.4f}", flush=True)
+    plain = integrate_labels(solver, ode, x, 950.0)
+    bs_plain = basin_stability_by_group(plain, group, 46)
+    g = group.long()
+    for gate in (0.95, 0.999):
+        res = classify_gated(model, solver, ode, x, gate=gate, group=group, time_steady=950.0,
+                             evolve_t=50.0, n_groups=46)
+        err = (basin_stability_by_group(res.labels, group, 46) - bs_plain).abs()
+        # audit: 2 % of every torque, integrated over the full span
+        aidx = res.audit_idx
+        a_cnt = torch.bincount(g[aidx], minlength=46).double()
+        frac = (a_cnt / torch.bincount(g, minlength=46).double()).cpu()
+        assert bool(((frac > 0.015) & (frac < 0.025)).all()), frac
+        # the audit's estimate of each torque's basin-stability error: FP
+        # fraction of the answers the pass gives the audited rows (had they
+        # not been audited) against their integrated labels
+        d = (res.audit_final == 0).double() - (res.audit_integrated == 0).double()
+        est = torch.zeros(46, dtype=torch.float64, device=x.device).index_add_(0, g[aidx], d) / a_cnt
+        n_dis = torch.zeros(46, dtype=torch.float64, device=x.device).index_add_(0, g[aidx], d.abs())
+        surr_err = float((res.audit_surrogate != res.audit_integrated).float().mean())
+        signed = basin_stability_by_group(res.labels, group, 46) - bs_plain
+        print(f"[result] gate {gate}: max BS error vs plain {float(err.max()):.4f} at T = "
+              f"{T[int(err.argmax())]:.2f}; routed {float(res.routed.float().mean()):.4f}, "
+              f"integrated over the full span {float(res.integrated.float().mean()):.4f}; audit "
+              f"{len(aidx)} rows, network label error on them {surr_err:.4f}, largest audit "
+              f"estimate of a BS error {float(est.abs().max()):.4f}", flush=True)
+        assert bool(((res.labels == 0) | (res.labels == 1)).all())
+        if gate == 0.999:
+            assert float(err.max()) <= 0.005
+        # the audit's estimate agrees with the true error within its
+        # sampling error: the mean of per-row differences in {-1, 0, 1},
+        # standard error sqrt(disagreements) / audited rows (at least one)
+        se = torch.sqrt(n_dis.clamp_min(1.0)) / a_cnt
+        assert bool(((signed - est).abs() <= 4 * se).all()), ((signed - est) / se).abs().max()
