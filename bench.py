@@ -19,8 +19,8 @@ gated labels against the plain labels is measured, not assumed.  The plain
 Duffing workload (100,000 ICs, float32) is measured as well and reported in
 `config.duffing_100k`.
 
-Other workloads: --workload duffing | lorenz | rossler | mlp_infer | hdbscan
-| estimator | estimator_defaults.  Inputs are resident in HBM before timing
+Other workloads: --workload duffing | pendulum_c1 | lorenz | rossler |
+mlp_infer | hdbscan | estimator | estimator_defaults.  Inputs are resident in HBM before timing
 starts.  Prints one JSON line.  --gpus above 1 is not supported: this build
 targets one GPU and runs no collectives.
 """
@@ -87,7 +87,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baselines")
     ap.add_argument("--keep-traj", action="store_true",
                     help="duffing: also write the full trajectories to HBM (reports write GB/s)")
-    ap.add_argument("--workload", choices=["surrogate", "duffing", "lorenz", "rossler", "mlp_infer",
+    ap.add_argument("--workload", choices=["surrogate", "duffing", "pendulum_c1", "lorenz", "rossler",
+                                           "mlp_infer",
                                            "hdbscan", "estimator", "estimator_defaults"],
                     default="surrogate")
     ap.add_argument("--gate", type=float, default=0.999,
@@ -471,6 +472,32 @@ def plain_launch_roofline(solver, ode, x, time_steady, reps: int = 3):
             "attempted_steps": int(steps), "save_hit_steps": int(hits)}
 
 
+def rossler_cpu_baseline(y0_cpu, Ks, n_ics: int = 100):
+    """The oracle (per-lane Dopri5 in C with the edge scatter-add coupling of
+    rossler_network_jax_ode.py:72-73 and the stop at |y|inf >= 400) on the
+    first n_ics ICs x all coupling strengths, float32.  The reference runs
+    this case with JaxSolver; a shared-step torch run cannot, since its
+    unbounded trajectories end the whole batch."""
+    import numpy as np
+
+    from oracle import oracle as O
+    from pybasin_amd.systems import rossler_edges
+
+    cores = host_cores()
+    O.set_rossler_edges(rossler_edges())
+    rows = np.repeat(y0_cpu[:n_ics].double().numpy(), len(Ks), axis=0)
+    prow = np.tile(np.array([[0.2, 0.2, 7.0, k] for k in Ks]), (n_ics, 1))
+    t0 = time.perf_counter()
+    O.solve(O.SYS_ROSSLER, rows, prow, np.array([1000.0]), 0.0, rtol=1e-3, atol=1e-6, bits=32,
+            threads=cores["used"])
+    dt = time.perf_counter() - t0
+    return {"value": round(len(rows) / dt, 1), "unit": "ICs/s", "cores": cores["used"], "kind": "port",
+            "path": "oracle (per-lane Dopri5 in C, edge scatter-add coupling, stop at 400)",
+            "sample": f"first {n_ics} ICs x {len(Ks)} coupling strengths, float32, t in [0,1000], "
+                      f"{dt:.2f} s", "cpu_model": cpu_model(), "os_cpu_count": cores["os_cpu_count"],
+            "affinity_cores": cores["affinity"]}
+
+
 def rossler_workload(args):
     """Rossler network, N = 100 nodes on the fixed Watts-Strogatz graph, 11
     coupling strengths K in [0.119, 0.159] x 10,000 ICs each (IC-major, one
@@ -487,7 +514,11 @@ def rossler_workload(args):
     torch.manual_seed(args.seed)
     lo = torch.tensor([-15.0] * 200 + [-5.0] * 100)
     hi = torch.tensor([15.0] * 200 + [35.0] * 100)
-    y0 = (torch.rand(B, 300) * (hi - lo) + lo).to("cuda")
+    y0_cpu = torch.rand(B, 300) * (hi - lo) + lo
+    base = None
+    if not args.no_cpu:
+        base = rossler_cpu_baseline(y0_cpu, Ks)
+    y0 = y0_cpu.to("cuda")
     params = torch.tensor([[0.2, 0.2, 7.0, k] for k in Ks], dtype=torch.float64)
     ode = RosslerNetworkODE({"a": 0.2, "b": 0.2, "c": 7.0, "K": 0.119})
     solver = HipDopri5Solver(t_span=(0, 1000), t_steps=2, t_eval=(999.0, 1000.0), device="cuda",
@@ -540,6 +571,8 @@ def rossler_workload(args):
                      "valu_tflops": round(valu_flop / (kernel_ms * 1e-3) / 1e12, 3),
                      "kernel": "bh::ross::tile_kernel", "kernel_ms": round(kernel_ms, 3)},
     }
+    if base is not None:
+        out["cpu_baseline"] = base
     print(json.dumps(out), flush=True)
 
 
@@ -772,6 +805,12 @@ def lorenz_workload(args):
     lo = torch.tensor([-10.0, -20.0, 0.0])
     hi = torch.tensor([10.0, 20.0, 0.0])
     y0 = (torch.rand(n, 3) * (hi - lo) + lo)
+    base = others = None
+    if not args.no_cpu:
+        prow = torch.tensor([[0.12, 0.0, -0.6]]).expand(n, 3)
+        # the shared step follows the batch through every escape: ~10 ms a row
+        base, others = cpu_baselines("lorenz", y0, prow, torch.linspace(900.0, 1000.0, 1000),
+                                     n_torch=1024)
     templates = torch.tensor([[0.8, -3.0, 0.0], [-0.8, 3.0, 0.0]])
     y0_all = torch.cat([y0, templates]).to("cuda")
     ode = LorenzODE({"sigma": 0.12, "r": 0.0, "b": -0.6})
@@ -813,9 +852,12 @@ def lorenz_workload(args):
     torch.cuda.synchronize()
     ms = 1e3 * (time.perf_counter() - t0) / args.steps
     kernel_ms = sum(a.elapsed_time(b) for a, b in kev) / len(kev)
+    from pybasin_amd.solvers.hip_dopri5 import count_save_steps
+
     nsteps = res.nsteps.double()
-    flops = float(nsteps.sum()) * (6 * F_RHS["lorenz"] + 63 * 3 + 16) + \
-        float((res.status == 0).sum()) * 1000 * (36 + 9 + 6) * 3
+    steps, hits = count_save_steps(solver, ode, y0_all, save_ts, None, None, steady_idx=0)
+    ok = int((res.status == 0).sum())
+    flops = _lane_flops("lorenz", steps, hits, ok * 1000, ok * 1000)
     out = {
         "metric": "classified initial conditions/sec at <=0.5% basin-stability error vs plain",
         "value": round(n / (ms * 1e-3), 1), "unit": "ICs/s", "n_gpus": 1, "steps": args.steps,
@@ -836,8 +878,12 @@ def lorenz_workload(args):
                      "frac": round(flops / (kernel_ms * 1e-3) / PEAK_FP32_VALU, 4), "traffic": None,
                      "kernel": ("bh::dopri5_phase_kernel<Lorenz,float> phases 1 + 2" if solver.phase_iters
                                 else "bh::dopri5_lane_kernel<Lorenz,float>"),
-                     "kernel_ms": round(kernel_ms, 3)},
+                     "kernel_ms": round(kernel_ms, 3),
+                     "attempted_steps": int(steps), "save_hit_steps": int(hits)},
     }
+    if base is not None:
+        out["cpu_baseline"] = base
+        out["cpu_other_paths"] = others
     print(json.dumps(out), flush=True)
 
 
@@ -981,6 +1027,92 @@ def duffing_workload(args, cpu: bool = True, emit: bool = True):
     return out
 
 
+def pendulum_c1_workload(args):
+    """The CPU-runnable configuration: forced damped pendulum (alpha 0.1,
+    T 0.5, K 1), n = 2,000 ICs seeded-uniform in theta in [-pi + asin 0.5,
+    pi + asin 0.5], omega in [-10, 10]; t in [0, 1000], rtol 1e-8, atol
+    1e-6, float32; 1,000 saves on [900, 1000]; features per-state mean and
+    std of that window (fused); 1-NN on the FP / LC templates (0.4, 0) and
+    (2.7, 0) of setup_pendulum_system.py:53-59, integrated in the same
+    launch.  The CPU paths run the whole workload (the scipy one on a
+    sample)."""
+    from pybasin_amd.cluster import knn
+    from pybasin_amd.solvers import HipDopri5Solver
+    from pybasin_amd.systems import PendulumODE
+
+    n = 2000
+    a = math.asin(0.5)
+    torch.manual_seed(args.seed)
+    lo = torch.tensor([-math.pi + a, -10.0])
+    hi = torch.tensor([math.pi + a, 10.0])
+    y0 = torch.rand(n, 2) * (hi - lo) + lo
+    save_cpu = torch.linspace(900.0, 1000.0, 1000, dtype=torch.float32)
+    base = others = None
+    if not args.no_cpu:
+        prow = torch.tensor([[0.1, 0.5, 1.0]]).expand(n, 3)
+        base, others = cpu_baselines("pendulum", y0, prow, save_cpu, n_torch=n, n_scipy=128,
+                                     n_oracle=n)
+        base["sample"] = "the whole workload: " + base["sample"]
+    templates = torch.tensor([[0.4, 0.0], [2.7, 0.0]])
+    y_all = torch.cat([y0, templates]).to("cuda")
+    ode = PendulumODE({"alpha": 0.1, "T": 0.5, "K": 1.0})
+    solver = HipDopri5Solver(t_span=(0, 1000), t_steps=1000, t_eval=(900.0, 1000.0), device="cuda",
+                             rtol=1e-8, atol=1e-6, cache_dir=None)
+    save_ts = save_cpu.to("cuda")
+    stream = torch.cuda.current_stream()
+    kev = []
+
+    def step(record=False):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        res = solver.run_fused(ode, y_all, save_ts, None, keep_trajectories=False, steady_idx=0,
+                               stream=stream)
+        e1.record(stream)
+        if record:
+            kev.append((e0, e1))
+        feats = torch.stack([res.stats[:, 0, 0], res.stats[:, 0, 1], res.stats[:, 1, 0],
+                             res.stats[:, 1, 1]], 1).contiguous()  # mean, std per state
+        lab = knn(feats[:n].contiguous(), feats[n:].contiguous(), 1)[1][:, 0]
+        return res, _label_counts(lab, 2)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        res, counts = step(record=True)
+    torch.cuda.synchronize()
+    ms = 1e3 * (time.perf_counter() - t0) / args.steps
+    kernel_ms = sum(x.elapsed_time(y) for x, y in kev) / len(kev)
+    from pybasin_amd.solvers.hip_dopri5 import count_save_steps
+
+    steps, hits = count_save_steps(solver, ode, y_all, save_ts, None, None, steady_idx=0)
+    flops = _lane_flops("pendulum", steps, hits, (n + 2) * 1000, (n + 2) * 1000)
+    out = {
+        "metric": "classified initial conditions/sec at <=0.5% basin-stability error vs plain",
+        "value": round(n / (ms * 1e-3), 1), "unit": "ICs/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic: seeded-uniform ICs (torch.manual_seed, CPU float32 draw)",
+        "config": {"workload": "pendulum_c1_plain_hip_dopri5", "n_ics": n, "t_span": [0, 1000],
+                   "save_window": [900, 1000], "features": "fused mean/std per state",
+                   "classifier": "1-NN, templates (0.4, 0) FP / (2.7, 0) LC",
+                   "basin_stability": {"FP": round(float(counts[0]) / n, 5),
+                                       "LC": round(float(counts[1]) / n, 5)},
+                   "max_steps_per_ic": int(res.nsteps.max())},
+        "roofline": {"bound": "valu", "kernel": "bh::dopri5_lane_kernel<Pendulum,float>",
+                     "achieved": round(flops / (kernel_ms * 1e-3) / 1e12, 3), "peak": 157.3,
+                     "unit": "TFLOP/s", "frac": round(flops / (kernel_ms * 1e-3) / PEAK_FP32_VALU, 4),
+                     "traffic": None, "kernel_ms": round(kernel_ms, 3),
+                     "attempted_steps": int(steps), "save_hit_steps": int(hits),
+                     "note": "2,002 trajectories fill under 1 % of the lanes: latency-bound"},
+    }
+    if base is not None:
+        out["cpu_baseline"] = base
+        out["cpu_other_paths"] = others
+    print(json.dumps(out), flush=True)
+
+
 def main():
     args = parse()
     if args.copy_calibration:
@@ -992,6 +1124,7 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     workloads = {"surrogate": surrogate_workload, "duffing": duffing_workload,
+                 "pendulum_c1": pendulum_c1_workload,
                  "lorenz": lorenz_workload, "estimator": estimator_workload,
                  "estimator_defaults": estimator_defaults_workload, "hdbscan": hdbscan_workload,
                  "mlp_infer": mlp_infer_workload, "rossler": rossler_workload}
