@@ -161,6 +161,34 @@ int32_t bh_solve_batch_phased(int32_t system_id, int32_t dtype, const void *y0s,
                               void *workspace, int64_t workspace_bytes, void *stream);
 
 /*
+ * bh_solve_batch_phased_noalloc -- bh_solve_batch_phased that allocates
+ * nothing and takes no workspace: its counter, trajectory indices and
+ * suspended lane records live in a static device buffer of
+ * BH_NOALLOC_SCRATCH_BYTES (64 MiB) that each GPU the library runs on holds
+ * until the process ends.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  The caller passes
+ * the inputs and outputs as for bh_solve_batch_phased.  n is limited to
+ * bh_solve_batch_phased_noalloc_max_n(system_id, dtype, dim) (about 360 k
+ * Duffing or pendulum trajectories in float32); larger n is rejected with
+ * -5 before any device work.  Every call on a GPU shares that GPU's buffer:
+ * calls that overlap on one GPU (concurrent streams or host threads) race;
+ * serialise them.  The counter is reset inside each launch by a kernel.
+ */
+#define BH_NOALLOC_SCRATCH_BYTES (64ll << 20)
+int32_t bh_solve_batch_phased_noalloc(int32_t system_id, int32_t dtype, const void *y0s, int64_t n,
+                                      int32_t dim, const double *params, int32_t n_params,
+                                      int64_t params_stride, double t0, const double *save_ts,
+                                      int32_t n_save, double rtol, double atol, int64_t max_steps,
+                                      uint32_t flags, double stop_threshold, int32_t steady_idx,
+                                      void *out_traj, double *out_feats, void *out_final,
+                                      int32_t *out_status, int64_t *out_nsteps,
+                                      int32_t phase_iters, void *stream);
+/* largest n bh_solve_batch_phased_noalloc accepts for this system, dtype and
+   dim (a negative error code for an unknown system or dtype) */
+int64_t bh_solve_batch_phased_noalloc_max_n(int32_t system_id, int32_t dtype, int32_t dim);
+
+/*
  * bh_autotune_lane -- the launch-shape tuner of the per-lane solver as one
  * call: solves the caller's problem once per candidate -- one untimed
  * launch, then `reps` launches timed with HIP events on `stream` -- and

@@ -22,10 +22,12 @@ BH_FLAG_COUNT_SAVE_STEPS = 0x4
 BH_REFILL_QUORUM_SHIFT = 8
 BH_NFEAT = 4
 BH_WORKSPACE_BYTES = 256
+BH_NOALLOC_SCRATCH_BYTES = 64 << 20
 
 # every symbol include/basinhip.h declares
 EXPORTS = ("bh_version", "bh_last_error", "bh_solve_batch", "bh_solve_batch_peaks",
-           "bh_solve_batch_phased_workspace_bytes", "bh_solve_batch_phased", "bh_autotune_lane", "bh_solve_rossler", "bh_gemm_bf16",
+           "bh_solve_batch_phased_workspace_bytes", "bh_solve_batch_phased",
+           "bh_solve_batch_phased_noalloc", "bh_solve_batch_phased_noalloc_max_n", "bh_autotune_lane", "bh_solve_rossler", "bh_gemm_bf16",
            "bh_mlp_bias_act", "bh_mlp_silu_bwd", "bh_mlp_softmax_ce", "bh_mlp_colsum",
            "bh_mlp_adam", "bh_mlp_prep_input", "bh_mlp_classify", "bh_mlp_infer",
            "bh_mlp_train_workspace_bytes", "bh_mlp_train_step", "bh_knn",
@@ -60,6 +62,12 @@ def _declare(L: ctypes.CDLL) -> None:
     L.bh_solve_batch_phased.argtypes = [i32, i32, vp, i64, i32, vp, i32, i64, f64, vp, i32, f64,
                                         f64, i64, u32, f64, i32, vp, vp, vp, vp, vp, i32, vp, i64,
                                         vp]
+    L.bh_solve_batch_phased_noalloc.restype = i32
+    L.bh_solve_batch_phased_noalloc.argtypes = [i32, i32, vp, i64, i32, vp, i32, i64, f64, vp, i32,
+                                                f64, f64, i64, u32, f64, i32, vp, vp, vp, vp, vp,
+                                                i32, vp]
+    L.bh_solve_batch_phased_noalloc_max_n.restype = i64
+    L.bh_solve_batch_phased_noalloc_max_n.argtypes = [i32, i32, i32]
     L.bh_autotune_lane.restype = i32
     L.bh_autotune_lane.argtypes = [i32, i32, vp, i64, i32, vp, i32, i64, f64, vp, i32, f64, f64,
                                    i64, u32, f64, i32, vp, vp, vp, vp, vp, vp, i64, vp, i32, vp,

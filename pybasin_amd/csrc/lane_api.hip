@@ -5,6 +5,8 @@
 // scheduler that favours independent instructions: the step loop of a
 // wave alone on its SIMD is bound by the latency of its dependent
 // arithmetic.
+#include <mutex>
+
 #include "api_common.hpp"
 #include "dopri5_lane.hip"
 
@@ -204,6 +206,37 @@ int32_t solve_batch_impl(int32_t system_id, int32_t dtype, const void *y0s, int6
     }
 }
 
+// Scratch of the allocation-free two-phase entry point
+// (bh_solve_batch_phased_noalloc): the refill counter, the trajectory
+// indices and the suspended lane records, laid out as the caller workspace
+// of bh_solve_batch_phased.  Each device holds its own copy for the life of
+// the process.  In host code `scratch` names a host-side stand-in, never the
+// device memory: kernels get the address hipGetSymbolAddress returns for
+// the current device.
+static __device__ __align__(256) unsigned char scratch[BH_NOALLOC_SCRATCH_BYTES];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+
+// device address of `scratch` on the current device, looked up once per device
+int32_t scratch_on_current_device(void **out) {
+    int dev = 0;
+    BH_HIP_CHECK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices)
+        return fail(-5, "device %d: the static scratch table covers devices 0..%d", dev,
+                    kMaxDevices - 1);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        BH_HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return fail(-6, "hipGetSymbolAddress(scratch) returned a null address");
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -269,6 +302,54 @@ int32_t bh_solve_batch_phased(int32_t system_id, int32_t dtype, const void *y0s,
                             save_ts, n_save, rtol, atol, max_steps, flags, stop_threshold,
                             steady_idx, out_traj, out_feats, out_final, out_status, out_nsteps, 0,
                             nullptr, nullptr, workspace, workspace_bytes, 0, stream, phase_iters);
+}
+
+int32_t bh_solve_batch_phased_noalloc(int32_t system_id, int32_t dtype, const void *y0s, int64_t n,
+                                      int32_t dim, const double *params, int32_t n_params,
+                                      int64_t params_stride, double t0, const double *save_ts,
+                                      int32_t n_save, double rtol, double atol, int64_t max_steps,
+                                      uint32_t flags, double stop_threshold, int32_t steady_idx,
+                                      void *out_traj, double *out_feats, void *out_final,
+                                      int32_t *out_status, int64_t *out_nsteps,
+                                      int32_t phase_iters, void *stream) {
+    g_err[0] = 0;
+    if (phase_iters < 1) return fail(-5, "phase_iters must be >= 1, got %d", phase_iters);
+    int32_t rc = check_system(system_id, dim, n_params);
+    if (rc) return rc;
+    if (dtype != BH_F32 && dtype != BH_F64) return fail(-5, "dtype must be BH_F32 or BH_F64");
+    if (n < 0) return fail(-5, "n must be >= 0, got %lld", (long long)n);
+    if (n == 0) return 0;
+    const int64_t need = phased_workspace(system_id, dtype, dim, n);
+    if (need > (int64_t)BH_NOALLOC_SCRATCH_BYTES)
+        return fail(-5,
+                    "bh_solve_batch_phased_noalloc: n = %lld needs %lld scratch bytes, the static "
+                    "buffer holds %lld (at most %lld trajectories of this system and dtype; use "
+                    "bh_solve_batch_phased with a workspace)",
+                    (long long)n, (long long)need, (long long)BH_NOALLOC_SCRATCH_BYTES,
+                    (long long)bh_solve_batch_phased_noalloc_max_n(system_id, dtype, dim));
+    void *ws = nullptr;
+    rc = scratch_on_current_device(&ws);
+    if (rc) return rc;
+    // the counter is reset by a kernel inside the launch (no memset)
+    return solve_batch_impl(system_id, dtype, y0s, n, dim, params, n_params, params_stride, t0,
+                            save_ts, n_save, rtol, atol, max_steps, flags, stop_threshold,
+                            steady_idx, out_traj, out_feats, out_final, out_status, out_nsteps, 0,
+                            nullptr, nullptr, ws, (int64_t)BH_NOALLOC_SCRATCH_BYTES, 0, stream,
+                            phase_iters);
+}
+
+int64_t bh_solve_batch_phased_noalloc_max_n(int32_t system_id, int32_t dtype, int32_t dim) {
+    g_err[0] = 0;
+    if (dtype != BH_F32 && dtype != BH_F64) return fail(-5, "dtype must be BH_F32 or BH_F64");
+    if (system_id == BH_SYS_DECAY && dim != 1 && dim != 2)
+        return fail(-5, "decay system supports dim 1 or 2, got %d", dim);
+    const int64_t one = phased_workspace(system_id, dtype, dim, 1);
+    if (one < 0) return fail(-1, "system id %d has no per-lane device functor", system_id);
+    // largest n whose workspace fits (the index array rounds up to 256 bytes)
+    const int64_t per = one - BH_WORKSPACE_BYTES - 256 + 8;
+    int64_t hi = ((int64_t)BH_NOALLOC_SCRATCH_BYTES - BH_WORKSPACE_BYTES) / per;
+    while (hi > 0 && phased_workspace(system_id, dtype, dim, hi) > (int64_t)BH_NOALLOC_SCRATCH_BYTES) hi--;
+    return hi;
 }
 
 int32_t bh_autotune_lane(int32_t system_id, int32_t dtype, const void *y0s, int64_t n,

@@ -67,3 +67,21 @@ def test_version_and_argument_errors_without_gpu():
     rc = L.bh_solve_batch_phased(4, 0, fake, 1000, 3, fake, 3, 0, 0.0, fake, 10, 1e-8, 1e-6, 10, 0,
                                  0.0, 0, None, None, None, None, None, 5, fake, need - 1, None)
     assert rc == -5 and b"workspace" in L.bh_last_error()
+
+
+def test_noalloc_entry_checks_size_before_device_work():
+    """The allocation-free two-phase entry point: its size limit, and the
+    errors it returns before touching a device."""
+    L = _lib.lib()
+    sid, dt = _lib.BH_SYS_DUFFING, _lib.BH_F32
+    m = L.bh_solve_batch_phased_noalloc_max_n(sid, dt, 2)
+    assert m > 300_000
+    assert L.bh_solve_batch_phased_workspace_bytes(sid, dt, 2, m) <= _lib.BH_NOALLOC_SCRATCH_BYTES
+    assert L.bh_solve_batch_phased_workspace_bytes(sid, dt, 2, m + 1) > _lib.BH_NOALLOC_SCRATCH_BYTES
+    args = lambda n, k: (sid, dt, None, n, 2, None, 3, 0, 0.0, None, 10, 1e-8, 1e-6,  # noqa: E731
+                         1000, 0, 0.0, 0, None, None, None, None, None, k, None)
+    assert L.bh_solve_batch_phased_noalloc(*args(m + 1, 100)) == -5
+    assert b"static" in L.bh_last_error()
+    assert L.bh_solve_batch_phased_noalloc(*args(10, 0)) == -5  # phase_iters >= 1
+    assert L.bh_solve_batch_phased_noalloc(*args(0, 100)) == 0  # nothing to do
+    assert L.bh_solve_batch_phased_noalloc_max_n(99, dt, 2) < 0
