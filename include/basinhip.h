@@ -51,6 +51,11 @@ enum {
    instead of the attempted steps alone (the work count of the dense output,
    for roofline accounting; the per-lane entry points) */
 #define BH_FLAG_COUNT_SAVE_STEPS 0x4u
+/* out_feats holds BH_NFEAT_TREND values per state instead of BH_NFEAT: the
+   four statistics and the least-squares slope of the window against the
+   sample position 0 .. N'-1 (linear_trend attr "slope",
+   src/pybasin/ts_torch/calculators/torch_features_trend.py:11-27) */
+#define BH_FLAG_TREND 0x8u
 /* Refill quorum, bits 8-14 of flags (a launch-shape choice; results do not
    depend on it): finished lanes of a wave take new trajectories once at
    least q of them wait (or every lane still holding one has finished).
@@ -63,6 +68,8 @@ enum {
 /* number of statistics per state written to out_feats: mean, std (ddof 0),
    min, max over save indices [steady_idx, n_save) */
 #define BH_NFEAT 4
+/* with BH_FLAG_TREND: mean, std, min, max, slope */
+#define BH_NFEAT_TREND 5
 /* bytes of device workspace bh_solve_batch needs (a lane-refill counter) */
 #define BH_WORKSPACE_BYTES 256
 
@@ -87,7 +94,8 @@ const char *bh_last_error(void);
  *   steady_idx     first save index of the feature window (filter_time,
  *                  feature_extractor.py:102-130); >= n_save disables features
  *   out_traj       (n_save, n, dim) in dtype, or NULL (trajectories not kept)
- *   out_feats      (n, dim, BH_NFEAT) float64, or NULL
+ *   out_feats      (n, dim, BH_NFEAT) float64 ((n, dim, BH_NFEAT_TREND) with
+ *                  BH_FLAG_TREND), or NULL
  *   out_final      (n, dim) in dtype: state at the last save time, or NULL
  *   out_status     (n) int32, or NULL; out_nsteps (n) int64 attempted steps, or NULL
  *   workspace      >= BH_WORKSPACE_BYTES of device memory (refill counter)
@@ -328,6 +336,28 @@ int32_t bh_mlp_train_step(const float *x, const int32_t *labels, int64_t M, cons
 int32_t bh_window_order_stats(int32_t dtype, const void *traj, int64_t n_save, int64_t n,
                               int32_t dim, int64_t start, const int64_t *ranks, int32_t n_ranks,
                               void *out, void *stream);
+
+/* Autocorrelation periodicity of each (trajectory, state) series of the
+   window rows [start, n_save) of a kept trajectory buffer (n_save, n, dim) in
+   dtype: out_strength and out_period (n, dim) in dtype hold the height and
+   lag of the first local maximum (scipy's definition: plateaus marked at
+   their midpoint) of the normalised autocorrelation at a lag >= min_lag
+   above peak_threshold, 0 and 0 if there is none.  Windows of up to 15,872
+   (float) / 7,936 (double) rows.  Replaces
+   src/pybasin/ts_torch/calculators/torch_features_autocorrelation.py:268-331
+   (defaults min_lag 2, peak_threshold 0.3). */
+int32_t bh_window_autocorr_periodicity(int32_t dtype, const void *traj, int64_t n_save, int64_t n,
+                                       int32_t dim, int64_t start, int32_t min_lag,
+                                       double peak_threshold, void *out_strength,
+                                       void *out_period, void *stream);
+
+/* Spectral frequency ratio of each series of the same window: f2 / f1 for
+   the dominant rfft bin f1 of the centred series and the strongest bin f2
+   at least two bins from it; 0 when the largest power is below 1e-10.  out
+   (n, dim) in dtype; same window limits.  Replaces
+   src/pybasin/ts_torch/calculators/torch_features_frequency.py:390-441. */
+int32_t bh_window_spectral_ratio(int32_t dtype, const void *traj, int64_t n_save, int64_t n,
+                                 int32_t dim, int64_t start, void *out_ratio, void *stream);
 
 /* ---- label layer (replaces the sklearn calls of the reference's
    HDBSCANClusterer, src/pybasin/predictors/hdbscan_clusterer.py:12-148, and

@@ -14,6 +14,9 @@ Contents
 * ``features``         mean / std(ddof=0) / min / max / delta / log_delta
                        (ts_torch/calculators/torch_features_statistical.py:29-110)
                        with ``impute_extreme`` (torch_feature_utilities.py:60-94).
+* ``amplitude``, ``linear_trend_slope``, ``autocorrelation_periodicity``,
+  ``spectral_frequency_ratio``, ``local_maxima_1d``
+                       the dynamical-system feature set (ts_torch/settings.py:407-417).
 * ``sync_features``    SynchronizationFeatureExtractor._compute_sync_features
                        (case_studies/rossler_network/synchronization_feature_extractor.py:73-106).
 * ``sync_labels``      SynchronizationClassifier.fit_predict
@@ -175,6 +178,92 @@ def features(window: np.ndarray) -> dict[str, np.ndarray]:
         "mean": mean, "standard_deviation": std, "maximum": mx, "minimum": mn,
         "delta": delta, "log_delta": np.log(delta + 1e-10),
     }
+
+
+def amplitude(window: np.ndarray) -> np.ndarray:
+    """max - min over the window (torch_features_statistical.py:307-319)."""
+    w = window.astype(np.float64)
+    return w.max(axis=0) - w.min(axis=0)
+
+
+def linear_trend_slope(window: np.ndarray) -> np.ndarray:
+    """Least-squares slope against the sample position 0 .. N'-1
+    (torch_features_trend.py:11-27, attr "slope")."""
+    w = window.astype(np.float64)
+    n = w.shape[0]
+    t = np.arange(n, dtype=np.float64)
+    tc = (t - t.mean()).reshape((n,) + (1,) * (w.ndim - 1))
+    ss_tt = float(((t - t.mean()) ** 2).sum())
+    return (tc * (w - w.mean(axis=0))).sum(axis=0) / (ss_tt + 1e-10)
+
+
+def local_maxima_1d(x: np.ndarray) -> np.ndarray:
+    """Indices of the local maxima of a 1-D sequence: runs of equal values
+    with a smaller value on each side, marked at the run's midpoint rounded
+    down; the first and last samples never qualify
+    (torch_feature_utilities.py:146-211, scipy's _local_maxima_1d)."""
+    x = np.asarray(x)
+    out = []
+    i, i_max = 1, len(x) - 1
+    while i < i_max:
+        if x[i - 1] < x[i]:
+            i_ahead = i + 1
+            while i_ahead < i_max and x[i_ahead] == x[i]:
+                i_ahead += 1
+            if x[i_ahead] < x[i]:
+                out.append((i + i_ahead - 1) // 2)
+                i = i_ahead
+        i += 1
+    return np.array(out, dtype=np.int64)
+
+
+def autocorrelation_periodicity(window: np.ndarray, min_lag: int = 2, peak_threshold: float = 0.3):
+    """(strength, period) per series of a (N', B, S) window: the normalised
+    autocorrelation (z-scored series with the unbiased std, zero-padded FFT
+    of length 2 N', as torch_features_autocorrelation.py:293-304), its
+    first local maximum at a lag >= min_lag above peak_threshold; 0 and 0
+    when there is none (:306-331)."""
+    w = window.astype(np.float64)
+    n = w.shape[0]
+    z = (w - w.mean(axis=0)) / (w.std(axis=0, ddof=1) + 1e-10)
+    f = np.fft.rfft(z, n=2 * n, axis=0)
+    ac = np.fft.irfft(f.real ** 2 + f.imag ** 2, n=2 * n, axis=0)[:n]
+    ac = ac / (ac[0:1] + 1e-10)
+    flat = ac.reshape(n, -1)
+    strength = np.zeros(flat.shape[1])
+    period = np.zeros(flat.shape[1])
+    for j in range(flat.shape[1]):
+        for k in local_maxima_1d(flat[:, j]):
+            if k >= min_lag and flat[k, j] > peak_threshold:
+                strength[j], period[j] = flat[k, j], float(k)
+                break
+    return strength.reshape(w.shape[1:]), period.reshape(w.shape[1:])
+
+
+def spectral_frequency_ratio(window: np.ndarray) -> np.ndarray:
+    """f2 / f1 per series: f1 the dominant rfft bin of the centred series
+    (bin 0 zeroed), f2 the strongest bin at least two bins from f1; 0 when
+    the largest power is below 1e-10 (torch_features_frequency.py:390-441).
+    Equal powers resolve to the lower bin; a zero-power runner-up gives 0."""
+    w = window.astype(np.float64)
+    c = w - w.mean(axis=0)
+    f = np.fft.rfft(c, axis=0)
+    p = (f.real ** 2 + f.imag ** 2).reshape(f.shape[0], -1)
+    p[0] = 0.0
+    out = np.zeros(p.shape[1])
+    for j in range(p.shape[1]):
+        psd = p[:, j]
+        if not psd.max() >= 1e-10:
+            continue
+        first = int(np.argmax(psd))
+        far = np.abs(np.arange(len(psd)) - first) >= 2
+        if not far.any():
+            continue
+        cand = np.where(far, psd, -1.0)
+        second = int(np.argmax(cand))
+        if cand[second] > 0 and first > 0:
+            out[j] = second / first
+    return out.reshape(w.shape[1:])
 
 
 def sync_features(final_state: np.ndarray, n_nodes: int) -> np.ndarray:

@@ -19,8 +19,10 @@ BH_LANE_OK, BH_LANE_MAX_STEPS, BH_LANE_DT_UNDERFLOW, BH_LANE_NONFINITE, BH_LANE_
 BH_FLAG_STOP = 0x1
 BH_FLAG_INF_FILL = 0x2
 BH_FLAG_COUNT_SAVE_STEPS = 0x4
+BH_FLAG_TREND = 0x8
 BH_REFILL_QUORUM_SHIFT = 8
 BH_NFEAT = 4
+BH_NFEAT_TREND = 5
 BH_WORKSPACE_BYTES = 256
 BH_NOALLOC_SCRATCH_BYTES = 64 << 20
 
@@ -33,7 +35,7 @@ EXPORTS = ("bh_version", "bh_last_error", "bh_solve_batch", "bh_solve_batch_peak
            "bh_mlp_train_workspace_bytes", "bh_mlp_train_step", "bh_knn",
            "bh_hdbscan_workspace_bytes", "bh_core_distances", "bh_hdbscan_mst", "bh_hdbscan_labels",
            "bh_silhouette",
-           "bh_window_order_stats")
+           "bh_window_order_stats", "bh_window_autocorr_periodicity", "bh_window_spectral_ratio")
 
 _lock = threading.Lock()
 _lib: ctypes.CDLL | None = None
@@ -98,6 +100,8 @@ def _declare(L: ctypes.CDLL) -> None:
         "bh_hdbscan_labels": [vp, vp, vp, i64, vp, i32, vp, vp],
         "bh_silhouette": [vp, i64, i32, vp, vp, i32, vp, vp, vp, vp, vp],
         "bh_window_order_stats": [i32, vp, i64, i64, i32, i64, vp, i32, vp, vp],
+        "bh_window_autocorr_periodicity": [i32, vp, i64, i64, i32, i64, i32, f64, vp, vp, vp],
+        "bh_window_spectral_ratio": [i32, vp, i64, i64, i32, i64, vp, vp],
     }
     for name, args in sigs.items():
         fn = getattr(L, name)

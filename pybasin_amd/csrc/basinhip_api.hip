@@ -654,4 +654,98 @@ int32_t bh_window_order_stats(int32_t dtype, const void *traj, int64_t n_save, i
     return 0;
 }
 
+// ---- series features of a kept window ---------------------------------------
+
+namespace {
+
+// series per workgroup and dynamic LDS bytes for a window of w rows (62 KB
+// of series data plus the autocorrelation blocks of the four waves)
+int32_t series_launch(int64_t w, size_t elem, int *G, size_t *lds, bool ac) {
+    const int64_t budget = 62 * 1024;
+    const int64_t per = w * (int64_t)elem;
+    if (per > budget)
+        return fail(-5, "window of %lld rows exceeds the %lld-byte LDS budget of a series",
+                    (long long)w, (long long)budget);
+    int64_t g = budget / per;
+    if (g > 16) g = 16;
+    *G = (int)g;
+    *lds = (size_t)((g * per + 15) / 16 * 16 + (ac ? 4 * 64 * sizeof(double) : 0));
+    return 0;
+}
+
+int32_t check_series_args(const char *fn, int32_t dtype, const void *traj, int64_t n_save,
+                          int64_t n, int32_t dim, int64_t start) {
+    if (n < 0 || dim < 1 || n_save < 1) return fail(-5, "%s: bad sizes", fn);
+    if (start < 0 || start >= n_save) return fail(-5, "%s: empty window", fn);
+    if (dtype != BH_F32 && dtype != BH_F64) return fail(-5, "%s: dtype must be BH_F32 or BH_F64", fn);
+    if (n > 0 && !traj) return fail(-5, "%s: null trajectory buffer", fn);
+    return 0;
+}
+
+}  // namespace
+
+int32_t bh_window_autocorr_periodicity(int32_t dtype, const void *traj, int64_t n_save, int64_t n,
+                                       int32_t dim, int64_t start, int32_t min_lag,
+                                       double peak_threshold, void *out_strength,
+                                       void *out_period, void *stream) {
+    g_err[0] = 0;
+    int32_t rc = check_series_args("bh_window_autocorr_periodicity", dtype, traj, n_save, n, dim,
+                                   start);
+    if (rc) return rc;
+    if (min_lag < 0) return fail(-5, "bh_window_autocorr_periodicity: min_lag must be >= 0");
+    if (n == 0) return 0;
+    if (!out_strength || !out_period) return fail(-5, "bh_window_autocorr_periodicity: null output");
+    const size_t elem = dtype == BH_F32 ? 4 : 8;
+    int G;
+    size_t lds;
+    rc = series_launch(n_save - start, elem, &G, &lds, true);
+    if (rc) return rc;
+    const int64_t np = n * dim;
+    const int64_t blocks = (np + G - 1) / G;
+    if (blocks > 2147483647LL) return fail(-5, "bh_window_autocorr_periodicity: too many series");
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    if (dtype == BH_F32)
+        hipLaunchKernelGGL(bh::win::autocorr_kernel<float>, dim3((unsigned)blocks),
+                           dim3(bh::win::kSeriesBlock), lds, s, reinterpret_cast<const float *>(traj),
+                           n_save, n, dim, start, G, min_lag, peak_threshold,
+                           reinterpret_cast<float *>(out_strength), reinterpret_cast<float *>(out_period));
+    else
+        hipLaunchKernelGGL(bh::win::autocorr_kernel<double>, dim3((unsigned)blocks),
+                           dim3(bh::win::kSeriesBlock), lds, s,
+                           reinterpret_cast<const double *>(traj), n_save, n, dim, start, G, min_lag,
+                           peak_threshold, reinterpret_cast<double *>(out_strength),
+                           reinterpret_cast<double *>(out_period));
+    BH_HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
+int32_t bh_window_spectral_ratio(int32_t dtype, const void *traj, int64_t n_save, int64_t n,
+                                 int32_t dim, int64_t start, void *out_ratio, void *stream) {
+    g_err[0] = 0;
+    int32_t rc = check_series_args("bh_window_spectral_ratio", dtype, traj, n_save, n, dim, start);
+    if (rc) return rc;
+    if (n == 0) return 0;
+    if (!out_ratio) return fail(-5, "bh_window_spectral_ratio: null output");
+    const size_t elem = dtype == BH_F32 ? 4 : 8;
+    int G;
+    size_t lds;
+    rc = series_launch(n_save - start, elem, &G, &lds, false);
+    if (rc) return rc;
+    const int64_t np = n * dim;
+    const int64_t blocks = (np + G - 1) / G;
+    if (blocks > 2147483647LL) return fail(-5, "bh_window_spectral_ratio: too many series");
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    if (dtype == BH_F32)
+        hipLaunchKernelGGL(bh::win::spectral_ratio_kernel<float>, dim3((unsigned)blocks),
+                           dim3(bh::win::kSeriesBlock), lds, s, reinterpret_cast<const float *>(traj),
+                           n_save, n, dim, start, G, reinterpret_cast<float *>(out_ratio));
+    else
+        hipLaunchKernelGGL(bh::win::spectral_ratio_kernel<double>, dim3((unsigned)blocks),
+                           dim3(bh::win::kSeriesBlock), lds, s,
+                           reinterpret_cast<const double *>(traj), n_save, n, dim, start, G,
+                           reinterpret_cast<double *>(out_ratio));
+    BH_HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
 }  // extern "C"

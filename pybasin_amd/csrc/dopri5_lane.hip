@@ -166,7 +166,7 @@ struct Lane {
     int32_t status;
     bool done;
     // streaming statistics over the feature window
-    double shift[S], s1[S], s2[S];
+    double shift[S], s1[S], s2[S], s3[S];  // s3: sum of window position x (value - shift)
     T mn[S], mx[S];
     int32_t cnt;
     T last[S];
@@ -199,6 +199,7 @@ __device__ __forceinline__ void window_sample(const LaneArgs &a, LaneT &L, int64
         const double d = (double)v[i] - L.shift[i];
         L.s1[i] += d;
         L.s2[i] = fma(d, d, L.s2[i]);
+        L.s3[i] = fma((double)L.cnt, d, L.s3[i]);
         L.mn[i] = fmin(L.mn[i], v[i]);
         L.mx[i] = fmax(L.mx[i], v[i]);
     }
@@ -246,6 +247,7 @@ __device__ __forceinline__ void lane_init(const LaneArgs &a, LaneT &L, int64_t i
         L.y[i] = y0[i];
         L.s1[i] = 0.0;
         L.s2[i] = 0.0;
+        L.s3[i] = 0.0;
         L.shift[i] = 0.0;
     }
     if constexpr (LaneT::kP) {
@@ -326,12 +328,15 @@ __device__ __forceinline__ void lane_fill_rest(const LaneArgs &a, LaneT &L, int6
     window_sample<Sys, T>(a, L, ic, v);
     const int32_t k = a.n_save - w0 - 1;
     if (k > 0) {
+        // window positions cnt .. cnt + k - 1
+        const double pos_sum = (double)k * (double)L.cnt + 0.5 * (double)k * (double)(k - 1);
 #pragma unroll
         for (int i = 0; i < S; i++) {
             const double d = (double)v[i] - L.shift[i];
             const double kd = (double)k * d;
             L.s1[i] += kd;
             L.s2[i] = fma(kd, d, L.s2[i]);
+            L.s3[i] = fma(pos_sum, d, L.s3[i]);
         }
         if constexpr (LaneT::kP) {
 #pragma unroll
@@ -494,7 +499,9 @@ template <class Sys, class T, class LaneT>
 __device__ __forceinline__ void lane_finish(const LaneArgs &a, LaneT &L, int64_t ic) {
     constexpr int S = Sys::S;
     if (a.feats) {
-        double *o = a.feats + ic * S * BH_NFEAT;
+        const bool trend = (a.flags & BH_FLAG_TREND) != 0;
+        const int nf = trend ? BH_NFEAT_TREND : BH_NFEAT;
+        double *o = a.feats + ic * S * nf;
 #pragma unroll
         for (int i = 0; i < S; i++) {
             if (L.cnt > 0) {
@@ -502,13 +509,22 @@ __device__ __forceinline__ void lane_finish(const LaneArgs &a, LaneT &L, int64_t
                 const double m1 = L.s1[i] * inv;
                 const double var = fmax(L.s2[i] * inv - m1 * m1, 0.0);
                 const bool fin = isfinite((double)L.mn[i]) && isfinite((double)L.mx[i]);
-                o[i * BH_NFEAT + 0] = fin ? L.shift[i] + m1 : (double)L.mx[i];
-                o[i * BH_NFEAT + 1] = fin ? sqrt(var) : NAN;
-                o[i * BH_NFEAT + 2] = (double)L.mn[i];
-                o[i * BH_NFEAT + 3] = (double)L.mx[i];
+                o[i * nf + 0] = fin ? L.shift[i] + m1 : (double)L.mx[i];
+                o[i * nf + 1] = fin ? sqrt(var) : NAN;
+                o[i * nf + 2] = (double)L.mn[i];
+                o[i * nf + 3] = (double)L.mx[i];
+                if (trend) {
+                    // least-squares slope against the window position 0 .. cnt-1:
+                    // sum (t - tbar)(x - xbar) / (sum (t - tbar)^2 + 1e-10), the
+                    // shift cancelling in the numerator
+                    const double c = (double)L.cnt;
+                    const double tbar = 0.5 * (c - 1.0);
+                    const double ss_tt = c * (c * c - 1.0) / 12.0;
+                    const double ss_tx = L.s3[i] - tbar * L.s1[i];
+                    o[i * nf + 4] = fin ? ss_tx / (ss_tt + 1e-10) : NAN;
+                }
             } else {
-#pragma unroll
-                for (int j = 0; j < BH_NFEAT; j++) o[i * BH_NFEAT + j] = NAN;
+                for (int j = 0; j < nf; j++) o[i * nf + j] = NAN;
             }
         }
     }

@@ -89,5 +89,244 @@ __global__ void __launch_bounds__(256) order_stat_kernel(const T *traj, int64_t 
     }
 }
 
+// ---- series kernels: autocorrelation periodicity, spectral frequency ratio ----
+//
+// A workgroup of 4 waves takes G consecutive (trajectory, state) series --
+// consecutive addresses within each saved row -- and copies their window
+// [start, n_save) into LDS with coalesced loads (G values per row); each
+// wave then processes G / 4 series, one at a time, with its 64 lanes.
+constexpr int kSeriesBlock = 256;
+
+// wave-wide sum of a double (xor butterfly over the 64 lanes)
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+    return v;
+}
+
+template <class T>
+__device__ __forceinline__ void load_series(const T *traj, int64_t n, int S, int64_t start,
+                                            int64_t w, int64_t p0, int G, int64_t np, T *lds) {
+    // lds[g * w + i] = traj[(start + i) * n * S + p0 + g]
+    const int64_t stride = n * S;
+    for (int64_t e = threadIdx.x; e < (int64_t)G * w; e += blockDim.x) {
+        const int64_t i = e / G;
+        const int g = (int)(e - i * G);
+        const int64_t p = p0 + g;
+        lds[(int64_t)g * w + i] = p < np ? traj[(start + i) * stride + p] : T(0);
+    }
+}
+
+// mean and unbiased / population variance pieces of one series in LDS
+template <class T>
+__device__ __forceinline__ void series_moments(const T *x, int64_t w, double &mean, double &ss) {
+    const int lane = threadIdx.x & 63;
+    double s1 = 0.0;
+    for (int64_t i = lane; i < w; i += 64) s1 += (double)x[i];
+    mean = wave_sum(s1) / (double)w;
+    double s2 = 0.0;
+    for (int64_t i = lane; i < w; i += 64) {
+        const double d = (double)x[i] - mean;
+        s2 = fma(d, d, s2);
+    }
+    ss = wave_sum(s2);  // sum of squared deviations
+}
+
+// Autocorrelation periodicity of each series (the reference's
+// autocorrelation_periodicity, torch_features_autocorrelation.py:268-331):
+// z = (x - mean) / (std + 1e-10) with the unbiased std; r_k = sum_i z_i z_{i+k}
+// (the zero-padded FFT product of the reference is this linear
+// autocorrelation); ac_k = r_k / (r_0 + 1e-10); the first local maximum of ac
+// in scipy's sense (a run of equal values with a smaller value on each side,
+// marked at its midpoint rounded down; torch_feature_utilities.py:146-211) at
+// a lag >= min_lag with ac > peak_threshold gives strength = ac there and
+// period = its lag; none gives 0 and 0.  Lags are computed 64 at a time (one
+// per lane, double sums over z in LDS) and scanned in order, so the work
+// stops at the first block holding a valid peak.
+template <class T>
+__global__ void __launch_bounds__(kSeriesBlock)
+    autocorr_kernel(const T *traj, int64_t n_save, int64_t n, int S, int64_t start, int G,
+                    int min_lag, double thr, T *out_strength, T *out_period) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int64_t w = n_save - start;
+    const int64_t np = n * S;
+    const int64_t p0 = (int64_t)blockIdx.x * G;
+    T *lds = reinterpret_cast<T *>(smem);
+    double *acbuf = reinterpret_cast<double *>(smem + ((int64_t)G * w * sizeof(T) + 15) / 16 * 16);
+    load_series<T>(traj, n, S, start, w, p0, G, np, lds);
+    __syncthreads();
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    double *ac = acbuf + wave * 64;
+    for (int g = wave; g < G; g += kSeriesBlock / 64) {
+        const int64_t p = p0 + g;
+        if (p >= np) break;  // wave-uniform
+        T *z = lds + (int64_t)g * w;
+        double mean, ss;
+        series_moments<T>(z, w, mean, ss);
+        const double sd = w > 1 ? sqrt(ss / (double)(w - 1)) : NAN;
+        const double inv = 1.0 / (sd + 1e-10);
+        for (int64_t i = lane; i < w; i += 64) z[i] = (T)(((double)z[i] - mean) * inv);
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        double r0 = 0.0;
+        // scipy-style streaming peak search state (lane 0 only)
+        double prev = 0.0, cand_val = 0.0;
+        int64_t cand = -1;
+        bool found = false;
+        double strength = 0.0, period = 0.0;
+        for (int64_t base = 0; base < w && !found; base += 64) {
+            const int64_t k = base + lane;
+            double r = 0.0;
+            if (k < w)
+                for (int64_t i = 0; i + k < w; i++) r = fma((double)z[i], (double)z[i + k], r);
+            if (base == 0) r0 = __shfl(r, 0, 64);
+            ac[lane] = r / (r0 + 1e-10);
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            if (lane == 0) {
+                const int64_t kend = base + 64 < w ? base + 64 : w;
+                for (int64_t j = base; j < kend && !found; j++) {
+                    const double v = ac[j - base];
+                    if (j >= 1) {
+                        if (cand >= 0) {
+                            if (v == cand_val) {
+                                // the plateau goes on (to the last value at most)
+                            } else if (v < cand_val) {
+                                const int64_t mid = (cand + j - 1) / 2;
+                                if (mid >= min_lag && cand_val > thr) {
+                                    found = true;
+                                    strength = cand_val;
+                                    period = (double)mid;
+                                }
+                                cand = -1;
+                            } else {
+                                cand = j <= w - 2 ? j : -1;  // a new rise
+                                cand_val = v;
+                            }
+                        } else if (prev < v && j <= w - 2) {
+                            cand = j;
+                            cand_val = v;
+                        }
+                    }
+                    prev = v;
+                }
+            }
+            found = __shfl((int)found, 0, 64) != 0;
+            __builtin_amdgcn_wave_barrier();
+        }
+        if (lane == 0) {
+            out_strength[p] = (T)strength;
+            out_period[p] = (T)period;
+        }
+    }
+}
+
+// Spectral frequency ratio of each series (the reference's
+// spectral_frequency_ratio, torch_features_frequency.py:390-441): power
+// |X_b|^2 of the centred window at the rfft bins b = 1 .. floor(w / 2)
+// (bin 0 set to zero), each by Goertzel's recurrence in double
+// (s_i = c_i + 2 cos(2 pi b / w) s_{i-1} - s_{i-2}; |X_b|^2 = s1^2 + s2^2 -
+// 2 cos(2 pi b / w) s1 s2), four bins per lane at a time; the dominant bin
+// f1, the strongest bin f2 at least two bins away from it, ratio f2 / f1;
+// 0 when the largest power is below 1e-10.  Equal powers resolve to the
+// lower bin.
+template <class T>
+__global__ void __launch_bounds__(kSeriesBlock)
+    spectral_ratio_kernel(const T *traj, int64_t n_save, int64_t n, int S, int64_t start, int G,
+                          T *out_ratio) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int64_t w = n_save - start;
+    const int64_t np = n * S;
+    const int64_t p0 = (int64_t)blockIdx.x * G;
+    T *lds = reinterpret_cast<T *>(smem);
+    load_series<T>(traj, n, S, start, w, p0, G, np, lds);
+    __syncthreads();
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int64_t nb = w / 2 + 1;  // rfft bins
+    for (int g = wave; g < G; g += kSeriesBlock / 64) {
+        const int64_t p = p0 + g;
+        if (p >= np) break;  // wave-uniform
+        const T *x = lds + (int64_t)g * w;
+        double mean, ss;
+        series_moments<T>(x, w, mean, ss);
+        // best and runner-up over this lane's bins, then across the wave
+        double best = -1.0;
+        int64_t best_b = 0;
+        // powers are kept for the second search: recomputed (cheap relative
+        // to memory) rather than stored
+        for (int pass = 0; pass < 2; pass++) {
+            int64_t first = 0;
+            if (pass == 1) {
+                // wave argmax of the first pass: largest power, lowest bin
+                double bv = best;
+                int64_t bb = best_b;
+#pragma unroll
+                for (int m = 32; m >= 1; m >>= 1) {
+                    const double ov = __shfl_xor(bv, m, 64);
+                    const int64_t ob = __shfl_xor(bb, m, 64);
+                    if (ov > bv || (ov == bv && ob < bb)) {
+                        bv = ov;
+                        bb = ob;
+                    }
+                }
+                if (!(bv >= 1e-10)) {  // no spectral content (or NaN)
+                    if (lane == 0) out_ratio[p] = (T)0;
+                    break;
+                }
+                first = bb;
+                best = -1.0;
+                best_b = 0;
+            }
+            for (int64_t b0 = 1 + lane * 4; b0 < nb; b0 += 64 * 4) {
+                double cw[4], s1[4], s2[4];
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    cw[q] = 2.0 * cospi(2.0 * (double)(b0 + q) / (double)w);
+                    s1[q] = 0.0;
+                    s2[q] = 0.0;
+                }
+                for (int64_t i = 0; i < w; i++) {
+                    const double c = (double)x[i] - mean;
+#pragma unroll
+                    for (int q = 0; q < 4; q++) {
+                        const double s0 = fma(cw[q], s1[q], c - s2[q]);
+                        s2[q] = s1[q];
+                        s1[q] = s0;
+                    }
+                }
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    const int64_t b = b0 + q;
+                    if (b >= nb) continue;
+                    if (pass == 1 && (b - first <= 1 && first - b <= 1)) continue;
+                    const double pw = fma(s1[q], s1[q], s2[q] * s2[q]) - cw[q] * s1[q] * s2[q];
+                    if (pw > best) {
+                        best = pw;
+                        best_b = b;
+                    }
+                }
+            }
+            if (pass == 1) {
+                double bv = best;
+                int64_t bb = best_b;
+#pragma unroll
+                for (int m = 32; m >= 1; m >>= 1) {
+                    const double ov = __shfl_xor(bv, m, 64);
+                    const int64_t ob = __shfl_xor(bb, m, 64);
+                    if (ov > bv || (ov == bv && ob < bb)) {
+                        bv = ov;
+                        bb = ob;
+                    }
+                }
+                // bin 0 (power set to zero) is a candidate too: when no bin
+                // two or more away has positive power, the runner-up is a
+                // zero-power bin and the ratio is taken as 0
+                const int64_t second = bv > 0.0 ? bb : 0;
+                if (lane == 0) out_ratio[p] = (T)((double)second / (double)first);
+            }
+        }
+    }
+}
+
 }  // namespace win
 }  // namespace bh

@@ -217,7 +217,8 @@ class BasinStabilityEstimator:
             else:
                 res = self.solver.run_fused(self.ode_system, batch_p, save_ts, params,
                                             keep_trajectories=self.keep_trajectories,
-                                            time_steady=steady)
+                                            time_steady=steady,
+                                            trend=bool(getattr(fx, "needs_trend", False)))
                 extra_orbit = None
             if res.y is not None:
                 FUSED_RESULTS[res.y] = res

@@ -29,14 +29,27 @@ DEFAULT_STEADY_FRACTION = 0.85
 
 # every feature here is computable from mean / std / min / max
 STREAMING_FEATURES = ("mean", "standard_deviation", "variance", "root_mean_square", "minimum",
-                      "maximum", "absolute_maximum", "delta", "log_delta")
-# order statistics of the window (need the saved window: keep_trajectories)
-WINDOW_FEATURES = ("median", "quantile")
+                      "maximum", "absolute_maximum", "delta", "log_delta", "amplitude")
+# the least-squares slope of the window, a fifth streaming statistic of the
+# kernel (integrate with trend=True); attr "slope" only
+TREND_FEATURES = ("linear_trend",)
+# features of the whole window series (need the saved window: keep_trajectories):
+# order statistics, autocorrelation and spectrum
+WINDOW_FEATURES = ("median", "quantile", "autocorrelation_periodicity", "spectral_frequency_ratio")
 DEFAULT_FEATURES = {"mean": None, "standard_deviation": None}
 # the reference's default set (settings.py:376-395: minimal features + delta, log_delta)
 MINIMAL_FEATURES = dict.fromkeys(["median", "mean", "standard_deviation", "variance",
                                   "root_mean_square", "maximum", "absolute_maximum", "minimum",
                                   "delta", "log_delta"])
+# the reference's dynamical-system set (settings.py:407-417)
+DYNAMICAL_SYSTEM_FEATURES = {
+    "variance": None,
+    "amplitude": None,
+    "mean": None,
+    "linear_trend": [{"attr": "slope"}],
+    "autocorrelation_periodicity": [{"output": "strength"}, {"output": "period"}],
+    "spectral_frequency_ratio": None,
+}
 
 
 class Solution:
@@ -88,15 +101,24 @@ def impute_extreme(features: torch.Tensor, extreme_value: float = 1e10) -> torch
     return out
 
 
-def stats_from_window(window: torch.Tensor) -> torch.Tensor:
-    """(N', B, S) -> (B, S, 4) float64: mean, std (ddof 0), min, max."""
+def stats_from_window(window: torch.Tensor, trend: bool = False) -> torch.Tensor:
+    """(N', B, S) -> (B, S, 4) float64: mean, std (ddof 0), min, max; with
+    trend=True (B, S, 5), the least-squares slope against the sample
+    position appended (torch_features_trend.py:11-27)."""
     w = window.to(torch.float64)
-    return torch.stack([w.mean(0), w.std(0, correction=0), w.min(0).values, w.max(0).values], -1)
+    cols = [w.mean(0), w.std(0, correction=0), w.min(0).values, w.max(0).values]
+    if trend:
+        n = w.shape[0]
+        t = torch.arange(n, dtype=torch.float64, device=w.device)
+        tc = (t - t.mean()).reshape(n, 1, 1)
+        ss_tt = float(((t - t.mean()) ** 2).sum())
+        cols.append((tc * (w - cols[0])).sum(0) / (ss_tt + 1e-10))
+    return torch.stack(cols, -1)
 
 
 def feature_from_stats(name: str, stats: torch.Tensor) -> torch.Tensor:
     """One streaming feature (torch_features_statistical.py:39-110) from (B, S, 4) stats."""
-    mean, std, mn, mx = stats.unbind(-1)
+    mean, std, mn, mx = stats.unbind(-1)[:4]
     if name == "mean":
         return mean
     if name == "standard_deviation":
@@ -115,6 +137,8 @@ def feature_from_stats(name: str, stats: torch.Tensor) -> torch.Tensor:
         return (mx - mean).abs()
     if name == "log_delta":
         return torch.log((mx - mean).abs() + 1e-10)
+    if name == "amplitude":
+        return mx - mn
     raise KeyError(f"feature {name!r} is not a streaming feature; available: {STREAMING_FEATURES}")
 
 
@@ -152,13 +176,103 @@ def window_order_stats(y: torch.Tensor, start: int, ranks: list[int]) -> torch.T
     return torch.where(torch.isnan(w).any(0, keepdim=True), torch.full_like(out, float("nan")), out)
 
 
+def _local_maxima_first(ac: torch.Tensor, min_lag: int, thr: float):
+    """First local maximum (scipy's definition, plateaus at their midpoint;
+    torch_feature_utilities.py:146-211) of each column of ac (N, M) at an
+    index >= min_lag with a value > thr: (values, indices), 0 where none."""
+    n, m = ac.shape
+    val = torch.zeros(m, dtype=ac.dtype)
+    idx = torch.zeros(m, dtype=ac.dtype)
+    a = ac.cpu().tolist()
+    for j in range(m):
+        col = [row[j] for row in a]
+        i = 1
+        while i < n - 1:
+            if col[i - 1] < col[i]:
+                k = i + 1
+                while k < n - 1 and col[k] == col[i]:
+                    k += 1
+                if col[k] < col[i]:
+                    mid = (i + k - 1) // 2
+                    if mid >= min_lag and col[mid] > thr:
+                        val[j], idx[j] = col[mid], float(mid)
+                        break
+                    i = k
+            i += 1
+    return val, idx
+
+
+def _series_cpu(name: str, params: dict | None, w: torch.Tensor) -> torch.Tensor:
+    """Torch restatement of the series features on a CPU window (N', B, S)
+    (the CPU template path): the reference's FFT formulation."""
+    n, B, S = w.shape
+    x = w.to(torch.float64)
+    if name == "autocorrelation_periodicity":
+        p = params or {}
+        z = (x - x.mean(0, keepdim=True)) / (x.std(0, keepdim=True) + 1e-10)
+        f = torch.fft.rfft(z, n=2 * n, dim=0)
+        ac = torch.fft.irfft(f.real ** 2 + f.imag ** 2, n=2 * n, dim=0)[:n]
+        ac = ac / (ac[0:1] + 1e-10)
+        val, idx = _local_maxima_first(ac.reshape(n, -1), int(p.get("min_lag", 2)),
+                                       float(p.get("peak_threshold", 0.3)))
+        out = idx if p.get("output", "strength") == "period" else val
+        return out.reshape(B, S).to(w.dtype)
+    c = x - x.mean(0, keepdim=True)
+    f = torch.fft.rfft(c, dim=0)
+    pw = (f.real ** 2 + f.imag ** 2).reshape(f.shape[0], -1)
+    pw[0] = 0.0
+    out = torch.zeros(pw.shape[1], dtype=torch.float64)
+    bins = torch.arange(pw.shape[0])
+    for j in range(pw.shape[1]):
+        psd = pw[:, j]
+        if not float(psd.max()) >= 1e-10:
+            continue
+        first = int(torch.argmax(psd))
+        cand = torch.where((bins - first).abs() >= 2, psd, torch.full_like(psd, -1.0))
+        second = int(torch.argmax(cand))
+        if float(cand[second]) > 0 and first > 0:
+            out[j] = second / first
+    return out.reshape(B, S).to(w.dtype)
+
+
+def _series_hip(name: str, params: dict | None, y: torch.Tensor, start: int) -> torch.Tensor:
+    from . import _lib
+
+    y = y.contiguous()
+    N, B, S = y.shape
+    dt = _lib.BH_F32 if y.dtype == torch.float32 else _lib.BH_F64
+    s = torch.cuda.current_stream(y.device)
+    out = torch.empty((B, S), dtype=y.dtype, device=y.device)
+    with torch.cuda.device(y.device):
+        if name == "autocorrelation_periodicity":
+            p = params or {}
+            other = torch.empty_like(out)
+            strength, period = (out, other) if p.get("output", "strength") != "period" else (other, out)
+            _lib.check(_lib.lib().bh_window_autocorr_periodicity(
+                dt, y.data_ptr(), N, B, S, int(start), int(p.get("min_lag", 2)),
+                float(p.get("peak_threshold", 0.3)), strength.data_ptr(), period.data_ptr(),
+                s.cuda_stream), "bh_window_autocorr_periodicity")
+        else:
+            _lib.check(_lib.lib().bh_window_spectral_ratio(dt, y.data_ptr(), N, B, S, int(start),
+                                                            out.data_ptr(), s.cuda_stream),
+                       "bh_window_spectral_ratio")
+    return out
+
+
 def window_feature(name: str, params: dict | None, y: torch.Tensor, start: int) -> torch.Tensor:
-    """median / quantile of the window rows [start, N) of y (N, B, S) -> (B, S),
-    as x.median(dim=0).values and torch.quantile(x, q, dim=0)
-    (torch_features_statistical.py:30-38, 154-157)."""
+    """A feature of the window rows [start, N) of y (N, B, S) -> (B, S):
+    median / quantile as x.median(dim=0).values and torch.quantile(x, q, dim=0)
+    (torch_features_statistical.py:30-38, 154-157); autocorrelation_periodicity
+    (output strength or period; torch_features_autocorrelation.py:268-331) and
+    spectral_frequency_ratio (torch_features_frequency.py:390-441) by the HIP
+    series kernels on the device, by a torch restatement on the CPU."""
     n = y.shape[0] - start
     if n < 1:
         raise ValueError("empty feature window")
+    if name in ("autocorrelation_periodicity", "spectral_frequency_ratio"):
+        if y.device.type == "cuda":
+            return _series_hip(name, params, y, start)
+        return _series_cpu(name, params, y[start:])
     if name == "median":
         return window_order_stats(y, start, [(n - 1) // 2])[0]
     if name == "quantile":
@@ -198,9 +312,13 @@ class StatFeatureExtractor(FeatureExtractor):
                 if fdict is None:
                     continue
             for name, plist in fdict.items():
-                if name not in STREAMING_FEATURES and name not in WINDOW_FEATURES:
+                if name not in STREAMING_FEATURES + TREND_FEATURES + WINDOW_FEATURES:
                     raise KeyError(f"feature {name!r} is not supported; available: "
-                                   f"{STREAMING_FEATURES + WINDOW_FEATURES}")
+                                   f"{STREAMING_FEATURES + TREND_FEATURES + WINDOW_FEATURES}")
+                if name == "linear_trend":
+                    for prm in plist or [None]:
+                        if (prm or {}).get("attr", "slope") != "slope":
+                            raise KeyError("linear_trend: only attr 'slope' is supported")
                 for params in (plist if plist else [None]):
                     plan.append((s, name, params))
         return plan
@@ -215,6 +333,14 @@ class StatFeatureExtractor(FeatureExtractor):
                                      "keep_trajectories=True")
                 cols.append(window_feature(name, params, y, start)[:, s].to(stats.device,
                                                                              torch.float64))
+            elif name in TREND_FEATURES:
+                if stats.shape[-1] >= 5:
+                    cols.append(stats[:, s, 4])
+                elif y is not None:
+                    cols.append(stats_from_window(y[start:], trend=True)[:, s, 4].to(stats.device))
+                else:
+                    raise ValueError("linear_trend needs the slope statistic: integrate with "
+                                     "trend=True (or keep_trajectories=True)")
             else:
                 cols.append(feature_from_stats(name, stats)[:, s])
         self._names = [feature_name(s, name, params) for s, name, params in plan]
@@ -226,14 +352,23 @@ class StatFeatureExtractor(FeatureExtractor):
         return list(self._names)
 
     def _stats(self, solution: Solution) -> torch.Tensor:
-        return stats_from_window(self.filter_time(solution))
+        return stats_from_window(self.filter_time(solution), trend=self.needs_trend)
+
+    def _requested(self) -> set:
+        dicts = [self.features] + list(self.features_per_state.values())
+        return {name for d in dicts if d for name in d}
 
     @property
     def needs_window(self) -> bool:
-        """Whether some requested feature is a window order statistic (the
-        saved window must be kept)."""
-        dicts = [self.features] + list(self.features_per_state.values())
-        return any(name in WINDOW_FEATURES for d in dicts if d for name in d)
+        """Whether some requested feature reads the whole window series
+        (order statistics, autocorrelation, spectrum: the saved window must
+        be kept)."""
+        return any(name in WINDOW_FEATURES for name in self._requested())
+
+    @property
+    def needs_trend(self) -> bool:
+        """Whether the slope statistic is requested (integrate with trend=True)."""
+        return any(name in TREND_FEATURES for name in self._requested())
 
     def _normalized(self, feats: torch.Tensor) -> torch.Tensor:
         # z-score with the mean / std of the first batch seen (then fixed)
@@ -261,10 +396,10 @@ class HipFusedFeatureExtractor(StatFeatureExtractor):
         res: FusedResult | None = FUSED_RESULTS.get(solution.y)
         if res is not None and res.stats is not None:
             want = steady_index(solution.time, self._resolve_time_steady(solution))
-            if want == res.steady_idx:
+            if want == res.steady_idx and (res.stats.shape[-1] >= 5 or not self.needs_trend):
                 return res.stats
             if res.y is not None:
-                return stats_from_window(res.y[want:])
+                return stats_from_window(res.y[want:], trend=self.needs_trend)
         return super()._stats(solution)
 
     def extract_from_result(self, res: FusedResult) -> torch.Tensor:

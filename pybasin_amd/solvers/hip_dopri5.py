@@ -115,6 +115,7 @@ class FusedResult:
     y       (n_save, n, S) trajectories, or None when not kept
     final   (n, S) state at the last save time
     stats   (n, S, 4) float64: mean, std (ddof 0), min, max over the window
+            ((n, S, 5) with the least-squares slope when run with trend=True)
     peak_values / peak_counts  orbit peaks over the window when requested
     status  (n,) int32 lane status (BH_LANE_*)
     nsteps  (n,) int64 attempted steps per trajectory
@@ -225,7 +226,7 @@ class HipDopri5Solver(Solver):
 
     def integrate_fused(self, ode_system, y0: torch.Tensor, params: torch.Tensor | None = None,
                         *, time_steady: float | None = None,
-                        keep_trajectories: bool = False) -> FusedResult:
+                        keep_trajectories: bool = False, trend: bool = False) -> FusedResult:
         """integrate() with the feature window fused into the kernel; trajectories
         reach HBM only when keep_trajectories is set.  With a cache_dir the
         result is read from / written to the on-disk cache (a hit skips the
@@ -236,24 +237,27 @@ class HipDopri5Solver(Solver):
         save_ts, y0 = self._prepare_tensors(y0)
         if self._cache_manager is None:
             return self.run_fused(ode_system, y0, save_ts, params,
-                                  keep_trajectories=keep_trajectories, time_steady=time_steady)
+                                  keep_trajectories=keep_trajectories, time_steady=time_steady,
+                                  trend=trend)
         cfg = dict(self._get_cache_config(), fused=True,
                    steady_idx=steady_index(save_ts, time_steady),
-                   keep_trajectories=bool(keep_trajectories))
+                   keep_trajectories=bool(keep_trajectories), trend=bool(trend))
         key = self._cache_manager.build_key(type(self).__name__, ode_system, y0, save_ts, cfg,
                                             params)
         hit = self._cache_manager.load_fused(key, self.device)
         if hit is not None:
             return hit
         res = self.run_fused(ode_system, y0, save_ts, params, keep_trajectories=keep_trajectories,
-                             time_steady=time_steady)
+                             time_steady=time_steady, trend=trend)
         self._cache_manager.save_fused(key, res)
         return res
 
     def run_fused(self, ode_system, y0: torch.Tensor, save_ts: torch.Tensor,
                   params: torch.Tensor | None, *, keep_trajectories: bool,
                   time_steady: float | None = None, steady_idx: int | None = None,
-                  stream: torch.cuda.Stream | None = None) -> FusedResult:
+                  stream: torch.cuda.Stream | None = None, trend: bool = False) -> FusedResult:
+        """One fused launch on prepared tensors.  trend=True adds the window's
+        least-squares slope per state as a fifth statistic (BH_FLAG_TREND)."""
         sys_id = functor_id(ode_system)
         if sys_id is None:
             raise ValueError(f"{type(ode_system).__name__} has no HIP device functor "
@@ -265,7 +269,8 @@ class HipDopri5Solver(Solver):
                                keep_trajectories=keep_trajectories, stream=stream)
         plan = self.plan_fused(ode_system, y0, save_ts, params, time_steady=time_steady,
                                steady_idx=steady_idx)
-        return self.launch_fused(plan, keep_trajectories=keep_trajectories, stream=stream)
+        return self.launch_fused(plan, keep_trajectories=keep_trajectories, stream=stream,
+                                 extra_flags=_lib.BH_FLAG_TREND if trend else 0)
 
     def plan_fused(self, ode_system, y0: torch.Tensor, save_ts: torch.Tensor,
                    params: torch.Tensor | None, *, time_steady: float | None = None,
@@ -299,9 +304,10 @@ class HipDopri5Solver(Solver):
         dtype_id = _lib.BH_F32 if y0.dtype == torch.float32 else _lib.BH_F64
         n, S = y0.shape
         n_save = st64.shape[0]
+        nf = _lib.BH_NFEAT_TREND if extra_flags & _lib.BH_FLAG_TREND else _lib.BH_NFEAT
         out = dict(
             traj=torch.empty((n_save, n, S), dtype=y0.dtype, device=dev) if keep_trajectories else None,
-            stats=torch.empty((n, S, _lib.BH_NFEAT), dtype=torch.float64, device=dev),
+            stats=torch.empty((n, S, nf), dtype=torch.float64, device=dev),
             final=torch.empty((n, S), dtype=y0.dtype, device=dev),
             status=torch.empty((n,), dtype=torch.int32, device=dev),
             nsteps=torch.empty((n,), dtype=torch.int64, device=dev),

@@ -1,0 +1,121 @@
+"""GPU parity of the dynamical-system feature set (ts_torch/settings.py:407-417).
+
+* amplitude and the linear-trend slope are streaming statistics of the
+  integration kernel (the slope under BH_FLAG_TREND): equal to the same
+  statistics of the kernel's own kept window;
+* autocorrelation_periodicity and spectral_frequency_ratio are series
+  kernels over a kept window (bh_window_autocorr_periodicity,
+  bh_window_spectral_ratio): against the oracle's restatements (the
+  reference's FFT formulation, pinned by closed forms) on integrated
+  Duffing windows and on closed-form signals.
+"""
+
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as O
+from pybasin_amd import _lib
+from pybasin_amd.features import (DYNAMICAL_SYSTEM_FEATURES, HipFusedFeatureExtractor, Solution,
+                                  StatFeatureExtractor, stats_from_window, window_feature)
+from pybasin_amd.solvers import HipDopri5Solver
+from pybasin_amd.systems import DuffingODE, LorenzODE
+
+pytestmark = pytest.mark.gpu
+DUFF = {"delta": 0.08, "k3": 1.0, "A": 0.2}
+
+
+def _duffing(dtype, n=2000, seed=9):
+    y0 = torch.tensor(O.sample_uniform(n, [-1.0, -0.5], [1.0, 1.0], seed=seed), dtype=dtype).cuda()
+    solver = HipDopri5Solver(t_span=(0, 400), t_steps=1000, t_eval=(200.0, 400.0), device="cuda",
+                             cache_dir=None)
+    return solver.integrate_fused(DuffingODE(DUFF), y0, time_steady=300.0, keep_trajectories=True,
+                                  trend=True)
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_fused_slope_and_amplitude_equal_kept_window(cuda_device, dtype):
+    res = _duffing(dtype)
+    assert res.stats.shape[-1] == _lib.BH_NFEAT_TREND
+    w = res.y[res.steady_idx:]
+    ref = stats_from_window(w, trend=True)
+    scale = w.double().abs().amax(0) / w.shape[0]
+    np.testing.assert_allclose(res.stats[..., 4].cpu().numpy(), ref[..., 4].cpu().numpy(), rtol=1e-7,
+                               atol=float(1e-9 * scale.max()))
+    torch.testing.assert_close(res.stats[..., :4], ref[..., :4], rtol=1e-9, atol=1e-9)
+
+
+def test_fused_slope_of_stopped_lanes(cuda_device):
+    """Lanes stopped by the divergence test fill the rest of the window at
+    once; the slope's position sums cover those samples too."""
+    y0 = torch.tensor(O.sample_uniform(512, [-10.0, -20.0, 0.0], [10.0, 20.0, 0.0], seed=4)).cuda()
+    solver = HipDopri5Solver(t_span=(0, 60), t_steps=300, t_eval=(20.0, 60.0), device="cuda",
+                             cache_dir=None, stop_threshold=200.0)
+    res = solver.integrate_fused(LorenzODE({"sigma": 0.12, "r": 0.0, "b": -0.6}), y0,
+                                 time_steady=30.0, keep_trajectories=True, trend=True)
+    assert (res.status == _lib.BH_LANE_DIVERGED).any() and (res.status == 0).any()
+    ref = stats_from_window(res.y[res.steady_idx:], trend=True)
+    torch.testing.assert_close(res.stats[..., 4], ref[..., 4], rtol=1e-7, atol=1e-9)
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_series_kernels_match_oracle_on_duffing_windows(cuda_device, dtype):
+    res = _duffing(dtype)
+    si = res.steady_idx
+    w = res.y[si:].double().cpu().numpy()
+    np_dt = np.float32 if dtype == torch.float32 else np.float64
+    strength, period = O.autocorrelation_periodicity(w)
+    # outputs are in the window's dtype
+    ratio = O.spectral_frequency_ratio(w).astype(np_dt).astype(np.float64)
+    g_s = window_feature("autocorrelation_periodicity", {"output": "strength"}, res.y, si).double().cpu().numpy()
+    g_p = window_feature("autocorrelation_periodicity", {"output": "period"}, res.y, si).double().cpu().numpy()
+    g_r = window_feature("spectral_frequency_ratio", None, res.y, si).double().cpu().numpy()
+    same_p = g_p == period
+    same_r = g_r == ratio
+    print(f"[result] {dtype}: periods equal {same_p.mean():.5f}, ratios equal {same_r.mean():.5f}, "
+          f"max strength diff where periods agree {np.abs(g_s - strength)[same_p].max():.2e}; "
+          f"periods seen {sorted(set(period.ravel().tolist()))[:8]}", flush=True)
+    assert same_p.mean() >= 0.999 and same_r.mean() >= 0.995
+    tol = 1e-4 if dtype == torch.float32 else 1e-9
+    assert np.abs(g_s - strength)[same_p].max() <= tol
+
+
+def test_series_kernels_closed_forms(cuda_device):
+    n = 1000
+    i = torch.arange(n, dtype=torch.float64)
+    sig = torch.stack([torch.sin(2 * math.pi * i / 50),
+                       torch.sin(2 * math.pi * 10 * i / n) + 0.5 * torch.sin(2 * math.pi * 30 * i / n),
+                       torch.sin(2 * math.pi * 20 * i / n) + 0.6 * torch.sin(2 * math.pi * 10 * i / n),
+                       torch.full((n,), 4.0)], 1)[:, :, None]  # (n, 4, 1)
+    for dtype in (torch.float32, torch.float64):
+        y = sig.to(dtype).cuda()
+        per = window_feature("autocorrelation_periodicity", {"output": "period"}, y, 0)[:, 0].cpu()
+        st = window_feature("autocorrelation_periodicity", {"output": "strength"}, y, 0)[:, 0].cpu()
+        rat = window_feature("spectral_frequency_ratio", None, y, 0)[:, 0].cpu()
+        assert per[0] == 50 and abs(float(st[0]) - 0.95) < 5e-3
+        assert per[3] == 0 and st[3] == 0  # constant: no peak
+        assert rat[1] == 3.0 and rat[2] == 0.5 and rat[3] == 0.0
+
+
+def test_dynamical_system_set_fused_equals_cpu(cuda_device):
+    res = _duffing(torch.float32, n=600, seed=2)
+    fx = HipFusedFeatureExtractor(time_steady=300.0, features=DYNAMICAL_SYSTEM_FEATURES)
+    assert fx.needs_window and fx.needs_trend
+    got = fx.extract_from_result(res).cpu()
+    cpu = StatFeatureExtractor(time_steady=300.0, features=DYNAMICAL_SYSTEM_FEATURES)
+    y = res.y.cpu()
+    want = cpu.extract_features(Solution(y[0], res.t.cpu(), y))
+    assert fx.feature_names == cpu.feature_names and len(fx.feature_names) == 14
+    for j, name in enumerate(fx.feature_names):
+        if name.endswith("output_period") or name.endswith("spectral_frequency_ratio"):
+            assert (got[:, j] == want[:, j]).float().mean() >= 0.995, name
+        else:
+            torch.testing.assert_close(got[:, j], want[:, j], rtol=2e-4, atol=2e-5, msg=name)
+
+
+def test_series_window_limit_is_an_error(cuda_device):
+    y = torch.zeros((20_000, 2, 1), dtype=torch.float32, device="cuda")
+    with pytest.raises(ValueError, match="LDS budget"):
+        window_feature("spectral_frequency_ratio", None, y, 0)

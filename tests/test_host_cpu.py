@@ -387,3 +387,39 @@ def test_zscore_keeps_constant_columns_at_unit_scale():
     y2[:, :, 1] = 3.0
     f2 = fx.extract_features(Solution(y2[0], t, y2))
     assert torch.allclose(f2[:, 1], torch.full((3,), 1.0))
+
+
+def test_dynamical_system_feature_set_on_cpu_equals_oracle():
+    """The reference's dynamical-system set (settings.py:407-417) through the
+    CPU path of StatFeatureExtractor, column by column against the oracle's
+    restatements (each pinned by closed forms in test_oracle_golden.py)."""
+    from pybasin_amd.features import DYNAMICAL_SYSTEM_FEATURES
+
+    g = torch.Generator().manual_seed(0)
+    t = torch.linspace(0, 60, 600)
+    B = 6
+    amp = torch.rand(B, generator=g) + 0.5
+    per = torch.tensor([3.0, 5.0, 7.0, 2.0, 11.0, 4.0])
+    y0 = torch.stack([amp[None, :] * torch.sin(2 * math.pi * t[:, None] / per[None, :]) + 0.01 * t[:, None],
+                      torch.cos(2 * math.pi * t[:, None] / 6.0).expand(600, B) * amp[None, :]], -1)
+    y = y0 + 0.01 * torch.randn(600, B, 2, generator=g)
+    sol = Solution(y[0], t, y)
+    fx = StatFeatureExtractor(time_steady=20.0, features=DYNAMICAL_SYSTEM_FEATURES)
+    f = fx.extract_features(sol).double()
+    assert fx.feature_names[:7] == [
+        "state_0__variance", "state_0__amplitude", "state_0__mean", "state_0__linear_trend__attr_slope",
+        "state_0__autocorrelation_periodicity__output_strength",
+        "state_0__autocorrelation_periodicity__output_period", "state_0__spectral_frequency_ratio"]
+    start = int(torch.nonzero(t >= 20.0)[0])
+    w = y[start:].double().numpy()
+    strength, period = O.autocorrelation_periodicity(w)
+    want = {"variance": w.var(0), "amplitude": O.amplitude(w), "mean": w.mean(0),
+            "linear_trend__attr_slope": O.linear_trend_slope(w),
+            "autocorrelation_periodicity__output_strength": strength,
+            "autocorrelation_periodicity__output_period": period,
+            "spectral_frequency_ratio": O.spectral_frequency_ratio(w)}
+    for j, name in enumerate(fx.feature_names):
+        s, feat = int(name.split("__")[0][6:]), name.split("__", 1)[1]
+        np.testing.assert_allclose(f[:, j].numpy(), want[feat][:, s].astype(np.float32),
+                                   rtol=2e-5, atol=2e-6, err_msg=name)
+    assert fx.needs_window and fx.needs_trend

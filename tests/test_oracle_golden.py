@@ -337,3 +337,59 @@ def test_basin_stability_closed_form():
     bs = O.basin_stability(np.array(["a"] * 3 + ["b"] * 1))
     assert bs["a"] == (0.75, pytest.approx(math.sqrt(0.75 * 0.25 / 4)))
     assert bs["b"] == (0.25, pytest.approx(math.sqrt(0.75 * 0.25 / 4)))
+
+
+# ---- the dynamical-system feature set (ts_torch/settings.py:407-417) ------------
+
+def test_local_maxima_1d_matches_scipy_find_peaks():
+    """scipy.signal.find_peaks with no conditions returns exactly the
+    midpoints of scipy's _local_maxima_1d, the definition the reference
+    restates: an independent library check, plateaus included."""
+    from scipy.signal import find_peaks
+
+    assert O.local_maxima_1d([0, 1, 1, 0]).tolist() == [1]
+    assert O.local_maxima_1d([0, 1, 1, 1, 0]).tolist() == [2]
+    assert O.local_maxima_1d([0, 1, 2, 2]).tolist() == []
+    assert O.local_maxima_1d([0, 2, 1]).tolist() == [1]
+    rng = np.random.default_rng(0)
+    for _ in range(300):
+        x = rng.integers(0, 4, rng.integers(3, 40)).astype(np.float64)
+        assert O.local_maxima_1d(x).tolist() == find_peaks(x)[0].tolist(), x
+
+
+def test_amplitude_and_slope_closed_forms():
+    i = np.arange(500, dtype=np.float64)
+    w = np.stack([3.0 - 0.25 * i, np.full_like(i, 2.0), np.sin(2 * np.pi * i / 4)], 1)[:, None, :]
+    np.testing.assert_allclose(O.linear_trend_slope(w)[0], [-0.25, 0.0, 0.0], atol=1e-3)
+    assert abs(O.linear_trend_slope(w)[0, 0] + 0.25) < 1e-12  # a ramp's slope is exact
+    np.testing.assert_allclose(O.amplitude(w)[0], [0.25 * 499, 0.0, 2.0], atol=1e-12)
+
+
+def test_autocorrelation_periodicity_closed_forms():
+    n, P = 1000, 50
+    i = np.arange(n, dtype=np.float64)
+    w = np.stack([np.sin(2 * np.pi * i / P), np.full(n, 1.5), np.cos(2 * np.pi * i / 37.0)], 1)[:, None, :]
+    strength, period = O.autocorrelation_periodicity(w)
+    # a sinusoid of period P samples: first peak at lag P, height (n - P) / n
+    # (the overlap of the finite autocorrelation), here 0.95
+    assert period[0, 0] == P and abs(strength[0, 0] - (n - P) / n) < 5e-3
+    assert period[0, 1] == 0 and strength[0, 1] == 0  # constant: no peak
+    assert period[0, 2] == 37 and strength[0, 2] > 0.9
+    # a slowly decaying signal has no peak above the threshold
+    s2, p2 = O.autocorrelation_periodicity(np.exp(-i / 100.0)[:, None, None])
+    assert p2[0, 0] == 0 and s2[0, 0] == 0
+
+
+def test_spectral_frequency_ratio_closed_forms():
+    n = 1000
+    i = np.arange(n, dtype=np.float64)
+    w = np.stack([np.sin(2 * np.pi * 10 * i / n) + 0.5 * np.sin(2 * np.pi * 30 * i / n),
+                  np.sin(2 * np.pi * 20 * i / n) + 0.6 * np.sin(2 * np.pi * 10 * i / n),
+                  np.full(n, 4.0),
+                  np.sin(2 * np.pi * 12 * i / n)], 1)[:, None, :]
+    r = O.spectral_frequency_ratio(w)[0]
+    assert r[0] == 3.0  # third harmonic
+    assert r[1] == 0.5  # subharmonic
+    assert r[2] == 0.0  # no spectral content
+    # a pure tone: the runner-up is a rounding-level bin; the ratio is finite
+    assert np.isfinite(r[3])
