@@ -423,3 +423,48 @@ def test_dynamical_system_feature_set_on_cpu_equals_oracle():
         np.testing.assert_allclose(f[:, j].numpy(), want[feat][:, s].astype(np.float32),
                                    rtol=2e-5, atol=2e-6, err_msg=name)
     assert fx.needs_window and fx.needs_trend
+
+
+def test_pendulum_float_sine_is_within_bound():
+    """The pendulum functor's float sine (Cody-Waite reduction by pi/2 in
+    three FMA steps, polynomials on |r| <= 0.8; constants read from
+    systems.hpp), evaluated in float32 in the kernel's order with exact FMAs,
+    stays within 7.5e-8 of sin x for |x| <= 2^17, near multiples of pi/2
+    included; a dropped reduction step or polynomial term breaks the bound."""
+    import os
+    import re
+
+    src = open(os.path.join(os.path.dirname(__file__), "..", "pybasin_amd", "csrc",
+                            "systems.hpp")).read()
+    body = src[src.index("float sin_cw(float x)"):]
+    body = body[:body.index("\n}")]
+    lit = [np.float32(v) for v in re.findall(r"(-?\d\.\d+e[-+]\d+)f", body)]
+    assert len(lit) == 11, lit
+    two_pi_inv, c1, c2, c3, s2, s1, s0, k3, k2, k1, k0 = lit
+    f = np.float32
+
+    def fma(a, b, c):
+        return (np.float64(a) * np.float64(b) + np.float64(c)).astype(f)
+
+    def sin_cw(x, drop=None):
+        k = np.rint((x * two_pi_inv).astype(f)).astype(f)
+        r = fma(-k, c1, x)
+        r = fma(-k, c2, r) if drop != "c2" else r
+        r = fma(-k, c3, r)
+        r2 = (r * r).astype(f)
+        ps = fma(r2, fma(r2, s2 if drop != "s2" else f(0), s1), s0)
+        sv = fma((r2 * r).astype(f), ps, r)
+        cv = fma(r2, fma(r2, fma(r2, fma(r2, k3, k2), k1), k0), f(1))
+        q = k.astype(np.int64)
+        v = np.where(q & 1, cv, sv)
+        return np.where(q & 2, -v, v)
+
+    assert k0 == f(-0.5)
+    rng = np.random.default_rng(3)
+    x = rng.uniform(-131072, 131072, 400_000).astype(f)
+    m = (rng.integers(-80_000, 80_000, 100_000) * np.pi / 2).astype(f)
+    x = np.concatenate([x, m, np.nextafter(m, f(np.inf)), rng.uniform(-4, 4, 100_000).astype(f)])
+    t = np.sin(x.astype(np.float64))
+    assert np.abs(sin_cw(x) - t).max() <= 7.5e-8
+    for d in ("c2", "s2"):
+        assert np.abs(sin_cw(x, drop=d) - t).max() > 1e-6, d
