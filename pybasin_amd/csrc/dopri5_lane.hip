@@ -43,6 +43,8 @@
 #include <math.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "../../include/basinhip.h"
 #include "systems.hpp"
 
@@ -165,6 +167,7 @@ struct Lane {
     int32_t si;
     int32_t status;
     bool done;
+    bool oor;  // a float step took an argument outside the branch-free forms' range
     // streaming statistics over the feature window
     double shift[S], s1[S], s2[S], s3[S];  // s3: sum of window position x (value - shift)
     T mn[S], mx[S];
@@ -262,6 +265,7 @@ __device__ __forceinline__ void lane_init(const LaneArgs &a, LaneT &L, int64_t i
     L.cnt = 0;
     L.status = BH_LANE_OK;
     L.done = false;
+    L.oor = false;
     // f0 and the initial step (torchdiffeq _select_initial_step, order 4)
     Sys::f(L.p, (T)a.t0, L.y, L.k1);
     const T ratol = (T)a.atol, rrtol = (T)a.rtol;
@@ -346,75 +350,88 @@ __device__ __forceinline__ void lane_fill_rest(const LaneArgs &a, LaneT &L, int6
     }
 }
 
-// One attempted step.  Sets L.done when the lane has finished.
-template <class Sys, class T, class LaneT>
-__device__ __forceinline__ void lane_step(const LaneArgs &a, LaneT &L, int64_t ic) {
+// The stages of one attempted step from (L.y, L.k1) with step h and the
+// step's time terms; returns the squared RMS error ratio (accept iff <= 1;
+// the controller takes its -1/10 power, which is the -1/5 power of the
+// ratio).  kFast takes the systems' branch-free float forms, which add
+// their argument magnitudes to amax.
+template <class Sys, class T, bool kFast>
+__device__ __forceinline__ T lane_stages(const LaneArgs &a, const T (&p)[Sys::NP], const T (&y)[Sys::S],
+                                         const T (&k1)[Sys::S], T h, const T (&aux)[5],
+                                         T (&k3)[Sys::S], T (&k4)[Sys::S], T (&k5)[Sys::S],
+                                         T (&k6)[Sys::S], T (&k7)[Sys::S], T (&y1)[Sys::S], T &amax) {
     constexpr int S = Sys::S;
-    const double tA = L.t, tB = L.t + L.dt;
-    const T tf = (T)tA, h = (T)L.dt, tBf = (T)tB, t1p = prev_value<T>(tBf);
-    T k2[S], k3[S], k4[S], k5[S], k6[S], k7[S], ys[S];
-    const T tst[4] = {tf + (T)C2 * h, tf + (T)C3 * h, tf + (T)C4 * h, tf + (T)C5 * h};
-    T aux[5];
-    // sin / cos of the end time: the c = 1 stages here, the start of the next
-    // step after an accept
-    T sB = 0, cB = 0;
-    if constexpr (Sys::kTime) {
-        // rotated from the step's start by d = tB - t0 (exact, as the stage
-        // offsets), with a fresh range-reduced sincos every 8th attempted
-        // step of the lane, so at most 7 rotations accumulate into (s0, c0)
-        const T dB = tBf - tf;
-        if ((L.steps & 7) == 0 || !(fabs(dB) <= T(1))) {
-            dsincos<T>(tBf, sB, cB);
-        } else {
-            T sd, cd;
-            sincos_unit<T>(dB, sd, cd);
-            sB = fma(L.s0, cd, L.c0 * sd);
-            cB = fma(L.c0, cd, -(L.s0 * sd));
-        }
-    }
-    Sys::time_terms(tf, tst, L.s0, L.c0, tBf, sB, cB, t1p, aux);
+    T k2[S], ys[S];
     // Stage inputs y + h * sum_j a_ij k_j: the tableau constants are
     // immediate operands and h is applied once per component.
 #pragma unroll
-    for (int i = 0; i < S; i++) ys[i] = fma(h, L.k1[i] * (T)A21, L.y[i]);
-    rhs_aux<Sys, T>(L.p, aux[0], ys, k2);
+    for (int i = 0; i < S; i++) ys[i] = fma(h, k1[i] * (T)A21, y[i]);
+    rhs_aux<Sys, T, kFast>(p, aux[0], ys, k2, amax);
 #pragma unroll
-    for (int i = 0; i < S; i++) ys[i] = fma(h, L.k1[i] * (T)A31 + k2[i] * (T)A32, L.y[i]);
-    rhs_aux<Sys, T>(L.p, aux[1], ys, k3);
-#pragma unroll
-    for (int i = 0; i < S; i++)
-        ys[i] = fma(h, L.k1[i] * (T)A41 + k2[i] * (T)A42 + k3[i] * (T)A43, L.y[i]);
-    rhs_aux<Sys, T>(L.p, aux[2], ys, k4);
+    for (int i = 0; i < S; i++) ys[i] = fma(h, k1[i] * (T)A31 + k2[i] * (T)A32, y[i]);
+    rhs_aux<Sys, T, kFast>(p, aux[1], ys, k3, amax);
 #pragma unroll
     for (int i = 0; i < S; i++)
-        ys[i] = fma(h, L.k1[i] * (T)A51 + k2[i] * (T)A52 + k3[i] * (T)A53 + k4[i] * (T)A54, L.y[i]);
-    rhs_aux<Sys, T>(L.p, aux[3], ys, k5);
+        ys[i] = fma(h, k1[i] * (T)A41 + k2[i] * (T)A42 + k3[i] * (T)A43, y[i]);
+    rhs_aux<Sys, T, kFast>(p, aux[2], ys, k4, amax);
 #pragma unroll
     for (int i = 0; i < S; i++)
-        ys[i] = fma(h, L.k1[i] * (T)A61 + k2[i] * (T)A62 + k3[i] * (T)A63 + k4[i] * (T)A64 +
-                           k5[i] * (T)A65, L.y[i]);
-    rhs_aux<Sys, T>(L.p, aux[4], ys, k6);
-    T y1[S];
+        ys[i] = fma(h, k1[i] * (T)A51 + k2[i] * (T)A52 + k3[i] * (T)A53 + k4[i] * (T)A54, y[i]);
+    rhs_aux<Sys, T, kFast>(p, aux[3], ys, k5, amax);
 #pragma unroll
     for (int i = 0; i < S; i++)
-        y1[i] = fma(h, L.k1[i] * (T)B1 + k3[i] * (T)B3 + k4[i] * (T)B4 + k5[i] * (T)B5 +
-                           k6[i] * (T)B6, L.y[i]);
-    rhs_aux<Sys, T>(L.p, aux[4], y1, k7);
-    // squared RMS error ratio (accept iff <= 1; the controller takes its
-    // -1/10 power, which is the -1/5 power of the ratio)
+        ys[i] = fma(h, k1[i] * (T)A61 + k2[i] * (T)A62 + k3[i] * (T)A63 + k4[i] * (T)A64 +
+                           k5[i] * (T)A65, y[i]);
+    rhs_aux<Sys, T, kFast>(p, aux[4], ys, k6, amax);
+#pragma unroll
+    for (int i = 0; i < S; i++)
+        y1[i] = fma(h, k1[i] * (T)B1 + k3[i] * (T)B3 + k4[i] * (T)B4 + k5[i] * (T)B5 +
+                           k6[i] * (T)B6, y[i]);
+    rhs_aux<Sys, T, kFast>(p, aux[4], y1, k7, amax);
     T acc = 0;
-    {
-        const T ratol = (T)a.atol, rrtol = (T)a.rtol;
+    const T ratol = (T)a.atol, rrtol = (T)a.rtol;
 #pragma unroll
-        for (int i = 0; i < S; i++) {
-            const T err = h * (L.k1[i] * (T)E1 + k3[i] * (T)E3 + k4[i] * (T)E4 + k5[i] * (T)E5 +
-                               k6[i] * (T)E6 + k7[i] * (T)E7);
-            const T tol = ratol + rrtol * fmax(fabs(L.y[i]), fabs(y1[i]));
-            const T r = err * fast_rcp<T>(tol);
-            acc += r * r;
-        }
+    for (int i = 0; i < S; i++) {
+        const T err = h * (k1[i] * (T)E1 + k3[i] * (T)E3 + k4[i] * (T)E4 + k5[i] * (T)E5 +
+                           k6[i] * (T)E6 + k7[i] * (T)E7);
+        const T tol = ratol + rrtol * fmax(fabs(y[i]), fabs(y1[i]));
+        const T r = err * fast_rcp<T>(tol);
+        acc += r * r;
     }
-    const T ratio = acc * (T)(1.0 / S);  // squared ratio
+    return acc * (T)(1.0 / S);
+}
+
+// One attempted step.  Sets L.done when the lane has finished.
+//
+// In float the stages use the systems' branch-free forms (the sine of the
+// pendulum, the time terms of Duffing): on a wave alone on its SIMD, which
+// is how the long trajectories of a launch finish, a divergent branch costs
+// about as much as fifteen arithmetic instructions, and the general forms
+// have one per evaluation.  A step whose arguments leave the branch-free
+// range marks the lane (L.oor); the lane's trajectory is then integrated
+// again from its initial state with the general forms (kExact) before its
+// results are written, so every result comes from in-range evaluations.
+template <class Sys, class T, bool kExact = false, class LaneT>
+__device__ __forceinline__ void lane_step(const LaneArgs &a, LaneT &L, int64_t ic) {
+    constexpr int S = Sys::S;
+    constexpr bool kFast = std::is_same<T, float>::value && !kExact;
+    const double tA = L.t, tB = L.t + L.dt;
+    const T tf = (T)tA, h = (T)L.dt, tBf = (T)tB, t1p = prev_value<T>(tBf);
+    const T tst[4] = {tf + (T)C2 * h, tf + (T)C3 * h, tf + (T)C4 * h, tf + (T)C5 * h};
+    T k3[S], k4[S], k5[S], k6[S], k7[S], y1[S], aux[5];
+    // sin / cos of the end time: the c = 1 stages here, the start of the next
+    // step after an accept
+    T sB, cB, amax = 0;
+    bool in_range = true;
+    if constexpr (kFast)
+        in_range = Sys::time_terms_fast(tf, tst, L.s0, L.c0, tBf, t1p, sB, cB, aux);
+    else
+        Sys::time_terms(L.steps, tf, tst, L.s0, L.c0, tBf, t1p, sB, cB, aux);
+    const T ratio = lane_stages<Sys, T, kFast>(a, L.p, L.y, L.k1, h, aux, k3, k4, k5, k6, k7, y1, amax);
+    if constexpr (kFast && (Sys::kTime || Sys::kFastRhs)) {
+        if (Sys::kFastRhs) in_range = in_range && amax <= kFastTrigMax;
+        L.oor = L.oor || !in_range;
+    }
     // Failure and acceptance are decided after the stages, so the common path
     // has one divergent branch (the dense output) and one exit test; the
     // state update is a select.
@@ -465,8 +482,9 @@ __device__ __forceinline__ void lane_step(const LaneArgs &a, LaneT &L, int64_t i
     T ymax = 0;
 #pragma unroll
     for (int i = 0; i < S; i++) {
-        L.y[i] = accept ? y1[i] : L.y[i];
-        L.k1[i] = accept ? k7[i] : L.k1[i];
+        const T yn = y1[i], yo = L.y[i], kn = k7[i], ko = L.k1[i];
+        L.y[i] = accept ? yn : yo;
+        L.k1[i] = accept ? kn : ko;
         ymax = fmax(ymax, fabs(L.y[i]));
     }
     L.t = accept ? tB : L.t;
@@ -478,8 +496,10 @@ __device__ __forceinline__ void lane_step(const LaneArgs &a, LaneT &L, int64_t i
     const bool finished = accept && L.si >= a.n_save;
     const bool stop = accept && !finished && (a.flags & BH_FLAG_STOP) && ymax >= (T)a.stop_thr;
     const bool maxed = L.steps >= (int32_t)a.max_steps;
-    if (bad || finished || stop || maxed) {
-        if (!finished) {
+    // a lane marked out of range ends here and is integrated again by
+    // lane_redo_exact before its results are written
+    if (bad || finished || stop || maxed || L.oor) {
+        if (!finished && !L.oor) {
             bool inf = false;
             if (bad) {
                 L.status = finite ? BH_LANE_DT_UNDERFLOW : BH_LANE_NONFINITE;
@@ -495,9 +515,23 @@ __device__ __forceinline__ void lane_step(const LaneArgs &a, LaneT &L, int64_t i
     }
 }
 
+// A lane marked by an out-of-range float step integrates its trajectory
+// again with the general forms (see lane_step); its first pass's writes to
+// the trajectory and peak buffers are all written again.
+template <class Sys, class T, class LaneT>
+__device__ __forceinline__ void lane_redo_exact(const LaneArgs &a, LaneT &L, int64_t ic) {
+    if constexpr (std::is_same<T, float>::value && (Sys::kTime || Sys::kFastRhs)) {
+        if (L.oor) {
+            lane_init<Sys, T>(a, L, ic);
+            while (!L.done) lane_step<Sys, T, true>(a, L, ic);
+        }
+    }
+}
+
 template <class Sys, class T, class LaneT>
 __device__ __forceinline__ void lane_finish(const LaneArgs &a, LaneT &L, int64_t ic) {
     constexpr int S = Sys::S;
+    lane_redo_exact<Sys, T>(a, L, ic);
     if (a.feats) {
         const bool trend = (a.flags & BH_FLAG_TREND) != 0;
         const int nf = trend ? BH_NFEAT_TREND : BH_NFEAT;

@@ -11,31 +11,64 @@
 
 namespace bh {
 
-// sin of a float argument: Cody-Waite reduction by pi/2 in three FMA steps
-// (pi/2 split into three floats, each product formed exactly inside its FMA;
-// quadrant k = rint(x * 2/pi)), then sin or cos of the remainder |r| <= 0.8
-// by polynomials of degree 7 and 8 fitted on [-0.8, 0.8] by Lawson-weighted
-// least squares (near-minimax; sin weighted by its relative error) and
-// rounded to float: within 7.3e-8 absolute of sin x for |x| <= 2^17 (one
-// unit in the last place of values near 1), checked on 4.5e6 sampled floats
-// including the neighbours of multiples of pi/2.  Larger |x| go to the
-// library sinf.  About a third fewer instructions than sinf on the common
-// path, which matters for the pendulum's six sines per step.
-__device__ __forceinline__ float sin_cw(float x) {
-    if (!(fabsf(x) <= 131072.0f)) return sinf(x);
+// Largest |x| for which the branch-free float sine and cosine below are
+// used; the step code checks its arguments against it and recomputes a
+// step with the library functions when one exceeds it.
+constexpr float kFastTrigMax = 131072.0f;
+
+// sin and cos of a float argument |x| <= kFastTrigMax, branch-free:
+// Cody-Waite reduction by pi/2 in three FMA steps (pi/2 split into three
+// floats, each product formed exactly inside its FMA; quadrant k = rint(x *
+// 2/pi)), then sin and cos of the remainder |r| <= 0.8 by polynomials of
+// degree 7 and 8 fitted on [-0.8, 0.8] by Lawson-weighted least squares
+// (near-minimax; sin weighted by its relative error) and rounded to float:
+// within 7.3e-8 absolute of sin x for |x| <= 2^17 (one unit in the last
+// place of values near 1), checked on 4.5e6 sampled floats including the
+// neighbours of multiples of pi/2.  About a third of the instructions of
+// the library sinf, and no branch: on a wave running alone on its SIMD a
+// branch on a fresh comparison costs about as much as ten arithmetic
+// instructions.
+__device__ __forceinline__ void cw_reduce(float x, float &r, int &q) {
     const float k = rintf(x * 6.36619747e-01f);
-    float r = fmaf(-k, 1.57079637e+00f, x);
+    r = fmaf(-k, 1.57079637e+00f, x);
     r = fmaf(-k, -4.37113883e-08f, r);
     r = fmaf(-k, -1.71512451e-15f, r);
-    const float r2 = r * r;
+    q = (int)k;
+}
+__device__ __forceinline__ float cw_sin_poly(float r, float r2) {
     const float ps = fmaf(r2, fmaf(r2, -1.95031665e-04f, 8.33207183e-03f), -1.66666538e-01f);
-    const float sv = fmaf(r2 * r, ps, r);
+    return fmaf(r2 * r, ps, r);
+}
+__device__ __forceinline__ float cw_cos_poly(float r2) {
     const float pc = fmaf(r2, fmaf(r2, fmaf(r2, 2.43752329e-05f, -1.38866028e-03f), 4.16666195e-02f),
                           -5.0e-01f);
-    const float cv = fmaf(r2, pc, 1.0f);
-    const int q = (int)k;
+    return fmaf(r2, pc, 1.0f);
+}
+__device__ __forceinline__ float sin_fast(float x) {
+    float r;
+    int q;
+    cw_reduce(x, r, q);
+    const float r2 = r * r;
+    const float sv = cw_sin_poly(r, r2), cv = cw_cos_poly(r2);
     const float v = (q & 1) ? cv : sv;
     return (q & 2) ? -v : v;
+}
+__device__ __forceinline__ void sincos_fast(float x, float &s, float &c) {
+    float r;
+    int q;
+    cw_reduce(x, r, q);
+    const float r2 = r * r;
+    const float sv = cw_sin_poly(r, r2), cv = cw_cos_poly(r2);
+    // quadrant q: sin = sv, cv, -sv, -cv; cos = cv, -sv, -cv, sv
+    const float a = (q & 1) ? cv : sv, b = (q & 1) ? sv : cv;
+    s = (q & 2) ? -a : a;
+    c = ((q + 1) & 2) ? -b : b;
+}
+// sin of any float: the branch-free form inside its range, the library
+// sinf beyond
+__device__ __forceinline__ float sin_cw(float x) {
+    if (!(fabsf(x) <= kFastTrigMax)) return sinf(x);
+    return sin_fast(x);
 }
 
 template <class T> __device__ __forceinline__ T dsin(T x);
@@ -87,17 +120,31 @@ template <> __device__ __forceinline__ void dsincos<double>(double x, double &s,
 
 // Per-step time terms.  A system whose right-hand side depends on t gets the
 // step's stage times (t0 + c_i h for c = 1/5, 3/10, 4/5, 8/9, in the state
-// dtype), the sine and cosine of t0 carried from the previous step, those of
-// the end time tB = (T)(t + dt) and the c = 1 stage time t1 (the previous
-// representable value of tB), and fills one auxiliary value per stage; the
-// others ignore them.
+// dtype), the sine and cosine of t0 carried from the previous step, the end
+// time tB = (T)(t + dt) and the c = 1 stage time t1 (the previous
+// representable value of tB); it returns the sine and cosine of tB (carried
+// to the next step after an accept) and fills one auxiliary value per stage.
+// The others ignore them.  time_terms is the form for any argument;
+// time_terms_fast (float) takes the end time's sine and cosine from the
+// branch-free form and returns false when |tB| is outside its range (the
+// step code then integrates the lane again with time_terms).
 struct NoTimeTerms {
     static constexpr bool kTime = false;
+    static constexpr bool kFastRhs = false;
     template <class T>
-    __device__ __forceinline__ static void time_terms(T, const T (&)[4], T, T, T, T, T, T,
-                                                      T (&aux)[5]) {
+    __device__ __forceinline__ static void time_terms(int32_t, T, const T (&)[4], T, T, T, T, T &sB,
+                                                      T &cB, T (&aux)[5]) {
+        sB = cB = T(0);
 #pragma unroll
         for (int i = 0; i < 5; i++) aux[i] = T(0);
+    }
+    template <class T>
+    __device__ __forceinline__ static bool time_terms_fast(T, const T (&)[4], T, T, T, T, T &sB, T &cB,
+                                                           T (&aux)[5]) {
+        sB = cB = T(0);
+#pragma unroll
+        for (int i = 0; i < 5; i++) aux[i] = T(0);
+        return true;
     }
 };
 
@@ -128,10 +175,19 @@ struct Linear2 : NoTimeTerms {
 struct Pendulum : NoTimeTerms {
     static constexpr int S = 2, NP = 3;
     static constexpr int FLOPS = 5;
+    static constexpr bool kFastRhs = true;
     template <class T>
     __device__ __forceinline__ static void f(const T (&p)[NP], T, const T (&y)[S], T (&dy)[S]) {
         dy[0] = y[1];
         dy[1] = -p[0] * y[1] + p[1] - p[2] * dsin<T>(y[0]);
+    }
+    // float, branch-free: the same terms with the sine of |theta| <=
+    // kFastTrigMax; amax collects |theta| for the step's range check
+    __device__ __forceinline__ static void f_fast(const float (&p)[NP], const float (&y)[S],
+                                                  float (&dy)[S], float &amax) {
+        dy[0] = y[1];
+        dy[1] = -p[0] * y[1] + p[1] - p[2] * sin_fast(y[0]);
+        amax = fmaxf(amax, fabsf(y[0]));
     }
 };
 
@@ -157,39 +213,90 @@ struct Duffing {
         dy[0] = y[1];
         dy[1] = fma(-(p[1] * y[0]), y[0] * y[0], fma(p[2], cos_t, -(p[0] * y[1])));
     }
+    static constexpr bool kFastRhs = false;
     template <class T>
-    __device__ __forceinline__ static void time_terms(T t0, const T (&ts)[4], T s0, T c0, T tB,
-                                                      T sB, T cB, T t1, T (&aux)[5]) {
+    __device__ __forceinline__ static void time_terms(int32_t steps, T t0, const T (&ts)[4], T s0,
+                                                      T c0, T tB, T t1, T &sB, T &cB, T (&aux)[5]) {
+        // sin / cos of the end time rotated from the step's start by
+        // d = tB - t0 (exact, as the stage offsets), with a fresh
+        // range-reduced sincos every 8th attempted step of the lane, so at
+        // most 7 rotations accumulate into (s0, c0)
+        const T dB = tB - t0;
+        if ((steps & 7) == 0 || !(fabs(dB) <= T(1))) {
+            dsincos<T>(tB, sB, cB);
+        } else {
+            T sd, cd;
+            sincos_unit<T>(dB, sd, cd);
+            sB = fma(s0, cd, c0 * sd);
+            cB = fma(c0, cd, -(s0 * sd));
+        }
         // cos(t0 + d_i) = cos t0 cos d_i - sin t0 sin d_i with d_i = t_i - t0
         // exact (Sterbenz) and sin/cos of d_i by one short polynomial.  The
         // c = 8/9 stage has the largest offset, so one test per lane decides;
         // nothing depends on the other lanes of the wave, so a trajectory's
-        // rounding does not depend on which ICs share its wave.
-        const T dmax = fabs(ts[3] - t0);
-        if (dmax <= T(1)) {
+        // rounding does not depend on which ICs share its wave.  The c = 1
+        // stage time t1 = tB - e (one unit in the last place below tB) takes
+        // cos t1 from (sin tB, cos tB) by the series of cos e and sin e while
+        // e <= 2^-6; a larger e (|t| beyond 2^17 in float) takes the direct
+        // cosine with the other stages.
+        const T dmax = fabs(ts[3] - t0), e = tB - t1;
+        if (dmax <= T(1) && e <= T(0.015625)) {
 #pragma unroll
             for (int i = 0; i < 4; i++) {
                 T sd, cd;
                 sincos_unit<T>(ts[i] - t0, sd, cd);
                 aux[i] = c0 * cd - s0 * sd;
             }
+            aux[4] = cos_prev(sB, cB, e);
         } else {
 #pragma unroll
             for (int i = 0; i < 4; i++) aux[i] = dcos<T>(ts[i]);
+            aux[4] = dcos<T>(t1);
         }
-        // t1 = tB - e with e one unit in the last place: cos e rounds to 1 and
-        // sin e to e, so cos t1 = cos tB + sin tB * e
-        aux[4] = fma(sB, tB - t1, cB);
+    }
+    // cos(tB - e) from sin tB, cos tB for 0 <= e <= 2^-6: cos e = 1 - e^2/2 and
+    // sin e = e - e^3/6 (the next terms are below 3e-9 relative); for e below
+    // 1e-8 both round to 1 and e exactly
+    template <class T>
+    __device__ __forceinline__ static T cos_prev(T sB, T cB, T e) {
+        const T e2 = e * e;
+        return fma(sB, fma(e2 * e, T(-1.0 / 6.0), e), cB * fma(e2, T(-0.5), T(1)));
+    }
+    // float: a fresh branch-free sincos of tB every step (no rotation carried
+    // from step to step, so no refresh branch), the stage terms as in
+    // time_terms; false when |tB| > kFastTrigMax
+    __device__ __forceinline__ static bool time_terms_fast(float t0, const float (&ts)[4], float s0,
+                                                           float c0, float tB, float t1, float &sB,
+                                                           float &cB, float (&aux)[5]) {
+        sincos_fast(tB, sB, cB);
+        const float dmax = fabsf(ts[3] - t0);
+        if (dmax <= 1.0f) {
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                float sd, cd;
+                sincos_unit<float>(ts[i] - t0, sd, cd);
+                aux[i] = c0 * cd - s0 * sd;
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < 4; i++) aux[i] = cosf(ts[i]);
+        }
+        // |tB| <= 2^17 bounds e = tB - t1 by 2^-6
+        aux[4] = cos_prev(sB, cB, tB - t1);
+        return fabsf(tB) <= kFastTrigMax;
     }
 };
 
 // right-hand side with the per-step time terms (time-independent systems
-// ignore them)
-template <class Sys, class T>
+// ignore them); kFast (float) takes a system's branch-free form where it
+// has one, which adds its argument magnitude to amax
+template <class Sys, class T, bool kFast>
 __device__ __forceinline__ void rhs_aux(const T (&p)[Sys::NP], T aux, const T (&y)[Sys::S],
-                                        T (&dy)[Sys::S]) {
+                                        T (&dy)[Sys::S], T &amax) {
     if constexpr (Sys::kTime)
         Sys::f_aux(p, aux, y, dy);
+    else if constexpr (kFast && Sys::kFastRhs)
+        Sys::f_fast(p, y, dy, amax);
     else
         Sys::f(p, T(0), y, dy);
 }

@@ -84,6 +84,68 @@ def test_trajectories_match_oracle_short_horizon(cuda_device, name, dtype):
     assert torch.equal(y[0].cpu(), y0)
 
 
+def _f32_bound(sysid, y0, p, ts, ref, t0=0.0):
+    """Per-IC bound of the short-horizon float32 comparison: 2e-4 of the
+    scaled state plus four times how far a one-ulp change of the float32 IC
+    moves the float64 oracle's answer."""
+    scale = 1.0 + np.abs(ref)
+    ref64, _, _ = O.solve(sysid, y0.double().numpy(), p, ts, t0, bits=64)
+    sens = np.zeros(y0.shape[0])
+    for direction in (np.inf, -np.inf):
+        yp = np.nextafter(y0.numpy(), np.float32(direction)).astype(np.float64)
+        rp, _, _ = O.solve(sysid, yp, p, ts, t0, bits=64)
+        sens = np.maximum(sens, (np.abs(rp - ref64) / scale).max(axis=(0, 2)))
+    return 2e-4 + 4 * sens
+
+
+@pytest.mark.parametrize("name", ["pendulum", "duffing"])
+def test_float_lanes_outside_fast_trig_range_are_redone(cuda_device, name):
+    """float32 evaluates the pendulum's sine and Duffing's end-time sincos by
+    branch-free forms valid for |argument| <= 2^17; a lane whose step leaves
+    that range is integrated again with the general forms.
+
+    Pendulum: every other lane starts at an angle near +-1e10 rad, where the
+    branch-free reduction is wrong (its quadrant overflows) and float32 holds
+    the angle fixed (one ulp is 1024 rad), so omega relaxes to
+    (T - K sin theta0) / alpha with the true sine: those lanes match the
+    float32 oracle to 1e-5, and the in-range lanes of the launch give the
+    same bits as a launch without them.  Duffing: times from 1.4e5, every
+    lane redone, against the float32 oracle; there float32 stage times are
+    multiples of 2^-6, and a rounding-level difference in a step size moves a
+    stage time by one of them (a forcing change of A 2^-6 = 3e-3), so the
+    bound is 5e-2 of the scaled state, which still separates a wrong
+    forcing phase (errors of order one)."""
+    ode, sysid, lo, hi, _ = CASES[name]
+    print(f"[case] fast trig range {name}", flush=True)
+    p = _oracle_params(ode)
+    y0 = torch.tensor(O.sample_uniform(64, lo, hi, seed=11), dtype=torch.float32)
+    t0 = 0.0
+    far = torch.arange(64) % 2 == 1
+    if name == "pendulum":
+        k = torch.arange(32, dtype=torch.float32)
+        y0[far, 0] = torch.where(k % 2 == 0, 1.0, -1.0) * (1.0e10 + 1024.0 * k)
+    else:
+        t0 = 140_000.0
+    solver = HipDopri5Solver(t_span=(t0, t0 + 10.0), t_steps=101, device="cuda", cache_dir=None)
+    t, y = solver.integrate(ode, y0.to(cuda_device))
+    ts = t.double().cpu().numpy()
+    ref, st, _ = O.solve(sysid, y0.double().numpy(), p, ts, t0, bits=32)
+    assert (st == 0).all()
+    got = y.double().cpu().numpy()
+    err_ic = (np.abs(got - ref) / (1.0 + np.abs(ref))).max(axis=(0, 2))
+    if name == "pendulum":
+        fm = far.numpy()
+        assert (got[:, fm, 0] == y0.numpy()[fm, 0]).all()  # the angle stays put in float32
+        bound = np.where(fm, 1e-5, _f32_bound(sysid, y0, p, ts, ref, t0))
+    else:
+        bound = np.full(len(err_ic), 5e-2)
+    print(f"[result] {name} worst ratio {float((err_ic / bound).max()):.3f}", flush=True)
+    assert (err_ic <= bound).all()
+    if name == "pendulum":
+        _, y_near = solver.integrate(ode, y0[~far].contiguous().to(cuda_device))
+        assert torch.equal(y_near.cpu(), y[:, ~far.to(y.device)].cpu())
+
+
 def test_pendulum_golden_ics_float64(cuda_device):
     with open(os.path.join(GOLD, "zigode_pendulum_golden.json")) as f:
         gold = json.load(f)

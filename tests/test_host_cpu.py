@@ -425,28 +425,37 @@ def test_dynamical_system_feature_set_on_cpu_equals_oracle():
     assert fx.needs_window and fx.needs_trend
 
 
-def test_pendulum_float_sine_is_within_bound():
-    """The pendulum functor's float sine (Cody-Waite reduction by pi/2 in
-    three FMA steps, polynomials on |r| <= 0.8; constants read from
-    systems.hpp), evaluated in float32 in the kernel's order with exact FMAs,
-    stays within 7.5e-8 of sin x for |x| <= 2^17, near multiples of pi/2
-    included; a dropped reduction step or polynomial term breaks the bound."""
+def test_float_sine_cosine_are_within_bound():
+    """The branch-free float sine and cosine of systems.hpp (Cody-Waite
+    reduction by pi/2 in three FMA steps, polynomials on |r| <= 0.8,
+    quadrant selection; constants read from the source), evaluated in
+    float32 in the kernel's order with exact FMAs, stay within 7.5e-8 of
+    sin x and cos x for |x| <= 2^17, near multiples of pi/2 included; a
+    dropped reduction step or polynomial term, or a quadrant sign taken from
+    the wrong bit, breaks the bound."""
     import os
     import re
 
     src = open(os.path.join(os.path.dirname(__file__), "..", "pybasin_amd", "csrc",
                             "systems.hpp")).read()
-    body = src[src.index("float sin_cw(float x)"):]
-    body = body[:body.index("\n}")]
-    lit = [np.float32(v) for v in re.findall(r"(-?\d\.\d+e[-+]\d+)f", body)]
+
+    def body(sig):
+        b = src[src.index(sig):]
+        return b[:b.index("\n}")]
+
+    lit = []
+    for sig in ("void cw_reduce(", "float cw_sin_poly(", "float cw_cos_poly("):
+        lit += [np.float32(v) for v in re.findall(r"(-?\d\.\d+e[-+]\d+)f", body(sig))]
     assert len(lit) == 11, lit
     two_pi_inv, c1, c2, c3, s2, s1, s0, k3, k2, k1, k0 = lit
+    assert re.search(r"a = \(q & 1\) \? cv : sv, b = \(q & 1\) \? sv : cv", body("void sincos_fast("))
+    assert "c = ((q + 1) & 2) ? -b : b" in body("void sincos_fast(")
     f = np.float32
 
     def fma(a, b, c):
         return (np.float64(a) * np.float64(b) + np.float64(c)).astype(f)
 
-    def sin_cw(x, drop=None):
+    def sincos(x, drop=None, cos_bit=1):
         k = np.rint((x * two_pi_inv).astype(f)).astype(f)
         r = fma(-k, c1, x)
         r = fma(-k, c2, r) if drop != "c2" else r
@@ -456,15 +465,18 @@ def test_pendulum_float_sine_is_within_bound():
         sv = fma((r2 * r).astype(f), ps, r)
         cv = fma(r2, fma(r2, fma(r2, fma(r2, k3, k2), k1), k0), f(1))
         q = k.astype(np.int64)
-        v = np.where(q & 1, cv, sv)
-        return np.where(q & 2, -v, v)
+        a, b = np.where(q & 1, cv, sv), np.where(q & 1, sv, cv)
+        return np.where(q & 2, -a, a), np.where((q + cos_bit) & 2, -b, b)
 
     assert k0 == f(-0.5)
     rng = np.random.default_rng(3)
     x = rng.uniform(-131072, 131072, 400_000).astype(f)
     m = (rng.integers(-80_000, 80_000, 100_000) * np.pi / 2).astype(f)
     x = np.concatenate([x, m, np.nextafter(m, f(np.inf)), rng.uniform(-4, 4, 100_000).astype(f)])
-    t = np.sin(x.astype(np.float64))
-    assert np.abs(sin_cw(x) - t).max() <= 7.5e-8
+    ts, tc = np.sin(x.astype(np.float64)), np.cos(x.astype(np.float64))
+    s, c = sincos(x)
+    assert np.abs(s - ts).max() <= 7.5e-8
+    assert np.abs(c - tc).max() <= 7.5e-8
     for d in ("c2", "s2"):
-        assert np.abs(sin_cw(x, drop=d) - t).max() > 1e-6, d
+        assert np.abs(sincos(x, drop=d)[0] - ts).max() > 1e-6, d
+    assert np.abs(sincos(x, cos_bit=0)[1] - tc).max() > 0.5
