@@ -98,6 +98,8 @@ def parse():
     ap.add_argument("--inflight", type=int, default=2,
                     help="surrogate: passes in flight (each on its own stream and host thread)")
     ap.add_argument("--epochs", type=int, default=600)
+    ap.add_argument("--rossler-coupling", choices=["edges", "dense"], default="edges",
+                    help="rossler workload: the edge-list wavefront kernel or the dense MFMA tile kernel")
     ap.add_argument("--evolve-t", type=float, default=50.0,
                     help="surrogate: below-gate ICs are integrated to this time and classified "
                          "again before a full integration (0 disables)")
@@ -523,6 +525,7 @@ def rossler_workload(args):
     ode = RosslerNetworkODE({"a": 0.2, "b": 0.2, "c": 7.0, "K": 0.119})
     solver = HipDopri5Solver(t_span=(0, 1000), t_steps=2, t_eval=(999.0, 1000.0), device="cuda",
                              rtol=1e-3, atol=1e-6, cache_dir=None, stop_threshold=400.0)
+    solver.rossler_coupling = args.rossler_coupling
     stream = torch.cuda.current_stream()
     kev = []
 
@@ -545,12 +548,39 @@ def rossler_workload(args):
     torch.cuda.synchronize()
     ms = 1e3 * (time.perf_counter() - t0) / args.steps
     kernel_ms = sum(a.elapsed_time(b) for a, b in kev) / len(kev)
+    # the other coupling form once, outside the timed steps, for comparison
+    other = "dense" if args.rossler_coupling == "edges" else "edges"
+    solver.rossler_coupling = other
+    solver.integrate_fused(ode, y0, params)  # warm-up
+    o0, o1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    o0.record(stream)
+    res_o = solver.integrate_fused(ode, y0, params)
+    o1.record(stream)
+    torch.cuda.synchronize()
+    other_ms = o0.elapsed_time(o1)
+    lab_o = sync_labels(res_o.final, res_o.status)
     n = B * len(Ks)
     nsteps = res.nsteps.double()
-    # FLOP per attempted step: 6 evaluations x (coupling product 2 N^2 on the
-    # matrix cores + 8 N local) + 63 S RK arithmetic + 16
-    mfma_flop = float(nsteps.sum()) * 6 * 2 * 100 * 100
-    valu_flop = float(nsteps.sum()) * (6 * 8 * 100 + 63 * 300 + 16)
+    # FLOP per attempted step: 6 evaluations x (coupling + 8 N local) + 63 S
+    # RK arithmetic + 16, the coupling being the reference's edge form (2 FLOP
+    # per edge entry: x_i - x_j and its scatter-add; 800 entries) for the
+    # wavefront kernel and the 2 N^2 product for the dense matrix-core kernel
+    steps_sum = float(nsteps.sum())
+    rk_flop = steps_sum * (6 * 8 * 100 + 63 * 300 + 16)
+    if args.rossler_coupling == "edges":
+        flop = rk_flop + steps_sum * 6 * 2 * 800
+        roof = {"bound": "valu", "achieved": round(flop / (kernel_ms * 1e-3) / 1e12, 3),
+                "peak": 157.3, "unit": "TFLOP/s", "frac": round(flop / (kernel_ms * 1e-3) / 157.3e12, 4),
+                "traffic": None, "kernel": "bh::ross::wave_kernel (edge-list coupling)",
+                "kernel_ms": round(kernel_ms, 3)}
+    else:
+        mfma_flop = steps_sum * 6 * 2 * 100 * 100
+        roof = {"bound": "mfma", "achieved": round(mfma_flop / (kernel_ms * 1e-3) / 1e12, 3),
+                "peak": 157.3, "unit": "TFLOP/s",
+                "frac": round(mfma_flop / (kernel_ms * 1e-3) / 157.3e12, 4), "traffic": None,
+                "valu_tflops": round(rk_flop / (kernel_ms * 1e-3) / 1e12, 3),
+                "kernel": "bh::ross::tile_kernel (dense matrix-core coupling)",
+                "kernel_ms": round(kernel_ms, 3)}
     per_k = {}
     for i, k in enumerate(Ks):
         li = lab[i::len(Ks)]
@@ -561,15 +591,14 @@ def rossler_workload(args):
         "warmup": args.warmup, "ms_per_step": round(ms, 3), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic: seeded-uniform ICs, 11 coupling strengths in one launch",
-        "config": {"workload": "rossler100_k_sweep", "n_trajectories": n, "ics_per_k": B,
+        "config": {"workload": "rossler100_k_sweep", "coupling": args.rossler_coupling,
+                   "n_trajectories": n, "ics_per_k": B,
                    "rtol": 1e-3, "atol": 1e-6, "t_span": [0, 1000],
                    "mean_steps": round(float(nsteps.mean()), 1), "max_steps": int(nsteps.max()),
-                   "basin_stability": per_k},
-        "roofline": {"bound": "mfma", "achieved": round(mfma_flop / (kernel_ms * 1e-3) / 1e12, 3),
-                     "peak": 157.3, "unit": "TFLOP/s",
-                     "frac": round(mfma_flop / (kernel_ms * 1e-3) / 157.3e12, 4), "traffic": None,
-                     "valu_tflops": round(valu_flop / (kernel_ms * 1e-3) / 1e12, 3),
-                     "kernel": "bh::ross::tile_kernel", "kernel_ms": round(kernel_ms, 3)},
+                   "basin_stability": per_k,
+                   "other_coupling": {"coupling": other, "kernel_ms": round(other_ms, 3),
+                                      "label_agreement": round(float(np.mean(lab == lab_o)), 4)}},
+        "roofline": roof,
     }
     if base is not None:
         out["cpu_baseline"] = base

@@ -29,7 +29,7 @@ BH_NOALLOC_SCRATCH_BYTES = 64 << 20
 # every symbol include/basinhip.h declares
 EXPORTS = ("bh_version", "bh_last_error", "bh_solve_batch", "bh_solve_batch_peaks",
            "bh_solve_batch_phased_workspace_bytes", "bh_solve_batch_phased",
-           "bh_solve_batch_phased_noalloc", "bh_solve_batch_phased_noalloc_max_n", "bh_autotune_lane", "bh_solve_rossler", "bh_gemm_bf16",
+           "bh_solve_batch_phased_noalloc", "bh_solve_batch_phased_noalloc_max_n", "bh_autotune_lane", "bh_solve_rossler", "bh_solve_rossler_edges", "bh_gemm_bf16",
            "bh_mlp_bias_act", "bh_mlp_silu_bwd", "bh_mlp_softmax_ce", "bh_mlp_colsum",
            "bh_mlp_adam", "bh_mlp_prep_input", "bh_mlp_classify", "bh_mlp_infer",
            "bh_mlp_train_workspace_bytes", "bh_mlp_train_step", "bh_knn",
@@ -77,6 +77,9 @@ def _declare(L: ctypes.CDLL) -> None:
     L.bh_solve_rossler.restype = i32
     L.bh_solve_rossler.argtypes = [i32, vp, i64, vp, i64, vp, f64, f64, f64, f64, i64, u32,
                                    f64, vp, vp, vp, vp, i64, i32, vp]
+    L.bh_solve_rossler_edges.restype = i32
+    L.bh_solve_rossler_edges.argtypes = [i32, vp, i64, vp, i64, vp, vp, i64, f64, f64, f64, f64, i64,
+                                         u32, f64, vp, vp, vp, vp, i64, i32, vp]
     L.bh_hdbscan_workspace_bytes.restype = i64
     L.bh_hdbscan_workspace_bytes.argtypes = [i64]
     L.bh_mlp_train_workspace_bytes.restype = i64

@@ -1,7 +1,10 @@
 // basinhip_api.hip -- C-ABI entry points of libbasinhip.so (see include/basinhip.h);
 // the per-lane solver's entry points are in lane_api.hip.
 #include "api_common.hpp"
+#include <algorithm>
+
 #include "rossler_tile.hip"
+#include "rossler_wave.hip"
 #include "mlp.hip"
 #include "cluster.hip"
 #include "window.hip"
@@ -124,19 +127,24 @@ int32_t bh_version(void) { return 10000; }
 
 const char *bh_last_error(void) { return g_err; }
 
-int32_t bh_solve_rossler(int32_t dtype, const void *y0s, int64_t n, const double *params,
-                         int64_t params_stride, const float *laplacian, double t0, double t_end,
-                         double rtol, double atol, int64_t max_steps, uint32_t flags,
-                         double stop_threshold, void *out_final, int32_t *out_status,
-                         int64_t *out_nsteps, void *workspace, int64_t workspace_bytes,
-                         int32_t max_tiles, void *stream) {
-    g_err[0] = 0;
-    namespace R = bh::ross;
+}  // extern "C"
+
+namespace {
+
+// Argument checks and launch shared by the dense (tile kernel) and the
+// edge-list (wavefront kernel) entry points; slots = trajectories per
+// workgroup.
+template <class Kernel>
+int32_t rossler_launch(Kernel kernel, int threads, int slots, bh::ross::Args &a, int32_t dtype,
+                       const void *y0s, int64_t n, const double *params, int64_t params_stride,
+                       double t0, double t_end, double rtol, double atol, int64_t max_steps,
+                       uint32_t flags, double stop_threshold, void *out_final, int32_t *out_status,
+                       int64_t *out_nsteps, void *workspace, int64_t workspace_bytes,
+                       int32_t max_tiles, void *stream) {
     if (dtype != BH_F32) return fail(-5, "the Rossler network kernel runs in float32 only");
     if (n < 0) return fail(-5, "n must be >= 0");
     if (n == 0) return 0;
-    if (!y0s || !params || !laplacian || !out_final)
-        return fail(-5, "y0s, params, laplacian and out_final are required");
+    if (!y0s || !params || !out_final) return fail(-5, "y0s, params and out_final are required");
     if (params_stride != 0 && params_stride < 4) return fail(-5, "params_stride must be 0 or >= 4");
     if (!(rtol >= 0.0) || !(atol >= 0.0) || rtol + atol <= 0.0)
         return fail(-5, "rtol/atol must be >= 0 with rtol + atol > 0");
@@ -147,7 +155,6 @@ int32_t bh_solve_rossler(int32_t dtype, const void *y0s, int64_t n, const double
     if ((flags & BH_FLAG_STOP) && !(stop_threshold > 0.0))
         return fail(-5, "BH_FLAG_STOP needs stop_threshold > 0");
     if (max_tiles < 0) return fail(-5, "max_tiles must be >= 0");
-    R::Args a;
     a.y0 = reinterpret_cast<const float *>(y0s);
     a.params = params;
     a.p_stride = params_stride;
@@ -159,26 +166,114 @@ int32_t bh_solve_rossler(int32_t dtype, const void *y0s, int64_t n, const double
     a.max_steps = max_steps;
     a.flags = flags;
     a.stop_thr = stop_threshold;
-    a.lap = laplacian;
     a.fin = reinterpret_cast<float *>(out_final);
     a.status = out_status;
     a.nsteps = out_nsteps;
     a.counter = reinterpret_cast<unsigned long long *>(workspace);
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     int64_t lanes = 0;
-    int32_t rc = device_lanes(R::tile_kernel, R::THREADS, &lanes);
+    int32_t rc = device_lanes(kernel, threads, &lanes);
     if (rc) return rc;
-    int64_t tiles = lanes / R::THREADS;
-    if (max_tiles > 0 && max_tiles < tiles) tiles = max_tiles;
-    const int64_t need = (n + R::TILE - 1) / R::TILE;
-    if (tiles > need) tiles = need;
-    if (tiles < 1) tiles = 1;
+    // max_tiles caps the resident trajectories at 16 * max_tiles
+    int64_t blocks = lanes / threads;
+    if (max_tiles > 0 && (int64_t)max_tiles * 16 / slots < blocks)
+        blocks = std::max<int64_t>(1, (int64_t)max_tiles * 16 / slots);
+    const int64_t need = (n + slots - 1) / slots;
+    if (blocks > need) blocks = need;
+    if (blocks < 1) blocks = 1;
     hipLaunchKernelGGL(bh::ross::set_counter_kernel, dim3(1), dim3(64), 0, s, a.counter,
-                       (unsigned long long)(tiles * R::TILE));
+                       (unsigned long long)(blocks * slots));
     BH_HIP_CHECK(hipGetLastError());
-    hipLaunchKernelGGL(R::tile_kernel, dim3((unsigned)tiles), dim3(R::THREADS), 0, s, a);
+    hipLaunchKernelGGL(kernel, dim3((unsigned)blocks), dim3(threads), 0, s, a);
     BH_HIP_CHECK(hipGetLastError());
     return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t bh_solve_rossler(int32_t dtype, const void *y0s, int64_t n, const double *params,
+                         int64_t params_stride, const float *laplacian, double t0, double t_end,
+                         double rtol, double atol, int64_t max_steps, uint32_t flags,
+                         double stop_threshold, void *out_final, int32_t *out_status,
+                         int64_t *out_nsteps, void *workspace, int64_t workspace_bytes,
+                         int32_t max_tiles, void *stream) {
+    g_err[0] = 0;
+    if (n > 0 && !laplacian) return fail(-5, "laplacian is required");
+    bh::ross::Args a{};
+    a.lap = laplacian;
+    return rossler_launch(bh::ross::tile_kernel, bh::ross::THREADS, bh::ross::TILE, a, dtype,
+                          y0s, n, params, params_stride, t0, t_end, rtol, atol, max_steps, flags,
+                          stop_threshold, out_final, out_status, out_nsteps, workspace,
+                          workspace_bytes, max_tiles, stream);
+}
+
+int32_t bh_solve_rossler_edges(int32_t dtype, const void *y0s, int64_t n, const double *params,
+                               int64_t params_stride, const int32_t *edges_i,
+                               const int32_t *edges_j, int64_t n_edges, double t0, double t_end,
+                               double rtol, double atol, int64_t max_steps, uint32_t flags,
+                               double stop_threshold, void *out_final, int32_t *out_status,
+                               int64_t *out_nsteps, void *workspace, int64_t workspace_bytes,
+                               int32_t max_tiles, void *stream) {
+    g_err[0] = 0;
+    namespace R = bh::ross;
+    if (n_edges < 0) return fail(-5, "n_edges must be >= 0");
+    if (n_edges > 0 && (!edges_i || !edges_j)) return fail(-5, "edges_i and edges_j are required");
+    // host-side graph table: degrees, neighbour lists, positions by degree
+    int deg[R::NN] = {0};
+    uint8_t nbr[R::NN][R::DMAX];
+    for (int64_t e = 0; e < n_edges; e++) {
+        const int32_t i = edges_i[e], j = edges_j[e];
+        if (i < 0 || i >= R::NN || j < 0 || j >= R::NN)
+            return fail(-5, "edge %lld (%d, %d) outside the %d nodes", (long long)e, (int)i, (int)j,
+                        R::NN);
+        if (deg[i] == R::DMAX)
+            return fail(-5, "node %d has more than %d edge entries (use bh_solve_rossler with the "
+                        "dense Laplacian)", (int)i, R::DMAX);
+        nbr[i][deg[i]++] = (uint8_t)j;
+    }
+    bh::ross::Args a{};
+    // positions in order of decreasing degree, ties by node index
+    int order[R::NN];
+    for (int i = 0; i < R::NN; i++) order[i] = i;
+    std::stable_sort(order, order + R::NN, [&](int u, int v) { return deg[u] > deg[v]; });
+    for (int p = 0; p < R::NPOS; p++) {
+        const int node = p < R::NN ? order[p] : 0;
+        a.g.node[p] = (uint8_t)(p < R::NN ? node : R::NN);
+        a.g.deg[p] = (uint8_t)(p < R::NN ? deg[node] : 0);
+        for (int d = 0; d < R::DMAX; d++) a.g.nbr[p][d] = (uint8_t)R::NN;
+    }
+    // Neighbour slot of each edge entry: one gather instruction reads slot d
+    // of the 32 positions r * 32 .. r * 32 + 31, so for each such group the
+    // entries are coloured greedily (positions of high degree first, then
+    // neighbours in order) with the lowest slot whose gather has not yet
+    // used the neighbour's bank (its id modulo 32); an entry that finds none
+    // takes the lowest free slot.  Any order gives the same sums; this one
+    // spreads the gathers over the LDS banks.
+    for (int r = 0; r < R::NPT; r++) {
+        uint32_t used[R::DMAX] = {0};  // banks taken per slot
+        for (int p = 32 * r; p < 32 * r + 32 && p < R::NN; p++) {
+            const int node = order[p];
+            uint32_t taken = 0;  // slots of this position
+            for (int e = 0; e < deg[node]; e++) {
+                const int j = nbr[node][e], bank = j & 31;
+                int slot = -1;
+                for (int d = 0; d < R::DMAX && slot < 0; d++)
+                    if (!(taken >> d & 1u) && !(used[d] >> bank & 1u)) slot = d;
+                for (int d = 0; d < R::DMAX && slot < 0; d++)
+                    if (!(taken >> d & 1u)) slot = d;
+                taken |= 1u << slot;
+                used[slot] |= 1u << bank;
+                a.g.nbr[p][slot] = (uint8_t)j;
+            }
+        }
+    }
+
+    return rossler_launch(bh::ross::wave_kernel, bh::ross::WTHREADS, bh::ross::WSLOTS, a, dtype, y0s,
+                          n, params, params_stride, t0, t_end, rtol, atol, max_steps, flags,
+                          stop_threshold, out_final, out_status, out_nsteps, workspace,
+                          workspace_bytes, max_tiles, stream);
 }
 
 // ---- surrogate MLP --------------------------------------------------------

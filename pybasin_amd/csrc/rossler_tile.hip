@@ -27,8 +27,12 @@
 // so bounded and quickly-diverging trajectories share tiles without idling.
 //
 // Thread layout: tid = m + 16 q, m = slot (0..15), q = node group (0..31);
-// the thread owns nodes q, q + 32, q + 64 and (q < 4) q + 96, i.e. x, y, z of
-// up to four nodes.
+// the thread owns node positions q, q + 32, q + 64 and q + 96 (real below
+// 100), i.e. x, y, z of up to four nodes.
+//
+// The edge-list form of the coupling (bh_solve_rossler_edges) runs in the
+// wavefront kernel of rossler_wave.hip; the graph table and the argument
+// block are shared.
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <stdint.h>
@@ -49,7 +53,18 @@ constexpr int NB = 7;         // 16-column blocks covering 100 nodes
 constexpr int KS = 22;        // k-blocks of each block done by its own wave; the
                               // eighth wave does the last KB - KS of all seven
 
+constexpr int DMAX = 12;      // neighbour slots per node position (edge-list kernel)
+constexpr int NPOS = 128;     // node positions: 32 node groups x NPT
+
 enum Mode : int { IDLE = 0, INIT = 1, STEP = 2 };
+
+// Edge-list graph of the coupling (wavefront kernel), built on the host by
+// bh_solve_rossler_edges and passed by value with the launch arguments.
+struct Graph {
+    uint8_t node[NPOS];       // position -> node (positions >= NN unused)
+    uint8_t deg[NPOS];        // edge entries of the position's node
+    uint8_t nbr[NPOS][DMAX];  // neighbour nodes; unused slots hold NN (the zero cell)
+};
 
 struct Args {
     const float *y0;
@@ -61,7 +76,8 @@ struct Args {
     int64_t max_steps;
     uint32_t flags;
     double stop_thr;
-    const float *lap;  // (100, 100) row-major
+    const float *lap;  // (100, 100) row-major (dense kernel)
+    Graph g;           // edge-list (wavefront) kernel
     float *fin;        // (n, 300)
     int32_t *status;
     int64_t *nsteps;
@@ -151,7 +167,15 @@ __global__ void __launch_bounds__(THREADS) tile_kernel(Args a) {
 
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int m = tid & 15, q = tid >> 4;
-    const bool has4 = q < NN - 3 * 32;  // fourth node slot (q + 96) is real
+    // node of each position slot (position q + 32 r is node q + 32 r, real
+    // below 100)
+    bool real[NPT];
+    int node[NPT];
+#pragma unroll
+    for (int r = 0; r < NPT; r++) {
+        node[r] = q + 32 * r;
+        real[r] = node[r] < NN;
+    }
 
     // B operands: L[4 kb .. 4 kb + 3][16-column block] for this wave's share
     // (columns past 99 are zero)
@@ -172,7 +196,6 @@ __global__ void __launch_bounds__(THREADS) tile_kernel(Args a) {
                 Bop[(KB - KS) * b + t] = (j < NN) ? a.lap[k * NN + j] : 0.0f;
             }
     }
-
     // per-thread state for its node slots: x, y, z of nodes q + 32 r
     float y[3][NPT], k1[3][NPT], k2[3][NPT], k3[3][NPT], k4[3][NPT], k5[3][NPT], k6[3][NPT],
         k7[3][NPT], ys[3][NPT];
@@ -199,10 +222,8 @@ __global__ void __launch_bounds__(THREADS) tile_kernel(Args a) {
         const float *src = a.y0 + idx * SD;
 #pragma unroll
         for (int r = 0; r < NPT; r++) {
-            const int node = q + 32 * r;
-            const bool real = r < 3 || has4;
 #pragma unroll
-            for (int c = 0; c < 3; c++) y[c][r] = real ? src[c * NN + node] : 0.0f;
+            for (int c = 0; c < 3; c++) y[c][r] = real[r] ? src[c * NN + node[r]] : 0.0f;
         }
         // the FSAL slot is multiplied by a zero step in the initial rounds
 #pragma unroll
@@ -219,9 +240,9 @@ __global__ void __launch_bounds__(THREADS) tile_kernel(Args a) {
         float *dst = a.fin + idx * SD;
 #pragma unroll
         for (int r = 0; r < NPT; r++) {
-            if (r < 3 || has4) {
+            if (real[r]) {
 #pragma unroll
-                for (int c = 0; c < 3; c++) dst[c * NN + q + 32 * r] = v[c][r];
+                for (int c = 0; c < 3; c++) dst[c * NN + node[r]] = v[c][r];
             }
         }
     };
@@ -230,7 +251,7 @@ __global__ void __launch_bounds__(THREADS) tile_kernel(Args a) {
     auto rhs = [&](float (&out)[3][NPT]) {
 #pragma unroll
         for (int r = 0; r < NPT; r++)
-            if (r < 3 || has4) Xs[m * XS + q + 32 * r] = ys[0][r];
+            if (real[r]) Xs[m * XS + q + 32 * r] = ys[0][r];
         __syncthreads();
         // local terms (no coupling) next to the coupling product
         float part[NPT];
@@ -271,7 +292,7 @@ __global__ void __launch_bounds__(THREADS) tile_kernel(Args a) {
 #pragma unroll
         for (int r = 0; r < NPT; r++) {
             const int e = m * XS + q + 32 * r;
-            const float cpl = (r < 3 || has4) ? Cs[e] + Cs2[e] : 0.0f;
+            const float cpl = real[r] ? Cs[e] + Cs2[e] : 0.0f;
             out[0][r] = part[r] - pK * cpl;
         }
     };
@@ -301,7 +322,7 @@ __global__ void __launch_bounds__(THREADS) tile_kernel(Args a) {
             for (int c = 0; c < 3; c++)
 #pragma unroll
                 for (int r = 0; r < NPT; r++) {
-                    if (r < 3 || has4) {
+                    if (real[r]) {
                         const float sc = ratol + fabsf(y[c][r]) * rrtol;
                         const float r0 = y[c][r] / sc, r1 = k2[c][r] / sc;
                         s0 += r0 * r0;
@@ -336,7 +357,7 @@ __global__ void __launch_bounds__(THREADS) tile_kernel(Args a) {
             for (int c = 0; c < 3; c++)
 #pragma unroll
                 for (int r = 0; r < NPT; r++) {
-                    if (r < 3 || has4) {
+                    if (real[r]) {
                         const float sc = ratol + fabsf(y[c][r]) * rrtol;
                         const float v = (k3[c][r] - k2[c][r]) / sc;
                         s2 += v * v;
@@ -404,7 +425,7 @@ __global__ void __launch_bounds__(THREADS) tile_kernel(Args a) {
             for (int c = 0; c < 3; c++)
 #pragma unroll
                 for (int r = 0; r < NPT; r++) {
-                    if (r < 3 || has4) {
+                    if (real[r]) {
                         const float err = k1[c][r] * e1 + k3[c][r] * e3 + k4[c][r] * e4 +
                                           k5[c][r] * e5 + k6[c][r] * e6 + k7[c][r] * e7;
                         const float tol = ratol + rrtol * fmaxf(fabsf(y[c][r]), fabsf(ys[c][r]));

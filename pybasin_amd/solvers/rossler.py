@@ -11,12 +11,17 @@ from __future__ import annotations
 import torch
 
 from .. import _lib
-from ..systems.rossler_network import N_NODES, rossler_laplacian
+from ..systems.rossler_network import N_NODES, rossler_edge_arrays, rossler_laplacian
 
 
 def run_rossler(solver, ode_system, y0: torch.Tensor, save_ts: torch.Tensor,
                 params: torch.Tensor | None, *, keep_trajectories: bool,
-                stream: torch.cuda.Stream | None = None, max_tiles: int = 0):
+                stream: torch.cuda.Stream | None = None, max_tiles: int = 0,
+                coupling: str | None = None):
+    """coupling: "edges" (the reference's edge-list form on the vector units,
+    bh_solve_rossler_edges; the default) or "dense" (the Laplacian product on
+    the matrix cores, bh_solve_rossler); the solver's rossler_coupling
+    attribute sets the default."""
     from .hip_dopri5 import FusedResult, device_params
 
     if keep_trajectories:
@@ -33,7 +38,9 @@ def run_rossler(solver, ode_system, y0: torch.Tensor, save_ts: torch.Tensor,
     p = p.to(dev).contiguous()
     if p_stride and p.shape[0] != n:
         raise ValueError(f"params rows {p.shape[0]} != trajectories {n}")
-    lap = rossler_laplacian(torch.float32, dev).contiguous()
+    coupling = coupling or getattr(solver, "rossler_coupling", "edges")
+    if coupling not in ("edges", "dense"):
+        raise ValueError(f"coupling must be 'edges' or 'dense', got {coupling!r}")
     final = torch.empty((n, S), dtype=torch.float32, device=dev)
     status = torch.empty((n,), dtype=torch.int32, device=dev)
     nsteps = torch.empty((n,), dtype=torch.int64, device=dev)
@@ -46,14 +53,25 @@ def run_rossler(solver, ode_system, y0: torch.Tensor, save_ts: torch.Tensor,
             flags |= _lib.BH_FLAG_INF_FILL
     t_end = float(save_ts[-1])
     s = stream if stream is not None else torch.cuda.current_stream(dev)
+    tiles = int(max_tiles or getattr(solver, "max_lanes", 0) // 16)
+    common = (float(solver.t_span[0]), t_end, float(solver.rtol), float(solver.atol),
+              solver.max_steps, flags, thr, final.data_ptr(), status.data_ptr(), nsteps.data_ptr(),
+              ws.data_ptr(), ws.numel(), tiles, s.cuda_stream)
     with torch.cuda.device(dev):
-        rc = _lib.lib().bh_solve_rossler(
-            _lib.BH_F32, y0.data_ptr(), n, p.data_ptr(), p_stride, lap.data_ptr(),
-            float(solver.t_span[0]), t_end, float(solver.rtol), float(solver.atol),
-            solver.max_steps, flags, thr, final.data_ptr(), status.data_ptr(), nsteps.data_ptr(),
-            ws.data_ptr(), ws.numel(), int(max_tiles or getattr(solver, "max_lanes", 0) // 16),
-            s.cuda_stream)
-    _lib.check(rc, "bh_solve_rossler")
+        if coupling == "edges":
+            ei, ej = rossler_edge_arrays()  # host arrays, read by the call
+            rc = _lib.lib().bh_solve_rossler_edges(
+                _lib.BH_F32, y0.data_ptr(), n, p.data_ptr(), p_stride, ei.ctypes.data,
+                ej.ctypes.data, len(ei), *common)
+        else:
+            lap = rossler_laplacian(torch.float32, dev).contiguous()
+            rc = _lib.lib().bh_solve_rossler(
+                _lib.BH_F32, y0.data_ptr(), n, p.data_ptr(), p_stride, lap.data_ptr(), *common)
+    _lib.check(rc, "bh_solve_rossler_edges" if coupling == "edges" else "bh_solve_rossler")
+    if stream is not None:  # the launch reads them on its own stream
+        ws.record_stream(s)
+        if coupling == "dense":
+            lap.record_stream(s)
     return FusedResult(t=save_ts[-1:], y=None, final=final, stats=None, status=status,
                        nsteps=nsteps, steady_idx=len(save_ts) - 1)
 

@@ -41,6 +41,18 @@ def rossler_edges() -> np.ndarray:
 
 
 @lru_cache(maxsize=1)
+def rossler_edge_arrays() -> tuple[np.ndarray, np.ndarray]:
+    """(edges_i, edges_j), int32, both directions of every edge in file order:
+    the arrays of the reference's build_edge_arrays
+    (rossler_network_topology.py:26-48), (L x)_i = sum over e with
+    edges_i[e] = i of (x_i - x_edges_j[e])."""
+    e = rossler_edges()
+    ei = np.ascontiguousarray(np.stack([e[:, 0], e[:, 1]], 1).reshape(-1), dtype=np.int32)
+    ej = np.ascontiguousarray(np.stack([e[:, 1], e[:, 0]], 1).reshape(-1), dtype=np.int32)
+    return ei, ej
+
+
+@lru_cache(maxsize=1)
 def _laplacian_np() -> np.ndarray:
     adj = np.zeros((N_NODES, N_NODES), dtype=np.float32)
     for i, j in rossler_edges():

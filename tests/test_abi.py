@@ -85,3 +85,32 @@ def test_noalloc_entry_checks_size_before_device_work():
     assert L.bh_solve_batch_phased_noalloc(*args(10, 0)) == -5  # phase_iters >= 1
     assert L.bh_solve_batch_phased_noalloc(*args(0, 100)) == 0  # nothing to do
     assert L.bh_solve_batch_phased_noalloc_max_n(99, dt, 2) < 0
+
+
+def test_rossler_edges_entry_checks_the_graph_before_device_work():
+    """bh_solve_rossler_edges validates its host edge arrays (node range, at
+    most 12 entries per node) before anything reaches a device; n = 0 with a
+    valid graph does nothing."""
+    import numpy as np
+
+    L = _lib.lib()
+
+    def call(ei, ej, n=0):
+        ei = np.ascontiguousarray(ei, dtype=np.int32)
+        ej = np.ascontiguousarray(ej, dtype=np.int32)
+        return L.bh_solve_rossler_edges(_lib.BH_F32, None, n, None, 0, ei.ctypes.data, ej.ctypes.data,
+                                        len(ei), 0.0, 1.0, 1e-3, 1e-6, 1000, 0, 0.0, None, None, None,
+                                        None, 0, 0, None)
+
+    ring = np.arange(100)
+    ei = np.concatenate([ring, (ring + 1) % 100])
+    ej = np.concatenate([(ring + 1) % 100, ring])
+    assert call(ei, ej) == 0
+    assert call(np.append(ei, 100), np.append(ej, 0)) == -5
+    assert b"outside" in L.bh_last_error()
+    assert call(np.append(ei, 3), np.append(ej, -1)) == -5
+    hub_i = np.zeros(13, dtype=np.int32)
+    hub_j = np.arange(1, 14, dtype=np.int32)
+    assert call(hub_i, hub_j) == -5
+    assert b"more than 12" in L.bh_last_error()
+    assert call(ei[:0], ej[:0]) == 0  # no edges: uncoupled nodes

@@ -480,3 +480,19 @@ def test_float_sine_cosine_are_within_bound():
     for d in ("c2", "s2"):
         assert np.abs(sincos(x, drop=d)[0] - ts).max() > 1e-6, d
     assert np.abs(sincos(x, cos_bit=0)[1] - tc).max() > 0.5
+
+
+def test_rossler_edge_arrays_give_the_laplacian():
+    """The edge arrays handed to bh_solve_rossler_edges (both directions of
+    every edge, the reference's build_edge_arrays) reproduce L = D - A of the
+    dense kernel: sum over e with edges_i[e] = i of (x_i - x_edges_j[e]) equals
+    (L x)_i for random x, and no node has more than the kernel's 12 entries."""
+    from pybasin_amd.systems.rossler_network import _laplacian_np, rossler_edge_arrays
+
+    ei, ej = rossler_edge_arrays()
+    assert ei.dtype == np.int32 and len(ei) == 800 and ei.flags.c_contiguous
+    x = np.random.default_rng(0).normal(size=(5, 100))
+    cpl = np.zeros_like(x)
+    np.add.at(cpl.T, ei, (x[:, ei] - x[:, ej]).T)
+    np.testing.assert_allclose(cpl, x @ _laplacian_np().T.astype(np.float64), atol=1e-12)
+    assert np.bincount(ei, minlength=100).max() <= 12
