@@ -244,13 +244,24 @@ int32_t bh_solve_rossler_edges(int32_t dtype, const void *y0s, int64_t n, const 
         a.g.deg[p] = (uint8_t)(p < R::NN ? deg[node] : 0);
         for (int d = 0; d < R::DMAX; d++) a.g.nbr[p][d] = (uint8_t)R::NN;
     }
+    // gather bounds: the largest degree among each slot's 32 positions,
+    // covered by the tighter instantiation when it fits
+    int cap[R::NPT];
+    for (int r = 0; r < R::NPT; r++) {
+        cap[r] = 0;
+        for (int p = 32 * r; p < 32 * r + 32 && p < R::NN; p++) cap[r] = std::max(cap[r], deg[order[p]]);
+    }
+    static constexpr int kTight[R::NPT] = {12, 10, 8, 6};
+    bool tight = true;
+    for (int r = 0; r < R::NPT; r++) tight = tight && cap[r] <= kTight[r];
+    for (int r = 0; r < R::NPT; r++) cap[r] = tight ? kTight[r] : R::DMAX;
     // Neighbour slot of each edge entry: one gather instruction reads slot d
     // of the 32 positions r * 32 .. r * 32 + 31, so for each such group the
     // entries are coloured greedily (positions of high degree first, then
-    // neighbours in order) with the lowest slot whose gather has not yet
-    // used the neighbour's bank (its id modulo 32); an entry that finds none
-    // takes the lowest free slot.  Any order gives the same sums; this one
-    // spreads the gathers over the LDS banks.
+    // neighbours in order) with the lowest slot below the group's bound whose
+    // gather has not yet used the neighbour's bank (its id modulo 32); an
+    // entry that finds none takes the lowest free slot.  Any order gives the
+    // same sums; this one spreads the gathers over the LDS banks.
     for (int r = 0; r < R::NPT; r++) {
         uint32_t used[R::DMAX] = {0};  // banks taken per slot
         for (int p = 32 * r; p < 32 * r + 32 && p < R::NN; p++) {
@@ -259,9 +270,9 @@ int32_t bh_solve_rossler_edges(int32_t dtype, const void *y0s, int64_t n, const 
             for (int e = 0; e < deg[node]; e++) {
                 const int j = nbr[node][e], bank = j & 31;
                 int slot = -1;
-                for (int d = 0; d < R::DMAX && slot < 0; d++)
+                for (int d = 0; d < cap[r] && slot < 0; d++)
                     if (!(taken >> d & 1u) && !(used[d] >> bank & 1u)) slot = d;
-                for (int d = 0; d < R::DMAX && slot < 0; d++)
+                for (int d = 0; d < cap[r] && slot < 0; d++)
                     if (!(taken >> d & 1u)) slot = d;
                 taken |= 1u << slot;
                 used[slot] |= 1u << bank;
@@ -269,8 +280,12 @@ int32_t bh_solve_rossler_edges(int32_t dtype, const void *y0s, int64_t n, const 
             }
         }
     }
-
-    return rossler_launch(bh::ross::wave_kernel, bh::ross::WTHREADS, bh::ross::WSLOTS, a, dtype, y0s,
+    if (tight)
+        return rossler_launch(bh::ross::wave_kernel<12, 10, 8, 6>, bh::ross::WTHREADS,
+                              bh::ross::WSLOTS, a, dtype, y0s, n, params, params_stride, t0, t_end,
+                              rtol, atol, max_steps, flags, stop_threshold, out_final, out_status,
+                              out_nsteps, workspace, workspace_bytes, max_tiles, stream);
+    return rossler_launch(bh::ross::wave_kernel<12, 12, 12, 12>, bh::ross::WTHREADS, bh::ross::WSLOTS, a, dtype, y0s,
                           n, params, params_stride, t0, t_end, rtol, atol, max_steps, flags,
                           stop_threshold, out_final, out_status, out_nsteps, workspace,
                           workspace_bytes, max_tiles, stream);

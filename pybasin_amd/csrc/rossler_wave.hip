@@ -41,7 +41,13 @@ __device__ __forceinline__ float half_reduce(float v) {
     return v;
 }
 
+// Neighbour slots gathered per position slot r (compile time, even): the
+// positions g + 32 r of slot r hold nodes of nearly equal degree (sorted),
+// so the host picks the smallest instantiation whose bounds cover every
+// slot's largest degree.
+template <int G0, int G1, int G2, int G3>
 __global__ void __launch_bounds__(WTHREADS) wave_kernel(Args a) {
+    constexpr int GATHER[NPT] = {G0, G1, G2, G3};
     // per slot: the x of its 100 nodes and a zero cell for unused neighbour slots
     __shared__ float X[WSLOTS * WXS];
     const int lane = threadIdx.x;
@@ -131,7 +137,7 @@ __global__ void __launch_bounds__(WTHREADS) wave_kernel(Args a) {
         for (int r = 0; r < NPT; r++) {
             float s0 = 0.0f, s1 = 0.0f;
 #pragma unroll
-            for (int d = 0; d < DMAX; d += 2) {
+            for (int d = 0; d < GATHER[r]; d += 2) {
                 s0 += *reinterpret_cast<const float *>(Xb + off[r][d]);
                 s1 += *reinterpret_cast<const float *>(Xb + off[r][d + 1]);
             }
