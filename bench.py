@@ -11,8 +11,10 @@ once, outside the timed steps, by the project's kernels on 50,000
 integrator-labelled ICs) labels confident ICs; the rest go to the per-lane
 Dopri5 kernel; a 2 % audit per torque is integrated over the full span.  A
 step is one gated pass over all 4,431,800 ICs; `value` is the pass rate with
-two passes in flight (each on its own stream, as a server would overlap the
-latency-bound tail of one pass with the next), and the plain path (full
+four passes in flight (each on its own stream and host thread, taking the
+next pass when it has issued one, as a server would overlap the
+latency-bound tail of one pass with the next; four measured best against
+two, six and eight), and the plain path (full
 integration of every IC, with the same passes in flight and its own tuned
 launch shape) is timed in the same run; the basin-stability error of the
 gated labels against the plain labels is measured, not assumed.  The plain
@@ -95,7 +97,7 @@ def parse():
                     help="surrogate: confidence gate of the headline pass")
     ap.add_argument("--config-gate", type=float, default=0.95,
                     help="surrogate: the configured gate, run and reported beside the headline")
-    ap.add_argument("--inflight", type=int, default=2,
+    ap.add_argument("--inflight", type=int, default=4,
                     help="surrogate: passes in flight (each on its own stream and host thread)")
     ap.add_argument("--epochs", type=int, default=600)
     ap.add_argument("--rossler-coupling", choices=["edges", "dense"], default="edges",
@@ -240,18 +242,27 @@ def cpu_baselines(sysname, y0_cpu, params_rows, save_ts, *, n_torch=4096, n_scip
 # ---- passes in flight -------------------------------------------------------------
 
 def run_passes(fn, k: int, inflight: int):
-    """k calls of fn with up to `inflight` in flight: each of `inflight` host
-    threads issues its share of the calls on a stream of its own.  Returns
-    (wall ms per call between two device-wide synchronisations, the last
-    result of the first thread)."""
+    """k calls of fn with up to `inflight` in flight: `inflight` host threads,
+    each issuing on a stream of its own, take the next of the k calls as they
+    finish issuing the previous one.  Returns (wall ms per call between two
+    device-wide synchronisations, the last result of the first thread)."""
     streams = [torch.cuda.Stream() for _ in range(inflight)]
     out = [None] * inflight
     err = []
+    lock = threading.Lock()
+    left = [k]
+
+    def take():
+        with lock:
+            if left[0] == 0:
+                return False
+            left[0] -= 1
+            return True
 
     def worker(i):
         try:
             with torch.cuda.stream(streams[i]):
-                for _ in range(i, k, inflight):
+                while take():
                     out[i] = fn()
         except BaseException as e:  # surfaced after the join
             err.append(e)
