@@ -237,13 +237,13 @@ int or_rhs(int sys, int bits, int S, double t, const double *y, const double *p,
 {
     if (!sys_dim_ok(sys, S) || n_p > 16) return OR_ERR_ARG;
     if (bits == 32) {
-        float yy[OR_ROSSLER_N * 3], dd[OR_ROSSLER_N * 3], pp[16];
+        float yy[OR_ROSSLER_N * 3] = {0}, dd[OR_ROSSLER_N * 3], pp[16] = {0};
         for (int i = 0; i < S; i++) yy[i] = (float)y[i];
         for (int i = 0; i < n_p; i++) pp[i] = (float)p[i];
         rhs_f32(sys, (float)t, yy, dd, pp, S);
         for (int i = 0; i < S; i++) dy[i] = dd[i];
     } else {
-        double pp[16];
+        double pp[16] = {0};
         for (int i = 0; i < n_p; i++) pp[i] = p[i];
         rhs_f64(sys, t, y, dy, pp, S);
     }

@@ -482,6 +482,58 @@ def test_float_sine_cosine_are_within_bound():
     assert np.abs(sincos(x, cos_bit=0)[1] - tc).max() > 0.5
 
 
+def test_float_sine_by_pi_reduction_is_within_bound():
+    """The pendulum's branch-free float sine (systems.hpp sin_fast:
+    Cody-Waite reduction by pi in three FMA steps, the parity of k read from
+    the last bit of x / pi + 1.5 * 2^23, the sign applied to r, an odd
+    polynomial of degree 9; constants read from the source), evaluated in
+    float32 in the kernel's order with exact FMAs, stays within 1.3e-7 of
+    sin x for |x| <= 2^17, near multiples of pi/2 and pi included; a dropped
+    reduction step or polynomial term, or the sign taken from the wrong bit,
+    breaks the bound."""
+    import os
+    import re
+
+    src = open(os.path.join(os.path.dirname(__file__), "..", "pybasin_amd", "csrc",
+                            "systems.hpp")).read()
+    b = src[src.index("float sin_fast(float x) {"):]
+    b = b[:b.index("\n}")]
+    lit = [np.float32(v) for v in re.findall(r"(-?\d\.\d+e[-+]\d+)f", b)]
+    assert len(lit) == 10, lit
+    inv_pi, magic, magic2, p1, p2, p3, c9, c7, c5, c3 = lit
+    assert magic == magic2 == np.float32(1.5 * 2**23)
+    assert "(__float_as_int(t) << 31)" in b
+    f = np.float32
+
+    def fma(a, b, c):
+        return (np.float64(a) * np.float64(b) + np.float64(c)).astype(f)
+
+    def sin32(x, drop=None, parity_bit=0):
+        t = fma(x, inv_pi, magic)
+        k = (t - magic).astype(f)
+        r = fma(-k, p1, x)
+        r = fma(-k, p2, r) if drop != "p2" else r
+        r = fma(-k, p3, r)
+        par = (t.view(np.int32) >> parity_bit) & 1
+        r = (r.view(np.int32) ^ (par << 31)).view(f)
+        r2 = (r * r).astype(f)
+        p = fma(fma(fma(c9 if drop != "c9" else f(0), r2, c7), r2, c5), r2, c3)
+        return fma((r2 * r).astype(f), p, r)
+
+    rng = np.random.default_rng(5)
+    x = rng.uniform(-131072, 131072, 400_000).astype(f)
+    m = rng.integers(-80_000, 80_000, 100_000)
+    near = np.concatenate([(m * np.pi / 2).astype(f), (m * np.pi / 2 / 2).astype(f)])
+    x = np.concatenate([x, near, np.nextafter(near, f(np.inf)), np.nextafter(near, f(-np.inf)),
+                        rng.uniform(-4, 4, 100_000).astype(f)])
+    x = x[np.abs(x) <= 131072]
+    ts = np.sin(x.astype(np.float64))
+    assert np.abs(sin32(x) - ts).max() <= 1.3e-7
+    for d in ("p2", "c9"):
+        assert np.abs(sin32(x, drop=d) - ts).max() > 1e-6, d
+    assert np.abs(sin32(x, parity_bit=1) - ts).max() > 0.5
+
+
 def test_rossler_edge_arrays_give_the_laplacian():
     """The edge arrays handed to bh_solve_rossler_edges (both directions of
     every edge, the reference's build_edge_arrays) reproduce L = D - A of the
