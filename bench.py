@@ -60,8 +60,9 @@ MLP_MACS = 6 * 256 + 256 * 256 * 2 + 256 * 2  # multiply-adds per row (unpadded 
 TRAFFIC_BYTES = {("duffing", "f32", 100_000, False): 9.50e6}
 # the two-phase launch adds the round trip of each suspended trajectory's
 # state record and index through HBM, per suspended trajectory (same
-# counters, phase budget 3,239: profiles/dopri5_phase_duffing_100k_f32.md)
-TRAFFIC_PER_SUSPENDED = {("duffing", "f32", 100_000): 390.0}
+# counters at the tuner's phase budget 2,779, 64,422 suspended: 3.725e7
+# bytes per launch pair; profiles/dopri5_phase_duffing_100k_f32.md)
+TRAFFIC_PER_SUSPENDED = {("duffing", "f32", 100_000): 431.0}
 
 DUFFING_TEMPLATES = [[-0.21, 0.02], [1.05, 0.77], [-0.67, 0.02], [-0.46, 0.30], [-0.43, 0.12]]
 
