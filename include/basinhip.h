@@ -382,6 +382,88 @@ int32_t bh_window_autocorr_periodicity(int32_t dtype, const void *traj, int64_t 
 int32_t bh_window_spectral_ratio(int32_t dtype, const void *traj, int64_t n_save, int64_t n,
                                  int32_t dim, int64_t start, void *out_ratio, void *stream);
 
+/* Raw accumulators of the comprehensive window features: for each series of
+   the window rows [start, n_save) of a kept trajectory buffer (n_save, n,
+   dim) in dtype, out (BH_WF_SLOTS, n, dim) float64 holds the slots below
+   (out[k][ic][state]); a window holding a non-finite value gives NaN in every
+   slot.  Indices are 0-based window positions; sums run in double.  The
+   named features of the reference's comprehensive set
+   (src/pybasin/ts_torch/settings.py:223-340) follow from them on the host:
+   sum_values, length, abs_energy, kurtosis, skewness, variation_coefficient
+   (torch_features_statistical.py:20-162), absolute_sum_of_changes,
+   mean_abs_change, mean_change, mean_second_derivative_central
+   (torch_features_change.py), count_above / count_below (t = 0),
+   count_above_mean, count_below_mean, count_value (0, 1, -1), count_in_range
+   ([-1, 1], [-1e12, 0], [0, 1e12]) (torch_features_count.py),
+   has_duplicate_max / _min, has_variance_larger_than_standard_deviation,
+   large_standard_deviation (torch_features_boolean.py),
+   first/last_location_of_maximum/minimum, index_mass_quantile
+   (q = 0.1 .. 0.9 without 0.5) (torch_features_location.py),
+   longest_strike_above/below_mean, number_crossing_m (m = 0, -1, 1),
+   number_peaks (n = 1, 3, 5) (torch_features_pattern.py:10-108),
+   autocorrelation (lags 0..9) (torch_features_autocorrelation.py:13-28),
+   cid_ce (torch_features_entropy_complexity.py:182-190), c3,
+   time_reversal_asymmetry_statistic (lags 1..3), energy_ratio_by_chunks (10
+   segments) (torch_features_advanced.py:84-123), ratio_beyond_r_sigma and
+   mean_n_absolute_max (n <= 7) (torch_features_statistical.py:165-181),
+   linear_trend (slope, intercept, rvalue, pvalue, stderr)
+   (torch_features_trend.py:11-51), symmetry_looking (with the median of
+   bh_window_order_stats). */
+enum {
+    BH_WF_N = 0,             /* window length N'                                   */
+    BH_WF_SUM = 1,           /* sum x                                              */
+    BH_WF_SUMSQ = 2,         /* sum x^2                                            */
+    BH_WF_MIN = 3,
+    BH_WF_MAX = 4,
+    BH_WF_FIRST_MAX = 5,     /* first index of the maximum                         */
+    BH_WF_LAST_MAX = 6,
+    BH_WF_FIRST_MIN = 7,
+    BH_WF_LAST_MIN = 8,
+    BH_WF_ABS_CHANGE = 9,    /* sum |x[i+1] - x[i]|                                */
+    BH_WF_SQ_CHANGE = 10,    /* sum (x[i+1] - x[i])^2                              */
+    BH_WF_X_FIRST = 11,      /* x[0], x[1], x[N'-2], x[N'-1]                       */
+    BH_WF_X_SECOND = 12,
+    BH_WF_X_SECOND_LAST = 13,
+    BH_WF_X_LAST = 14,
+    BH_WF_COUNT_POS = 15,    /* #(x > 0), #(x < 0), #(x == 0), #(x == 1), #(x == -1) */
+    BH_WF_COUNT_NEG = 16,
+    BH_WF_COUNT_ZERO = 17,
+    BH_WF_COUNT_ONE = 18,
+    BH_WF_COUNT_MINUS_ONE = 19,
+    BH_WF_RANGE_UNIT = 20,   /* #(-1 <= x <= 1), #(-1e12 <= x <= 0), #(0 <= x <= 1e12) */
+    BH_WF_RANGE_NONPOS = 21,
+    BH_WF_RANGE_NONNEG = 22,
+    BH_WF_CROSS_0 = 23,      /* #(i: (x[i] > m) != (x[i+1] > m)) for m = 0, -1, 1  */
+    BH_WF_CROSS_M1 = 24,
+    BH_WF_CROSS_P1 = 25,
+    BH_WF_CHUNK = 26,        /* 10 slots: sum x^2 over [j s, (j+1) s), s = N' / 10 */
+    BH_WF_C3 = 36,           /* 3 slots: sum x[i] x[i+l] x[i+2l], l = 1..3         */
+    BH_WF_TREV = 39,         /* 3 slots: sum x[i+2l]^2 x[i+l] - x[i+l] x[i]^2      */
+    BH_WF_SUM_ABS = 42,      /* sum |x|                                            */
+    BH_WF_TOP_ABS = 43,      /* 7 slots: the largest |x|, descending (-1: none)    */
+    BH_WF_PEAKS_1 = 50,      /* strict maxima of support 1, 3, 5                   */
+    BH_WF_PEAKS_3 = 51,
+    BH_WF_PEAKS_5 = 52,
+    BH_WF_CM2 = 53,          /* sum (x - mean)^2, ^3, ^4                           */
+    BH_WF_CM3 = 54,
+    BH_WF_CM4 = 55,
+    BH_WF_SS_TX = 56,        /* sum (i - (N'-1)/2) (x - mean)                      */
+    BH_WF_COUNT_ABOVE_MEAN = 57,
+    BH_WF_COUNT_BELOW_MEAN = 58,
+    BH_WF_STRIKE_ABOVE = 59, /* longest run of x > mean, of x < mean               */
+    BH_WF_STRIKE_BELOW = 60,
+    BH_WF_COUNT_MAX = 61,    /* #(x == max), #(x == min)                           */
+    BH_WF_COUNT_MIN = 62,
+    BH_WF_BEYOND = 63,       /* 10 slots: #(|x - mean| > r sd), r = 0.5, 1, 1.5, 2,
+                                2.5, 3, 5, 6, 7, 10 (sd: population)               */
+    BH_WF_AUTOCOV = 73,      /* 9 slots: sum (x[i] - mean)(x[i+l] - mean), l = 1..9 */
+    BH_WF_MASS = 82,         /* 8 slots: first i with sum_{j<=i} |x[j]| >= q sum |x|,
+                                q = 0.1, 0.2, 0.3, 0.4, 0.6, 0.7, 0.8, 0.9          */
+    BH_WF_SLOTS = 90
+};
+int32_t bh_window_features(int32_t dtype, const void *traj, int64_t n_save, int64_t n,
+                           int32_t dim, int64_t start, double *out, void *stream);
+
 /* ---- label layer (replaces the sklearn calls of the reference's
    HDBSCANClusterer, src/pybasin/predictors/hdbscan_clusterer.py:12-148, and
    the 1-NN template classifiers of the case studies) --------------------- */

@@ -17,6 +17,10 @@ Contents
 * ``amplitude``, ``linear_trend_slope``, ``autocorrelation_periodicity``,
   ``spectral_frequency_ratio``, ``local_maxima_1d``
                        the dynamical-system feature set (ts_torch/settings.py:407-417).
+* ``comprehensive``    one calculator of the comprehensive set on one window
+                       series (ts_torch/settings.py:223-340; calculators in
+                       ts_torch/calculators/torch_features_*.py), per series in
+                       float64, directly from the reference's formulas.
 * ``sync_features``    SynchronizationFeatureExtractor._compute_sync_features
                        (case_studies/rossler_network/synchronization_feature_extractor.py:73-106).
 * ``sync_labels``      SynchronizationClassifier.fit_predict
@@ -297,3 +301,150 @@ def basin_stability(labels: np.ndarray) -> dict:
         s = float(np.sum(labels == lab)) / n
         out[str(lab)] = (s, float(np.sqrt(s * (1 - s) / n)))
     return out
+
+
+def comprehensive(x: np.ndarray, name: str, params: dict | None = None) -> float:
+    """One calculator of the reference's comprehensive set on one series x
+    (N,), float64, written from the reference's torch calculators
+    (src/pybasin/ts_torch/calculators/): NaN if x holds a non-finite value
+    (the convention of the window kernel this checks)."""
+    x = np.asarray(x, dtype=np.float64)
+    p = params or {}
+    n = x.size
+    if not np.all(np.isfinite(x)):
+        return float("nan")
+    mu = x.mean()
+    sd = x.std()  # population (correction=0)
+    if name == "sum_values":
+        return float(x.sum())
+    if name == "length":
+        return float(n)
+    if name == "abs_energy":
+        return float((x ** 2).sum())
+    if name == "kurtosis":  # torch_features_statistical.py:125-135
+        m2 = ((x - mu) ** 2).mean()
+        m4 = ((x - mu) ** 4).mean()
+        g2 = m4 / (m2 ** 2 + 1e-10) - 3
+        return float(((n - 1) / ((n - 2) * (n - 3) + 1e-10)) * ((n + 1) * g2 + 6))
+    if name == "skewness":  # :139-150
+        m2 = ((x - mu) ** 2).mean()
+        m3 = ((x - mu) ** 3).mean()
+        g1 = m3 / (m2 ** 1.5 + 1e-10)
+        return float(g1 * np.sqrt(n * (n - 1)) / (n - 2 + 1e-10))
+    if name == "variation_coefficient":  # :160-162
+        return float(sd / (abs(mu) + 1e-10))
+    if name == "absolute_sum_of_changes":  # torch_features_change.py
+        return float(np.abs(np.diff(x)).sum())
+    if name == "mean_abs_change":
+        return float(np.abs(np.diff(x)).mean())
+    if name == "mean_change":
+        return float((x[-1] - x[0]) / (n - 1))
+    if name == "mean_second_derivative_central":
+        if n < 2:  # the reference indexes x[-2] (an error); NaN here
+            return float("nan")
+        return float((x[-1] - x[-2] - x[1] + x[0]) / (2 * (n - 2)))
+    if name == "count_above_mean":  # torch_features_count.py
+        return float((x > mu).sum())
+    if name == "count_below_mean":
+        return float((x < mu).sum())
+    if name == "count_above":
+        return float((x > p["t"]).mean())
+    if name == "count_below":
+        return float((x < p["t"]).mean())
+    if name == "count_value":
+        return float((x == p["value"]).sum())
+    if name == "count_in_range":
+        return float(((x >= p["min_val"]) & (x <= p["max_val"])).sum())
+    if name == "has_duplicate_max":  # torch_features_boolean.py
+        return float((x == x.max()).sum() > 1)
+    if name == "has_duplicate_min":
+        return float((x == x.min()).sum() > 1)
+    if name == "has_variance_larger_than_standard_deviation":
+        return float(sd > 1)
+    if name == "large_standard_deviation":
+        return float(sd > p["r"] * (x.max() - x.min()))
+    if name == "first_location_of_maximum":  # torch_features_location.py
+        return float(np.argmax(x) / n)
+    if name == "first_location_of_minimum":
+        return float(np.argmin(x) / n)
+    if name == "last_location_of_maximum":
+        return float((n - 1 - np.argmax(x[::-1])) / n)
+    if name == "last_location_of_minimum":
+        return float((n - 1 - np.argmin(x[::-1])) / n)
+    if name == "index_mass_quantile":
+        cum = np.cumsum(np.abs(x))
+        return float(np.argmax(cum >= p["q"] * cum[-1]) / n)
+    if name in ("longest_strike_above_mean", "longest_strike_below_mean"):
+        mask = x > mu if name.endswith("above_mean") else x < mu
+        best = run = 0
+        for m in mask:
+            run = run + 1 if m else 0
+            best = max(best, run)
+        return float(best)
+    if name == "number_crossing_m":  # torch_features_pattern.py:56-61
+        a = x > p["m"]
+        return float((a[1:] != a[:-1]).sum())
+    if name == "number_peaks":  # :64-107 (strict maximum of its 2n + 1 window)
+        k = int(p["n"])
+        cnt = 0
+        for i in range(k, n - k):
+            if all(x[i] > x[i - j] and x[i] > x[i + j] for j in range(1, k + 1)):
+                cnt += 1
+        return float(cnt)
+    if name == "autocorrelation":  # torch_features_autocorrelation.py:13-28
+        lag = int(p["lag"])
+        if lag == 0:
+            return 1.0
+        if lag >= n:
+            return 0.0
+        c = x - mu
+        return float((c[:-lag] * c[lag:]).sum() / ((c ** 2).sum() + 1e-10))
+    if name == "cid_ce":  # torch_features_entropy_complexity.py:182-190
+        ce = np.sqrt((np.diff(x) ** 2).sum())
+        return float(ce / (sd + 1e-10) if p.get("normalize", True) else ce)
+    if name == "ratio_beyond_r_sigma":  # torch_features_statistical.py:175-181
+        return float((np.abs(x - mu) > p["r"] * sd).mean())
+    if name == "c3":  # torch_features_advanced.py:84-90
+        lag = int(p["lag"])
+        if 2 * lag >= n:
+            return 0.0
+        return float((x[:-2 * lag] * x[lag:-lag] * x[2 * lag:]).mean())
+    if name == "time_reversal_asymmetry_statistic":  # :109-118
+        lag = int(p["lag"])
+        if 2 * lag >= n:
+            return 0.0
+        a, b, c = x[:-2 * lag], x[lag:-lag], x[2 * lag:]
+        return float((c ** 2 * b - b * a ** 2).mean())
+    if name == "energy_ratio_by_chunks":  # :93-106
+        seg = n // int(p["num_segments"])
+        if seg == 0:
+            return 0.0
+        st = int(p["segment_focus"]) * seg
+        return float((x[st:min(st + seg, n)] ** 2).sum() / ((x ** 2).sum() + 1e-10))
+    if name == "mean_n_absolute_max":  # torch_features_statistical.py:165-172
+        k = min(int(p["number_of_maxima"]), n)
+        return float(np.sort(np.abs(x))[::-1][:k].mean())
+    if name == "linear_trend":  # torch_features_trend.py:11-51
+        t = np.arange(n, dtype=np.float64)
+        ss_tt = ((t - t.mean()) ** 2).sum()
+        ss_tx = ((t - t.mean()) * (x - mu)).sum()
+        slope = ss_tx / (ss_tt + 1e-10)
+        intercept = mu - slope * t.mean()
+        ss_xx = ((x - mu) ** 2).sum()
+        rvalue = ss_tx / (np.sqrt(ss_tt * ss_xx) + 1e-10)
+        attr = p.get("attr", "slope")
+        if attr == "slope":
+            return float(slope)
+        if attr == "intercept":
+            return float(intercept)
+        if attr == "rvalue":
+            return float(rvalue)
+        if attr == "pvalue":
+            t_stat = rvalue * np.sqrt(n - 2) / (np.sqrt(1 - rvalue ** 2) + 1e-10)
+            return float(1 / (1 + abs(t_stat)))
+        resid = x - (slope * t + intercept)
+        return float(np.sqrt((resid ** 2).sum() / (n - 2) / (ss_tt + 1e-10)))
+    if name == "symmetry_looking":  # torch_features_statistical.py:184-190 (torch median: lower middle)
+        med = np.sort(x)[(n - 1) // 2]
+        return float(abs(mu - med) < p["r"] * (x.max() - x.min()))
+    raise KeyError(name)

@@ -24,6 +24,16 @@ BH_REFILL_QUORUM_SHIFT = 8
 BH_NFEAT = 4
 BH_NFEAT_TREND = 5
 BH_WORKSPACE_BYTES = 256
+# slots of bh_window_features (include/basinhip.h BH_WF_*)
+WF = dict(N=0, SUM=1, SUMSQ=2, MIN=3, MAX=4, FIRST_MAX=5, LAST_MAX=6, FIRST_MIN=7, LAST_MIN=8,
+          ABS_CHANGE=9, SQ_CHANGE=10, X_FIRST=11, X_SECOND=12, X_SECOND_LAST=13, X_LAST=14,
+          COUNT_POS=15, COUNT_NEG=16, COUNT_ZERO=17, COUNT_ONE=18, COUNT_MINUS_ONE=19,
+          RANGE_UNIT=20, RANGE_NONPOS=21, RANGE_NONNEG=22, CROSS_0=23, CROSS_M1=24, CROSS_P1=25,
+          CHUNK=26, C3=36, TREV=39, SUM_ABS=42, TOP_ABS=43, PEAKS_1=50, PEAKS_3=51, PEAKS_5=52,
+          CM2=53, CM3=54, CM4=55, SS_TX=56, COUNT_ABOVE_MEAN=57, COUNT_BELOW_MEAN=58,
+          STRIKE_ABOVE=59, STRIKE_BELOW=60, COUNT_MAX=61, COUNT_MIN=62, BEYOND=63, AUTOCOV=73,
+          MASS=82)
+BH_WF_SLOTS = 90
 BH_NOALLOC_SCRATCH_BYTES = 64 << 20
 
 # every symbol include/basinhip.h declares
@@ -35,7 +45,8 @@ EXPORTS = ("bh_version", "bh_last_error", "bh_solve_batch", "bh_solve_batch_peak
            "bh_mlp_train_workspace_bytes", "bh_mlp_train_step", "bh_knn",
            "bh_hdbscan_workspace_bytes", "bh_core_distances", "bh_hdbscan_mst", "bh_hdbscan_labels",
            "bh_silhouette",
-           "bh_window_order_stats", "bh_window_autocorr_periodicity", "bh_window_spectral_ratio")
+           "bh_window_order_stats", "bh_window_autocorr_periodicity", "bh_window_spectral_ratio",
+           "bh_window_features")
 
 _lock = threading.Lock()
 _lib: ctypes.CDLL | None = None
@@ -105,6 +116,7 @@ def _declare(L: ctypes.CDLL) -> None:
         "bh_window_order_stats": [i32, vp, i64, i64, i32, i64, vp, i32, vp, vp],
         "bh_window_autocorr_periodicity": [i32, vp, i64, i64, i32, i64, i32, f64, vp, vp, vp],
         "bh_window_spectral_ratio": [i32, vp, i64, i64, i32, i64, vp, vp],
+        "bh_window_features": [i32, vp, i64, i64, i32, i64, vp, vp],
     }
     for name, args in sigs.items():
         fn = getattr(L, name)

@@ -858,4 +858,25 @@ int32_t bh_window_spectral_ratio(int32_t dtype, const void *traj, int64_t n_save
     return 0;
 }
 
+int32_t bh_window_features(int32_t dtype, const void *traj, int64_t n_save, int64_t n,
+                           int32_t dim, int64_t start, double *out, void *stream) {
+    g_err[0] = 0;
+    int32_t rc = check_series_args("bh_window_features", dtype, traj, n_save, n, dim, start);
+    if (rc) return rc;
+    if (n == 0) return 0;
+    if (!out) return fail(-5, "bh_window_features: null output");
+    const int64_t np = n * dim;
+    if ((np + 255) / 256 > 2147483647LL) return fail(-5, "bh_window_features: too many series");
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    const dim3 grid((unsigned)((np + 255) / 256));
+    if (dtype == BH_F32)
+        hipLaunchKernelGGL(bh::win::window_features_kernel<float>, grid, dim3(256), 0, s,
+                           reinterpret_cast<const float *>(traj), n_save, n, dim, start, out);
+    else
+        hipLaunchKernelGGL(bh::win::window_features_kernel<double>, grid, dim3(256), 0, s,
+                           reinterpret_cast<const double *>(traj), n_save, n, dim, start, out);
+    BH_HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
 }  // extern "C"

@@ -14,6 +14,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "../../include/basinhip.h"
+
 namespace bh {
 namespace win {
 
@@ -326,6 +328,248 @@ __global__ void __launch_bounds__(kSeriesBlock)
             }
         }
     }
+}
+
+// ---- comprehensive window features: raw accumulators per series ----------
+//
+// One lane per (trajectory, state) series of the window rows [start,
+// n_save), consecutive lanes on consecutive addresses of each saved row.
+// Two passes in double: the first collects sums, extremes and their first
+// and last positions, changes, counts, crossings, chunk energies, the lag
+// products of c3 and the time-reversal statistic, the seven largest |x| and
+// the peaks of support 1, 3 and 5; the second, given the mean and the
+// standard deviation of the first (from sums shifted by the first value),
+// the central moments, the counts and longest runs about the mean, the
+// counts beyond r standard deviations, the autocovariances at lags 1..9,
+// the duplicate-extreme counts and the index-mass quantiles.  The slots are
+// the BH_WF_* indices of include/basinhip.h; a window holding a non-finite
+// value gives NaN in every slot.
+
+constexpr int kWfTop = 7;     // largest |x| kept (mean_n_absolute_max up to 7)
+constexpr int kWfLags = 9;    // autocovariance lags 1..9
+constexpr int kWfChunks = 10; // energy_ratio_by_chunks segments
+__device__ constexpr double kWfSigma[10] = {0.5, 1.0, 1.5, 2.0, 2.5, 3.0, 5.0, 6.0, 7.0, 10.0};
+__device__ constexpr double kWfMass[8] = {0.1, 0.2, 0.3, 0.4, 0.6, 0.7, 0.8, 0.9};
+
+// peak of support k at the centre of the last 2k + 1 values of the window
+// w (w[0] newest): strictly greater than each of its k neighbours on both
+// sides
+template <class T, int K>
+__device__ __forceinline__ bool wf_peak(const T (&w)[11]) {
+    const T c = w[K];
+    bool pk = true;
+#pragma unroll
+    for (int j = 0; j < K; j++) pk = pk && c > w[j] && c > w[2 * K - j];
+    return pk;
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) window_features_kernel(const T *traj, int64_t n_save, int64_t n,
+                                                              int S, int64_t start, double *out) {
+    const int64_t np = n * S;
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // ic * S + s
+    if (p >= np) return;
+    const T *col = traj + start * np + p;
+    const int64_t N = n_save - start;
+    double *o = out + p;
+    auto put = [&](int slot, double v) { o[(int64_t)slot * np] = v; };
+
+    // ---- pass 1 ----
+    const double x0 = (double)col[0];
+    bool fin = true;
+    double s1 = 0, s2 = 0, sh1 = 0, sh2 = 0, sabs = 0, mn = x0, mx = x0;
+    int64_t first_max = 0, last_max = 0, first_min = 0, last_min = 0;
+    double sad = 0, ssd = 0;
+    int64_t c_pos = 0, c_neg = 0, c_zero = 0, c_one = 0, c_mone = 0, r_unit = 0, r_neg = 0, r_pos = 0;
+    int64_t cr0 = 0, crm = 0, crp = 0;
+    const int64_t seg = N / kWfChunks;
+    double chunk[kWfChunks];
+#pragma unroll
+    for (int j = 0; j < kWfChunks; j++) chunk[j] = 0;
+    double c3s[3] = {0, 0, 0}, trs[3] = {0, 0, 0};
+    double hist[6];  // x_{i-1} .. x_{i-6}
+#pragma unroll
+    for (int j = 0; j < 6; j++) hist[j] = 0;
+    double top[kWfTop];
+#pragma unroll
+    for (int j = 0; j < kWfTop; j++) top[j] = -1.0;
+    T w[11];  // x_i .. x_{i-10} for the peak tests
+#pragma unroll
+    for (int j = 0; j < 11; j++) w[j] = T(0);
+    int64_t pk1 = 0, pk3 = 0, pk5 = 0;
+    double prev = x0;
+    for (int64_t i = 0; i < N; i++) {
+        const T xt = col[i * np];
+        const double x = (double)xt;
+        fin = fin && isfinite(x);
+        s1 += x;
+        s2 = fma(x, x, s2);
+        const double d = x - x0;
+        sh1 += d;
+        sh2 = fma(d, d, sh2);
+        sabs += fabs(x);
+        if (x > mx) { mx = x; first_max = i; }
+        if (x >= mx) last_max = i;
+        if (x < mn) { mn = x; first_min = i; }
+        if (x <= mn) last_min = i;
+        if (i > 0) {
+            const double dx = x - prev;
+            sad += fabs(dx);
+            ssd = fma(dx, dx, ssd);
+            cr0 += (prev > 0.0) != (x > 0.0);
+            crm += (prev > -1.0) != (x > -1.0);
+            crp += (prev > 1.0) != (x > 1.0);
+        }
+        c_pos += x > 0.0;
+        c_neg += x < 0.0;
+        c_zero += x == 0.0;
+        c_one += x == 1.0;
+        c_mone += x == -1.0;
+        r_unit += (x >= -1.0) && (x <= 1.0);
+        r_neg += (x >= -1e12) && (x <= 0.0);
+        r_pos += (x >= 0.0) && (x <= 1e12);
+        if (seg > 0) {
+            const int64_t cj = i / seg;
+#pragma unroll
+            for (int j = 0; j < kWfChunks; j++)
+                if (cj == j) chunk[j] = fma(x, x, chunk[j]);
+        }
+        // lag products ending at i: x_{i-2l} x_{i-l} x_i and x_i^2 x_{i-l} - x_{i-l} x_{i-2l}^2
+#pragma unroll
+        for (int l = 1; l <= 3; l++) {
+            if (i >= 2 * l) {
+                const double a = hist[2 * l - 1], b = hist[l - 1];
+                c3s[l - 1] += a * b * x;
+                trs[l - 1] += x * x * b - b * a * a;
+            }
+        }
+#pragma unroll
+        for (int j = 5; j > 0; j--) hist[j] = hist[j - 1];
+        hist[0] = x;
+        // the seven largest |x| in descending order
+        double v = fabs(x);
+#pragma unroll
+        for (int j = 0; j < kWfTop; j++) {
+            const double hi = fmax(top[j], v), lo = fmin(top[j], v);
+            top[j] = hi;
+            v = lo;
+        }
+#pragma unroll
+        for (int j = 10; j > 0; j--) w[j] = w[j - 1];
+        w[0] = xt;
+        if (i >= 2) pk1 += wf_peak<T, 1>(w);
+        if (i >= 6) pk3 += wf_peak<T, 3>(w);
+        if (i >= 10) pk5 += wf_peak<T, 5>(w);
+        prev = x;
+    }
+    if (!fin) {
+        for (int k = 0; k < BH_WF_SLOTS; k++) put(k, NAN);
+        return;
+    }
+    const double Nd = (double)N;
+    put(BH_WF_N, Nd);
+    put(BH_WF_SUM, s1);
+    put(BH_WF_SUMSQ, s2);
+    put(BH_WF_MIN, mn);
+    put(BH_WF_MAX, mx);
+    put(BH_WF_FIRST_MAX, (double)first_max);
+    put(BH_WF_LAST_MAX, (double)last_max);
+    put(BH_WF_FIRST_MIN, (double)first_min);
+    put(BH_WF_LAST_MIN, (double)last_min);
+    put(BH_WF_ABS_CHANGE, sad);
+    put(BH_WF_SQ_CHANGE, ssd);
+    put(BH_WF_X_FIRST, x0);
+    put(BH_WF_X_SECOND, N > 1 ? (double)col[np] : NAN);
+    put(BH_WF_X_SECOND_LAST, N > 1 ? hist[1] : NAN);
+    put(BH_WF_X_LAST, hist[0]);
+    put(BH_WF_COUNT_POS, (double)c_pos);
+    put(BH_WF_COUNT_NEG, (double)c_neg);
+    put(BH_WF_COUNT_ZERO, (double)c_zero);
+    put(BH_WF_COUNT_ONE, (double)c_one);
+    put(BH_WF_COUNT_MINUS_ONE, (double)c_mone);
+    put(BH_WF_RANGE_UNIT, (double)r_unit);
+    put(BH_WF_RANGE_NONPOS, (double)r_neg);
+    put(BH_WF_RANGE_NONNEG, (double)r_pos);
+    put(BH_WF_CROSS_0, (double)cr0);
+    put(BH_WF_CROSS_M1, (double)crm);
+    put(BH_WF_CROSS_P1, (double)crp);
+#pragma unroll
+    for (int j = 0; j < kWfChunks; j++) put(BH_WF_CHUNK + j, chunk[j]);
+#pragma unroll
+    for (int l = 0; l < 3; l++) {
+        put(BH_WF_C3 + l, c3s[l]);
+        put(BH_WF_TREV + l, trs[l]);
+    }
+    put(BH_WF_SUM_ABS, sabs);
+#pragma unroll
+    for (int j = 0; j < kWfTop; j++) put(BH_WF_TOP_ABS + j, top[j]);
+    put(BH_WF_PEAKS_1, (double)pk1);
+    put(BH_WF_PEAKS_3, (double)pk3);
+    put(BH_WF_PEAKS_5, (double)pk5);
+
+    // ---- pass 2: about the mean ----
+    const double mu = s1 / Nd;
+    const double m1 = sh1 / Nd;
+    const double sd = sqrt(fmax(sh2 / Nd - m1 * m1, 0.0));
+    const double tbar = 0.5 * (Nd - 1.0);
+    double c2 = 0, c3 = 0, c4 = 0, stx = 0;
+    int64_t above = 0, below = 0, run_a = 0, run_b = 0, best_a = 0, best_b = 0, eq_max = 0, eq_min = 0;
+    int64_t beyond[10];
+#pragma unroll
+    for (int j = 0; j < 10; j++) beyond[j] = 0;
+    double acv[kWfLags], dh[kWfLags];
+#pragma unroll
+    for (int j = 0; j < kWfLags; j++) acv[j] = dh[j] = 0;
+    int64_t mass_idx[8];
+#pragma unroll
+    for (int j = 0; j < 8; j++) mass_idx[j] = -1;
+    double cum = 0;
+    for (int64_t i = 0; i < N; i++) {
+        const double x = (double)col[i * np];
+        const double d = x - mu;
+        const double d2 = d * d;
+        c2 += d2;
+        c3 = fma(d2, d, c3);
+        c4 = fma(d2, d2, c4);
+        stx = fma((double)i - tbar, d, stx);
+        above += x > mu;
+        below += x < mu;
+        run_a = x > mu ? run_a + 1 : 0;
+        run_b = x < mu ? run_b + 1 : 0;
+        best_a = max(best_a, run_a);
+        best_b = max(best_b, run_b);
+        eq_max += x == mx;
+        eq_min += x == mn;
+        const double ad = fabs(d);
+#pragma unroll
+        for (int j = 0; j < 10; j++) beyond[j] += ad > kWfSigma[j] * sd;
+#pragma unroll
+        for (int l = 0; l < kWfLags; l++)
+            if (i > l) acv[l] = fma(dh[l], d, acv[l]);
+#pragma unroll
+        for (int j = kWfLags - 1; j > 0; j--) dh[j] = dh[j - 1];
+        dh[0] = d;
+        cum += fabs(x);
+#pragma unroll
+        for (int j = 0; j < 8; j++)
+            if (mass_idx[j] < 0 && cum >= kWfMass[j] * sabs) mass_idx[j] = i;
+    }
+    put(BH_WF_CM2, c2);
+    put(BH_WF_CM3, c3);
+    put(BH_WF_CM4, c4);
+    put(BH_WF_SS_TX, stx);
+    put(BH_WF_COUNT_ABOVE_MEAN, (double)above);
+    put(BH_WF_COUNT_BELOW_MEAN, (double)below);
+    put(BH_WF_STRIKE_ABOVE, (double)best_a);
+    put(BH_WF_STRIKE_BELOW, (double)best_b);
+    put(BH_WF_COUNT_MAX, (double)eq_max);
+    put(BH_WF_COUNT_MIN, (double)eq_min);
+#pragma unroll
+    for (int j = 0; j < 10; j++) put(BH_WF_BEYOND + j, (double)beyond[j]);
+#pragma unroll
+    for (int l = 0; l < kWfLags; l++) put(BH_WF_AUTOCOV + l, acv[l]);
+#pragma unroll
+    for (int j = 0; j < 8; j++) put(BH_WF_MASS + j, (double)(mass_idx[j] < 0 ? N - 1 : mass_idx[j]));
 }
 
 }  // namespace win

@@ -24,6 +24,8 @@ from typing import Any
 import torch
 
 from .solvers.hip_dopri5 import FUSED_RESULTS, FusedResult, steady_index
+from .window_features import (COMPREHENSIVE_FEATURES, check_params, comprehensive_feature,
+                              window_feature_block)
 
 DEFAULT_STEADY_FRACTION = 0.85
 
@@ -37,6 +39,18 @@ TREND_FEATURES = ("linear_trend",)
 # order statistics, autocorrelation and spectrum
 WINDOW_FEATURES = ("median", "quantile", "autocorrelation_periodicity", "spectral_frequency_ratio")
 DEFAULT_FEATURES = {"mean": None, "standard_deviation": None}
+# the calculators of the reference's comprehensive set reduced by
+# bh_window_features (window_features.py); they need the saved window too
+
+
+def is_comprehensive(name: str, params: dict | None) -> bool:
+    """Whether a requested feature is formed from the window kernel's slots
+    (linear_trend with an attribute other than the fused slope is)."""
+    if name in STREAMING_FEATURES or name in WINDOW_FEATURES:
+        return False
+    if name == "linear_trend":
+        return (params or {}).get("attr", "slope") != "slope"
+    return name in COMPREHENSIVE_FEATURES
 # the reference's default set (settings.py:376-395: minimal features + delta, log_delta)
 MINIMAL_FEATURES = dict.fromkeys(["median", "mean", "standard_deviation", "variance",
                                   "root_mean_square", "maximum", "absolute_maximum", "minimum",
@@ -312,22 +326,32 @@ class StatFeatureExtractor(FeatureExtractor):
                 if fdict is None:
                     continue
             for name, plist in fdict.items():
-                if name not in STREAMING_FEATURES + TREND_FEATURES + WINDOW_FEATURES:
+                known = STREAMING_FEATURES + TREND_FEATURES + WINDOW_FEATURES
+                if name not in known and name not in COMPREHENSIVE_FEATURES:
                     raise KeyError(f"feature {name!r} is not supported; available: "
-                                   f"{STREAMING_FEATURES + TREND_FEATURES + WINDOW_FEATURES}")
-                if name == "linear_trend":
-                    for prm in plist or [None]:
-                        if (prm or {}).get("attr", "slope") != "slope":
-                            raise KeyError("linear_trend: only attr 'slope' is supported")
+                                   f"{sorted(set(known) | set(COMPREHENSIVE_FEATURES))}")
                 for params in (plist if plist else [None]):
+                    if is_comprehensive(name, params):
+                        check_params(name, params)
                     plan.append((s, name, params))
         return plan
 
     def _columns(self, plan, stats: torch.Tensor, y: torch.Tensor | None,
                  start: int) -> torch.Tensor:
         cols = []
+        blk = med = None
         for s, name, params in plan:
-            if name in WINDOW_FEATURES:
+            if is_comprehensive(name, params):
+                if y is None:
+                    raise ValueError(f"{name} needs the saved window: integrate with "
+                                     "keep_trajectories=True")
+                if blk is None:
+                    blk = window_feature_block(y, start)
+                if name == "symmetry_looking" and med is None:
+                    med = window_feature("median", None, y, start)
+                cols.append(comprehensive_feature(name, params, blk, y.shape[0] - start, med)[:, s]
+                            .to(stats.device, torch.float64))
+            elif name in WINDOW_FEATURES:
                 if y is None:
                     raise ValueError(f"{name} needs the saved window: integrate with "
                                      "keep_trajectories=True")
@@ -363,7 +387,10 @@ class StatFeatureExtractor(FeatureExtractor):
         """Whether some requested feature reads the whole window series
         (order statistics, autocorrelation, spectrum: the saved window must
         be kept)."""
-        return any(name in WINDOW_FEATURES for name in self._requested())
+        dicts = [self.features] + list(self.features_per_state.values())
+        return any(name in WINDOW_FEATURES or is_comprehensive(name, params)
+                   for d in dicts if d for name, plist in d.items()
+                   for params in (plist if plist else [None]))
 
     @property
     def needs_trend(self) -> bool:

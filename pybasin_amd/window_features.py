@@ -1,0 +1,354 @@
+"""Comprehensive window features: the calculators of the reference's
+comprehensive feature set (src/pybasin/ts_torch/settings.py:223-340) that
+follow from one two-pass reduction of each window series.
+
+bh_window_features (csrc/window.hip) reduces every (trajectory, state)
+series of the kept window rows [start, N) to the raw slots of
+include/basinhip.h (sums, extremes and their positions, changes, counts,
+crossings, chunk energies, lag products, central moments, runs,
+autocovariances, index-mass positions); the named features are formed here
+from those slots, in float64, following the reference's torch calculators
+term by term (the 1e-10 guards and the edge rules for short windows
+included).  On the CPU the same slots come from torch (the CPU template
+path).  A window holding a non-finite value gives NaN for every feature
+here (the extractor then imputes them), where the reference's counting
+calculators would count its finite entries.
+
+Differences of rounding only: the reference reduces in the trajectory dtype
+with torch's summation order, these sums run in double; counts that compare
+a sample with the mean or with r standard deviations can differ for a
+sample within rounding of the threshold.
+"""
+
+from __future__ import annotations
+
+import math
+
+import torch
+
+from . import _lib
+
+WF = _lib.WF
+
+# the calculators built here with the parameter grids of the reference's
+# comprehensive set (settings.py:223-340); None = no parameters.  Other
+# parameter values of the count / crossing / lag / quantile families are not
+# reduced by the kernel and are rejected.
+COMPREHENSIVE_FEATURES: dict[str, list[dict] | None] = {
+    "sum_values": None,
+    "length": None,
+    "abs_energy": None,
+    "kurtosis": None,
+    "skewness": None,
+    "variation_coefficient": None,
+    "absolute_sum_of_changes": None,
+    "mean_abs_change": None,
+    "mean_change": None,
+    "mean_second_derivative_central": None,
+    "count_above_mean": None,
+    "count_below_mean": None,
+    "count_above": [{"t": 0}],
+    "count_below": [{"t": 0}],
+    "has_duplicate_max": None,
+    "has_duplicate_min": None,
+    "has_variance_larger_than_standard_deviation": None,
+    "large_standard_deviation": [{"r": r * 0.05} for r in range(1, 20)],
+    "first_location_of_maximum": None,
+    "first_location_of_minimum": None,
+    "last_location_of_maximum": None,
+    "last_location_of_minimum": None,
+    "index_mass_quantile": [{"q": q} for q in [0.1, 0.2, 0.3, 0.4, 0.6, 0.7, 0.8, 0.9]],
+    "longest_strike_above_mean": None,
+    "longest_strike_below_mean": None,
+    "number_crossing_m": [{"m": 0}, {"m": -1}, {"m": 1}],
+    "number_peaks": [{"n": n} for n in [1, 3, 5]],
+    "autocorrelation": [{"lag": lag} for lag in range(10)],
+    "cid_ce": [{"normalize": True}, {"normalize": False}],
+    "ratio_beyond_r_sigma": [{"r": x} for x in [0.5, 1, 1.5, 2, 2.5, 3, 5, 6, 7, 10]],
+    "c3": [{"lag": lag} for lag in range(1, 4)],
+    "time_reversal_asymmetry_statistic": [{"lag": lag} for lag in range(1, 4)],
+    "count_value": [{"value": v} for v in [0, 1, -1]],
+    "count_in_range": [{"min_val": -1, "max_val": 1}, {"min_val": -1e12, "max_val": 0},
+                       {"min_val": 0, "max_val": 1e12}],
+    "energy_ratio_by_chunks": [{"num_segments": 10, "segment_focus": i} for i in range(10)],
+    "mean_n_absolute_max": [{"number_of_maxima": n} for n in [3, 5, 7]],
+    "linear_trend": [{"attr": a} for a in ["pvalue", "rvalue", "intercept", "slope", "stderr"]],
+    "symmetry_looking": [{"r": r * 0.05} for r in range(20)],
+}
+# calculators whose parameter may take any value (formed from the slots by
+# a comparison with the parameter)
+_FREE_PARAMS = {"large_standard_deviation", "symmetry_looking", "linear_trend"}
+_SIGMAS = [0.5, 1.0, 1.5, 2.0, 2.5, 3.0, 5.0, 6.0, 7.0, 10.0]
+_MASS_QS = [0.1, 0.2, 0.3, 0.4, 0.6, 0.7, 0.8, 0.9]
+
+
+def check_params(name: str, params: dict | None) -> None:
+    """ValueError unless the kernel's slots cover this parameter set."""
+    grid = COMPREHENSIVE_FEATURES[name]
+    if grid is None:
+        if params:
+            raise ValueError(f"{name} takes no parameters, got {params}")
+        return
+    if not params:
+        raise ValueError(f"{name} needs parameters, one of {grid}")
+    if name in _FREE_PARAMS:
+        if name == "linear_trend" and params.get("attr") not in (
+                "pvalue", "rvalue", "intercept", "slope", "stderr"):
+            raise ValueError(f"linear_trend: unknown attr {params.get('attr')!r}")
+        return
+    if name == "mean_n_absolute_max" and 1 <= int(params.get("number_of_maxima", 0)) <= 7:
+        return
+    if name == "autocorrelation" and 0 <= int(params.get("lag", -1)) <= 9:
+        return
+    if name == "energy_ratio_by_chunks" and int(params.get("num_segments", 0)) == 10 and \
+            0 <= int(params.get("segment_focus", -1)) < 10:
+        return
+    if name == "cid_ce" and "normalize" in params:
+        return
+    if params not in grid:
+        raise ValueError(f"{name}: parameters {params} are not reduced by the window kernel; "
+                         f"supported: {grid}")
+
+
+def _block_cpu(w: torch.Tensor) -> torch.Tensor:
+    """The slots of bh_window_features for a CPU window (N, B, S), in torch."""
+    x = w.to(torch.float64)
+    N, B, S = x.shape
+    out = torch.full((_lib.BH_WF_SLOTS, B, S), float("nan"), dtype=torch.float64)
+    ok = torch.isfinite(x).all(0)
+    ar = torch.arange(N, dtype=torch.float64).view(N, 1, 1)
+    mx, mn = x.max(0).values, x.min(0).values
+    o = {}
+    o["N"] = torch.full((B, S), float(N), dtype=torch.float64)
+    o["SUM"] = x.sum(0)
+    o["SUMSQ"] = (x * x).sum(0)
+    o["MIN"], o["MAX"] = mn, mx
+    big = torch.full_like(x, float(N))
+    o["FIRST_MAX"] = torch.where(x == mx, ar.expand_as(x), big).min(0).values
+    o["LAST_MAX"] = torch.where(x == mx, ar.expand_as(x), -big).max(0).values
+    o["FIRST_MIN"] = torch.where(x == mn, ar.expand_as(x), big).min(0).values
+    o["LAST_MIN"] = torch.where(x == mn, ar.expand_as(x), -big).max(0).values
+    dx = x[1:] - x[:-1]
+    o["ABS_CHANGE"] = dx.abs().sum(0)
+    o["SQ_CHANGE"] = (dx * dx).sum(0)
+    o["X_FIRST"], o["X_LAST"] = x[0], x[-1]
+    nanv = torch.full((B, S), float("nan"), dtype=torch.float64)
+    o["X_SECOND"] = x[1] if N > 1 else nanv
+    o["X_SECOND_LAST"] = x[-2] if N > 1 else nanv
+    o["COUNT_POS"] = (x > 0).double().sum(0)
+    o["COUNT_NEG"] = (x < 0).double().sum(0)
+    o["COUNT_ZERO"] = (x == 0).double().sum(0)
+    o["COUNT_ONE"] = (x == 1).double().sum(0)
+    o["COUNT_MINUS_ONE"] = (x == -1).double().sum(0)
+    o["RANGE_UNIT"] = ((x >= -1) & (x <= 1)).double().sum(0)
+    o["RANGE_NONPOS"] = ((x >= -1e12) & (x <= 0)).double().sum(0)
+    o["RANGE_NONNEG"] = ((x >= 0) & (x <= 1e12)).double().sum(0)
+    for key, m in (("CROSS_0", 0.0), ("CROSS_M1", -1.0), ("CROSS_P1", 1.0)):
+        a = x > m
+        o[key] = (a[1:] != a[:-1]).double().sum(0)
+    slots = {k: v for k, v in o.items()}
+    seg = N // 10
+    for j in range(10):
+        slots[("CHUNK", j)] = (x[j * seg:(j + 1) * seg] ** 2).sum(0) if seg > 0 else torch.zeros(B, S, dtype=torch.float64)
+    for lag in (1, 2, 3):
+        if 2 * lag < N:
+            a, b, c = x[:-2 * lag], x[lag:-lag], x[2 * lag:]
+            slots[("C3", lag - 1)] = (a * b * c).sum(0)
+            slots[("TREV", lag - 1)] = (c * c * b - b * a * a).sum(0)
+        else:
+            slots[("C3", lag - 1)] = slots[("TREV", lag - 1)] = torch.zeros(B, S, dtype=torch.float64)
+    ax = x.abs()
+    slots["SUM_ABS"] = ax.sum(0)
+    k = min(7, N)
+    top = torch.full((7, B, S), -1.0, dtype=torch.float64)
+    top[:k] = ax.topk(k, dim=0).values
+    for j in range(7):
+        slots[("TOP_ABS", j)] = top[j]
+    for key, n in (("PEAKS_1", 1), ("PEAKS_3", 3), ("PEAKS_5", 5)):
+        if 2 * n >= N:
+            slots[key] = torch.zeros(B, S, dtype=torch.float64)
+            continue
+        u = x.unfold(0, 2 * n + 1, 1)
+        c = u[..., n]
+        pk = (c > u[..., :n].max(-1).values) & (c > u[..., n + 1:].max(-1).values)
+        slots[key] = pk.double().sum(0)
+    mu = slots["SUM"] / N
+    sh = x - x[0:1]
+    m1 = sh.sum(0) / N
+    sd = torch.sqrt(torch.clamp((sh * sh).sum(0) / N - m1 * m1, min=0.0))
+    d = x - mu
+    slots["CM2"] = (d * d).sum(0)
+    slots["CM3"] = (d ** 3).sum(0)
+    slots["CM4"] = (d ** 4).sum(0)
+    slots["SS_TX"] = ((ar - 0.5 * (N - 1)) * d).sum(0)
+    slots["COUNT_ABOVE_MEAN"] = (x > mu).double().sum(0)
+    slots["COUNT_BELOW_MEAN"] = (x < mu).double().sum(0)
+    for key, mask in (("STRIKE_ABOVE", x > mu), ("STRIKE_BELOW", x < mu)):
+        run = torch.zeros(B, S, dtype=torch.float64)
+        best = torch.zeros(B, S, dtype=torch.float64)
+        for i in range(N):
+            run = (run + 1) * mask[i].double()
+            best = torch.maximum(best, run)
+        slots[key] = best
+    slots["COUNT_MAX"] = (x == mx).double().sum(0)
+    slots["COUNT_MIN"] = (x == mn).double().sum(0)
+    for j, r in enumerate(_SIGMAS):
+        slots[("BEYOND", j)] = (d.abs() > r * sd).double().sum(0)
+    for lag in range(1, 10):
+        slots[("AUTOCOV", lag - 1)] = (d[:-lag] * d[lag:]).sum(0) if lag < N else torch.zeros(B, S, dtype=torch.float64)
+    cum = ax.cumsum(0)
+    for j, q in enumerate(_MASS_QS):
+        hit = cum >= q * slots["SUM_ABS"]
+        slots[("MASS", j)] = torch.where(hit.any(0), hit.double().argmax(0).double(),
+                                         torch.full((B, S), float(N - 1), dtype=torch.float64))
+    for key, v in slots.items():
+        idx = WF[key] if isinstance(key, str) else WF[key[0]] + key[1]
+        out[idx] = v
+    return torch.where(ok.unsqueeze(0), out, torch.full_like(out, float("nan")))
+
+
+def window_feature_block(y: torch.Tensor, start: int) -> torch.Tensor:
+    """(BH_WF_SLOTS, B, S) float64 slots of the window rows [start, N) of a
+    kept trajectory tensor y (N, B, S): bh_window_features on the device, the
+    torch restatement on the CPU."""
+    N, B, S = y.shape
+    if not 0 <= start < N:
+        raise ValueError("empty feature window")
+    if y.device.type != "cuda":
+        return _block_cpu(y[start:])
+    y = y.contiguous()
+    out = torch.empty((_lib.BH_WF_SLOTS, B, S), dtype=torch.float64, device=y.device)
+    dt = _lib.BH_F32 if y.dtype == torch.float32 else _lib.BH_F64
+    s = torch.cuda.current_stream(y.device)
+    with torch.cuda.device(y.device):
+        _lib.check(_lib.lib().bh_window_features(dt, y.data_ptr(), N, B, S, int(start),
+                                                 out.data_ptr(), s.cuda_stream),
+                   "bh_window_features")
+    return out
+
+
+def comprehensive_feature(name: str, params: dict | None, blk: torch.Tensor, n_win: int,
+                          median: torch.Tensor | None = None) -> torch.Tensor:
+    """Feature `name` (B, S) float64 from the slots of windows of n_win rows
+    (the reference calculator it restates is cited beside each).
+    symmetry_looking needs the window median.  NaN where the window holds a
+    non-finite value (its N slot is NaN)."""
+    p = params or {}
+    g = lambda key, j=0: blk[WF[key] + j]  # noqa: E731
+    N = g("N")
+    n = float(n_win)
+    mean = g("SUM") / N
+    m2 = g("CM2") / N
+    sd = torch.sqrt(m2)
+    rng = g("MAX") - g("MIN")
+    if name == "sum_values":  # statistical.py:20-26
+        return g("SUM")
+    if name == "length":  # statistical.py:50-57
+        return N
+    if name == "abs_energy":  # statistical.py:119-121
+        return g("SUMSQ")
+    if name == "kurtosis":  # statistical.py:125-135
+        g2 = (g("CM4") / N) / (m2 ** 2 + 1e-10) - 3
+        return ((n - 1) / ((n - 2) * (n - 3) + 1e-10)) * ((n + 1) * g2 + 6)
+    if name == "skewness":  # statistical.py:139-150
+        g1 = (g("CM3") / N) / (m2 ** 1.5 + 1e-10)
+        return g1 * math.sqrt(max(n * (n - 1), 0.0)) / (n - 2 + 1e-10)
+    if name == "variation_coefficient":  # statistical.py:160-162
+        return sd / (mean.abs() + 1e-10)
+    if name == "absolute_sum_of_changes":  # change.py
+        return g("ABS_CHANGE")
+    if name == "mean_abs_change":
+        return g("ABS_CHANGE") / (N - 1)
+    if name == "mean_change":
+        return (g("X_LAST") - g("X_FIRST")) / (N - 1)
+    if name == "mean_second_derivative_central":
+        return (g("X_LAST") - g("X_SECOND_LAST") - g("X_SECOND") + g("X_FIRST")) / (2 * (N - 2))
+    if name == "count_above_mean":  # count.py
+        return g("COUNT_ABOVE_MEAN")
+    if name == "count_below_mean":
+        return g("COUNT_BELOW_MEAN")
+    if name == "count_above":
+        return g("COUNT_POS") / N
+    if name == "count_below":
+        return g("COUNT_NEG") / N
+    if name == "count_value":
+        return g({0: "COUNT_ZERO", 1: "COUNT_ONE", -1: "COUNT_MINUS_ONE"}[int(p["value"])])
+    if name == "count_in_range":
+        key = {(-1, 1): "RANGE_UNIT", (-1e12, 0): "RANGE_NONPOS", (0, 1e12): "RANGE_NONNEG"}
+        return g(key[(p["min_val"], p["max_val"])])
+    if name == "has_duplicate_max":  # boolean.py
+        return (g("COUNT_MAX") > 1).double().where(~N.isnan(), N)
+    if name == "has_duplicate_min":
+        return (g("COUNT_MIN") > 1).double().where(~N.isnan(), N)
+    if name == "has_variance_larger_than_standard_deviation":
+        return (sd > 1).double().where(~N.isnan(), N)
+    if name == "large_standard_deviation":
+        return (sd > float(p["r"]) * rng).double().where(~N.isnan(), N)
+    if name == "first_location_of_maximum":  # location.py
+        return g("FIRST_MAX") / N
+    if name == "first_location_of_minimum":
+        return g("FIRST_MIN") / N
+    if name == "last_location_of_maximum":
+        return g("LAST_MAX") / N
+    if name == "last_location_of_minimum":
+        return g("LAST_MIN") / N
+    if name == "index_mass_quantile":
+        return g("MASS", _MASS_QS.index(float(p["q"]))) / N
+    if name == "longest_strike_above_mean":  # pattern.py
+        return g("STRIKE_ABOVE")
+    if name == "longest_strike_below_mean":
+        return g("STRIKE_BELOW")
+    if name == "number_crossing_m":
+        return g({0: "CROSS_0", -1: "CROSS_M1", 1: "CROSS_P1"}[int(p["m"])])
+    if name == "number_peaks":
+        return g({1: "PEAKS_1", 3: "PEAKS_3", 5: "PEAKS_5"}[int(p["n"])])
+    if name == "autocorrelation":  # autocorrelation.py:13-28
+        lag = int(p["lag"])
+        if lag == 0:
+            return torch.ones_like(N).where(~N.isnan(), N)
+        if lag >= n:
+            return torch.zeros_like(N).where(~N.isnan(), N)
+        return g("AUTOCOV", lag - 1) / (g("CM2") + 1e-10)
+    if name == "cid_ce":  # entropy_complexity.py:182-190
+        ce = torch.sqrt(g("SQ_CHANGE"))
+        return ce / (sd + 1e-10) if p.get("normalize", True) else ce
+    if name == "ratio_beyond_r_sigma":  # statistical.py:175-181
+        return g("BEYOND", _SIGMAS.index(float(p["r"]))) / N
+    if name in ("c3", "time_reversal_asymmetry_statistic"):  # advanced.py:84-123
+        lag = int(p["lag"])
+        if 2 * lag >= n:
+            return torch.zeros_like(N).where(~N.isnan(), N)
+        return g("C3" if name == "c3" else "TREV", lag - 1) / (N - 2 * lag)
+    if name == "energy_ratio_by_chunks":  # advanced.py:93-106
+        if int(n) // 10 == 0:
+            return torch.zeros_like(N).where(~N.isnan(), N)
+        return g("CHUNK", int(p["segment_focus"])) / (g("SUMSQ") + 1e-10)
+    if name == "mean_n_absolute_max":  # statistical.py:165-172
+        k = min(int(p["number_of_maxima"]), int(n))
+        return sum(g("TOP_ABS", j) for j in range(k)) / k
+    if name == "linear_trend":  # trend.py:11-51
+        tbar = 0.5 * (n - 1)
+        ss_tt = n * (n * n - 1) / 12.0
+        ss_tx = g("SS_TX")
+        ss_xx = g("CM2")
+        slope = ss_tx / (ss_tt + 1e-10)
+        attr = p.get("attr", "slope")
+        if attr == "slope":
+            return slope
+        if attr == "intercept":
+            return mean - slope * tbar
+        rvalue = ss_tx / (torch.sqrt(ss_tt * ss_xx) + 1e-10)
+        if attr == "rvalue":
+            return rvalue
+        if attr == "pvalue":
+            # sqrt(n - 2) of a window of one row is NaN, as in the reference
+            t_stat = rvalue * (math.sqrt(n - 2) if n >= 2 else math.nan) / (
+                torch.sqrt(1 - rvalue ** 2) + 1e-10)
+            return 1 / (1 + t_stat.abs())
+        resid = torch.clamp(ss_xx - 2 * slope * ss_tx + slope * slope * ss_tt, min=0.0)
+        return torch.sqrt(resid / (n - 2) / (ss_tt + 1e-10))
+    if name == "symmetry_looking":  # statistical.py:184-190
+        if median is None:
+            raise ValueError("symmetry_looking needs the window median")
+        return ((mean - median.double()).abs() < float(p["r"]) * rng).double().where(~N.isnan(), N)
+    raise KeyError(f"feature {name!r} is not a comprehensive window feature")

@@ -119,3 +119,89 @@ def test_series_window_limit_is_an_error(cuda_device):
     y = torch.zeros((20_000, 2, 1), dtype=torch.float32, device="cuda")
     with pytest.raises(ValueError, match="LDS budget"):
         window_feature("spectral_frequency_ratio", None, y, 0)
+
+
+# ---- comprehensive window features (bh_window_features) ---------------------
+
+def _synthetic_windows():
+    rng = np.random.default_rng(21)
+    yield "normal", rng.normal(size=(300, 64, 2))
+    yield "ties", np.round(rng.normal(size=(160, 40, 3)) * 2) / 2
+    yield "constant", np.full((80, 5, 2), -1.0)
+    yield "ramp", np.arange(50, dtype=float)[:, None, None] * np.ones((1, 3, 2)) * 0.1 - 2
+    for n in (1, 2, 4, 7, 11, 12, 30):
+        yield f"short{n}", rng.normal(size=(n, 9, 2))
+    bad = rng.normal(size=(60, 6, 2))
+    bad[5, 1, 0] = np.nan
+    bad[7, 3, 1] = np.inf
+    bad[0, 4, 0] = -np.inf
+    yield "nonfinite", bad
+
+
+def _check_against_oracle(blk, w64, start, tag):
+    from pybasin_amd.window_features import COMPREHENSIVE_FEATURES, comprehensive_feature
+
+    x = w64[start:]
+    N, B, S = x.shape
+    med = torch.tensor(np.sort(x, axis=0)[(N - 1) // 2])
+    bad = 0
+    for name, grid in COMPREHENSIVE_FEATURES.items():
+        for p in (grid or [None]):
+            if name in ("skewness", "kurtosis") and N <= 3:
+                continue  # noise amplified by the reference's 1e-10 guards
+            got = comprehensive_feature(name, p, blk, N, med).numpy()
+            for b in range(B):
+                for s in range(S):
+                    want = O.comprehensive(x[:, b, s], name, p)
+                    g = float(got[b, s])
+                    # pvalue and stderr of a perfect linear fit (1 - r^2 and the
+                    # residuals at rounding level) are rounding noise of size ~1e-8
+                    tol = 1e-6 if name == "linear_trend" and p["attr"] in ("pvalue", "stderr") \
+                        else 1e-11
+                    if not math.isfinite(want):
+                        ok = not math.isfinite(g)
+                    else:
+                        ok = g == pytest.approx(want, rel=1e-9, abs=tol)
+                    if not ok:
+                        bad += 1
+                        assert bad < 3, (tag, name, p, b, s, g, want)
+    assert bad == 0
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_window_features_kernel_equals_oracle_synthetic(cuda_device, dtype):
+    from pybasin_amd.window_features import window_feature_block
+
+    for tag, w in _synthetic_windows():
+        wt = torch.tensor(w, dtype=dtype)
+        for start in (0, 1) if w.shape[0] > 1 else (0,):
+            blk = window_feature_block(wt.cuda(), start).cpu()
+            _check_against_oracle(blk, wt.double().numpy(), start, f"{tag}/{start}")
+            # the slots of the CPU path (same definitions, another summation
+            # order): integer-valued slots equal
+            from pybasin_amd.window_features import window_feature_block as wfb
+            cpu = wfb(wt, start)
+            ints = [k for k in range(_lib.BH_WF_SLOTS)
+                    if k not in (1, 2, 9, 10, 42, 53, 54, 55, 56) and not 26 <= k <= 41
+                    and not 73 <= k <= 81]
+            assert torch.equal(torch.nan_to_num(blk[ints], 7.0), torch.nan_to_num(cpu[ints], 7.0)), tag
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_window_features_on_integrated_duffing(cuda_device, dtype):
+    """All comprehensive features of 2,000 integrated Duffing windows (the
+    kept trajectory of a fused launch, window t in [300, 400]) against the
+    oracle; the extractor's columns equal the derivation."""
+    from pybasin_amd.window_features import window_feature_block
+
+    res = _duffing(dtype)
+    blk = window_feature_block(res.y, res.steady_idx).cpu()
+    _check_against_oracle(blk, res.y.double().cpu().numpy(), res.steady_idx, "duffing")
+    fx = StatFeatureExtractor(time_steady=300.0, features={"kurtosis": None, "number_peaks": [{"n": 3}],
+                                                          "linear_trend": [{"attr": "rvalue"}]})
+    f = fx.extract_features(Solution(res.y[0], res.t, res.y)).cpu().numpy()
+    x = res.y.double().cpu().numpy()[res.steady_idx:]
+    want = np.array([[O.comprehensive(x[:, b, s], n_, p_) for s in range(2)
+                      for n_, p_ in (("kurtosis", None), ("number_peaks", {"n": 3}),
+                                     ("linear_trend", {"attr": "rvalue"}))] for b in range(x.shape[1])])
+    np.testing.assert_allclose(f, want.astype(np.float32), rtol=2e-6, atol=1e-6)

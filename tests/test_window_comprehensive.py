@@ -1,0 +1,194 @@
+"""Comprehensive window features (the reference's comprehensive set,
+src/pybasin/ts_torch/settings.py:223-340) on the CPU.
+
+* the oracle's restatements (oracle.comprehensive) are pinned against
+  library routines that compute the same quantities another way
+  (scipy.stats skew / kurtosis with bias=False, scipy.stats.linregress,
+  numpy.correlate, scipy.signal.argrelmax) and against hand-computed
+  answers on short series a plausible slip (a strict comparison, an
+  off-by-one position, a wrong normaliser) would fail;
+* the product's CPU path (the torch restatement of the window kernel's
+  slots, then the shared derivation) equals the oracle on random, tied,
+  constant and short windows, and gives NaN for windows with a non-finite
+  value;
+* the extractor names the columns as the reference does and rejects
+  parameter values the kernel does not reduce.
+"""
+
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as O
+from pybasin_amd.features import StatFeatureExtractor, Solution
+from pybasin_amd.window_features import (COMPREHENSIVE_FEATURES, _block_cpu, check_params,
+                                         comprehensive_feature)
+
+
+def _all_params():
+    for name, grid in COMPREHENSIVE_FEATURES.items():
+        for p in (grid or [None]):
+            yield name, p
+
+
+def test_oracle_moments_and_trend_match_scipy():
+    from scipy import stats
+
+    rng = np.random.default_rng(2)
+    for n in (7, 50, 1000):
+        x = rng.normal(size=n) ** 3 + 0.3 * np.arange(n) / n
+        assert O.comprehensive(x, "skewness") == pytest.approx(stats.skew(x, bias=False), rel=1e-8)
+        assert O.comprehensive(x, "kurtosis") == pytest.approx(stats.kurtosis(x, bias=False),
+                                                               rel=1e-8)
+        lr = stats.linregress(np.arange(n), x)
+        for attr, want in (("slope", lr.slope), ("intercept", lr.intercept),
+                           ("rvalue", lr.rvalue), ("stderr", lr.stderr)):
+            assert O.comprehensive(x, "linear_trend", {"attr": attr}) == pytest.approx(
+                want, rel=1e-7, abs=1e-12), attr
+        # the reference's pseudo p-value 1 / (1 + |t|), t = r sqrt(n - 2) / sqrt(1 - r^2)
+        t = lr.rvalue * math.sqrt(n - 2) / math.sqrt(1 - lr.rvalue ** 2)
+        assert O.comprehensive(x, "linear_trend", {"attr": "pvalue"}) == pytest.approx(
+            1 / (1 + abs(t)), rel=1e-7)
+
+
+def test_oracle_autocorrelation_and_peaks_match_numpy_scipy():
+    from scipy.signal import argrelmax
+
+    rng = np.random.default_rng(3)
+    x = np.sin(np.arange(300) * 0.37) + 0.4 * rng.normal(size=300)
+    c = x - x.mean()
+    full = np.correlate(c, c, mode="full")
+    for lag in range(1, 10):
+        assert O.comprehensive(x, "autocorrelation", {"lag": lag}) == pytest.approx(
+            full[len(x) - 1 + lag] / full[len(x) - 1], rel=1e-9)
+    assert O.comprehensive(x, "autocorrelation", {"lag": 0}) == 1.0
+    assert O.comprehensive(x[:5], "autocorrelation", {"lag": 7}) == 0.0
+    for n in (1, 3, 5):
+        idx = argrelmax(x, order=n)[0]
+        idx = idx[(idx >= n) & (idx < len(x) - n)]  # the reference never marks the edges
+        assert O.comprehensive(x, "number_peaks", {"n": n}) == len(idx)
+
+
+def test_oracle_known_answers():
+    C = O.comprehensive
+    x = np.array([1.0, 3.0, 3.0, -2.0, 0.0, 3.0, -2.0, 1.0])  # n = 8, mean 7/8
+    assert C(x, "sum_values") == 7.0 and C(x, "length") == 8.0
+    assert C(x, "abs_energy") == 1 + 9 + 9 + 4 + 0 + 9 + 4 + 1
+    assert C(x, "absolute_sum_of_changes") == 2 + 0 + 5 + 2 + 3 + 5 + 3
+    assert C(x, "mean_abs_change") == pytest.approx(20 / 7)
+    assert C(x, "mean_change") == pytest.approx(0.0)
+    assert C(x, "mean_second_derivative_central") == pytest.approx((1 - (-2) - 3 + 1) / 12)
+    assert C(x, "count_above_mean") == 5.0 and C(x, "count_below_mean") == 3.0
+    assert C(x, "count_above", {"t": 0}) == 5 / 8 and C(x, "count_below", {"t": 0}) == 2 / 8
+    assert C(x, "count_value", {"value": 0}) == 1.0 and C(x, "count_value", {"value": 1}) == 2.0
+    assert C(x, "count_in_range", {"min_val": -1, "max_val": 1}) == 3.0
+    assert C(x, "has_duplicate_max") == 1.0 and C(x, "has_duplicate_min") == 1.0
+    assert C(x, "first_location_of_maximum") == 1 / 8 and C(x, "last_location_of_maximum") == 5 / 8
+    assert C(x, "first_location_of_minimum") == 3 / 8 and C(x, "last_location_of_minimum") == 6 / 8
+    # above the mean 7/8: T T T F F T F T; below: F F F T T F T F
+    assert C(x, "longest_strike_above_mean") == 3.0 and C(x, "longest_strike_below_mean") == 2.0
+    # above 0: T T T F F T F T -> 4 changes; above 1: F T T F F T F F -> 4; above -1: T T T F T T F T -> 4
+    assert C(x, "number_crossing_m", {"m": 0}) == 4.0 and C(x, "number_crossing_m", {"m": 1}) == 4.0
+    assert C(x, "number_crossing_m", {"m": -1}) == 4.0
+    # c3 lag 1: mean of x0 x1 x2 ... over 6 triples
+    trip = [x[i] * x[i + 1] * x[i + 2] for i in range(6)]
+    assert C(x, "c3", {"lag": 1}) == pytest.approx(np.mean(trip))
+    tr = [x[i + 2] ** 2 * x[i + 1] - x[i + 1] * x[i] ** 2 for i in range(6)]
+    assert C(x, "time_reversal_asymmetry_statistic", {"lag": 1}) == pytest.approx(np.mean(tr))
+    assert C(x, "c3", {"lag": 4}) == 0.0
+    assert C(x, "mean_n_absolute_max", {"number_of_maxima": 3}) == 3.0
+    assert C(x, "mean_n_absolute_max", {"number_of_maxima": 5}) == pytest.approx(13 / 5)
+    # |x| mass 15: cumulative 1 4 7 9 9 12 14 15 -> q = 0.3 (>= 4.5): index 2; q = 0.6 (>= 9): 3
+    assert C(x, "index_mass_quantile", {"q": 0.6}) == 3 / 8
+    assert C(x, "index_mass_quantile", {"q": 0.3}) == 2 / 8
+    # ramp: every change 1; constant: equal chunks
+    r = np.arange(20, dtype=float)
+    assert C(r, "cid_ce", {"normalize": False}) == pytest.approx(math.sqrt(19))
+    assert C(r, "cid_ce", {"normalize": True}) == pytest.approx(math.sqrt(19) / r.std())
+    k = np.full(30, 2.0)
+    for j in range(10):
+        assert C(k, "energy_ratio_by_chunks", {"num_segments": 10, "segment_focus": j}) == \
+            pytest.approx(0.1)
+    assert C(k[:9], "energy_ratio_by_chunks", {"num_segments": 10, "segment_focus": 0}) == 0.0
+    # ratio beyond r sigma: values 0 x 8, 10 x 2 -> mean 2, sd 4; |x - 2| = 2 (8x), 8 (2x)
+    b = np.array([0.0] * 8 + [10.0] * 2)
+    assert C(b, "ratio_beyond_r_sigma", {"r": 1}) == 0.2 and C(b, "ratio_beyond_r_sigma", {"r": 0.25}) == 1.0
+    assert C(b, "ratio_beyond_r_sigma", {"r": 0.5}) == 0.2  # |x - mean| = r sd exactly is not beyond
+    assert C(b, "has_variance_larger_than_standard_deviation") == 1.0
+    assert C(b, "large_standard_deviation", {"r": 0.35}) == 1.0
+    assert C(b, "large_standard_deviation", {"r": 0.45}) == 0.0
+    s = np.array([-1.0, 0.0, 1.0])
+    assert C(s, "symmetry_looking", {"r": 0.05}) == 1.0
+    assert C(np.array([0.0, 0.0, 3.0]), "symmetry_looking", {"r": 0.25}) == 0.0
+    assert math.isnan(C(np.array([1.0, np.inf, 2.0]), "sum_values"))
+
+
+def _windows():
+    rng = np.random.default_rng(11)
+    out = [rng.normal(size=(200, 3, 2)),
+           np.round(rng.normal(size=(120, 2, 2)) * 2) / 2,  # ties and repeated extremes
+           np.full((50, 1, 2), 0.75),  # constant
+           np.sin(np.arange(997) * 0.05)[:, None, None] * np.ones((1, 2, 1))]
+    for n in (1, 2, 3, 5, 9, 11, 12):
+        out.append(rng.normal(size=(n, 2, 1)))
+    return out
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_cpu_path_equals_oracle(dtype):
+    for w in _windows():
+        wt = torch.tensor(w, dtype=dtype)
+        x64 = wt.double().numpy()
+        N, B, S = w.shape
+        blk = _block_cpu(wt)
+        med = wt.median(0).values
+        for name, p in _all_params():
+            if name in ("skewness", "kurtosis") and N <= 3:
+                continue  # the 1e-10 guards of n - 2 and (n - 2)(n - 3) amplify rounding noise
+            got = comprehensive_feature(name, p, blk, N, med).numpy()
+            for b in range(B):
+                for s in range(S):
+                    want = O.comprehensive(x64[:, b, s], name, p)
+                    g = got[b, s]
+                    if not math.isfinite(want):
+                        # degenerate windows (a division by n - 2 = 0 and the like): both
+                        # sides non-finite (NaN and +-inf are imputed alike)
+                        assert not math.isfinite(g), (name, p, N)
+                    else:
+                        assert g == pytest.approx(want, rel=1e-9, abs=1e-12), (name, p, N, b, s)
+
+
+def test_cpu_path_nonfinite_window_is_nan():
+    w = torch.randn(40, 2, 1, dtype=torch.float64)
+    w[7, 1, 0] = float("inf")
+    blk = _block_cpu(w)
+    assert torch.isnan(blk[:, 1, 0]).all() and torch.isfinite(blk[:, 0, 0]).all()
+
+
+def test_extractor_names_and_rejects_unreduced_params():
+    torch.manual_seed(1)
+    t = torch.linspace(0, 10, 200, dtype=torch.float64)
+    y = torch.randn(200, 4, 2, dtype=torch.float64)
+    sol = Solution(torch.zeros(4, 2), t, y)
+    feats = {"kurtosis": None, "autocorrelation": [{"lag": 3}], "linear_trend": [{"attr": "stderr"}],
+             "symmetry_looking": [{"r": 0.1}], "mean": None}
+    fx = StatFeatureExtractor(time_steady=5.0, features=feats)
+    assert fx.needs_window
+    f = fx.extract_features(sol)
+    assert fx.feature_names[:5] == ["state_0__kurtosis", "state_0__autocorrelation__lag_3",
+                                    "state_0__linear_trend__attr_stderr",
+                                    "state_0__symmetry_looking__r_0.1", "state_0__mean"]
+    start = int((t >= 5.0).nonzero()[0, 0])
+    x = y[start:, 2, 1].numpy()
+    got = dict(zip(fx.feature_names, f[2].tolist()))
+    assert got["state_1__kurtosis"] == pytest.approx(O.comprehensive(x, "kurtosis"), rel=1e-5)
+    assert got["state_1__linear_trend__attr_stderr"] == pytest.approx(
+        O.comprehensive(x, "linear_trend", {"attr": "stderr"}), rel=1e-5)
+    for bad in ({"count_above": [{"t": 0.5}]}, {"autocorrelation": [{"lag": 12}]},
+                {"linear_trend": [{"attr": "bogus"}]}):
+        with pytest.raises(ValueError):
+            StatFeatureExtractor(features=bad).extract_features(sol)
+    for name, p in _all_params():
+        check_params(name, p)
