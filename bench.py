@@ -91,7 +91,7 @@ def parse():
     ap.add_argument("--keep-traj", action="store_true",
                     help="duffing: also write the full trajectories to HBM (reports write GB/s)")
     ap.add_argument("--workload", choices=["surrogate", "duffing", "pendulum_c1", "lorenz", "rossler",
-                                           "mlp_infer",
+                                           "mlp_infer", "window_features",
                                            "hdbscan", "estimator", "estimator_defaults"],
                     default="surrogate")
     ap.add_argument("--gate", type=float, default=0.999,
@@ -659,6 +659,71 @@ def mlp_infer_workload(args):
     print(json.dumps(out), flush=True)
 
 
+def window_features_workload(args):
+    """The comprehensive window features of the reference's comprehensive set
+    (bh_window_features, then the named features on the host) over the kept
+    window of the Duffing workload: n = 100,000 ICs (+ 5 templates), float32,
+    1,000 saves on [900, 1000], integrated once outside the timed region; a
+    step = the kernel's two passes over the (1000, n, 2) window plus the 128
+    named features per state.  Roofline: HBM, two reads of the window
+    (algorithmic bytes) plus the 90 float64 slots written per series."""
+    from pybasin_amd.solvers import HipDopri5Solver
+    from pybasin_amd.systems import DuffingODE
+    from pybasin_amd.window_features import (COMPREHENSIVE_FEATURES, comprehensive_feature,
+                                             window_feature_block)
+
+    dev = torch.device("cuda:0")
+    dtype = torch.float32 if args.dtype == "f32" else torch.float64
+    n = args.n
+    torch.manual_seed(args.seed)
+    lo, hi = torch.tensor([-1.0, -0.5]), torch.tensor([1.0, 1.0])
+    y0 = torch.cat([torch.rand(n, 2) * (hi - lo) + lo, torch.tensor(DUFFING_TEMPLATES)]).to(dtype).to(dev)
+    solver = HipDopri5Solver(t_span=(0, 1000), t_steps=1000, t_eval=(900.0, 1000.0), device="cuda",
+                             rtol=1e-8, atol=1e-6, cache_dir=None)
+    res = solver.integrate_fused(DuffingODE({"delta": 0.08, "k3": 1.0, "A": 0.2}), y0,
+                                 time_steady=900.0, keep_trajectories=True)
+    y = res.y
+    N, B, S = y.shape
+    plan = [(name, p) for name, grid in COMPREHENSIVE_FEATURES.items() for p in (grid or [None])
+            if name != "symmetry_looking"]
+    stream = torch.cuda.current_stream(dev)
+
+    def step():
+        blk = window_feature_block(y, 0)
+        return [comprehensive_feature(name, p, blk, N) for name, p in plan]
+
+    for _ in range(max(1, args.warmup)):
+        step()
+    torch.cuda.synchronize()
+    k0, k1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    k0.record(stream)
+    for _ in range(args.steps):
+        window_feature_block(y, 0)
+    k1.record(stream)
+    torch.cuda.synchronize()
+    kernel_ms = k0.elapsed_time(k1) / args.steps
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    ms = 1e3 * (time.perf_counter() - t0) / args.steps
+    elem = y.element_size()
+    alg_bytes = 2.0 * N * B * S * elem + 90.0 * 8 * B * S
+    out = {
+        "metric": "comprehensive window features, series/sec", "value": round(B * S / (ms * 1e-3), 1),
+        "unit": "series/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(ms, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": args.dtype, "data": "synthetic: kept Duffing windows of seeded-uniform ICs",
+        "config": {"workload": "window_features_duffing", "n_ics": B, "window_rows": N, "states": S,
+                   "features_per_state": len(plan), "kernel": "bh::win::window_features_kernel"},
+        "roofline": {"bound": "hbm", "achieved": round(alg_bytes / (kernel_ms * 1e-3) / 1e9, 1),
+                     "peak": 8000.0, "unit": "GB/s",
+                     "frac": round(alg_bytes / (kernel_ms * 1e-3) / 8e12, 4), "traffic": None,
+                     "kernel_ms": round(kernel_ms, 3), "algorithmic_bytes": alg_bytes},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def hdbscan_workload(args):
     """The reference's default predictor (HDBSCANClusterer(auto_tune=True,
     assign_noise=True), hdbscan_clusterer.py:12-148) on the GPU: Duffing
@@ -1168,7 +1233,8 @@ def main():
                  "pendulum_c1": pendulum_c1_workload,
                  "lorenz": lorenz_workload, "estimator": estimator_workload,
                  "estimator_defaults": estimator_defaults_workload, "hdbscan": hdbscan_workload,
-                 "mlp_infer": mlp_infer_workload, "rossler": rossler_workload}
+                 "mlp_infer": mlp_infer_workload, "rossler": rossler_workload,
+                 "window_features": window_features_workload}
     workloads[args.workload](args)
 
 

@@ -382,10 +382,12 @@ __global__ void __launch_bounds__(256) window_features_kernel(const T *traj, int
     double sad = 0, ssd = 0;
     int64_t c_pos = 0, c_neg = 0, c_zero = 0, c_one = 0, c_mone = 0, r_unit = 0, r_neg = 0, r_pos = 0;
     int64_t cr0 = 0, crm = 0, crp = 0;
+    // chunk energies: a running sum stored when its chunk ends (the chunk
+    // boundaries are the same for every lane)
     const int64_t seg = N / kWfChunks;
-    double chunk[kWfChunks];
-#pragma unroll
-    for (int j = 0; j < kWfChunks; j++) chunk[j] = 0;
+    double chunk_sum = 0;
+    int chunk_at = 0;
+    int64_t chunk_end = seg;
     double c3s[3] = {0, 0, 0}, trs[3] = {0, 0, 0};
     double hist[6];  // x_{i-1} .. x_{i-6}
 #pragma unroll
@@ -398,8 +400,18 @@ __global__ void __launch_bounds__(256) window_features_kernel(const T *traj, int
     for (int j = 0; j < 11; j++) w[j] = T(0);
     int64_t pk1 = 0, pk3 = 0, pk5 = 0;
     double prev = x0;
+    // four samples in flight: the load of sample i + 4 is issued while
+    // sample i is reduced (indices clamped to the window, so every load is
+    // in bounds)
+    const int64_t last = N - 1;
+    T q0 = col[0], q1 = col[min((int64_t)1, last) * np], q2 = col[min((int64_t)2, last) * np],
+      q3 = col[min((int64_t)3, last) * np];
     for (int64_t i = 0; i < N; i++) {
-        const T xt = col[i * np];
+        const T xt = q0;
+        q0 = q1;
+        q1 = q2;
+        q2 = q3;
+        q3 = col[min(i + 4, last) * np];
         const double x = (double)xt;
         fin = fin && isfinite(x);
         s1 += x;
@@ -428,11 +440,14 @@ __global__ void __launch_bounds__(256) window_features_kernel(const T *traj, int
         r_unit += (x >= -1.0) && (x <= 1.0);
         r_neg += (x >= -1e12) && (x <= 0.0);
         r_pos += (x >= 0.0) && (x <= 1e12);
-        if (seg > 0) {
-            const int64_t cj = i / seg;
-#pragma unroll
-            for (int j = 0; j < kWfChunks; j++)
-                if (cj == j) chunk[j] = fma(x, x, chunk[j]);
+        if (chunk_at < kWfChunks) {
+            chunk_sum = fma(x, x, chunk_sum);
+            if (i + 1 == chunk_end) {
+                put(BH_WF_CHUNK + chunk_at, chunk_sum);
+                chunk_sum = 0;
+                chunk_at++;
+                chunk_end += seg;
+            }
         }
         // lag products ending at i: x_{i-2l} x_{i-l} x_i and x_i^2 x_{i-l} - x_{i-l} x_{i-2l}^2
 #pragma unroll
@@ -446,13 +461,16 @@ __global__ void __launch_bounds__(256) window_features_kernel(const T *traj, int
 #pragma unroll
         for (int j = 5; j > 0; j--) hist[j] = hist[j - 1];
         hist[0] = x;
-        // the seven largest |x| in descending order
+        // the seven largest |x| in descending order (most samples fall below
+        // the seventh and skip the insertion)
         double v = fabs(x);
+        if (v > top[kWfTop - 1]) {
 #pragma unroll
-        for (int j = 0; j < kWfTop; j++) {
-            const double hi = fmax(top[j], v), lo = fmin(top[j], v);
-            top[j] = hi;
-            v = lo;
+            for (int j = 0; j < kWfTop; j++) {
+                const double hi = fmax(top[j], v), lo = fmin(top[j], v);
+                top[j] = hi;
+                v = lo;
+            }
         }
 #pragma unroll
         for (int j = 10; j > 0; j--) w[j] = w[j - 1];
@@ -493,8 +511,8 @@ __global__ void __launch_bounds__(256) window_features_kernel(const T *traj, int
     put(BH_WF_CROSS_0, (double)cr0);
     put(BH_WF_CROSS_M1, (double)crm);
     put(BH_WF_CROSS_P1, (double)crp);
-#pragma unroll
-    for (int j = 0; j < kWfChunks; j++) put(BH_WF_CHUNK + j, chunk[j]);
+    // windows shorter than ten rows have empty chunks
+    for (int j = chunk_at; j < kWfChunks; j++) put(BH_WF_CHUNK + j, 0.0);
 #pragma unroll
     for (int l = 0; l < 3; l++) {
         put(BH_WF_C3 + l, c3s[l]);
@@ -524,8 +542,16 @@ __global__ void __launch_bounds__(256) window_features_kernel(const T *traj, int
 #pragma unroll
     for (int j = 0; j < 8; j++) mass_idx[j] = -1;
     double cum = 0;
+    q0 = col[0];
+    q1 = col[min((int64_t)1, last) * np];
+    q2 = col[min((int64_t)2, last) * np];
+    q3 = col[min((int64_t)3, last) * np];
     for (int64_t i = 0; i < N; i++) {
-        const double x = (double)col[i * np];
+        const double x = (double)q0;
+        q0 = q1;
+        q1 = q2;
+        q2 = q3;
+        q3 = col[min(i + 4, last) * np];
         const double d = x - mu;
         const double d2 = d * d;
         c2 += d2;
