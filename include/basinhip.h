@@ -406,9 +406,12 @@ int32_t bh_window_spectral_ratio(int32_t dtype, const void *traj, int64_t n_save
    time_reversal_asymmetry_statistic (lags 1..3), energy_ratio_by_chunks (10
    segments) (torch_features_advanced.py:84-123), ratio_beyond_r_sigma and
    mean_n_absolute_max (n <= 7) (torch_features_statistical.py:165-181),
-   linear_trend (slope, intercept, rvalue, pvalue, stderr)
-   (torch_features_trend.py:11-51), symmetry_looking (with the median of
-   bh_window_order_stats). */
+   linear_trend and linear_trend_timewise (slope, intercept, rvalue, pvalue,
+   stderr) (torch_features_trend.py:11-58), partial_autocorrelation (lags
+   0..9, Durbin-Levinson over the autocovariances)
+   (torch_features_autocorrelation.py:31-69), binned_entropy (max_bins 10)
+   (torch_features_entropy_complexity.py:58-94), symmetry_looking (with the
+   median of bh_window_order_stats). */
 enum {
     BH_WF_N = 0,             /* window length N'                                   */
     BH_WF_SUM = 1,           /* sum x                                              */
@@ -459,7 +462,10 @@ enum {
     BH_WF_AUTOCOV = 73,      /* 9 slots: sum (x[i] - mean)(x[i+l] - mean), l = 1..9 */
     BH_WF_MASS = 82,         /* 8 slots: first i with sum_{j<=i} |x[j]| >= q sum |x|,
                                 q = 0.1, 0.2, 0.3, 0.4, 0.6, 0.7, 0.8, 0.9          */
-    BH_WF_SLOTS = 90
+    BH_WF_BINS = 90,         /* 10 slots: binned_entropy counts, bin = trunc((x - min)
+                                / (range + 1e-10) * (10 - 1e-6)) in the series dtype,
+                                range 0 taken as 1                                  */
+    BH_WF_SLOTS = 100
 };
 int32_t bh_window_features(int32_t dtype, const void *traj, int64_t n_save, int64_t n,
                            int32_t dim, int64_t start, double *out, void *stream);

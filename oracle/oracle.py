@@ -303,11 +303,13 @@ def basin_stability(labels: np.ndarray) -> dict:
     return out
 
 
-def comprehensive(x: np.ndarray, name: str, params: dict | None = None) -> float:
+def comprehensive(x: np.ndarray, name: str, params: dict | None = None, dtype=np.float64) -> float:
     """One calculator of the reference's comprehensive set on one series x
     (N,), float64, written from the reference's torch calculators
     (src/pybasin/ts_torch/calculators/): NaN if x holds a non-finite value
-    (the convention of the window kernel this checks)."""
+    (the convention of the window kernel this checks).  dtype is the series'
+    own dtype, in which binned_entropy forms its bins as the reference does."""
+    xs = np.asarray(x).astype(dtype)
     x = np.asarray(x, dtype=np.float64)
     p = params or {}
     n = x.size
@@ -424,7 +426,36 @@ def comprehensive(x: np.ndarray, name: str, params: dict | None = None) -> float
     if name == "mean_n_absolute_max":  # torch_features_statistical.py:165-172
         k = min(int(p["number_of_maxima"]), n)
         return float(np.sort(np.abs(x))[::-1][:k].mean())
-    if name == "linear_trend":  # torch_features_trend.py:11-51
+    if name == "partial_autocorrelation":
+        # Durbin-Levinson over the sample autocorrelations with the
+        # reference's 1e-10 guard (torch_features_autocorrelation.py:31-69)
+        lag = int(p["lag"])
+        if lag == 0:
+            return 1.0
+        if lag > n - 1:
+            return 0.0
+        c = x - mu
+        v = (c ** 2).sum()
+        r = [1.0] + [(c[:-l] * c[l:]).sum() / (v + 1e-10) for l in range(1, lag + 1)]
+        phi = {1: r[1]}
+        for k in range(2, lag + 1):
+            num = r[k] - sum(phi[j] * r[k - j] for j in range(1, k))
+            den = 1.0 - sum(phi[j] * r[j] for j in range(1, k))
+            pk = num / (den + 1e-10)
+            phi = {j: phi[j] - pk * phi[k - j] for j in range(1, k)}
+            phi[k] = pk
+        return float(phi[lag])
+    if name == "binned_entropy":  # torch_features_entropy_complexity.py:58-94
+        mb = int(p.get("max_bins", 10))
+        lo, hi = xs.min(), xs.max()
+        rng = hi - lo
+        rng = dtype(1) if rng == 0 else rng
+        xn = (xs - lo) / (rng + dtype(1e-10)) * dtype(mb - 1e-6)
+        idx = np.clip(xn.astype(np.int64), 0, mb - 1)
+        counts = np.bincount(idx, minlength=mb)
+        pr = counts[counts > 0] / n
+        return 0.0 if hi == lo else float(-(pr * np.log(pr)).sum())
+    if name in ("linear_trend", "linear_trend_timewise"):  # torch_features_trend.py:11-58
         t = np.arange(n, dtype=np.float64)
         ss_tt = ((t - t.mean()) ** 2).sum()
         ss_tx = ((t - t.mean()) * (x - mu)).sum()

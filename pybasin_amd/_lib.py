@@ -32,8 +32,8 @@ WF = dict(N=0, SUM=1, SUMSQ=2, MIN=3, MAX=4, FIRST_MAX=5, LAST_MAX=6, FIRST_MIN=
           CHUNK=26, C3=36, TREV=39, SUM_ABS=42, TOP_ABS=43, PEAKS_1=50, PEAKS_3=51, PEAKS_5=52,
           CM2=53, CM3=54, CM4=55, SS_TX=56, COUNT_ABOVE_MEAN=57, COUNT_BELOW_MEAN=58,
           STRIKE_ABOVE=59, STRIKE_BELOW=60, COUNT_MAX=61, COUNT_MIN=62, BEYOND=63, AUTOCOV=73,
-          MASS=82)
-BH_WF_SLOTS = 90
+          MASS=82, BINS=90)
+BH_WF_SLOTS = 100
 BH_NOALLOC_SCRATCH_BYTES = 64 << 20
 
 # every symbol include/basinhip.h declares

@@ -341,13 +341,15 @@ __global__ void __launch_bounds__(kSeriesBlock)
 // standard deviation of the first (from sums shifted by the first value),
 // the central moments, the counts and longest runs about the mean, the
 // counts beyond r standard deviations, the autocovariances at lags 1..9,
-// the duplicate-extreme counts and the index-mass quantiles.  The slots are
+// the duplicate-extreme counts, the index-mass quantiles and the ten bins of
+// binned_entropy.  The slots are
 // the BH_WF_* indices of include/basinhip.h; a window holding a non-finite
 // value gives NaN in every slot.
 
 constexpr int kWfTop = 7;     // largest |x| kept (mean_n_absolute_max up to 7)
 constexpr int kWfLags = 9;    // autocovariance lags 1..9
 constexpr int kWfChunks = 10; // energy_ratio_by_chunks segments
+constexpr int kWfBins = 10;   // binned_entropy bins (max_bins = 10)
 __device__ constexpr double kWfSigma[10] = {0.5, 1.0, 1.5, 2.0, 2.5, 3.0, 5.0, 6.0, 7.0, 10.0};
 __device__ constexpr double kWfMass[8] = {0.1, 0.2, 0.3, 0.4, 0.6, 0.7, 0.8, 0.9};
 
@@ -542,12 +544,22 @@ __global__ void __launch_bounds__(256) window_features_kernel(const T *traj, int
 #pragma unroll
     for (int j = 0; j < 8; j++) mass_idx[j] = -1;
     double cum = 0;
+    // binned_entropy's bins, formed in the series dtype as the reference
+    // does: (x - min) / (range + 1e-10) * (10 - 1e-6) truncated, range 0 -> 1
+    const T tmn = (T)mn;
+    T trng = (T)mx - tmn;
+    trng = trng == T(0) ? T(1) : trng;
+    const T tden = trng + T(1e-10), tscale = T(10.0 - 1e-6);
+    int64_t bins[kWfBins];
+#pragma unroll
+    for (int j = 0; j < kWfBins; j++) bins[j] = 0;
     q0 = col[0];
     q1 = col[min((int64_t)1, last) * np];
     q2 = col[min((int64_t)2, last) * np];
     q3 = col[min((int64_t)3, last) * np];
     for (int64_t i = 0; i < N; i++) {
-        const double x = (double)q0;
+        const T xt = q0;
+        const double x = (double)xt;
         q0 = q1;
         q1 = q2;
         q2 = q3;
@@ -579,7 +591,13 @@ __global__ void __launch_bounds__(256) window_features_kernel(const T *traj, int
 #pragma unroll
         for (int j = 0; j < 8; j++)
             if (mass_idx[j] < 0 && cum >= kWfMass[j] * sabs) mass_idx[j] = i;
+        int bin = (int)((xt - tmn) / tden * tscale);
+        bin = bin < 0 ? 0 : (bin > kWfBins - 1 ? kWfBins - 1 : bin);
+#pragma unroll
+        for (int j = 0; j < kWfBins; j++) bins[j] += bin == j;
     }
+#pragma unroll
+    for (int j = 0; j < kWfBins; j++) put(BH_WF_BINS + j, (double)bins[j]);
     put(BH_WF_CM2, c2);
     put(BH_WF_CM3, c3);
     put(BH_WF_CM4, c4);

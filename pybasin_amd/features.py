@@ -340,6 +340,7 @@ class StatFeatureExtractor(FeatureExtractor):
                  start: int) -> torch.Tensor:
         cols = []
         blk = med = None
+        memo: dict = {}
         for s, name, params in plan:
             if is_comprehensive(name, params):
                 if y is None:
@@ -349,8 +350,8 @@ class StatFeatureExtractor(FeatureExtractor):
                     blk = window_feature_block(y, start)
                 if name == "symmetry_looking" and med is None:
                     med = window_feature("median", None, y, start)
-                cols.append(comprehensive_feature(name, params, blk, y.shape[0] - start, med)[:, s]
-                            .to(stats.device, torch.float64))
+                cols.append(comprehensive_feature(name, params, blk, y.shape[0] - start, med,
+                                                  memo)[:, s].to(stats.device, torch.float64))
             elif name in WINDOW_FEATURES:
                 if y is None:
                     raise ValueError(f"{name} needs the saved window: integrate with "

@@ -73,11 +73,16 @@ COMPREHENSIVE_FEATURES: dict[str, list[dict] | None] = {
     "energy_ratio_by_chunks": [{"num_segments": 10, "segment_focus": i} for i in range(10)],
     "mean_n_absolute_max": [{"number_of_maxima": n} for n in [3, 5, 7]],
     "linear_trend": [{"attr": a} for a in ["pvalue", "rvalue", "intercept", "slope", "stderr"]],
+    "linear_trend_timewise": [{"attr": a} for a in ["pvalue", "rvalue", "intercept", "slope",
+                                                    "stderr"]],
+    "partial_autocorrelation": [{"lag": lag} for lag in range(10)],
+    "binned_entropy": [{"max_bins": 10}],
     "symmetry_looking": [{"r": r * 0.05} for r in range(20)],
 }
 # calculators whose parameter may take any value (formed from the slots by
 # a comparison with the parameter)
-_FREE_PARAMS = {"large_standard_deviation", "symmetry_looking", "linear_trend"}
+_FREE_PARAMS = {"large_standard_deviation", "symmetry_looking", "linear_trend",
+                "linear_trend_timewise"}
 _SIGMAS = [0.5, 1.0, 1.5, 2.0, 2.5, 3.0, 5.0, 6.0, 7.0, 10.0]
 _MASS_QS = [0.1, 0.2, 0.3, 0.4, 0.6, 0.7, 0.8, 0.9]
 
@@ -92,13 +97,13 @@ def check_params(name: str, params: dict | None) -> None:
     if not params:
         raise ValueError(f"{name} needs parameters, one of {grid}")
     if name in _FREE_PARAMS:
-        if name == "linear_trend" and params.get("attr") not in (
+        if name.startswith("linear_trend") and params.get("attr") not in (
                 "pvalue", "rvalue", "intercept", "slope", "stderr"):
             raise ValueError(f"linear_trend: unknown attr {params.get('attr')!r}")
         return
     if name == "mean_n_absolute_max" and 1 <= int(params.get("number_of_maxima", 0)) <= 7:
         return
-    if name == "autocorrelation" and 0 <= int(params.get("lag", -1)) <= 9:
+    if name in ("autocorrelation", "partial_autocorrelation") and 0 <= int(params.get("lag", -1)) <= 9:
         return
     if name == "energy_ratio_by_chunks" and int(params.get("num_segments", 0)) == 10 and \
             0 <= int(params.get("segment_focus", -1)) < 10:
@@ -201,6 +206,13 @@ def _block_cpu(w: torch.Tensor) -> torch.Tensor:
         hit = cum >= q * slots["SUM_ABS"]
         slots[("MASS", j)] = torch.where(hit.any(0), hit.double().argmax(0).double(),
                                          torch.full((B, S), float(N - 1), dtype=torch.float64))
+    # binned_entropy bins in the series dtype (torch_features_entropy_complexity.py:58-94)
+    wmn, wmx = w.min(0).values, w.max(0).values
+    rng = wmx - wmn
+    rng = torch.where(rng == 0, torch.ones_like(rng), rng)
+    bins = ((w - wmn) / (rng + 1e-10) * (10 - 1e-6)).long().clamp(0, 9)
+    for j in range(10):
+        slots[("BINS", j)] = (bins == j).double().sum(0)
     for key, v in slots.items():
         idx = WF[key] if isinstance(key, str) else WF[key[0]] + key[1]
         out[idx] = v
@@ -227,12 +239,36 @@ def window_feature_block(y: torch.Tensor, start: int) -> torch.Tensor:
     return out
 
 
+def _pacf_all(blk: torch.Tensor, n: float) -> list:
+    """Partial autocorrelations at lags 1..min(9, n - 1) by one Durbin-Levinson
+    recursion (torch_features_autocorrelation.py:31-69): entry k is the
+    coefficient phi_k of step k."""
+    g = lambda key, j=0: blk[WF[key] + j]  # noqa: E731
+    maxlag = min(9, int(n) - 1)
+    out = [None]
+    if maxlag < 1:
+        return out
+    acf = [None] + [g("AUTOCOV", l - 1) / (g("CM2") + 1e-10) for l in range(1, maxlag + 1)]
+    phi = [None, acf[1]]
+    out.append(acf[1])
+    for k in range(2, maxlag + 1):
+        num = acf[k] - sum(phi[j] * acf[k - j] for j in range(1, k))
+        den = 1.0 - sum(phi[j] * acf[j] for j in range(1, k))
+        pk = num / (den + 1e-10)
+        phi = [None] + [phi[j] - pk * phi[k - j] for j in range(1, k)] + [pk]
+        out.append(pk)
+    return out
+
+
 def comprehensive_feature(name: str, params: dict | None, blk: torch.Tensor, n_win: int,
-                          median: torch.Tensor | None = None) -> torch.Tensor:
+                          median: torch.Tensor | None = None,
+                          cache: dict | None = None) -> torch.Tensor:
     """Feature `name` (B, S) float64 from the slots of windows of n_win rows
     (the reference calculator it restates is cited beside each).
     symmetry_looking needs the window median.  NaN where the window holds a
-    non-finite value (its N slot is NaN)."""
+    non-finite value (its N slot is NaN).  cache: a dict kept across the
+    calls on one block (the partial autocorrelations of all lags come from
+    one recursion)."""
     p = params or {}
     g = lambda key, j=0: blk[WF[key] + j]  # noqa: E731
     N = g("N")
@@ -326,7 +362,28 @@ def comprehensive_feature(name: str, params: dict | None, blk: torch.Tensor, n_w
     if name == "mean_n_absolute_max":  # statistical.py:165-172
         k = min(int(p["number_of_maxima"]), int(n))
         return sum(g("TOP_ABS", j) for j in range(k)) / k
-    if name == "linear_trend":  # trend.py:11-51
+    if name == "partial_autocorrelation":  # autocorrelation.py:31-69 (Durbin-Levinson)
+        lag = int(p["lag"])
+        if lag == 0:
+            return torch.ones_like(N).where(~N.isnan(), N)
+        if lag > int(n) - 1:
+            return torch.zeros_like(N).where(~N.isnan(), N)
+        # the coefficient phi_lag of step lag is the same whether the
+        # recursion stops there (the reference) or runs on to lag 9
+        pacf = cache.get("pacf") if cache is not None else None
+        if pacf is None:
+            pacf = _pacf_all(blk, n)
+            if cache is not None:
+                cache["pacf"] = pacf
+        return pacf[lag]
+    if name == "binned_entropy":  # entropy_complexity.py:58-94
+        ent = torch.zeros_like(N)
+        for j in range(10):
+            c = g("BINS", j)
+            pr = c / N
+            ent = ent - torch.where(c > 0, pr * torch.log(pr.clamp(min=1e-10)), torch.zeros_like(pr))
+        return torch.where(g("MAX") == g("MIN"), torch.zeros_like(ent), ent).where(~N.isnan(), N)
+    if name in ("linear_trend", "linear_trend_timewise"):  # trend.py:11-58
         tbar = 0.5 * (n - 1)
         ss_tt = n * (n * n - 1) / 12.0
         ss_tx = g("SS_TX")

@@ -138,13 +138,13 @@ def _synthetic_windows():
     yield "nonfinite", bad
 
 
-def _check_against_oracle(blk, w64, start, tag):
+def _check_against_oracle(blk, w64, start, tag, series_dtype=np.float64):
     from pybasin_amd.window_features import COMPREHENSIVE_FEATURES, comprehensive_feature
 
     x = w64[start:]
     N, B, S = x.shape
     med = torch.tensor(np.sort(x, axis=0)[(N - 1) // 2])
-    bad = 0
+    bad = []
     for name, grid in COMPREHENSIVE_FEATURES.items():
         for p in (grid or [None]):
             if name in ("skewness", "kurtosis") and N <= 3:
@@ -152,20 +152,25 @@ def _check_against_oracle(blk, w64, start, tag):
             got = comprehensive_feature(name, p, blk, N, med).numpy()
             for b in range(B):
                 for s in range(S):
-                    want = O.comprehensive(x[:, b, s], name, p)
+                    want = O.comprehensive(x[:, b, s], name, p, series_dtype)
                     g = float(got[b, s])
                     # pvalue and stderr of a perfect linear fit (1 - r^2 and the
                     # residuals at rounding level) are rounding noise of size ~1e-8
-                    tol = 1e-6 if name == "linear_trend" and p["attr"] in ("pvalue", "stderr") \
+                    tol = 1e-6 if name.startswith("linear_trend") and p["attr"] in ("pvalue", "stderr") \
                         else 1e-11
+                    if name == "partial_autocorrelation":
+                        # the recursion divides by 1 - sum phi_j r_j, small for nearly
+                        # periodic windows: summation-order differences of the
+                        # autocorrelations (1e-16) grow to ~1e-8 there
+                        tol = 1e-6
                     if not math.isfinite(want):
                         ok = not math.isfinite(g)
                     else:
                         ok = g == pytest.approx(want, rel=1e-9, abs=tol)
                     if not ok:
-                        bad += 1
-                        assert bad < 3, (tag, name, p, b, s, g, want)
-    assert bad == 0
+                        bad.append((tag, name, p, b, s, g, want))
+                        assert len(bad) < 3, bad
+    assert not bad, bad
 
 
 @pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
@@ -176,7 +181,8 @@ def test_window_features_kernel_equals_oracle_synthetic(cuda_device, dtype):
         wt = torch.tensor(w, dtype=dtype)
         for start in (0, 1) if w.shape[0] > 1 else (0,):
             blk = window_feature_block(wt.cuda(), start).cpu()
-            _check_against_oracle(blk, wt.double().numpy(), start, f"{tag}/{start}")
+            _check_against_oracle(blk, wt.double().numpy(), start, f"{tag}/{start}",
+                                  np.float32 if dtype == torch.float32 else np.float64)
             # the slots of the CPU path (same definitions, another summation
             # order): integer-valued slots equal
             from pybasin_amd.window_features import window_feature_block as wfb
@@ -196,7 +202,8 @@ def test_window_features_on_integrated_duffing(cuda_device, dtype):
 
     res = _duffing(dtype)
     blk = window_feature_block(res.y, res.steady_idx).cpu()
-    _check_against_oracle(blk, res.y.double().cpu().numpy(), res.steady_idx, "duffing")
+    _check_against_oracle(blk, res.y.double().cpu().numpy(), res.steady_idx, "duffing",
+                          np.float32 if dtype == torch.float32 else np.float64)
     fx = StatFeatureExtractor(time_steady=300.0, features={"kurtosis": None, "number_peaks": [{"n": 3}],
                                                           "linear_trend": [{"attr": "rvalue"}]})
     f = fx.extract_features(Solution(res.y[0], res.t, res.y)).cpu().numpy()

@@ -71,6 +71,42 @@ def test_oracle_autocorrelation_and_peaks_match_numpy_scipy():
         assert O.comprehensive(x, "number_peaks", {"n": n}) == len(idx)
 
 
+def test_oracle_partial_autocorrelation_and_binned_entropy():
+    rng = np.random.default_rng(4)
+    # AR(1) with phi = 0.6: partial autocorrelation 0.6 at lag 1, 0 beyond
+    # (within sampling error 3 / sqrt(n))
+    e = rng.normal(size=20000)
+    x = np.zeros_like(e)
+    for i in range(1, len(e)):
+        x[i] = 0.6 * x[i - 1] + e[i]
+    assert O.comprehensive(x, "partial_autocorrelation", {"lag": 1}) == pytest.approx(0.6, abs=0.03)
+    for lag in range(2, 10):
+        assert abs(O.comprehensive(x, "partial_autocorrelation", {"lag": lag})) < 0.03
+    assert O.comprehensive(x, "partial_autocorrelation", {"lag": 1}) == pytest.approx(
+        O.comprehensive(x, "autocorrelation", {"lag": 1}), rel=1e-12)
+    # on well-conditioned series the recursion equals the last coefficient of
+    # the order-k Yule-Walker system solved directly
+    for seed in range(3):
+        y = np.random.default_rng(seed).normal(size=400).cumsum() * 0.1 + rng.normal(size=400)
+        c = y - y.mean()
+        r = np.array([1.0] + [(c[:-l] * c[l:]).sum() / (c @ c) for l in range(1, 10)])
+        for k in range(1, 10):
+            R = np.array([[r[abs(i - j)] for j in range(k)] for i in range(k)])
+            assert np.linalg.cond(R) < 1e4
+            assert O.comprehensive(y, "partial_autocorrelation", {"lag": k}) == pytest.approx(
+                np.linalg.solve(R, r[1:k + 1])[-1], rel=1e-7, abs=1e-10)
+    assert O.comprehensive(x[:4], "partial_autocorrelation", {"lag": 5}) == 0.0
+    # entropies: ten equally filled bins -> ln 10; two -> ln 2; constant -> 0
+    assert O.comprehensive(np.arange(100.0), "binned_entropy", {"max_bins": 10}) == pytest.approx(
+        math.log(10))
+    assert O.comprehensive(np.array([0.0, 1.0] * 50), "binned_entropy", {"max_bins": 10}) == \
+        pytest.approx(math.log(2))
+    assert O.comprehensive(np.full(9, 3.0), "binned_entropy", {"max_bins": 10}) == 0.0
+    # the bin of the maximum is the last one, (x - min) / range * (10 - 1e-6) < 10
+    assert O.comprehensive(np.array([0.0, 1.0]), "binned_entropy", {"max_bins": 10}) == \
+        pytest.approx(math.log(2))
+
+
 def test_oracle_known_answers():
     C = O.comprehensive
     x = np.array([1.0, 3.0, 3.0, -2.0, 0.0, 3.0, -2.0, 1.0])  # n = 8, mean 7/8
@@ -150,7 +186,8 @@ def test_cpu_path_equals_oracle(dtype):
             got = comprehensive_feature(name, p, blk, N, med).numpy()
             for b in range(B):
                 for s in range(S):
-                    want = O.comprehensive(x64[:, b, s], name, p)
+                    want = O.comprehensive(x64[:, b, s], name, p,
+                                           np.float32 if dtype == torch.float32 else np.float64)
                     g = got[b, s]
                     if not math.isfinite(want):
                         # degenerate windows (a division by n - 2 = 0 and the like): both
