@@ -91,7 +91,7 @@ def parse():
     ap.add_argument("--keep-traj", action="store_true",
                     help="duffing: also write the full trajectories to HBM (reports write GB/s)")
     ap.add_argument("--workload", choices=["surrogate", "duffing", "pendulum_c1", "lorenz", "rossler",
-                                           "mlp_infer", "window_features",
+                                           "mlp_infer", "window_features", "window_spectrum",
                                            "hdbscan", "estimator", "estimator_defaults"],
                     default="surrogate")
     ap.add_argument("--gate", type=float, default=0.999,
@@ -659,6 +659,79 @@ def mlp_infer_workload(args):
     print(json.dumps(out), flush=True)
 
 
+def _duffing_window(args, dev):
+    """The kept window of the Duffing workload: n = args.n seeded-uniform ICs
+    (+ 5 templates), 1,000 saves on [900, 1000], integrated once (not timed)."""
+    from pybasin_amd.solvers import HipDopri5Solver
+    from pybasin_amd.systems import DuffingODE
+
+    dtype = torch.float32 if args.dtype == "f32" else torch.float64
+    torch.manual_seed(args.seed)
+    lo, hi = torch.tensor([-1.0, -0.5]), torch.tensor([1.0, 1.0])
+    y0 = torch.cat([torch.rand(args.n, 2) * (hi - lo) + lo,
+                    torch.tensor(DUFFING_TEMPLATES)]).to(dtype).to(dev)
+    solver = HipDopri5Solver(t_span=(0, 1000), t_steps=1000, t_eval=(900.0, 1000.0), device="cuda",
+                             rtol=1e-8, atol=1e-6, cache_dir=None)
+    res = solver.integrate_fused(DuffingODE({"delta": 0.08, "k3": 1.0, "A": 0.2}), y0,
+                                 time_steady=900.0, keep_trajectories=True)
+    return res.y
+
+
+def window_spectrum_workload(args):
+    """The spectrum calculators of the reference's comprehensive set
+    (fft_coefficient k = 0..99 x real / imag / abs / angle, fft_aggregated,
+    spkt_welch_density, fourier_entropy: 412 features per state) over the kept
+    Duffing window (n = 100,000 ICs + 5 templates, float32, 1,000 rows); a step
+    = bh_window_spectrum (bins 0..99 and the aggregates of all 501 bins) plus
+    the named features on the device.  Roofline: the kernel is bound by its
+    double-precision vector issue: 3 flops (one FMA, one subtraction) per bin
+    and sample, all 501 bins of each series (the aggregates need them), over
+    the 78.6 TFLOP/s FP64 vector peak."""
+    from pybasin_amd.window_features import SPECTRUM_FEATURES, spectrum_feature, window_spectrum
+
+    dev = torch.device("cuda:0")
+    y = _duffing_window(args, dev)
+    N, B, S = y.shape
+    plan = [(name, p) for name, grid in SPECTRUM_FEATURES.items() for p in grid]
+    stream = torch.cuda.current_stream(dev)
+
+    def step():
+        c, a = window_spectrum(y, 0, 100, True)
+        return [spectrum_feature(name, p, c, a, N) for name, p in plan]
+
+    for _ in range(max(1, args.warmup)):
+        step()
+    torch.cuda.synchronize()
+    k0, k1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    k0.record(stream)
+    for _ in range(args.steps):
+        window_spectrum(y, 0, 100, True)
+    k1.record(stream)
+    torch.cuda.synchronize()
+    kernel_ms = k0.elapsed_time(k1) / args.steps
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    ms = 1e3 * (time.perf_counter() - t0) / args.steps
+    nb = N // 2 + 1
+    flops = 3.0 * nb * N * B * S
+    out = {
+        "metric": "window spectrum features, series/sec", "value": round(B * S / (ms * 1e-3), 1),
+        "unit": "series/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(ms, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic: kept Duffing windows of seeded-uniform ICs",
+        "config": {"workload": "window_spectrum_duffing", "n_ics": B, "window_rows": N, "states": S,
+                   "series_dtype": args.dtype, "features_per_state": len(plan),
+                   "kernel": "bh::win::spectrum_kernel"},
+        "roofline": {"bound": "fp64-valu", "achieved": round(flops / (kernel_ms * 1e-3) / 1e12, 2),
+                     "peak": 78.6, "unit": "TFLOP/s",
+                     "frac": round(flops / (kernel_ms * 1e-3) / 78.6e12, 4), "traffic": None,
+                     "kernel_ms": round(kernel_ms, 3), "algorithmic_flops": flops},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def window_features_workload(args):
     """The comprehensive window features of the reference's comprehensive set
     (bh_window_features, then the named features on the host) over the kept
@@ -667,25 +740,14 @@ def window_features_workload(args):
     step = the kernel's two passes over the (1000, n, 2) window plus the 128
     named features per state.  Roofline: HBM, two reads of the window
     (algorithmic bytes) plus the 90 float64 slots written per series."""
-    from pybasin_amd.solvers import HipDopri5Solver
-    from pybasin_amd.systems import DuffingODE
-    from pybasin_amd.window_features import (COMPREHENSIVE_FEATURES, comprehensive_feature,
-                                             window_feature_block)
+    from pybasin_amd.window_features import (COMPREHENSIVE_FEATURES, SPECTRUM_FEATURES,
+                                             comprehensive_feature, window_feature_block)
 
     dev = torch.device("cuda:0")
-    dtype = torch.float32 if args.dtype == "f32" else torch.float64
-    n = args.n
-    torch.manual_seed(args.seed)
-    lo, hi = torch.tensor([-1.0, -0.5]), torch.tensor([1.0, 1.0])
-    y0 = torch.cat([torch.rand(n, 2) * (hi - lo) + lo, torch.tensor(DUFFING_TEMPLATES)]).to(dtype).to(dev)
-    solver = HipDopri5Solver(t_span=(0, 1000), t_steps=1000, t_eval=(900.0, 1000.0), device="cuda",
-                             rtol=1e-8, atol=1e-6, cache_dir=None)
-    res = solver.integrate_fused(DuffingODE({"delta": 0.08, "k3": 1.0, "A": 0.2}), y0,
-                                 time_steady=900.0, keep_trajectories=True)
-    y = res.y
+    y = _duffing_window(args, dev)
     N, B, S = y.shape
     plan = [(name, p) for name, grid in COMPREHENSIVE_FEATURES.items() for p in (grid or [None])
-            if name != "symmetry_looking"]
+            if name != "symmetry_looking" and name not in SPECTRUM_FEATURES]
     stream = torch.cuda.current_stream(dev)
 
     def step():
@@ -1237,7 +1299,8 @@ def main():
                  "lorenz": lorenz_workload, "estimator": estimator_workload,
                  "estimator_defaults": estimator_defaults_workload, "hdbscan": hdbscan_workload,
                  "mlp_infer": mlp_infer_workload, "rossler": rossler_workload,
-                 "window_features": window_features_workload}
+                 "window_features": window_features_workload,
+                 "window_spectrum": window_spectrum_workload}
     workloads[args.workload](args)
 
 

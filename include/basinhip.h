@@ -470,6 +470,23 @@ enum {
 int32_t bh_window_features(int32_t dtype, const void *traj, int64_t n_save, int64_t n,
                            int32_t dim, int64_t start, double *out, void *stream);
 
+/* The rfft of each series of the same window (torch.fft.rfft of the raw
+   window, as the reference's fft_coefficient, spkt_welch_density and
+   fft_aggregated take it; src/pybasin/ts_torch/calculators/
+   torch_features_frequency.py:11-70), in double: out_coeff (n_coeff, 2, n,
+   dim) float64 holds the real and imaginary parts of bins 0 .. n_coeff - 1
+   (0 past the last bin w / 2); out_agg (5, n, dim) float64, or NULL, holds the
+   centroid, variance, skew and kurtosis of the normalised power spectrum
+   p_b = P_b / (sum P + 1e-10) over all w / 2 + 1 bins (skew over
+   var^1.5 + 1e-10, kurtosis over var^2 + 1e-10) and its entropy
+   -sum c log c, c = max(p_b, 1e-10) (fourier_entropy,
+   torch_features_entropy_complexity.py:97-107).  0 <= n_coeff <= 4096; windows whose rows and
+   (with out_agg) four power rows fit 62 KB of LDS (7,000 float rows with
+   out_agg). */
+int32_t bh_window_spectrum(int32_t dtype, const void *traj, int64_t n_save, int64_t n,
+                           int32_t dim, int64_t start, int32_t n_coeff, double *out_coeff,
+                           double *out_agg, void *stream);
+
 /* ---- label layer (replaces the sklearn calls of the reference's
    HDBSCANClusterer, src/pybasin/predictors/hdbscan_clusterer.py:12-148, and
    the 1-NN template classifiers of the case studies) --------------------- */

@@ -20,7 +20,9 @@ Contents
 * ``comprehensive``    one calculator of the comprehensive set on one window
                        series (ts_torch/settings.py:223-340; calculators in
                        ts_torch/calculators/torch_features_*.py), per series in
-                       float64, directly from the reference's formulas.
+                       float64, directly from the reference's formulas (the
+                       spectrum calculators on the rfft written as the
+                       explicit DFT sum).
 * ``sync_features``    SynchronizationFeatureExtractor._compute_sync_features
                        (case_studies/rossler_network/synchronization_feature_extractor.py:73-106).
 * ``sync_labels``      SynchronizationClassifier.fit_predict
@@ -478,4 +480,45 @@ def comprehensive(x: np.ndarray, name: str, params: dict | None = None, dtype=np
     if name == "symmetry_looking":  # torch_features_statistical.py:184-190 (torch median: lower middle)
         med = np.sort(x)[(n - 1) // 2]
         return float(abs(mu - med) < p["r"] * (x.max() - x.min()))
+    if name in ("fft_coefficient", "spkt_welch_density", "fft_aggregated", "fourier_entropy"):
+        # the rfft as the explicit sum X_b = sum_i x_i exp(-2 pi i b i / n),
+        # b = 0 .. n // 2 (torch_features_frequency.py:11-67)
+        nb = n // 2 + 1
+        ang = 2.0 * np.pi * np.outer(np.arange(nb), np.arange(n)) / n
+        re = (np.cos(ang) * x).sum(1)
+        im = -(np.sin(ang) * x).sum(1)
+        im[0] = 0.0  # the DC and Nyquist bins are real (+0, as the FFT gives)
+        if n % 2 == 0:
+            im[-1] = 0.0
+        psd = re ** 2 + im ** 2
+        if name == "fft_coefficient":  # :11-28
+            k = int(p.get("coeff", 0))
+            if k >= nb:
+                return 0.0
+            attr = p.get("attr", "abs")
+            if attr == "real":
+                return float(re[k])
+            if attr == "imag":
+                return float(im[k])
+            if attr == "angle":
+                return float(np.arctan2(im[k], re[k]))
+            return float(np.hypot(re[k], im[k]))
+        if name == "spkt_welch_density":  # :59-67
+            k = int(p.get("coeff", 0))
+            return float(psd[k] / n) if k < nb else 0.0
+        pn = psd / (psd.sum() + 1e-10)
+        if name == "fourier_entropy":  # torch_features_entropy_complexity.py:97-107
+            c = np.maximum(pn, 1e-10)
+            return float(-(c * np.log(c)).sum())
+        b = np.arange(nb, dtype=np.float64)  # fft_aggregated, :31-56
+        cen = (b * pn).sum()
+        var = ((b - cen) ** 2 * pn).sum()
+        agg = p.get("aggtype", "centroid")
+        if agg == "variance":
+            return float(var)
+        if agg == "skew":
+            return float(((b - cen) ** 3 * pn).sum() / (var ** 1.5 + 1e-10))
+        if agg == "kurtosis":
+            return float(((b - cen) ** 4 * pn).sum() / (var ** 2 + 1e-10))
+        return float(cen)
     raise KeyError(name)

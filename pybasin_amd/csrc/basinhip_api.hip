@@ -858,6 +858,44 @@ int32_t bh_window_spectral_ratio(int32_t dtype, const void *traj, int64_t n_save
     return 0;
 }
 
+int32_t bh_window_spectrum(int32_t dtype, const void *traj, int64_t n_save, int64_t n,
+                           int32_t dim, int64_t start, int32_t n_coeff, double *out_coeff,
+                           double *out_agg, void *stream) {
+    g_err[0] = 0;
+    int32_t rc = check_series_args("bh_window_spectrum", dtype, traj, n_save, n, dim, start);
+    if (rc) return rc;
+    if (n_coeff < 0 || n_coeff > 4096) return fail(-5, "bh_window_spectrum: n_coeff must be in [0, 4096]");
+    if (n_coeff > 0 && !out_coeff) return fail(-5, "bh_window_spectrum: null out_coeff");
+    if (n_coeff == 0 && !out_agg) return fail(-5, "bh_window_spectrum: nothing to compute");
+    if (n == 0) return 0;
+    const int64_t w = n_save - start;
+    const size_t elem = dtype == BH_F32 ? 4 : 8;
+    // series rows plus (with out_agg) one row of w / 2 + 1 powers per wave
+    const int64_t budget = 62 * 1024;
+    const int64_t pw_bytes = out_agg ? 4 * (w / 2 + 1) * 8 : 0;
+    const int64_t per = w * (int64_t)elem;
+    if (per + pw_bytes > budget)
+        return fail(-5, "bh_window_spectrum: window of %lld rows exceeds the %lld-byte LDS budget",
+                    (long long)w, (long long)budget);
+    int64_t G = (budget - pw_bytes) / per;
+    if (G > 16) G = 16;
+    const size_t lds = (size_t)((G * per + 15) / 16 * 16 + pw_bytes);
+    const int64_t np = n * dim;
+    const int64_t blocks = (np + G - 1) / G;
+    if (blocks > 2147483647LL) return fail(-5, "bh_window_spectrum: too many series");
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    if (dtype == BH_F32)
+        hipLaunchKernelGGL(bh::win::spectrum_kernel<float>, dim3((unsigned)blocks),
+                           dim3(bh::win::kSeriesBlock), lds, s, reinterpret_cast<const float *>(traj),
+                           n_save, n, dim, start, (int)G, n_coeff, out_coeff, out_agg);
+    else
+        hipLaunchKernelGGL(bh::win::spectrum_kernel<double>, dim3((unsigned)blocks),
+                           dim3(bh::win::kSeriesBlock), lds, s, reinterpret_cast<const double *>(traj),
+                           n_save, n, dim, start, (int)G, n_coeff, out_coeff, out_agg);
+    BH_HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
 int32_t bh_window_features(int32_t dtype, const void *traj, int64_t n_save, int64_t n,
                            int32_t dim, int64_t start, double *out, void *stream) {
     g_err[0] = 0;

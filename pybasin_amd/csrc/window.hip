@@ -330,6 +330,138 @@ __global__ void __launch_bounds__(kSeriesBlock)
     }
 }
 
+// Bins b0 .. b0 + Q - 1 of one staged series by Goertzel, in double; bins at
+// or past `top` are skipped.  Writes (re, im) of bins below n_coeff and, with
+// pw, keeps each power in the wave's LDS row and adds it into m0 and b * P_b
+// into m1.
+template <class T, int Q>
+__device__ __forceinline__ void goertzel_bins(const T *x, int64_t w, int64_t b0, int64_t top,
+                                              int n_coeff, int64_t np, int64_t p,
+                                              double *out_coeff, double *pw, double &m0,
+                                              double &m1) {
+    double cw[Q], s1[Q], s2[Q];
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+        cw[q] = 2.0 * cospi(2.0 * (double)(b0 + q) / (double)w);
+        s1[q] = 0.0;
+        s2[q] = 0.0;
+    }
+#pragma unroll 4
+    for (int64_t i = 0; i < w; i++) {
+        const double c = (double)x[i];
+#pragma unroll
+        for (int q = 0; q < Q; q++) {
+            const double s0 = fma(cw[q], s1[q], c - s2[q]);
+            s2[q] = s1[q];
+            s1[q] = s0;
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+        const int64_t b = b0 + q;
+        if (b >= top) continue;
+        const double re = fma(0.5 * cw[q], s1[q], -s2[q]);
+        // the DC and Nyquist bins are real: +0, as the FFT gives (their angle is 0 or pi)
+        const double im = (b == 0 || 2 * b == w) ? 0.0 : sinpi(2.0 * (double)b / (double)w) * s1[q];
+        if (b < n_coeff) {
+            out_coeff[(b * 2) * np + p] = re;
+            out_coeff[(b * 2 + 1) * np + p] = im;
+        }
+        if (pw) {
+            const double pwr = fma(re, re, im * im);
+            pw[b] = pwr;
+            m0 += pwr;
+            m1 = fma((double)b, pwr, m1);
+        }
+    }
+}
+
+template <class T, int Q>
+__device__ __forceinline__ void series_bins(const T *x, int64_t w, int64_t top, int n_coeff,
+                                            int64_t np, int64_t p, double *out_coeff,
+                                            double *pw, double &m0, double &m1, int lane) {
+    for (int64_t b0 = (int64_t)lane * Q; b0 < top; b0 += 64 * Q)
+        goertzel_bins<T, Q>(x, w, b0, top, n_coeff, np, p, out_coeff, pw, m0, m1);
+}
+
+// Complex rfft bins and the aggregated spectrum of each series of the
+// window (the reference's fft_coefficient, spkt_welch_density and
+// fft_aggregated, all on torch.fft.rfft of the raw window;
+// torch_features_frequency.py:11-70): X_b = sum_i x_i e^{-2 pi i b i / w} by
+// Goertzel's recurrence in double, s_i = x_i + 2 cos(om) s_{i-1} - s_{i-2},
+// X_b = e^{i om} s_{w-1} - s_{w-2} (om = 2 pi b / w), Q = 1..4 bins per lane
+// at a time (the fewest that cover the bins asked for in one round of 64
+// lanes).  Bins 0 .. n_coeff - 1 are written as (re, im) (bins past the last
+// rfft bin as 0); with out_agg, the powers P_b = |X_b|^2 of all w / 2 + 1
+// bins are kept in the wave's LDS row and the centroid, variance, skew and
+// kurtosis of the normalised power spectrum p_b = P_b / (sum P + 1e-10)
+// follow from centred sums, and its entropy -sum c log c over
+// c = max(p_b, 1e-10) (fourier_entropy, torch_features_entropy_complexity.py:97-107),
+// the reference's 1e-10 guards included.
+template <class T>
+__global__ void __launch_bounds__(kSeriesBlock)
+    spectrum_kernel(const T *traj, int64_t n_save, int64_t n, int S, int64_t start, int G,
+                    int n_coeff, double *out_coeff, double *out_agg) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int64_t w = n_save - start;
+    const int64_t np = n * S;
+    const int64_t p0 = (int64_t)blockIdx.x * G;
+    const int64_t nb = w / 2 + 1;  // rfft bins
+    T *lds = reinterpret_cast<T *>(smem);
+    double *pbase = reinterpret_cast<double *>(smem + ((int64_t)G * w * sizeof(T) + 15) / 16 * 16);
+    load_series<T>(traj, n, S, start, w, p0, G, np, lds);
+    __syncthreads();
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    double *pw = out_agg ? pbase + (int64_t)wave * nb : nullptr;
+    const int64_t top = out_agg ? nb : (n_coeff < nb ? (int64_t)n_coeff : nb);
+    const int64_t rounds = (top + 63) / 64;
+    const int Q = rounds >= 4 ? 4 : (int)rounds;
+    for (int g = wave; g < G; g += kSeriesBlock / 64) {
+        const int64_t p = p0 + g;
+        if (p >= np) break;  // wave-uniform
+        const T *x = lds + (int64_t)g * w;
+        double m0 = 0.0, m1 = 0.0;
+        switch (Q) {
+            case 1: series_bins<T, 1>(x, w, top, n_coeff, np, p, out_coeff, pw, m0, m1, lane); break;
+            case 2: series_bins<T, 2>(x, w, top, n_coeff, np, p, out_coeff, pw, m0, m1, lane); break;
+            case 3: series_bins<T, 3>(x, w, top, n_coeff, np, p, out_coeff, pw, m0, m1, lane); break;
+            default: series_bins<T, 4>(x, w, top, n_coeff, np, p, out_coeff, pw, m0, m1, lane); break;
+        }
+        for (int64_t b = nb + lane; b < n_coeff; b += 64) {  // past the last bin
+            out_coeff[(b * 2) * np + p] = 0.0;
+            out_coeff[(b * 2 + 1) * np + p] = 0.0;
+        }
+        if (out_agg) {
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            const double tot = wave_sum(m0) + 1e-10;
+            const double centroid = wave_sum(m1) / tot;
+            double c2 = 0.0, c3 = 0.0, c4 = 0.0, ent = 0.0;
+            for (int64_t b = lane; b < nb; b += 64) {
+                const double d = (double)b - centroid, pn = pw[b] / tot;
+                const double d2 = d * d;
+                c2 = fma(d2, pn, c2);
+                c3 = fma(d2 * d, pn, c3);
+                c4 = fma(d2 * d2, pn, c4);
+                const double pc = pn < 1e-10 ? 1e-10 : pn;  // NaN stays NaN, as torch's clamp
+                ent = fma(-pc, log(pc), ent);
+            }
+            const double var = wave_sum(c2);
+            c3 = wave_sum(c3);
+            c4 = wave_sum(c4);
+            ent = wave_sum(ent);
+            if (lane == 0) {
+                out_agg[0 * np + p] = centroid;
+                out_agg[1 * np + p] = var;
+                out_agg[2 * np + p] = c3 / (pow(var, 1.5) + 1e-10);
+                out_agg[3 * np + p] = c4 / (var * var + 1e-10);
+                out_agg[4 * np + p] = ent;
+            }
+            __builtin_amdgcn_wave_barrier();  // pw reused by the wave's next series
+        }
+    }
+}
+
 // ---- comprehensive window features: raw accumulators per series ----------
 //
 // One lane per (trajectory, state) series of the window rows [start,

@@ -24,8 +24,9 @@ from typing import Any
 import torch
 
 from .solvers.hip_dopri5 import FUSED_RESULTS, FusedResult, steady_index
-from .window_features import (COMPREHENSIVE_FEATURES, check_params, comprehensive_feature,
-                              window_feature_block)
+from .window_features import (COMPREHENSIVE_FEATURES, SPECTRUM_FEATURES, check_params,
+                              comprehensive_feature, spectrum_feature, spectrum_needs,
+                              window_feature_block, window_spectrum)
 
 DEFAULT_STEADY_FRACTION = 0.85
 
@@ -339,10 +340,18 @@ class StatFeatureExtractor(FeatureExtractor):
     def _columns(self, plan, stats: torch.Tensor, y: torch.Tensor | None,
                  start: int) -> torch.Tensor:
         cols = []
-        blk = med = None
+        blk = med = spec = None
         memo: dict = {}
         for s, name, params in plan:
-            if is_comprehensive(name, params):
+            if is_comprehensive(name, params) and name in SPECTRUM_FEATURES:
+                if y is None:
+                    raise ValueError(f"{name} needs the saved window: integrate with "
+                                     "keep_trajectories=True")
+                if spec is None:
+                    spec = window_spectrum(y, start, *spectrum_needs(plan))
+                cols.append(spectrum_feature(name, params, *spec, y.shape[0] - start)[:, s].to(
+                    stats.device, torch.float64))
+            elif is_comprehensive(name, params):
                 if y is None:
                     raise ValueError(f"{name} needs the saved window: integrate with "
                                      "keep_trajectories=True")

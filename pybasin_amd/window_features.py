@@ -79,6 +79,18 @@ COMPREHENSIVE_FEATURES: dict[str, list[dict] | None] = {
     "binned_entropy": [{"max_bins": 10}],
     "symmetry_looking": [{"r": r * 0.05} for r in range(20)],
 }
+# the calculators of the same set formed from the rfft of the window
+# (bh_window_spectrum): complex bins 0 .. 99 and the aggregated power spectrum
+SPECTRUM_FEATURES: dict[str, list[dict]] = {
+    "fft_coefficient": [{"coeff": k, "attr": a} for a in ["real", "imag", "abs", "angle"]
+                        for k in range(100)],
+    "fft_aggregated": [{"aggtype": s} for s in ["centroid", "variance", "skew", "kurtosis"]],
+    "spkt_welch_density": [{"coeff": c} for c in [2, 5, 8]],
+    "fourier_entropy": [{"bins": x} for x in [2, 3, 5, 10, 100]],
+}
+COMPREHENSIVE_FEATURES.update(SPECTRUM_FEATURES)
+# the largest rfft bin index bh_window_spectrum writes
+MAX_SPECTRUM_COEFF = 4095
 # calculators whose parameter may take any value (formed from the slots by
 # a comparison with the parameter)
 _FREE_PARAMS = {"large_standard_deviation", "symmetry_looking", "linear_trend",
@@ -109,6 +121,14 @@ def check_params(name: str, params: dict | None) -> None:
             0 <= int(params.get("segment_focus", -1)) < 10:
         return
     if name == "cid_ce" and "normalize" in params:
+        return
+    if name in ("fft_coefficient", "spkt_welch_density"):
+        k = params.get("coeff", 0)
+        if not (isinstance(k, int) and 0 <= k <= MAX_SPECTRUM_COEFF):
+            raise ValueError(f"{name}: coeff must be an integer in [0, {MAX_SPECTRUM_COEFF}], "
+                             f"got {k!r}")
+        return
+    if name in ("fft_aggregated", "fourier_entropy"):
         return
     if params not in grid:
         raise ValueError(f"{name}: parameters {params} are not reduced by the window kernel; "
@@ -409,3 +429,103 @@ def comprehensive_feature(name: str, params: dict | None, blk: torch.Tensor, n_w
             raise ValueError("symmetry_looking needs the window median")
         return ((mean - median.double()).abs() < float(p["r"]) * rng).double().where(~N.isnan(), N)
     raise KeyError(f"feature {name!r} is not a comprehensive window feature")
+
+
+def spectrum_needs(plan) -> tuple[int, bool]:
+    """(number of complex bins, whether the aggregated spectrum is needed)
+    for the spectrum features of a plan of (state, name, params)."""
+    n_coeff, agg = 0, False
+    for _, name, params in plan:
+        if name in ("fft_coefficient", "spkt_welch_density"):
+            n_coeff = max(n_coeff, int((params or {}).get("coeff", 0)) + 1)
+        elif name in ("fft_aggregated", "fourier_entropy"):
+            agg = True
+    return n_coeff, agg
+
+
+def _spectrum_cpu(w: torch.Tensor, n_coeff: int, agg: bool):
+    """The outputs of bh_window_spectrum for a CPU window (N, B, S), in torch."""
+    X = torch.fft.rfft(w.to(torch.float64), dim=0)
+    nb = X.shape[0]
+    coeff = None
+    if n_coeff:
+        coeff = torch.zeros((n_coeff, 2) + tuple(w.shape[1:]), dtype=torch.float64)
+        k = min(n_coeff, nb)
+        coeff[:k, 0] = X[:k].real
+        coeff[:k, 1] = X[:k].imag
+        coeff[0, 1] = 0.0  # the DC and Nyquist bins are real (+0)
+        if w.shape[0] % 2 == 0 and w.shape[0] // 2 < n_coeff:
+            coeff[w.shape[0] // 2, 1] = 0.0
+    aggs = None
+    if agg:
+        P = X.real ** 2 + X.imag ** 2
+        pn = P / (P.sum(0) + 1e-10)
+        b = torch.arange(nb, dtype=torch.float64).view(nb, 1, 1)
+        cen = (b * pn).sum(0)
+        d = b - cen
+        var = (d ** 2 * pn).sum(0)
+        pc = pn.clamp(min=1e-10)
+        aggs = torch.stack([cen, var, (d ** 3 * pn).sum(0) / (var ** 1.5 + 1e-10),
+                            (d ** 4 * pn).sum(0) / (var ** 2 + 1e-10),
+                            -(pc * pc.log()).sum(0)])
+    return coeff, aggs
+
+
+def window_spectrum(y: torch.Tensor, start: int, n_coeff: int, agg: bool):
+    """(coeff (n_coeff, 2, B, S) or None, agg (5, B, S) or None), float64:
+    the rfft bins 0 .. n_coeff - 1 of each series of the window rows
+    [start, N) of y (N, B, S) as (re, im) (0 past the last bin), and the
+    centroid, variance, skew, kurtosis and entropy of its normalised power
+    spectrum; bh_window_spectrum on the device, torch.fft on the CPU."""
+    N, B, S = y.shape
+    if not 0 <= start < N:
+        raise ValueError("empty feature window")
+    if not 0 <= n_coeff <= MAX_SPECTRUM_COEFF + 1:
+        raise ValueError(f"n_coeff must be in [0, {MAX_SPECTRUM_COEFF + 1}]")
+    if not n_coeff and not agg:
+        return None, None
+    if y.device.type != "cuda":
+        return _spectrum_cpu(y[start:], n_coeff, agg)
+    y = y.contiguous()
+    coeff = (torch.empty((n_coeff, 2, B, S), dtype=torch.float64, device=y.device)
+             if n_coeff else None)
+    aggs = torch.empty((5, B, S), dtype=torch.float64, device=y.device) if agg else None
+    dt = _lib.BH_F32 if y.dtype == torch.float32 else _lib.BH_F64
+    s = torch.cuda.current_stream(y.device)
+    with torch.cuda.device(y.device):
+        _lib.check(_lib.lib().bh_window_spectrum(
+            dt, y.data_ptr(), N, B, S, int(start), int(n_coeff),
+            coeff.data_ptr() if coeff is not None else None,
+            aggs.data_ptr() if aggs is not None else None, s.cuda_stream),
+            "bh_window_spectrum")
+    return coeff, aggs
+
+
+_AGGTYPES = {"centroid": 0, "variance": 1, "skew": 2, "kurtosis": 3}
+
+
+def spectrum_feature(name: str, params: dict | None, coeff: torch.Tensor | None,
+                     agg: torch.Tensor | None, n_win: int) -> torch.Tensor:
+    """Spectrum feature `name` (B, S) float64 from the outputs of
+    window_spectrum (torch_features_frequency.py:11-70,
+    torch_features_entropy_complexity.py:97-107)."""
+    p = params or {}
+    if name == "fft_coefficient":  # frequency.py:11-28; an unknown attr gives abs
+        k = int(p.get("coeff", 0))
+        re, im = coeff[k, 0], coeff[k, 1]
+        attr = p.get("attr", "abs")
+        if attr == "real":
+            return re
+        if attr == "imag":
+            return im
+        if attr == "angle":
+            return torch.atan2(im, re)
+        return torch.hypot(re, im)
+    if name == "spkt_welch_density":  # frequency.py:59-67
+        k = int(p.get("coeff", 0))
+        return (coeff[k, 0] ** 2 + coeff[k, 1] ** 2) / float(n_win)
+    if name == "fft_aggregated":  # frequency.py:31-56; an unknown aggtype gives the centroid
+        return agg[_AGGTYPES.get(p.get("aggtype", "centroid"), 0)]
+    if name == "fourier_entropy":  # entropy_complexity.py:97-107; bins is not used
+        return agg[4]
+    raise KeyError(f"feature {name!r} is not a spectrum feature")

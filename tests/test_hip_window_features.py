@@ -145,7 +145,11 @@ def _check_against_oracle(blk, w64, start, tag, series_dtype=np.float64):
     N, B, S = x.shape
     med = torch.tensor(np.sort(x, axis=0)[(N - 1) // 2])
     bad = []
+    from pybasin_amd.window_features import SPECTRUM_FEATURES
+
     for name, grid in COMPREHENSIVE_FEATURES.items():
+        if name in SPECTRUM_FEATURES:
+            continue  # bh_window_spectrum, checked below
         for p in (grid or [None]):
             if name in ("skewness", "kurtosis") and N <= 3:
                 continue  # noise amplified by the reference's 1e-10 guards
@@ -212,3 +216,107 @@ def test_window_features_on_integrated_duffing(cuda_device, dtype):
                       for n_, p_ in (("kurtosis", None), ("number_peaks", {"n": 3}),
                                      ("linear_trend", {"attr": "rvalue"}))] for b in range(x.shape[1])])
     np.testing.assert_allclose(f, want.astype(np.float32), rtol=2e-6, atol=1e-6)
+
+
+def _check_spectrum(coeff, agg, x, tag, n_coeff, with_agg):
+    """bh_window_spectrum's outputs against the oracle's DFT sums, series by
+    series: bins to 1e-9 of the series' sum |x|, angles (where the bin is
+    above 1e-6 of it) to 1e-6 rad, the aggregates to 1e-9."""
+    N, B, S = x.shape
+    nb = N // 2 + 1
+    ks = sorted({k for k in (0, 1, 2, 3, 5, 8, 13, 31, 64, 99, nb - 1, nb, nb + 1) if k < n_coeff})
+    for b in range(B):
+        for s in range(S):
+            xs = x[:, b, s]
+            finite = np.all(np.isfinite(xs))
+            scale = max(np.abs(xs).sum(), 1e-300) if finite else 1.0
+            for k in ks:
+                re, im = float(coeff[k, 0, b, s]), float(coeff[k, 1, b, s])
+                if not finite:
+                    if k < nb:
+                        assert not (math.isfinite(re) and math.isfinite(im)), (tag, k, b, s)
+                    continue
+                want_re = O.comprehensive(xs, "fft_coefficient", {"coeff": k, "attr": "real"})
+                want_im = O.comprehensive(xs, "fft_coefficient", {"coeff": k, "attr": "imag"})
+                assert abs(re - want_re) <= 1e-9 * scale and abs(im - want_im) <= 1e-9 * scale, \
+                    (tag, k, b, s, re, im, want_re, want_im)
+                if k >= nb:
+                    assert re == 0.0 and im == 0.0, (tag, k)
+                if math.hypot(want_re, want_im) > 1e-6 * scale:
+                    d = abs(math.atan2(im, re) - math.atan2(want_im, want_re))
+                    assert min(d, 2 * math.pi - d) <= 1e-6, (tag, k, b, s)
+                if k in (0, nb - 1) and (k == 0 or N % 2 == 0):
+                    assert im == 0.0 and math.copysign(1.0, im) > 0, (tag, k)  # real bins, +0
+            if with_agg:
+                for j, (name, p) in enumerate((("fft_aggregated", {"aggtype": "centroid"}),
+                                               ("fft_aggregated", {"aggtype": "variance"}),
+                                               ("fft_aggregated", {"aggtype": "skew"}),
+                                               ("fft_aggregated", {"aggtype": "kurtosis"}),
+                                               ("fourier_entropy", {"bins": 10}))):
+                    want = O.comprehensive(xs, name, p)
+                    g = float(agg[j, b, s])
+                    if not math.isfinite(want):
+                        assert not math.isfinite(g), (tag, name, p, b, s)
+                    else:
+                        assert g == pytest.approx(want, rel=1e-9, abs=1e-9), (tag, name, p, b, s, g,
+                                                                              want)
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_window_spectrum_kernel_equals_oracle_synthetic(cuda_device, dtype):
+    """Bins 0..99 alone (one or two bins per lane), bins past the last one
+    (zeros), the full spectrum with its aggregates (up to four bins per lane
+    in several rounds), the aggregates alone; ragged, short, constant and
+    non-finite windows; windows starting at row 0 and 1."""
+    from pybasin_amd.window_features import window_spectrum
+
+    for tag, w in _synthetic_windows():
+        wt = torch.tensor(w, dtype=dtype)
+        x = wt.double().numpy()
+        for start in (0, 1) if w.shape[0] > 1 else (0,):
+            nb = (w.shape[0] - start) // 2 + 1
+            for n_coeff, with_agg in ((100, False), (nb + 2, True), (0, True), (40, False)):
+                coeff, agg = window_spectrum(wt.cuda(), start, n_coeff, with_agg)
+                if n_coeff:
+                    assert coeff.shape == (n_coeff, 2) + w.shape[1:]
+                else:
+                    assert coeff is None
+                    coeff = torch.zeros((0, 2) + w.shape[1:], dtype=torch.float64)
+                _check_spectrum(coeff.cpu(), agg.cpu() if with_agg else None, x[start:],
+                                f"{tag}/{start}/{n_coeff}/{with_agg}", n_coeff, with_agg)
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_window_spectrum_on_integrated_duffing(cuda_device, dtype):
+    """The rfft of 2,000 integrated Duffing windows (the kept trajectory of a
+    fused launch) equals numpy's FFT on every series (bins 0..99 and the
+    aggregates formed from numpy's full spectrum), the oracle on eight; the
+    extractor's spectrum columns equal the kernel's outputs."""
+    from pybasin_amd.window_features import spectrum_feature, window_spectrum
+
+    res = _duffing(dtype)
+    coeff, agg = window_spectrum(res.y, res.steady_idx, 100, True)
+    coeff, agg = coeff.cpu().numpy(), agg.cpu().numpy()
+    x = res.y.double().cpu().numpy()[res.steady_idx:]
+    X = np.fft.rfft(x, axis=0)
+    scale = np.abs(x).sum(0)
+    assert np.all(np.abs(coeff[:, 0] - X[:100].real) <= 1e-9 * scale)
+    assert np.all(np.abs(coeff[:, 1] - X[:100].imag) <= 1e-9 * scale)
+    P = X.real ** 2 + X.imag ** 2
+    pn = P / (P.sum(0) + 1e-10)
+    bins = np.arange(P.shape[0], dtype=float)[:, None, None]
+    cen = (bins * pn).sum(0)
+    np.testing.assert_allclose(agg[0], cen, rtol=1e-9)
+    np.testing.assert_allclose(agg[1], ((bins - cen) ** 2 * pn).sum(0), rtol=1e-9)
+    pc = np.maximum(pn, 1e-10)
+    np.testing.assert_allclose(agg[4], -(pc * np.log(pc)).sum(0), rtol=1e-9)
+    _check_spectrum(coeff[:, :, :4], agg[:, :4], x[:, :4], "duffing", 100, True)
+    feats = {"fft_coefficient": [{"coeff": 7, "attr": "angle"}], "fft_aggregated": [{"aggtype": "kurtosis"}],
+             "spkt_welch_density": [{"coeff": 5}]}
+    fx = StatFeatureExtractor(time_steady=300.0, features=feats)
+    f = fx.extract_features(Solution(res.y[0], res.t, res.y)).cpu().numpy()
+    c, a = torch.tensor(coeff), torch.tensor(agg)
+    n = x.shape[0]
+    want = np.stack([spectrum_feature(nm, p, c, a, n).numpy()[:, s] for s in range(2)
+                     for nm, ps in feats.items() for p in ps], 1)
+    np.testing.assert_array_equal(f, want.astype(np.float32))

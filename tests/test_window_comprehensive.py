@@ -23,13 +23,30 @@ import torch
 
 from oracle import oracle as O
 from pybasin_amd.features import StatFeatureExtractor, Solution
-from pybasin_amd.window_features import (COMPREHENSIVE_FEATURES, _block_cpu, check_params,
-                                         comprehensive_feature)
+from pybasin_amd.window_features import (COMPREHENSIVE_FEATURES, SPECTRUM_FEATURES, _block_cpu,
+                                         check_params, comprehensive_feature, spectrum_feature,
+                                         window_spectrum)
 
 
 def _all_params():
     for name, grid in COMPREHENSIVE_FEATURES.items():
+        if name in SPECTRUM_FEATURES:
+            continue
         for p in (grid or [None]):
+            yield name, p
+
+
+def _spectrum_params(nb=None):
+    """The spectrum calculators of the comprehensive set, with the bins
+    limited to a spread of indices (and nb, nb + 1: past the last bin)."""
+    ks = [0, 1, 2, 3, 5, 8, 13, 31, 64, 99] + ([nb, nb + 1] if nb is not None else [])
+    for a in ("real", "imag", "abs", "angle"):
+        for k in ks:
+            yield "fft_coefficient", {"coeff": k, "attr": a}
+    for k in ks:
+        yield "spkt_welch_density", {"coeff": k}
+    for name in ("fft_aggregated", "fourier_entropy"):
+        for p in SPECTRUM_FEATURES[name]:
             yield name, p
 
 
@@ -161,6 +178,117 @@ def test_oracle_known_answers():
     assert math.isnan(C(np.array([1.0, np.inf, 2.0]), "sum_values"))
 
 
+def test_oracle_spectrum_matches_numpy_rfft():
+    """The oracle's explicit DFT sum against numpy's FFT (another algorithm),
+    odd and even lengths, every attr, bins past the last one."""
+    rng = np.random.default_rng(5)
+    for n in (1, 2, 7, 64, 255, 1000):
+        x = rng.normal(size=n) + 0.3
+        X = np.fft.rfft(x)
+        nb = X.size
+        for k in list(range(min(nb, 12))) + [nb - 1, nb, nb + 7]:
+            want = X[k] if k < nb else 0j
+            for attr, v in (("real", want.real), ("imag", want.imag), ("abs", abs(want))):
+                assert O.comprehensive(x, "fft_coefficient", {"coeff": k, "attr": attr}) == \
+                    pytest.approx(v, abs=1e-9 * n), (n, k, attr)
+            if abs(want) > 1e-6:
+                d = abs(O.comprehensive(x, "fft_coefficient", {"coeff": k, "attr": "angle"})
+                        - np.angle(want))
+                assert min(d, 2 * np.pi - d) <= 1e-9, (n, k)
+            assert O.comprehensive(x, "spkt_welch_density", {"coeff": k}) == pytest.approx(
+                abs(want) ** 2 / n, rel=1e-9, abs=1e-9), (n, k)
+
+
+def test_oracle_spectrum_closed_forms():
+    """Pure tones put their power in known bins: one tone -> centroid k,
+    variance 0; two tones with powers w1 : w2 -> the two-point law's moments
+    (centroid, variance w1 w2 d^2, skew (w1 - w2) / sqrt(w1 w2), kurtosis
+    1 / (w1 w2) - 3, entropy -sum w ln w); sine -> angle -pi / 2; DC and
+    Nyquist bins of constant and alternating series."""
+    C = O.comprehensive
+    n = 240
+    i = np.arange(n)
+    x = np.cos(2 * np.pi * 17 * i / n)
+    assert C(x, "fft_coefficient", {"coeff": 17, "attr": "real"}) == pytest.approx(n / 2)
+    assert C(x, "fft_aggregated", {"aggtype": "centroid"}) == pytest.approx(17.0)
+    assert C(x, "fft_aggregated", {"aggtype": "variance"}) == pytest.approx(0.0, abs=1e-9)
+    nb = n // 2 + 1  # one bin holds all power; the other nb - 1 clamp to 1e-10
+    assert C(x, "fourier_entropy", {"bins": 10}) == pytest.approx(
+        (nb - 1) * 1e-10 * math.log(1e10), rel=1e-3)
+    s = np.sin(2 * np.pi * 9 * i / n)
+    assert C(s, "fft_coefficient", {"coeff": 9, "attr": "imag"}) == pytest.approx(-n / 2)
+    assert C(s, "fft_coefficient", {"coeff": 9, "attr": "angle"}) == pytest.approx(-np.pi / 2)
+    a, b, k1, k2 = 1.0, 2.0, 10, 40
+    two = a * np.cos(2 * np.pi * k1 * i / n) + b * np.cos(2 * np.pi * k2 * i / n)
+    w1, w2 = a * a / (a * a + b * b), b * b / (a * a + b * b)
+    d = k2 - k1
+    assert C(two, "fft_aggregated", {"aggtype": "centroid"}) == pytest.approx(w1 * k1 + w2 * k2)
+    assert C(two, "fft_aggregated", {"aggtype": "variance"}) == pytest.approx(w1 * w2 * d * d)
+    assert C(two, "fft_aggregated", {"aggtype": "skew"}) == pytest.approx(
+        (w1 - w2) / math.sqrt(w1 * w2), rel=1e-9)
+    assert C(two, "fft_aggregated", {"aggtype": "kurtosis"}) == pytest.approx(
+        1 / (w1 * w2) - 3, rel=1e-9)
+    assert C(two, "fourier_entropy", {"bins": 3}) == pytest.approx(
+        -(w1 * math.log(w1) + w2 * math.log(w2)), rel=1e-6)
+    assert C(np.full(30, -2.0), "fft_coefficient", {"coeff": 0, "attr": "real"}) == \
+        pytest.approx(-60.0)
+    assert C(np.full(30, -2.0), "fft_coefficient", {"coeff": 0, "attr": "angle"}) == \
+        pytest.approx(np.pi)
+    alt = (-1.0) ** np.arange(16)
+    assert C(alt, "fft_coefficient", {"coeff": 8, "attr": "real"}) == pytest.approx(16.0)
+    assert C(alt, "spkt_welch_density", {"coeff": 8}) == pytest.approx(16.0)
+    assert C(alt, "fft_coefficient", {"coeff": 9, "attr": "abs"}) == 0.0
+
+
+def _spectrum_close(g, want, name, p, scale):
+    """The spectrum tolerance: bins to 1e-9 of the series' sum |x| (angles
+    only where the bin is well above that), the normalised aggregates to
+    1e-9."""
+    if not math.isfinite(want):
+        return not math.isfinite(g)
+    if name == "fft_coefficient":
+        if p["attr"] == "angle":
+            return True  # checked through real and imag
+        return abs(g - want) <= 1e-9 * scale
+    if name == "spkt_welch_density":
+        return abs(g - want) <= 1e-9 * scale * scale
+    return g == pytest.approx(want, rel=1e-9, abs=1e-9)
+
+
+def _angle_ok(got_re, got_im, x, k, scale):
+    X = O.comprehensive(x, "fft_coefficient", {"coeff": k, "attr": "abs"})
+    if X <= 1e-6 * scale:
+        return True
+    want = O.comprehensive(x, "fft_coefficient", {"coeff": k, "attr": "angle"})
+    d = abs(math.atan2(got_im, got_re) - want)
+    return min(d, 2 * math.pi - d) <= 1e-6
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_cpu_spectrum_path_equals_oracle(dtype):
+    for w in _windows():
+        wt = torch.tensor(w, dtype=dtype)
+        x64 = wt.double().numpy()
+        N, B, S = w.shape
+        nb = N // 2 + 1
+        plist = list(_spectrum_params(nb))
+        coeff, agg = window_spectrum(wt, 0, 102 + nb, True)
+        for name, p in plist:
+            got = spectrum_feature(name, p, coeff, agg, N).numpy()
+            for b in range(B):
+                for s in range(S):
+                    x = x64[:, b, s]
+                    want = O.comprehensive(x, name, p)
+                    scale = max(np.abs(x).sum(), 1e-300)
+                    assert _spectrum_close(float(got[b, s]), want, name, p, scale), \
+                        (name, p, N, b, s, float(got[b, s]), want)
+                    if name == "fft_coefficient" and p["attr"] == "angle":
+                        k = p["coeff"]
+                        assert _angle_ok(float(coeff[min(k, coeff.shape[0] - 1), 0, b, s]),
+                                         float(coeff[min(k, coeff.shape[0] - 1), 1, b, s]),
+                                         x, k, scale), (N, k, b, s)
+
+
 def _windows():
     rng = np.random.default_rng(11)
     out = [rng.normal(size=(200, 3, 2)),
@@ -224,8 +352,41 @@ def test_extractor_names_and_rejects_unreduced_params():
     assert got["state_1__linear_trend__attr_stderr"] == pytest.approx(
         O.comprehensive(x, "linear_trend", {"attr": "stderr"}), rel=1e-5)
     for bad in ({"count_above": [{"t": 0.5}]}, {"autocorrelation": [{"lag": 12}]},
-                {"linear_trend": [{"attr": "bogus"}]}):
+                {"linear_trend": [{"attr": "bogus"}]}, {"fft_coefficient": [{"coeff": -1}]},
+                {"spkt_welch_density": [{"coeff": 5000}]}):
         with pytest.raises(ValueError):
             StatFeatureExtractor(features=bad).extract_features(sol)
     for name, p in _all_params():
         check_params(name, p)
+    for name, grid in SPECTRUM_FEATURES.items():
+        for p in grid:
+            check_params(name, p)
+
+
+def test_extractor_spectrum_columns():
+    """The spectrum calculators through the extractor: one rfft per
+    extraction, bins past the last one are 0, columns equal the oracle."""
+    torch.manual_seed(3)
+    t = torch.linspace(0, 10, 101, dtype=torch.float64)
+    y = torch.randn(101, 3, 2, dtype=torch.float64)
+    sol = Solution(torch.zeros(3, 2), t, y)
+    feats = {"fft_coefficient": [{"coeff": 3, "attr": "abs"}, {"coeff": 60, "attr": "real"}],
+             "spkt_welch_density": [{"coeff": 2}], "fft_aggregated": [{"aggtype": "skew"}],
+             "fourier_entropy": [{"bins": 5}], "mean": None}
+    fx = StatFeatureExtractor(time_steady=0.0, features=feats)
+    assert fx.needs_window
+    f = fx.extract_features(sol)
+    assert fx.feature_names[:5] == ["state_0__fft_coefficient__attr_abs__coeff_3",
+                                    "state_0__fft_coefficient__attr_real__coeff_60",
+                                    "state_0__spkt_welch_density__coeff_2",
+                                    "state_0__fft_aggregated__aggtype_skew",
+                                    "state_0__fourier_entropy__bins_5"]
+    got = dict(zip(fx.feature_names, f[1].tolist()))
+    x = y[:, 1, 1].numpy()
+    assert got["state_1__fft_coefficient__attr_abs__coeff_3"] == pytest.approx(
+        O.comprehensive(x, "fft_coefficient", {"coeff": 3, "attr": "abs"}), rel=1e-6)
+    assert got["state_1__fft_coefficient__attr_real__coeff_60"] == 0.0
+    assert got["state_1__fft_aggregated__aggtype_skew"] == pytest.approx(
+        O.comprehensive(x, "fft_aggregated", {"aggtype": "skew"}), rel=1e-5)
+    assert got["state_1__fourier_entropy__bins_5"] == pytest.approx(
+        O.comprehensive(x, "fourier_entropy", {"bins": 5}), rel=1e-6)
