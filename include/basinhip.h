@@ -487,6 +487,24 @@ int32_t bh_window_spectrum(int32_t dtype, const void *traj, int64_t n_save, int6
                            int32_t dim, int64_t start, int32_t n_coeff, double *out_coeff,
                            double *out_agg, void *stream);
 
+/* Statistics of the sorted values of each series of the same window, in
+   double: out_dup (5, n, dim) float64 = the adjacent equal pairs D of the
+   sorted series, the values that lie in groups of equal values R, the number
+   of such groups, the sum of the values in groups and the sum of one value per
+   group (has_duplicate, ratio_value_number_to_time_series_length and the four
+   reoccurrence calculators: src/pybasin/ts_torch/calculators/
+   torch_features_boolean.py:11-19, torch_features_reocurrance.py:11-161).
+   out_cq (15, 4, n, dim) float64, or NULL: change_quantiles
+   (torch_features_change.py:20-139) for the 15 corridors ql < qh of the
+   levels 0, 0.2, .., 1 in the order (0, .2), (0, .4), .., (0, 1), (.2, .4),
+   .., (.8, 1): mean and population variance of the changes x_{i+1} - x_i
+   whose two samples lie in [quantile(ql), quantile(qh)], then of their
+   absolute values (0 where none does); the quantiles as torch.quantile forms
+   them in the series dtype.  Non-finite windows give NaN.  Windows up to
+   2,048 float32 or 1,024 float64 rows. */
+int32_t bh_window_sorted(int32_t dtype, const void *traj, int64_t n_save, int64_t n, int32_t dim,
+                         int64_t start, double *out_dup, double *out_cq, void *stream);
+
 /* ---- label layer (replaces the sklearn calls of the reference's
    HDBSCANClusterer, src/pybasin/predictors/hdbscan_clusterer.py:12-148, and
    the 1-NN template classifiers of the case studies) --------------------- */

@@ -305,6 +305,31 @@ def basin_stability(labels: np.ndarray) -> dict:
     return out
 
 
+def _fma_round(a: float, b: float, c: float, dtype) -> float:
+    """a * b + c rounded once (through exact rationals) to dtype."""
+    from fractions import Fraction
+
+    v = Fraction(float(a)) * Fraction(float(b)) + Fraction(float(c))
+    return dtype(float(v))
+
+
+def _torch_quantile(srt: np.ndarray, q: float, dtype) -> float:
+    """torch.quantile(x, q) of a sorted series with linear interpolation in
+    the series dtype: rank = q (n - 1) in that dtype, weight = rank - floor,
+    lerp as a fused multiply-add (below + w (above - below) for w < 0.5, else
+    above - (above - below)(1 - w))."""
+    n = srt.size
+    rank = dtype(dtype(q) * dtype(n - 1))
+    lo = int(rank)
+    hi = min(int(np.ceil(rank)), n - 1)
+    wgt = dtype(rank - dtype(lo))
+    a, b = dtype(srt[lo]), dtype(srt[hi])
+    d = dtype(b - a)
+    if wgt < 0.5:
+        return _fma_round(wgt, d, a, dtype)
+    return _fma_round(-d, dtype(1 - wgt), b, dtype)
+
+
 def comprehensive(x: np.ndarray, name: str, params: dict | None = None, dtype=np.float64) -> float:
     """One calculator of the reference's comprehensive set on one series x
     (N,), float64, written from the reference's torch calculators
@@ -521,4 +546,39 @@ def comprehensive(x: np.ndarray, name: str, params: dict | None = None, dtype=np
         if agg == "kurtosis":
             return float(((b - cen) ** 4 * pn).sum() / (var ** 2 + 1e-10))
         return float(cen)
+    if name in ("has_duplicate", "ratio_value_number_to_time_series_length",
+                "percentage_of_reoccurring_datapoints_to_all_datapoints",
+                "percentage_of_reoccurring_values_to_all_values",
+                "sum_of_reoccurring_data_points", "sum_of_reoccurring_values"):
+        # multiplicities of the distinct values (torch_features_boolean.py:11-19,
+        # torch_features_reocurrance.py:11-161)
+        vals, mult = np.unique(xs, return_counts=True)
+        rep = mult >= 2
+        if name == "has_duplicate":
+            return float(np.any(rep))
+        if name == "ratio_value_number_to_time_series_length":
+            return float(vals.size / n)
+        if name == "percentage_of_reoccurring_datapoints_to_all_datapoints":
+            return float(rep.sum() / (vals.size + 1e-10))
+        if name == "percentage_of_reoccurring_values_to_all_values":
+            return float(mult[rep].sum() / n) if n > 1 else 0.0
+        if name == "sum_of_reoccurring_data_points":
+            return float((vals[rep].astype(np.float64) * mult[rep]).sum())
+        return float(vals[rep].astype(np.float64).sum())
+    if name == "change_quantiles":  # torch_features_change.py:20-139
+        srt = np.sort(xs)
+        q_lo = _torch_quantile(srt, p["ql"], dtype)
+        q_hi = _torch_quantile(srt, p["qh"], dtype)
+        inside = (xs >= q_lo) & (xs <= q_hi)
+        ok = inside[:-1] & inside[1:]
+        d = np.diff(x)
+        if p.get("isabs", True):
+            d = np.abs(d)
+        c = ok.sum()
+        if c == 0:
+            return 0.0
+        mean = d[ok].sum() / (c + 1e-10)
+        if p.get("f_agg", "mean") == "var":
+            return float(((d[ok] - mean) ** 2).sum() / (c + 1e-10))
+        return float(mean)
     raise KeyError(name)

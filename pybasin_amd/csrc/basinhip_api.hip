@@ -896,6 +896,42 @@ int32_t bh_window_spectrum(int32_t dtype, const void *traj, int64_t n_save, int6
     return 0;
 }
 
+int32_t bh_window_sorted(int32_t dtype, const void *traj, int64_t n_save, int64_t n, int32_t dim,
+                         int64_t start, double *out_dup, double *out_cq, void *stream) {
+    g_err[0] = 0;
+    int32_t rc = check_series_args("bh_window_sorted", dtype, traj, n_save, n, dim, start);
+    if (rc) return rc;
+    if (!out_dup) return fail(-5, "bh_window_sorted: null out_dup");
+    if (n == 0) return 0;
+    const int64_t w = n_save - start;
+    const int64_t elem = dtype == BH_F32 ? 4 : 8;
+    int64_t P = 1;
+    while (P < w) P <<= 1;
+    // the block's series rows plus one sort row of P values per wave
+    const int64_t budget = 62 * 1024;
+    const int64_t rows = 4 * P * elem;
+    if (rows + w * elem > budget)
+        return fail(-5, "bh_window_sorted: window of %lld rows exceeds the LDS budget (at most %d %s rows)",
+                    (long long)w, dtype == BH_F32 ? 2048 : 1024, dtype == BH_F32 ? "float32" : "float64");
+    int64_t G = (budget - rows) / (w * elem);
+    if (G > 16) G = 16;
+    const size_t lds = (size_t)((G * w * elem + 15) / 16 * 16 + rows);
+    const int64_t np = n * dim;
+    const int64_t blocks = (np + G - 1) / G;
+    if (blocks > 2147483647LL) return fail(-5, "bh_window_sorted: too many series");
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    if (dtype == BH_F32)
+        hipLaunchKernelGGL(bh::win::sorted_kernel<float>, dim3((unsigned)blocks),
+                           dim3(bh::win::kSeriesBlock), lds, s, reinterpret_cast<const float *>(traj),
+                           n_save, n, dim, start, (int)G, (int)P, out_dup, out_cq);
+    else
+        hipLaunchKernelGGL(bh::win::sorted_kernel<double>, dim3((unsigned)blocks),
+                           dim3(bh::win::kSeriesBlock), lds, s, reinterpret_cast<const double *>(traj),
+                           n_save, n, dim, start, (int)G, (int)P, out_dup, out_cq);
+    BH_HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
 int32_t bh_window_features(int32_t dtype, const void *traj, int64_t n_save, int64_t n,
                            int32_t dim, int64_t start, double *out, void *stream) {
     g_err[0] = 0;

@@ -748,5 +748,185 @@ __global__ void __launch_bounds__(256) window_features_kernel(const T *traj, int
     for (int j = 0; j < 8; j++) put(BH_WF_MASS + j, (double)(mass_idx[j] < 0 ? N - 1 : mass_idx[j]));
 }
 
+// ---- sorted-window statistics: duplicates and change quantiles -------------
+
+// the six quantile levels whose corridors change_quantiles takes
+// (ql in 0 .. 0.8, qh in 0.2 .. 1, ql < qh; settings.py:315-322)
+__constant__ double kCqLevel[6] = {0.0, 0.2, 0.4, 0.6, 0.8, 1.0};
+constexpr int kCqCorr = 15;
+
+// torch.quantile's linear interpolation on a sorted series s of w values, in
+// the series dtype: rank = q * (w - 1) with q in that dtype, below = floor,
+// above = ceil, weight = rank - below, then lerp as a fused multiply-add
+// (weight < 0.5: below + weight (above - below); else above - (above -
+// below)(1 - weight)).
+template <class T>
+__device__ __forceinline__ T torch_quantile(const T *s, int64_t w, double q) {
+    const T rank = (T)q * (T)(w - 1);
+    const int64_t lo = (int64_t)rank;
+    const int64_t hi = (int64_t)ceil((double)rank);
+    const T wgt = rank - (T)lo;
+    const T a = s[lo], b = s[hi < w ? hi : w - 1];
+    const T d = b - a;
+    return wgt < (T)0.5 ? fma(wgt, d, a) : fma(-d, (T)1 - wgt, b);
+}
+
+// One wave per series: the series (staged in LDS with the block's others) is
+// copied into the wave's sort row, padded with +inf to a power of two and
+// sorted by a bitonic network (log2(P)(log2(P)+1)/2 compare-exchange stages of
+// P / 2 pairs, the lanes taking pairs in turn).  On the sorted values:
+// out_dup (5, n, dim) = the adjacent equal pairs D, the values in groups of
+// equal values R, the groups G, the sum of the values in groups and the sum of
+// one value per group (has_duplicate, ratio_value_number_to_time_series_length
+// and the reoccurrence counts, torch_features_boolean.py:11-19,
+// torch_features_reocurrance.py:11-161).  With out_cq, the six quantile
+// levels of kCqLevel bound the 15 corridors of change_quantiles
+// (torch_features_change.py:20-139): a change x_{i+1} - x_i counts in a
+// corridor when both samples lie in [v_ql, v_qh] (compared in the series
+// dtype); out_cq (15, 4, n, dim) = mean and population variance of the
+// changes, then of their absolute values, 0 where no change counts.  Sums in
+// double.  A window holding a non-finite value gives NaN throughout.
+template <class T>
+__global__ void __launch_bounds__(kSeriesBlock)
+    sorted_kernel(const T *traj, int64_t n_save, int64_t n, int S, int64_t start, int G, int P,
+                  double *out_dup, double *out_cq) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int64_t w = n_save - start;
+    const int64_t np = n * S;
+    const int64_t p0 = (int64_t)blockIdx.x * G;
+    T *lds = reinterpret_cast<T *>(smem);
+    T *sortbase = reinterpret_cast<T *>(smem + ((int64_t)G * w * sizeof(T) + 15) / 16 * 16);
+    load_series<T>(traj, n, S, start, w, p0, G, np, lds);
+    __syncthreads();
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    T *buf = sortbase + (int64_t)wave * P;
+    for (int g = wave; g < G; g += kSeriesBlock / 64) {
+        const int64_t p = p0 + g;
+        if (p >= np) break;  // wave-uniform
+        const T *x = lds + (int64_t)g * w;
+        bool bad = false;
+        for (int64_t i = lane; i < w; i += 64) bad |= !isfinite((double)x[i]);
+        if (__any(bad)) {
+            if (lane < 5) out_dup[lane * np + p] = NAN;
+            if (out_cq)
+                for (int k = lane; k < kCqCorr * 4; k += 64) out_cq[k * np + p] = NAN;
+            continue;
+        }
+        for (int64_t i = lane; i < P; i += 64) buf[i] = i < w ? x[i] : (T)INFINITY;
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        for (int k = 2; k <= P; k <<= 1) {
+            for (int j = k >> 1; j > 0; j >>= 1) {
+                for (int t = lane; t < (P >> 1); t += 64) {
+                    const int i = ((t & ~(j - 1)) << 1) | (t & (j - 1));
+                    const T a = buf[i], b = buf[i + j];
+                    if ((a > b) == ((i & k) == 0)) {
+                        buf[i] = b;
+                        buf[i + j] = a;
+                    }
+                }
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            }
+        }
+        double nd = 0.0, nr = 0.0, ng = 0.0, sp = 0.0, sv = 0.0;
+        for (int64_t i = lane; i < w; i += 64) {
+            const T v = buf[i];
+            const bool prev = i > 0 && buf[i - 1] == v;
+            const bool next = i + 1 < w && buf[i + 1] == v;
+            nd += prev ? 1.0 : 0.0;
+            if (prev || next) {
+                nr += 1.0;
+                sp += (double)v;
+            }
+            if (next && !prev) {
+                ng += 1.0;
+                sv += (double)v;
+            }
+        }
+        nd = wave_sum(nd);
+        nr = wave_sum(nr);
+        ng = wave_sum(ng);
+        sp = wave_sum(sp);
+        sv = wave_sum(sv);
+        if (lane == 0) {
+            out_dup[0 * np + p] = nd;
+            out_dup[1 * np + p] = nr;
+            out_dup[2 * np + p] = ng;
+            out_dup[3 * np + p] = sp;
+            out_dup[4 * np + p] = sv;
+        }
+        if (out_cq) {
+            T lv[6];
+#pragma unroll
+            for (int j = 0; j < 6; j++) lv[j] = torch_quantile<T>(buf, w, kCqLevel[j]);
+            // one lower level at a time (its corridors to the upper levels
+            // b > a), so that few accumulators are live
+            for (int a = 0; a < 5; a++) {
+                const T la = lv[a];
+                double cnt[5], sd[5], sa[5];
+#pragma unroll
+                for (int b = 0; b < 5; b++) cnt[b] = sd[b] = sa[b] = 0.0;
+                // pass A: counts and sums of the changes in each corridor
+                for (int64_t i = lane; i + 1 < w; i += 64) {
+                    const T u = x[i], v = x[i + 1];
+                    const double d = (double)v - (double)u, ad = fabs(d);
+                    const T lo = u < v ? u : v, hi = u < v ? v : u;
+                    if (!(lo >= la)) continue;
+#pragma unroll
+                    for (int b = 0; b < 5; b++) {
+                        const bool in = hi <= lv[b + 1];
+                        cnt[b] += in ? 1.0 : 0.0;
+                        sd[b] += in ? d : 0.0;
+                        sa[b] += in ? ad : 0.0;
+                    }
+                }
+                double m[5], ma[5], vd[5], va[5];
+#pragma unroll
+                for (int b = 0; b < 5; b++) {
+                    cnt[b] = wave_sum(cnt[b]);
+                    m[b] = wave_sum(sd[b]) / (cnt[b] + 1e-10);
+                    ma[b] = wave_sum(sa[b]) / (cnt[b] + 1e-10);
+                    vd[b] = va[b] = 0.0;
+                }
+                // pass B: squared deviations about the corridor means
+                for (int64_t i = lane; i + 1 < w; i += 64) {
+                    const T u = x[i], v = x[i + 1];
+                    const double d = (double)v - (double)u, ad = fabs(d);
+                    const T lo = u < v ? u : v, hi = u < v ? v : u;
+                    if (!(lo >= la)) continue;
+#pragma unroll
+                    for (int b = 0; b < 5; b++) {
+                        const bool in = hi <= lv[b + 1];
+                        const double e = d - m[b], ea = ad - ma[b];
+                        vd[b] += in ? e * e : 0.0;
+                        va[b] += in ? ea * ea : 0.0;
+                    }
+                }
+#pragma unroll
+                for (int b = 0; b < 5; b++) {
+                    vd[b] = wave_sum(vd[b]) / (cnt[b] + 1e-10);
+                    va[b] = wave_sum(va[b]) / (cnt[b] + 1e-10);
+                }
+                if (lane == 0) {
+                    const int c0 = a * 5 - a * (a - 1) / 2;  // corridor of (a, a + 1)
+#pragma unroll
+                    for (int b = 0; b < 5; b++) {
+                        if (b < a) continue;  // upper level b + 1 <= a: not a corridor
+                        const int c = c0 + b - a;
+                        const bool any = cnt[b] > 0.0;
+                        out_cq[(c * 4 + 0) * np + p] = any ? m[b] : 0.0;
+                        out_cq[(c * 4 + 1) * np + p] = any ? vd[b] : 0.0;
+                        out_cq[(c * 4 + 2) * np + p] = any ? ma[b] : 0.0;
+                        out_cq[(c * 4 + 3) * np + p] = any ? va[b] : 0.0;
+                    }
+                }
+            }
+        }
+        __builtin_amdgcn_wave_barrier();  // buf reused by the wave's next series
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    }
+}
+
 }  // namespace win
 }  // namespace bh

@@ -24,9 +24,7 @@ from typing import Any
 import torch
 
 from .solvers.hip_dopri5 import FUSED_RESULTS, FusedResult, steady_index
-from .window_features import (COMPREHENSIVE_FEATURES, SPECTRUM_FEATURES, check_params,
-                              comprehensive_feature, spectrum_feature, spectrum_needs,
-                              window_feature_block, window_spectrum)
+from .window_features import COMPREHENSIVE_FEATURES, WindowBlocks, check_params
 
 DEFAULT_STEADY_FRACTION = 0.85
 
@@ -340,27 +338,16 @@ class StatFeatureExtractor(FeatureExtractor):
     def _columns(self, plan, stats: torch.Tensor, y: torch.Tensor | None,
                  start: int) -> torch.Tensor:
         cols = []
-        blk = med = spec = None
-        memo: dict = {}
+        wb = None
         for s, name, params in plan:
-            if is_comprehensive(name, params) and name in SPECTRUM_FEATURES:
+            if is_comprehensive(name, params):
                 if y is None:
                     raise ValueError(f"{name} needs the saved window: integrate with "
                                      "keep_trajectories=True")
-                if spec is None:
-                    spec = window_spectrum(y, start, *spectrum_needs(plan))
-                cols.append(spectrum_feature(name, params, *spec, y.shape[0] - start)[:, s].to(
-                    stats.device, torch.float64))
-            elif is_comprehensive(name, params):
-                if y is None:
-                    raise ValueError(f"{name} needs the saved window: integrate with "
-                                     "keep_trajectories=True")
-                if blk is None:
-                    blk = window_feature_block(y, start)
-                if name == "symmetry_looking" and med is None:
-                    med = window_feature("median", None, y, start)
-                cols.append(comprehensive_feature(name, params, blk, y.shape[0] - start, med,
-                                                  memo)[:, s].to(stats.device, torch.float64))
+                if wb is None:
+                    wb = WindowBlocks(y, start, plan,
+                                      median=lambda: window_feature("median", None, y, start))
+                cols.append(wb.feature(name, params)[:, s].to(stats.device, torch.float64))
             elif name in WINDOW_FEATURES:
                 if y is None:
                     raise ValueError(f"{name} needs the saved window: integrate with "

@@ -89,6 +89,21 @@ SPECTRUM_FEATURES: dict[str, list[dict]] = {
     "fourier_entropy": [{"bins": x} for x in [2, 3, 5, 10, 100]],
 }
 COMPREHENSIVE_FEATURES.update(SPECTRUM_FEATURES)
+# the calculators formed from the sorted window (bh_window_sorted)
+_CQ_LEVELS = [0.0, 0.2, 0.4, 0.6, 0.8, 1.0]
+_CQ_CORRIDORS = [(a, b) for a in range(5) for b in range(a + 1, 6)]
+SORTED_FEATURES: dict[str, list[dict] | None] = {
+    "has_duplicate": None,
+    "ratio_value_number_to_time_series_length": None,
+    "percentage_of_reoccurring_datapoints_to_all_datapoints": None,
+    "percentage_of_reoccurring_values_to_all_values": None,
+    "sum_of_reoccurring_data_points": None,
+    "sum_of_reoccurring_values": None,
+    "change_quantiles": [{"ql": ql, "qh": qh, "isabs": b, "f_agg": f}
+                         for ql in [0.0, 0.2, 0.4, 0.6, 0.8] for qh in [0.2, 0.4, 0.6, 0.8, 1.0]
+                         for b in [False, True] for f in ["mean", "var"] if ql < qh],
+}
+COMPREHENSIVE_FEATURES.update(SORTED_FEATURES)
 # the largest rfft bin index bh_window_spectrum writes
 MAX_SPECTRUM_COEFF = 4095
 # calculators whose parameter may take any value (formed from the slots by
@@ -130,6 +145,12 @@ def check_params(name: str, params: dict | None) -> None:
         return
     if name in ("fft_aggregated", "fourier_entropy"):
         return
+    if name == "change_quantiles":
+        if params.get("ql") in _CQ_LEVELS and params.get("qh") in _CQ_LEVELS and \
+                params["ql"] < params["qh"] and params.get("f_agg", "mean") in ("mean", "var"):
+            return
+        raise ValueError(f"change_quantiles: ql < qh must be levels {_CQ_LEVELS} and f_agg "
+                         f"mean or var, got {params}")
     if params not in grid:
         raise ValueError(f"{name}: parameters {params} are not reduced by the window kernel; "
                          f"supported: {grid}")
@@ -529,3 +550,156 @@ def spectrum_feature(name: str, params: dict | None, coeff: torch.Tensor | None,
     if name == "fourier_entropy":  # entropy_complexity.py:97-107; bins is not used
         return agg[4]
     raise KeyError(f"feature {name!r} is not a spectrum feature")
+
+
+def _torch_quantile_sorted(srt: torch.Tensor, q: float) -> torch.Tensor:
+    """torch.quantile's linear interpolation of sorted rows (N, ...) in
+    their dtype (rank = q (N - 1) in that dtype, then lerp)."""
+    N = srt.shape[0]
+    rank = torch.tensor(q, dtype=srt.dtype) * (N - 1)
+    lo = int(rank)
+    hi = min(int(torch.ceil(rank)), N - 1)
+    return torch.lerp(srt[lo], srt[hi], rank - lo)
+
+
+def _sorted_cpu(w: torch.Tensor, cq: bool):
+    """The outputs of bh_window_sorted for a CPU window (N, B, S), in torch."""
+    N = w.shape[0]
+    srt = w.sort(0).values
+    x = srt.double()
+    ok = torch.isfinite(x).all(0)
+    eq = srt[1:] == srt[:-1]
+    z = torch.zeros((1,) + tuple(w.shape[1:]), dtype=torch.bool)
+    prev = torch.cat([z, eq])
+    nxt = torch.cat([eq, z])
+    grp = prev | nxt
+    first = nxt & ~prev
+    dup = torch.stack([eq.double().sum(0), grp.double().sum(0), first.double().sum(0),
+                       (x * grp).sum(0), (x * first).sum(0)])
+    dup = torch.where(ok, dup, torch.full_like(dup, float("nan")))
+    out_cq = None
+    if cq:
+        lv = [_torch_quantile_sorted(srt, q) for q in _CQ_LEVELS]
+        d = (w[1:].double() - w[:-1].double())
+        lo, hi = torch.minimum(w[1:], w[:-1]), torch.maximum(w[1:], w[:-1])
+        rows = []
+        for a, b in _CQ_CORRIDORS:
+            m = ((lo >= lv[a]) & (hi <= lv[b])).double()
+            c = m.sum(0)
+            for v in (d, d.abs()):
+                mean = (v * m).sum(0) / (c + 1e-10)
+                var = ((v - mean) ** 2 * m).sum(0) / (c + 1e-10)
+                rows += [torch.where(c > 0, mean, torch.zeros_like(mean)),
+                         torch.where(c > 0, var, torch.zeros_like(var))]
+        out_cq = torch.stack(rows)
+        out_cq = torch.where(ok, out_cq, torch.full_like(out_cq, float("nan")))
+    return dup, out_cq
+
+
+def window_sorted(y: torch.Tensor, start: int, cq: bool):
+    """(dup (5, B, S), cq (60, B, S) or None), float64: the sorted-window
+    statistics of bh_window_sorted for the window rows [start, N) of y
+    (N, B, S) (the kernel on the device, torch on the CPU).  dup = adjacent
+    equal pairs, values in groups of equal values, groups, sum of the values
+    in groups, sum of one value per group; cq row 4 c + k = corridor c of
+    _CQ_CORRIDORS, k = mean, var of the changes, mean, var of |changes|."""
+    N, B, S = y.shape
+    if not 0 <= start < N:
+        raise ValueError("empty feature window")
+    if y.device.type != "cuda":
+        return _sorted_cpu(y[start:], cq)
+    y = y.contiguous()
+    dup = torch.empty((5, B, S), dtype=torch.float64, device=y.device)
+    out_cq = torch.empty((60, B, S), dtype=torch.float64, device=y.device) if cq else None
+    dt = _lib.BH_F32 if y.dtype == torch.float32 else _lib.BH_F64
+    s = torch.cuda.current_stream(y.device)
+    with torch.cuda.device(y.device):
+        _lib.check(_lib.lib().bh_window_sorted(
+            dt, y.data_ptr(), N, B, S, int(start), dup.data_ptr(),
+            out_cq.data_ptr() if out_cq is not None else None, s.cuda_stream),
+            "bh_window_sorted")
+    return dup, out_cq
+
+
+def sorted_feature(name: str, params: dict | None, dup: torch.Tensor, cq: torch.Tensor | None,
+                   n_win: int) -> torch.Tensor:
+    """Sorted-window feature `name` (B, S) float64 from window_sorted's
+    outputs (torch_features_boolean.py:11-19, torch_features_reocurrance.py:
+    11-161, torch_features_change.py:20-139)."""
+    n = float(n_win)
+    D, R, G, SP, SV = dup
+    if name == "has_duplicate":
+        return (D > 0).double().where(~D.isnan(), D)
+    if name == "ratio_value_number_to_time_series_length":
+        return (n - D) / n
+    if name == "percentage_of_reoccurring_datapoints_to_all_datapoints":
+        return G / ((n - D) + 1e-10)
+    if name == "percentage_of_reoccurring_values_to_all_values":
+        return R / n
+    if name == "sum_of_reoccurring_data_points":
+        return SP
+    if name == "sum_of_reoccurring_values":
+        return SV
+    if name == "change_quantiles":
+        p = params or {}
+        c = _CQ_CORRIDORS.index((_CQ_LEVELS.index(p["ql"]), _CQ_LEVELS.index(p["qh"])))
+        k = (2 if p.get("isabs", True) else 0) + (1 if p.get("f_agg", "mean") == "var" else 0)
+        return cq[4 * c + k]
+    raise KeyError(f"feature {name!r} is not a sorted-window feature")
+
+
+def feature_family(name: str) -> str:
+    """Which kernel output a comprehensive feature is formed from: "block"
+    (bh_window_features), "spectrum" (bh_window_spectrum) or "sorted"
+    (bh_window_sorted)."""
+    if name in SPECTRUM_FEATURES:
+        return "spectrum"
+    if name in SORTED_FEATURES:
+        return "sorted"
+    if name in COMPREHENSIVE_FEATURES:
+        return "block"
+    raise KeyError(f"feature {name!r} is not a comprehensive window feature")
+
+
+class WindowBlocks:
+    """The kernel outputs of one kept window (rows [start, N) of y), each
+    computed once, when a feature of its family is first asked for: the
+    window-feature slots, the spectrum (as many bins as the plan's spectrum
+    features need), the sorted-window statistics.  median: a callable giving
+    the window median (for symmetry_looking)."""
+
+    def __init__(self, y: torch.Tensor, start: int, plan, median=None):
+        self.y, self.start, self.n = y, int(start), y.shape[0] - int(start)
+        self.plan = list(plan)
+        self._median_fn = median
+        self._out: dict = {}
+        self.memo: dict = {}
+
+    def _get(self, key, make):
+        if key not in self._out:
+            self._out[key] = make()
+        return self._out[key]
+
+    def block(self) -> torch.Tensor:
+        return self._get("block", lambda: window_feature_block(self.y, self.start))
+
+    def spectrum(self):
+        return self._get("spectrum", lambda: window_spectrum(self.y, self.start,
+                                                             *spectrum_needs(self.plan)))
+
+    def sorted(self):
+        cq = any(name == "change_quantiles" for _, name, _ in self.plan)
+        return self._get("sorted", lambda: window_sorted(self.y, self.start, cq))
+
+    def feature(self, name: str, params: dict | None) -> torch.Tensor:
+        """Comprehensive feature `name` (B, S) float64 of the window."""
+        if name in SPECTRUM_FEATURES:
+            return spectrum_feature(name, params, *self.spectrum(), self.n)
+        if name in SORTED_FEATURES:
+            return sorted_feature(name, params, *self.sorted(), self.n)
+        med = None
+        if name == "symmetry_looking":
+            if self._median_fn is None:
+                raise ValueError("symmetry_looking needs the window median")
+            med = self._get("median", self._median_fn)
+        return comprehensive_feature(name, params, self.block(), self.n, med, self.memo)

@@ -145,11 +145,11 @@ def _check_against_oracle(blk, w64, start, tag, series_dtype=np.float64):
     N, B, S = x.shape
     med = torch.tensor(np.sort(x, axis=0)[(N - 1) // 2])
     bad = []
-    from pybasin_amd.window_features import SPECTRUM_FEATURES
+    from pybasin_amd.window_features import feature_family
 
     for name, grid in COMPREHENSIVE_FEATURES.items():
-        if name in SPECTRUM_FEATURES:
-            continue  # bh_window_spectrum, checked below
+        if feature_family(name) != "block":
+            continue  # the spectrum and sorted-window kernels, checked below
         for p in (grid or [None]):
             if name in ("skewness", "kurtosis") and N <= 3:
                 continue  # noise amplified by the reference's 1e-10 guards
@@ -319,4 +319,84 @@ def test_window_spectrum_on_integrated_duffing(cuda_device, dtype):
     n = x.shape[0]
     want = np.stack([spectrum_feature(nm, p, c, a, n).numpy()[:, s] for s in range(2)
                      for nm, ps in feats.items() for p in ps], 1)
+    np.testing.assert_array_equal(f, want.astype(np.float32))
+
+
+def _check_sorted(dup, cq, x, tag, series_dtype):
+    """bh_window_sorted's features against the oracle, series by series: the
+    counts exactly, sums and change statistics to rel 1e-9."""
+    from pybasin_amd.window_features import SORTED_FEATURES, sorted_feature
+
+    N, B, S = x.shape
+    for name, grid in SORTED_FEATURES.items():
+        for p in (grid or [None]):
+            got = sorted_feature(name, p, dup, cq, N).numpy()
+            for b in range(B):
+                for s in range(S):
+                    want = O.comprehensive(x[:, b, s], name, p, series_dtype)
+                    g = float(got[b, s])
+                    if not math.isfinite(want):
+                        assert not math.isfinite(g), (tag, name, p, b, s)
+                    else:
+                        assert g == pytest.approx(want, rel=1e-9, abs=1e-12), (tag, name, p, b, s,
+                                                                               g, want)
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_window_sorted_kernel_equals_oracle_synthetic(cuda_device, dtype):
+    """Duplicate / reoccurrence counts and the 60 change quantiles of the
+    bitonic-sorted windows: ties, constants, ramps, short and non-finite
+    windows, windows of 2,048 (float32) and 1,024 (float64) rows; the counts
+    of the CPU path equal the kernel's."""
+    from pybasin_amd.window_features import window_sorted
+
+    rng = np.random.default_rng(31)
+    big = 2048 if dtype == torch.float32 else 1024
+    cases = list(_synthetic_windows()) + [("big", np.round(rng.normal(size=(big, 7, 2)) * 8) / 8)]
+    for tag, w in cases:
+        wt = torch.tensor(w, dtype=dtype)
+        sd = np.float32 if dtype == torch.float32 else np.float64
+        for start in (0, 1) if w.shape[0] > 1 else (0,):
+            dup, cq = window_sorted(wt.cuda(), start, True)
+            dup, cq = dup.cpu(), cq.cpu()
+            _check_sorted(dup, cq, wt[start:].numpy().astype(sd), f"{tag}/{start}", sd)
+            cdup, _ = window_sorted(wt, start, False)
+            assert torch.equal(torch.nan_to_num(dup[:3], 7.0), torch.nan_to_num(cdup[:3], 7.0)), tag
+            dup2, none = window_sorted(wt.cuda(), start, False)
+            assert none is None and torch.equal(torch.nan_to_num(dup2.cpu(), 7.0),
+                                                torch.nan_to_num(dup, 7.0)), tag
+
+
+def test_window_sorted_limit_is_an_error(cuda_device):
+    from pybasin_amd.window_features import window_sorted
+
+    with pytest.raises(ValueError, match="LDS budget"):
+        window_sorted(torch.zeros(1025, 4, 1, dtype=torch.float64, device="cuda"), 0, True)
+    with pytest.raises(ValueError, match="LDS budget"):
+        window_sorted(torch.zeros(2049, 4, 1, dtype=torch.float32, device="cuda"), 0, False)
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_window_sorted_on_integrated_duffing(cuda_device, dtype):
+    """Sorted-window features of 2,000 integrated Duffing windows against the
+    oracle (on 64 series) and the CPU path's counts (on all); the extractor's
+    columns equal the kernel's."""
+    from pybasin_amd.window_features import sorted_feature, window_sorted
+
+    res = _duffing(dtype)
+    dup, cq = window_sorted(res.y, res.steady_idx, True)
+    dup, cq = dup.cpu(), cq.cpu()
+    sd = np.float32 if dtype == torch.float32 else np.float64
+    x = res.y.cpu().numpy()[res.steady_idx:]
+    _check_sorted(dup[:, :32], cq[:, :32], x[:, :32], "duffing", sd)
+    cdup, ccq = window_sorted(res.y.cpu(), res.steady_idx, True)
+    assert torch.equal(dup[:3], cdup[:3])
+    np.testing.assert_allclose(cq.numpy(), ccq.numpy(), rtol=1e-9, atol=1e-12)
+    feats = {"change_quantiles": [{"ql": 0.2, "qh": 0.8, "isabs": True, "f_agg": "var"}],
+             "ratio_value_number_to_time_series_length": None}
+    fx = StatFeatureExtractor(time_steady=300.0, features=feats)
+    f = fx.extract_features(Solution(res.y[0], res.t, res.y)).cpu().numpy()
+    n = x.shape[0]
+    want = np.stack([sorted_feature(nm, p, dup, cq, n).numpy()[:, s] for s in range(2)
+                     for nm, ps in feats.items() for p in (ps or [None])], 1)
     np.testing.assert_array_equal(f, want.astype(np.float32))

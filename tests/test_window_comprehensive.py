@@ -23,14 +23,16 @@ import torch
 
 from oracle import oracle as O
 from pybasin_amd.features import StatFeatureExtractor, Solution
-from pybasin_amd.window_features import (COMPREHENSIVE_FEATURES, SPECTRUM_FEATURES, _block_cpu,
-                                         check_params, comprehensive_feature, spectrum_feature,
-                                         window_spectrum)
+from pybasin_amd.window_features import (COMPREHENSIVE_FEATURES, SORTED_FEATURES,
+                                         SPECTRUM_FEATURES, _block_cpu, check_params,
+                                         comprehensive_feature, feature_family, sorted_feature,
+                                         spectrum_feature, window_sorted, window_spectrum)
 
 
 def _all_params():
+    """The calculators formed from the window-feature slots."""
     for name, grid in COMPREHENSIVE_FEATURES.items():
-        if name in SPECTRUM_FEATURES:
+        if feature_family(name) != "block":
             continue
         for p in (grid or [None]):
             yield name, p
@@ -287,6 +289,72 @@ def test_cpu_spectrum_path_equals_oracle(dtype):
                         assert _angle_ok(float(coeff[min(k, coeff.shape[0] - 1), 0, b, s]),
                                          float(coeff[min(k, coeff.shape[0] - 1), 1, b, s]),
                                          x, k, scale), (N, k, b, s)
+
+
+def test_oracle_sorted_family_known_answers():
+    """Hand-counted multiplicities and a change-quantile corridor whose
+    bounds equal sample values (a strict comparison would drop them)."""
+    C = O.comprehensive
+    x = np.array([1.0, 2, 2, 3, 3, 3, 5])  # distinct 1, 2, 3, 5; 2 and 3 reoccur
+    assert C(x, "has_duplicate") == 1.0
+    assert C(x, "ratio_value_number_to_time_series_length") == pytest.approx(4 / 7)
+    assert C(x, "percentage_of_reoccurring_datapoints_to_all_datapoints") == pytest.approx(2 / 4)
+    assert C(x, "percentage_of_reoccurring_values_to_all_values") == pytest.approx(5 / 7)
+    assert C(x, "sum_of_reoccurring_data_points") == 13.0
+    assert C(x, "sum_of_reoccurring_values") == 5.0
+    # quantiles 0.2 and 0.8 of 7 values: ranks 1.2 and 4.8 -> 2 and 3; the
+    # samples 2, 2, 3, 3, 3 lie in [2, 3]; their changes 0, 1, 0, 0
+    cq = lambda isabs, f: C(x, "change_quantiles", {"ql": 0.2, "qh": 0.8, "isabs": isabs,  # noqa: E731
+                                                     "f_agg": f})
+    assert cq(True, "mean") == pytest.approx(0.25)
+    assert cq(True, "var") == pytest.approx(0.1875)
+    y = np.array([0.0, 4.0, 1.0, 3.0, 2.0])  # changes 4, -3, 2, -1
+    # ql = 0 and qh = 1 take every change
+    assert C(y, "change_quantiles", {"ql": 0.0, "qh": 1.0, "isabs": False, "f_agg": "mean"}) == \
+        pytest.approx(0.5)
+    assert C(y, "change_quantiles", {"ql": 0.0, "qh": 1.0, "isabs": True, "f_agg": "mean"}) == \
+        pytest.approx(2.5)
+    # quantile 0.6 of 5 values: rank 2.4 -> 2.4; each adjacent pair (0, 4), (4, 1),
+    # (1, 3), (3, 2) holds a sample above it, so no change counts
+    assert C(y, "change_quantiles", {"ql": 0.0, "qh": 0.6, "isabs": True, "f_agg": "mean"}) == 0.0
+    d = np.arange(12.0)
+    assert C(d, "has_duplicate") == 0.0 and C(d, "sum_of_reoccurring_values") == 0.0
+    assert C(np.full(9, 2.5), "percentage_of_reoccurring_values_to_all_values") == 1.0
+    assert C(np.full(9, 2.5), "sum_of_reoccurring_data_points") == 22.5
+    assert C(np.array([3.0]), "percentage_of_reoccurring_values_to_all_values") == 0.0
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_oracle_quantile_matches_torch(dtype):
+    """The oracle's quantile (the corridor bounds of change_quantiles)
+    equals torch.quantile in the series dtype, ties and odd sizes included."""
+    rng = np.random.default_rng(8)
+    for n in (1, 2, 5, 7, 100, 999, 1000):
+        for x in (rng.normal(size=n), np.round(rng.normal(size=n) * 2) / 2):
+            xs = x.astype(dtype)
+            srt = np.sort(xs)
+            for q in (0.0, 0.1, 0.2, 0.4, 0.6, 0.8, 0.9, 1.0):
+                want = torch.quantile(torch.from_numpy(xs), q).item()
+                got = O._torch_quantile(srt, q, dtype)
+                assert got == pytest.approx(want, rel=4 * np.finfo(dtype).eps, abs=0), (n, q)
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_cpu_sorted_path_equals_oracle(dtype):
+    for w in _windows():
+        wt = torch.tensor(w, dtype=dtype)
+        x64 = wt.double().numpy()
+        N, B, S = w.shape
+        dup, cq = window_sorted(wt, 0, True)
+        for name, grid in SORTED_FEATURES.items():
+            for p in (grid or [None]):
+                got = sorted_feature(name, p, dup, cq, N).numpy()
+                for b in range(B):
+                    for s in range(S):
+                        want = O.comprehensive(wt[:, b, s].numpy(), name, p,
+                                               np.float32 if dtype == torch.float32 else np.float64)
+                        assert got[b, s] == pytest.approx(want, rel=1e-9, abs=1e-12), \
+                            (name, p, N, b, s)
 
 
 def _windows():
