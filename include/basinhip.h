@@ -487,23 +487,31 @@ int32_t bh_window_spectrum(int32_t dtype, const void *traj, int64_t n_save, int6
                            int32_t dim, int64_t start, int32_t n_coeff, double *out_coeff,
                            double *out_agg, void *stream);
 
-/* Statistics of the sorted values of each series of the same window, in
-   double: out_dup (5, n, dim) float64 = the adjacent equal pairs D of the
+/* Statistics of the sorted values of each series of the same window (sorted
+   in LDS): out_dup (5, n, dim) float64 = the adjacent equal pairs D of the
    sorted series, the values that lie in groups of equal values R, the number
    of such groups, the sum of the values in groups and the sum of one value per
-   group (has_duplicate, ratio_value_number_to_time_series_length and the four
-   reoccurrence calculators: src/pybasin/ts_torch/calculators/
+   group, in double (has_duplicate, ratio_value_number_to_time_series_length
+   and the four reoccurrence calculators: src/pybasin/ts_torch/calculators/
    torch_features_boolean.py:11-19, torch_features_reocurrance.py:11-161).
-   out_cq (15, 4, n, dim) float64, or NULL: change_quantiles
-   (torch_features_change.py:20-139) for the 15 corridors ql < qh of the
-   levels 0, 0.2, .., 1 in the order (0, .2), (0, .4), .., (0, 1), (.2, .4),
-   .., (.8, 1): mean and population variance of the changes x_{i+1} - x_i
-   whose two samples lie in [quantile(ql), quantile(qh)], then of their
-   absolute values (0 where none does); the quantiles as torch.quantile forms
-   them in the series dtype.  Non-finite windows give NaN.  Windows up to
-   2,048 float32 or 1,024 float64 rows. */
+   out_levels (6, n, dim) in the series dtype, or NULL: torch.quantile of the
+   window at 0, 0.2, 0.4, 0.6, 0.8, 1 (linear interpolation in the series
+   dtype), the corridor bounds of bh_window_change_quantiles.  Non-finite
+   windows give NaN.  Windows up to 2,048 float32 or 1,024 float64 rows. */
 int32_t bh_window_sorted(int32_t dtype, const void *traj, int64_t n_save, int64_t n, int32_t dim,
-                         int64_t start, double *out_dup, double *out_cq, void *stream);
+                         int64_t start, double *out_dup, void *out_levels, void *stream);
+
+/* change_quantiles (torch_features_change.py:20-139) of each series of the
+   same window for the 15 corridors ql < qh of the levels 0, 0.2, .., 1 in the
+   order (0, .2), (0, .4), .., (0, 1), (.2, .4), .., (.8, 1), given their
+   bounds `levels` (6, n, dim) in the series dtype (bh_window_sorted's
+   out_levels): out (15, 4, n, dim) float64 = mean and population variance of
+   the changes x_{i+1} - x_i whose two samples lie in [level(ql), level(qh)],
+   then of their absolute values (0 where none does; NaN where a level is
+   NaN). */
+int32_t bh_window_change_quantiles(int32_t dtype, const void *traj, int64_t n_save, int64_t n,
+                                   int32_t dim, int64_t start, const void *levels, double *out,
+                                   void *stream);
 
 /* ---- label layer (replaces the sklearn calls of the reference's
    HDBSCANClusterer, src/pybasin/predictors/hdbscan_clusterer.py:12-148, and

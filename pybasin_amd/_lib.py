@@ -46,7 +46,8 @@ EXPORTS = ("bh_version", "bh_last_error", "bh_solve_batch", "bh_solve_batch_peak
            "bh_hdbscan_workspace_bytes", "bh_core_distances", "bh_hdbscan_mst", "bh_hdbscan_labels",
            "bh_silhouette",
            "bh_window_order_stats", "bh_window_autocorr_periodicity", "bh_window_spectral_ratio",
-           "bh_window_features", "bh_window_spectrum", "bh_window_sorted")
+           "bh_window_features", "bh_window_spectrum", "bh_window_sorted",
+           "bh_window_change_quantiles")
 
 _lock = threading.Lock()
 _lib: ctypes.CDLL | None = None
@@ -119,6 +120,7 @@ def _declare(L: ctypes.CDLL) -> None:
         "bh_window_features": [i32, vp, i64, i64, i32, i64, vp, vp],
         "bh_window_spectrum": [i32, vp, i64, i64, i32, i64, i32, vp, vp, vp],
         "bh_window_sorted": [i32, vp, i64, i64, i32, i64, vp, vp, vp],
+        "bh_window_change_quantiles": [i32, vp, i64, i64, i32, i64, vp, vp, vp],
     }
     for name, args in sigs.items():
         fn = getattr(L, name)

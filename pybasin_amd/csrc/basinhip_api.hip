@@ -896,8 +896,29 @@ int32_t bh_window_spectrum(int32_t dtype, const void *traj, int64_t n_save, int6
     return 0;
 }
 
+extern "C++" template <class T>
+static void launch_sorted(int64_t P, int64_t blocks, size_t lds, hipStream_t s, const T *traj,
+                          int64_t n_save, int64_t n, int32_t dim, int64_t start, int64_t G,
+                          double *out_dup, T *out_levels) {
+    // values per lane: P / 64 (P >= 64)
+    const int R = (int)(P / 64);
+#define BH_SORTED_LAUNCH(RR)                                                                   \
+    hipLaunchKernelGGL((bh::win::sorted_kernel<T, RR>), dim3((unsigned)blocks),                 \
+                       dim3(bh::win::kSeriesBlock), lds, s, traj, n_save, n, dim, start, (int)G, \
+                       out_dup, out_levels)
+    switch (R) {
+        case 1: BH_SORTED_LAUNCH(1); break;
+        case 2: BH_SORTED_LAUNCH(2); break;
+        case 4: BH_SORTED_LAUNCH(4); break;
+        case 8: BH_SORTED_LAUNCH(8); break;
+        case 16: BH_SORTED_LAUNCH(16); break;
+        default: BH_SORTED_LAUNCH(32); break;
+    }
+#undef BH_SORTED_LAUNCH
+}
+
 int32_t bh_window_sorted(int32_t dtype, const void *traj, int64_t n_save, int64_t n, int32_t dim,
-                         int64_t start, double *out_dup, double *out_cq, void *stream) {
+                         int64_t start, double *out_dup, void *out_levels, void *stream) {
     g_err[0] = 0;
     int32_t rc = check_series_args("bh_window_sorted", dtype, traj, n_save, n, dim, start);
     if (rc) return rc;
@@ -905,12 +926,12 @@ int32_t bh_window_sorted(int32_t dtype, const void *traj, int64_t n_save, int64_
     if (n == 0) return 0;
     const int64_t w = n_save - start;
     const int64_t elem = dtype == BH_F32 ? 4 : 8;
-    int64_t P = 1;
+    int64_t P = 64;
     while (P < w) P <<= 1;
-    // the block's series rows plus one sort row of P values per wave
+    // the block's series rows plus one sorted row of P values per wave
     const int64_t budget = 62 * 1024;
     const int64_t rows = 4 * P * elem;
-    if (rows + w * elem > budget)
+    if (P > 2048 || rows + w * elem > budget)
         return fail(-5, "bh_window_sorted: window of %lld rows exceeds the LDS budget (at most %d %s rows)",
                     (long long)w, dtype == BH_F32 ? 2048 : 1024, dtype == BH_F32 ? "float32" : "float64");
     int64_t G = (budget - rows) / (w * elem);
@@ -921,13 +942,35 @@ int32_t bh_window_sorted(int32_t dtype, const void *traj, int64_t n_save, int64_
     if (blocks > 2147483647LL) return fail(-5, "bh_window_sorted: too many series");
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     if (dtype == BH_F32)
-        hipLaunchKernelGGL(bh::win::sorted_kernel<float>, dim3((unsigned)blocks),
-                           dim3(bh::win::kSeriesBlock), lds, s, reinterpret_cast<const float *>(traj),
-                           n_save, n, dim, start, (int)G, (int)P, out_dup, out_cq);
+        launch_sorted<float>(P, blocks, lds, s, reinterpret_cast<const float *>(traj), n_save, n, dim,
+                             start, G, out_dup, reinterpret_cast<float *>(out_levels));
     else
-        hipLaunchKernelGGL(bh::win::sorted_kernel<double>, dim3((unsigned)blocks),
-                           dim3(bh::win::kSeriesBlock), lds, s, reinterpret_cast<const double *>(traj),
-                           n_save, n, dim, start, (int)G, (int)P, out_dup, out_cq);
+        launch_sorted<double>(P, blocks, lds, s, reinterpret_cast<const double *>(traj), n_save, n,
+                              dim, start, G, out_dup, reinterpret_cast<double *>(out_levels));
+    BH_HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
+int32_t bh_window_change_quantiles(int32_t dtype, const void *traj, int64_t n_save, int64_t n,
+                                   int32_t dim, int64_t start, const void *levels, double *out,
+                                   void *stream) {
+    g_err[0] = 0;
+    int32_t rc = check_series_args("bh_window_change_quantiles", dtype, traj, n_save, n, dim, start);
+    if (rc) return rc;
+    if (!levels || !out) return fail(-5, "bh_window_change_quantiles: null levels or out");
+    if (n == 0) return 0;
+    const int64_t np = n * dim;
+    const int64_t blocks = (np + 255) / 256;
+    if (blocks > 2147483647LL) return fail(-5, "bh_window_change_quantiles: too many series");
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    if (dtype == BH_F32)
+        hipLaunchKernelGGL(bh::win::change_quantiles_kernel<float>, dim3((unsigned)blocks), dim3(256), 0,
+                           s, reinterpret_cast<const float *>(traj), n_save, n, dim, start,
+                           reinterpret_cast<const float *>(levels), out);
+    else
+        hipLaunchKernelGGL(bh::win::change_quantiles_kernel<double>, dim3((unsigned)blocks), dim3(256),
+                           0, s, reinterpret_cast<const double *>(traj), n_save, n, dim, start,
+                           reinterpret_cast<const double *>(levels), out);
     BH_HIP_CHECK(hipGetLastError());
     return 0;
 }

@@ -753,6 +753,7 @@ __global__ void __launch_bounds__(256) window_features_kernel(const T *traj, int
 // the six quantile levels whose corridors change_quantiles takes
 // (ql in 0 .. 0.8, qh in 0.2 .. 1, ql < qh; settings.py:315-322)
 __constant__ double kCqLevel[6] = {0.0, 0.2, 0.4, 0.6, 0.8, 1.0};
+constexpr int kCqLevels = 6;
 constexpr int kCqCorr = 15;
 
 // torch.quantile's linear interpolation on a sorted series s of w values, in
@@ -771,25 +772,61 @@ __device__ __forceinline__ T torch_quantile(const T *s, int64_t w, double q) {
     return wgt < (T)0.5 ? fma(wgt, d, a) : fma(-d, (T)1 - wgt, b);
 }
 
+// Ascending bitonic sort of the P = 64 R values held R per lane, element e =
+// lane * R + r in v[r], by one wave: a compare-exchange stage (k, j) pairs e
+// with e ^ j and keeps the smaller value at the lower index where e & k is
+// clear (the larger where it is set).  For j < R both elements sit in the
+// lane's registers; for j >= R the partner is the same register of lane
+// lane ^ (j / R), exchanged by a cross-lane shuffle.  All loop bounds are
+// compile-time, so every register index is.
+template <class T, int R>
+__device__ __forceinline__ void wave_bitonic_sort(T (&v)[R], int lane) {
+    constexpr int P = 64 * R;
+#pragma unroll
+    for (int k = 2; k <= P; k <<= 1) {
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            if (j >= R) {
+                const int m = j / R;
+                const bool lower = (lane & m) == 0;
+#pragma unroll
+                for (int r = 0; r < R; r++) {
+                    const bool asc = (((lane * R + r) & k) == 0);
+                    const T o = __shfl_xor(v[r], m, 64);
+                    v[r] = (lower == asc) ? (o < v[r] ? o : v[r]) : (o > v[r] ? o : v[r]);
+                }
+            } else {
+#pragma unroll
+                for (int r = 0; r < R; r++) {
+                    if (r & j) continue;
+                    const bool asc = (((lane * R + r) & k) == 0);
+                    const T a = v[r], b = v[r + j];
+                    const T lo = a < b ? a : b, hi = a < b ? b : a;
+                    v[r] = asc ? lo : hi;
+                    v[r + j] = asc ? hi : lo;
+                }
+            }
+        }
+    }
+}
+
 // One wave per series: the series (staged in LDS with the block's others) is
-// copied into the wave's sort row, padded with +inf to a power of two and
-// sorted by a bitonic network (log2(P)(log2(P)+1)/2 compare-exchange stages of
-// P / 2 pairs, the lanes taking pairs in turn).  On the sorted values:
-// out_dup (5, n, dim) = the adjacent equal pairs D, the values in groups of
-// equal values R, the groups G, the sum of the values in groups and the sum of
-// one value per group (has_duplicate, ratio_value_number_to_time_series_length
-// and the reoccurrence counts, torch_features_boolean.py:11-19,
-// torch_features_reocurrance.py:11-161).  With out_cq, the six quantile
-// levels of kCqLevel bound the 15 corridors of change_quantiles
-// (torch_features_change.py:20-139): a change x_{i+1} - x_i counts in a
-// corridor when both samples lie in [v_ql, v_qh] (compared in the series
-// dtype); out_cq (15, 4, n, dim) = mean and population variance of the
-// changes, then of their absolute values, 0 where no change counts.  Sums in
-// double.  A window holding a non-finite value gives NaN throughout.
-template <class T>
+// taken into registers, R values per lane, padded with +inf to P = 64 R (a
+// power of two), sorted (wave_bitonic_sort) and written to the wave's sort
+// row.  On the sorted values: out_dup (5, n, dim) =
+// the adjacent equal pairs D, the values in groups of equal values R, the
+// groups G, the sum of the values in groups and the sum of one value per group
+// (has_duplicate, ratio_value_number_to_time_series_length and the
+// reoccurrence counts, torch_features_boolean.py:11-19,
+// torch_features_reocurrance.py:11-161), sums in double; out_levels (6, n,
+// dim) in the series dtype, or null: torch.quantile at the levels of
+// kCqLevel (the corridor bounds of change_quantiles).  A window holding a
+// non-finite value gives NaN throughout.
+template <class T, int R>
 __global__ void __launch_bounds__(kSeriesBlock)
-    sorted_kernel(const T *traj, int64_t n_save, int64_t n, int S, int64_t start, int G, int P,
-                  double *out_dup, double *out_cq) {
+    sorted_kernel(const T *traj, int64_t n_save, int64_t n, int S, int64_t start, int G,
+                  double *out_dup, T *out_levels) {
+    constexpr int P = 64 * R;
     extern __shared__ __align__(16) unsigned char smem[];
     const int64_t w = n_save - start;
     const int64_t np = n * S;
@@ -808,27 +845,20 @@ __global__ void __launch_bounds__(kSeriesBlock)
         for (int64_t i = lane; i < w; i += 64) bad |= !isfinite((double)x[i]);
         if (__any(bad)) {
             if (lane < 5) out_dup[lane * np + p] = NAN;
-            if (out_cq)
-                for (int k = lane; k < kCqCorr * 4; k += 64) out_cq[k * np + p] = NAN;
+            if (out_levels && lane < kCqLevels) out_levels[lane * np + p] = (T)NAN;
             continue;
         }
-        for (int64_t i = lane; i < P; i += 64) buf[i] = i < w ? x[i] : (T)INFINITY;
+        T v[R];
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const int64_t e = (int64_t)lane * R + r;
+            v[r] = e < w ? x[e] : (T)INFINITY;
+        }
+        wave_bitonic_sort<T, R>(v, lane);
+#pragma unroll
+        for (int r = 0; r < R; r++) buf[lane * R + r] = v[r];
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        for (int k = 2; k <= P; k <<= 1) {
-            for (int j = k >> 1; j > 0; j >>= 1) {
-                for (int t = lane; t < (P >> 1); t += 64) {
-                    const int i = ((t & ~(j - 1)) << 1) | (t & (j - 1));
-                    const T a = buf[i], b = buf[i + j];
-                    if ((a > b) == ((i & k) == 0)) {
-                        buf[i] = b;
-                        buf[i + j] = a;
-                    }
-                }
-                __builtin_amdgcn_wave_barrier();
-                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-            }
-        }
         double nd = 0.0, nr = 0.0, ng = 0.0, sp = 0.0, sv = 0.0;
         for (int64_t i = lane; i < w; i += 64) {
             const T v = buf[i];
@@ -856,75 +886,95 @@ __global__ void __launch_bounds__(kSeriesBlock)
             out_dup[3 * np + p] = sp;
             out_dup[4 * np + p] = sv;
         }
-        if (out_cq) {
-            T lv[6];
-#pragma unroll
-            for (int j = 0; j < 6; j++) lv[j] = torch_quantile<T>(buf, w, kCqLevel[j]);
-            // one lower level at a time (its corridors to the upper levels
-            // b > a), so that few accumulators are live
-            for (int a = 0; a < 5; a++) {
-                const T la = lv[a];
-                double cnt[5], sd[5], sa[5];
-#pragma unroll
-                for (int b = 0; b < 5; b++) cnt[b] = sd[b] = sa[b] = 0.0;
-                // pass A: counts and sums of the changes in each corridor
-                for (int64_t i = lane; i + 1 < w; i += 64) {
-                    const T u = x[i], v = x[i + 1];
-                    const double d = (double)v - (double)u, ad = fabs(d);
-                    const T lo = u < v ? u : v, hi = u < v ? v : u;
-                    if (!(lo >= la)) continue;
-#pragma unroll
-                    for (int b = 0; b < 5; b++) {
-                        const bool in = hi <= lv[b + 1];
-                        cnt[b] += in ? 1.0 : 0.0;
-                        sd[b] += in ? d : 0.0;
-                        sa[b] += in ? ad : 0.0;
-                    }
-                }
-                double m[5], ma[5], vd[5], va[5];
-#pragma unroll
-                for (int b = 0; b < 5; b++) {
-                    cnt[b] = wave_sum(cnt[b]);
-                    m[b] = wave_sum(sd[b]) / (cnt[b] + 1e-10);
-                    ma[b] = wave_sum(sa[b]) / (cnt[b] + 1e-10);
-                    vd[b] = va[b] = 0.0;
-                }
-                // pass B: squared deviations about the corridor means
-                for (int64_t i = lane; i + 1 < w; i += 64) {
-                    const T u = x[i], v = x[i + 1];
-                    const double d = (double)v - (double)u, ad = fabs(d);
-                    const T lo = u < v ? u : v, hi = u < v ? v : u;
-                    if (!(lo >= la)) continue;
-#pragma unroll
-                    for (int b = 0; b < 5; b++) {
-                        const bool in = hi <= lv[b + 1];
-                        const double e = d - m[b], ea = ad - ma[b];
-                        vd[b] += in ? e * e : 0.0;
-                        va[b] += in ? ea * ea : 0.0;
-                    }
-                }
-#pragma unroll
-                for (int b = 0; b < 5; b++) {
-                    vd[b] = wave_sum(vd[b]) / (cnt[b] + 1e-10);
-                    va[b] = wave_sum(va[b]) / (cnt[b] + 1e-10);
-                }
-                if (lane == 0) {
-                    const int c0 = a * 5 - a * (a - 1) / 2;  // corridor of (a, a + 1)
-#pragma unroll
-                    for (int b = 0; b < 5; b++) {
-                        if (b < a) continue;  // upper level b + 1 <= a: not a corridor
-                        const int c = c0 + b - a;
-                        const bool any = cnt[b] > 0.0;
-                        out_cq[(c * 4 + 0) * np + p] = any ? m[b] : 0.0;
-                        out_cq[(c * 4 + 1) * np + p] = any ? vd[b] : 0.0;
-                        out_cq[(c * 4 + 2) * np + p] = any ? ma[b] : 0.0;
-                        out_cq[(c * 4 + 3) * np + p] = any ? va[b] : 0.0;
-                    }
-                }
-            }
-        }
+        if (out_levels && lane < kCqLevels) out_levels[lane * np + p] = torch_quantile<T>(buf, w, kCqLevel[lane]);
         __builtin_amdgcn_wave_barrier();  // buf reused by the wave's next series
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    }
+}
+
+// change_quantiles (torch_features_change.py:20-139) for the 15 corridors
+// [v_a, v_b], a < b, of the six levels of kCqLevel, one lane per series: a
+// change d_i = x_{i+1} - x_i counts in a corridor when both samples lie in it
+// (compared in the series dtype with the levels of bh_window_sorted).  Pass 1
+// counts and sums d and |d| per corridor; pass 2 sums the squared deviations
+// from the corridor means.  out (15, 4, n, dim): mean, population variance of
+// d, then of |d|, 0 for a corridor no change falls in; NaN where a level is
+// NaN (a non-finite window).  Sums in double.
+template <class T>
+__global__ void __launch_bounds__(256)
+    change_quantiles_kernel(const T *traj, int64_t n_save, int64_t n, int S, int64_t start,
+                            const T *levels, double *out) {
+    const int64_t np = n * S;
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= np) return;
+    T lv[kCqLevels];
+    bool ok = true;
+#pragma unroll
+    for (int j = 0; j < kCqLevels; j++) {
+        lv[j] = levels[j * np + p];
+        ok &= !isnan((double)lv[j]);
+    }
+    if (!ok) {
+        for (int k = 0; k < kCqCorr * 4; k++) out[k * np + p] = NAN;
+        return;
+    }
+    const T *col = traj + start * np + p;
+    const int64_t w = n_save - start;
+    double cnt[kCqCorr], s1[kCqCorr], sa[kCqCorr];
+#pragma unroll
+    for (int c = 0; c < kCqCorr; c++) cnt[c] = s1[c] = sa[c] = 0.0;
+    T u = col[0];
+    for (int64_t i = 1; i < w; i++) {
+        const T v = col[i * np];
+        const double d = (double)v - (double)u, ad = fabs(d);
+        const T lo = u < v ? u : v, hi = u < v ? v : u;
+        u = v;
+        int c = 0;
+#pragma unroll
+        for (int a = 0; a < kCqLevels - 1; a++) {
+            const bool ge = lo >= lv[a];
+#pragma unroll
+            for (int b = a + 1; b < kCqLevels; b++, c++) {
+                const bool in = ge && hi <= lv[b];
+                cnt[c] += in ? 1.0 : 0.0;
+                s1[c] += in ? d : 0.0;
+                sa[c] += in ? ad : 0.0;
+            }
+        }
+    }
+    double vd[kCqCorr], va[kCqCorr];
+#pragma unroll
+    for (int c = 0; c < kCqCorr; c++) {
+        s1[c] /= cnt[c] + 1e-10;  // the corridor means
+        sa[c] /= cnt[c] + 1e-10;
+        vd[c] = va[c] = 0.0;
+    }
+    u = col[0];
+    for (int64_t i = 1; i < w; i++) {
+        const T v = col[i * np];
+        const double d = (double)v - (double)u, ad = fabs(d);
+        const T lo = u < v ? u : v, hi = u < v ? v : u;
+        u = v;
+        int c = 0;
+#pragma unroll
+        for (int a = 0; a < kCqLevels - 1; a++) {
+            const bool ge = lo >= lv[a];
+#pragma unroll
+            for (int b = a + 1; b < kCqLevels; b++, c++) {
+                const bool in = ge && hi <= lv[b];
+                const double e = d - s1[c], ea = ad - sa[c];
+                vd[c] += in ? e * e : 0.0;
+                va[c] += in ? ea * ea : 0.0;
+            }
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < kCqCorr; c++) {
+        const bool any = cnt[c] > 0.0;
+        out[(c * 4 + 0) * np + p] = any ? s1[c] : 0.0;
+        out[(c * 4 + 1) * np + p] = any ? vd[c] / (cnt[c] + 1e-10) : 0.0;
+        out[(c * 4 + 2) * np + p] = any ? sa[c] : 0.0;
+        out[(c * 4 + 3) * np + p] = any ? va[c] / (cnt[c] + 1e-10) : 0.0;
     }
 }
 

@@ -610,14 +610,19 @@ def window_sorted(y: torch.Tensor, start: int, cq: bool):
         return _sorted_cpu(y[start:], cq)
     y = y.contiguous()
     dup = torch.empty((5, B, S), dtype=torch.float64, device=y.device)
+    levels = torch.empty((6, B, S), dtype=y.dtype, device=y.device) if cq else None
     out_cq = torch.empty((60, B, S), dtype=torch.float64, device=y.device) if cq else None
     dt = _lib.BH_F32 if y.dtype == torch.float32 else _lib.BH_F64
     s = torch.cuda.current_stream(y.device)
     with torch.cuda.device(y.device):
         _lib.check(_lib.lib().bh_window_sorted(
             dt, y.data_ptr(), N, B, S, int(start), dup.data_ptr(),
-            out_cq.data_ptr() if out_cq is not None else None, s.cuda_stream),
+            levels.data_ptr() if levels is not None else None, s.cuda_stream),
             "bh_window_sorted")
+        if cq:
+            _lib.check(_lib.lib().bh_window_change_quantiles(
+                dt, y.data_ptr(), N, B, S, int(start), levels.data_ptr(), out_cq.data_ptr(),
+                s.cuda_stream), "bh_window_change_quantiles")
     return dup, out_cq
 
 
