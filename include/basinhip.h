@@ -487,6 +487,16 @@ int32_t bh_window_spectrum(int32_t dtype, const void *traj, int64_t n_save, int6
                            int32_t dim, int64_t start, int32_t n_coeff, double *out_coeff,
                            double *out_agg, void *stream);
 
+/* Autocovariances of each series of the same window: out (max_lag + 1, n,
+   dim) float64 holds C_k = sum_{i < w - k} (x_i - mean)(x_{i+k} - mean) for
+   k = 0 .. max_lag (0 for k >= w), in double: the linear autocorrelation of
+   the centred window that agg_autocorrelation and ar_coefficient form by FFT
+   (src/pybasin/ts_torch/calculators/torch_features_autocorrelation.py:72-96,
+   torch_features_trend.py:86-131).  0 <= max_lag <= 4095; windows that fit
+   the 62 KB LDS budget. */
+int32_t bh_window_autocov(int32_t dtype, const void *traj, int64_t n_save, int64_t n, int32_t dim,
+                          int64_t start, int32_t max_lag, double *out, void *stream);
+
 /* Statistics of the sorted values of each series of the same window (sorted
    in LDS): out_dup (5, n, dim) float64 = the adjacent equal pairs D of the
    sorted series, the values that lie in groups of equal values R, the number

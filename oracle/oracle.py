@@ -581,4 +581,42 @@ def comprehensive(x: np.ndarray, name: str, params: dict | None = None, dtype=np
         if p.get("f_agg", "mean") == "var":
             return float(((d[ok] - mean) ** 2).sum() / (c + 1e-10))
         return float(mean)
+    if name in ("agg_autocorrelation", "ar_coefficient"):
+        # linear autocovariances of the centred window (the reference's
+        # zero-padded FFT product; torch_features_autocorrelation.py:72-96,
+        # torch_features_trend.py:86-131)
+        xc = x - mu
+        kmax = int(p.get("maxlag", 40)) if name == "agg_autocorrelation" else int(p.get("k", 1))
+        C = np.array([(xc[: n - k] * xc[k:]).sum() if k < n else 0.0 for k in range(kmax + 1)])
+        acf = C / (C[0] + 1e-10)
+        if name == "agg_autocorrelation":
+            L = min(kmax, n - 1)
+            a = acf[1:L + 1]
+            if a.size == 0:
+                return float("nan")
+            f = p.get("f_agg", "mean")
+            if f == "median":
+                return float(np.sort(a)[(L - 1) // 2])  # torch.median: the lower middle
+            if f == "var":
+                return float(a.var())
+            return float(a.mean())
+        k, coeff = kmax, int(p.get("coeff", 0))
+        if coeff >= k:
+            return 0.0
+        R = np.array([[acf[abs(i - j)] for j in range(k)] for i in range(k)]) + 1e-6 * np.eye(k)
+        return float(np.linalg.solve(R, acf[1:k + 1])[coeff])
+    if name == "augmented_dickey_fuller":  # torch_features_trend.py:134-172
+        if p.get("attr", "teststat") == "usedlag":
+            return 1.0
+        dif = np.diff(x)
+        lag = x[:-1]
+        xm, ym = lag.mean(), dif.mean()
+        ss_xx = ((lag - xm) ** 2).sum()
+        ss_xy = ((lag - xm) * (dif - ym)).sum()
+        beta = ss_xy / (ss_xx + 1e-10)
+        resid = dif - (ym + beta * (lag - xm))
+        mse = (resid ** 2).sum() / (n - 3)
+        se = np.sqrt(mse) / (np.sqrt(ss_xx) + 1e-10)
+        t = beta / (se + 1e-10)
+        return float(1 / (1 + abs(t))) if p.get("attr") == "pvalue" else float(t)
     raise KeyError(name)

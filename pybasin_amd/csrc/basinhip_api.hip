@@ -896,6 +896,36 @@ int32_t bh_window_spectrum(int32_t dtype, const void *traj, int64_t n_save, int6
     return 0;
 }
 
+int32_t bh_window_autocov(int32_t dtype, const void *traj, int64_t n_save, int64_t n, int32_t dim,
+                          int64_t start, int32_t max_lag, double *out, void *stream) {
+    g_err[0] = 0;
+    int32_t rc = check_series_args("bh_window_autocov", dtype, traj, n_save, n, dim, start);
+    if (rc) return rc;
+    if (!out) return fail(-5, "bh_window_autocov: null out");
+    if (max_lag < 0 || max_lag > 4095) return fail(-5, "bh_window_autocov: max_lag must be in [0, 4095]");
+    if (n == 0) return 0;
+    const int64_t w = n_save - start;
+    const size_t elem = dtype == BH_F32 ? 4 : 8;
+    int G = 0;
+    size_t lds = 0;
+    rc = series_launch(w, elem, &G, &lds, false);
+    if (rc) return rc;
+    const int64_t np = n * dim;
+    const int64_t blocks = (np + G - 1) / G;
+    if (blocks > 2147483647LL) return fail(-5, "bh_window_autocov: too many series");
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    if (dtype == BH_F32)
+        hipLaunchKernelGGL(bh::win::autocov_kernel<float>, dim3((unsigned)blocks),
+                           dim3(bh::win::kSeriesBlock), lds, s, reinterpret_cast<const float *>(traj),
+                           n_save, n, dim, start, G, max_lag, out);
+    else
+        hipLaunchKernelGGL(bh::win::autocov_kernel<double>, dim3((unsigned)blocks),
+                           dim3(bh::win::kSeriesBlock), lds, s, reinterpret_cast<const double *>(traj),
+                           n_save, n, dim, start, G, max_lag, out);
+    BH_HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
 extern "C++" template <class T>
 static void launch_sorted(int64_t P, int64_t blocks, size_t lds, hipStream_t s, const T *traj,
                           int64_t n_save, int64_t n, int32_t dim, int64_t start, int64_t G,

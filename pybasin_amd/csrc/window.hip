@@ -748,6 +748,51 @@ __global__ void __launch_bounds__(256) window_features_kernel(const T *traj, int
     for (int j = 0; j < 8; j++) put(BH_WF_MASS + j, (double)(mass_idx[j] < 0 ? N - 1 : mass_idx[j]));
 }
 
+// ---- autocovariances of many lags ---------------------------------------------
+
+// C_k = sum_{i < w - k} (x_i - mean)(x_{i+k} - mean) for k = 0 .. max_lag of
+// each series of the window (0 for k >= w), in double: the linear
+// autocorrelation of the centred series that the reference forms as the
+// inverse FFT of the zero-padded power spectrum (agg_autocorrelation and
+// ar_coefficient, torch_features_autocorrelation.py:72-96,
+// torch_features_trend.py:86-131).  One wave per series staged in LDS, one
+// lag per lane (64 lags a round); x_i is read by all lanes at once (an LDS
+// broadcast), x_{i+k} by consecutive lanes.  out (max_lag + 1, n, dim).
+template <class T>
+__global__ void __launch_bounds__(kSeriesBlock)
+    autocov_kernel(const T *traj, int64_t n_save, int64_t n, int S, int64_t start, int G,
+                   int max_lag, double *out) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int64_t w = n_save - start;
+    const int64_t np = n * S;
+    const int64_t p0 = (int64_t)blockIdx.x * G;
+    T *lds = reinterpret_cast<T *>(smem);
+    load_series<T>(traj, n, S, start, w, p0, G, np, lds);
+    __syncthreads();
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (int g = wave; g < G; g += kSeriesBlock / 64) {
+        const int64_t p = p0 + g;
+        if (p >= np) break;  // wave-uniform
+        const T *x = lds + (int64_t)g * w;
+        double mean, ss;
+        series_moments<T>(x, w, mean, ss);
+        for (int base = 0; base <= max_lag; base += 64) {
+            const int64_t k = base + lane;
+            double c0 = 0.0, c1 = 0.0;
+            if (k < w) {
+                const int64_t m = w - k;
+                int64_t i = 0;
+                for (; i + 1 < m; i += 2) {  // two independent sums
+                    c0 = fma((double)x[i] - mean, (double)x[i + k] - mean, c0);
+                    c1 = fma((double)x[i + 1] - mean, (double)x[i + 1 + k] - mean, c1);
+                }
+                if (i < m) c0 = fma((double)x[i] - mean, (double)x[i + k] - mean, c0);
+            }
+            if (k <= max_lag) out[k * np + p] = c0 + c1;
+        }
+    }
+}
+
 // ---- sorted-window statistics: duplicates and change quantiles -------------
 
 // the six quantile levels whose corridors change_quantiles takes

@@ -104,6 +104,16 @@ SORTED_FEATURES: dict[str, list[dict] | None] = {
                          for b in [False, True] for f in ["mean", "var"] if ql < qh],
 }
 COMPREHENSIVE_FEATURES.update(SORTED_FEATURES)
+# the calculators formed from the autocovariances of many lags (bh_window_autocov)
+AUTOCOV_FEATURES: dict[str, list[dict]] = {
+    "agg_autocorrelation": [{"f_agg": s, "maxlag": 40} for s in ["mean", "median", "var"]],
+    "ar_coefficient": [{"coeff": c, "k": 10} for c in range(11)],
+}
+COMPREHENSIVE_FEATURES.update(AUTOCOV_FEATURES)
+# the remaining regression calculator formed from the window-feature slots
+COMPREHENSIVE_FEATURES["augmented_dickey_fuller"] = [{"attr": a} for a in ["teststat", "pvalue",
+                                                                          "usedlag"]]
+MAX_AUTOCOV_LAG = 4095
 # the largest rfft bin index bh_window_spectrum writes
 MAX_SPECTRUM_COEFF = 4095
 # calculators whose parameter may take any value (formed from the slots by
@@ -144,6 +154,17 @@ def check_params(name: str, params: dict | None) -> None:
                              f"got {k!r}")
         return
     if name in ("fft_aggregated", "fourier_entropy"):
+        return
+    if name == "agg_autocorrelation":
+        if isinstance(params.get("maxlag", 40), int) and 1 <= params.get("maxlag", 40) <= MAX_AUTOCOV_LAG:
+            return
+        raise ValueError(f"agg_autocorrelation: maxlag must be in [1, {MAX_AUTOCOV_LAG}]")
+    if name == "ar_coefficient":
+        k, c = params.get("k", 1), params.get("coeff", 0)
+        if isinstance(k, int) and isinstance(c, int) and 1 <= k <= 64 and c >= 0:
+            return
+        raise ValueError("ar_coefficient: k must be in [1, 64] and coeff >= 0")
+    if name == "augmented_dickey_fuller":
         return
     if name == "change_quantiles":
         if params.get("ql") in _CQ_LEVELS and params.get("qh") in _CQ_LEVELS and \
@@ -445,6 +466,27 @@ def comprehensive_feature(name: str, params: dict | None, blk: torch.Tensor, n_w
             return 1 / (1 + t_stat.abs())
         resid = torch.clamp(ss_xx - 2 * slope * ss_tx + slope * slope * ss_tt, min=0.0)
         return torch.sqrt(resid / (n - 2) / (ss_tt + 1e-10))
+    if name == "augmented_dickey_fuller":  # trend.py:134-172: regression of the changes
+        # on the lagged values, from the slots: with dl = x_last - mean, the
+        # lagged values' centred square sum is CM2 - dl^2 - dl^2 / (n - 1) and
+        # their cross sum with the changes AUTOCOV_1 - (CM2 - dl^2) + dl / (n - 1)
+        # (x_last - x_first); the changes' own is SQ_CHANGE - (x_last - x_first)^2 / (n - 1)
+        if p.get("attr", "teststat") == "usedlag":
+            return torch.ones_like(N).where(~N.isnan(), N)
+        m = n - 1
+        dl = g("X_LAST") - mean
+        span = g("X_LAST") - g("X_FIRST")
+        sl2 = g("CM2") - dl * dl
+        ss_xx = sl2 - dl * dl / m
+        ss_xy = g("AUTOCOV") - sl2 + dl / m * span
+        ss_yy = g("SQ_CHANGE") - span * span / m
+        beta = ss_xy / (ss_xx + 1e-10)
+        resid = torch.clamp(ss_yy - 2 * beta * ss_xy + beta * beta * ss_xx, min=0.0)
+        se = torch.sqrt(resid / (n - 3)) / (torch.sqrt(ss_xx) + 1e-10)
+        t_stat = beta / (se + 1e-10)
+        if p.get("attr", "teststat") == "pvalue":
+            return 1 / (1 + t_stat.abs())
+        return t_stat
     if name == "symmetry_looking":  # statistical.py:184-190
         if median is None:
             raise ValueError("symmetry_looking needs the window median")
@@ -653,6 +695,95 @@ def sorted_feature(name: str, params: dict | None, dup: torch.Tensor, cq: torch.
     raise KeyError(f"feature {name!r} is not a sorted-window feature")
 
 
+def autocov_needs(plan) -> int:
+    """The largest lag the autocovariance features of a plan read."""
+    lag = 0
+    for _, name, params in plan:
+        p = params or {}
+        if name == "agg_autocorrelation":
+            lag = max(lag, int(p.get("maxlag", 40)))
+        elif name == "ar_coefficient":
+            lag = max(lag, int(p.get("k", 1)))
+    return lag
+
+
+def _autocov_cpu(w: torch.Tensor, max_lag: int) -> torch.Tensor:
+    """The outputs of bh_window_autocov for a CPU window (N, B, S), in torch."""
+    x = w.double()
+    N = x.shape[0]
+    d = x - x.mean(0)
+    out = torch.zeros((max_lag + 1,) + tuple(x.shape[1:]), dtype=torch.float64)
+    for k in range(min(max_lag, N - 1) + 1):
+        out[k] = (d[: N - k] * d[k:]).sum(0)
+    return out
+
+
+def window_autocov(y: torch.Tensor, start: int, max_lag: int) -> torch.Tensor:
+    """(max_lag + 1, B, S) float64 autocovariances C_k of the window rows
+    [start, N) of y (bh_window_autocov on the device, torch on the CPU)."""
+    N, B, S = y.shape
+    if not 0 <= start < N:
+        raise ValueError("empty feature window")
+    if not 0 <= max_lag <= MAX_AUTOCOV_LAG:
+        raise ValueError(f"max_lag must be in [0, {MAX_AUTOCOV_LAG}]")
+    if y.device.type != "cuda":
+        return _autocov_cpu(y[start:], max_lag)
+    y = y.contiguous()
+    out = torch.empty((max_lag + 1, B, S), dtype=torch.float64, device=y.device)
+    dt = _lib.BH_F32 if y.dtype == torch.float32 else _lib.BH_F64
+    s = torch.cuda.current_stream(y.device)
+    with torch.cuda.device(y.device):
+        _lib.check(_lib.lib().bh_window_autocov(dt, y.data_ptr(), N, B, S, int(start), int(max_lag),
+                                                out.data_ptr(), s.cuda_stream),
+                   "bh_window_autocov")
+    return out
+
+
+def _yule_walker(t: list, k: int) -> list:
+    """phi_1..phi_k of the symmetric Toeplitz system sum_j phi_j t_|i-j| = t_i
+    (i = 1..k) by the Durbin-Levinson recursion, element-wise over series."""
+    E = t[0]
+    phi: list = []
+    for m in range(1, k + 1):
+        num = t[m] - sum((phi[j] * t[m - 1 - j] for j in range(m - 1)), torch.zeros_like(E))
+        kap = num / E
+        phi = [phi[j] - kap * phi[m - 2 - j] for j in range(m - 1)] + [kap]
+        E = E * (1 - kap * kap)
+    return phi
+
+
+def autocov_feature(name: str, params: dict | None, C: torch.Tensor, n_win: int,
+                    cache: dict | None = None) -> torch.Tensor:
+    """Autocovariance feature `name` (B, S) float64 from window_autocov's
+    output (torch_features_autocorrelation.py:72-96, torch_features_trend.py:
+    86-131).  cache: a dict kept across calls on one window (the AR fit of
+    one order serves all its coefficients)."""
+    p = params or {}
+    if name == "agg_autocorrelation":
+        L = min(int(p.get("maxlag", 40)), n_win - 1)
+        acf = C[1:L + 1] / (C[0] + 1e-10)
+        f = p.get("f_agg", "mean")
+        if f == "median":
+            return acf.sort(0).values[(L - 1) // 2] if L >= 1 else torch.full_like(C[0], float("nan"))
+        if f == "var":
+            return acf.var(0, correction=0)
+        return acf.mean(0)
+    if name == "ar_coefficient":
+        k, coeff = int(p.get("k", 1)), int(p.get("coeff", 0))
+        if coeff >= k:  # coeff == k is zero in the reference as well as coeff > k
+            return torch.where(torch.isfinite(C[0]), 0.0, float("nan")).to(C.dtype)
+        key = ("ar", k)
+        if cache is None or key not in cache:
+            acf = C[: k + 1] / (C[0] + 1e-10)
+            t = [acf[0] + 1e-6] + [acf[j] for j in range(1, k + 1)]
+            phi = _yule_walker(t, k)
+            if cache is None:
+                return phi[coeff]
+            cache[key] = phi
+        return cache[key][coeff]
+    raise KeyError(f"feature {name!r} is not an autocovariance feature")
+
+
 def feature_family(name: str) -> str:
     """Which kernel output a comprehensive feature is formed from: "block"
     (bh_window_features), "spectrum" (bh_window_spectrum) or "sorted"
@@ -661,6 +792,8 @@ def feature_family(name: str) -> str:
         return "spectrum"
     if name in SORTED_FEATURES:
         return "sorted"
+    if name in AUTOCOV_FEATURES:
+        return "autocov"
     if name in COMPREHENSIVE_FEATURES:
         return "block"
     raise KeyError(f"feature {name!r} is not a comprehensive window feature")
@@ -670,7 +803,8 @@ class WindowBlocks:
     """The kernel outputs of one kept window (rows [start, N) of y), each
     computed once, when a feature of its family is first asked for: the
     window-feature slots, the spectrum (as many bins as the plan's spectrum
-    features need), the sorted-window statistics.  median: a callable giving
+    features need), the sorted-window statistics, the autocovariances (to
+    the largest lag the plan reads).  median: a callable giving
     the window median (for symmetry_looking)."""
 
     def __init__(self, y: torch.Tensor, start: int, plan, median=None):
@@ -696,8 +830,14 @@ class WindowBlocks:
         cq = any(name == "change_quantiles" for _, name, _ in self.plan)
         return self._get("sorted", lambda: window_sorted(self.y, self.start, cq))
 
+    def autocov(self):
+        return self._get("autocov", lambda: window_autocov(self.y, self.start,
+                                                           autocov_needs(self.plan)))
+
     def feature(self, name: str, params: dict | None) -> torch.Tensor:
         """Comprehensive feature `name` (B, S) float64 of the window."""
+        if name in AUTOCOV_FEATURES:
+            return autocov_feature(name, params, self.autocov(), self.n, self.memo)
         if name in SPECTRUM_FEATURES:
             return spectrum_feature(name, params, *self.spectrum(), self.n)
         if name in SORTED_FEATURES:

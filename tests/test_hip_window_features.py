@@ -151,11 +151,17 @@ def _check_against_oracle(blk, w64, start, tag, series_dtype=np.float64):
         if feature_family(name) != "block":
             continue  # the spectrum and sorted-window kernels, checked below
         for p in (grid or [None]):
-            if name in ("skewness", "kurtosis") and N <= 3:
-                continue  # noise amplified by the reference's 1e-10 guards
+            if name in ("skewness", "kurtosis", "augmented_dickey_fuller") and N <= 3:
+                continue  # noise amplified by the reference's 1e-10 guards (and n - 3 = 0)
             got = comprehensive_feature(name, p, blk, N, med).numpy()
             for b in range(B):
                 for s in range(S):
+                    if name == "augmented_dickey_fuller" and np.all(np.isfinite(x[:, b, s])):
+                        dd = np.diff(x[:, b, s])
+                        if dd.var() <= 1e-12 * (dd * dd).mean():
+                            # constant changes (a ramp): the t statistic is a ratio
+                            # of two rounding residues on either side
+                            continue
                     want = O.comprehensive(x[:, b, s], name, p, series_dtype)
                     g = float(got[b, s])
                     # pvalue and stderr of a perfect linear fit (1 - r^2 and the
@@ -400,3 +406,76 @@ def test_window_sorted_on_integrated_duffing(cuda_device, dtype):
     want = np.stack([sorted_feature(nm, p, dup, cq, n).numpy()[:, s] for s in range(2)
                      for nm, ps in feats.items() for p in (ps or [None])], 1)
     np.testing.assert_array_equal(f, want.astype(np.float32))
+
+
+def _check_autocov(C, x, tag, max_lag):
+    """bh_window_autocov against direct centred sums, and its features
+    against the oracle (ar_coefficient: the Durbin-Levinson recursion against
+    the oracle's LU solve, 1e-7)."""
+    from pybasin_amd.window_features import AUTOCOV_FEATURES, autocov_feature
+
+    N, B, S = x.shape
+    memo: dict = {}
+    xc = x - x.mean(0)
+    scale = (xc * xc).sum(0)
+    for k in range(max_lag + 1):
+        want = (xc[: N - k] * xc[k:]).sum(0) if k < N else np.zeros((B, S))
+        ok = np.isfinite(want) & np.isfinite(scale)
+        assert np.all(np.abs(C[k][ok] - want[ok]) <= 1e-12 * (scale[ok] + 1)), (tag, k)
+        if k < N:
+            assert np.all(~np.isfinite(C[k][~ok])), (tag, k)
+    for name, grid in AUTOCOV_FEATURES.items():
+        for p in grid:
+            got = autocov_feature(name, p, torch.tensor(C), N, memo).numpy()
+            for b in range(B):
+                for s in range(S):
+                    want = O.comprehensive(x[:, b, s], name, p)
+                    g = float(got[b, s])
+                    if not math.isfinite(want):
+                        assert not math.isfinite(g), (tag, name, p, b, s)
+                        continue
+                    tol = 1e-7 if name == "ar_coefficient" else 1e-11
+                    assert g == pytest.approx(want, rel=1e-7, abs=tol), (tag, name, p, b, s, g, want)
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_window_autocov_kernel_equals_oracle_synthetic(cuda_device, dtype):
+    """Autocovariances of lags 0..40 and 0..70 (two rounds of 64 lanes), lags
+    past short windows (zeros), non-finite windows; agg_autocorrelation and
+    ar_coefficient from them equal the oracle."""
+    from pybasin_amd.window_features import window_autocov
+
+    for tag, w in _synthetic_windows():
+        wt = torch.tensor(w, dtype=dtype)
+        x = wt.double().numpy()
+        for start in (0, 1) if w.shape[0] > 1 else (0,):
+            for max_lag in (40, 70):
+                C = window_autocov(wt.cuda(), start, max_lag).cpu().numpy()
+                _check_autocov(C, x[start:], f"{tag}/{start}/{max_lag}", max_lag)
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_window_autocov_on_integrated_duffing(cuda_device, dtype):
+    """The autocovariance features of integrated Duffing windows against the
+    oracle (24 series) and direct sums (all); the extractor's columns equal the
+    kernel's, augmented_dickey_fuller included."""
+    from pybasin_amd.window_features import autocov_feature, window_autocov
+
+    res = _duffing(dtype)
+    C = window_autocov(res.y, res.steady_idx, 40).cpu().numpy()
+    x = res.y.double().cpu().numpy()[res.steady_idx:]
+    _check_autocov(C[:, :12], x[:, :12], "duffing", 40)
+    feats = {"agg_autocorrelation": [{"f_agg": "median", "maxlag": 40}],
+             "ar_coefficient": [{"coeff": 3, "k": 10}],
+             "augmented_dickey_fuller": [{"attr": "teststat"}]}
+    fx = StatFeatureExtractor(time_steady=300.0, features=feats)
+    f = fx.extract_features(Solution(res.y[0], res.t, res.y)).cpu().numpy()
+    n = x.shape[0]
+    Ct = torch.tensor(C)
+    want_agg = autocov_feature("agg_autocorrelation", feats["agg_autocorrelation"][0], Ct, n).numpy()
+    want_ar = autocov_feature("ar_coefficient", feats["ar_coefficient"][0], Ct, n).numpy()
+    np.testing.assert_array_equal(f[:, 0], want_agg[:, 0].astype(np.float32))
+    np.testing.assert_array_equal(f[:, 1], want_ar[:, 0].astype(np.float32))
+    adf = np.array([O.comprehensive(x[:, b, 0], "augmented_dickey_fuller", {"attr": "teststat"})
+                    for b in range(x.shape[1])])
+    np.testing.assert_allclose(f[:, 2], adf.astype(np.float32), rtol=2e-6, atol=1e-6)

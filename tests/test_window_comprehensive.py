@@ -23,10 +23,11 @@ import torch
 
 from oracle import oracle as O
 from pybasin_amd.features import StatFeatureExtractor, Solution
-from pybasin_amd.window_features import (COMPREHENSIVE_FEATURES, SORTED_FEATURES,
-                                         SPECTRUM_FEATURES, _block_cpu, check_params,
-                                         comprehensive_feature, feature_family, sorted_feature,
-                                         spectrum_feature, window_sorted, window_spectrum)
+from pybasin_amd.window_features import (AUTOCOV_FEATURES, COMPREHENSIVE_FEATURES,
+                                         SORTED_FEATURES, SPECTRUM_FEATURES, _block_cpu,
+                                         autocov_feature, check_params, comprehensive_feature,
+                                         feature_family, sorted_feature, spectrum_feature,
+                                         window_autocov, window_sorted, window_spectrum)
 
 
 def _all_params():
@@ -357,6 +358,73 @@ def test_cpu_sorted_path_equals_oracle(dtype):
                             (name, p, N, b, s)
 
 
+def test_oracle_autocov_family_pins():
+    """agg_autocorrelation against the reference's own route (the inverse FFT
+    of the zero-padded power spectrum, by numpy), maxlag past the window;
+    ar_coefficient against scipy's Levinson Toeplitz solver and an AR(2)
+    process's known coefficients; augmented_dickey_fuller against
+    scipy.stats.linregress of the changes on the lagged values."""
+    from scipy import linalg, stats
+
+    rng = np.random.default_rng(12)
+    for n in (20, 300):
+        x = np.cumsum(rng.normal(size=n)) * 0.1 + np.sin(np.arange(n) * 0.3)
+        xc = x - x.mean()
+        C = np.fft.irfft(np.abs(np.fft.rfft(xc, 2 * n)) ** 2, 2 * n)[:41]
+        L = min(40, n - 1)
+        acf = C[1:L + 1] / (C[0] + 1e-10)
+        for f, want in (("mean", acf.mean()), ("median", np.sort(acf)[(L - 1) // 2]),
+                        ("var", acf.var())):
+            assert O.comprehensive(x, "agg_autocorrelation", {"f_agg": f, "maxlag": 40}) == \
+                pytest.approx(want, rel=1e-9, abs=1e-12), (n, f)
+        a = C[:11] / (C[0] + 1e-10)
+        if n < 11:
+            a[n:] = 0.0
+        col = a[:10].copy()
+        col[0] += 1e-6
+        phi = linalg.solve_toeplitz(col, a[1:11])
+        for c in range(10):
+            assert O.comprehensive(x, "ar_coefficient", {"coeff": c, "k": 10}) == pytest.approx(
+                phi[c], rel=1e-7, abs=1e-10), (n, c)
+        assert O.comprehensive(x, "ar_coefficient", {"coeff": 10, "k": 10}) == 0.0
+        lr = stats.linregress(x[:-1], np.diff(x))
+        t = O.comprehensive(x, "augmented_dickey_fuller", {"attr": "teststat"})
+        assert t == pytest.approx(lr.slope / lr.stderr, rel=1e-7)
+        assert O.comprehensive(x, "augmented_dickey_fuller", {"attr": "pvalue"}) == \
+            pytest.approx(1 / (1 + abs(t)))
+        assert O.comprehensive(x, "augmented_dickey_fuller", {"attr": "usedlag"}) == 1.0
+    e = rng.normal(size=40_000)
+    ar = np.zeros_like(e)
+    for i in range(2, e.size):
+        ar[i] = 0.6 * ar[i - 1] - 0.3 * ar[i - 2] + e[i]
+    got = [O.comprehensive(ar, "ar_coefficient", {"coeff": c, "k": 10}) for c in range(10)]
+    assert got[0] == pytest.approx(0.6, abs=0.03) and got[1] == pytest.approx(-0.3, abs=0.03)
+    assert max(abs(v) for v in got[2:]) < 0.03
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_cpu_autocov_path_equals_oracle(dtype):
+    for w in _windows():
+        wt = torch.tensor(w, dtype=dtype)
+        x64 = wt.double().numpy()
+        N, B, S = w.shape
+        C = window_autocov(wt, 0, 40)
+        memo: dict = {}
+        for name, grid in AUTOCOV_FEATURES.items():
+            for p in grid:
+                got = autocov_feature(name, p, C, N, memo).numpy()
+                for b in range(B):
+                    for s in range(S):
+                        want = O.comprehensive(x64[:, b, s], name, p)
+                        if not math.isfinite(want):
+                            assert not math.isfinite(got[b, s]), (name, p, N)
+                            continue
+                        # the Durbin-Levinson recursion against the oracle's LU solve
+                        tol = 1e-7 if name == "ar_coefficient" else 1e-12
+                        assert got[b, s] == pytest.approx(want, rel=1e-7, abs=tol), \
+                            (name, p, N, b, s)
+
+
 def _windows():
     rng = np.random.default_rng(11)
     out = [rng.normal(size=(200, 3, 2)),
@@ -377,8 +445,9 @@ def test_cpu_path_equals_oracle(dtype):
         blk = _block_cpu(wt)
         med = wt.median(0).values
         for name, p in _all_params():
-            if name in ("skewness", "kurtosis") and N <= 3:
-                continue  # the 1e-10 guards of n - 2 and (n - 2)(n - 3) amplify rounding noise
+            if name in ("skewness", "kurtosis", "augmented_dickey_fuller") and N <= 3:
+                continue  # the 1e-10 guards of n - 2 and (n - 2)(n - 3), and the
+                # division by n - 3, turn rounding noise into the result
             got = comprehensive_feature(name, p, blk, N, med).numpy()
             for b in range(B):
                 for s in range(S):
