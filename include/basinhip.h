@@ -487,6 +487,36 @@ int32_t bh_window_spectrum(int32_t dtype, const void *traj, int64_t n_save, int6
                            int32_t dim, int64_t start, int32_t n_coeff, double *out_coeff,
                            double *out_agg, void *stream);
 
+/* Rows of bh_window_trends' output (each (n, dim) float64). */
+enum {
+    BH_TR_ALT = 0,        /* 48: agg_linear_trend, (size * 4 + agg) * 4 + attr; sizes 5, 10,
+                             50; aggs max, min, mean, var; attrs rvalue, intercept, slope,
+                             stderr */
+    BH_TR_FRIED = 48,     /* 4: friedrich_coefficients (m = 3), coefficients of x^3 .. x^0 */
+    BH_TR_LANGEVIN = 52,  /* max_langevin_fixed_point (m = 3) */
+    BH_TR_BENFORD = 53,   /* benford_correlation */
+    BH_TR_CWT = 54,       /* 60: cwt_coefficients, width index * 15 + coeff; widths 2, 5, 10,
+                             20; coeff 0..14 */
+    BH_TR_SLOTS = 114
+};
+
+/* Chunk trends, polynomial fits, leading digits and wavelet coefficients of
+   each series of the same window, one lane per series in double (out
+   (BH_TR_SLOTS or BH_TR_CWT rows, n, dim) float64, rows as above):
+   agg_linear_trend (src/pybasin/ts_torch/calculators/torch_features_trend.py:
+   60-83; chunks of 5 / 10 / 50 samples, their max / min / mean / population
+   variance fitted against the chunk index; fewer than two chunks give 0);
+   friedrich_coefficients and max_langevin_fixed_point with m = 3
+   (torch_features_dynamical.py:87-177: the cubic's normal equations + 1e-6 I
+   solved with partial pivoting; the largest x_i where the fitted cubic
+   changes sign to x_{i+1}, 0 if none); benford_correlation
+   (torch_features_advanced.py:21-78); with want_cwt, cwt_coefficients
+   (torch_features_frequency.py:71-122: Ricker widths 2, 5, 10, 20,
+   coefficients 0..14, reflect padding, replicate for windows of at most 4 w
+   rows, 0 past the window).  Non-finite windows give NaN. */
+int32_t bh_window_trends(int32_t dtype, const void *traj, int64_t n_save, int64_t n, int32_t dim,
+                         int64_t start, int32_t want_cwt, double *out, void *stream);
+
 /* Autocovariances of each series of the same window: out (max_lag + 1, n,
    dim) float64 holds C_k = sum_{i < w - k} (x_i - mean)(x_{i+k} - mean) for
    k = 0 .. max_lag (0 for k >= w), in double: the linear autocorrelation of

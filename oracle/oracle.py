@@ -619,4 +619,59 @@ def comprehensive(x: np.ndarray, name: str, params: dict | None = None, dtype=np
         se = np.sqrt(mse) / (np.sqrt(ss_xx) + 1e-10)
         t = beta / (se + 1e-10)
         return float(1 / (1 + abs(t))) if p.get("attr") == "pvalue" else float(t)
+    if name == "agg_linear_trend":  # torch_features_trend.py:60-83, 174-212
+        cs = int(p["chunk_size"])
+        nc = n // cs
+        if nc < 2:
+            return 0.0
+        ch = x[: nc * cs].reshape(nc, cs)
+        a = {"max": ch.max(1), "min": ch.min(1), "mean": ch.mean(1), "var": ch.var(1)}[p["f_agg"]]
+        t = np.arange(nc, dtype=np.float64)
+        ss_tt = ((t - t.mean()) ** 2).sum()
+        ss_tx = ((t - t.mean()) * (a - a.mean())).sum()
+        ss_xx = ((a - a.mean()) ** 2).sum()
+        slope = ss_tx / (ss_tt + 1e-10)
+        icpt = a.mean() - slope * t.mean()
+        attr = p["attr"]
+        if attr == "slope":
+            return float(slope)
+        if attr == "intercept":
+            return float(icpt)
+        if attr == "rvalue":
+            return float(ss_tx / (np.sqrt(ss_tt * ss_xx) + 1e-10))
+        with np.errstate(divide="ignore", invalid="ignore"):
+            return float(np.sqrt(((a - slope * t - icpt) ** 2).sum() / (nc - 2) / (ss_tt + 1e-10)))
+    if name in ("friedrich_coefficients", "max_langevin_fixed_point"):
+        # the cubic's normal equations + 1e-6 I (torch_features_dynamical.py:87-177)
+        pos, vel = x[:-1], np.diff(x)
+        V = np.stack([pos ** 3, pos ** 2, pos, np.ones_like(pos)], 1)
+        cf = np.linalg.solve(V.T @ V + 1e-6 * np.eye(4), V.T @ vel)
+        if name == "friedrich_coefficients":
+            c = int(p.get("coeff", 0))
+            return float(cf[c]) if c < 4 else 0.0
+        fit = V @ cf
+        chg = fit[:-1] * fit[1:] < 0
+        return float(pos[:-1][chg].max()) if chg.any() else 0.0
+    if name == "benford_correlation":  # torch_features_advanced.py:21-78
+        nzv = np.abs(x[x != 0])
+        if nzv.size < 10:
+            return 0.0
+        dg = np.array([int(f"{v:.17e}"[0]) for v in nzv])  # the decimal expansion's first digit
+        cnt = np.array([(dg == d).sum() for d in range(1, 10)], dtype=np.float64)
+        fr = cnt / (cnt.sum() + 1e-10)
+        bf = np.log10(1 + 1 / np.arange(1, 10))
+        num = ((fr - fr.mean()) * (bf - bf.mean())).sum()
+        return float(num / (fr.std(ddof=1) * bf.std(ddof=1) * 9 + 1e-10))
+    if name == "cwt_coefficients":  # torch_features_frequency.py:71-122
+        wd = int(p.get("w", 2))
+        if wd not in tuple(p.get("widths", (2,))):
+            return 0.0
+        c = int(p.get("coeff", 0))
+        if c >= n:
+            return 0.0
+        t = np.arange(-4 * wd, 4 * wd + 1, dtype=np.float64)
+        psi = 2 / (np.sqrt(3.0 * wd) * np.pi ** 0.25) * (1 - (t / wd) ** 2) * np.exp(-t ** 2 / (2.0 * wd * wd))
+        pad = 4 * wd
+        xp = np.pad(x, pad, mode="reflect" if n > pad else "edge")
+        return float((xp[c: c + 2 * pad + 1] * psi).sum())
     raise KeyError(name)

@@ -47,7 +47,7 @@ EXPORTS = ("bh_version", "bh_last_error", "bh_solve_batch", "bh_solve_batch_peak
            "bh_silhouette",
            "bh_window_order_stats", "bh_window_autocorr_periodicity", "bh_window_spectral_ratio",
            "bh_window_features", "bh_window_spectrum", "bh_window_sorted",
-           "bh_window_change_quantiles", "bh_window_autocov")
+           "bh_window_change_quantiles", "bh_window_autocov", "bh_window_trends")
 
 _lock = threading.Lock()
 _lib: ctypes.CDLL | None = None
@@ -122,6 +122,7 @@ def _declare(L: ctypes.CDLL) -> None:
         "bh_window_sorted": [i32, vp, i64, i64, i32, i64, vp, vp, vp],
         "bh_window_change_quantiles": [i32, vp, i64, i64, i32, i64, vp, vp, vp],
         "bh_window_autocov": [i32, vp, i64, i64, i32, i64, i32, vp, vp],
+        "bh_window_trends": [i32, vp, i64, i64, i32, i64, i32, vp, vp],
     }
     for name, args in sigs.items():
         fn = getattr(L, name)

@@ -926,6 +926,29 @@ int32_t bh_window_autocov(int32_t dtype, const void *traj, int64_t n_save, int64
     return 0;
 }
 
+int32_t bh_window_trends(int32_t dtype, const void *traj, int64_t n_save, int64_t n, int32_t dim,
+                         int64_t start, int32_t want_cwt, double *out, void *stream) {
+    g_err[0] = 0;
+    int32_t rc = check_series_args("bh_window_trends", dtype, traj, n_save, n, dim, start);
+    if (rc) return rc;
+    if (!out) return fail(-5, "bh_window_trends: null out");
+    if (n == 0) return 0;
+    const int64_t np = n * dim;
+    const int64_t blocks = (np + 255) / 256;
+    if (blocks > 2147483647LL) return fail(-5, "bh_window_trends: too many series");
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    if (dtype == BH_F32)
+        hipLaunchKernelGGL(bh::win::trends_kernel<float>, dim3((unsigned)blocks), dim3(256), 0, s,
+                           reinterpret_cast<const float *>(traj), n_save, n, dim, start,
+                           want_cwt ? 1 : 0, out);
+    else
+        hipLaunchKernelGGL(bh::win::trends_kernel<double>, dim3((unsigned)blocks), dim3(256), 0, s,
+                           reinterpret_cast<const double *>(traj), n_save, n, dim, start,
+                           want_cwt ? 1 : 0, out);
+    BH_HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
 extern "C++" template <class T>
 static void launch_sorted(int64_t P, int64_t blocks, size_t lds, hipStream_t s, const T *traj,
                           int64_t n_save, int64_t n, int32_t dim, int64_t start, int64_t G,

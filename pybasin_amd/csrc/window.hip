@@ -793,6 +793,328 @@ __global__ void __launch_bounds__(kSeriesBlock)
     }
 }
 
+// ---- chunk trends, polynomial fits, leading digits, wavelets ----------------
+
+// a chunk's maximum, minimum, mean and square deviation sum, merged from two
+// disjoint chunks by Chan's update
+struct ChunkAgg {
+    double mx, mn, mean, m2, cnt;
+};
+
+__device__ __forceinline__ void chunk_merge(ChunkAgg &a, const ChunkAgg &b) {
+    if (a.cnt == 0.0) {
+        a = b;
+        return;
+    }
+    const double tot = a.cnt + b.cnt, d = b.mean - a.mean;
+    a.mean = fma(d, b.cnt / tot, a.mean);
+    a.m2 += b.m2 + d * d * (a.cnt * b.cnt / tot);
+    a.cnt = tot;
+    a.mx = fmax(a.mx, b.mx);
+    a.mn = fmin(a.mn, b.mn);
+}
+
+// shifted sums of one aggregate sequence a_c over chunks c = 0, 1, ..:
+// b_c = a_c - a_0, sum b, sum b^2, sum c b
+struct TrendSums {
+    double a0[4], sb[4], sbb[4], scb[4];
+    double nc;
+};
+
+__device__ __forceinline__ void trend_add(TrendSums &t, const ChunkAgg &g) {
+    const double v[4] = {g.mx, g.mn, g.mean, g.m2 / g.cnt};
+#pragma unroll
+    for (int f = 0; f < 4; f++) {
+        if (t.nc == 0.0) t.a0[f] = v[f];
+        const double b = v[f] - t.a0[f];
+        t.sb[f] += b;
+        t.sbb[f] = fma(b, b, t.sbb[f]);
+        t.scb[f] = fma(t.nc, b, t.scb[f]);
+    }
+    t.nc += 1.0;
+}
+
+// rvalue, intercept, slope, stderr of the least-squares line through the
+// aggregates (linear_trend_batched, torch_features_trend.py:174-212, over
+// the chunk index); zeros when there are fewer than two chunks
+__device__ __forceinline__ void trend_write(const TrendSums &t, double *out, int64_t np,
+                                            int64_t p, int base) {
+    const double nc = t.nc;
+#pragma unroll
+    for (int f = 0; f < 4; f++) {
+        double r = 0.0, icpt = 0.0, slope = 0.0, se = 0.0;
+        if (nc >= 2.0) {
+            const double tbar = 0.5 * (nc - 1.0);
+            const double ss_tt = nc * (nc * nc - 1.0) / 12.0;
+            const double ss_xx = fmax(t.sbb[f] - t.sb[f] * t.sb[f] / nc, 0.0);
+            const double ss_tx = t.scb[f] - tbar * t.sb[f];
+            slope = ss_tx / (ss_tt + 1e-10);
+            icpt = (t.a0[f] + t.sb[f] / nc) - slope * tbar;
+            r = ss_tx / (sqrt(ss_tt * ss_xx) + 1e-10);
+            const double res = fmax(ss_xx - 2.0 * slope * ss_tx + slope * slope * ss_tt, 0.0);
+            se = sqrt(res / (nc - 2.0) / (ss_tt + 1e-10));
+        }
+        out[(base + f * 4 + 0) * np + p] = r;
+        out[(base + f * 4 + 1) * np + p] = icpt;
+        out[(base + f * 4 + 2) * np + p] = slope;
+        out[(base + f * 4 + 3) * np + p] = se;
+    }
+}
+
+// the leading decimal digit of a > 0 (1..9)
+__device__ __forceinline__ int leading_digit(double a) {
+    const double e = floor(log10(a));
+    double s = a / pow(10.0, e);
+    if (s >= 10.0) s *= 0.1;
+    if (s < 1.0) s *= 10.0;
+    int d = (int)s;
+    return d < 1 ? 1 : (d > 9 ? 9 : d);
+}
+
+constexpr int kCwtWidths[4] = {2, 5, 10, 20};
+constexpr int kCwtCoeffs = 15;
+
+// One lane per series, two passes over the window (in double):
+// * agg_linear_trend (torch_features_trend.py:60-83): chunks of 5 samples
+//   give their max, min, mean and population variance; pairs of them merge
+//   into chunks of 10 and fives of those into chunks of 50 (Chan's update);
+//   each aggregate sequence is fitted against its chunk index (out rows
+//   BH_TR_ALT + (size * 4 + agg) * 4 + attr: sizes 5, 10, 50; aggs max, min,
+//   mean, var; attrs rvalue, intercept, slope, stderr; a sample tail shorter
+//   than a chunk is left out, fewer than two chunks give zeros);
+// * friedrich_coefficients and max_langevin_fixed_point (m = 3;
+//   torch_features_dynamical.py:87-177): the normal equations of the cubic
+//   through (x_i, x_{i+1} - x_i) plus 1e-6 I, solved by Gaussian elimination
+//   with partial pivoting (rows BH_TR_FRIED + 0..3, coefficients of x^3 ..
+//   x^0); pass 2 evaluates the cubic at every x_i and keeps the largest x_i
+//   whose value changes sign to the next one (row BH_TR_LANGEVIN, 0 if none);
+// * benford_correlation (torch_features_advanced.py:21-78): counts of the
+//   leading decimal digits of the nonzero |x_i| and their correlation with
+//   Benford's law as the reference forms it (unbiased deviations, divided by
+//   9 s_f s_b + 1e-10; 0 for fewer than 10 nonzero samples; row
+//   BH_TR_BENFORD);
+// * with want_cwt, cwt_coefficients (torch_features_frequency.py:71-122):
+//   the first 15 values of the correlation of the padded window with the
+//   Ricker wavelet of width 2, 5, 10, 20 (taps t = -4w .. 4w, reflect padding,
+//   replicate when the window has at most 4w rows; rows BH_TR_CWT + wi * 15 +
+//   coeff; coefficients past the window are 0).
+// A window holding a non-finite value gives NaN throughout.
+template <class T>
+__global__ void __launch_bounds__(256)
+    trends_kernel(const T *traj, int64_t n_save, int64_t n, int S, int64_t start, int want_cwt,
+                  double *out) {
+    __shared__ double psi[17 + 41 + 81 + 161];
+    for (int i = threadIdx.x; i < 300; i += blockDim.x) {
+        const int wi = i < 17 ? 0 : (i < 58 ? 1 : (i < 139 ? 2 : 3));
+        const int off = wi == 0 ? 0 : (wi == 1 ? 17 : (wi == 2 ? 58 : 139));
+        const double sg = (double)kCwtWidths[wi], t = (double)(i - off - 4 * kCwtWidths[wi]);
+        const double u = t / sg;
+        psi[i] = 2.0 / (sqrt(3.0 * sg) * 1.3313353638003897) * (1.0 - u * u) * exp(-0.5 * u * u);
+    }
+    __syncthreads();
+    const int64_t np = n * S;
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= np) return;
+    const T *col = traj + start * np + p;
+    const int64_t w = n_save - start;
+
+    TrendSums tr[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        tr[k].nc = 0.0;
+#pragma unroll
+        for (int f = 0; f < 4; f++) tr[k].a0[f] = tr[k].sb[f] = tr[k].sbb[f] = tr[k].scb[f] = 0.0;
+    }
+    ChunkAgg c10 = {0, 0, 0, 0, 0}, c50 = {0, 0, 0, 0, 0};
+    int k10 = 0, k50 = 0, c5 = 0;
+    double x0 = 0.0, s1 = 0.0, s2 = 0.0, mx = 0.0, mn = 0.0;
+    double sp[7] = {0, 0, 0, 0, 0, 0, 0}, svp[4] = {0, 0, 0, 0};
+    int dig[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    bool bad = false;
+    double prev = 0.0;
+    for (int64_t i = 0; i < w; i++) {
+        const double xv = (double)col[i * np];
+        bad |= !isfinite(xv);
+        // chunks of 5, merged into 10 and 50
+        if (c5 == 0) {
+            x0 = xv;
+            s1 = s2 = 0.0;
+            mx = mn = xv;
+        }
+        const double d = xv - x0;
+        s1 += d;
+        s2 = fma(d, d, s2);
+        mx = fmax(mx, xv);
+        mn = fmin(mn, xv);
+        if (++c5 == 5) {
+            c5 = 0;
+            const ChunkAgg g5 = {mx, mn, x0 + s1 / 5.0, fmax(s2 - s1 * s1 / 5.0, 0.0), 5.0};
+            trend_add(tr[0], g5);
+            chunk_merge(c10, g5);
+            if (++k10 == 2) {
+                trend_add(tr[1], c10);
+                chunk_merge(c50, c10);
+                c10.cnt = 0.0;
+                k10 = 0;
+                if (++k50 == 5) {
+                    trend_add(tr[2], c50);
+                    c50.cnt = 0.0;
+                    k50 = 0;
+                }
+            }
+        }
+        // the cubic's normal equations over the pairs (x_{i-1}, x_i - x_{i-1})
+        if (i > 0) {
+            const double pp = prev, v = xv - prev;
+            double pk = 1.0;
+#pragma unroll
+            for (int k = 0; k < 7; k++) {
+                sp[k] += pk;
+                if (k < 4) svp[k] = fma(v, pk, svp[k]);
+                pk *= pp;
+            }
+        }
+        prev = xv;
+        // leading digits
+        const double a = fabs(xv);
+        if (a > 0.0 && a < INFINITY) {
+            const int dg = leading_digit(a);
+#pragma unroll
+            for (int k = 0; k < 9; k++) dig[k] += dg == k + 1 ? 1 : 0;
+        }
+    }
+    if (bad) {
+        const int rows = want_cwt ? BH_TR_SLOTS : BH_TR_CWT;
+        for (int k = 0; k < rows; k++) out[k * np + p] = NAN;
+        return;
+    }
+#pragma unroll
+    for (int k = 0; k < 3; k++) trend_write(tr[k], out, np, p, BH_TR_ALT + k * 16);
+
+    // (V^T V + 1e-6 I) c = V^T v, V = [x^3 x^2 x 1]
+    double A[4][5];
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+#pragma unroll
+        for (int c = 0; c < 4; c++) A[r][c] = sp[6 - r - c] + (r == c ? 1e-6 : 0.0);
+        A[r][4] = svp[3 - r];
+    }
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        int piv = k;
+        double big = fabs(A[k][k]);
+#pragma unroll
+        for (int r = k + 1; r < 4; r++)
+            if (fabs(A[r][k]) > big) {
+                big = fabs(A[r][k]);
+                piv = r;
+            }
+        if (piv != k) {
+#pragma unroll
+            for (int c = 0; c < 5; c++) {
+                // swap rows k and piv without dynamic register indices
+                double hold = A[k][c];
+#pragma unroll
+                for (int r = k + 1; r < 4; r++)
+                    if (r == piv) {
+                        A[k][c] = A[r][c];
+                        A[r][c] = hold;
+                    }
+            }
+        }
+#pragma unroll
+        for (int r = k + 1; r < 4; r++) {
+            const double f = A[r][k] / A[k][k];
+#pragma unroll
+            for (int c = k; c < 5; c++) A[r][c] = fma(-f, A[k][c], A[r][c]);
+        }
+    }
+    double cf[4];
+#pragma unroll
+    for (int r = 3; r >= 0; r--) {
+        double v = A[r][4];
+#pragma unroll
+        for (int c = r + 1; c < 4; c++) v = fma(-A[r][c], cf[c], v);
+        cf[r] = v / A[r][r];
+    }
+#pragma unroll
+    for (int k = 0; k < 4; k++) out[(BH_TR_FRIED + k) * np + p] = cf[k];
+
+    // the largest x_i at which the fitted cubic changes sign to x_{i+1}
+    double best = -INFINITY;
+    if (w >= 3) {
+        double pa = (double)col[0];
+        double fa = fma(fma(fma(cf[0], pa, cf[1]), pa, cf[2]), pa, cf[3]);
+        for (int64_t i = 1; i + 1 < w; i++) {
+            const double pb = (double)col[i * np];
+            const double fb = fma(fma(fma(cf[0], pb, cf[1]), pb, cf[2]), pb, cf[3]);
+            if (fa * fb < 0.0) best = fmax(best, pa);
+            pa = pb;
+            fa = fb;
+        }
+    }
+    out[BH_TR_LANGEVIN * np + p] = best == -INFINITY ? 0.0 : best;
+
+    // Benford: Pearson-like correlation as the reference forms it
+    double tot = 0.0;
+#pragma unroll
+    for (int k = 0; k < 9; k++) tot += (double)dig[k];
+    double corr = 0.0;
+    if (tot >= 10.0) {
+        double bf[9], fr[9], bm = 0.0, fm = 0.0;
+#pragma unroll
+        for (int k = 0; k < 9; k++) {
+            bf[k] = log10(1.0 + 1.0 / (double)(k + 1));
+            fr[k] = (double)dig[k] / (tot + 1e-10);
+            bm += bf[k];
+            fm += fr[k];
+        }
+        bm /= 9.0;
+        fm /= 9.0;
+        double sbb = 0.0, sff = 0.0, sfb = 0.0;
+#pragma unroll
+        for (int k = 0; k < 9; k++) {
+            sbb += (bf[k] - bm) * (bf[k] - bm);
+            sff += (fr[k] - fm) * (fr[k] - fm);
+            sfb += (fr[k] - fm) * (bf[k] - bm);
+        }
+        corr = sfb / (sqrt(sff / 8.0) * sqrt(sbb / 8.0) * 9.0 + 1e-10);
+    }
+    out[BH_TR_BENFORD * np + p] = corr;
+
+    if (!want_cwt) return;
+    int off = 0;
+#pragma unroll
+    for (int wi = 0; wi < 4; wi++) {
+        const int wd = kCwtWidths[wi], pad = 4 * wd;
+        const bool reflect = w > pad;
+        double acc[kCwtCoeffs];
+#pragma unroll
+        for (int c = 0; c < kCwtCoeffs; c++) acc[c] = 0.0;
+        // samples past x_{w-1+pad} feed only coefficients c >= w (written as 0)
+        const int uend = (int)(w < kCwtCoeffs ? w : kCwtCoeffs) + pad;
+        for (int u = -pad; u < uend; u++) {
+            int64_t m = u;
+            if (reflect) {
+                if (m < 0) m = -m;
+                if (m > w - 1) m = 2 * (w - 1) - m;
+            } else {
+                m = m < 0 ? 0 : (m > w - 1 ? w - 1 : m);
+            }
+            const double xu = (double)col[m * np];
+#pragma unroll
+            for (int c = 0; c < kCwtCoeffs; c++) {
+                const int j = u - c + pad;
+                if (j >= 0 && j <= 2 * pad) acc[c] = fma(psi[off + j], xu, acc[c]);
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < kCwtCoeffs; c++)
+            out[(BH_TR_CWT + wi * kCwtCoeffs + c) * np + p] = c < w ? acc[c] : 0.0;
+        off += 2 * pad + 1;
+    }
+}
+
 // ---- sorted-window statistics: duplicates and change quantiles -------------
 
 // the six quantile levels whose corridors change_quantiles takes

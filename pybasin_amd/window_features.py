@@ -110,6 +110,23 @@ AUTOCOV_FEATURES: dict[str, list[dict]] = {
     "ar_coefficient": [{"coeff": c, "k": 10} for c in range(11)],
 }
 COMPREHENSIVE_FEATURES.update(AUTOCOV_FEATURES)
+# the calculators of bh_window_trends: chunk trends, cubic fits, leading
+# digits, wavelet coefficients
+_ALT_SIZES, _ALT_AGGS = [5, 10, 50], ["max", "min", "mean", "var"]
+_ALT_ATTRS = ["rvalue", "intercept", "slope", "stderr"]
+_CWT_WIDTHS = [2, 5, 10, 20]
+TRENDS_FEATURES: dict[str, list[dict] | None] = {
+    "agg_linear_trend": [{"attr": a, "chunk_size": i, "f_agg": f} for a in _ALT_ATTRS
+                         for i in _ALT_SIZES for f in _ALT_AGGS],
+    "friedrich_coefficients": [{"coeff": c, "m": 3, "r": 30} for c in range(4)],
+    "max_langevin_fixed_point": [{"m": 3, "r": 30}],
+    "benford_correlation": None,
+    "cwt_coefficients": [{"widths": (w,), "coeff": c, "w": w} for w in _CWT_WIDTHS
+                         for c in range(15)],
+}
+COMPREHENSIVE_FEATURES.update(TRENDS_FEATURES)
+TR = dict(ALT=0, FRIED=48, LANGEVIN=52, BENFORD=53, CWT=54)
+BH_TR_SLOTS = 114
 # the remaining regression calculator formed from the window-feature slots
 COMPREHENSIVE_FEATURES["augmented_dickey_fuller"] = [{"attr": a} for a in ["teststat", "pvalue",
                                                                           "usedlag"]]
@@ -166,6 +183,20 @@ def check_params(name: str, params: dict | None) -> None:
         raise ValueError("ar_coefficient: k must be in [1, 64] and coeff >= 0")
     if name == "augmented_dickey_fuller":
         return
+    if name == "agg_linear_trend":
+        if params.get("chunk_size") in _ALT_SIZES and params.get("f_agg") in _ALT_AGGS and \
+                params.get("attr") in _ALT_ATTRS:
+            return
+        raise ValueError(f"agg_linear_trend: chunk_size in {_ALT_SIZES}, f_agg in {_ALT_AGGS}, "
+                         f"attr in {_ALT_ATTRS}; got {params}")
+    if name in ("friedrich_coefficients", "max_langevin_fixed_point"):
+        if int(params.get("m", 3)) == 3 and int(params.get("coeff", 0)) >= 0:
+            return
+        raise ValueError(f"{name}: the cubic (m = 3) is fitted, got {params}")
+    if name == "cwt_coefficients":
+        if params.get("w") in _CWT_WIDTHS and 0 <= int(params.get("coeff", -1)) < 15:
+            return
+        raise ValueError(f"cwt_coefficients: w in {_CWT_WIDTHS} and coeff in 0..14, got {params}")
     if name == "change_quantiles":
         if params.get("ql") in _CQ_LEVELS and params.get("qh") in _CQ_LEVELS and \
                 params["ql"] < params["qh"] and params.get("f_agg", "mean") in ("mean", "var"):
@@ -784,6 +815,117 @@ def autocov_feature(name: str, params: dict | None, C: torch.Tensor, n_win: int,
     raise KeyError(f"feature {name!r} is not an autocovariance feature")
 
 
+def _trends_cpu(w: torch.Tensor, want_cwt: bool) -> torch.Tensor:
+    """The outputs of bh_window_trends for a CPU window (N, B, S), in torch."""
+    x = w.double()
+    N, B, S = x.shape
+    rows = BH_TR_SLOTS if want_cwt else TR["CWT"]
+    out = torch.zeros((rows, B, S), dtype=torch.float64)
+    ok = torch.isfinite(x).all(0)
+    for si, cs in enumerate(_ALT_SIZES):
+        nc = N // cs
+        if nc < 2:
+            continue
+        ch = x[: nc * cs].reshape(nc, cs, B, S)
+        aggs = [ch.max(1).values, ch.min(1).values, ch.mean(1), ch.var(1, correction=0)]
+        t = torch.arange(nc, dtype=torch.float64).view(nc, 1, 1)
+        tb = t.mean()
+        ss_tt = ((t - tb) ** 2).sum()
+        for fi, a in enumerate(aggs):
+            am = a.mean(0)
+            ss_tx = ((t - tb) * (a - am)).sum(0)
+            ss_xx = ((a - am) ** 2).sum(0)
+            slope = ss_tx / (ss_tt + 1e-10)
+            res = ((a - (slope * t + (am - slope * tb))) ** 2).sum(0)
+            base = TR["ALT"] + (si * 4 + fi) * 4
+            out[base] = ss_tx / (torch.sqrt(ss_tt * ss_xx) + 1e-10)
+            out[base + 1] = am - slope * tb
+            out[base + 2] = slope
+            out[base + 3] = torch.sqrt(res / (nc - 2) / (ss_tt + 1e-10))
+    pos, vel = x[:-1], x[1:] - x[:-1]
+    V = torch.stack([pos ** 3, pos ** 2, pos, torch.ones_like(pos)], -1)  # (N-1, B, S, 4)
+    A = torch.einsum("nbsi,nbsj->bsij", V, V) + 1e-6 * torch.eye(4, dtype=torch.float64)
+    rhs = torch.einsum("nbsi,nbs->bsi", V, vel)
+    cf = torch.linalg.solve(torch.nan_to_num(A), torch.nan_to_num(rhs))  # (B, S, 4)
+    for k in range(4):
+        out[TR["FRIED"] + k] = cf[..., k]
+    fit = (V * cf.unsqueeze(0)).sum(-1)  # (N-1, B, S)
+    if N >= 3:
+        chg = fit[:-1] * fit[1:] < 0
+        best = torch.where(chg, pos[:-1], torch.full_like(pos[:-1], -math.inf)).max(0).values
+        out[TR["LANGEVIN"]] = torch.where(best == -math.inf, torch.zeros_like(best), best)
+    a = x.abs()
+    nz = (a > 0) & torch.isfinite(a)
+    e = torch.floor(torch.log10(torch.where(nz, a, torch.ones_like(a))))
+    sc = torch.where(nz, a, torch.ones_like(a)) / 10.0 ** e
+    sc = torch.where(sc >= 10, sc * 0.1, sc)
+    sc = torch.where(sc < 1, sc * 10, sc)
+    dg = sc.long().clamp(1, 9)
+    cnt = torch.stack([((dg == d) & nz).double().sum(0) for d in range(1, 10)])  # (9, B, S)
+    tot = cnt.sum(0)
+    bf = torch.tensor([math.log10(1 + 1 / d) for d in range(1, 10)], dtype=torch.float64).view(9, 1, 1)
+    fr = cnt / (tot + 1e-10)
+    corr = ((fr - fr.mean(0)) * (bf - bf.mean())).sum(0) / (fr.std(0) * bf.std() * 9 + 1e-10)
+    out[TR["BENFORD"]] = torch.where(tot >= 10, corr, torch.zeros_like(corr))
+    if want_cwt:
+        xs = x.permute(1, 2, 0).reshape(B * S, 1, N)
+        for wi, wd in enumerate(_CWT_WIDTHS):
+            t = torch.arange(-4 * wd, 4 * wd + 1, dtype=torch.float64)
+            psi = (2 / (math.sqrt(3.0 * wd) * math.pi ** 0.25)) * (1 - (t / wd) ** 2) * \
+                torch.exp(-t ** 2 / (2.0 * wd * wd))
+            pad = 4 * wd
+            padded = torch.nn.functional.pad(xs, (pad, pad), mode="reflect" if N > pad else "replicate")
+            conv = torch.nn.functional.conv1d(padded, psi.view(1, 1, -1))[:, 0]  # (B*S, N)
+            for c in range(15):
+                out[TR["CWT"] + wi * 15 + c] = conv[:, c].reshape(B, S) if c < N else 0.0
+    return torch.where(ok, out, torch.full_like(out, float("nan")))
+
+
+def window_trends(y: torch.Tensor, start: int, want_cwt: bool) -> torch.Tensor:
+    """(BH_TR_SLOTS or TR["CWT"] rows, B, S) float64 outputs of
+    bh_window_trends for the window rows [start, N) of y (the kernel on the
+    device, torch on the CPU)."""
+    N, B, S = y.shape
+    if not 0 <= start < N:
+        raise ValueError("empty feature window")
+    if y.device.type != "cuda":
+        return _trends_cpu(y[start:], want_cwt)
+    y = y.contiguous()
+    rows = BH_TR_SLOTS if want_cwt else TR["CWT"]
+    out = torch.empty((rows, B, S), dtype=torch.float64, device=y.device)
+    dt = _lib.BH_F32 if y.dtype == torch.float32 else _lib.BH_F64
+    s = torch.cuda.current_stream(y.device)
+    with torch.cuda.device(y.device):
+        _lib.check(_lib.lib().bh_window_trends(dt, y.data_ptr(), N, B, S, int(start), int(want_cwt),
+                                               out.data_ptr(), s.cuda_stream), "bh_window_trends")
+    return out
+
+
+def trends_feature(name: str, params: dict | None, out: torch.Tensor) -> torch.Tensor:
+    """Feature `name` (B, S) float64 from window_trends' output
+    (torch_features_trend.py:60-83, torch_features_dynamical.py:87-177,
+    torch_features_advanced.py:21-78, torch_features_frequency.py:71-122)."""
+    p = params or {}
+    nan_or = lambda v: torch.where(torch.isnan(out[TR["BENFORD"]]), out[TR["BENFORD"]], v)  # noqa: E731
+    if name == "agg_linear_trend":
+        k = (_ALT_SIZES.index(p["chunk_size"]) * 4 + _ALT_AGGS.index(p["f_agg"])) * 4 + \
+            _ALT_ATTRS.index(p["attr"])
+        return out[TR["ALT"] + k]
+    if name == "friedrich_coefficients":
+        c = int(p.get("coeff", 0))
+        return out[TR["FRIED"] + c] if c < 4 else nan_or(torch.zeros_like(out[0]))
+    if name == "max_langevin_fixed_point":
+        return out[TR["LANGEVIN"]]
+    if name == "benford_correlation":
+        return out[TR["BENFORD"]]
+    if name == "cwt_coefficients":
+        w = p.get("w", 2)
+        if w not in tuple(p.get("widths", (2,))):
+            return nan_or(torch.zeros_like(out[0]))
+        return out[TR["CWT"] + _CWT_WIDTHS.index(w) * 15 + int(p.get("coeff", 0))]
+    raise KeyError(f"feature {name!r} is not a chunk-trend / fit feature")
+
+
 def feature_family(name: str) -> str:
     """Which kernel output a comprehensive feature is formed from: "block"
     (bh_window_features), "spectrum" (bh_window_spectrum) or "sorted"
@@ -794,6 +936,8 @@ def feature_family(name: str) -> str:
         return "sorted"
     if name in AUTOCOV_FEATURES:
         return "autocov"
+    if name in TRENDS_FEATURES:
+        return "trends"
     if name in COMPREHENSIVE_FEATURES:
         return "block"
     raise KeyError(f"feature {name!r} is not a comprehensive window feature")
@@ -804,7 +948,7 @@ class WindowBlocks:
     computed once, when a feature of its family is first asked for: the
     window-feature slots, the spectrum (as many bins as the plan's spectrum
     features need), the sorted-window statistics, the autocovariances (to
-    the largest lag the plan reads).  median: a callable giving
+    the largest lag the plan reads), the chunk trends and fits.  median: a callable giving
     the window median (for symmetry_looking)."""
 
     def __init__(self, y: torch.Tensor, start: int, plan, median=None):
@@ -834,8 +978,14 @@ class WindowBlocks:
         return self._get("autocov", lambda: window_autocov(self.y, self.start,
                                                            autocov_needs(self.plan)))
 
+    def trends(self):
+        cwt = any(name == "cwt_coefficients" for _, name, _ in self.plan)
+        return self._get("trends", lambda: window_trends(self.y, self.start, cwt))
+
     def feature(self, name: str, params: dict | None) -> torch.Tensor:
         """Comprehensive feature `name` (B, S) float64 of the window."""
+        if name in TRENDS_FEATURES:
+            return trends_feature(name, params, self.trends())
         if name in AUTOCOV_FEATURES:
             return autocov_feature(name, params, self.autocov(), self.n, self.memo)
         if name in SPECTRUM_FEATURES:

@@ -479,3 +479,89 @@ def test_window_autocov_on_integrated_duffing(cuda_device, dtype):
     adf = np.array([O.comprehensive(x[:, b, 0], "augmented_dickey_fuller", {"attr": "teststat"})
                     for b in range(x.shape[1])])
     np.testing.assert_allclose(f[:, 2], adf.astype(np.float32), rtol=2e-6, atol=1e-6)
+
+
+def _alt_degenerate(xs, p):
+    """Whether the chunk aggregates agg_linear_trend fits are constant to
+    rounding (a ramp's chunk variances): its rvalue is then a ratio of two
+    rounding residues on either side."""
+    cs = p["chunk_size"]
+    nc = xs.size // cs
+    if nc < 2:
+        return False
+    ch = xs[: nc * cs].reshape(nc, cs)
+    a = {"max": ch.max(1), "min": ch.min(1), "mean": ch.mean(1), "var": ch.var(1)}[p["f_agg"]]
+    return np.ptp(a) <= 1e-9 * (np.abs(a).max() + 1e-300)
+
+
+def _check_trends(out, x, tag, nseries=None):
+    """bh_window_trends' features against the oracle (the cubic fit: the
+    kernel's elimination against the oracle's LAPACK solve, 1e-7)."""
+    from pybasin_amd.window_features import TRENDS_FEATURES, trends_feature
+
+    N, B, S = x.shape
+    bs = range(B if nseries is None else min(B, nseries))
+    for name, grid in TRENDS_FEATURES.items():
+        for p in (grid or [None]):
+            got = trends_feature(name, p, out).numpy()
+            for b in bs:
+                for s in range(S):
+                    want = O.comprehensive(x[:, b, s], name, p)
+                    g = float(got[b, s])
+                    if not math.isfinite(want):
+                        assert not math.isfinite(g), (tag, name, p, b, s)
+                        continue
+                    tol = 1e-7 if name == "friedrich_coefficients" else 1e-11
+                    if name == "agg_linear_trend" and p["attr"] == "stderr":
+                        tol = 1e-7  # a perfect fit's residual is rounding, under a square root
+                    if name == "agg_linear_trend" and p["attr"] == "rvalue" and \
+                            _alt_degenerate(x[:, b, s], p):
+                        continue
+                    assert g == pytest.approx(want, rel=1e-7, abs=tol), (tag, name, p, b, s, g, want)
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_window_trends_kernel_equals_oracle_synthetic(cuda_device, dtype):
+    """agg_linear_trend (chunk merges of 5 -> 10 -> 50, tails, fewer than two
+    chunks), the cubic fit and its sign changes, Benford digits, wavelet
+    coefficients (reflect and replicate padding, coefficients past short
+    windows) against the oracle; the CPU path agrees."""
+    from pybasin_amd.window_features import window_trends
+
+    rng = np.random.default_rng(41)
+    cases = list(_synthetic_windows()) + [
+        ("walk", np.cumsum(rng.normal(size=(523, 6, 2)), 0) * 0.3),
+        ("digits", rng.lognormal(0, 3, size=(200, 5, 1)) * rng.choice([-1.0, 1.0], size=(200, 5, 1)))]
+    for tag, w in cases:
+        wt = torch.tensor(w, dtype=dtype)
+        x = wt.double().numpy()
+        for start in (0, 1) if w.shape[0] > 1 else (0,):
+            out = window_trends(wt.cuda(), start, True).cpu()
+            _check_trends(out, x[start:], f"{tag}/{start}")
+            short = window_trends(wt.cuda(), start, False).cpu()
+            assert short.shape[0] == 54
+            assert torch.equal(torch.nan_to_num(short, 7.0), torch.nan_to_num(out[:54], 7.0)), tag
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_window_trends_on_integrated_duffing(cuda_device, dtype):
+    """The chunk-trend / fit / digit / wavelet features of integrated Duffing
+    windows against the oracle (16 series) and the CPU path (all); the
+    extractor's columns equal the kernel's."""
+    from pybasin_amd.window_features import trends_feature, window_trends
+
+    res = _duffing(dtype)
+    out = window_trends(res.y, res.steady_idx, True).cpu()
+    x = res.y.double().cpu().numpy()[res.steady_idx:]
+    _check_trends(out, x, "duffing", nseries=8)
+    cpu = window_trends(res.y.cpu(), res.steady_idx, True)
+    np.testing.assert_allclose(out[:48].numpy(), cpu[:48].numpy(), rtol=1e-9, atol=1e-12)
+    np.testing.assert_allclose(out[53:].numpy(), cpu[53:].numpy(), rtol=1e-9, atol=1e-12)
+    feats = {"agg_linear_trend": [{"attr": "slope", "chunk_size": 50, "f_agg": "var"}],
+             "cwt_coefficients": [{"widths": (5,), "coeff": 7, "w": 5}],
+             "benford_correlation": None}
+    fx = StatFeatureExtractor(time_steady=300.0, features=feats)
+    f = fx.extract_features(Solution(res.y[0], res.t, res.y)).cpu().numpy()
+    want = np.stack([trends_feature(nm, p, out).numpy()[:, s] for s in range(2)
+                     for nm, ps in feats.items() for p in (ps or [None])], 1)
+    np.testing.assert_array_equal(f, want.astype(np.float32))

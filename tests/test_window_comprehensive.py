@@ -24,10 +24,11 @@ import torch
 from oracle import oracle as O
 from pybasin_amd.features import StatFeatureExtractor, Solution
 from pybasin_amd.window_features import (AUTOCOV_FEATURES, COMPREHENSIVE_FEATURES,
-                                         SORTED_FEATURES, SPECTRUM_FEATURES, _block_cpu,
-                                         autocov_feature, check_params, comprehensive_feature,
-                                         feature_family, sorted_feature, spectrum_feature,
-                                         window_autocov, window_sorted, window_spectrum)
+                                         SORTED_FEATURES, SPECTRUM_FEATURES, TRENDS_FEATURES,
+                                         _block_cpu, autocov_feature, check_params,
+                                         comprehensive_feature, feature_family, sorted_feature,
+                                         spectrum_feature, trends_feature, window_autocov,
+                                         window_sorted, window_spectrum, window_trends)
 
 
 def _all_params():
@@ -423,6 +424,83 @@ def test_cpu_autocov_path_equals_oracle(dtype):
                         tol = 1e-7 if name == "ar_coefficient" else 1e-12
                         assert got[b, s] == pytest.approx(want, rel=1e-7, abs=tol), \
                             (name, p, N, b, s)
+
+
+def test_oracle_trends_family_pins():
+    """agg_linear_trend against scipy.stats.linregress of numpy's chunk
+    aggregates; the cubic fit against numpy.polyfit (the 1e-6 ridge is
+    negligible here) and its sign-change scan against numpy's sign changes;
+    benford_correlation against 8/9 of numpy.corrcoef over digits read from
+    the decimal strings; cwt_coefficients against numpy.convolve of the
+    reflect-padded series with the Ricker wavelet, and a constant series'
+    coefficients against the wavelet's sum."""
+    from scipy import stats
+
+    rng = np.random.default_rng(13)
+    x = np.cumsum(rng.normal(size=503)) * 0.2 + 3 * np.sin(np.arange(503) * 0.05)
+    for cs in (5, 10, 50):
+        nc = 503 // cs
+        ch = x[: nc * cs].reshape(nc, cs)
+        for f, a in (("max", ch.max(1)), ("min", ch.min(1)), ("mean", ch.mean(1)), ("var", ch.var(1))):
+            lr = stats.linregress(np.arange(nc), a)
+            for attr, want in (("slope", lr.slope), ("intercept", lr.intercept),
+                               ("rvalue", lr.rvalue), ("stderr", lr.stderr)):
+                got = O.comprehensive(x, "agg_linear_trend", {"attr": attr, "chunk_size": cs, "f_agg": f})
+                assert got == pytest.approx(want, rel=1e-7, abs=1e-12), (cs, f, attr)
+    assert O.comprehensive(x[:9], "agg_linear_trend", {"attr": "slope", "chunk_size": 5,
+                                                       "f_agg": "max"}) == 0.0  # one chunk
+    pos, vel = x[:-1], np.diff(x)
+    pf = np.polyfit(pos, vel, 3)
+    for c in range(4):
+        assert O.comprehensive(x, "friedrich_coefficients", {"coeff": c, "m": 3, "r": 30}) == \
+            pytest.approx(pf[c], rel=1e-4, abs=1e-7), c
+    cf = [O.comprehensive(x, "friedrich_coefficients", {"coeff": c, "m": 3, "r": 30}) for c in range(4)]
+    fit = np.polyval(cf, pos)
+    sgn = np.nonzero(np.sign(fit[:-1]) * np.sign(fit[1:]) < 0)[0]
+    want = pos[sgn].max() if sgn.size else 0.0
+    assert O.comprehensive(x, "max_langevin_fixed_point", {"m": 3, "r": 30}) == pytest.approx(want)
+    y = rng.lognormal(0, 2, size=400) * rng.choice([-1, 1], size=400)
+    y[::7] = 0.0
+    nz = y[y != 0]
+    dg = np.array([int(str(f"{abs(v):e}")[0]) for v in nz])
+    fr = np.array([(dg == d).mean() for d in range(1, 10)])
+    bf = np.log10(1 + 1 / np.arange(1, 10))
+    assert O.comprehensive(y, "benford_correlation") == pytest.approx(
+        8 / 9 * np.corrcoef(fr, bf)[0, 1], rel=1e-6)
+    assert O.comprehensive(np.r_[np.zeros(20), np.ones(9)], "benford_correlation") == 0.0
+    for wd in (2, 5, 10, 20):
+        t = np.arange(-4 * wd, 4 * wd + 1)
+        psi = 2 / (np.sqrt(3 * wd) * np.pi ** 0.25) * (1 - (t / wd) ** 2) * np.exp(-t ** 2 / (2 * wd ** 2))
+        conv = np.convolve(np.pad(x, 4 * wd, mode="reflect"), psi, mode="valid")
+        for c in (0, 3, 14):
+            assert O.comprehensive(x, "cwt_coefficients", {"widths": (wd,), "coeff": c, "w": wd}) == \
+                pytest.approx(conv[c], rel=1e-9, abs=1e-12), (wd, c)
+        assert O.comprehensive(np.full(60, 1.5), "cwt_coefficients",
+                               {"widths": (wd,), "coeff": 4, "w": wd}) == pytest.approx(1.5 * psi.sum())
+    assert O.comprehensive(x, "cwt_coefficients", {"widths": (5,), "coeff": 1, "w": 2}) == 0.0
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_cpu_trends_path_equals_oracle(dtype):
+    for w in _windows():
+        wt = torch.tensor(w, dtype=dtype)
+        x64 = wt.double().numpy()
+        N, B, S = w.shape
+        out = window_trends(wt, 0, True)
+        for name, grid in TRENDS_FEATURES.items():
+            for p in (grid or [None]):
+                got = trends_feature(name, p, out).numpy()
+                for b in range(B):
+                    for s in range(S):
+                        want = O.comprehensive(x64[:, b, s], name, p)
+                        g = float(got[b, s])
+                        if not math.isfinite(want):
+                            assert not math.isfinite(g), (name, p, N)
+                            continue
+                        tol = 1e-7 if name in ("friedrich_coefficients",) else 1e-11
+                        if name == "agg_linear_trend" and p["attr"] == "stderr":
+                            tol = 1e-7  # a perfect fit's residual is rounding, under a square root
+                        assert g == pytest.approx(want, rel=1e-7, abs=tol), (name, p, N, b, s, g, want)
 
 
 def _windows():
