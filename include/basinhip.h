@@ -517,6 +517,26 @@ enum {
 int32_t bh_window_trends(int32_t dtype, const void *traj, int64_t n_save, int64_t n, int32_t dim,
                          int64_t start, int32_t want_cwt, double *out, void *stream);
 
+/* Peak counts of each series of the same window: out (12, n, dim) float64,
+   rows 0..3 = the positions strictly greater than their s_r neighbours on
+   each side (number_peaks with n = s_r, src/pybasin/ts_torch/calculators/
+   torch_features_pattern.py:64-105; s_r in [0, 64], 0 counts nothing), rows
+   4..11 = the positions equal to the maximum of the 2 w + 1 samples about
+   them for w = 1..8 (the per-width counts of number_cwt_peaks, :154-190).
+   Windows that fit the 62 KB LDS budget; non-finite windows give NaN. */
+int32_t bh_window_peak_counts(int32_t dtype, const void *traj, int64_t n_save, int64_t n,
+                              int32_t dim, int64_t start, int32_t s0, int32_t s1, int32_t s2,
+                              int32_t s3, double *out, void *stream);
+
+/* permutation_entropy (tau = 1) of each series of the same window for the
+   embedding dimensions 3..7 (torch_features_entropy_complexity.py:11-53):
+   out (5, n, dim) float64, the entropy of the counts of the reference's
+   pattern ids (sum_k argsort_k (d-1-k)!, equal values in index order) over
+   ln d!; 0 for windows shorter than d, NaN for non-finite windows.  Windows
+   up to 4,052 float32 or 2,026 float64 rows. */
+int32_t bh_window_perm_entropy(int32_t dtype, const void *traj, int64_t n_save, int64_t n,
+                               int32_t dim, int64_t start, double *out, void *stream);
+
 /* Autocovariances of each series of the same window: out (max_lag + 1, n,
    dim) float64 holds C_k = sum_{i < w - k} (x_i - mean)(x_{i+k} - mean) for
    k = 0 .. max_lag (0 for k >= w), in double: the linear autocorrelation of

@@ -39,6 +39,7 @@ default fraction) would fail.
 from __future__ import annotations
 
 import ctypes
+import math
 import os
 import subprocess
 
@@ -674,4 +675,24 @@ def comprehensive(x: np.ndarray, name: str, params: dict | None = None, dtype=np
         pad = 4 * wd
         xp = np.pad(x, pad, mode="reflect" if n > pad else "edge")
         return float((xp[c: c + 2 * pad + 1] * psi).sum())
+    if name == "number_cwt_peaks":  # torch_features_pattern.py:154-190
+        widths = [wd for wd in range(1, int(p.get("max_width", 5)) + 1) if 2 * wd < n]
+        if not widths:
+            return 0.0
+        tot = 0
+        for wd in widths:
+            tot += sum(1 for i in range(wd, n - wd) if x[i] == x[i - wd: i + wd + 1].max())
+        return float(tot / len(widths))
+    if name == "permutation_entropy":  # torch_features_entropy_complexity.py:11-53
+        d = int(p.get("dimension", 3))
+        npat = n - d + 1
+        if npat <= 0:
+            return 0.0
+        counts: dict = {}
+        for i in range(npat):
+            order = sorted(range(d), key=lambda j: (x[i + j], j))  # argsort, ties in index order
+            pid = sum(order[k] * math.factorial(d - 1 - k) for k in range(d))
+            counts[pid] = counts.get(pid, 0) + 1
+        q = np.array(list(counts.values()), dtype=np.float64) / npat
+        return float(-(q * np.log(q)).sum() / math.log(math.factorial(d)))
     raise KeyError(name)

@@ -949,6 +949,73 @@ int32_t bh_window_trends(int32_t dtype, const void *traj, int64_t n_save, int64_
     return 0;
 }
 
+int32_t bh_window_peak_counts(int32_t dtype, const void *traj, int64_t n_save, int64_t n,
+                              int32_t dim, int64_t start, int32_t s0, int32_t s1, int32_t s2,
+                              int32_t s3, double *out, void *stream) {
+    g_err[0] = 0;
+    int32_t rc = check_series_args("bh_window_peak_counts", dtype, traj, n_save, n, dim, start);
+    if (rc) return rc;
+    if (!out) return fail(-5, "bh_window_peak_counts: null out");
+    const int32_t sup[4] = {s0, s1, s2, s3};
+    for (int k = 0; k < 4; k++)
+        if (sup[k] < 0 || sup[k] > 64) return fail(-5, "bh_window_peak_counts: supports must be in [0, 64]");
+    if (n == 0) return 0;
+    const int64_t w = n_save - start;
+    const size_t elem = dtype == BH_F32 ? 4 : 8;
+    int G = 0;
+    size_t lds = 0;
+    rc = series_launch(w, elem, &G, &lds, false);
+    if (rc) return rc;
+    const int64_t np = n * dim;
+    const int64_t blocks = (np + G - 1) / G;
+    if (blocks > 2147483647LL) return fail(-5, "bh_window_peak_counts: too many series");
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    const int4 sv = make_int4(s0, s1, s2, s3);
+    if (dtype == BH_F32)
+        hipLaunchKernelGGL(bh::win::peak_counts_kernel<float>, dim3((unsigned)blocks),
+                           dim3(bh::win::kSeriesBlock), lds, s, reinterpret_cast<const float *>(traj),
+                           n_save, n, dim, start, G, sv, out);
+    else
+        hipLaunchKernelGGL(bh::win::peak_counts_kernel<double>, dim3((unsigned)blocks),
+                           dim3(bh::win::kSeriesBlock), lds, s, reinterpret_cast<const double *>(traj),
+                           n_save, n, dim, start, G, sv, out);
+    BH_HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
+int32_t bh_window_perm_entropy(int32_t dtype, const void *traj, int64_t n_save, int64_t n,
+                               int32_t dim, int64_t start, double *out, void *stream) {
+    g_err[0] = 0;
+    int32_t rc = check_series_args("bh_window_perm_entropy", dtype, traj, n_save, n, dim, start);
+    if (rc) return rc;
+    if (!out) return fail(-5, "bh_window_perm_entropy: null out");
+    if (n == 0) return 0;
+    const int64_t w = n_save - start;
+    const int64_t elem = dtype == BH_F32 ? 4 : 8;
+    // the block's series rows plus one histogram of 5,910 16-bit counts per wave
+    const int64_t budget = 62 * 1024, hist = 4 * (5910 / 2) * 4;
+    if (hist + w * elem > budget)
+        return fail(-5, "bh_window_perm_entropy: window of %lld rows exceeds the LDS budget (at most %lld rows)",
+                    (long long)w, (long long)((budget - hist) / elem));
+    int64_t G = (budget - hist) / (w * elem);
+    if (G > 16) G = 16;
+    const size_t lds = (size_t)((G * w * elem + 15) / 16 * 16 + hist);
+    const int64_t np = n * dim;
+    const int64_t blocks = (np + G - 1) / G;
+    if (blocks > 2147483647LL) return fail(-5, "bh_window_perm_entropy: too many series");
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    if (dtype == BH_F32)
+        hipLaunchKernelGGL(bh::win::perm_entropy_kernel<float>, dim3((unsigned)blocks),
+                           dim3(bh::win::kSeriesBlock), lds, s, reinterpret_cast<const float *>(traj),
+                           n_save, n, dim, start, (int)G, out);
+    else
+        hipLaunchKernelGGL(bh::win::perm_entropy_kernel<double>, dim3((unsigned)blocks),
+                           dim3(bh::win::kSeriesBlock), lds, s, reinterpret_cast<const double *>(traj),
+                           n_save, n, dim, start, (int)G, out);
+    BH_HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
 extern "C++" template <class T>
 static void launch_sorted(int64_t P, int64_t blocks, size_t lds, hipStream_t s, const T *traj,
                           int64_t n_save, int64_t n, int32_t dim, int64_t start, int64_t G,

@@ -1115,6 +1115,161 @@ __global__ void __launch_bounds__(256)
     }
 }
 
+// ---- peaks of wide support, ordinal patterns -------------------------------
+
+constexpr int kPeakSupports = 4;
+constexpr int kPeakWidths = 8;
+constexpr int kPeakCap = 64;
+
+// One wave per series staged in LDS, one position per lane at a time: the
+// strict peak radius of x_i (the largest k <= 64 with x_i > x_{i +- j} for all
+// j <= k inside the window) and its non-strict radius (x_i >= x_{i +- j}, at
+// most 8).  out (12, n, dim): rows 0..3 count the positions whose strict
+// radius reaches supports[r] (number_peaks, the reference's find_peak_mask:
+// torch_features_pattern.py:64-105; a support of 0 counts nothing), rows
+// 4..11 the positions whose non-strict radius reaches width 1..8 (the
+// per-width counts of number_cwt_peaks, :154-190: x_i equal to the maximum of
+// x_{i-w} .. x_{i+w}, the w first and last positions left out).  A window
+// holding a non-finite value gives NaN.
+template <class T>
+__global__ void __launch_bounds__(kSeriesBlock)
+    peak_counts_kernel(const T *traj, int64_t n_save, int64_t n, int S, int64_t start, int G,
+                       int4 supports, double *out) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int64_t w = n_save - start;
+    const int64_t np = n * S;
+    const int64_t p0 = (int64_t)blockIdx.x * G;
+    T *lds = reinterpret_cast<T *>(smem);
+    load_series<T>(traj, n, S, start, w, p0, G, np, lds);
+    __syncthreads();
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int sup[kPeakSupports] = {supports.x, supports.y, supports.z, supports.w};
+    for (int g = wave; g < G; g += kSeriesBlock / 64) {
+        const int64_t p = p0 + g;
+        if (p >= np) break;  // wave-uniform
+        const T *x = lds + (int64_t)g * w;
+        bool bad = false;
+        for (int64_t i = lane; i < w; i += 64) bad |= !isfinite((double)x[i]);
+        if (__any(bad)) {
+            if (lane < kPeakSupports + kPeakWidths) out[lane * np + p] = NAN;
+            continue;
+        }
+        double cs[kPeakSupports] = {0, 0, 0, 0};
+        double cw[kPeakWidths] = {0, 0, 0, 0, 0, 0, 0, 0};
+        for (int64_t i = lane; i < w; i += 64) {
+            const T c = x[i];
+            int ls = 0, rs = 0, ln = 0, rn = 0;
+            while (ls < kPeakCap && i - ls - 1 >= 0 && c > x[i - ls - 1]) ls++;
+            while (rs < kPeakCap && i + rs + 1 < w && c > x[i + rs + 1]) rs++;
+            while (ln < kPeakWidths && i - ln - 1 >= 0 && c >= x[i - ln - 1]) ln++;
+            while (rn < kPeakWidths && i + rn + 1 < w && c >= x[i + rn + 1]) rn++;
+            const int r = ls < rs ? ls : rs, q = ln < rn ? ln : rn;
+#pragma unroll
+            for (int k = 0; k < kPeakSupports; k++) cs[k] += (sup[k] > 0 && r >= sup[k]) ? 1.0 : 0.0;
+#pragma unroll
+            for (int k = 0; k < kPeakWidths; k++) cw[k] += q >= k + 1 ? 1.0 : 0.0;
+        }
+#pragma unroll
+        for (int k = 0; k < kPeakSupports; k++) cs[k] = wave_sum(cs[k]);
+#pragma unroll
+        for (int k = 0; k < kPeakWidths; k++) cw[k] = wave_sum(cw[k]);
+        if (lane == 0) {
+#pragma unroll
+            for (int k = 0; k < kPeakSupports; k++) out[k * np + p] = cs[k];
+#pragma unroll
+            for (int k = 0; k < kPeakWidths; k++) out[(kPeakSupports + k) * np + p] = cw[k];
+        }
+    }
+}
+
+constexpr int kPermDims = 5;  // embedding dimensions 3 .. 7
+// first histogram bin of dimension 3 + k: sums of d! for the smaller d
+__constant__ int kPermBase[kPermDims + 1] = {0, 6, 30, 150, 870, 5910};
+__constant__ int kFact[8] = {1, 1, 2, 6, 24, 120, 720, 5040};
+
+// permutation_entropy (torch_features_entropy_complexity.py:11-53) with
+// tau = 1 and dimensions 3 .. 7, one wave per series staged in LDS: each
+// pattern x_i .. x_{i+d-1} gets the reference's id sum_k argsort_k (d-1-k)!
+// (argsort ascending, equal values in index order), counted in the wave's
+// LDS histogram (16-bit counts, two per word); the entropy of the counts over
+// the n - d + 1 patterns is divided by ln d!.  out (5, n, dim); a window
+// shorter than d gives 0, a non-finite window NaN.
+template <class T>
+__global__ void __launch_bounds__(kSeriesBlock)
+    perm_entropy_kernel(const T *traj, int64_t n_save, int64_t n, int S, int64_t start, int G,
+                        double *out) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int64_t w = n_save - start;
+    const int64_t np = n * S;
+    const int64_t p0 = (int64_t)blockIdx.x * G;
+    T *lds = reinterpret_cast<T *>(smem);
+    unsigned *hbase = reinterpret_cast<unsigned *>(smem + ((int64_t)G * w * sizeof(T) + 15) / 16 * 16);
+    load_series<T>(traj, n, S, start, w, p0, G, np, lds);
+    __syncthreads();
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    constexpr int kWords = 5910 / 2;
+    unsigned *hist = hbase + wave * kWords;
+    for (int g = wave; g < G; g += kSeriesBlock / 64) {
+        const int64_t p = p0 + g;
+        if (p >= np) break;  // wave-uniform
+        const T *x = lds + (int64_t)g * w;
+        bool bad = false;
+        for (int64_t i = lane; i < w; i += 64) bad |= !isfinite((double)x[i]);
+        if (__any(bad)) {
+            if (lane < kPermDims) out[lane * np + p] = NAN;
+            continue;
+        }
+        for (int k = lane; k < kWords; k += 64) hist[k] = 0u;
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        for (int64_t i = lane; i + 2 < w; i += 64) {
+            T v[7];
+#pragma unroll
+            for (int j = 0; j < 7; j++) v[j] = i + j < w ? x[i + j] : (T)0;
+#pragma unroll
+            for (int k = 0; k < kPermDims; k++) {
+                const int d = 3 + k;
+                if (i + d > w) break;
+                // rank of element j among the first d (ties by index); the
+                // pattern id is sum_j j * (d - 1 - rank_j)!
+                int id = 0;
+#pragma unroll
+                for (int j = 0; j < 7; j++) {
+                    if (j >= d) break;
+                    int rk = 0;
+#pragma unroll
+                    for (int m = 0; m < 7; m++) {
+                        if (m >= d) break;
+                        rk += (v[m] < v[j] || (v[m] == v[j] && m < j)) ? 1 : 0;
+                    }
+                    id += j * kFact[d - 1 - rk];
+                }
+                const int bin = kPermBase[k] + id;
+                atomicAdd(&hist[bin >> 1], 1u << (16 * (bin & 1)));
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+#pragma unroll
+        for (int k = 0; k < kPermDims; k++) {
+            const int d = 3 + k;
+            const double npat = (double)(w - d + 1);
+            double e = 0.0;
+            for (int b = kPermBase[k] + lane; b < kPermBase[k + 1]; b += 64) {
+                const unsigned c = (hist[b >> 1] >> (16 * (b & 1))) & 0xffffu;
+                if (c) {
+                    const double q = (double)c / npat;
+                    e = fma(-q, log(q), e);
+                }
+            }
+            e = wave_sum(e);
+            if (lane == 0) out[k * np + p] = npat > 0.0 ? e / log((double)kFact[d]) : 0.0;
+        }
+        __builtin_amdgcn_wave_barrier();  // the histogram is cleared for the next series
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    }
+}
+
 // ---- sorted-window statistics: duplicates and change quantiles -------------
 
 // the six quantile levels whose corridors change_quantiles takes

@@ -61,7 +61,7 @@ COMPREHENSIVE_FEATURES: dict[str, list[dict] | None] = {
     "longest_strike_above_mean": None,
     "longest_strike_below_mean": None,
     "number_crossing_m": [{"m": 0}, {"m": -1}, {"m": 1}],
-    "number_peaks": [{"n": n} for n in [1, 3, 5]],
+    "number_peaks": [{"n": n} for n in [1, 3, 5, 10, 50]],
     "autocorrelation": [{"lag": lag} for lag in range(10)],
     "cid_ce": [{"normalize": True}, {"normalize": False}],
     "ratio_beyond_r_sigma": [{"r": x} for x in [0.5, 1, 1.5, 2, 2.5, 3, 5, 6, 7, 10]],
@@ -125,6 +125,14 @@ TRENDS_FEATURES: dict[str, list[dict] | None] = {
                          for c in range(15)],
 }
 COMPREHENSIVE_FEATURES.update(TRENDS_FEATURES)
+# peaks of wide support and ordinal patterns (bh_window_peak_counts,
+# bh_window_perm_entropy); number_peaks with n other than 1, 3, 5 joins them
+PEAK_FEATURES: dict[str, list[dict]] = {
+    "number_cwt_peaks": [{"max_width": n} for n in [1, 5]],
+    "permutation_entropy": [{"tau": 1, "dimension": x} for x in [3, 4, 5, 6, 7]],
+}
+COMPREHENSIVE_FEATURES.update(PEAK_FEATURES)
+_BLOCK_PEAKS = (1, 3, 5)
 TR = dict(ALT=0, FRIED=48, LANGEVIN=52, BENFORD=53, CWT=54)
 BH_TR_SLOTS = 114
 # the remaining regression calculator formed from the window-feature slots
@@ -157,6 +165,18 @@ def check_params(name: str, params: dict | None) -> None:
         return
     if name == "mean_n_absolute_max" and 1 <= int(params.get("number_of_maxima", 0)) <= 7:
         return
+    if name == "number_peaks":
+        if isinstance(params.get("n"), int) and 1 <= params["n"] <= 64:
+            return
+        raise ValueError(f"number_peaks: n must be an integer in [1, 64], got {params}")
+    if name == "number_cwt_peaks":
+        if isinstance(params.get("max_width"), int) and 1 <= params["max_width"] <= 8:
+            return
+        raise ValueError(f"number_cwt_peaks: max_width must be in [1, 8], got {params}")
+    if name == "permutation_entropy":
+        if params.get("tau", 1) == 1 and params.get("dimension") in (3, 4, 5, 6, 7):
+            return
+        raise ValueError(f"permutation_entropy: tau 1 and dimension 3..7, got {params}")
     if name in ("autocorrelation", "partial_autocorrelation") and 0 <= int(params.get("lag", -1)) <= 9:
         return
     if name == "energy_ratio_by_chunks" and int(params.get("num_segments", 0)) == 10 and \
@@ -926,10 +946,119 @@ def trends_feature(name: str, params: dict | None, out: torch.Tensor) -> torch.T
     raise KeyError(f"feature {name!r} is not a chunk-trend / fit feature")
 
 
-def feature_family(name: str) -> str:
+def _peak_counts_cpu(w: torch.Tensor, supports: list) -> torch.Tensor:
+    """The outputs of bh_window_peak_counts for a CPU window (N, B, S)."""
+    x = w.double()
+    N, B, S = x.shape
+    out = torch.zeros((12, B, S), dtype=torch.float64)
+    for r, k in enumerate(supports):
+        if k <= 0 or 2 * k >= N:
+            continue
+        u = x.unfold(0, 2 * k + 1, 1)  # (N - 2k, B, S, 2k + 1)
+        c = u[..., k]
+        out[r] = ((c > u[..., :k].max(-1).values) & (c > u[..., k + 1:].max(-1).values)).double().sum(0)
+    for wd in range(1, 9):
+        if 2 * wd >= N:
+            continue
+        u = x.unfold(0, 2 * wd + 1, 1)
+        out[4 + wd - 1] = (u[..., wd] == u.max(-1).values).double().sum(0)
+    ok = torch.isfinite(x).all(0)
+    return torch.where(ok, out, torch.full_like(out, float("nan")))
+
+
+def window_peak_counts(y: torch.Tensor, start: int, supports: list) -> torch.Tensor:
+    """(12, B, S) float64: strict peak counts for up to four supports (rows
+    0..3) and the per-width counts of number_cwt_peaks (rows 4..11) of the
+    window rows [start, N) of y (bh_window_peak_counts on the device)."""
+    N, B, S = y.shape
+    if not 0 <= start < N:
+        raise ValueError("empty feature window")
+    sup = (list(supports) + [0, 0, 0, 0])[:4]
+    if y.device.type != "cuda":
+        return _peak_counts_cpu(y[start:], sup)
+    y = y.contiguous()
+    out = torch.empty((12, B, S), dtype=torch.float64, device=y.device)
+    dt = _lib.BH_F32 if y.dtype == torch.float32 else _lib.BH_F64
+    s = torch.cuda.current_stream(y.device)
+    with torch.cuda.device(y.device):
+        _lib.check(_lib.lib().bh_window_peak_counts(dt, y.data_ptr(), N, B, S, int(start), *sup,
+                                                    out.data_ptr(), s.cuda_stream),
+                   "bh_window_peak_counts")
+    return out
+
+
+_FACT = [1, 1, 2, 6, 24, 120, 720, 5040]
+
+
+def _perm_entropy_cpu(w: torch.Tensor) -> torch.Tensor:
+    """The outputs of bh_window_perm_entropy for a CPU window (N, B, S)."""
+    x = w.double()
+    N, B, S = x.shape
+    out = torch.zeros((5, B, S), dtype=torch.float64)
+    for k, d in enumerate(range(3, 8)):
+        npat = N - d + 1
+        if npat <= 0:
+            continue
+        emb = x.unfold(0, d, 1)  # (npat, B, S, d)
+        order = torch.argsort(emb, dim=-1, stable=True)
+        mult = torch.tensor([_FACT[d - 1 - i] for i in range(d)], dtype=torch.long)
+        ids = (order * mult).sum(-1).reshape(npat, B * S)
+        cnt = torch.zeros((B * S, _FACT[d]), dtype=torch.float64)
+        cnt.scatter_add_(1, ids.T, torch.ones_like(ids.T, dtype=torch.float64))
+        q = cnt / npat
+        e = -(q * torch.where(q > 0, q.log(), torch.zeros_like(q))).sum(1)
+        out[k] = (e / math.log(_FACT[d])).reshape(B, S)
+    ok = torch.isfinite(x).all(0)
+    return torch.where(ok, out, torch.full_like(out, float("nan")))
+
+
+def window_perm_entropy(y: torch.Tensor, start: int) -> torch.Tensor:
+    """(5, B, S) float64 permutation entropies (tau 1, dimensions 3..7) of the
+    window rows [start, N) of y (bh_window_perm_entropy on the device)."""
+    N, B, S = y.shape
+    if not 0 <= start < N:
+        raise ValueError("empty feature window")
+    if y.device.type != "cuda":
+        return _perm_entropy_cpu(y[start:])
+    y = y.contiguous()
+    out = torch.empty((5, B, S), dtype=torch.float64, device=y.device)
+    dt = _lib.BH_F32 if y.dtype == torch.float32 else _lib.BH_F64
+    s = torch.cuda.current_stream(y.device)
+    with torch.cuda.device(y.device):
+        _lib.check(_lib.lib().bh_window_perm_entropy(dt, y.data_ptr(), N, B, S, int(start),
+                                                     out.data_ptr(), s.cuda_stream),
+                   "bh_window_perm_entropy")
+    return out
+
+
+def peak_feature(name: str, params: dict | None, counts: dict, perm: torch.Tensor | None,
+                 n_win: int) -> torch.Tensor:
+    """number_peaks (wide supports), number_cwt_peaks and
+    permutation_entropy (B, S) float64 (torch_features_pattern.py:64-190,
+    torch_features_entropy_complexity.py:11-53).  counts: support -> its
+    row of strict counts, and "widths" -> the (8, B, S) per-width counts."""
+    p = params or {}
+    if name == "number_peaks":
+        return counts[int(p["n"])]
+    if name == "number_cwt_peaks":
+        valid = [wd for wd in range(1, int(p.get("max_width", 5)) + 1) if 2 * wd < n_win]
+        cw = counts["widths"]
+        if not valid:
+            return torch.where(torch.isnan(cw[0]), cw[0], torch.zeros_like(cw[0]))
+        return sum(cw[wd - 1] for wd in valid) / len(valid)
+    if name == "permutation_entropy":
+        return perm[int(p["dimension"]) - 3]
+    raise KeyError(f"feature {name!r} is not a peak / pattern feature")
+
+
+def feature_family(name: str, params: dict | None = None) -> str:
     """Which kernel output a comprehensive feature is formed from: "block"
-    (bh_window_features), "spectrum" (bh_window_spectrum) or "sorted"
-    (bh_window_sorted)."""
+    (bh_window_features), "spectrum", "sorted", "autocov", "trends" or
+    "peaks" (number_peaks of supports other than 1, 3, 5 among them)."""
+    if name == "number_peaks" and params and int(params.get("n", 1)) not in _BLOCK_PEAKS:
+        return "peaks"
+    if name in PEAK_FEATURES:
+        return "peaks"
     if name in SPECTRUM_FEATURES:
         return "spectrum"
     if name in SORTED_FEATURES:
@@ -982,8 +1111,28 @@ class WindowBlocks:
         cwt = any(name == "cwt_coefficients" for _, name, _ in self.plan)
         return self._get("trends", lambda: window_trends(self.y, self.start, cwt))
 
+    def peak_counts(self) -> dict:
+        def make():
+            sup = sorted({int(p["n"]) for _, nm, p in self.plan
+                          if nm == "number_peaks" and p and int(p["n"]) not in _BLOCK_PEAKS})
+            rows: dict = {}
+            for i in range(0, max(len(sup), 1), 4):
+                chunk = sup[i:i + 4]
+                out = window_peak_counts(self.y, self.start, chunk)
+                rows.update({k: out[r] for r, k in enumerate(chunk)})
+                rows["widths"] = out[4:]
+            return rows
+        return self._get("peaks", make)
+
+    def perm_entropy(self):
+        return self._get("perm", lambda: window_perm_entropy(self.y, self.start))
+
     def feature(self, name: str, params: dict | None) -> torch.Tensor:
         """Comprehensive feature `name` (B, S) float64 of the window."""
+        if feature_family(name, params) == "peaks":
+            perm = self.perm_entropy() if name == "permutation_entropy" else None
+            counts = self.peak_counts() if name != "permutation_entropy" else {}
+            return peak_feature(name, params, counts, perm, self.n)
         if name in TRENDS_FEATURES:
             return trends_feature(name, params, self.trends())
         if name in AUTOCOV_FEATURES:

@@ -148,9 +148,9 @@ def _check_against_oracle(blk, w64, start, tag, series_dtype=np.float64):
     from pybasin_amd.window_features import feature_family
 
     for name, grid in COMPREHENSIVE_FEATURES.items():
-        if feature_family(name) != "block":
-            continue  # the spectrum and sorted-window kernels, checked below
         for p in (grid or [None]):
+            if feature_family(name, p) != "block":
+                continue  # the other window kernels, checked below
             if name in ("skewness", "kurtosis", "augmented_dickey_fuller") and N <= 3:
                 continue  # noise amplified by the reference's 1e-10 guards (and n - 3 = 0)
             got = comprehensive_feature(name, p, blk, N, med).numpy()
@@ -565,3 +565,75 @@ def test_window_trends_on_integrated_duffing(cuda_device, dtype):
     want = np.stack([trends_feature(nm, p, out).numpy()[:, s] for s in range(2)
                      for nm, ps in feats.items() for p in (ps or [None])], 1)
     np.testing.assert_array_equal(f, want.astype(np.float32))
+
+
+def _check_peaks(counts, perm, x, tag):
+    from pybasin_amd.window_features import PEAK_FEATURES, peak_feature
+
+    N, B, S = x.shape
+    plist = [("number_peaks", {"n": k}) for k in (10, 50, 1, 64)] + \
+        [(nm, p) for nm, g in PEAK_FEATURES.items() for p in g] + \
+        [("number_cwt_peaks", {"max_width": 8})]
+    for name, p in plist:
+        got = peak_feature(name, p, counts, perm, N).numpy()
+        for b in range(B):
+            for s in range(S):
+                want = O.comprehensive(x[:, b, s], name, p)
+                g = float(got[b, s])
+                if not math.isfinite(want):
+                    assert not math.isfinite(g), (tag, name, p, b, s)
+                else:
+                    assert g == pytest.approx(want, rel=1e-12, abs=1e-12), (tag, name, p, b, s, g, want)
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_window_peaks_and_patterns_kernels_equal_oracle(cuda_device, dtype):
+    """Strict peaks of supports 1, 10, 50, 64 (runs capped at 64), plateau
+    maxima of widths 1..8, permutation entropies of dimensions 3..7 (ties in
+    index order, the reference's colliding pattern ids) against the oracle;
+    the CPU path agrees exactly."""
+    from pybasin_amd.window_features import window_peak_counts, window_perm_entropy
+
+    rng = np.random.default_rng(43)
+    cases = list(_synthetic_windows()) + [
+        ("smooth", np.sin(np.arange(700)[:, None, None] * rng.uniform(0.01, 0.3, size=(1, 6, 2)))),
+        ("plateaus", np.round(np.sin(np.arange(400)[:, None, None] * 0.05 + rng.normal(size=(1, 5, 2))) * 3))]
+    for tag, w in cases:
+        wt = torch.tensor(w, dtype=dtype)
+        x = wt.double().numpy()
+        for start in (0, 1) if w.shape[0] > 1 else (0,):
+            c = window_peak_counts(wt.cuda(), start, [10, 50, 1, 64]).cpu()
+            counts = {10: c[0], 50: c[1], 1: c[2], 64: c[3], "widths": c[4:]}
+            perm = window_perm_entropy(wt.cuda(), start).cpu()
+            _check_peaks(counts, perm, x[start:], f"{tag}/{start}")
+            cc = window_peak_counts(wt, start, [10, 50, 1, 64])
+            assert torch.equal(torch.nan_to_num(c, 7.0), torch.nan_to_num(cc, 7.0)), tag
+            cp = window_perm_entropy(wt, start)
+            np.testing.assert_allclose(np.nan_to_num(perm.numpy(), nan=7.0),
+                                       np.nan_to_num(cp.numpy(), nan=7.0), rtol=1e-12, atol=1e-12)
+
+
+def test_window_perm_entropy_limit_is_an_error(cuda_device):
+    from pybasin_amd.window_features import window_perm_entropy
+
+    with pytest.raises(ValueError, match="LDS budget"):
+        window_perm_entropy(torch.zeros(4053, 3, 1, dtype=torch.float32, device="cuda"), 0)
+    window_perm_entropy(torch.zeros(4052, 3, 1, dtype=torch.float32, device="cuda"), 0)
+
+
+def test_window_peaks_on_integrated_duffing(cuda_device):
+    """Wide-support peaks, cwt peaks and permutation entropies of the
+    integrated Duffing windows through the extractor equal the CPU path."""
+    res = _duffing(torch.float32)
+    feats = {"number_peaks": [{"n": 10}, {"n": 50}, {"n": 3}], "number_cwt_peaks": [{"max_width": 5}],
+             "permutation_entropy": [{"tau": 1, "dimension": 5}]}
+    fx = StatFeatureExtractor(time_steady=300.0, features=feats)
+    f = fx.extract_features(Solution(res.y[0], res.t, res.y)).cpu().numpy()
+    sol_cpu = Solution(res.y[0].cpu(), res.t.cpu(), res.y.cpu())
+    g = StatFeatureExtractor(time_steady=300.0, features=feats).extract_features(sol_cpu).numpy()
+    np.testing.assert_array_equal(f, g)
+    x = res.y.double().cpu().numpy()[res.steady_idx:]
+    for b in range(4):
+        assert f[b, 0] == O.comprehensive(x[:, b, 0], "number_peaks", {"n": 10})
+        assert f[b, 4] == pytest.approx(O.comprehensive(x[:, b, 0], "permutation_entropy",
+                                                        {"tau": 1, "dimension": 5}), rel=1e-6)

@@ -24,20 +24,21 @@ import torch
 from oracle import oracle as O
 from pybasin_amd.features import StatFeatureExtractor, Solution
 from pybasin_amd.window_features import (AUTOCOV_FEATURES, COMPREHENSIVE_FEATURES,
-                                         SORTED_FEATURES, SPECTRUM_FEATURES, TRENDS_FEATURES,
-                                         _block_cpu, autocov_feature, check_params,
-                                         comprehensive_feature, feature_family, sorted_feature,
-                                         spectrum_feature, trends_feature, window_autocov,
-                                         window_sorted, window_spectrum, window_trends)
+                                         PEAK_FEATURES, SORTED_FEATURES, SPECTRUM_FEATURES,
+                                         TRENDS_FEATURES, _block_cpu, autocov_feature,
+                                         check_params, comprehensive_feature, feature_family,
+                                         peak_feature, sorted_feature, spectrum_feature,
+                                         trends_feature, window_autocov, window_peak_counts,
+                                         window_perm_entropy, window_sorted, window_spectrum,
+                                         window_trends)
 
 
 def _all_params():
     """The calculators formed from the window-feature slots."""
     for name, grid in COMPREHENSIVE_FEATURES.items():
-        if feature_family(name) != "block":
-            continue
         for p in (grid or [None]):
-            yield name, p
+            if feature_family(name, p) == "block":
+                yield name, p
 
 
 def _spectrum_params(nb=None):
@@ -501,6 +502,58 @@ def test_cpu_trends_path_equals_oracle(dtype):
                         if name == "agg_linear_trend" and p["attr"] == "stderr":
                             tol = 1e-7  # a perfect fit's residual is rounding, under a square root
                         assert g == pytest.approx(want, rel=1e-7, abs=tol), (name, p, N, b, s, g, want)
+
+
+def test_oracle_peaks_and_patterns_pins():
+    """number_peaks of wide support against scipy.signal.argrelmax (edges
+    left out as the reference does); number_cwt_peaks against
+    scipy.ndimage.maximum_filter1d; permutation_entropy's pattern ids on
+    hand-built series: a monotone series has entropy 0, and the windows
+    (0, 1, 3) and (1, 3, 2) share the reference's id 3 (a Lehmer code would
+    tell them apart and give ln 2 / ln 6)."""
+    from scipy.ndimage import maximum_filter1d
+    from scipy.signal import argrelmax
+
+    rng = np.random.default_rng(14)
+    x = np.sin(np.arange(800) * 0.07) + 0.3 * rng.normal(size=800)
+    for k in (10, 50):
+        idx = argrelmax(x, order=k)[0]
+        want = np.sum((idx >= k) & (idx < x.size - k))
+        assert O.comprehensive(x, "number_peaks", {"n": k}) == want
+    xr = np.round(x * 4) / 4  # plateaus: equal maxima count for the cwt peaks
+    for mw in (1, 5, 8):
+        tot = 0
+        for wd in range(1, mw + 1):
+            mf = maximum_filter1d(xr, size=2 * wd + 1, mode="nearest")
+            tot += np.sum(xr[wd:-wd] == mf[wd:-wd])
+        assert O.comprehensive(xr, "number_cwt_peaks", {"max_width": mw}) == pytest.approx(tot / mw)
+    assert O.comprehensive(x[:3], "number_cwt_peaks", {"max_width": 5}) == \
+        pytest.approx(float(x[1] == x[:3].max()))  # only w = 1 fits
+    for d in range(3, 8):
+        assert O.comprehensive(np.arange(30.0), "permutation_entropy", {"tau": 1, "dimension": d}) == 0.0
+    assert O.comprehensive(np.array([0.0, 1, 3, 2]), "permutation_entropy", {"tau": 1, "dimension": 3}) == 0.0
+    assert O.comprehensive(np.array([0.0, 2, 1, 3]), "permutation_entropy", {"tau": 1, "dimension": 3}) == \
+        pytest.approx(math.log(2) / math.log(6))
+    assert O.comprehensive(np.array([0.0, 1.0]), "permutation_entropy", {"tau": 1, "dimension": 3}) == 0.0
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_cpu_peaks_path_equals_oracle(dtype):
+    for w in _windows():
+        wt = torch.tensor(w, dtype=dtype)
+        x64 = wt.double().numpy()
+        N, B, S = w.shape
+        c = window_peak_counts(wt, 0, [10, 50])
+        counts = {10: c[0], 50: c[1], "widths": c[4:]}
+        perm = window_perm_entropy(wt, 0)
+        plist = [("number_peaks", {"n": 10}), ("number_peaks", {"n": 50})] + \
+            [(nm, p) for nm, g in PEAK_FEATURES.items() for p in g]
+        for name, p in plist:
+            got = peak_feature(name, p, counts, perm, N).numpy()
+            for b in range(B):
+                for s in range(S):
+                    want = O.comprehensive(x64[:, b, s], name, p)
+                    assert got[b, s] == pytest.approx(want, rel=1e-12, abs=1e-12), (name, p, N, b, s)
 
 
 def _windows():
