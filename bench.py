@@ -92,6 +92,7 @@ def parse():
                     help="duffing: also write the full trajectories to HBM (reports write GB/s)")
     ap.add_argument("--workload", choices=["surrogate", "duffing", "pendulum_c1", "lorenz", "rossler",
                                            "mlp_infer", "window_features", "window_spectrum",
+                                           "comprehensive",
                                            "hdbscan", "estimator", "estimator_defaults"],
                     default="surrogate")
     ap.add_argument("--gate", type=float, default=0.999,
@@ -732,6 +733,77 @@ def window_spectrum_workload(args):
     print(json.dumps(out), flush=True)
 
 
+def comprehensive_workload(args):
+    """Every calculator of the reference's comprehensive set built here
+    (settings.py:223-340) over the kept Duffing window (n = 100,000 ICs + 5
+    templates, float32, 1,000 rows, 2 states), integrated once outside the
+    timed region; a step = all window kernels (window-feature slots,
+    spectrum, sort + change quantiles, autocovariances, chunk trends / fits /
+    wavelets, peak counts, ordinal patterns, the median) plus every named
+    feature of every state.  Each kernel is also timed alone with HIP events.
+    Roofline: HBM over the kernels' window reads (block 2, spectrum 1, sort 1,
+    change quantiles 2, autocovariances 1, trends 2, peaks 1, patterns 1) and
+    their float64 outputs, against the summed kernel time."""
+    from pybasin_amd.features import window_feature
+    from pybasin_amd.window_features import (COMPREHENSIVE_FEATURES, WindowBlocks, window_autocov,
+                                             window_feature_block, window_peak_counts,
+                                             window_perm_entropy, window_sorted, window_spectrum,
+                                             window_trends)
+
+    dev = torch.device("cuda:0")
+    y = _duffing_window(args, dev)
+    N, B, S = y.shape
+    names = [(name, p) for name, grid in COMPREHENSIVE_FEATURES.items() for p in (grid or [None])]
+    plan = [(0, name, p) for name, p in names]
+
+    def step():
+        wb = WindowBlocks(y, 0, plan, median=lambda: window_feature("median", None, y, 0))
+        return [wb.feature(name, p) for name, p in names]
+
+    for _ in range(max(1, args.warmup)):
+        step()
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream(dev)
+    kernels = {
+        "window_features": (lambda: window_feature_block(y, 0), 2, 100),
+        "spectrum": (lambda: window_spectrum(y, 0, 100, True), 1, 205),
+        "sorted+change_quantiles": (lambda: window_sorted(y, 0, True), 3, 71),
+        "autocov": (lambda: window_autocov(y, 0, 40), 1, 41),
+        "trends": (lambda: window_trends(y, 0, True), 2, 114),
+        "peak_counts": (lambda: window_peak_counts(y, 0, [10, 50]), 1, 12),
+        "perm_entropy": (lambda: window_perm_entropy(y, 0), 1, 5),
+    }
+    kernel_ms, alg_bytes = {}, 0.0
+    for name, (fn, reads, rows) in kernels.items():
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(args.steps):
+            fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        kernel_ms[name] = round(e0.elapsed_time(e1) / args.steps, 3)
+        alg_bytes += reads * N * B * S * y.element_size() + rows * 8.0 * B * S
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    ms = 1e3 * (time.perf_counter() - t0) / args.steps
+    ksum = sum(kernel_ms.values())
+    out = {
+        "metric": "comprehensive window features, series/sec", "value": round(B * S / (ms * 1e-3), 1),
+        "unit": "series/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(ms, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": args.dtype, "data": "synthetic: kept Duffing windows of seeded-uniform ICs",
+        "config": {"workload": "comprehensive_duffing", "n_ics": B, "window_rows": N, "states": S,
+                   "calculators": len(COMPREHENSIVE_FEATURES), "features_per_state": len(names)},
+        "kernels_ms": kernel_ms,
+        "roofline": {"bound": "hbm", "achieved": round(alg_bytes / (ksum * 1e-3) / 1e9, 1),
+                     "peak": 8000.0, "unit": "GB/s", "frac": round(alg_bytes / (ksum * 1e-3) / 8e12, 4),
+                     "traffic": None, "kernel_ms": round(ksum, 3), "algorithmic_bytes": alg_bytes},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def window_features_workload(args):
     """The comprehensive window features of the reference's comprehensive set
     (bh_window_features, then the named features on the host) over the kept
@@ -740,14 +812,14 @@ def window_features_workload(args):
     step = the kernel's two passes over the (1000, n, 2) window plus the 128
     named features per state.  Roofline: HBM, two reads of the window
     (algorithmic bytes) plus the 90 float64 slots written per series."""
-    from pybasin_amd.window_features import (COMPREHENSIVE_FEATURES, SPECTRUM_FEATURES,
-                                             comprehensive_feature, window_feature_block)
+    from pybasin_amd.window_features import (COMPREHENSIVE_FEATURES, comprehensive_feature,
+                                             feature_family, window_feature_block)
 
     dev = torch.device("cuda:0")
     y = _duffing_window(args, dev)
     N, B, S = y.shape
     plan = [(name, p) for name, grid in COMPREHENSIVE_FEATURES.items() for p in (grid or [None])
-            if name != "symmetry_looking" and name not in SPECTRUM_FEATURES]
+            if name != "symmetry_looking" and feature_family(name, p) == "block"]
     stream = torch.cuda.current_stream(dev)
 
     def step():
@@ -1300,7 +1372,8 @@ def main():
                  "estimator_defaults": estimator_defaults_workload, "hdbscan": hdbscan_workload,
                  "mlp_infer": mlp_infer_workload, "rossler": rossler_workload,
                  "window_features": window_features_workload,
-                 "window_spectrum": window_spectrum_workload}
+                 "window_spectrum": window_spectrum_workload,
+                 "comprehensive": comprehensive_workload}
     workloads[args.workload](args)
 
 

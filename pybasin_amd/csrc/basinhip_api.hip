@@ -961,11 +961,15 @@ int32_t bh_window_peak_counts(int32_t dtype, const void *traj, int64_t n_save, i
         if (sup[k] < 0 || sup[k] > 64) return fail(-5, "bh_window_peak_counts: supports must be in [0, 64]");
     if (n == 0) return 0;
     const int64_t w = n_save - start;
-    const size_t elem = dtype == BH_F32 ? 4 : 8;
-    int G = 0;
-    size_t lds = 0;
-    rc = series_launch(w, elem, &G, &lds, false);
-    if (rc) return rc;
+    const int64_t elem = dtype == BH_F32 ? 4 : 8;
+    // the block's series rows plus one list of w candidate positions per wave
+    const int64_t budget = 62 * 1024, lists = 4 * w * 4;
+    if (lists + w * elem > budget)
+        return fail(-5, "bh_window_peak_counts: window of %lld rows exceeds the LDS budget (at most %lld rows)",
+                    (long long)w, (long long)(budget / (16 + elem)));
+    int64_t G = (budget - lists) / (w * elem);
+    if (G > 16) G = 16;
+    const size_t lds = (size_t)((G * w * elem + 15) / 16 * 16 + lists);
     const int64_t np = n * dim;
     const int64_t blocks = (np + G - 1) / G;
     if (blocks > 2147483647LL) return fail(-5, "bh_window_peak_counts: too many series");
@@ -974,11 +978,11 @@ int32_t bh_window_peak_counts(int32_t dtype, const void *traj, int64_t n_save, i
     if (dtype == BH_F32)
         hipLaunchKernelGGL(bh::win::peak_counts_kernel<float>, dim3((unsigned)blocks),
                            dim3(bh::win::kSeriesBlock), lds, s, reinterpret_cast<const float *>(traj),
-                           n_save, n, dim, start, G, sv, out);
+                           n_save, n, dim, start, (int)G, sv, out);
     else
         hipLaunchKernelGGL(bh::win::peak_counts_kernel<double>, dim3((unsigned)blocks),
                            dim3(bh::win::kSeriesBlock), lds, s, reinterpret_cast<const double *>(traj),
-                           n_save, n, dim, start, G, sv, out);
+                           n_save, n, dim, start, (int)G, sv, out);
     BH_HIP_CHECK(hipGetLastError());
     return 0;
 }

@@ -1119,12 +1119,37 @@ __global__ void __launch_bounds__(256)
 
 constexpr int kPeakSupports = 4;
 constexpr int kPeakWidths = 8;
-constexpr int kPeakCap = 64;
+
+// The number of consecutive samples x[i + dir], x[i + 2 dir], .. (at most
+// cap, inside [0, w)) that c exceeds (Strict) or is not below: eight
+// neighbours are loaded at a time, so a long run costs one LDS latency per
+// eight samples rather than one per sample.
+template <bool Strict, class T>
+__device__ __forceinline__ int run_length(const T *x, int64_t i, int dir, int cap, int64_t w, T c) {
+    int run = 0;
+    while (run < cap) {
+        T v[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const int64_t idx = i + (int64_t)dir * (run + u + 1);
+            v[u] = (idx >= 0 && idx < w) ? x[idx] : (T)INFINITY;
+        }
+        int f = 8;
+#pragma unroll
+        for (int u = 7; u >= 0; u--)
+            if (!(Strict ? c > v[u] : c >= v[u])) f = u;
+        run += f < cap - run ? f : cap - run;
+        if (f < 8) break;
+    }
+    return run;
+}
 
 // One wave per series staged in LDS, one position per lane at a time: the
-// strict peak radius of x_i (the largest k <= 64 with x_i > x_{i +- j} for all
-// j <= k inside the window) and its non-strict radius (x_i >= x_{i +- j}, at
-// most 8).  out (12, n, dim): rows 0..3 count the positions whose strict
+// strict peak radius of x_i (the largest k, up to the largest support asked
+// for, with x_i > x_{i +- j} for all j <= k inside the window) and its
+// non-strict radius (x_i >= x_{i +- j}, at most 8); the right-hand run is
+// walked first and bounds the left-hand one, so a position on a slope costs
+// one comparison.  out (12, n, dim): rows 0..3 count the positions whose strict
 // radius reaches supports[r] (number_peaks, the reference's find_peak_mask:
 // torch_features_pattern.py:64-105; a support of 0 counts nothing), rows
 // 4..11 the positions whose non-strict radius reaches width 1..8 (the
@@ -1140,10 +1165,15 @@ __global__ void __launch_bounds__(kSeriesBlock)
     const int64_t np = n * S;
     const int64_t p0 = (int64_t)blockIdx.x * G;
     T *lds = reinterpret_cast<T *>(smem);
+    int *cbase = reinterpret_cast<int *>(smem + ((int64_t)G * w * sizeof(T) + 15) / 16 * 16);
     load_series<T>(traj, n, S, start, w, p0, G, np, lds);
     __syncthreads();
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    int *cand = cbase + (int64_t)wave * w;  // the wave's list of candidate positions
     const int sup[kPeakSupports] = {supports.x, supports.y, supports.z, supports.w};
+    int cap = 1;  // the strict run need not go past the largest support
+#pragma unroll
+    for (int k = 0; k < kPeakSupports; k++) cap = sup[k] > cap ? sup[k] : cap;
     for (int g = wave; g < G; g += kSeriesBlock / 64) {
         const int64_t p = p0 + g;
         if (p >= np) break;  // wave-uniform
@@ -1154,16 +1184,33 @@ __global__ void __launch_bounds__(kSeriesBlock)
             if (lane < kPeakSupports + kPeakWidths) out[lane * np + p] = NAN;
             continue;
         }
+        // pass 1: the positions not below either neighbour (both inside the
+        // window) are the only ones with a radius of 1 or more; they are
+        // listed (ballot order) so that pass 2 spreads them over the lanes
+        int nc = 0;
+        for (int64_t i0 = 0; i0 < w; i0 += 64) {
+            const int64_t i = i0 + lane;
+            bool keep = false;
+            if (i >= 1 && i + 1 < w) {
+                const T c = x[i];
+                keep = c >= x[i - 1] && c >= x[i + 1];
+            }
+            const unsigned long long m = __ballot(keep);
+            if (keep) cand[nc + __popcll(m & ((1ull << lane) - 1ull))] = (int)i;
+            nc += __popcll(m);
+        }
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         double cs[kPeakSupports] = {0, 0, 0, 0};
         double cw[kPeakWidths] = {0, 0, 0, 0, 0, 0, 0, 0};
-        for (int64_t i = lane; i < w; i += 64) {
+        for (int k0 = lane; k0 < nc; k0 += 64) {
+            const int64_t i = cand[k0];
             const T c = x[i];
-            int ls = 0, rs = 0, ln = 0, rn = 0;
-            while (ls < kPeakCap && i - ls - 1 >= 0 && c > x[i - ls - 1]) ls++;
-            while (rs < kPeakCap && i + rs + 1 < w && c > x[i + rs + 1]) rs++;
-            while (ln < kPeakWidths && i - ln - 1 >= 0 && c >= x[i - ln - 1]) ln++;
-            while (rn < kPeakWidths && i + rn + 1 < w && c >= x[i + rn + 1]) rn++;
-            const int r = ls < rs ? ls : rs, q = ln < rn ? ln : rn;
+            // only min(left, right) matters: the right run caps the left one
+            const int rs = run_length<true, T>(x, i, 1, cap, w, c);
+            const int r = rs ? run_length<true, T>(x, i, -1, rs, w, c) : 0;
+            const int rn = run_length<false, T>(x, i, 1, kPeakWidths, w, c);
+            const int q = rn ? run_length<false, T>(x, i, -1, rn, w, c) : 0;
 #pragma unroll
             for (int k = 0; k < kPeakSupports; k++) cs[k] += (sup[k] > 0 && r >= sup[k]) ? 1.0 : 0.0;
 #pragma unroll
@@ -1179,6 +1226,8 @@ __global__ void __launch_bounds__(kSeriesBlock)
 #pragma unroll
             for (int k = 0; k < kPeakWidths; k++) out[(kPeakSupports + k) * np + p] = cw[k];
         }
+        __builtin_amdgcn_wave_barrier();  // the list is rebuilt for the next series
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     }
 }
 

@@ -658,3 +658,25 @@ def test_extractor_spectrum_columns():
         O.comprehensive(x, "fft_aggregated", {"aggtype": "skew"}), rel=1e-5)
     assert got["state_1__fourier_entropy__bins_5"] == pytest.approx(
         O.comprehensive(x, "fourier_entropy", {"bins": 5}), rel=1e-6)
+
+
+def test_every_comprehensive_calculator_through_the_extractor():
+    """The extractor takes every calculator of the comprehensive grid built
+    here on one window (one kernel output per family) and names its columns
+    as the reference does."""
+    from pybasin_amd.window_features import WindowBlocks
+
+    torch.manual_seed(4)
+    t = torch.linspace(0, 30, 300, dtype=torch.float64)
+    y = torch.randn(300, 3, 2, dtype=torch.float64).cumsum(0) * 0.1
+    sol = Solution(torch.zeros(3, 2), t, y)
+    fx = StatFeatureExtractor(time_steady=0.0, features=COMPREHENSIVE_FEATURES)
+    f = fx.extract_features(sol)
+    n_cols = sum(len(g or [None]) for g in COMPREHENSIVE_FEATURES.values())
+    assert f.shape == (3, 2 * n_cols)
+    assert torch.isfinite(f).all()
+    assert len(set(fx.feature_names)) == f.shape[1]
+    assert "state_1__cwt_coefficients__coeff_3__w_5__widths_(5,)" in fx.feature_names
+    assert "state_0__change_quantiles__f_agg_var__isabs_True__qh_0.8__ql_0.2" in fx.feature_names
+    wb = WindowBlocks(y, 0, [(0, n, p) for n, g in COMPREHENSIVE_FEATURES.items() for p in (g or [None])])
+    assert wb.feature("number_peaks", {"n": 50}).shape == (3, 2)
