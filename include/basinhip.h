@@ -557,10 +557,14 @@ int32_t bh_window_autocov(int32_t dtype, const void *traj, int64_t n_save, int64
    torch_features_boolean.py:11-19, torch_features_reocurrance.py:11-161).
    out_levels (6, n, dim) in the series dtype, or NULL: torch.quantile of the
    window at 0, 0.2, 0.4, 0.6, 0.8, 1 (linear interpolation in the series
-   dtype), the corridor bounds of bh_window_change_quantiles.  Non-finite
-   windows give NaN.  Windows up to 2,048 float32 or 1,024 float64 rows. */
+   dtype), the corridor bounds of bh_window_change_quantiles.  out_median
+   (n, dim) in the series dtype, or NULL: the lower middle value of the
+   sorted window (x.median(dim=0), torch_features_statistical.py:30-33).
+   Non-finite windows give NaN.  Windows up to 2,048 float32 or 1,024 float64
+   rows. */
 int32_t bh_window_sorted(int32_t dtype, const void *traj, int64_t n_save, int64_t n, int32_t dim,
-                         int64_t start, double *out_dup, void *out_levels, void *stream);
+                         int64_t start, double *out_dup, void *out_levels, void *out_median,
+                         void *stream);
 
 /* change_quantiles (torch_features_change.py:20-139) of each series of the
    same window for the 15 corridors ql < qh of the levels 0, 0.2, .., 1 in the

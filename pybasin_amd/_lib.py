@@ -120,7 +120,7 @@ def _declare(L: ctypes.CDLL) -> None:
         "bh_window_spectral_ratio": [i32, vp, i64, i64, i32, i64, vp, vp],
         "bh_window_features": [i32, vp, i64, i64, i32, i64, vp, vp],
         "bh_window_spectrum": [i32, vp, i64, i64, i32, i64, i32, vp, vp, vp],
-        "bh_window_sorted": [i32, vp, i64, i64, i32, i64, vp, vp, vp],
+        "bh_window_sorted": [i32, vp, i64, i64, i32, i64, vp, vp, vp, vp],
         "bh_window_change_quantiles": [i32, vp, i64, i64, i32, i64, vp, vp, vp],
         "bh_window_autocov": [i32, vp, i64, i64, i32, i64, i32, vp, vp],
         "bh_window_trends": [i32, vp, i64, i64, i32, i64, i32, vp, vp],

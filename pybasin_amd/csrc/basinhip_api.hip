@@ -1023,13 +1023,13 @@ int32_t bh_window_perm_entropy(int32_t dtype, const void *traj, int64_t n_save, 
 extern "C++" template <class T>
 static void launch_sorted(int64_t P, int64_t blocks, size_t lds, hipStream_t s, const T *traj,
                           int64_t n_save, int64_t n, int32_t dim, int64_t start, int64_t G,
-                          double *out_dup, T *out_levels) {
+                          double *out_dup, T *out_levels, T *out_median) {
     // values per lane: P / 64 (P >= 64)
     const int R = (int)(P / 64);
 #define BH_SORTED_LAUNCH(RR)                                                                   \
     hipLaunchKernelGGL((bh::win::sorted_kernel<T, RR>), dim3((unsigned)blocks),                 \
                        dim3(bh::win::kSeriesBlock), lds, s, traj, n_save, n, dim, start, (int)G, \
-                       out_dup, out_levels)
+                       out_dup, out_levels, out_median)
     switch (R) {
         case 1: BH_SORTED_LAUNCH(1); break;
         case 2: BH_SORTED_LAUNCH(2); break;
@@ -1042,7 +1042,8 @@ static void launch_sorted(int64_t P, int64_t blocks, size_t lds, hipStream_t s, 
 }
 
 int32_t bh_window_sorted(int32_t dtype, const void *traj, int64_t n_save, int64_t n, int32_t dim,
-                         int64_t start, double *out_dup, void *out_levels, void *stream) {
+                         int64_t start, double *out_dup, void *out_levels, void *out_median,
+                         void *stream) {
     g_err[0] = 0;
     int32_t rc = check_series_args("bh_window_sorted", dtype, traj, n_save, n, dim, start);
     if (rc) return rc;
@@ -1067,10 +1068,12 @@ int32_t bh_window_sorted(int32_t dtype, const void *traj, int64_t n_save, int64_
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     if (dtype == BH_F32)
         launch_sorted<float>(P, blocks, lds, s, reinterpret_cast<const float *>(traj), n_save, n, dim,
-                             start, G, out_dup, reinterpret_cast<float *>(out_levels));
+                             start, G, out_dup, reinterpret_cast<float *>(out_levels),
+                             reinterpret_cast<float *>(out_median));
     else
         launch_sorted<double>(P, blocks, lds, s, reinterpret_cast<const double *>(traj), n_save, n,
-                              dim, start, G, out_dup, reinterpret_cast<double *>(out_levels));
+                              dim, start, G, out_dup, reinterpret_cast<double *>(out_levels),
+                              reinterpret_cast<double *>(out_median));
     BH_HIP_CHECK(hipGetLastError());
     return 0;
 }

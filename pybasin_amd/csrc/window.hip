@@ -1391,12 +1391,13 @@ __device__ __forceinline__ void wave_bitonic_sort(T (&v)[R], int lane) {
 // reoccurrence counts, torch_features_boolean.py:11-19,
 // torch_features_reocurrance.py:11-161), sums in double; out_levels (6, n,
 // dim) in the series dtype, or null: torch.quantile at the levels of
-// kCqLevel (the corridor bounds of change_quantiles).  A window holding a
-// non-finite value gives NaN throughout.
+// kCqLevel (the corridor bounds of change_quantiles); out_median (n, dim) in
+// the series dtype, or null: the lower middle value (torch.median).  A
+// window holding a non-finite value gives NaN throughout.
 template <class T, int R>
 __global__ void __launch_bounds__(kSeriesBlock)
     sorted_kernel(const T *traj, int64_t n_save, int64_t n, int S, int64_t start, int G,
-                  double *out_dup, T *out_levels) {
+                  double *out_dup, T *out_levels, T *out_median) {
     constexpr int P = 64 * R;
     extern __shared__ __align__(16) unsigned char smem[];
     const int64_t w = n_save - start;
@@ -1417,6 +1418,7 @@ __global__ void __launch_bounds__(kSeriesBlock)
         if (__any(bad)) {
             if (lane < 5) out_dup[lane * np + p] = NAN;
             if (out_levels && lane < kCqLevels) out_levels[lane * np + p] = (T)NAN;
+            if (out_median && lane == 0) out_median[p] = (T)NAN;
             continue;
         }
         T v[R];
@@ -1458,6 +1460,7 @@ __global__ void __launch_bounds__(kSeriesBlock)
             out_dup[4 * np + p] = sv;
         }
         if (out_levels && lane < kCqLevels) out_levels[lane * np + p] = torch_quantile<T>(buf, w, kCqLevel[lane]);
+        if (out_median && lane == 0) out_median[p] = buf[(w - 1) / 2];
         __builtin_amdgcn_wave_barrier();  // buf reused by the wave's next series
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     }

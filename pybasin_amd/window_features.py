@@ -670,6 +670,7 @@ def _sorted_cpu(w: torch.Tensor, cq: bool):
     dup = torch.stack([eq.double().sum(0), grp.double().sum(0), first.double().sum(0),
                        (x * grp).sum(0), (x * first).sum(0)])
     dup = torch.where(ok, dup, torch.full_like(dup, float("nan")))
+    med = torch.where(ok, srt[(N - 1) // 2], torch.full_like(srt[0], float("nan")))
     out_cq = None
     if cq:
         lv = [_torch_quantile_sorted(srt, q) for q in _CQ_LEVELS]
@@ -686,16 +687,18 @@ def _sorted_cpu(w: torch.Tensor, cq: bool):
                          torch.where(c > 0, var, torch.zeros_like(var))]
         out_cq = torch.stack(rows)
         out_cq = torch.where(ok, out_cq, torch.full_like(out_cq, float("nan")))
-    return dup, out_cq
+    return dup, out_cq, med
 
 
 def window_sorted(y: torch.Tensor, start: int, cq: bool):
-    """(dup (5, B, S), cq (60, B, S) or None), float64: the sorted-window
-    statistics of bh_window_sorted for the window rows [start, N) of y
-    (N, B, S) (the kernel on the device, torch on the CPU).  dup = adjacent
-    equal pairs, values in groups of equal values, groups, sum of the values
-    in groups, sum of one value per group; cq row 4 c + k = corridor c of
-    _CQ_CORRIDORS, k = mean, var of the changes, mean, var of |changes|."""
+    """(dup (5, B, S), cq (60, B, S) or None, median (B, S)): the
+    sorted-window statistics of bh_window_sorted for the window rows
+    [start, N) of y (N, B, S) (the kernel on the device, torch on the CPU).
+    dup (float64) = adjacent equal pairs, values in groups of equal values,
+    groups, sum of the values in groups, sum of one value per group; cq
+    (float64) row 4 c + k = corridor c of _CQ_CORRIDORS, k = mean, var of the
+    changes, mean, var of |changes|; median in the series dtype (the lower
+    middle value, NaN for a non-finite window)."""
     N, B, S = y.shape
     if not 0 <= start < N:
         raise ValueError("empty feature window")
@@ -705,18 +708,19 @@ def window_sorted(y: torch.Tensor, start: int, cq: bool):
     dup = torch.empty((5, B, S), dtype=torch.float64, device=y.device)
     levels = torch.empty((6, B, S), dtype=y.dtype, device=y.device) if cq else None
     out_cq = torch.empty((60, B, S), dtype=torch.float64, device=y.device) if cq else None
+    med = torch.empty((B, S), dtype=y.dtype, device=y.device)
     dt = _lib.BH_F32 if y.dtype == torch.float32 else _lib.BH_F64
     s = torch.cuda.current_stream(y.device)
     with torch.cuda.device(y.device):
         _lib.check(_lib.lib().bh_window_sorted(
             dt, y.data_ptr(), N, B, S, int(start), dup.data_ptr(),
-            levels.data_ptr() if levels is not None else None, s.cuda_stream),
+            levels.data_ptr() if levels is not None else None, med.data_ptr(), s.cuda_stream),
             "bh_window_sorted")
         if cq:
             _lib.check(_lib.lib().bh_window_change_quantiles(
                 dt, y.data_ptr(), N, B, S, int(start), levels.data_ptr(), out_cq.data_ptr(),
                 s.cuda_stream), "bh_window_change_quantiles")
-    return dup, out_cq
+    return dup, out_cq, med
 
 
 def sorted_feature(name: str, params: dict | None, dup: torch.Tensor, cq: torch.Tensor | None,
@@ -1140,10 +1144,11 @@ class WindowBlocks:
         if name in SPECTRUM_FEATURES:
             return spectrum_feature(name, params, *self.spectrum(), self.n)
         if name in SORTED_FEATURES:
-            return sorted_feature(name, params, *self.sorted(), self.n)
+            return sorted_feature(name, params, *self.sorted()[:2], self.n)
         med = None
         if name == "symmetry_looking":
-            if self._median_fn is None:
-                raise ValueError("symmetry_looking needs the window median")
-            med = self._get("median", self._median_fn)
+            if self.n <= (2048 if self.y.dtype == torch.float32 else 1024) or self._median_fn is None:
+                med = self.sorted()[2]  # the sorted window's middle value
+            else:
+                med = self._get("median", self._median_fn)
         return comprehensive_feature(name, params, self.block(), self.n, med, self.memo)

@@ -363,12 +363,16 @@ def test_window_sorted_kernel_equals_oracle_synthetic(cuda_device, dtype):
         wt = torch.tensor(w, dtype=dtype)
         sd = np.float32 if dtype == torch.float32 else np.float64
         for start in (0, 1) if w.shape[0] > 1 else (0,):
-            dup, cq = window_sorted(wt.cuda(), start, True)
-            dup, cq = dup.cpu(), cq.cpu()
+            dup, cq, med = window_sorted(wt.cuda(), start, True)
+            dup, cq, med = dup.cpu(), cq.cpu(), med.cpu()
             _check_sorted(dup, cq, wt[start:].numpy().astype(sd), f"{tag}/{start}", sd)
-            cdup, _ = window_sorted(wt, start, False)
+            cdup, _, cmed = window_sorted(wt, start, False)
             assert torch.equal(torch.nan_to_num(dup[:3], 7.0), torch.nan_to_num(cdup[:3], 7.0)), tag
-            dup2, none = window_sorted(wt.cuda(), start, False)
+            # the lower middle value, as torch.median (NaN for a non-finite window)
+            assert torch.equal(torch.nan_to_num(med, 7.0), torch.nan_to_num(cmed, 7.0)), tag
+            fin = torch.isfinite(wt[start:]).all(0)
+            assert torch.equal(med[fin], wt[start:].median(0).values[fin]), tag
+            dup2, none, _ = window_sorted(wt.cuda(), start, False)
             assert none is None and torch.equal(torch.nan_to_num(dup2.cpu(), 7.0),
                                                 torch.nan_to_num(dup, 7.0)), tag
 
@@ -390,12 +394,12 @@ def test_window_sorted_on_integrated_duffing(cuda_device, dtype):
     from pybasin_amd.window_features import sorted_feature, window_sorted
 
     res = _duffing(dtype)
-    dup, cq = window_sorted(res.y, res.steady_idx, True)
+    dup, cq, _ = window_sorted(res.y, res.steady_idx, True)
     dup, cq = dup.cpu(), cq.cpu()
     sd = np.float32 if dtype == torch.float32 else np.float64
     x = res.y.cpu().numpy()[res.steady_idx:]
     _check_sorted(dup[:, :32], cq[:, :32], x[:, :32], "duffing", sd)
-    cdup, ccq = window_sorted(res.y.cpu(), res.steady_idx, True)
+    cdup, ccq, _ = window_sorted(res.y.cpu(), res.steady_idx, True)
     assert torch.equal(dup[:3], cdup[:3])
     np.testing.assert_allclose(cq.numpy(), ccq.numpy(), rtol=1e-9, atol=1e-12)
     feats = {"change_quantiles": [{"ql": 0.2, "qh": 0.8, "isabs": True, "f_agg": "var"}],

@@ -348,7 +348,7 @@ def test_cpu_sorted_path_equals_oracle(dtype):
         wt = torch.tensor(w, dtype=dtype)
         x64 = wt.double().numpy()
         N, B, S = w.shape
-        dup, cq = window_sorted(wt, 0, True)
+        dup, cq, _ = window_sorted(wt, 0, True)
         for name, grid in SORTED_FEATURES.items():
             for p in (grid or [None]):
                 got = sorted_feature(name, p, dup, cq, N).numpy()
