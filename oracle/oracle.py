@@ -695,4 +695,52 @@ def comprehensive(x: np.ndarray, name: str, params: dict | None = None, dtype=np
             counts[pid] = counts.get(pid, 0) + 1
         q = np.array(list(counts.values()), dtype=np.float64) / npat
         return float(-(q * np.log(q)).sum() / math.log(math.factorial(d)))
+    if name == "correlation_dimension":  # torch_corr_dim.py:44-124 (emb 4, lag 1, 50 radii)
+        m = n - 3
+        if m < 2:
+            return float("nan")
+        orb = np.stack([x[e: e + m] for e in range(4)], 1)
+        d = np.sqrt(((orb[:, None, :] - orb[None, :, :]) ** 2).sum(-1))
+        np.fill_diagonal(d, np.inf)
+        sd = x.std(ddof=1)
+        L0, L1 = np.log(0.1 * sd + 1e-10), np.log(0.5 * sd + 1e-10)
+        lr = L0 + np.arange(50) / 49 * (L1 - L0)
+        c = np.array([(d <= np.exp(v)).sum() for v in lr]) / (m * (m - 1))
+        ok = c > 0
+        if ok.sum() < 2:
+            return float("nan")
+        a, b = lr[ok], np.log(c[ok] + 1e-30)
+        k = ok.sum()
+        den = k * (a * a).sum() - a.sum() ** 2
+        return float((k * (a * b).sum() - a.sum() * b.sum()) / den) if abs(den) > 1e-10 else float("nan")
+    if name == "lyapunov_r":  # torch_lyapunov_r.py:20-150 (emb 10, lag 1, 20 steps)
+        m = n - 9
+        if m < 1:
+            return float("nan")
+        M = 2 * n - 1
+        P = np.abs(np.fft.rfft(x, M)) ** 2
+        f = np.fft.rfftfreq(M)
+        mf = (f[1:] * P[1:]).sum() / (P[1:].sum() + 1e-10)
+        tsep = min(max(int(np.ceil(1.0 / (mf + 1e-10))), 1), int(0.25 * n))
+        orb = np.stack([x[e: e + m] for e in range(10)], 1)
+        d = np.sqrt(((orb[:, None, :] - orb[None, :, :]) ** 2).sum(-1))
+        idx = np.arange(m)
+        d[np.abs(idx[:, None] - idx[None, :]) <= tsep] = np.inf
+        ntraj = max(m - 19, 1)
+        nb = np.argmin(d[:ntraj, :ntraj], axis=1)
+        div = np.full(20, np.nan)
+        for k in range(20):
+            ia, ib = np.arange(ntraj) + k, nb + k
+            ok = (ia < m) & (ib < m)
+            v = d[ia[ok], ib[ok]]
+            v = v[v > 0]
+            if v.size:
+                div[k] = np.log(v).mean()
+        ks = np.nonzero(np.isfinite(div))[0]
+        if ks.size < 2:
+            return float("nan")
+        kk, dd = ks.astype(np.float64), div[ks]
+        den = ks.size * (kk * kk).sum() - kk.sum() ** 2
+        slope = (ks.size * (kk * dd).sum() - kk.sum() * dd.sum()) / den if abs(den) > 1e-10 else np.nan
+        return float(slope / float(p.get("tau", 1.0)))
     raise KeyError(name)

@@ -34,6 +34,7 @@ WF = dict(N=0, SUM=1, SUMSQ=2, MIN=3, MAX=4, FIRST_MAX=5, LAST_MAX=6, FIRST_MIN=
           STRIKE_ABOVE=59, STRIKE_BELOW=60, COUNT_MAX=61, COUNT_MIN=62, BEYOND=63, AUTOCOV=73,
           MASS=82, BINS=90)
 BH_WF_SLOTS = 100
+BH_EMB_CORR_DIM, BH_EMB_LYAP_R = 0, 1
 BH_NOALLOC_SCRATCH_BYTES = 64 << 20
 
 # every symbol include/basinhip.h declares
@@ -48,7 +49,7 @@ EXPORTS = ("bh_version", "bh_last_error", "bh_solve_batch", "bh_solve_batch_peak
            "bh_window_order_stats", "bh_window_autocorr_periodicity", "bh_window_spectral_ratio",
            "bh_window_features", "bh_window_spectrum", "bh_window_sorted",
            "bh_window_change_quantiles", "bh_window_autocov", "bh_window_trends",
-           "bh_window_peak_counts", "bh_window_perm_entropy")
+           "bh_window_peak_counts", "bh_window_perm_entropy", "bh_window_embedding")
 
 _lock = threading.Lock()
 _lib: ctypes.CDLL | None = None
@@ -126,6 +127,7 @@ def _declare(L: ctypes.CDLL) -> None:
         "bh_window_trends": [i32, vp, i64, i64, i32, i64, i32, vp, vp],
         "bh_window_peak_counts": [i32, vp, i64, i64, i32, i64, i32, i32, i32, i32, vp, vp],
         "bh_window_perm_entropy": [i32, vp, i64, i64, i32, i64, vp, vp],
+        "bh_window_embedding": [i32, vp, i64, i64, i32, i64, i32, vp, vp],
     }
     for name, args in sigs.items():
         fn = getattr(L, name)

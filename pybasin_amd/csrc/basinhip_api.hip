@@ -1020,6 +1020,54 @@ int32_t bh_window_perm_entropy(int32_t dtype, const void *traj, int64_t n_save, 
     return 0;
 }
 
+int32_t bh_window_embedding(int32_t dtype, const void *traj, int64_t n_save, int64_t n,
+                            int32_t dim, int64_t start, int32_t which, double *out, void *stream) {
+    g_err[0] = 0;
+    int32_t rc = check_series_args("bh_window_embedding", dtype, traj, n_save, n, dim, start);
+    if (rc) return rc;
+    if (!out) return fail(-5, "bh_window_embedding: null out");
+    if (which != BH_EMB_CORR_DIM && which != BH_EMB_LYAP_R)
+        return fail(-5, "bh_window_embedding: which must be BH_EMB_CORR_DIM or BH_EMB_LYAP_R");
+    if (n == 0) return 0;
+    const int64_t w = n_save - start;
+    const int64_t elem = dtype == BH_F32 ? 4 : 8;
+    // the block's series rows (plus, for the correlation sum, the radii and
+    // the histogram of each wave)
+    const int64_t budget = 62 * 1024;
+    const int64_t extra = which == BH_EMB_CORR_DIM ? 4 * 50 * 8 + 4 * 64 * 4 : 0;
+    if (extra + w * elem > budget)
+        return fail(-5, "bh_window_embedding: window of %lld rows exceeds the %lld-byte LDS budget",
+                    (long long)w, (long long)budget);
+    int64_t G = (budget - extra) / (w * elem);
+    if (G > 16) G = 16;
+    const size_t lds = (size_t)((G * w * elem + 15) / 16 * 16 + extra);
+    const int64_t np = n * dim;
+    const int64_t blocks = (np + G - 1) / G;
+    if (blocks > 2147483647LL) return fail(-5, "bh_window_embedding: too many series");
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    if (which == BH_EMB_CORR_DIM) {
+        if (dtype == BH_F32)
+            hipLaunchKernelGGL(bh::win::corr_dim_kernel<float>, dim3((unsigned)blocks),
+                               dim3(bh::win::kSeriesBlock), lds, s, reinterpret_cast<const float *>(traj),
+                               n_save, n, dim, start, (int)G, out);
+        else
+            hipLaunchKernelGGL(bh::win::corr_dim_kernel<double>, dim3((unsigned)blocks),
+                               dim3(bh::win::kSeriesBlock), lds, s,
+                               reinterpret_cast<const double *>(traj), n_save, n, dim, start, (int)G, out);
+    } else {
+        if (dtype == BH_F32)
+            hipLaunchKernelGGL(bh::win::lyap_r_kernel<float>, dim3((unsigned)blocks),
+                               dim3(bh::win::kSeriesBlock), lds, s, reinterpret_cast<const float *>(traj),
+                               n_save, n, dim, start, (int)G, out);
+        else
+            hipLaunchKernelGGL(bh::win::lyap_r_kernel<double>, dim3((unsigned)blocks),
+                               dim3(bh::win::kSeriesBlock), lds, s,
+                               reinterpret_cast<const double *>(traj), n_save, n, dim, start, (int)G, out);
+    }
+    BH_HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
 extern "C++" template <class T>
 static void launch_sorted(int64_t P, int64_t blocks, size_t lds, hipStream_t s, const T *traj,
                           int64_t n_save, int64_t n, int32_t dim, int64_t start, int64_t G,

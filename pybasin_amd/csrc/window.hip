@@ -1319,6 +1319,266 @@ __global__ void __launch_bounds__(kSeriesBlock)
     }
 }
 
+// ---- delay-embedding estimates: correlation dimension, largest Lyapunov ----
+
+constexpr int kCdRadii = 50;
+
+// correlation_dimension (torch_corr_dim.py:44-124; emb_dim = 4, lag = 1,
+// n_rvals = 50): radii r_k = exp(L0 + k / 49 (L1 - L0)), L0 = ln(0.1 sd +
+// 1e-10), L1 = ln(0.5 sd + 1e-10) with the unbiased sd of the window; C_k = the
+// share of ordered pairs i != j of the m = w - 3 embedded vectors at
+// Euclidean distance <= r_k; the slope of ln(C_k + 1e-30) on ln r_k over the
+// nonzero C_k (NaN for fewer than two, or a degenerate fit).  One wave per
+// series staged in LDS, rows i one per lane, the four coordinates of row j
+// shared (an LDS broadcast); each pair i < j's squared distance is placed by a
+// float logarithm and settled against the exact r_k^2 (double), then counted
+// in the wave's LDS histogram (the counts of radius k add up the bins <= k).
+template <class T>
+__global__ void __launch_bounds__(kSeriesBlock)
+    corr_dim_kernel(const T *traj, int64_t n_save, int64_t n, int S, int64_t start, int G,
+                    double *out) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int64_t w = n_save - start;
+    const int64_t np = n * S;
+    const int64_t p0 = (int64_t)blockIdx.x * G;
+    T *lds = reinterpret_cast<T *>(smem);
+    unsigned char *tail = smem + ((int64_t)G * w * sizeof(T) + 15) / 16 * 16;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    double *r2 = reinterpret_cast<double *>(tail) + wave * kCdRadii;
+    unsigned *hist = reinterpret_cast<unsigned *>(tail + 4 * kCdRadii * sizeof(double)) + wave * 64;
+    load_series<T>(traj, n, S, start, w, p0, G, np, lds);
+    __syncthreads();
+    for (int g = wave; g < G; g += kSeriesBlock / 64) {
+        const int64_t p = p0 + g;
+        if (p >= np) break;  // wave-uniform
+        const T *x = lds + (int64_t)g * w;
+        bool bad = false;
+        for (int64_t i = lane; i < w; i += 64) bad |= !isfinite((double)x[i]);
+        const int64_t m = w - 3;
+        if (__any(bad) || m < 2) {
+            if (lane == 0) out[p] = NAN;
+            continue;
+        }
+        double mean, ss;
+        series_moments<T>(x, w, mean, ss);
+        const double sd = sqrt(ss / (double)(w - 1));
+        const double L0 = log(0.1 * sd + 1e-10), L1 = log(0.5 * sd + 1e-10);
+        const double dl = (L1 - L0) / (double)(kCdRadii - 1);
+        if (lane < kCdRadii) {
+            const double r = exp(fma((double)lane, dl, L0));
+            r2[lane] = r * r;
+        }
+        hist[lane] = 0u;
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        const float fl0 = (float)L0, finv = dl > 0.0 ? (float)(1.0 / dl) : 0.0f;
+        const double rmin2 = r2[0], rmax2 = r2[kCdRadii - 1];
+        const bool fast = dl > 1e-3;  // the float guess is within 1e-4 of exact
+        for (int64_t i0 = 0; i0 < m; i0 += 64) {
+            const int64_t i = i0 + lane;
+            const bool row = i < m;
+            double a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+            if (row) {
+                a0 = (double)x[i];
+                a1 = (double)x[i + 1];
+                a2 = (double)x[i + 2];
+                a3 = (double)x[i + 3];
+            }
+            // pairs (i, j > i): j runs from the chunk's first row + 1, rows
+            // below the lane's own are skipped; eight rows j at a time, their
+            // 11 samples loaded together (LDS broadcasts)
+            for (int64_t j0 = i0 + 1; j0 < m; j0 += 8) {
+                double v[11];
+#pragma unroll
+                for (int u = 0; u < 11; u++) v[u] = j0 + u < w ? (double)x[j0 + u] : 0.0;
+#pragma unroll
+                for (int u = 0; u < 8; u++) {
+                    const int64_t j = j0 + u;
+                    const double e0 = a0 - v[u], e1 = a1 - v[u + 1];
+                    const double e2 = a2 - v[u + 2], e3 = a3 - v[u + 3];
+                    const double d2 = fma(e0, e0, fma(e1, e1, fma(e2, e2, e3 * e3)));
+                    if (!row || j <= i || j >= m || !(d2 <= rmax2)) continue;
+                    // smallest k with r_k^2 >= d2: from a float logarithm,
+                    // settled against the exact r_k^2 only within 1e-3 of a
+                    // boundary (the float error is ~3e-5 there)
+                    int k = 0;
+                    if (d2 > rmin2) {
+                        const float t = (0.5f * __logf((float)d2) - fl0) * finv;
+                        const float fr = t - floorf(t);
+                        k = (int)ceilf(t);
+                        k = k < 1 ? 1 : (k > kCdRadii - 1 ? kCdRadii - 1 : k);
+                        if (!fast || fr < 1e-3f || fr > 0.999f) {
+                            while (k > 1 && r2[k - 1] >= d2) k--;
+                            while (k < kCdRadii - 1 && r2[k] < d2) k++;
+                        }
+                    }
+                    atomicAdd(&hist[k], 1u);
+                }
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        // counts per radius (twice: ordered pairs) and the log-log fit
+        const double pairs = (double)m * (double)(m - 1);
+        double cum = 0.0, nv = 0.0, sr = 0.0, sc = 0.0, srr = 0.0, src = 0.0;
+        for (int k = 0; k < kCdRadii; k++) {  // wave-uniform
+            cum += 2.0 * (double)hist[k];
+            const double c = cum / pairs;
+            if (c > 0.0) {
+                const double lr = fma((double)k, dl, L0), lc = log(c + 1e-30);
+                nv += 1.0;
+                sr += lr;
+                sc += lc;
+                srr = fma(lr, lr, srr);
+                src = fma(lr, lc, src);
+            }
+        }
+        const double den = nv * srr - sr * sr;
+        if (lane == 0) out[p] = (nv >= 2.0 && fabs(den) > 1e-10) ? (nv * src - sr * sc) / den : NAN;
+        __builtin_amdgcn_wave_barrier();  // the histogram is cleared for the next series
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    }
+}
+
+constexpr int kLrEmb = 10;
+constexpr int kLrTraj = 20;
+
+// lyapunov_r (torch_lyapunov_r.py:20-150; emb_dim = 10, lag = 1,
+// trajectory_len = 20; the result is the slope, the caller divides by tau).
+// One wave per series staged in LDS:
+// 1. the minimum temporal separation from the mean frequency of the
+//    zero-padded (2w - 1)-point spectrum of the raw window (Goertzel, bins 1 ..
+//    w - 1 spread over the lanes): ceil(1 / (mf + 1e-10)) clamped to
+//    [1, floor(w / 4)];
+// 2. for each of the ntraj = m - 19 first embedded vectors (m = w - 9) its
+//    nearest other vector among them at a temporal separation above the
+//    minimum (Euclidean, the first on ties; rows one per lane, the ten
+//    coordinates of row j shared in registers as a sliding window);
+// 3. the mean over the vectors of ln |v_{i+k} - v_{nb_i+k}| for k = 0..19
+//    (zero distances left out) and the least-squares slope over the finite
+//    means (NaN for fewer than two).
+// out (n, dim): the slope.  A non-finite window gives NaN.
+template <class T>
+__global__ void __launch_bounds__(kSeriesBlock)
+    lyap_r_kernel(const T *traj, int64_t n_save, int64_t n, int S, int64_t start, int G,
+                  double *out) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int64_t w = n_save - start;
+    const int64_t np = n * S;
+    const int64_t p0 = (int64_t)blockIdx.x * G;
+    T *lds = reinterpret_cast<T *>(smem);
+    load_series<T>(traj, n, S, start, w, p0, G, np, lds);
+    __syncthreads();
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (int g = wave; g < G; g += kSeriesBlock / 64) {
+        const int64_t p = p0 + g;
+        if (p >= np) break;  // wave-uniform
+        const T *x = lds + (int64_t)g * w;
+        bool bad = false;
+        for (int64_t i = lane; i < w; i += 64) bad |= !isfinite((double)x[i]);
+        const int64_t m = w - (kLrEmb - 1);
+        if (__any(bad) || m < 1) {
+            if (lane == 0) out[p] = NAN;
+            continue;
+        }
+        // 1. mean frequency of the (2w - 1)-point spectrum, bins 1 .. w - 1
+        const double M = (double)(2 * w - 1);
+        double sp = 0.0, sfp = 0.0;
+        for (int64_t k = 1 + lane; k < w; k += 64) {
+            const double cw = 2.0 * cospi(2.0 * (double)k / M);
+            double s1 = 0.0, s2 = 0.0;
+            for (int64_t i = 0; i < w; i++) {
+                const double s0 = fma(cw, s1, (double)x[i] - s2);
+                s2 = s1;
+                s1 = s0;
+            }
+            const double pw = fma(s1, s1, s2 * s2) - cw * s1 * s2;
+            sp += pw;
+            sfp = fma((double)k / M, pw, sfp);
+        }
+        sp = wave_sum(sp);
+        sfp = wave_sum(sfp);
+        const double mf = sfp / (sp + 1e-10);
+        int64_t tsep = (int64_t)ceil(1.0 / (mf + 1e-10));
+        const int64_t tmax = (int64_t)(0.25 * (double)w);
+        tsep = tsep < 1 ? 1 : tsep;
+        tsep = tsep > tmax ? tmax : tsep;
+        // 2. nearest neighbours among the first ntraj vectors
+        const int64_t ntraj = m - kLrTraj + 1 > 1 ? m - kLrTraj + 1 : 1;
+        double lsum[kLrTraj], lcnt[kLrTraj];
+#pragma unroll
+        for (int k = 0; k < kLrTraj; k++) lsum[k] = lcnt[k] = 0.0;
+        for (int64_t i0 = 0; i0 < ntraj; i0 += 64) {
+            const int64_t i = i0 + lane;
+            const bool row = i < ntraj;
+            double a[kLrEmb];
+#pragma unroll
+            for (int e = 0; e < kLrEmb; e++) a[e] = row ? (double)x[i + e] : 0.0;
+            double best = INFINITY;
+            int nb = 0;
+            // eight rows j at a time: their 17 samples loaded together (LDS
+            // broadcasts), the distances from registers; |i - j| > tsep as
+            // one unsigned comparison (window rows fit 32 bits)
+            const int ii = (int)i, nt = (int)ntraj, ts = (int)tsep;
+            for (int j0 = 0; j0 < nt; j0 += 8) {
+                double v[kLrEmb + 7];
+#pragma unroll
+                for (int u = 0; u < kLrEmb + 7; u++) v[u] = j0 + u < w ? (double)x[j0 + u] : 0.0;
+#pragma unroll
+                for (int u = 0; u < 8; u++) {
+                    const int j = j0 + u;
+                    double d2 = 0.0;
+#pragma unroll
+                    for (int e = 0; e < kLrEmb; e++) {
+                        const double df = a[e] - v[u + e];
+                        d2 = fma(df, df, d2);
+                    }
+                    const bool far = (unsigned)(j - ii + ts) > (unsigned)(2 * ts);
+                    if (j < nt && far && d2 < best) {
+                        best = d2;
+                        nb = j;
+                    }
+                }
+            }
+            // 3. the divergence of the pair over the next 20 steps
+            if (row) {
+#pragma unroll
+                for (int k = 0; k < kLrTraj; k++) {
+                    const int64_t ia = i + k, ib = nb + k;
+                    if (ia >= m || ib >= m) continue;  // past the last vector
+                    const int64_t sep = ia > ib ? ia - ib : ib - ia;
+                    double d2 = 0.0;
+                    for (int e = 0; e < kLrEmb; e++) {
+                        const double df = (double)x[ia + e] - (double)x[ib + e];
+                        d2 = fma(df, df, d2);
+                    }
+                    const double d = sep <= tsep ? INFINITY : sqrt(d2);
+                    if (d > 0.0) {
+                        lsum[k] += log(d);
+                        lcnt[k] += 1.0;
+                    }
+                }
+            }
+        }
+        double n_f = 0.0, sk = 0.0, sd = 0.0, skk = 0.0, skd = 0.0;
+#pragma unroll
+        for (int k = 0; k < kLrTraj; k++) {
+            const double c = wave_sum(lcnt[k]);
+            const double sm = wave_sum(lsum[k]);
+            const double dk = c > 0.0 ? sm / c : NAN;
+            if (isfinite(dk)) {
+                n_f += 1.0;
+                sk += (double)k;
+                sd += dk;
+                skk += (double)(k * k);
+                skd = fma((double)k, dk, skd);
+            }
+        }
+        const double den = n_f * skk - sk * sk;
+        if (lane == 0) out[p] = (n_f >= 2.0 && fabs(den) > 1e-10) ? (n_f * skd - sk * sd) / den : NAN;
+    }
+}
+
 // ---- sorted-window statistics: duplicates and change quantiles -------------
 
 // the six quantile levels whose corridors change_quantiles takes

@@ -133,6 +133,12 @@ PEAK_FEATURES: dict[str, list[dict]] = {
 }
 COMPREHENSIVE_FEATURES.update(PEAK_FEATURES)
 _BLOCK_PEAKS = (1, 3, 5)
+# delay-embedding estimates (bh_window_embedding)
+EMBED_FEATURES: dict[str, list[dict]] = {
+    "correlation_dimension": [{"emb_dim": 4, "lag": 1, "n_rvals": 50}],
+    "lyapunov_r": [{"emb_dim": 10, "lag": 1, "trajectory_len": 20, "tau": 1.0}],
+}
+COMPREHENSIVE_FEATURES.update(EMBED_FEATURES)
 TR = dict(ALT=0, FRIED=48, LANGEVIN=52, BENFORD=53, CWT=54)
 BH_TR_SLOTS = 114
 # the remaining regression calculator formed from the window-feature slots
@@ -173,6 +179,15 @@ def check_params(name: str, params: dict | None) -> None:
         if isinstance(params.get("max_width"), int) and 1 <= params["max_width"] <= 8:
             return
         raise ValueError(f"number_cwt_peaks: max_width must be in [1, 8], got {params}")
+    if name == "correlation_dimension":
+        if (params.get("emb_dim", 4), params.get("lag", 1), params.get("n_rvals", 50)) == (4, 1, 50):
+            return
+        raise ValueError(f"correlation_dimension: emb_dim 4, lag 1, n_rvals 50 are built, got {params}")
+    if name == "lyapunov_r":
+        if (params.get("emb_dim", 10), params.get("lag", 1), params.get("trajectory_len", 20)) == \
+                (10, 1, 20) and float(params.get("tau", 1.0)) > 0:
+            return
+        raise ValueError(f"lyapunov_r: emb_dim 10, lag 1, trajectory_len 20 are built, got {params}")
     if name == "permutation_entropy":
         if params.get("tau", 1) == 1 and params.get("dimension") in (3, 4, 5, 6, 7):
             return
@@ -1055,6 +1070,89 @@ def peak_feature(name: str, params: dict | None, counts: dict, perm: torch.Tenso
     raise KeyError(f"feature {name!r} is not a peak / pattern feature")
 
 
+def _corr_dim_cpu(x: torch.Tensor) -> float:
+    """correlation_dimension of one float64 series (emb 4, lag 1, 50 radii)."""
+    n = x.shape[0]
+    m = n - 3
+    if m < 2 or not bool(torch.isfinite(x).all()):
+        return math.nan
+    orb = torch.stack([x[e: e + m] for e in range(4)], 1)
+    d = torch.cdist(orb, orb, compute_mode="donot_use_mm_for_euclid_dist")
+    d.fill_diagonal_(math.inf)
+    sd = float(x.std())
+    L0, L1 = math.log(0.1 * sd + 1e-10), math.log(0.5 * sd + 1e-10)
+    lr = torch.tensor([L0 + k / 49 * (L1 - L0) for k in range(50)], dtype=torch.float64)
+    c = torch.stack([(d <= math.exp(v)).sum() for v in lr.tolist()]).double() / (m * (m - 1))
+    ok = c > 0
+    nv = int(ok.sum())
+    lc = torch.log(c[ok] + 1e-30)
+    r = lr[ok]
+    den = nv * float((r * r).sum()) - float(r.sum()) ** 2
+    if nv < 2 or abs(den) <= 1e-10:
+        return math.nan
+    return (nv * float((r * lc).sum()) - float(r.sum()) * float(lc.sum())) / den
+
+
+def _lyap_r_cpu(x: torch.Tensor) -> float:
+    """The Rosenstein slope of one float64 series (emb 10, lag 1, 20 steps)."""
+    n = x.shape[0]
+    m = n - 9
+    if m < 1 or not bool(torch.isfinite(x).all()):
+        return math.nan
+    M = 2 * n - 1
+    P = torch.fft.rfft(x, n=M).abs() ** 2
+    f = torch.arange(P.shape[0], dtype=torch.float64) / M
+    mf = float((f[1:] * P[1:]).sum() / (P[1:].sum() + 1e-10))
+    tsep = min(max(math.ceil(1.0 / (mf + 1e-10)), 1), int(0.25 * n))
+    orb = torch.stack([x[e: e + m] for e in range(10)], 1)
+    d = torch.cdist(orb, orb, compute_mode="donot_use_mm_for_euclid_dist")
+    idx = torch.arange(m)
+    d = torch.where((idx.view(-1, 1) - idx.view(1, -1)).abs() <= tsep, math.inf, d)
+    ntraj = max(m - 19, 1)
+    nb = torch.argmin(d[:ntraj, :ntraj], dim=1)
+    div = []
+    for k in range(20):
+        ia, ib = torch.arange(ntraj) + k, nb + k
+        ok = (ia < m) & (ib < m)
+        v = d[ia[ok], ib[ok]]
+        v = v[v > 0]
+        div.append(float(torch.log(v).mean()) if v.numel() else math.nan)
+    ks = [k for k in range(20) if math.isfinite(div[k])]
+    if len(ks) < 2:
+        return math.nan
+    kk = torch.tensor(ks, dtype=torch.float64)
+    dd = torch.tensor([div[k] for k in ks], dtype=torch.float64)
+    nf = len(ks)
+    den = nf * float((kk * kk).sum()) - float(kk.sum()) ** 2
+    if abs(den) <= 1e-10:
+        return math.nan
+    return (nf * float((kk * dd).sum()) - float(kk.sum()) * float(dd.sum())) / den
+
+
+def window_embedding(y: torch.Tensor, start: int, which: str) -> torch.Tensor:
+    """(B, S) float64 correlation_dimension ("corr_dim") or Rosenstein slope
+    ("lyap_r", not divided by tau) of the window rows [start, N) of y
+    (bh_window_embedding on the device, torch per series on the CPU)."""
+    N, B, S = y.shape
+    if not 0 <= start < N:
+        raise ValueError("empty feature window")
+    if y.device.type != "cuda":
+        fn = _corr_dim_cpu if which == "corr_dim" else _lyap_r_cpu
+        w = y[start:].double()
+        return torch.tensor([[fn(w[:, b, s]) for s in range(S)] for b in range(B)],
+                            dtype=torch.float64).reshape(B, S)
+    y = y.contiguous()
+    out = torch.empty((B, S), dtype=torch.float64, device=y.device)
+    dt = _lib.BH_F32 if y.dtype == torch.float32 else _lib.BH_F64
+    code = _lib.BH_EMB_CORR_DIM if which == "corr_dim" else _lib.BH_EMB_LYAP_R
+    s = torch.cuda.current_stream(y.device)
+    with torch.cuda.device(y.device):
+        _lib.check(_lib.lib().bh_window_embedding(dt, y.data_ptr(), N, B, S, int(start), code,
+                                                  out.data_ptr(), s.cuda_stream),
+                   "bh_window_embedding")
+    return out
+
+
 def feature_family(name: str, params: dict | None = None) -> str:
     """Which kernel output a comprehensive feature is formed from: "block"
     (bh_window_features), "spectrum", "sorted", "autocov", "trends" or
@@ -1063,6 +1161,8 @@ def feature_family(name: str, params: dict | None = None) -> str:
         return "peaks"
     if name in PEAK_FEATURES:
         return "peaks"
+    if name in EMBED_FEATURES:
+        return "embedding"
     if name in SPECTRUM_FEATURES:
         return "spectrum"
     if name in SORTED_FEATURES:
@@ -1133,6 +1233,11 @@ class WindowBlocks:
 
     def feature(self, name: str, params: dict | None) -> torch.Tensor:
         """Comprehensive feature `name` (B, S) float64 of the window."""
+        if name == "correlation_dimension":
+            return self._get("corr_dim", lambda: window_embedding(self.y, self.start, "corr_dim"))
+        if name == "lyapunov_r":  # torch_features_dynamical.py:23-42: the slope over tau
+            slope = self._get("lyap_r", lambda: window_embedding(self.y, self.start, "lyap_r"))
+            return slope / float((params or {}).get("tau", 1.0))
         if feature_family(name, params) == "peaks":
             perm = self.perm_entropy() if name == "permutation_entropy" else None
             counts = self.peak_counts() if name != "permutation_entropy" else {}

@@ -641,3 +641,62 @@ def test_window_peaks_on_integrated_duffing(cuda_device):
         assert f[b, 0] == O.comprehensive(x[:, b, 0], "number_peaks", {"n": 10})
         assert f[b, 4] == pytest.approx(O.comprehensive(x[:, b, 0], "permutation_entropy",
                                                         {"tau": 1, "dimension": 5}), rel=1e-6)
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_window_embedding_kernels_equal_oracle(cuda_device, dtype):
+    """correlation_dimension and the Rosenstein slope (lyapunov_r) of noisy
+    sinusoids, noise, the logistic map, constants, short and non-finite
+    windows against the oracle; the CPU path agrees."""
+    from pybasin_amd.window_features import window_embedding
+
+    rng = np.random.default_rng(47)
+    n = 260
+    logistic = np.empty(n)
+    logistic[0] = 0.3
+    for i in range(1, n):
+        logistic[i] = 4 * logistic[i - 1] * (1 - logistic[i - 1])
+    w = np.concatenate([np.sin(np.arange(n)[:, None, None] * rng.uniform(0.05, 0.5, size=(1, 6, 2)))
+                        + 0.05 * rng.normal(size=(n, 6, 2)), rng.normal(size=(n, 2, 2)),
+                        np.stack([logistic, logistic[::-1]], 1)[:, None, :],
+                        np.full((n, 1, 2), 0.5)], 1)
+    w[17, 9, 1] = np.nan
+    cases = [("mixed", w), ("short", rng.normal(size=(12, 3, 2))), ("tiny", rng.normal(size=(3, 2, 1)))]
+    for tag, win in cases:
+        wt = torch.tensor(win, dtype=dtype)
+        x = wt.double().numpy()
+        for which, name in (("corr_dim", "correlation_dimension"), ("lyap_r", "lyapunov_r")):
+            got = window_embedding(wt.cuda(), 0, which).cpu().numpy()
+            B, S = got.shape
+            for b in range(B):
+                for s in range(S):
+                    want = O.comprehensive(x[:, b, s], name, {"tau": 1.0})
+                    g = float(got[b, s])
+                    if not math.isfinite(want):
+                        assert not math.isfinite(g), (tag, name, b, s, g)
+                    else:
+                        assert g == pytest.approx(want, rel=1e-9, abs=1e-12), (tag, name, b, s, g, want)
+            cpu = window_embedding(wt, 0, which).numpy()
+            np.testing.assert_allclose(np.nan_to_num(got, nan=7.0), np.nan_to_num(cpu, nan=7.0),
+                                       rtol=1e-9, atol=1e-12)
+
+
+def test_window_embedding_on_integrated_duffing(cuda_device):
+    """Both estimates on integrated Duffing windows (kept float32 trajectory)
+    against the oracle on six series, and through the extractor."""
+    from pybasin_amd.window_features import window_embedding
+
+    res = _duffing(torch.float32, n=300)
+    x = res.y.double().cpu().numpy()[res.steady_idx:]
+    cd = window_embedding(res.y, res.steady_idx, "corr_dim").cpu().numpy()
+    ly = window_embedding(res.y, res.steady_idx, "lyap_r").cpu().numpy()
+    for b in range(3):
+        for s in range(2):
+            assert cd[b, s] == pytest.approx(O.comprehensive(x[:, b, s], "correlation_dimension", {}),
+                                             rel=1e-9)
+            assert ly[b, s] == pytest.approx(O.comprehensive(x[:, b, s], "lyapunov_r", {"tau": 1.0}),
+                                             rel=1e-9, abs=1e-12)
+    fx = StatFeatureExtractor(time_steady=300.0, features={
+        "lyapunov_r": [{"emb_dim": 10, "lag": 1, "trajectory_len": 20, "tau": 2.0}]})
+    f = fx.extract_features(Solution(res.y[0], res.t, res.y)).cpu().numpy()
+    np.testing.assert_array_equal(f[:, 0], (ly[:, 0] / 2.0).astype(np.float32))

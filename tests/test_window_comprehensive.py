@@ -556,6 +556,57 @@ def test_cpu_peaks_path_equals_oracle(dtype):
                     assert got[b, s] == pytest.approx(want, rel=1e-12, abs=1e-12), (name, p, N, b, s)
 
 
+def test_oracle_embedding_pins():
+    """correlation_dimension: a straight line and the logistic map's orbit
+    (one-dimensional sets) give about 1, white noise more; its correlation
+    sums equal scipy's pdist counts (ordered pairs: twice the unordered).
+    lyapunov_r: a sinusoid's divergence slope is about 0, the chaotic
+    logistic map's clearly positive."""
+    from scipy.spatial.distance import pdist
+
+    C = O.comprehensive
+    logistic = np.empty(400)
+    logistic[0] = 0.3
+    for i in range(1, 400):
+        logistic[i] = 4 * logistic[i - 1] * (1 - logistic[i - 1])
+    assert C(np.arange(300.0), "correlation_dimension", {}) == pytest.approx(1.0, abs=0.1)
+    assert C(logistic, "correlation_dimension", {}) == pytest.approx(1.0, abs=0.15)
+    noise = np.random.default_rng(5).normal(size=400)
+    assert C(noise, "correlation_dimension", {}) > 2.0
+    x = noise[:200]
+    m = x.size - 3
+    orb = np.stack([x[e: e + m] for e in range(4)], 1)
+    dd = pdist(orb)
+    sd = x.std(ddof=1)
+    L0, L1 = np.log(0.1 * sd + 1e-10), np.log(0.5 * sd + 1e-10)
+    lr = L0 + np.arange(50) / 49 * (L1 - L0)
+    c = np.array([2 * (dd <= np.exp(v)).sum() for v in lr]) / (m * (m - 1))
+    ok = c > 0
+    want = np.polyfit(lr[ok], np.log(c[ok] + 1e-30), 1)[0]
+    assert C(x, "correlation_dimension", {}) == pytest.approx(want, rel=1e-9)
+    assert abs(C(np.sin(np.arange(600) * 0.2), "lyapunov_r", {"tau": 1.0})) < 0.01
+    assert C(logistic, "lyapunov_r", {"tau": 1.0}) > 0.05
+    assert C(logistic, "lyapunov_r", {"tau": 2.0}) == pytest.approx(
+        C(logistic, "lyapunov_r", {"tau": 1.0}) / 2)
+    assert math.isnan(C(np.arange(8.0), "lyapunov_r", {"tau": 1.0}))  # no embedded vector
+
+
+def test_cpu_embedding_path_equals_oracle():
+    from pybasin_amd.window_features import window_embedding
+
+    rng = np.random.default_rng(15)
+    w = np.concatenate([np.sin(np.arange(180)[:, None, None] * rng.uniform(0.05, 0.5, size=(1, 3, 1)))
+                        + 0.05 * rng.normal(size=(180, 3, 1)), rng.normal(size=(180, 1, 1))], 1)
+    wt = torch.tensor(w)
+    cd = window_embedding(wt, 0, "corr_dim").numpy()
+    lr = window_embedding(wt, 0, "lyap_r").numpy()
+    for b in range(4):
+        assert cd[b, 0] == pytest.approx(O.comprehensive(w[:, b, 0], "correlation_dimension", {}),
+                                         rel=1e-9)
+        assert lr[b, 0] == pytest.approx(O.comprehensive(w[:, b, 0], "lyapunov_r", {"tau": 1.0}),
+                                         rel=1e-9, abs=1e-12)
+
+
 def _windows():
     rng = np.random.default_rng(11)
     out = [rng.normal(size=(200, 3, 2)),
