@@ -739,14 +739,18 @@ def comprehensive_workload(args):
     templates, float32, 1,000 rows, 2 states), integrated once outside the
     timed region; a step = all window kernels (window-feature slots,
     spectrum, sort + change quantiles, autocovariances, chunk trends / fits /
-    wavelets, peak counts, ordinal patterns, the median) plus every named
-    feature of every state.  Each kernel is also timed alone with HIP events.
-    Roofline: HBM over the kernels' window reads (block 2, spectrum 1, sort 1,
-    change quantiles 2, autocovariances 1, trends 2, peaks 1, patterns 1) and
-    their float64 outputs, against the summed kernel time."""
+    wavelets, peak counts, ordinal patterns, the delay-embedding estimates,
+    the Lempel-Ziv parses) plus every named feature of every state.  Each
+    kernel is also timed alone with HIP events.  Roofline: HBM over the
+    kernels' window reads (block 2, spectrum 1, sort 1, change quantiles 2,
+    autocovariances 1, trends 2, peaks 1, patterns 1, embeddings 1 each,
+    Lempel-Ziv 1) and their float64 outputs, against the summed kernel time
+    (the O(n^2) embedding estimates and the sequential parses dominate it and
+    are bound by their own issue, not by HBM)."""
     from pybasin_amd.features import window_feature
     from pybasin_amd.window_features import (COMPREHENSIVE_FEATURES, WindowBlocks, window_autocov,
-                                             window_feature_block, window_peak_counts,
+                                             window_embedding, window_feature_block,
+                                             window_lempel_ziv, window_peak_counts,
                                              window_perm_entropy, window_sorted, window_spectrum,
                                              window_trends)
 
@@ -772,6 +776,9 @@ def comprehensive_workload(args):
         "trends": (lambda: window_trends(y, 0, True), 2, 114),
         "peak_counts": (lambda: window_peak_counts(y, 0, [10, 50]), 1, 12),
         "perm_entropy": (lambda: window_perm_entropy(y, 0), 1, 5),
+        "corr_dim": (lambda: window_embedding(y, 0, "corr_dim"), 1, 1),
+        "lyap_r": (lambda: window_embedding(y, 0, "lyap_r"), 1, 1),
+        "lempel_ziv": (lambda: window_lempel_ziv(y, 0, [2, 3, 5, 10, 100]), 1, 5),
     }
     kernel_ms, alg_bytes = {}, 0.0
     for name, (fn, reads, rows) in kernels.items():

@@ -551,6 +551,19 @@ enum { BH_EMB_CORR_DIM = 0, BH_EMB_LYAP_R = 1 };
 int32_t bh_window_embedding(int32_t dtype, const void *traj, int64_t n_save, int64_t n,
                             int32_t dim, int64_t start, int32_t which, double *out, void *stream);
 
+/* lempel_ziv_complexity of each series of the same window for n_bins (1..8)
+   bins settings bins[k] (host array, 1..255 each;
+   src/pybasin/ts_torch/calculators/torch_features_entropy_complexity.py:
+   140-180): the window cut into bins[k] symbols between its min and max (in
+   the series dtype), parsed into phrases (each the shortest string at its
+   position not seen before; a tail of seen prefixes is not counted); out
+   (n_bins, n, dim) float64 = phrase count / (w / (ln w + 1e-10)), NaN for
+   non-finite windows.  Windows of at most 1,024 rows (five settings fit the
+   LDS budget at any such window). */
+int32_t bh_window_lempel_ziv(int32_t dtype, const void *traj, int64_t n_save, int64_t n,
+                             int32_t dim, int64_t start, const int32_t *bins, int32_t n_bins,
+                             double *out, void *stream);
+
 /* Autocovariances of each series of the same window: out (max_lag + 1, n,
    dim) float64 holds C_k = sum_{i < w - k} (x_i - mean)(x_{i+k} - mean) for
    k = 0 .. max_lag (0 for k >= w), in double: the linear autocorrelation of

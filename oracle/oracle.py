@@ -743,4 +743,23 @@ def comprehensive(x: np.ndarray, name: str, params: dict | None = None, dtype=np
         den = ks.size * (kk * kk).sum() - kk.sum() ** 2
         slope = (ks.size * (kk * dd).sum() - kk.sum() * dd.sum()) / den if abs(den) > 1e-10 else np.nan
         return float(slope / float(p.get("tau", 1.0)))
+    if name == "lempel_ziv_complexity":  # torch_features_entropy_complexity.py:140-180
+        nb = int(p.get("bins", 2))
+        mn, mx = xs.min(), xs.max()
+        rng = dtype(mx - mn + dtype(1e-10))
+        sym = np.clip(((xs - mn) / rng * dtype(nb)).astype(np.int64), 0, nb - 1)
+        s_ = tuple(int(v) for v in sym)
+        seen: set = set()
+        i, c = 0, 0
+        while i < n:  # the reference's set of seen substrings, literally
+            ln = 1
+            while i + ln <= n:
+                sub = s_[i: i + ln]
+                if sub not in seen:
+                    seen.add(sub)
+                    c += 1
+                    break
+                ln += 1
+            i += ln
+        return float(c / (n / (math.log(n) + 1e-10) if n > 1 else 1.0))
     raise KeyError(name)

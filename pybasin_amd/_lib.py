@@ -49,7 +49,8 @@ EXPORTS = ("bh_version", "bh_last_error", "bh_solve_batch", "bh_solve_batch_peak
            "bh_window_order_stats", "bh_window_autocorr_periodicity", "bh_window_spectral_ratio",
            "bh_window_features", "bh_window_spectrum", "bh_window_sorted",
            "bh_window_change_quantiles", "bh_window_autocov", "bh_window_trends",
-           "bh_window_peak_counts", "bh_window_perm_entropy", "bh_window_embedding")
+           "bh_window_peak_counts", "bh_window_perm_entropy", "bh_window_embedding",
+           "bh_window_lempel_ziv")
 
 _lock = threading.Lock()
 _lib: ctypes.CDLL | None = None
@@ -128,6 +129,7 @@ def _declare(L: ctypes.CDLL) -> None:
         "bh_window_peak_counts": [i32, vp, i64, i64, i32, i64, i32, i32, i32, i32, vp, vp],
         "bh_window_perm_entropy": [i32, vp, i64, i64, i32, i64, vp, vp],
         "bh_window_embedding": [i32, vp, i64, i64, i32, i64, i32, vp, vp],
+        "bh_window_lempel_ziv": [i32, vp, i64, i64, i32, i64, vp, i32, vp, vp],
     }
     for name, args in sigs.items():
         fn = getattr(L, name)

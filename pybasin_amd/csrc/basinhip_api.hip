@@ -1068,6 +1068,50 @@ int32_t bh_window_embedding(int32_t dtype, const void *traj, int64_t n_save, int
     return 0;
 }
 
+int32_t bh_window_lempel_ziv(int32_t dtype, const void *traj, int64_t n_save, int64_t n,
+                             int32_t dim, int64_t start, const int32_t *bins, int32_t n_bins,
+                             double *out, void *stream) {
+    g_err[0] = 0;
+    int32_t rc = check_series_args("bh_window_lempel_ziv", dtype, traj, n_save, n, dim, start);
+    if (rc) return rc;
+    if (!out || !bins) return fail(-5, "bh_window_lempel_ziv: null bins or out");
+    if (n_bins < 1 || n_bins > 8) return fail(-5, "bh_window_lempel_ziv: n_bins must be in [1, 8]");
+    int32_t bv[8] = {2, 2, 2, 2, 2, 2, 2, 2};
+    for (int k = 0; k < n_bins; k++) {
+        if (bins[k] < 1 || bins[k] > 255) return fail(-5, "bh_window_lempel_ziv: bins must be in [1, 255]");
+        bv[k] = bins[k];
+    }
+    const int64_t w = n_save - start;
+    if (w > 1024) return fail(-5, "bh_window_lempel_ziv: windows of at most 1,024 rows (got %lld)", (long long)w);
+    if (n == 0) return 0;
+    const int64_t elem = dtype == BH_F32 ? 4 : 8;
+    int cap_log = 6;
+    while ((1LL << cap_log) < 2 * w) cap_log++;
+    // one hash table of 2^cap_log 32-bit entries per bins setting, then the series rows
+    const int64_t budget = 62 * 1024, tables = (int64_t)n_bins * (4LL << cap_log);
+    if (tables + w * elem > budget)
+        return fail(-5, "bh_window_lempel_ziv: %d settings of a %lld-row window exceed the LDS budget",
+                    n_bins, (long long)w);
+    int64_t G = (budget - tables) / (w * elem);
+    if (G > 4) G = 4;
+    const size_t lds = (size_t)((G * w * elem + 15) / 16 * 16 + tables);
+    const int64_t np = n * dim;
+    const int64_t blocks = (np + G - 1) / G;
+    if (blocks > 2147483647LL) return fail(-5, "bh_window_lempel_ziv: too many series");
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    const int4 ba = make_int4(bv[0], bv[1], bv[2], bv[3]), bb = make_int4(bv[4], bv[5], bv[6], bv[7]);
+    if (dtype == BH_F32)
+        hipLaunchKernelGGL(bh::win::lempel_ziv_kernel<float>, dim3((unsigned)blocks), dim3(64), lds, s,
+                           reinterpret_cast<const float *>(traj), n_save, n, dim, start, (int)G, cap_log,
+                           ba, bb, n_bins, out);
+    else
+        hipLaunchKernelGGL(bh::win::lempel_ziv_kernel<double>, dim3((unsigned)blocks), dim3(64), lds, s,
+                           reinterpret_cast<const double *>(traj), n_save, n, dim, start, (int)G, cap_log,
+                           ba, bb, n_bins, out);
+    BH_HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
 extern "C++" template <class T>
 static void launch_sorted(int64_t P, int64_t blocks, size_t lds, hipStream_t s, const T *traj,
                           int64_t n_save, int64_t n, int32_t dim, int64_t start, int64_t G,

@@ -1579,6 +1579,101 @@ __global__ void __launch_bounds__(kSeriesBlock)
     }
 }
 
+// ---- Lempel-Ziv phrase counts ----------------------------------------------
+
+constexpr int kLzMaxBins = 8;  // bins settings per launch
+
+// lempel_ziv_complexity (torch_features_entropy_complexity.py:140-180): the
+// window is cut into symbols b_i = clamp(floor((x_i - min) / (max - min +
+// 1e-10) * bins), 0, bins - 1) (in the series dtype), then parsed left to
+// right into phrases, each the shortest string at its position not seen
+// before; a tail whose every prefix was seen is not counted.  The phrases
+// form a trie (a new phrase extends a seen one by a symbol), kept as an
+// open-addressing hash table (parent * bins + symbol + 1 in the upper 20
+// bits, the node in the lower 12; 2^cap_log >= 2 w entries, so at most half
+// full and every probe ends) in LDS.  One wave per block, G series staged
+// together, lane k parsing each for bins setting k (nb settings at most 8).
+// out (nb, n, dim): the phrase count over w / (ln w + 1e-10) (over 1 for
+// w = 1); NaN for a non-finite window.
+template <class T>
+__global__ void __launch_bounds__(64)
+    lempel_ziv_kernel(const T *traj, int64_t n_save, int64_t n, int S, int64_t start, int G,
+                      int cap_log, int4 bins_a, int4 bins_b, int nb, double *out) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int64_t w = n_save - start;
+    const int64_t np = n * S;
+    const int64_t p0 = (int64_t)blockIdx.x * G;
+    T *lds = reinterpret_cast<T *>(smem);
+    unsigned *tabs = reinterpret_cast<unsigned *>(smem + ((int64_t)G * w * sizeof(T) + 15) / 16 * 16);
+    const int lane = threadIdx.x;
+    const unsigned cap = 1u << cap_log;
+    load_series<T>(traj, n, S, start, w, p0, G, np, lds);
+    __syncthreads();
+    const int binsv[kLzMaxBins] = {bins_a.x, bins_a.y, bins_a.z, bins_a.w,
+                                   bins_b.x, bins_b.y, bins_b.z, bins_b.w};
+    int bins = 2;
+#pragma unroll
+    for (int k = 0; k < kLzMaxBins; k++)
+        if (k == lane) bins = binsv[k];
+    unsigned *tab = tabs + (lane < nb ? lane : 0) * cap;
+    const double bn = w > 1 ? (double)w / (log((double)w) + 1e-10) : 1.0;
+    for (int g = 0; g < G; g++) {
+        const int64_t p = p0 + g;
+        if (p >= np) break;  // block-uniform
+        const T *x = lds + (int64_t)g * w;
+        T mn = x[0], mx = x[0];
+        bool bad = false;
+        for (int64_t i = 0; i < w; i++) {  // every lane the same (broadcast reads)
+            const T v = x[i];
+            bad |= !isfinite((double)v);
+            mn = v < mn ? v : mn;
+            mx = v > mx ? v : mx;
+        }
+        if (lane >= nb) continue;
+        if (bad) {
+            out[lane * np + p] = NAN;
+            continue;
+        }
+        for (unsigned k = 0; k < cap; k++) tab[k] = 0u;
+        const T rng = mx - mn + (T)1e-10;
+        int count = 0, nodes = 1;  // node 0 is the root
+        int64_t i = 0;
+        while (i < w) {
+            int cur = 0;
+            int64_t len = 1;
+            bool added = false;
+            while (i + len <= w) {
+                int sym = (int)((x[i + len - 1] - mn) / rng * (T)bins);
+                sym = sym < 0 ? 0 : (sym > bins - 1 ? bins - 1 : sym);
+                const unsigned key = (unsigned)cur * (unsigned)bins + (unsigned)sym + 1u;
+                unsigned h = (key * 2654435761u) >> (32 - cap_log);
+                int child = -1;
+                for (;;) {
+                    const unsigned e = tab[h];
+                    if (e == 0u) break;
+                    if ((e >> 12) == key) {
+                        child = (int)(e & 0xfffu);
+                        break;
+                    }
+                    h = (h + 1u) & (cap - 1u);
+                }
+                if (child < 0) {  // a new phrase: add it as cur's child
+                    tab[h] = (key << 12) | (unsigned)nodes;
+                    nodes++;
+                    count++;
+                    added = true;
+                    break;
+                }
+                cur = child;
+                len++;
+            }
+            i += len;
+            if (!added) break;
+        }
+        out[lane * np + p] = (double)count / bn;
+    }
+}
+
 // ---- sorted-window statistics: duplicates and change quantiles -------------
 
 // the six quantile levels whose corridors change_quantiles takes

@@ -139,6 +139,11 @@ EMBED_FEATURES: dict[str, list[dict]] = {
     "lyapunov_r": [{"emb_dim": 10, "lag": 1, "trajectory_len": 20, "tau": 1.0}],
 }
 COMPREHENSIVE_FEATURES.update(EMBED_FEATURES)
+# phrase counts (bh_window_lempel_ziv)
+LZ_FEATURES: dict[str, list[dict]] = {
+    "lempel_ziv_complexity": [{"bins": x} for x in [2, 3, 5, 10, 100]],
+}
+COMPREHENSIVE_FEATURES.update(LZ_FEATURES)
 TR = dict(ALT=0, FRIED=48, LANGEVIN=52, BENFORD=53, CWT=54)
 BH_TR_SLOTS = 114
 # the remaining regression calculator formed from the window-feature slots
@@ -179,6 +184,10 @@ def check_params(name: str, params: dict | None) -> None:
         if isinstance(params.get("max_width"), int) and 1 <= params["max_width"] <= 8:
             return
         raise ValueError(f"number_cwt_peaks: max_width must be in [1, 8], got {params}")
+    if name == "lempel_ziv_complexity":
+        if isinstance(params.get("bins"), int) and 1 <= params["bins"] <= 255:
+            return
+        raise ValueError(f"lempel_ziv_complexity: bins must be an integer in [1, 255], got {params}")
     if name == "correlation_dimension":
         if (params.get("emb_dim", 4), params.get("lag", 1), params.get("n_rvals", 50)) == (4, 1, 50):
             return
@@ -1153,6 +1162,67 @@ def window_embedding(y: torch.Tensor, start: int, which: str) -> torch.Tensor:
     return out
 
 
+def _lempel_ziv_cpu(w: torch.Tensor, bins: list) -> torch.Tensor:
+    """The outputs of bh_window_lempel_ziv for a CPU window (N, B, S)."""
+    N, B, S = w.shape
+    mn, mx = w.min(0).values, w.max(0).values
+    rng = mx - mn + torch.tensor(1e-10, dtype=w.dtype)
+    ok = torch.isfinite(w).all(0)
+    bn = N / (math.log(N) + 1e-10) if N > 1 else 1.0
+    out = torch.empty((len(bins), B, S), dtype=torch.float64)
+    for k, nb in enumerate(bins):
+        sym = ((w - mn) / rng * nb).long().clamp(0, nb - 1)
+        for b in range(B):
+            for s in range(S):
+                if not bool(ok[b, s]):
+                    out[k, b, s] = math.nan
+                    continue
+                seq = sym[:, b, s].tolist()
+                trie: dict = {}
+                nodes, count, i = 1, 0, 0
+                while i < N:
+                    cur, ln, added = 0, 1, False
+                    while i + ln <= N:
+                        child = trie.get((cur, seq[i + ln - 1]))
+                        if child is None:
+                            trie[(cur, seq[i + ln - 1])] = nodes
+                            nodes += 1
+                            count += 1
+                            added = True
+                            break
+                        cur = child
+                        ln += 1
+                    i += ln
+                    if not added:
+                        break
+                out[k, b, s] = count / bn
+    return out
+
+
+def window_lempel_ziv(y: torch.Tensor, start: int, bins: list) -> torch.Tensor:
+    """(len(bins), B, S) float64 lempel_ziv_complexity of the window rows
+    [start, N) of y for each bins setting (bh_window_lempel_ziv on the
+    device)."""
+    N, B, S = y.shape
+    if not 0 <= start < N:
+        raise ValueError("empty feature window")
+    if y.device.type != "cuda":
+        return _lempel_ziv_cpu(y[start:], list(bins))
+    import ctypes
+
+    y = y.contiguous()
+    out = torch.empty((len(bins), B, S), dtype=torch.float64, device=y.device)
+    arr = (ctypes.c_int32 * len(bins))(*[int(b) for b in bins])
+    dt = _lib.BH_F32 if y.dtype == torch.float32 else _lib.BH_F64
+    s = torch.cuda.current_stream(y.device)
+    with torch.cuda.device(y.device):
+        _lib.check(_lib.lib().bh_window_lempel_ziv(dt, y.data_ptr(), N, B, S, int(start),
+                                                   ctypes.cast(arr, ctypes.c_void_p), len(bins),
+                                                   out.data_ptr(), s.cuda_stream),
+                   "bh_window_lempel_ziv")
+    return out
+
+
 def feature_family(name: str, params: dict | None = None) -> str:
     """Which kernel output a comprehensive feature is formed from: "block"
     (bh_window_features), "spectrum", "sorted", "autocov", "trends" or
@@ -1163,6 +1233,8 @@ def feature_family(name: str, params: dict | None = None) -> str:
         return "peaks"
     if name in EMBED_FEATURES:
         return "embedding"
+    if name in LZ_FEATURES:
+        return "lempel_ziv"
     if name in SPECTRUM_FEATURES:
         return "spectrum"
     if name in SORTED_FEATURES:
@@ -1231,8 +1303,21 @@ class WindowBlocks:
     def perm_entropy(self):
         return self._get("perm", lambda: window_perm_entropy(self.y, self.start))
 
+    def lempel_ziv(self) -> dict:
+        def make():
+            bins = sorted({int(p["bins"]) for _, nm, p in self.plan if nm == "lempel_ziv_complexity"})
+            rows: dict = {}
+            for i in range(0, len(bins), 8):
+                chunk = bins[i:i + 8]
+                out = window_lempel_ziv(self.y, self.start, chunk)
+                rows.update({b: out[k] for k, b in enumerate(chunk)})
+            return rows
+        return self._get("lz", make)
+
     def feature(self, name: str, params: dict | None) -> torch.Tensor:
         """Comprehensive feature `name` (B, S) float64 of the window."""
+        if name == "lempel_ziv_complexity":  # entropy_complexity.py:140-180
+            return self.lempel_ziv()[int((params or {})["bins"])]
         if name == "correlation_dimension":
             return self._get("corr_dim", lambda: window_embedding(self.y, self.start, "corr_dim"))
         if name == "lyapunov_r":  # torch_features_dynamical.py:23-42: the slope over tau

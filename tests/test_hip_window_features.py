@@ -700,3 +700,35 @@ def test_window_embedding_on_integrated_duffing(cuda_device):
         "lyapunov_r": [{"emb_dim": 10, "lag": 1, "trajectory_len": 20, "tau": 2.0}]})
     f = fx.extract_features(Solution(res.y[0], res.t, res.y)).cpu().numpy()
     np.testing.assert_array_equal(f[:, 0], (ly[:, 0] / 2.0).astype(np.float32))
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_window_lempel_ziv_kernel_equals_oracle(cuda_device, dtype):
+    """Phrase counts for bins 2, 3, 5, 10, 100 (and eight settings at once) on
+    the synthetic windows, a 1,024-row window and integrated Duffing windows
+    equal the oracle (the reference's set-of-substrings parse) exactly."""
+    from pybasin_amd.window_features import window_lempel_ziv
+
+    rng = np.random.default_rng(53)
+    res = _duffing(dtype, n=64)
+    duff = res.y.cpu().double().numpy()[res.steady_idx:]
+    cases = list(_synthetic_windows()) + [("big", rng.normal(size=(1024, 3, 2))), ("duffing", duff)]
+    for tag, w in cases:
+        wt = torch.tensor(w, dtype=dtype)
+        sd = np.float32 if dtype == torch.float32 else np.float64
+        for bins in ([2, 3, 5, 10, 100], [1, 2, 4, 7, 16, 33, 64, 255]):
+            if tag == "big" and len(bins) == 8:
+                continue  # eight tables of a 1,024-row window exceed the LDS budget
+            got = window_lempel_ziv(wt.cuda(), 0, bins).cpu().numpy()
+            x = wt.numpy()
+            for k, nb in enumerate(bins):
+                for b in range(min(w.shape[1], 16)):
+                    for s in range(w.shape[2]):
+                        want = O.comprehensive(x[:, b, s], "lempel_ziv_complexity", {"bins": nb}, sd)
+                        g = float(got[k, b, s])
+                        if not math.isfinite(want):
+                            assert not math.isfinite(g), (tag, nb, b, s)
+                        else:
+                            assert g == pytest.approx(want, rel=1e-12), (tag, nb, b, s, g, want)
+    with pytest.raises(ValueError, match="1,024 rows"):
+        window_lempel_ziv(torch.zeros(1025, 2, 1, dtype=dtype, device="cuda"), 0, [2])

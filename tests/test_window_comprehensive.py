@@ -607,6 +607,38 @@ def test_cpu_embedding_path_equals_oracle():
                                          rel=1e-9, abs=1e-12)
 
 
+def test_oracle_lempel_ziv_known_answers():
+    """Hand-parsed phrases: 0|1|00|11|01 then a seen tail 00 (5 phrases of
+    10 symbols); a constant series parses as 0|00|000|0000 (4 of 10 or 12
+    symbols: the seen tail 00 of the 12 is not counted); the count is over
+    n / ln n."""
+    C = O.comprehensive
+    x = np.array([0, 1, 0, 0, 1, 1, 0, 1, 0, 0.0])
+    assert C(x, "lempel_ziv_complexity", {"bins": 2}) == pytest.approx(5 * math.log(10) / 10)
+    assert C(np.zeros(10), "lempel_ziv_complexity", {"bins": 3}) == pytest.approx(4 * math.log(10) / 10)
+    assert C(np.zeros(12), "lempel_ziv_complexity", {"bins": 3}) == pytest.approx(4 * math.log(12) / 12)
+    # three bins between 0 and 2: symbols 0, 1, 2 (2 clamped from 3 * 2 / 2)
+    assert C(np.array([0.0, 1.0, 2.0]), "lempel_ziv_complexity", {"bins": 3}) == \
+        pytest.approx(3 * math.log(3) / 3)
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_cpu_lempel_ziv_path_equals_oracle(dtype):
+    from pybasin_amd.window_features import window_lempel_ziv
+
+    for w in _windows():
+        wt = torch.tensor(w, dtype=dtype)
+        N, B, S = w.shape
+        bins = [2, 3, 5, 10, 100]
+        out = window_lempel_ziv(wt, 0, bins).numpy()
+        for k, nb in enumerate(bins):
+            for b in range(B):
+                for s in range(S):
+                    want = O.comprehensive(wt[:, b, s].numpy(), "lempel_ziv_complexity", {"bins": nb},
+                                           np.float32 if dtype == torch.float32 else np.float64)
+                    assert out[k, b, s] == pytest.approx(want, rel=1e-12), (nb, N, b, s)
+
+
 def _windows():
     rng = np.random.default_rng(11)
     out = [rng.normal(size=(200, 3, 2)),
