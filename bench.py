@@ -778,6 +778,7 @@ def comprehensive_workload(args):
         "perm_entropy": (lambda: window_perm_entropy(y, 0), 1, 5),
         "corr_dim": (lambda: window_embedding(y, 0, "corr_dim"), 1, 1),
         "lyap_r": (lambda: window_embedding(y, 0, "lyap_r"), 1, 1),
+        "lyap_e": (lambda: window_embedding(y, 0, "lyap_e"), 1, 4),
         "lempel_ziv": (lambda: window_lempel_ziv(y, 0, [2, 3, 5, 10, 100]), 1, 5),
     }
     kernel_ms, alg_bytes = {}, 0.0

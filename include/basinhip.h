@@ -539,15 +539,19 @@ int32_t bh_window_perm_entropy(int32_t dtype, const void *traj, int64_t n_save, 
                                int32_t dim, int64_t start, double *out, void *stream);
 
 /* Delay-embedding estimates of each series of the same window, in double
-   (out (n, dim) float64; NaN for non-finite windows):
+   (out (n, dim) float64 unless stated; NaN for non-finite windows):
    BH_EMB_CORR_DIM: correlation_dimension with emb_dim 4, lag 1, 50 radii
    (src/pybasin/ts_torch/torch_corr_dim.py:44-124: the log-log slope of the
    correlation sum over radii log-spaced in [0.1 sd, 0.5 sd]);
    BH_EMB_LYAP_R: lyapunov_r with emb_dim 10, lag 1, trajectory_len 20
    (torch_lyapunov_r.py:20-150: Rosenstein's divergence slope, nearest
-   neighbours at a temporal separation above the mean period; divide by tau).
+   neighbours at a temporal separation above the mean period; divide by tau);
+   BH_EMB_LYAP_E: lyapunov_e with emb_dim 10, matrix_dim 4, min_nb 8,
+   min_tsep 0 (torch_lyapunov_e.py:15-150: Eckmann's four exponents, out
+   (4, n, dim), inf for non-finite windows and for exponents nothing was
+   accumulated for; divide by tau).
    Windows that fit the 62 KB LDS budget. */
-enum { BH_EMB_CORR_DIM = 0, BH_EMB_LYAP_R = 1 };
+enum { BH_EMB_CORR_DIM = 0, BH_EMB_LYAP_R = 1, BH_EMB_LYAP_E = 2 };
 int32_t bh_window_embedding(int32_t dtype, const void *traj, int64_t n_save, int64_t n,
                             int32_t dim, int64_t start, int32_t which, double *out, void *stream);
 

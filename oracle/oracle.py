@@ -17,6 +17,7 @@ Contents
 * ``amplitude``, ``linear_trend_slope``, ``autocorrelation_periodicity``,
   ``spectral_frequency_ratio``, ``local_maxima_1d``
                        the dynamical-system feature set (ts_torch/settings.py:407-417).
+* ``lyapunov_e``       Eckmann's exponents (ts_torch/torch_lyapunov_e.py:15-150).
 * ``comprehensive``    one calculator of the comprehensive set on one window
                        series (ts_torch/settings.py:223-340; calculators in
                        ts_torch/calculators/torch_features_*.py), per series in
@@ -331,6 +332,44 @@ def _torch_quantile(srt: np.ndarray, q: float, dtype) -> float:
     return _fma_round(-d, dtype(1 - wgt), b, dtype)
 
 
+def lyapunov_e(x: np.ndarray, tau: float = 1.0) -> list:
+    """Eckmann's four exponents (torch_lyapunov_e.py:15-150; emb_dim 10,
+    matrix_dim 4, min_nb 8, min_tsep 0), in float64: inf for a non-finite
+    series and for exponents nothing was accumulated for."""
+    x = np.asarray(x, dtype=np.float64)
+    if not np.all(np.isfinite(x)):
+        return [float("inf")] * 4
+    m = 3
+    no = x.size - m - 9
+    q = np.eye(4)
+    lexp, cnt = np.zeros(4), np.zeros(4)
+    if no >= 1:
+        orb = np.stack([x[e: e + no] for e in range(10)], 1)
+        stp = np.stack([x[m * k: m * k + no] for k in range(4)], 1)
+        beta = x[4 * m: 4 * m + no]
+        for i in range(no):
+            d = np.abs(orb - orb[i]).max(1)
+            d[i] = np.inf
+            r = np.sort(d)[7] if no > 8 else np.inf
+            if np.isinf(r):
+                continue
+            nb = d <= r
+            xd, bd = stp[nb] - stp[i], beta[nb] - beta[i]
+            a = np.linalg.solve(xd.T @ xd + 1e-10 * np.eye(4), xd.T @ bd)
+            t = np.zeros((4, 4))
+            t[:3, 1:] = np.eye(3)
+            t[3] = a
+            mq, mr = np.linalg.qr(t @ q)
+            sg = np.sign(np.diag(mr))
+            sg[sg == 0] = 1.0
+            q = mq * sg
+            dg = np.diag(mr) * sg
+            pos = dg > 0
+            lexp[pos] += np.log(dg[pos])
+            cnt[pos] += 1
+    return [float(lexp[k] / cnt[k] / tau / m) if cnt[k] > 0 else float("inf") for k in range(4)]
+
+
 def comprehensive(x: np.ndarray, name: str, params: dict | None = None, dtype=np.float64) -> float:
     """One calculator of the reference's comprehensive set on one series x
     (N,), float64, written from the reference's torch calculators
@@ -341,6 +380,8 @@ def comprehensive(x: np.ndarray, name: str, params: dict | None = None, dtype=np
     x = np.asarray(x, dtype=np.float64)
     p = params or {}
     n = x.size
+    if name == "lyapunov_e":
+        return lyapunov_e(x, float(p.get("tau", 1.0)))
     if not np.all(np.isfinite(x)):
         return float("nan")
     mu = x.mean()

@@ -34,7 +34,7 @@ WF = dict(N=0, SUM=1, SUMSQ=2, MIN=3, MAX=4, FIRST_MAX=5, LAST_MAX=6, FIRST_MIN=
           STRIKE_ABOVE=59, STRIKE_BELOW=60, COUNT_MAX=61, COUNT_MIN=62, BEYOND=63, AUTOCOV=73,
           MASS=82, BINS=90)
 BH_WF_SLOTS = 100
-BH_EMB_CORR_DIM, BH_EMB_LYAP_R = 0, 1
+BH_EMB_CORR_DIM, BH_EMB_LYAP_R, BH_EMB_LYAP_E = 0, 1, 2
 BH_NOALLOC_SCRATCH_BYTES = 64 << 20
 
 # every symbol include/basinhip.h declares

@@ -1026,15 +1026,16 @@ int32_t bh_window_embedding(int32_t dtype, const void *traj, int64_t n_save, int
     int32_t rc = check_series_args("bh_window_embedding", dtype, traj, n_save, n, dim, start);
     if (rc) return rc;
     if (!out) return fail(-5, "bh_window_embedding: null out");
-    if (which != BH_EMB_CORR_DIM && which != BH_EMB_LYAP_R)
-        return fail(-5, "bh_window_embedding: which must be BH_EMB_CORR_DIM or BH_EMB_LYAP_R");
+    if (which != BH_EMB_CORR_DIM && which != BH_EMB_LYAP_R && which != BH_EMB_LYAP_E)
+        return fail(-5, "bh_window_embedding: which must be BH_EMB_CORR_DIM, BH_EMB_LYAP_R or BH_EMB_LYAP_E");
     if (n == 0) return 0;
     const int64_t w = n_save - start;
     const int64_t elem = dtype == BH_F32 ? 4 : 8;
     // the block's series rows (plus, for the correlation sum, the radii and
     // the histogram of each wave)
     const int64_t budget = 62 * 1024;
-    const int64_t extra = which == BH_EMB_CORR_DIM ? 4 * 50 * 8 + 4 * 64 * 4 : 0;
+    const int64_t extra = which == BH_EMB_CORR_DIM ? 4 * 50 * 8 + 4 * 64 * 4
+                          : (which == BH_EMB_LYAP_E ? 4 * 64 * 15 * 8 : 0);
     if (extra + w * elem > budget)
         return fail(-5, "bh_window_embedding: window of %lld rows exceeds the %lld-byte LDS budget",
                     (long long)w, (long long)budget);
@@ -1052,6 +1053,15 @@ int32_t bh_window_embedding(int32_t dtype, const void *traj, int64_t n_save, int
                                n_save, n, dim, start, (int)G, out);
         else
             hipLaunchKernelGGL(bh::win::corr_dim_kernel<double>, dim3((unsigned)blocks),
+                               dim3(bh::win::kSeriesBlock), lds, s,
+                               reinterpret_cast<const double *>(traj), n_save, n, dim, start, (int)G, out);
+    } else if (which == BH_EMB_LYAP_E) {
+        if (dtype == BH_F32)
+            hipLaunchKernelGGL(bh::win::lyap_e_kernel<float>, dim3((unsigned)blocks),
+                               dim3(bh::win::kSeriesBlock), lds, s, reinterpret_cast<const float *>(traj),
+                               n_save, n, dim, start, (int)G, out);
+        else
+            hipLaunchKernelGGL(bh::win::lyap_e_kernel<double>, dim3((unsigned)blocks),
                                dim3(bh::win::kSeriesBlock), lds, s,
                                reinterpret_cast<const double *>(traj), n_save, n, dim, start, (int)G, out);
     } else {

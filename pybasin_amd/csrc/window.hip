@@ -1579,6 +1579,241 @@ __global__ void __launch_bounds__(kSeriesBlock)
     }
 }
 
+constexpr int kLeEmb = 10, kLeDim = 4, kLeStep = 3, kLeNb = 8;
+constexpr int kLeRec = 14;  // the normal equations of one point: 10 of X^T X, 4 of X^T b
+
+// Solves the 4 x 4 system A a = b (A symmetric, given by its upper triangle
+// u[0..9] row by row, plus reg on the diagonal) by Gaussian elimination with
+// partial pivoting; a is written to a[0..3].
+__device__ __forceinline__ void solve4(const double *u, const double *b, double reg, double *a) {
+    double M[4][5];
+    int t = 0;
+#pragma unroll
+    for (int r = 0; r < 4; r++)
+#pragma unroll
+        for (int c = r; c < 4; c++, t++) {
+            M[r][c] = u[t] + (r == c ? reg : 0.0);
+            M[c][r] = M[r][c];
+        }
+#pragma unroll
+    for (int r = 0; r < 4; r++) M[r][4] = b[r];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        int piv = k;
+        double big = fabs(M[k][k]);
+#pragma unroll
+        for (int r = k + 1; r < 4; r++)
+            if (fabs(M[r][k]) > big) {
+                big = fabs(M[r][k]);
+                piv = r;
+            }
+#pragma unroll
+        for (int r = k + 1; r < 4; r++)
+            if (r == piv) {
+#pragma unroll
+                for (int c = 0; c < 5; c++) {
+                    const double h = M[k][c];
+                    M[k][c] = M[r][c];
+                    M[r][c] = h;
+                }
+            }
+#pragma unroll
+        for (int r = k + 1; r < 4; r++) {
+            const double f = M[r][k] / M[k][k];
+#pragma unroll
+            for (int c = k; c < 5; c++) M[r][c] = fma(-f, M[k][c], M[r][c]);
+        }
+    }
+#pragma unroll
+    for (int r = 3; r >= 0; r--) {
+        double v = M[r][4];
+#pragma unroll
+        for (int c = r + 1; c < 4; c++) v = fma(-M[r][c], a[c], v);
+        a[r] = v / M[r][r];
+    }
+}
+
+// lyapunov_e (torch_lyapunov_e.py:15-150; emb_dim 10, matrix_dim 4, min_nb 8,
+// min_tsep 0; the caller divides by tau): Eckmann's method.  The orbit is the
+// 10-sample delay embedding of x[0 .. w - 4] (m = 3, n_orb = w - 12 points);
+// for every point i its neighbours are the points within the 8th smallest
+// Chebyshev distance to it (ties included, itself left out); the least-squares
+// map of the neighbours' offsets x[j + 3k] - x[i + 3k] (k = 0..3) to
+// x[j + 12] - x[i + 12] (normal equations + 1e-10 I) gives the last row of a
+// companion matrix T_i; the running Q of (T_i Q) = Q R (diag R made positive)
+// accumulates ln diag R where positive; the exponents are the means over the
+// points, over 3 (inf where nothing was accumulated).  One wave per series in
+// LDS: the points' neighbour searches and normal equations run 64 at a time
+// (one point per lane, the eight smallest distances kept in registers), their
+// 14 sums go through the wave's LDS rows, and the sequential QR chain runs
+// over them in every lane alike.  out (4, n, dim) in descending accumulation
+// order as the reference leaves it; a non-finite window gives inf.
+template <class T>
+__global__ void __launch_bounds__(kSeriesBlock)
+    lyap_e_kernel(const T *traj, int64_t n_save, int64_t n, int S, int64_t start, int G,
+                  double *out) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int64_t w = n_save - start;
+    const int64_t np = n * S;
+    const int64_t p0 = (int64_t)blockIdx.x * G;
+    T *lds = reinterpret_cast<T *>(smem);
+    double *recs = reinterpret_cast<double *>(smem + ((int64_t)G * w * sizeof(T) + 15) / 16 * 16);
+    load_series<T>(traj, n, S, start, w, p0, G, np, lds);
+    __syncthreads();
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    double *rec = recs + wave * 64 * (kLeRec + 1);
+    for (int g = wave; g < G; g += kSeriesBlock / 64) {
+        const int64_t p = p0 + g;
+        if (p >= np) break;  // wave-uniform
+        const T *x = lds + (int64_t)g * w;
+        bool bad = false;
+        for (int64_t i = lane; i < w; i += 64) bad |= !isfinite((double)x[i]);
+        if (__any(bad)) {
+            if (lane < kLeDim) out[lane * np + p] = INFINITY;
+            continue;
+        }
+        const int no = (int)(w - kLeStep) - (kLeEmb - 1);  // orbit points
+        double Q[4][4] = {{1, 0, 0, 0}, {0, 1, 0, 0}, {0, 0, 1, 0}, {0, 0, 0, 1}};
+        double lexp[4] = {0, 0, 0, 0}, lcnt[4] = {0, 0, 0, 0};
+        for (int i0 = 0; i0 < no; i0 += 64) {
+            const int i = i0 + lane;
+            double sums[kLeRec];
+#pragma unroll
+            for (int k = 0; k < kLeRec; k++) sums[k] = 0.0;
+            bool ok = false;
+            if (i < no) {
+                double a[kLeEmb];
+#pragma unroll
+                for (int e = 0; e < kLeEmb; e++) a[e] = (double)x[i + e];
+                // the eight smallest Chebyshev distances (ascending)
+                double top[kLeNb];
+#pragma unroll
+                for (int k = 0; k < kLeNb; k++) top[k] = INFINITY;
+                for (int j = 0; j < no; j++) {
+                    if (j == i) continue;
+                    double d = 0.0;
+#pragma unroll
+                    for (int e = 0; e < kLeEmb; e++) d = fmax(d, fabs(a[e] - (double)x[j + e]));
+                    if (d < top[kLeNb - 1]) {
+                        double v = d;
+#pragma unroll
+                        for (int k = 0; k < kLeNb; k++) {
+                            const double lo = fmin(top[k], v), hi = fmax(top[k], v);
+                            top[k] = lo;
+                            v = hi;
+                        }
+                    }
+                }
+                const double r = top[kLeNb - 1];
+                if (r < INFINITY) {
+                    ok = true;
+                    double xi[kLeDim];
+#pragma unroll
+                    for (int k = 0; k < kLeDim; k++) xi[k] = (double)x[i + kLeStep * k];
+                    const double bi = (double)x[i + kLeStep * kLeDim];
+                    for (int j = 0; j < no; j++) {
+                        if (j == i) continue;
+                        double d = 0.0;
+#pragma unroll
+                        for (int e = 0; e < kLeEmb; e++) d = fmax(d, fabs(a[e] - (double)x[j + e]));
+                        if (!(d <= r)) continue;
+                        double xd[kLeDim];
+#pragma unroll
+                        for (int k = 0; k < kLeDim; k++) xd[k] = (double)x[j + kLeStep * k] - xi[k];
+                        const double bd = (double)x[j + kLeStep * kLeDim] - bi;
+                        int t = 0;
+#pragma unroll
+                        for (int rr = 0; rr < kLeDim; rr++)
+#pragma unroll
+                            for (int c = rr; c < kLeDim; c++, t++) sums[t] = fma(xd[rr], xd[c], sums[t]);
+#pragma unroll
+                        for (int k = 0; k < kLeDim; k++) sums[10 + k] = fma(xd[k], bd, sums[10 + k]);
+                    }
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < kLeRec; k++) rec[lane * (kLeRec + 1) + k] = sums[k];
+            rec[lane * (kLeRec + 1) + kLeRec] = ok ? 1.0 : 0.0;
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            // the QR chain over this chunk's points, in order (every lane alike)
+            const int cnt = no - i0 < 64 ? no - i0 : 64;
+            for (int q = 0; q < cnt; q++) {
+                const double *rq = rec + q * (kLeRec + 1);
+                if (rq[kLeRec] == 0.0) continue;
+                double co[4];
+                solve4(rq, rq + 10, 1e-10, co);
+                // M = T Q: rows 0..2 of T shift (row r picks Q's row r + 1), row 3 is co
+                double Mt[4][4];
+#pragma unroll
+                for (int c = 0; c < 4; c++) {
+#pragma unroll
+                    for (int r = 0; r < 3; r++) Mt[r][c] = Q[r + 1][c];
+                    Mt[3][c] = fma(co[0], Q[0][c], fma(co[1], Q[1][c], fma(co[2], Q[2][c], co[3] * Q[3][c])));
+                }
+                // Householder QR of Mt: R in Mt's upper triangle, Q accumulated
+                double Qn[4][4] = {{1, 0, 0, 0}, {0, 1, 0, 0}, {0, 0, 1, 0}, {0, 0, 0, 1}};
+#pragma unroll
+                for (int k = 0; k < 3; k++) {
+                    double nrm = 0.0;
+#pragma unroll
+                    for (int r = k; r < 4; r++) nrm = fma(Mt[r][k], Mt[r][k], nrm);
+                    nrm = sqrt(nrm);
+                    if (nrm == 0.0) continue;
+                    const double alpha = Mt[k][k] > 0.0 ? -nrm : nrm;
+                    double v[4] = {0, 0, 0, 0};
+#pragma unroll
+                    for (int r = k; r < 4; r++) v[r] = Mt[r][k];
+                    v[k] -= alpha;
+                    double vv = 0.0;
+#pragma unroll
+                    for (int r = k; r < 4; r++) vv = fma(v[r], v[r], vv);
+                    if (vv == 0.0) continue;
+#pragma unroll
+                    for (int c = 0; c < 4; c++) {  // Mt = (I - 2 v v^T / vv) Mt
+                        double dot = 0.0;
+#pragma unroll
+                        for (int r = k; r < 4; r++) dot = fma(v[r], Mt[r][c], dot);
+                        const double f = 2.0 * dot / vv;
+#pragma unroll
+                        for (int r = k; r < 4; r++) Mt[r][c] = fma(-f, v[r], Mt[r][c]);
+                    }
+#pragma unroll
+                    for (int r = 0; r < 4; r++) {  // Qn = Qn (I - 2 v v^T / vv)
+                        double dot = 0.0;
+#pragma unroll
+                        for (int c = k; c < 4; c++) dot = fma(Qn[r][c], v[c], dot);
+                        const double f = 2.0 * dot / vv;
+#pragma unroll
+                        for (int c = k; c < 4; c++) Qn[r][c] = fma(-f, v[c], Qn[r][c]);
+                    }
+                }
+                // positive diagonal: flip the signs of R's rows and Q's columns
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    const double sg = Mt[k][k] < 0.0 ? -1.0 : 1.0;
+                    const double dg = sg * Mt[k][k];
+#pragma unroll
+                    for (int r = 0; r < 4; r++) Q[r][k] = sg * Qn[r][k];
+                    if (dg > 0.0) {
+                        lexp[k] += log(dg);
+                        lcnt[k] += 1.0;
+                    }
+                }
+            }
+            __builtin_amdgcn_wave_barrier();  // the records are rewritten by the next chunk
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        }
+        if (lane < kLeDim) {
+            double v = 0.0;
+#pragma unroll
+            for (int k = 0; k < kLeDim; k++)
+                if (k == lane) v = lcnt[k] > 0.0 ? lexp[k] / lcnt[k] / (double)kLeStep : INFINITY;
+            out[lane * np + p] = v;
+        }
+    }
+}
+
 // ---- Lempel-Ziv phrase counts ----------------------------------------------
 
 constexpr int kLzMaxBins = 8;  // bins settings per launch

@@ -24,7 +24,7 @@ from typing import Any
 import torch
 
 from .solvers.hip_dopri5 import FUSED_RESULTS, FusedResult, steady_index
-from .window_features import COMPREHENSIVE_FEATURES, WindowBlocks, check_params
+from .window_features import COMPREHENSIVE_FEATURES, MULTI_OUTPUT, WindowBlocks, check_params
 
 DEFAULT_STEADY_FRACTION = 0.85
 
@@ -338,8 +338,10 @@ class StatFeatureExtractor(FeatureExtractor):
     def _columns(self, plan, stats: torch.Tensor, y: torch.Tensor | None,
                  start: int) -> torch.Tensor:
         cols = []
+        names = []
         wb = None
         for s, name, params in plan:
+            base = feature_name(s, name, params)
             if is_comprehensive(name, params):
                 if y is None:
                     raise ValueError(f"{name} needs the saved window: integrate with "
@@ -347,7 +349,13 @@ class StatFeatureExtractor(FeatureExtractor):
                 if wb is None:
                     wb = WindowBlocks(y, start, plan,
                                       median=lambda: window_feature("median", None, y, start))
-                cols.append(wb.feature(name, params)[:, s].to(stats.device, torch.float64))
+                v = wb.feature(name, params)
+                if name in MULTI_OUTPUT:  # one column per output, named __0, __1, ..
+                    for k in range(MULTI_OUTPUT[name]):
+                        cols.append(v[k][:, s].to(stats.device, torch.float64))
+                        names.append(f"{base}__{k}")
+                    continue
+                cols.append(v[:, s].to(stats.device, torch.float64))
             elif name in WINDOW_FEATURES:
                 if y is None:
                     raise ValueError(f"{name} needs the saved window: integrate with "
@@ -364,7 +372,8 @@ class StatFeatureExtractor(FeatureExtractor):
                                      "trend=True (or keep_trajectories=True)")
             else:
                 cols.append(feature_from_stats(name, stats)[:, s])
-        self._names = [feature_name(s, name, params) for s, name, params in plan]
+            names.append(base)
+        self._names = names
         feats = torch.stack(cols, dim=1) if cols else stats.new_zeros((stats.shape[0], 0))
         return impute_extreme(feats.to(torch.float32))
 

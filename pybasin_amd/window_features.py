@@ -137,7 +137,10 @@ _BLOCK_PEAKS = (1, 3, 5)
 EMBED_FEATURES: dict[str, list[dict]] = {
     "correlation_dimension": [{"emb_dim": 4, "lag": 1, "n_rvals": 50}],
     "lyapunov_r": [{"emb_dim": 10, "lag": 1, "trajectory_len": 20, "tau": 1.0}],
+    "lyapunov_e": [{"emb_dim": 10, "matrix_dim": 4, "min_nb": 8, "min_tsep": 0, "tau": 1.0}],
 }
+# calculators giving several columns (the reference appends __0, __1, ..)
+MULTI_OUTPUT = {"lyapunov_e": 4}
 COMPREHENSIVE_FEATURES.update(EMBED_FEATURES)
 # phrase counts (bh_window_lempel_ziv)
 LZ_FEATURES: dict[str, list[dict]] = {
@@ -192,6 +195,12 @@ def check_params(name: str, params: dict | None) -> None:
         if (params.get("emb_dim", 4), params.get("lag", 1), params.get("n_rvals", 50)) == (4, 1, 50):
             return
         raise ValueError(f"correlation_dimension: emb_dim 4, lag 1, n_rvals 50 are built, got {params}")
+    if name == "lyapunov_e":
+        if (params.get("emb_dim", 10), params.get("matrix_dim", 4), params.get("min_nb", 8),
+                params.get("min_tsep", 0)) == (10, 4, 8, 0) and float(params.get("tau", 1.0)) > 0:
+            return
+        raise ValueError("lyapunov_e: emb_dim 10, matrix_dim 4, min_nb 8, min_tsep 0 are built, "
+                         f"got {params}")
     if name == "lyapunov_r":
         if (params.get("emb_dim", 10), params.get("lag", 1), params.get("trajectory_len", 20)) == \
                 (10, 1, 20) and float(params.get("tau", 1.0)) > 0:
@@ -1138,22 +1147,68 @@ def _lyap_r_cpu(x: torch.Tensor) -> float:
     return (nf * float((kk * dd).sum()) - float(kk.sum()) * float(dd.sum())) / den
 
 
+def _lyap_e_cpu(x: torch.Tensor) -> list:
+    """Eckmann's four exponents of one float64 series (emb 10, matrix 4,
+    min_nb 8, min_tsep 0; not divided by tau)."""
+    n = x.shape[0]
+    if not bool(torch.isfinite(x).all()):
+        return [math.inf] * 4
+    no = n - 3 - 9
+    q = torch.eye(4, dtype=torch.float64)
+    lexp = torch.zeros(4, dtype=torch.float64)
+    cnt = torch.zeros(4, dtype=torch.float64)
+    if no >= 1:
+        orb = torch.stack([x[e: e + no] for e in range(10)], 1)
+        stp = torch.stack([x[3 * k: 3 * k + no] for k in range(4)], 1)
+        beta = x[12: 12 + no]
+        for i in range(no):
+            d = (orb - orb[i]).abs().max(1).values
+            d[i] = math.inf
+            r = float(torch.sort(d).values[7]) if no > 8 else math.inf
+            if math.isinf(r):
+                continue
+            nbm = d <= r
+            xd = stp[nbm] - stp[i]
+            bd = beta[nbm] - beta[i]
+            a = torch.linalg.solve(xd.T @ xd + 1e-10 * torch.eye(4, dtype=torch.float64), xd.T @ bd)
+            t = torch.zeros((4, 4), dtype=torch.float64)
+            t[:3, 1:] = torch.eye(3, dtype=torch.float64)
+            t[3] = a
+            mq, mr = torch.linalg.qr(t @ q)
+            sg = torch.sign(torch.diag(mr))
+            sg = torch.where(sg == 0, torch.ones_like(sg), sg)
+            q = mq * sg
+            dg = torch.diag(mr) * sg
+            pos = dg > 0
+            lexp[pos] += torch.log(dg[pos])
+            cnt[pos] += 1
+    return [float(lexp[k] / cnt[k] / 3) if cnt[k] > 0 else math.inf for k in range(4)]
+
+
 def window_embedding(y: torch.Tensor, start: int, which: str) -> torch.Tensor:
-    """(B, S) float64 correlation_dimension ("corr_dim") or Rosenstein slope
-    ("lyap_r", not divided by tau) of the window rows [start, N) of y
-    (bh_window_embedding on the device, torch per series on the CPU)."""
+    """Delay-embedding estimates of the window rows [start, N) of y: (B, S)
+    float64 correlation_dimension ("corr_dim") or Rosenstein slope
+    ("lyap_r"), or (4, B, S) Eckmann exponents ("lyap_e"), the Lyapunov
+    estimates not divided by tau (bh_window_embedding on the device, torch
+    per series on the CPU)."""
     N, B, S = y.shape
     if not 0 <= start < N:
         raise ValueError("empty feature window")
     if y.device.type != "cuda":
-        fn = _corr_dim_cpu if which == "corr_dim" else _lyap_r_cpu
         w = y[start:].double()
+        if which == "lyap_e":
+            v = torch.tensor([[_lyap_e_cpu(w[:, b, s]) for s in range(S)] for b in range(B)],
+                             dtype=torch.float64)  # (B, S, 4)
+            return v.permute(2, 0, 1).contiguous()
+        fn = _corr_dim_cpu if which == "corr_dim" else _lyap_r_cpu
         return torch.tensor([[fn(w[:, b, s]) for s in range(S)] for b in range(B)],
                             dtype=torch.float64).reshape(B, S)
     y = y.contiguous()
-    out = torch.empty((B, S), dtype=torch.float64, device=y.device)
+    shape = (4, B, S) if which == "lyap_e" else (B, S)
+    out = torch.empty(shape, dtype=torch.float64, device=y.device)
     dt = _lib.BH_F32 if y.dtype == torch.float32 else _lib.BH_F64
-    code = _lib.BH_EMB_CORR_DIM if which == "corr_dim" else _lib.BH_EMB_LYAP_R
+    code = {"corr_dim": _lib.BH_EMB_CORR_DIM, "lyap_r": _lib.BH_EMB_LYAP_R,
+            "lyap_e": _lib.BH_EMB_LYAP_E}[which]
     s = torch.cuda.current_stream(y.device)
     with torch.cuda.device(y.device):
         _lib.check(_lib.lib().bh_window_embedding(dt, y.data_ptr(), N, B, S, int(start), code,
@@ -1323,6 +1378,9 @@ class WindowBlocks:
         if name == "lyapunov_r":  # torch_features_dynamical.py:23-42: the slope over tau
             slope = self._get("lyap_r", lambda: window_embedding(self.y, self.start, "lyap_r"))
             return slope / float((params or {}).get("tau", 1.0))
+        if name == "lyapunov_e":  # torch_features_dynamical.py:45-66: (4, B, S), over tau
+            ex = self._get("lyap_e", lambda: window_embedding(self.y, self.start, "lyap_e"))
+            return ex / float((params or {}).get("tau", 1.0))
         if feature_family(name, params) == "peaks":
             perm = self.perm_entropy() if name == "permutation_entropy" else None
             counts = self.peak_counts() if name != "permutation_entropy" else {}

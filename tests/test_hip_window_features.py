@@ -732,3 +732,47 @@ def test_window_lempel_ziv_kernel_equals_oracle(cuda_device, dtype):
                             assert g == pytest.approx(want, rel=1e-12), (tag, nb, b, s, g, want)
     with pytest.raises(ValueError, match="1,024 rows"):
         window_lempel_ziv(torch.zeros(1025, 2, 1, dtype=dtype, device="cuda"), 0, [2])
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_window_lyapunov_e_kernel_equals_oracle(cuda_device, dtype):
+    """Eckmann's four exponents (neighbours within the 8th smallest Chebyshev
+    distance, least-squares companion rows, the Householder QR chain with
+    positive diagonals) of noisy sinusoids, noise, the logistic map, short,
+    constant and non-finite windows against the oracle; the CPU path agrees;
+    the extractor's four columns equal the kernel's over tau."""
+    from pybasin_amd.window_features import window_embedding
+
+    rng = np.random.default_rng(59)
+    n = 220
+    logistic = np.empty(n)
+    logistic[0] = 0.3
+    for i in range(1, n):
+        logistic[i] = 4 * logistic[i - 1] * (1 - logistic[i - 1])
+    w = np.concatenate([np.sin(np.arange(n)[:, None, None] * rng.uniform(0.05, 0.5, size=(1, 4, 2)))
+                        + 0.05 * rng.normal(size=(n, 4, 2)), rng.normal(size=(n, 2, 2)),
+                        np.stack([logistic, logistic[::-1]], 1)[:, None, :],
+                        np.full((n, 1, 2), 0.5)], 1)
+    w[30, 5, 0] = np.inf
+    for tag, win in (("mixed", w), ("short", rng.normal(size=(20, 3, 2))), ("tiny", rng.normal(size=(12, 2, 1)))):
+        wt = torch.tensor(win, dtype=dtype)
+        x = wt.double().numpy()
+        got = window_embedding(wt.cuda(), 0, "lyap_e").cpu().numpy()
+        for b in range(win.shape[1]):
+            for s in range(win.shape[2]):
+                want = np.array(O.lyapunov_e(x[:, b, s]))
+                g = got[:, b, s]
+                fin = np.isfinite(want)
+                assert np.array_equal(np.isfinite(g), fin), (tag, b, s, g, want)
+                np.testing.assert_allclose(g[fin], want[fin], rtol=1e-7, atol=1e-9, err_msg=f"{tag} {b} {s}")
+        cpu = window_embedding(wt, 0, "lyap_e").numpy()
+        np.testing.assert_allclose(np.nan_to_num(got, posinf=7.0), np.nan_to_num(cpu, posinf=7.0),
+                                   rtol=1e-7, atol=1e-9)
+    t = torch.linspace(0, 10, n, dtype=torch.float64)
+    y = torch.tensor(w[:, :3], dtype=dtype).cuda()
+    fx = StatFeatureExtractor(time_steady=0.0, features={
+        "lyapunov_e": [{"emb_dim": 10, "matrix_dim": 4, "min_nb": 8, "min_tsep": 0, "tau": 0.5}]})
+    f = fx.extract_features(Solution(y[0], t.cuda(), y)).cpu().numpy()
+    assert fx.feature_names[3] == "state_0__lyapunov_e__emb_dim_10__matrix_dim_4__min_nb_8__min_tsep_0__tau_0.5__3"
+    ex = window_embedding(y, 0, "lyap_e").cpu().numpy() / 0.5
+    np.testing.assert_array_equal(f[:, :4], ex[:, :, 0].T.astype(np.float32))

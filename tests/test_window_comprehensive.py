@@ -591,6 +591,23 @@ def test_oracle_embedding_pins():
     assert math.isnan(C(np.arange(8.0), "lyapunov_r", {"tau": 1.0}))  # no embedded vector
 
 
+def test_oracle_lyapunov_e_pins():
+    """Eckmann's exponents: the largest is clearly positive for the logistic
+    map (ln 2 per step), all are at most about 0 for a sinusoid; a
+    non-finite series gives inf, as the reference; tau divides."""
+    logistic = np.empty(500)
+    logistic[0] = 0.3
+    for i in range(1, 500):
+        logistic[i] = 4 * logistic[i - 1] * (1 - logistic[i - 1])
+    ex = O.lyapunov_e(logistic)
+    assert max(ex) > 0.2
+    sine = np.sin(np.arange(400) * 0.3) + 1e-3 * np.random.default_rng(2).normal(size=400)
+    assert max(O.lyapunov_e(sine)) < 0.1
+    assert O.lyapunov_e(np.r_[logistic[:50], np.nan]) == [math.inf] * 4
+    assert O.lyapunov_e(logistic, tau=2.0) == pytest.approx([v / 2 for v in ex])
+    assert O.lyapunov_e(np.arange(12.0)) == [math.inf] * 4  # no orbit point
+
+
 def test_cpu_embedding_path_equals_oracle():
     from pybasin_amd.window_features import window_embedding
 
@@ -605,6 +622,9 @@ def test_cpu_embedding_path_equals_oracle():
                                          rel=1e-9)
         assert lr[b, 0] == pytest.approx(O.comprehensive(w[:, b, 0], "lyapunov_r", {"tau": 1.0}),
                                          rel=1e-9, abs=1e-12)
+    le = window_embedding(wt[:90], 0, "lyap_e").numpy()
+    for b in range(4):
+        np.testing.assert_allclose(le[:, b, 0], O.lyapunov_e(w[:90, b, 0]), rtol=1e-8, atol=1e-10)
 
 
 def test_oracle_lempel_ziv_known_answers():
@@ -755,11 +775,13 @@ def test_every_comprehensive_calculator_through_the_extractor():
     sol = Solution(torch.zeros(3, 2), t, y)
     fx = StatFeatureExtractor(time_steady=0.0, features=COMPREHENSIVE_FEATURES)
     f = fx.extract_features(sol)
-    n_cols = sum(len(g or [None]) for g in COMPREHENSIVE_FEATURES.values())
+    n_cols = sum(len(g or [None]) for g in COMPREHENSIVE_FEATURES.values()) + 3  # lyapunov_e: 4
     assert f.shape == (3, 2 * n_cols)
     assert torch.isfinite(f).all()
     assert len(set(fx.feature_names)) == f.shape[1]
     assert "state_1__cwt_coefficients__coeff_3__w_5__widths_(5,)" in fx.feature_names
     assert "state_0__change_quantiles__f_agg_var__isabs_True__qh_0.8__ql_0.2" in fx.feature_names
+    assert "state_1__lyapunov_e__emb_dim_10__matrix_dim_4__min_nb_8__min_tsep_0__tau_1.0__3" in \
+        fx.feature_names
     wb = WindowBlocks(y, 0, [(0, n, p) for n, g in COMPREHENSIVE_FEATURES.items() for p in (g or [None])])
     assert wb.feature("number_peaks", {"n": 50}).shape == (3, 2)
