@@ -1524,15 +1524,22 @@ __global__ void __launch_bounds__(kSeriesBlock)
                 double v[kLrEmb + 7];
 #pragma unroll
                 for (int u = 0; u < kLrEmb + 7; u++) v[u] = j0 + u < w ? (double)x[j0 + u] : 0.0;
+                double dd[8];
+#pragma unroll
+                for (int u = 0; u < 8; u++) {  // independent chains, two partial sums each
+                    double s0 = 0.0, s1 = 0.0;
+#pragma unroll
+                    for (int e = 0; e < kLrEmb; e += 2) {
+                        const double d0 = a[e] - v[u + e], d1 = a[e + 1] - v[u + e + 1];
+                        s0 = fma(d0, d0, s0);
+                        s1 = fma(d1, d1, s1);
+                    }
+                    dd[u] = s0 + s1;
+                }
 #pragma unroll
                 for (int u = 0; u < 8; u++) {
                     const int j = j0 + u;
-                    double d2 = 0.0;
-#pragma unroll
-                    for (int e = 0; e < kLrEmb; e++) {
-                        const double df = a[e] - v[u + e];
-                        d2 = fma(df, df, d2);
-                    }
+                    const double d2 = dd[u];
                     const bool far = (unsigned)(j - ii + ts) > (unsigned)(2 * ts);
                     if (j < nt && far && d2 < best) {
                         best = d2;
@@ -1685,24 +1692,51 @@ __global__ void __launch_bounds__(kSeriesBlock)
                 double a[kLeEmb];
 #pragma unroll
                 for (int e = 0; e < kLeEmb; e++) a[e] = (double)x[i + e];
-                // the eight smallest Chebyshev distances (ascending)
-                double top[kLeNb];
+                // the nine smallest Chebyshev distances (ascending) and their
+                // points: when the ninth lies above the eighth, the neighbours
+                // (distance <= the eighth) are exactly the first eight;
+                // otherwise (a tie at the eighth) a second pass collects them
+                double top[kLeNb + 1];
+                int tj[kLeNb + 1];
 #pragma unroll
-                for (int k = 0; k < kLeNb; k++) top[k] = INFINITY;
-                for (int j = 0; j < no; j++) {
-                    if (j == i) continue;
-                    double d = 0.0;
+                for (int k = 0; k <= kLeNb; k++) {
+                    top[k] = INFINITY;
+                    tj[k] = -1;
+                }
+                // eight points j at a time: their 17 samples loaded together
+                // (LDS broadcasts), the distances from registers
+                for (int j0 = 0; j0 < no; j0 += 8) {
+                  double xv[kLeEmb + 7];
 #pragma unroll
-                    for (int e = 0; e < kLeEmb; e++) d = fmax(d, fabs(a[e] - (double)x[j + e]));
-                    if (d < top[kLeNb - 1]) {
+                  for (int u = 0; u < kLeEmb + 7; u++) xv[u] = j0 + u < w ? (double)x[j0 + u] : 0.0;
+                  double dd[8];
+#pragma unroll
+                  for (int u = 0; u < 8; u++) {  // independent chains, a balanced max tree
+                    double t[kLeEmb];
+#pragma unroll
+                    for (int e = 0; e < kLeEmb; e++) t[e] = fabs(a[e] - xv[u + e]);
+                    dd[u] = fmax(fmax(fmax(fmax(t[0], t[1]), fmax(t[2], t[3])),
+                                      fmax(fmax(t[4], t[5]), fmax(t[6], t[7]))), fmax(t[8], t[9]));
+                  }
+#pragma unroll
+                  for (int u = 0; u < 8; u++) {
+                    const int j = j0 + u;
+                    const double d = dd[u];
+                    if (j < no && j != i && d < top[kLeNb]) {
                         double v = d;
+                        int vj = j;
 #pragma unroll
-                        for (int k = 0; k < kLeNb; k++) {
-                            const double lo = fmin(top[k], v), hi = fmax(top[k], v);
-                            top[k] = lo;
-                            v = hi;
+                        for (int k = 0; k <= kLeNb; k++) {
+                            const bool sw = v < top[k];
+                            const double hv = sw ? top[k] : v;
+                            const int hj = sw ? tj[k] : vj;
+                            top[k] = sw ? v : top[k];
+                            tj[k] = sw ? vj : tj[k];
+                            v = hv;
+                            vj = hj;
                         }
                     }
+                  }
                 }
                 const double r = top[kLeNb - 1];
                 if (r < INFINITY) {
@@ -1711,12 +1745,7 @@ __global__ void __launch_bounds__(kSeriesBlock)
 #pragma unroll
                     for (int k = 0; k < kLeDim; k++) xi[k] = (double)x[i + kLeStep * k];
                     const double bi = (double)x[i + kLeStep * kLeDim];
-                    for (int j = 0; j < no; j++) {
-                        if (j == i) continue;
-                        double d = 0.0;
-#pragma unroll
-                        for (int e = 0; e < kLeEmb; e++) d = fmax(d, fabs(a[e] - (double)x[j + e]));
-                        if (!(d <= r)) continue;
+                    auto add = [&](int j) {
                         double xd[kLeDim];
 #pragma unroll
                         for (int k = 0; k < kLeDim; k++) xd[k] = (double)x[j + kLeStep * k] - xi[k];
@@ -1728,6 +1757,18 @@ __global__ void __launch_bounds__(kSeriesBlock)
                             for (int c = rr; c < kLeDim; c++, t++) sums[t] = fma(xd[rr], xd[c], sums[t]);
 #pragma unroll
                         for (int k = 0; k < kLeDim; k++) sums[10 + k] = fma(xd[k], bd, sums[10 + k]);
+                    };
+                    if (top[kLeNb] > r) {
+#pragma unroll
+                        for (int k = 0; k < kLeNb; k++) add(tj[k]);
+                    } else {
+                        for (int j = 0; j < no; j++) {
+                            if (j == i) continue;
+                            double d = 0.0;
+#pragma unroll
+                            for (int e = 0; e < kLeEmb; e++) d = fmax(d, fabs(a[e] - (double)x[j + e]));
+                            if (d <= r) add(j);
+                        }
                     }
                 }
             }
