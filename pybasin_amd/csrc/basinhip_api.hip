@@ -1097,8 +1097,10 @@ int32_t bh_window_lempel_ziv(int32_t dtype, const void *traj, int64_t n_save, in
     const int64_t elem = dtype == BH_F32 ? 4 : 8;
     int cap_log = 6;
     while ((1LL << cap_log) < 2 * w) cap_log++;
-    // one hash table of 2^cap_log 32-bit entries per bins setting, then the series rows
-    const int64_t budget = 62 * 1024, tables = (int64_t)n_bins * (4LL << cap_log);
+    // one hash table of 2^cap_log 32-bit entries and one row of w one-byte
+    // symbols per bins setting, then the series rows
+    const int64_t budget = 62 * 1024,
+                  tables = (int64_t)n_bins * (4LL << cap_log) + ((int64_t)n_bins * w + 15) / 16 * 16;
     if (tables + w * elem > budget)
         return fail(-5, "bh_window_lempel_ziv: %d settings of a %lld-row window exceed the LDS budget",
                     n_bins, (long long)w);

@@ -1883,6 +1883,7 @@ __global__ void __launch_bounds__(64)
     unsigned *tabs = reinterpret_cast<unsigned *>(smem + ((int64_t)G * w * sizeof(T) + 15) / 16 * 16);
     const int lane = threadIdx.x;
     const unsigned cap = 1u << cap_log;
+    unsigned char *syms = reinterpret_cast<unsigned char *>(tabs + (int64_t)nb * cap);
     load_series<T>(traj, n, S, start, w, p0, G, np, lds);
     __syncthreads();
     const int binsv[kLzMaxBins] = {bins_a.x, bins_a.y, bins_a.z, bins_a.w,
@@ -1892,61 +1893,88 @@ __global__ void __launch_bounds__(64)
     for (int k = 0; k < kLzMaxBins; k++)
         if (k == lane) bins = binsv[k];
     unsigned *tab = tabs + (lane < nb ? lane : 0) * cap;
+    const unsigned char *sy = syms + (int64_t)(lane < nb ? lane : 0) * w;
     const double bn = w > 1 ? (double)w / (log((double)w) + 1e-10) : 1.0;
     for (int g = 0; g < G; g++) {
         const int64_t p = p0 + g;
         if (p >= np) break;  // block-uniform
         const T *x = lds + (int64_t)g * w;
-        T mn = x[0], mx = x[0];
+        // minimum, maximum and finiteness by the whole wave
+        double mnd = INFINITY, mxd = -INFINITY;
         bool bad = false;
-        for (int64_t i = 0; i < w; i++) {  // every lane the same (broadcast reads)
-            const T v = x[i];
-            bad |= !isfinite((double)v);
-            mn = v < mn ? v : mn;
-            mx = v > mx ? v : mx;
+        for (int64_t i = lane; i < w; i += 64) {
+            const double v = (double)x[i];
+            bad |= !isfinite(v);
+            mnd = fmin(mnd, v);
+            mxd = fmax(mxd, v);
         }
-        if (lane >= nb) continue;
-        if (bad) {
-            out[lane * np + p] = NAN;
+#pragma unroll
+        for (int m = 32; m >= 1; m >>= 1) {
+            mnd = fmin(mnd, __shfl_xor(mnd, m, 64));
+            mxd = fmax(mxd, __shfl_xor(mxd, m, 64));
+        }
+        if (__any(bad)) {
+            if (lane < nb) out[lane * np + p] = NAN;
             continue;
         }
-        for (unsigned k = 0; k < cap; k++) tab[k] = 0u;
-        const T rng = mx - mn + (T)1e-10;
-        int count = 0, nodes = 1;  // node 0 is the root
-        int64_t i = 0;
-        while (i < w) {
-            int cur = 0;
-            int64_t len = 1;
-            bool added = false;
-            while (i + len <= w) {
-                int sym = (int)((x[i + len - 1] - mn) / rng * (T)bins);
-                sym = sym < 0 ? 0 : (sym > bins - 1 ? bins - 1 : sym);
-                const unsigned key = (unsigned)cur * (unsigned)bins + (unsigned)sym + 1u;
-                unsigned h = (key * 2654435761u) >> (32 - cap_log);
-                int child = -1;
-                for (;;) {
-                    const unsigned e = tab[h];
-                    if (e == 0u) break;
-                    if ((e >> 12) == key) {
-                        child = (int)(e & 0xfffu);
+        // the symbols of every setting (one byte each), by the whole wave,
+        // and the hash tables cleared
+        const T mn = (T)mnd, rng = (T)mxd - mn + (T)1e-10;
+        for (int64_t i = lane; i < w; i += 64) {
+            const T v = x[i] - mn;
+#pragma unroll
+            for (int k = 0; k < kLzMaxBins; k++) {
+                if (k >= nb) break;
+                const int bk = binsv[k];
+                int sb = (int)(v / rng * (T)bk);
+                sb = sb < 0 ? 0 : (sb > bk - 1 ? bk - 1 : sb);
+                syms[(int64_t)k * w + i] = (unsigned char)sb;
+            }
+        }
+        for (int64_t t = lane; t < (int64_t)nb * cap; t += 64) tabs[t] = 0u;
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        if (lane < nb) {
+            int count = 0, nodes = 1;  // node 0 is the root
+            int64_t i = 0;
+            while (i < w) {
+                int cur = 0;
+                int64_t len = 1;
+                bool added = false;
+                unsigned sym = sy[i];
+                while (i + len <= w) {
+                    // the next symbol is read while this one's probe runs
+                    const unsigned nsym = i + len < w ? sy[i + len] : 0u;
+                    const unsigned key = (unsigned)cur * (unsigned)bins + sym + 1u;
+                    unsigned h = (key * 2654435761u) >> (32 - cap_log);
+                    int child = -1;
+                    for (;;) {
+                        const unsigned e = tab[h];
+                        if (e == 0u) break;
+                        if ((e >> 12) == key) {
+                            child = (int)(e & 0xfffu);
+                            break;
+                        }
+                        h = (h + 1u) & (cap - 1u);
+                    }
+                    if (child < 0) {  // a new phrase: add it as cur's child
+                        tab[h] = (key << 12) | (unsigned)nodes;
+                        nodes++;
+                        count++;
+                        added = true;
                         break;
                     }
-                    h = (h + 1u) & (cap - 1u);
+                    cur = child;
+                    len++;
+                    sym = nsym;
                 }
-                if (child < 0) {  // a new phrase: add it as cur's child
-                    tab[h] = (key << 12) | (unsigned)nodes;
-                    nodes++;
-                    count++;
-                    added = true;
-                    break;
-                }
-                cur = child;
-                len++;
+                i += len;
+                if (!added) break;
             }
-            i += len;
-            if (!added) break;
+            out[lane * np + p] = (double)count / bn;
         }
-        out[lane * np + p] = (double)count / bn;
+        __builtin_amdgcn_wave_barrier();  // symbols and tables are rebuilt for the next series
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     }
 }
 
