@@ -114,3 +114,40 @@ def test_rossler_edges_entry_checks_the_graph_before_device_work():
     assert call(hub_i, hub_j) == -5
     assert b"more than 12" in L.bh_last_error()
     assert call(ei[:0], ej[:0]) == 0  # no edges: uncoupled nodes
+
+
+def test_window_entries_check_arguments_before_device_work():
+    """Every window-feature entry point refuses bad arguments with -5 and a
+    message before any HIP call (the pointers below are never dereferenced)."""
+    L = _lib.lib()
+    fake, F32, F64 = 4096, _lib.BH_F32, _lib.BH_F64
+
+    def refused(rc, word):
+        assert rc == -5, rc
+        assert word in L.bh_last_error(), L.bh_last_error()
+
+    # common checks: empty window, dtype, null trajectory
+    refused(L.bh_window_features(F32, fake, 10, 4, 2, 10, fake, None), b"empty window")
+    refused(L.bh_window_spectrum(7, fake, 10, 4, 2, 0, 5, fake, None, None), b"dtype")
+    refused(L.bh_window_autocov(F32, None, 10, 4, 2, 0, 3, fake, None), b"null trajectory")
+    # per-entry limits
+    refused(L.bh_window_spectrum(F32, fake, 10, 4, 2, 0, 5000, fake, None, None), b"n_coeff")
+    refused(L.bh_window_spectrum(F32, fake, 10, 4, 2, 0, 0, None, None, None), b"nothing")
+    refused(L.bh_window_autocov(F32, fake, 10, 4, 2, 0, 5000, fake, None), b"max_lag")
+    refused(L.bh_window_sorted(F64, fake, 1025, 4, 2, 0, fake, None, None, None), b"LDS budget")
+    refused(L.bh_window_sorted(F32, fake, 100, 4, 2, 0, None, None, None, None), b"out_dup")
+    refused(L.bh_window_change_quantiles(F32, fake, 100, 4, 2, 0, None, fake, None), b"levels")
+    refused(L.bh_window_trends(F32, fake, 100, 4, 2, 0, 1, None, None), b"null out")
+    refused(L.bh_window_peak_counts(F32, fake, 100, 4, 2, 0, 10, 65, 0, 0, fake, None), b"supports")
+    refused(L.bh_window_perm_entropy(F32, fake, 4053, 4, 2, 0, fake, None), b"LDS budget")
+    refused(L.bh_window_embedding(F32, fake, 100, 4, 2, 0, 9, fake, None), b"which")
+    bins = (ctypes.c_int32 * 2)(2, 300)
+    refused(L.bh_window_lempel_ziv(F32, fake, 100, 4, 2, 0, ctypes.cast(bins, ctypes.c_void_p), 2,
+                                   fake, None), b"bins")
+    refused(L.bh_window_lempel_ziv(F32, fake, 100, 4, 2, 0, ctypes.cast(bins, ctypes.c_void_p), 9,
+                                   fake, None), b"n_bins")
+    ok_bins = (ctypes.c_int32 * 1)(2)
+    refused(L.bh_window_lempel_ziv(F32, fake, 1025, 4, 2, 0, ctypes.cast(ok_bins, ctypes.c_void_p), 1,
+                                   fake, None), b"1,024 rows")
+    # n = 0: nothing to do, success without device work
+    assert L.bh_window_trends(F32, fake, 100, 0, 2, 0, 1, fake, None) == 0
