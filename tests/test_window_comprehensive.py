@@ -608,6 +608,82 @@ def test_oracle_lyapunov_e_pins():
     assert O.lyapunov_e(np.arange(12.0)) == [math.inf] * 4  # no orbit point
 
 
+def _rosenstein_by_scipy(x):
+    """Rosenstein's slope from other library routines: the mean frequency
+    from scipy's periodogram of the (2n - 1)-point zero-padded series (the
+    one-sided doubling cancels in the ratio: the length is odd), the nearest
+    neighbours from a k-d tree queried for enough candidates to clear the
+    temporal band, the divergence slope by numpy.polyfit."""
+    from scipy.signal import periodogram
+    from scipy.spatial import cKDTree
+
+    n = x.size
+    f, P = periodogram(x, nfft=2 * n - 1, detrend=False, window="boxcar")
+    mf = (f[1:] * P[1:]).sum() / P[1:].sum()
+    tsep = min(max(int(np.ceil(1.0 / mf)), 1), int(0.25 * n))
+    m = n - 9
+    orb = np.stack([x[e: e + m] for e in range(10)], 1)
+    ntraj = m - 19
+    tree = cKDTree(orb[:ntraj])
+    dist, idx = tree.query(orb[:ntraj], k=min(ntraj, 2 * tsep + 40))
+    nb = np.empty(ntraj, dtype=int)
+    for i in range(ntraj):
+        ok = np.abs(idx[i] - i) > tsep
+        nb[i] = idx[i][ok][0]
+    div = [np.log(np.sqrt(((orb[np.arange(ntraj) + k] - orb[nb + k]) ** 2).sum(1))).mean()
+           for k in range(20)]
+    return np.polyfit(np.arange(20.0), div, 1)[0]
+
+
+def _eckmann_by_scipy(x):
+    """Eckmann's exponents from other library routines: neighbours within the
+    8th Chebyshev distance from a k-d tree (p = inf), the companion row from
+    the reference's ridge normal equations (+ 1e-10 I) by a Cholesky solve,
+    the QR steps by scipy.linalg.qr."""
+    from scipy.linalg import cho_factor, cho_solve, qr
+    from scipy.spatial import cKDTree
+
+    m, no = 3, x.size - 12
+    orb = np.stack([x[e: e + no] for e in range(10)], 1)
+    tree = cKDTree(orb)
+    q = np.eye(4)
+    lexp, cnt = np.zeros(4), np.zeros(4)
+    for i in range(no):
+        d, _ = tree.query(orb[i], k=9, p=np.inf)
+        r = d[8]  # the first is the point itself
+        nb = [j for j in tree.query_ball_point(orb[i], r, p=np.inf) if j != i]
+        X = np.stack([x[np.array(nb) + m * k] - x[i + m * k] for k in range(4)], 1)
+        b = x[np.array(nb) + 4 * m] - x[i + 4 * m]
+        a = cho_solve(cho_factor(X.T @ X + 1e-10 * np.eye(4)), X.T @ b)
+        t = np.zeros((4, 4))
+        t[:3, 1:] = np.eye(3)
+        t[3] = a
+        qq, rr = qr(t @ q)
+        sg = np.where(np.sign(np.diag(rr)) == 0, 1.0, np.sign(np.diag(rr)))
+        q = qq * sg
+        dg = np.diag(rr) * sg
+        lexp[dg > 0] += np.log(dg[dg > 0])
+        cnt[dg > 0] += 1
+    return lexp / cnt / m
+
+
+def test_oracle_lyapunov_estimates_match_library_routes():
+    """The oracle's Rosenstein and Eckmann estimates against the same methods
+    put together from scipy's periodogram, k-d trees, a Cholesky solve and QR
+    (no loop of the oracle retyped): a wrong temporal band, a neighbour off
+    by one, a stepped offset or a missing sign fix would show."""
+    rng = np.random.default_rng(16)
+    logistic = np.empty(420)
+    logistic[0] = 0.3
+    for i in range(1, 420):
+        logistic[i] = 4 * logistic[i - 1] * (1 - logistic[i - 1])
+    noisy_sine = np.sin(np.arange(400) * 0.23) + 0.1 * rng.normal(size=400)
+    for x in (logistic, noisy_sine, rng.normal(size=300)):
+        assert O.comprehensive(x, "lyapunov_r", {"tau": 1.0}) == pytest.approx(
+            _rosenstein_by_scipy(x), rel=1e-8, abs=1e-10)
+        np.testing.assert_allclose(O.lyapunov_e(x), _eckmann_by_scipy(x), rtol=1e-6, atol=1e-8)
+
+
 def test_cpu_embedding_path_equals_oracle():
     from pybasin_amd.window_features import window_embedding
 
