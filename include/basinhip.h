@@ -510,7 +510,8 @@ enum {
    (torch_features_dynamical.py:87-177: the cubic's normal equations + 1e-6 I
    solved with partial pivoting; the largest x_i where the fitted cubic
    changes sign to x_{i+1}, 0 if none); benford_correlation
-   (torch_features_advanced.py:21-78); with want_cwt, cwt_coefficients
+   (torch_features_advanced.py:21-78; the digit floor(|x| / 10^floor(log10
+   |x|)) in double, counted when 1..9); with want_cwt, cwt_coefficients
    (torch_features_frequency.py:71-122: Ricker widths 2, 5, 10, 20,
    coefficients 0..14, reflect padding, replicate for windows of at most 4 w
    rows, 0 past the window).  Non-finite windows give NaN. */

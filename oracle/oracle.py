@@ -696,10 +696,12 @@ def comprehensive(x: np.ndarray, name: str, params: dict | None = None, dtype=np
         return float(pos[:-1][chg].max()) if chg.any() else 0.0
     if name == "benford_correlation":  # torch_features_advanced.py:21-78
         nzv = np.abs(x[x != 0])
-        if nzv.size < 10:
-            return 0.0
-        dg = np.array([int(f"{v:.17e}"[0]) for v in nzv])  # the decimal expansion's first digit
+        # the reference's digit floor(|x| / 10^floor(log10 |x|)) (in double
+        # here), counted only when it is 1..9
+        dg = np.floor(nzv / 10.0 ** np.floor(np.log10(nzv))).astype(np.int64) if nzv.size else nzv
         cnt = np.array([(dg == d).sum() for d in range(1, 10)], dtype=np.float64)
+        if cnt.sum() < 10:
+            return 0.0
         fr = cnt / (cnt.sum() + 1e-10)
         bf = np.log10(1 + 1 / np.arange(1, 10))
         num = ((fr - fr.mean()) * (bf - bf.mean())).sum()

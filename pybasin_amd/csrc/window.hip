@@ -861,14 +861,28 @@ __device__ __forceinline__ void trend_write(const TrendSums &t, double *out, int
     }
 }
 
-// the leading decimal digit of a > 0 (1..9)
-__device__ __forceinline__ int leading_digit(double a) {
+// The reference's leading digit of a > 0 (torch_features_advanced.py:39-52):
+// floor(a / 10^floor(log10 a)), counted only when it is 1..9 (0 otherwise;
+// values within rounding of a power of ten can give 0 or 10 there), in
+// double.  The fast path takes the exponent from a float logarithm and
+// scales by a table of 10^-e (e in [-31, 32]); an exponent estimate within
+// 1e-4 of an integer, a scaled value within 1e-9 of a digit boundary or
+// outside [1, 10), or an a outside [1e-30, 1e30] takes the exact formula.
+__device__ __forceinline__ int leading_digit_exact(double a) {
     const double e = floor(log10(a));
-    double s = a / pow(10.0, e);
-    if (s >= 10.0) s *= 0.1;
-    if (s < 1.0) s *= 10.0;
-    int d = (int)s;
-    return d < 1 ? 1 : (d > 9 ? 9 : d);
+    const int d = (int)floor(a / pow(10.0, e));
+    return d >= 1 && d <= 9 ? d : 0;
+}
+
+__device__ __forceinline__ int leading_digit_fast(double a, const double *p10inv) {
+    if (!(a >= 1e-30 && a <= 1e30)) return leading_digit_exact(a);
+    const float lf = __log10f((float)a);
+    const float ef = floorf(lf);
+    if (lf - ef < 1e-4f || lf - ef > 1.0f - 1e-4f) return leading_digit_exact(a);
+    const double sc = a * p10inv[(int)ef + 31];
+    const double fr = sc - floor(sc);
+    if (!(sc >= 1.0 && sc < 10.0) || fr < 1e-9 || fr > 1.0 - 1e-9) return leading_digit_exact(a);
+    return (int)sc;
 }
 
 constexpr int kCwtWidths[4] = {2, 5, 10, 20};
@@ -889,10 +903,10 @@ constexpr int kCwtCoeffs = 15;
 //   x^0); pass 2 evaluates the cubic at every x_i and keeps the largest x_i
 //   whose value changes sign to the next one (row BH_TR_LANGEVIN, 0 if none);
 // * benford_correlation (torch_features_advanced.py:21-78): counts of the
-//   leading decimal digits of the nonzero |x_i| and their correlation with
-//   Benford's law as the reference forms it (unbiased deviations, divided by
-//   9 s_f s_b + 1e-10; 0 for fewer than 10 nonzero samples; row
-//   BH_TR_BENFORD);
+//   leading digits floor(|x| / 10^floor(log10 |x|)) in 1..9 of the nonzero
+//   |x_i| (in double) and their correlation with Benford's law as the
+//   reference forms it (unbiased deviations, divided by 9 s_f s_b + 1e-10; 0
+//   for fewer than 10 counted samples; row BH_TR_BENFORD);
 // * with want_cwt, cwt_coefficients (torch_features_frequency.py:71-122):
 //   the first 15 values of the correlation of the padded window with the
 //   Ricker wavelet of width 2, 5, 10, 20 (taps t = -4w .. 4w, reflect padding,
@@ -904,6 +918,8 @@ __global__ void __launch_bounds__(256)
     trends_kernel(const T *traj, int64_t n_save, int64_t n, int S, int64_t start, int want_cwt,
                   double *out) {
     __shared__ double psi[17 + 41 + 81 + 161];
+    __shared__ double p10inv[64];  // 10^-e for e = -31 .. 32
+    if (threadIdx.x < 64) p10inv[threadIdx.x] = pow(10.0, (double)(31 - (int)threadIdx.x));
     for (int i = threadIdx.x; i < 300; i += blockDim.x) {
         const int wi = i < 17 ? 0 : (i < 58 ? 1 : (i < 139 ? 2 : 3));
         const int off = wi == 0 ? 0 : (wi == 1 ? 17 : (wi == 2 ? 58 : 139));
@@ -978,7 +994,7 @@ __global__ void __launch_bounds__(256)
         // leading digits
         const double a = fabs(xv);
         if (a > 0.0 && a < INFINITY) {
-            const int dg = leading_digit(a);
+            const int dg = leading_digit_fast(a, p10inv);  // 0: not counted
 #pragma unroll
             for (int k = 0; k < 9; k++) dig[k] += dg == k + 1 ? 1 : 0;
         }

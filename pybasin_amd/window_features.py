@@ -913,11 +913,8 @@ def _trends_cpu(w: torch.Tensor, want_cwt: bool) -> torch.Tensor:
         out[TR["LANGEVIN"]] = torch.where(best == -math.inf, torch.zeros_like(best), best)
     a = x.abs()
     nz = (a > 0) & torch.isfinite(a)
-    e = torch.floor(torch.log10(torch.where(nz, a, torch.ones_like(a))))
-    sc = torch.where(nz, a, torch.ones_like(a)) / 10.0 ** e
-    sc = torch.where(sc >= 10, sc * 0.1, sc)
-    sc = torch.where(sc < 1, sc * 10, sc)
-    dg = sc.long().clamp(1, 9)
+    aa = torch.where(nz, a, torch.ones_like(a))
+    dg = torch.floor(aa / 10.0 ** torch.floor(torch.log10(aa))).long()  # 0 or 10: not counted
     cnt = torch.stack([((dg == d) & nz).double().sum(0) for d in range(1, 10)])  # (9, B, S)
     tot = cnt.sum(0)
     bf = torch.tensor([math.log10(1 + 1 / d) for d in range(1, 10)], dtype=torch.float64).view(9, 1, 1)

@@ -498,6 +498,12 @@ def _alt_degenerate(xs, p):
     return np.ptp(a) <= 1e-9 * (np.abs(a).max() + 1e-300)
 
 
+def _lib_tr_benford():
+    from pybasin_amd.window_features import TR
+
+    return TR["BENFORD"]
+
+
 def _check_trends(out, x, tag, nseries=None):
     """bh_window_trends' features against the oracle (the cubic fit: the
     kernel's elimination against the oracle's LAPACK solve, 1e-7)."""
@@ -535,7 +541,20 @@ def test_window_trends_kernel_equals_oracle_synthetic(cuda_device, dtype):
     rng = np.random.default_rng(41)
     cases = list(_synthetic_windows()) + [
         ("walk", np.cumsum(rng.normal(size=(523, 6, 2)), 0) * 0.3),
-        ("digits", rng.lognormal(0, 3, size=(200, 5, 1)) * rng.choice([-1.0, 1.0], size=(200, 5, 1)))]
+        ("digits", rng.lognormal(0, 3, size=(200, 5, 1)) * rng.choice([-1.0, 1.0], size=(200, 5, 1))),
+    ]
+    # leading digits of powers of ten and their neighbours, values outside
+    # [1e-30, 1e30] (the fallback of the fast digit path): Benford only (the
+    # cubic fit of such a window is meaningless)
+    dec = np.array([1e-35, 3e33, 1e6, 999999.9999999999, 0.3, 0.1, 1e-3, 2e-30, 9.5e29, 7.0, 10.0,
+                    99.99999999999999, 1e30, 5e-31, 0.0, -4e12])
+    dwin = np.stack([np.tile(dec, 4), np.tile(dec[::-1], 4)], 1)[:, :, None]
+    for dt_ in (torch.float64,) if dtype == torch.float64 else (torch.float32,):
+        dw = torch.tensor(dwin, dtype=dt_)
+        got = window_trends(dw.cuda(), 0, False).cpu()[_lib_tr_benford()].numpy()
+        for b in range(2):
+            want = O.comprehensive(dw[:, b, 0].double().numpy(), "benford_correlation")
+            assert got[b, 0] == pytest.approx(want, rel=1e-12, abs=1e-14), (b, got[b, 0], want)
     for tag, w in cases:
         wt = torch.tensor(w, dtype=dtype)
         x = wt.double().numpy()

@@ -469,6 +469,12 @@ def test_oracle_trends_family_pins():
     assert O.comprehensive(y, "benford_correlation") == pytest.approx(
         8 / 9 * np.corrcoef(fr, bf)[0, 1], rel=1e-6)
     assert O.comprehensive(np.r_[np.zeros(20), np.ones(9)], "benford_correlation") == 0.0
+    # the reference's digit floor(x / 10^floor(log10 x)): just below a power of
+    # ten, log10 rounds up to the integer and the digit is 0, not counted
+    assert O.comprehensive(np.full(12, 999999.9999999999), "benford_correlation") == 0.0
+    assert O.comprehensive(np.r_[np.full(12, 999999.9999999999), np.full(10, 2.0)],
+                           "benford_correlation") == O.comprehensive(np.full(10, 2.0),
+                                                                     "benford_correlation")
     for wd in (2, 5, 10, 20):
         t = np.arange(-4 * wd, 4 * wd + 1)
         psi = 2 / (np.sqrt(3 * wd) * np.pi ** 0.25) * (1 - (t / wd) ** 2) * np.exp(-t ** 2 / (2 * wd ** 2))
