@@ -225,7 +225,7 @@ __global__ void __launch_bounds__(THREADS) tile_kernel(Args a) {
 #pragma unroll
             for (int c = 0; c < 3; c++) y[c][r] = real[r] ? src[c * NN + node[r]] : 0.0f;
         }
-        // the FSAL slot is multiplied by a zero step in the initial rounds
+        // the FSAL slot is multiplied by a zero step in the initial evaluations
 #pragma unroll
         for (int c = 0; c < 3; c++)
 #pragma unroll
@@ -309,7 +309,7 @@ __global__ void __launch_bounds__(THREADS) tile_kernel(Args a) {
         const double tB = t + dt;
         const float h = stepping ? (float)dt : 0.0f;
 
-        // ---- round 1: k2 = f(y + a21 h k1); INIT: f0 = f(y0) into k2 ----
+        // ---- evaluation 1: k2 = f(y + a21 h k1); INIT: f0 = f(y0) into k2 ----
 #pragma unroll
         for (int c = 0; c < 3; c++)
 #pragma unroll
@@ -341,7 +341,7 @@ __global__ void __launch_bounds__(THREADS) tile_kernel(Args a) {
             __syncthreads();
         }
 
-        // ---- round 2: k3 = f(y + h(a31 k1 + a32 k2)); INIT: f1 = f(y0 + h0 f0) ----
+        // ---- evaluation 2: k3 = f(y + h(a31 k1 + a32 k2)); INIT: f1 = f(y0 + h0 f0) ----
         {
             const float c1 = stepping ? A31 * h : 0.0f;
             const float c2 = stepping ? A32 * h : (init ? h0 : 0.0f);
@@ -377,7 +377,7 @@ __global__ void __launch_bounds__(THREADS) tile_kernel(Args a) {
             __syncthreads();
         }
 
-        // ---- rounds 3..6 (stage inputs; zero coefficients for non-stepping slots) ----
+        // ---- evaluations 3..6 (stage inputs; zero coefficients for non-stepping slots) ----
         {
             const float c1 = A41 * h, c2 = A42 * h, c3 = A43 * h;
 #pragma unroll
