@@ -482,7 +482,7 @@ int32_t bh_window_features(int32_t dtype, const void *traj, int64_t n_save, int6
    -sum c log c, c = max(p_b, 1e-10) (fourier_entropy,
    torch_features_entropy_complexity.py:97-107).  0 <= n_coeff <= 4096; windows whose rows and
    (with out_agg) four power rows fit 62 KB of LDS (15,872 float32 rows,
-   3,172 with out_agg). */
+   3,173 with out_agg). */
 int32_t bh_window_spectrum(int32_t dtype, const void *traj, int64_t n_save, int64_t n,
                            int32_t dim, int64_t start, int32_t n_coeff, double *out_coeff,
                            double *out_agg, void *stream);
