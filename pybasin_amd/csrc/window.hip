@@ -1291,23 +1291,37 @@ __global__ void __launch_bounds__(kSeriesBlock)
             T v[7];
 #pragma unroll
             for (int j = 0; j < 7; j++) v[j] = i + j < w ? x[i + j] : (T)0;
+            // rank of element j among the first d (ties by index), grown one
+            // element at a time: element d - 1 joins with its comparisons to
+            // the d - 1 before it; the pattern id is sum_j j * (d - 1 - rank_j)!
+            int rk[7] = {0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+            for (int j = 1; j < 3; j++)
+#pragma unroll
+                for (int m = 0; m < j; m++) {
+                    const bool below = !(v[j] < v[m]);  // v[m] ranks below v[j] (m < j on ties)
+                    rk[j] += below ? 1 : 0;
+                    rk[m] += below ? 0 : 1;
+                }
 #pragma unroll
             for (int k = 0; k < kPermDims; k++) {
                 const int d = 3 + k;
                 if (i + d > w) break;
-                // rank of element j among the first d (ties by index); the
-                // pattern id is sum_j j * (d - 1 - rank_j)!
+                if (d > 3) {
+                    const int jn = d - 1;
+#pragma unroll
+                    for (int m = 0; m < 6; m++) {
+                        if (m >= jn) break;
+                        const bool below = !(v[jn] < v[m]);
+                        rk[jn] += below ? 1 : 0;
+                        rk[m] += below ? 0 : 1;
+                    }
+                }
                 int id = 0;
 #pragma unroll
-                for (int j = 0; j < 7; j++) {
+                for (int j = 1; j < 7; j++) {
                     if (j >= d) break;
-                    int rk = 0;
-#pragma unroll
-                    for (int m = 0; m < 7; m++) {
-                        if (m >= d) break;
-                        rk += (v[m] < v[j] || (v[m] == v[j] && m < j)) ? 1 : 0;
-                    }
-                    id += j * kFact[d - 1 - rk];
+                    id += j * kFact[d - 1 - rk[j]];
                 }
                 const int bin = kPermBase[k] + id;
                 atomicAdd(&hist[bin >> 1], 1u << (16 * (bin & 1)));
