@@ -795,3 +795,38 @@ def test_window_lyapunov_e_kernel_equals_oracle(cuda_device, dtype):
     assert fx.feature_names[3] == "state_0__lyapunov_e__emb_dim_10__matrix_dim_4__min_nb_8__min_tsep_0__tau_0.5__3"
     ex = window_embedding(y, 0, "lyap_e").cpu().numpy() / 0.5
     np.testing.assert_array_equal(f[:, :4], ex[:, :, 0].T.astype(np.float32))
+
+
+# the reference's own peak cases (tests/unit/ts_torch/calculators/
+# test_torch_features_pattern.py:18-58, 207-262): series and their strict
+# peak counts of support 1
+_REFERENCE_PEAK_CASES = [
+    ([0.0, 1.0, 0.0, 2.0, 0.0], 2),
+    ([0.0, 1.0, 2.0, 3.0, 4.0, 10.0, 4.0, 3.0, 2.0, 1.0], 1),
+    (list(range(10)), 0),
+    ([5.0] * 10, 0),
+    ([0.0, 1.0, 0.0], 1),
+    ([0.0, 1.0, 0.5, 2.0, 0.0], 2),
+    ([4.0, 3.0, 2.0, 1.0, 0.0], 0),
+    ([0.0, 2.0, 1.0, 2.0, 0.0], 2),
+    ([0.0, 1.0, 0.0, 1.0, 0.0, 1.0, 0.0], 3),
+    ([0.0, 1.0, 2.0, 2.0, 2.0, 1.0, 0.0], 0),
+]
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_reference_peak_cases(cuda_device, dtype):
+    """number_peaks (n = 1) through the window-feature slots and through the
+    peak-count kernel equals the reference tests' counts; a sinusoid's
+    count equals scipy.signal.argrelmax's (the reference's own check)."""
+    from scipy.signal import argrelmax
+
+    from pybasin_amd.window_features import comprehensive_feature, window_feature_block, window_peak_counts
+
+    t = np.linspace(0, 4 * np.pi, 1001)
+    cases = _REFERENCE_PEAK_CASES + [(list(np.sin(t)), len(argrelmax(np.sin(t))[0]))]
+    for data, want in cases:
+        y = torch.tensor(data, dtype=dtype).view(-1, 1, 1).cuda()
+        blk = window_feature_block(y, 0).cpu()
+        assert comprehensive_feature("number_peaks", {"n": 1}, blk, y.shape[0])[0, 0] == want, data
+        assert window_peak_counts(y, 0, [1])[0, 0, 0].item() == want, data
