@@ -1,22 +1,36 @@
-"""Comprehensive window features: the calculators of the reference's
-comprehensive feature set (src/pybasin/ts_torch/settings.py:223-340) that
-follow from one two-pass reduction of each window series.
+"""Comprehensive window features: every calculator of the reference's
+comprehensive feature set (src/pybasin/ts_torch/settings.py:223-340) on the
+kept window rows [start, N) of a trajectory tensor (N, B, S).
 
-bh_window_features (csrc/window.hip) reduces every (trajectory, state)
-series of the kept window rows [start, N) to the raw slots of
-include/basinhip.h (sums, extremes and their positions, changes, counts,
-crossings, chunk energies, lag products, central moments, runs,
-autocovariances, index-mass positions); the named features are formed here
-from those slots, in float64, following the reference's torch calculators
-term by term (the 1e-10 guards and the edge rules for short windows
-included).  On the CPU the same slots come from torch (the CPU template
-path).  A window holding a non-finite value gives NaN for every feature
-here (the extractor then imputes them), where the reference's counting
-calculators would count its finite entries.
+Each family is one kernel of csrc/window.hip (entry points in
+include/basinhip.h), run once per extraction when a feature of it is first
+asked for (WindowBlocks):
+* bh_window_features: every series reduced to raw slots (sums, extremes and
+  their positions, changes, counts, crossings, chunk energies, lag products,
+  central moments, runs, autocovariances 1..9, index-mass positions, binned
+  entropy bins) from which the moment, count, location, trend and
+  autocorrelation calculators follow here in float64;
+* bh_window_spectrum: rfft bins and the aggregated power spectrum;
+* bh_window_sorted + bh_window_change_quantiles: duplicates, reoccurrence
+  counts, the median and the change-quantile corridors of the sorted window;
+* bh_window_autocov: autocovariances of many lags (agg_autocorrelation,
+  ar_coefficient);
+* bh_window_trends: chunk trends, the cubic fits, leading digits, wavelets;
+* bh_window_peak_counts, bh_window_perm_entropy: wide peaks, ordinal
+  patterns;
+* bh_window_embedding, bh_window_lempel_ziv: correlation dimension, the
+  Lyapunov estimates, Lempel-Ziv phrase counts.
+Each calculator follows the reference's torch calculator term by term (the
+1e-10 guards and the edge rules for short windows included; the calculator
+is cited beside it).  On the CPU the same outputs come from torch (the CPU
+template path).  A window holding a non-finite value gives NaN (inf for
+lyapunov_e, as the reference) for every feature here (the extractor then
+imputes them), where the reference's counting calculators would count its
+finite entries.
 
 Differences of rounding only: the reference reduces in the trajectory dtype
 with torch's summation order, these sums run in double; counts that compare
-a sample with the mean or with r standard deviations can differ for a
+a sample with the mean, a quantile or r standard deviations can differ for a
 sample within rounding of the threshold.
 """
 
