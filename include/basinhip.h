@@ -591,8 +591,8 @@ int32_t bh_window_autocov(int32_t dtype, const void *traj, int64_t n_save, int64
    dtype), the corridor bounds of bh_window_change_quantiles.  out_median
    (n, dim) in the series dtype, or NULL: the lower middle value of the
    sorted window (x.median(dim=0), torch_features_statistical.py:30-33).
-   Non-finite windows give NaN.  Windows up to 2,048 float32 or 1,024 float64
-   rows. */
+   Non-finite windows give NaN.  Windows up to 8,192 float32 or 4,096 float64
+   rows (past 2,048 / 1,024 rows one block sorts each series). */
 int32_t bh_window_sorted(int32_t dtype, const void *traj, int64_t n_save, int64_t n, int32_t dim,
                          int64_t start, double *out_dup, void *out_levels, void *out_median,
                          void *stream);

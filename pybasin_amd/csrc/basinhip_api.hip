@@ -1160,9 +1160,26 @@ int32_t bh_window_sorted(int32_t dtype, const void *traj, int64_t n_save, int64_
     // the block's series rows plus one sorted row of P values per wave
     const int64_t budget = 62 * 1024;
     const int64_t rows = 4 * P * elem;
-    if (P > 2048 || rows + w * elem > budget)
-        return fail(-5, "bh_window_sorted: window of %lld rows exceeds the LDS budget (at most %d %s rows)",
-                    (long long)w, dtype == BH_F32 ? 2048 : 1024, dtype == BH_F32 ? "float32" : "float64");
+    hipStream_t s0 = reinterpret_cast<hipStream_t>(stream);
+    if (P > 2048 || rows + w * elem > budget) {
+        // longer windows: one series per block, sorted in LDS by the block
+        if (P * elem > 32 * 1024)
+            return fail(-5, "bh_window_sorted: window of %lld rows exceeds the LDS budget (at most %d %s rows)",
+                        (long long)w, dtype == BH_F32 ? 8192 : 4096, dtype == BH_F32 ? "float32" : "float64");
+        const int64_t np = n * dim;
+        if (np > 2147483647LL) return fail(-5, "bh_window_sorted: too many series");
+        const size_t lds = (size_t)(P * elem + 4 * 5 * 8);
+        if (dtype == BH_F32)
+            hipLaunchKernelGGL(bh::win::sorted_block_kernel<float>, dim3((unsigned)np), dim3(256), lds, s0,
+                               reinterpret_cast<const float *>(traj), n_save, n, dim, start, (int)P, out_dup,
+                               reinterpret_cast<float *>(out_levels), reinterpret_cast<float *>(out_median));
+        else
+            hipLaunchKernelGGL(bh::win::sorted_block_kernel<double>, dim3((unsigned)np), dim3(256), lds, s0,
+                               reinterpret_cast<const double *>(traj), n_save, n, dim, start, (int)P, out_dup,
+                               reinterpret_cast<double *>(out_levels), reinterpret_cast<double *>(out_median));
+        BH_HIP_CHECK(hipGetLastError());
+        return 0;
+    }
     int64_t G = (budget - rows) / (w * elem);
     if (G > 16) G = 16;
     const size_t lds = (size_t)((G * w * elem + 15) / 16 * 16 + rows);

@@ -2155,6 +2155,92 @@ __global__ void __launch_bounds__(kSeriesBlock)
     }
 }
 
+// The same statistics for windows too long for one wave's registers: one
+// series per block of 256 threads, padded with +inf to P (a power of two,
+// P * sizeof(T) <= 32 KB) and sorted in LDS by a block-wide bitonic network
+// (each stage's pairs loaded by every thread before any is stored; a block
+// barrier between stages); the sums by a block reduction through LDS.
+template <class T>
+__global__ void __launch_bounds__(256)
+    sorted_block_kernel(const T *traj, int64_t n_save, int64_t n, int S, int64_t start, int P,
+                        double *out_dup, T *out_levels, T *out_median) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    T *buf = reinterpret_cast<T *>(smem);
+    double *red = reinterpret_cast<double *>(smem + (int64_t)P * sizeof(T));  // 4 waves x 5
+    const int64_t w = n_save - start;
+    const int64_t np = n * S;
+    const int64_t p = blockIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    bool bad = false;
+    for (int64_t i = tid; i < P; i += 256) {
+        T v = (T)INFINITY;
+        if (i < w) {
+            v = traj[(start + i) * np + p];
+            bad |= !isfinite((double)v);
+        }
+        buf[i] = v;
+    }
+    const int anybad = __syncthreads_or(bad ? 1 : 0);
+    if (anybad) {
+        if (tid < 5) out_dup[tid * np + p] = NAN;
+        if (out_levels && tid < kCqLevels) out_levels[tid * np + p] = (T)NAN;
+        if (out_median && tid == 0) out_median[p] = (T)NAN;
+        return;
+    }
+    constexpr int kMaxPairs = 16;  // P / 2 / 256 for P <= 8192
+    for (int k = 2; k <= P; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            T a[kMaxPairs], b[kMaxPairs];
+            int idx[kMaxPairs];
+#pragma unroll
+            for (int r = 0; r < kMaxPairs; r++) {
+                const int t = tid + 256 * r;
+                idx[r] = ((t & ~(j - 1)) << 1) | (t & (j - 1));
+                if (t < (P >> 1)) {
+                    a[r] = buf[idx[r]];
+                    b[r] = buf[idx[r] + j];
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int r = 0; r < kMaxPairs; r++) {
+                const int t = tid + 256 * r;
+                if (t < (P >> 1)) {
+                    const bool asc = (idx[r] & k) == 0;
+                    const T lo = a[r] < b[r] ? a[r] : b[r], hi = a[r] < b[r] ? b[r] : a[r];
+                    buf[idx[r]] = asc ? lo : hi;
+                    buf[idx[r] + j] = asc ? hi : lo;
+                }
+            }
+            __syncthreads();
+        }
+    }
+    double v5[5] = {0, 0, 0, 0, 0};  // D, R, G, sum in groups, sum of one per group
+    for (int64_t i = tid; i < w; i += 256) {
+        const T v = buf[i];
+        const bool prev = i > 0 && buf[i - 1] == v;
+        const bool next = i + 1 < w && buf[i + 1] == v;
+        v5[0] += prev ? 1.0 : 0.0;
+        if (prev || next) {
+            v5[1] += 1.0;
+            v5[3] += (double)v;
+        }
+        if (next && !prev) {
+            v5[2] += 1.0;
+            v5[4] += (double)v;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 5; k++) {
+        v5[k] = wave_sum(v5[k]);
+        if (lane == 0) red[wave * 5 + k] = v5[k];
+    }
+    __syncthreads();
+    if (tid < 5) out_dup[tid * np + p] = red[tid] + red[5 + tid] + red[10 + tid] + red[15 + tid];
+    if (out_levels && tid < kCqLevels) out_levels[tid * np + p] = torch_quantile<T>(buf, w, kCqLevel[tid]);
+    if (out_median && tid == 0) out_median[p] = buf[(w - 1) / 2];
+}
+
 // change_quantiles (torch_features_change.py:20-139) for the 15 corridors
 // [v_a, v_b], a < b, of the six levels of kCqLevel, one lane per series: a
 // change d_i = x_{i+1} - x_i counts in a corridor when both samples lie in it

@@ -1406,7 +1406,7 @@ class WindowBlocks:
             return sorted_feature(name, params, *self.sorted()[:2], self.n)
         med = None
         if name == "symmetry_looking":
-            if self.n <= (2048 if self.y.dtype == torch.float32 else 1024) or self._median_fn is None:
+            if self.n <= (8192 if self.y.dtype == torch.float32 else 4096) or self._median_fn is None:
                 med = self.sorted()[2]  # the sorted window's middle value
             else:
                 med = self._get("median", self._median_fn)

@@ -139,7 +139,8 @@ def test_window_entries_check_arguments_before_device_work():
     refused(L.bh_window_embedding(F64, fake, 4097, 4, 2, 0, _lib.BH_EMB_LYAP_E, fake, None),
             b"LDS budget")
     refused(L.bh_window_autocov(F32, fake, 10, 4, 2, 0, 5000, fake, None), b"max_lag")
-    refused(L.bh_window_sorted(F64, fake, 1025, 4, 2, 0, fake, None, None, None), b"LDS budget")
+    refused(L.bh_window_sorted(F64, fake, 4097, 4, 2, 0, fake, None, None, None), b"LDS budget")
+    refused(L.bh_window_sorted(F32, fake, 8193, 4, 2, 0, fake, None, None, None), b"LDS budget")
     refused(L.bh_window_sorted(F32, fake, 100, 4, 2, 0, None, None, None, None), b"out_dup")
     refused(L.bh_window_change_quantiles(F32, fake, 100, 4, 2, 0, None, fake, None), b"levels")
     refused(L.bh_window_trends(F32, fake, 100, 4, 2, 0, 1, None, None), b"null out")

@@ -381,9 +381,30 @@ def test_window_sorted_limit_is_an_error(cuda_device):
     from pybasin_amd.window_features import window_sorted
 
     with pytest.raises(ValueError, match="LDS budget"):
-        window_sorted(torch.zeros(1025, 4, 1, dtype=torch.float64, device="cuda"), 0, True)
+        window_sorted(torch.zeros(4097, 4, 1, dtype=torch.float64, device="cuda"), 0, True)
     with pytest.raises(ValueError, match="LDS budget"):
-        window_sorted(torch.zeros(2049, 4, 1, dtype=torch.float32, device="cuda"), 0, False)
+        window_sorted(torch.zeros(8193, 4, 1, dtype=torch.float32, device="cuda"), 0, False)
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_window_sorted_long_windows_block_sort(cuda_device, dtype):
+    """Windows past one wave's registers (float32 2,049..8,192, float64
+    1,025..4,096 rows) are sorted by a whole block: the same statistics as
+    the oracle, ties and a non-finite window included."""
+    from pybasin_amd.window_features import window_sorted
+
+    rng = np.random.default_rng(61)
+    sd = np.float32 if dtype == torch.float32 else np.float64
+    for N in ((2049, 5000, 8192) if dtype == torch.float32 else (1025, 3000, 4096)):
+        w = np.round(rng.normal(size=(N, 3, 2)) * 16) / 16
+        w[7, 2, 1] = np.nan
+        wt = torch.tensor(w, dtype=dtype)
+        dup, cq, med = window_sorted(wt.cuda(), 0, True)
+        dup, cq, med = dup.cpu(), cq.cpu(), med.cpu()
+        _check_sorted(dup, cq, wt.numpy().astype(sd), f"long{N}", sd)
+        cdup, ccq, cmed = window_sorted(wt, 0, True)
+        assert torch.equal(torch.nan_to_num(dup[:3], 7.0), torch.nan_to_num(cdup[:3], 7.0)), N
+        assert torch.equal(torch.nan_to_num(med, 7.0), torch.nan_to_num(cmed, 7.0)), N
 
 
 @pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
