@@ -524,7 +524,7 @@ int32_t bh_window_trends(int32_t dtype, const void *traj, int64_t n_save, int64_
    torch_features_pattern.py:64-105; s_r in [0, 64], 0 counts nothing), rows
    4..11 = the positions equal to the maximum of the 2 w + 1 samples about
    them for w = 1..8 (the per-width counts of number_cwt_peaks, :154-190).
-   Windows up to 3,174 float32 or 2,645 float64 rows; non-finite windows give
+   Windows up to 5,290 float32 or 3,968 float64 rows; non-finite windows give
    NaN. */
 int32_t bh_window_peak_counts(int32_t dtype, const void *traj, int64_t n_save, int64_t n,
                               int32_t dim, int64_t start, int32_t s0, int32_t s1, int32_t s2,

@@ -962,11 +962,15 @@ int32_t bh_window_peak_counts(int32_t dtype, const void *traj, int64_t n_save, i
     if (n == 0) return 0;
     const int64_t w = n_save - start;
     const int64_t elem = dtype == BH_F32 ? 4 : 8;
-    // the block's series rows plus one list of w candidate positions per wave
-    const int64_t budget = 62 * 1024, lists = 4 * w * 4;
-    if (lists + w * elem > budget)
+    // the block's series rows plus one list of w 16-bit candidate positions per wave
+    const int64_t budget = 62 * 1024, lists = (4 * w * 2 + 15) / 16 * 16;
+    auto need = [&](int64_t r) { return (4 * r * 2 + 15) / 16 * 16 + (r * elem + 15) / 16 * 16; };
+    if (need(w) > budget || w > 65535) {
+        int64_t wmax = budget / (8 + elem);
+        while (wmax > 0 && need(wmax) > budget) wmax--;
         return fail(-5, "bh_window_peak_counts: window of %lld rows exceeds the LDS budget (at most %lld rows)",
-                    (long long)w, (long long)(budget / (16 + elem)));
+                    (long long)w, (long long)wmax);
+    }
     int64_t G = (budget - lists) / (w * elem);
     if (G > 16) G = 16;
     const size_t lds = (size_t)((G * w * elem + 15) / 16 * 16 + lists);

@@ -1181,11 +1181,12 @@ __global__ void __launch_bounds__(kSeriesBlock)
     const int64_t np = n * S;
     const int64_t p0 = (int64_t)blockIdx.x * G;
     T *lds = reinterpret_cast<T *>(smem);
-    int *cbase = reinterpret_cast<int *>(smem + ((int64_t)G * w * sizeof(T) + 15) / 16 * 16);
+    unsigned short *cbase =
+        reinterpret_cast<unsigned short *>(smem + ((int64_t)G * w * sizeof(T) + 15) / 16 * 16);
     load_series<T>(traj, n, S, start, w, p0, G, np, lds);
     __syncthreads();
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    int *cand = cbase + (int64_t)wave * w;  // the wave's list of candidate positions
+    unsigned short *cand = cbase + (int64_t)wave * w;  // the wave's candidate positions
     const int sup[kPeakSupports] = {supports.x, supports.y, supports.z, supports.w};
     int cap = 1;  // the strict run need not go past the largest support
 #pragma unroll
@@ -1212,7 +1213,7 @@ __global__ void __launch_bounds__(kSeriesBlock)
                 keep = c >= x[i - 1] && c >= x[i + 1];
             }
             const unsigned long long m = __ballot(keep);
-            if (keep) cand[nc + __popcll(m & ((1ull << lane) - 1ull))] = (int)i;
+            if (keep) cand[nc + __popcll(m & ((1ull << lane) - 1ull))] = (unsigned short)i;
             nc += __popcll(m);
         }
         __builtin_amdgcn_wave_barrier();

@@ -135,7 +135,8 @@ def test_window_entries_check_arguments_before_device_work():
     refused(L.bh_window_spectrum(F32, fake, 10, 4, 2, 0, 0, None, None, None), b"nothing")
     refused(L.bh_window_spectrum(F32, fake, 3174, 4, 2, 0, 0, None, fake, None), b"LDS budget")
     refused(L.bh_window_spectrum(F64, fake, 7937, 4, 2, 0, 5, fake, None, None), b"LDS budget")
-    refused(L.bh_window_peak_counts(F32, fake, 3175, 4, 2, 0, 10, 0, 0, 0, fake, None), b"LDS budget")
+    refused(L.bh_window_peak_counts(F32, fake, 5291, 4, 2, 0, 10, 0, 0, 0, fake, None), b"at most 5290")
+    refused(L.bh_window_peak_counts(F64, fake, 3969, 4, 2, 0, 10, 0, 0, 0, fake, None), b"at most 3968")
     refused(L.bh_window_embedding(F64, fake, 4097, 4, 2, 0, _lib.BH_EMB_LYAP_E, fake, None),
             b"LDS budget")
     refused(L.bh_window_autocov(F32, fake, 10, 4, 2, 0, 5000, fake, None), b"max_lag")

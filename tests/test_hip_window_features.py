@@ -665,6 +665,28 @@ def test_window_perm_entropy_limit_is_an_error(cuda_device):
     window_perm_entropy(torch.zeros(4052, 3, 1, dtype=torch.float32, device="cuda"), 0)
 
 
+@pytest.mark.parametrize("dtype,rows", [(torch.float32, 5290), (torch.float64, 3968)])
+def test_window_peaks_at_the_lds_limit(cuda_device, dtype, rows):
+    """Windows at the longest length the peak-count kernel stages (16-bit
+    candidate positions past 4,096 rows) equal the CPU path exactly and the
+    oracle's strict peak counts; one row more is a ValueError."""
+    from pybasin_amd.window_features import window_peak_counts
+
+    rng = np.random.default_rng(53)
+    t = np.arange(rows)[:, None, None]
+    w = np.sin(t * rng.uniform(0.01, 0.4, size=(1, 3, 2))) + 0.3 * rng.normal(size=(rows, 3, 2))
+    wt = torch.tensor(w, dtype=dtype)
+    c = window_peak_counts(wt.cuda(), 0, [10, 50, 1, 64]).cpu()
+    cc = window_peak_counts(wt, 0, [10, 50, 1, 64])
+    assert torch.equal(c, cc)
+    x = wt.double().numpy()
+    for b, s in ((0, 0), (2, 1)):
+        for k, sup in enumerate((10, 50, 1, 64)):
+            assert float(c[k, b, s]) == O.comprehensive(x[:, b, s], "number_peaks", {"n": sup}), (sup, b, s)
+    with pytest.raises(ValueError, match=f"at most {rows} rows"):
+        window_peak_counts(torch.zeros(rows + 1, 3, 1, dtype=dtype, device="cuda"), 0, [1, 1, 1, 1])
+
+
 def test_window_peaks_on_integrated_duffing(cuda_device):
     """Wide-support peaks, cwt peaks and permutation entropies of the
     integrated Duffing windows through the extractor equal the CPU path."""
