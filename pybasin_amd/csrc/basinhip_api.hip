@@ -1040,6 +1040,8 @@ int32_t bh_window_embedding(int32_t dtype, const void *traj, int64_t n_save, int
     const int64_t budget = 62 * 1024;
     const int64_t extra = which == BH_EMB_CORR_DIM ? 4 * 50 * 8 + 4 * 64 * 4
                           : (which == BH_EMB_LYAP_E ? 4 * 64 * 15 * 8 : 0);
+    // lyap_e keys carry an orbit point in 13 bits: the budget keeps windows below 8,204 rows
+    static_assert((62 * 1024 - 4 * 64 * 15 * 8) / 4 - 12 <= (int64_t)bh::win::kLeKeyMask, "lyap_e key bits");
     if (extra + w * elem > budget)
         return fail(-5, "bh_window_embedding: window of %lld rows exceeds the %lld-byte LDS budget",
                     (long long)w, (long long)budget);

@@ -1619,6 +1619,18 @@ __global__ void __launch_bounds__(kSeriesBlock)
 
 constexpr int kLeEmb = 10, kLeDim = 4, kLeStep = 3, kLeNb = 8;
 constexpr int kLeRec = 14;  // the normal equations of one point: 10 of X^T X, 4 of X^T b
+constexpr uint64_t kLeKeyMask = 0x1FFF;  // 13 bits of point index: windows below 8,204 rows
+
+// The search key of orbit point j at Chebyshev distance d (0 <= d < 2^1023):
+// d's bits with the exponent raised by one (no key is subnormal, so min and
+// max order keys whatever the denormal mode) and the low 13 bits replaced by
+// j.  Keys of non-negative doubles order as their bit patterns, so keys order
+// as (d truncated to 39 fraction bits, j): a key below another has a distance
+// no larger, and a distance part strictly below means a smaller distance.
+__device__ __forceinline__ double le_key(double d, int j) {
+    const uint64_t b = (uint64_t)__double_as_longlong(d) + (1ull << 52);
+    return __longlong_as_double((long long)((b & ~kLeKeyMask) | (uint64_t)j));
+}
 
 // Solves the 4 x 4 system A a = b (A symmetric, given by its upper triangle
 // u[0..9] row by row, plus reg on the diagonal) by Gaussian elimination with
@@ -1682,7 +1694,8 @@ __device__ __forceinline__ void solve4(const double *u, const double *b, double 
 // accumulates ln diag R where positive; the exponents are the means over the
 // points, over 3 (inf where nothing was accumulated).  One wave per series in
 // LDS: the points' neighbour searches and normal equations run 64 at a time
-// (one point per lane, the eight smallest distances kept in registers), their
+// (one point per lane, the nine smallest distances kept in registers as
+// keys that carry their points, le_key), their
 // 14 sums go through the wave's LDS rows, and the sequential QR chain runs
 // over them in every lane alike.  out (4, n, dim) in descending accumulation
 // order as the reference leaves it; a non-finite window gives inf.
@@ -1704,12 +1717,16 @@ __global__ void __launch_bounds__(kSeriesBlock)
         const int64_t p = p0 + g;
         if (p >= np) break;  // wave-uniform
         const T *x = lds + (int64_t)g * w;
-        bool bad = false;
-        for (int64_t i = lane; i < w; i += 64) bad |= !isfinite((double)x[i]);
+        bool bad = false, big = false;
+        for (int64_t i = lane; i < w; i += 64) {
+            bad |= !isfinite((double)x[i]);
+            big |= fabs((double)x[i]) >= 0x1p1021;
+        }
         if (__any(bad)) {
             if (lane < kLeDim) out[lane * np + p] = INFINITY;
             continue;
         }
+        const bool huge = __any(big);  // distances may reach 2^1022: no finite keys
         const int no = (int)(w - kLeStep) - (kLeEmb - 1);  // orbit points
         double Q[4][4] = {{1, 0, 0, 0}, {0, 1, 0, 0}, {0, 0, 1, 0}, {0, 0, 0, 1}};
         double lexp[4] = {0, 0, 0, 0}, lcnt[4] = {0, 0, 0, 0};
@@ -1723,53 +1740,121 @@ __global__ void __launch_bounds__(kSeriesBlock)
                 double a[kLeEmb];
 #pragma unroll
                 for (int e = 0; e < kLeEmb; e++) a[e] = (double)x[i + e];
-                // the nine smallest Chebyshev distances (ascending) and their
-                // points: when the ninth lies above the eighth, the neighbours
-                // (distance <= the eighth) are exactly the first eight;
-                // otherwise (a tie at the eighth) a second pass collects them
+                // the nine smallest Chebyshev distances as keys (le_key): one
+                // branch-free min/max network keeps distances and points
+                // together; a window with |x| >= 2^1021 makes no finite key
+                // and takes the exact search below
                 double top[kLeNb + 1];
-                int tj[kLeNb + 1];
 #pragma unroll
-                for (int k = 0; k <= kLeNb; k++) {
-                    top[k] = INFINITY;
-                    tj[k] = -1;
-                }
-                // eight points j at a time: their 17 samples loaded together
-                // (LDS broadcasts), the distances from registers
-                for (int j0 = 0; j0 < no; j0 += 8) {
-                  double xv[kLeEmb + 7];
+                for (int k = 0; k <= kLeNb; k++) top[k] = INFINITY;
+                if (!huge) {
+                  // eight points j at a time: their 17 samples loaded together
+                  // (LDS broadcasts), the distances from registers
+                  for (int j0 = 0; j0 < no; j0 += 8) {
+                    double xv[kLeEmb + 7];
 #pragma unroll
-                  for (int u = 0; u < kLeEmb + 7; u++) xv[u] = j0 + u < w ? (double)x[j0 + u] : 0.0;
-                  double dd[8];
+                    for (int u = 0; u < kLeEmb + 7; u++) xv[u] = j0 + u < w ? (double)x[j0 + u] : 0.0;
+                    double dd[8];
 #pragma unroll
-                  for (int u = 0; u < 8; u++) {  // independent chains, a balanced max tree
-                    double t[kLeEmb];
+                    for (int u = 0; u < 8; u++) {  // independent chains, a balanced max tree
+                      double t[kLeEmb];
 #pragma unroll
-                    for (int e = 0; e < kLeEmb; e++) t[e] = fabs(a[e] - xv[u + e]);
-                    dd[u] = fmax(fmax(fmax(fmax(t[0], t[1]), fmax(t[2], t[3])),
-                                      fmax(fmax(t[4], t[5]), fmax(t[6], t[7]))), fmax(t[8], t[9]));
-                  }
+                      for (int e = 0; e < kLeEmb; e++) t[e] = fabs(a[e] - xv[u + e]);
+                      dd[u] = fmax(fmax(fmax(fmax(t[0], t[1]), fmax(t[2], t[3])),
+                                        fmax(fmax(t[4], t[5]), fmax(t[6], t[7]))), fmax(t[8], t[9]));
+                    }
 #pragma unroll
-                  for (int u = 0; u < 8; u++) {
-                    const int j = j0 + u;
-                    const double d = dd[u];
-                    if (j < no && j != i && d < top[kLeNb]) {
-                        double v = d;
-                        int vj = j;
+                    for (int u = 0; u < 8; u++) {
+                      const int j = j0 + u;
+                      const double key = le_key(dd[u], j);
+                      if (j < no && j != i && key < top[kLeNb]) {
+                        double v = key;
 #pragma unroll
                         for (int k = 0; k <= kLeNb; k++) {
-                            const bool sw = v < top[k];
-                            const double hv = sw ? top[k] : v;
-                            const int hj = sw ? tj[k] : vj;
-                            top[k] = sw ? v : top[k];
-                            tj[k] = sw ? vj : tj[k];
-                            v = hv;
-                            vj = hj;
+                          const double lo = fmin(top[k], v);
+                          v = fmax(top[k], v);
+                          top[k] = lo;
                         }
+                      }
                     }
                   }
                 }
-                const double r = top[kLeNb - 1];
+                // The keys settle the neighbours when the ninth key's distance
+                // part lies above the eighth's: the ninth distance then lies
+                // above the eighth, and the neighbours are the eight keyed
+                // points.  Otherwise (a tie at the eighth distance, or fewer
+                // than eight points) the exact search decides.
+                const uint64_t k8 = (uint64_t)__double_as_longlong(top[kLeNb - 1]) & ~kLeKeyMask;
+                const uint64_t k9 = (uint64_t)__double_as_longlong(top[kLeNb]) & ~kLeKeyMask;
+                double nd[kLeNb], r = INFINITY;
+                int nj[kLeNb];
+                bool tie = false;
+                if (!huge && k9 > k8) {
+#pragma unroll
+                    for (int k = 0; k < kLeNb; k++) {
+                        nj[k] = (int)((uint64_t)__double_as_longlong(top[k]) & kLeKeyMask);
+                        double d = 0.0;
+#pragma unroll
+                        for (int e = 0; e < kLeEmb; e++) d = fmax(d, fabs(a[e] - (double)x[nj[k] + e]));
+                        nd[k] = d;
+                    }
+                    r = nd[kLeNb - 1];
+                } else {
+                    // exact search: the nine smallest distances (ascending,
+                    // ties by point) and their points; the neighbours are the
+                    // first eight when the ninth lies above the eighth, else
+                    // every point within the eighth distance
+                    double td[kLeNb + 1];
+                    int tj[kLeNb + 1];
+#pragma unroll
+                    for (int k = 0; k <= kLeNb; k++) {
+                        td[k] = INFINITY;
+                        tj[k] = -1;
+                    }
+                    for (int j = 0; j < no; j++) {
+                        double d = 0.0;
+#pragma unroll
+                        for (int e = 0; e < kLeEmb; e++) d = fmax(d, fabs(a[e] - (double)x[j + e]));
+                        if (j != i && d < td[kLeNb]) {
+                            double v = d;
+                            int vj = j;
+#pragma unroll
+                            for (int k = 0; k <= kLeNb; k++) {
+                                const bool sw = v < td[k];
+                                const double hv = sw ? td[k] : v;
+                                const int hj = sw ? tj[k] : vj;
+                                td[k] = sw ? v : td[k];
+                                tj[k] = sw ? vj : tj[k];
+                                v = hv;
+                                vj = hj;
+                            }
+                        }
+                    }
+                    r = td[kLeNb - 1];
+                    tie = !(td[kLeNb] > r);
+#pragma unroll
+                    for (int k = 0; k < kLeNb; k++) {
+                        nd[k] = td[k];
+                        nj[k] = tj[k];
+                    }
+                }
+                // the eight neighbours in the order of (distance, point), on
+                // either path (keys order by truncated distances, and the
+                // exact insertion can carry an element past an equal one)
+                if (!tie) {
+#pragma unroll
+                    for (int k = 1; k < kLeNb; k++)
+#pragma unroll
+                        for (int m = k; m > 0; m--) {
+                            const bool sw = nd[m] < nd[m - 1] || (nd[m] == nd[m - 1] && nj[m] < nj[m - 1]);
+                            const double hd = nd[m];
+                            const int hj = nj[m];
+                            nd[m] = sw ? nd[m - 1] : nd[m];
+                            nj[m] = sw ? nj[m - 1] : nj[m];
+                            nd[m - 1] = sw ? hd : nd[m - 1];
+                            nj[m - 1] = sw ? hj : nj[m - 1];
+                        }
+                }
                 if (r < INFINITY) {
                     ok = true;
                     double xi[kLeDim];
@@ -1789,9 +1874,9 @@ __global__ void __launch_bounds__(kSeriesBlock)
 #pragma unroll
                         for (int k = 0; k < kLeDim; k++) sums[10 + k] = fma(xd[k], bd, sums[10 + k]);
                     };
-                    if (top[kLeNb] > r) {
+                    if (!tie) {
 #pragma unroll
-                        for (int k = 0; k < kLeNb; k++) add(tj[k]);
+                        for (int k = 0; k < kLeNb; k++) add(nj[k]);
                     } else {
                         for (int j = 0; j < no; j++) {
                             if (j == i) continue;
