@@ -1763,12 +1763,20 @@ __global__ void __launch_bounds__(kSeriesBlock)
                       dd[u] = fmax(fmax(fmax(fmax(t[0], t[1]), fmax(t[2], t[3])),
                                         fmax(fmax(t[4], t[5]), fmax(t[6], t[7]))), fmax(t[8], t[9]));
                     }
+                    // one branch per block of eight: a key at or above the
+                    // ninth passes through the network unchanged
+                    double kv[8];
+                    double kmin = INFINITY;
 #pragma unroll
                     for (int u = 0; u < 8; u++) {
                       const int j = j0 + u;
-                      const double key = le_key(dd[u], j);
-                      if (j < no && j != i && key < top[kLeNb]) {
-                        double v = key;
+                      kv[u] = (j < no && j != i) ? le_key(dd[u], j) : INFINITY;
+                      kmin = fmin(kmin, kv[u]);
+                    }
+                    if (kmin < top[kLeNb]) {
+#pragma unroll
+                      for (int u = 0; u < 8; u++) {
+                        double v = kv[u];
 #pragma unroll
                         for (int k = 0; k <= kLeNb; k++) {
                           const double lo = fmin(top[k], v);
